@@ -1,0 +1,128 @@
+"""ctypes binding of include/mge.h (libmge.so, built in-tree).
+
+The library is required: importing a solver without it raises, there is
+no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmge.so")
+
+MGE_OK = 0
+ERRORS = {
+    -10: "MGE_ERR_NEG_C2", -11: "MGE_ERR_WALL_RHO", -12: "MGE_ERR_WSOL_NAN",
+    -13: "MGE_ERR_RES_NAN", -14: "MGE_ERR_PERIODIC", -15: "MGE_ERR_DISTYPE",
+    -16: "MGE_ERR_SMOO_DIM", -20: "MGE_ERR_ARG", -21: "MGE_ERR_HIP",
+    -22: "MGE_ERR_NOMEM", -23: "MGE_ERR_STATE",
+}
+WARNINGS = {1: "MGE_WARN_ZERO_CLOUD", 2: "MGE_WARN_BCTYPE"}
+
+BODY_NONE, BODY_CYLINDER, BODY_SPHERE, BODY_ELLIPSOID = 0, 1, 2, 3
+
+
+class MgeParams(C.Structure):
+    _fields_ = [
+        ("mstage", C.c_int),
+        ("cstp", C.c_double * 8),
+        ("cdis", C.c_double * 8),
+        ("cfl", C.c_double),
+        ("vt", C.c_int),
+        ("vis2", C.c_double),
+        ("vis4", C.c_double),
+        ("smoo", C.c_double),
+        ("mgtype", C.c_int),
+        ("gamma", C.c_double),
+        ("rho0", C.c_double), ("p0", C.c_double), ("c0", C.c_double), ("ei0", C.c_double),
+        ("u0", C.c_double), ("v0", C.c_double), ("w0", C.c_double), ("h0", C.c_double),
+        ("eqntype", C.c_int),
+        ("dissip", C.c_int),
+        ("viscl", C.c_double), ("prlmean", C.c_double), ("gas_const", C.c_double),
+        ("drop", C.c_double),
+        ("distype", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class MgeCutRec(C.Structure):
+    _fields_ = [
+        ("faceid", C.c_int), ("zz", C.c_int),
+        ("ccid", C.c_int * 3), ("scid", C.c_int * 3),
+        ("psurf", C.c_double * 3), ("nvec", C.c_double * 3),
+        ("radcur", C.c_double * 2), ("rr", C.c_double),
+        ("npoint", C.c_int),
+        ("ijk", (C.c_int * 3) * 8),
+        ("coef", C.c_double * 8),
+        ("psol", C.c_double), ("wsol", C.c_double * 5),
+    ]
+
+
+class MgeBody(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("axis", C.c_int),
+        ("center", C.c_double * 3), ("semi", C.c_double * 3), ("rot", C.c_double * 9),
+    ]
+
+
+class MgeError(RuntimeError):
+    def __init__(self, code, where, msg=""):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)} {msg}".strip())
+
+
+_lib = None
+
+
+def lib():
+    """Load libmge.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MgeError(-21, "load", f"{LIB_PATH} is missing; run code_mge_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    dp = C.POINTER(C.c_double)
+    L.mge_last_error.restype = C.c_char_p
+    L.mge_geom_level.argtypes = [C.POINTER(MgeBody), I, I, I, dp, dp, dp, C.c_void_p,
+                                 C.POINTER(MgeCutRec), I, C.POINTER(I)]
+    L.mge_case_create.argtypes = [C.POINTER(MgeParams), I, C.POINTER(P)]
+    L.mge_case_destroy.argtypes = [P]
+    L.mge_level_define.argtypes = [P, I, I, I, I, dp, dp, dp, D, C.POINTER(I), C.POINTER(I),
+                                   C.c_void_p, I, C.POINTER(MgeCutRec), I]
+    L.mge_init_flow.argtypes = [P]
+    L.mge_timestep.argtypes = [P, I, I]
+    L.mge_euler.argtypes = [P, I, I, I, I, I, I, I, I, dp]
+    L.mge_bcond.argtypes = [P, I, I]
+    L.mge_bc_cutcell.argtypes = [P, I, I]
+    L.mge_bc_overlap.argtypes = [P, I]
+    L.mge_movco.argtypes = [P, I]
+    L.mge_movfin.argtypes = [P, I, I]
+    L.mge_solve.argtypes = [P, C.POINTER(I), dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.mge_download.argtypes = [P, I, I, C.c_char_p, dp]
+    L.mge_upload.argtypes = [P, I, I, C.c_char_p, dp]
+    L.mge_get_cuts.argtypes = [P, I, I, C.POINTER(MgeCutRec)]
+    L.mge_get_dtim.argtypes = [P, I, I]
+    L.mge_get_dtim.restype = D
+    L.mge_wall_force.argtypes = [P, I, dp]
+    L.mge_synchronize.argtypes = [P]
+    L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong)]
+    _lib = L
+    return L
+
+
+def check(rc, where):
+    """Raise on a negative status; return warnings (positive) unchanged."""
+    if rc < 0:
+        msg = lib().mge_last_error()
+        raise MgeError(rc, where, msg.decode() if msg else "")
+    return rc
+
+
+EXPORTED = [
+    "mge_geom_level", "mge_case_create", "mge_case_destroy", "mge_level_define",
+    "mge_init_flow", "mge_timestep", "mge_euler", "mge_bcond", "mge_bc_cutcell",
+    "mge_bc_overlap", "mge_movco", "mge_movfin", "mge_solve", "mge_download",
+    "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
+    "mge_kernel_timing", "mge_last_error",
+]

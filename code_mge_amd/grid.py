@@ -1,0 +1,112 @@
+"""Cartesian level hierarchy and the cut-cell geometry of each level.
+
+Each multigrid level is one block covering the whole domain (the
+configuration every benchmark case uses).  Vertices follow
+init_metric.f90:156-169: x(i) = h*(i-2) + origin for i = 1..ib, with
+x(0) left at 0.  Coarsening halves a direction while it keeps >= 2 cells
+(psmoo_old needs two cells per line, psmoo.f90:132,166), otherwise the
+direction is kept (semi-coarsening).
+"""
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _ffi
+
+
+@dataclass
+class LevelSpec:
+    n: tuple               # cells (ni, nj, nk)
+    lo: tuple              # domain lower corner
+    hi: tuple              # domain upper corner
+    ratio: tuple           # coarsening ratio to the next coarser level
+    bctype: tuple          # faces 1..6: i-lo, j-lo, k-lo, i-hi, j-hi, k-hi
+
+    def vertices(self):
+        out = []
+        for a in range(3):
+            n = self.n[a]
+            h = (self.hi[a] - self.lo[a]) / n
+            v = np.zeros(n + 4, dtype=np.float64)
+            idx = np.arange(1, n + 4, dtype=np.float64)
+            v[1:] = h * (idx - 2.0) + self.lo[a]
+            out.append(v)
+        return out
+
+    def spacing(self):
+        return tuple((self.hi[a] - self.lo[a]) / self.n[a] for a in range(3))
+
+    def cellvolume(self):
+        hx, hy, hz = self.spacing()
+        return hx * hy * hz
+
+    @property
+    def ncell(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+
+def hierarchy(n_fine, lo, hi, nlevel, bctype, min_cells=2) -> List[LevelSpec]:
+    """Levels coarsest first (index 0 = reference level 1)."""
+    specs = []
+    n = list(n_fine)
+    for _ in range(nlevel):
+        specs.append([tuple(n)])
+        nxt = []
+        for a in range(3):
+            nxt.append(n[a] // 2 if (n[a] % 2 == 0 and n[a] // 2 >= min_cells) else n[a])
+        n = nxt
+    levels = []
+    for li in range(nlevel):
+        nn = specs[li][0]
+        if li + 1 < nlevel:
+            nc = specs[li + 1][0]
+            ratio = tuple(nn[a] // nc[a] for a in range(3))
+        else:
+            ratio = (2, 2, 2)
+        levels.append(LevelSpec(n=nn, lo=tuple(lo), hi=tuple(hi), ratio=ratio, bctype=tuple(bctype)))
+    levels.reverse()
+    return levels
+
+
+def body(kind, center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2, rot=None):
+    b = _ffi.MgeBody()
+    b.kind = {"none": 0, "cylinder": 1, "sphere": 2, "ellipsoid": 3}[kind] if isinstance(kind, str) else kind
+    b.axis = axis
+    for a in range(3):
+        b.center[a] = center[a]
+        b.semi[a] = semi[a]
+    R = np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64)
+    for r in range(3):
+        for s in range(3):
+            b.rot[r * 3 + s] = R[r, s]
+    return b
+
+
+@dataclass
+class LevelGeom:
+    spec: LevelSpec
+    xv: np.ndarray
+    yv: np.ndarray
+    zv: np.ndarray
+    blank: np.ndarray       # (nk, nj, ni) uint8, 1 fluid
+    recs: object            # ctypes array of MgeCutRec
+    nmir: int
+
+
+def level_geometry(spec: LevelSpec, bdy: Optional[_ffi.MgeBody]) -> LevelGeom:
+    """Run the geometry preprocessor (host code of libmge) for one level."""
+    L = _ffi.lib()
+    xv, yv, zv = spec.vertices()
+    ni, nj, nk = spec.n
+    blank = np.ones((nk, nj, ni), dtype=np.uint8)
+    nm = C.c_int(0)
+    dp = C.POINTER(C.c_double)
+    bptr = C.byref(bdy) if bdy is not None else None
+    args = (ni, nj, nk, xv.ctypes.data_as(dp), yv.ctypes.data_as(dp), zv.ctypes.data_as(dp),
+            blank.ctypes.data_as(C.c_void_p))
+    _ffi.check(L.mge_geom_level(bptr, *args, None, 0, C.byref(nm)), "mge_geom_level")
+    recs = (_ffi.MgeCutRec * max(nm.value, 1))()
+    _ffi.check(L.mge_geom_level(bptr, *args, recs, nm.value, C.byref(nm)), "mge_geom_level")
+    return LevelGeom(spec, xv, yv, zv, blank, recs, nm.value)

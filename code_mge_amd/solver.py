@@ -1,0 +1,153 @@
+"""Host-side mirror of the reference solve-step interface over libmge.
+
+`Case` keeps the reference's call surface (srcf90/solver.f90 calls
+timestep, euler, bc_overlap, movco, movfin per level) so a driver can run
+the FMG/FAS cycle call by call, and `Case.solve` runs the same cycle inside
+the library (mge_solve).  All state lives on the GPU; arrays cross the ABI
+only in download/upload.
+"""
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _ffi
+from .grid import LevelGeom, LevelSpec, level_geometry
+from .params import Params
+
+_dp = C.POINTER(C.c_double)
+
+
+class Case:
+    def __init__(self, params: Params, levels: Sequence[LevelSpec], body=None,
+                 mxcyc: int = 1000, geoms: Optional[List[LevelGeom]] = None):
+        self.lib = _ffi.lib()
+        self.params = params
+        self.levels = list(levels)
+        self.nlevel = len(self.levels)
+        self.geoms = geoms if geoms is not None else [level_geometry(s, body) for s in self.levels]
+        self.mxcyc = mxcyc
+        self._p = params.to_c()
+        h = C.c_void_p()
+        _ffi.check(self.lib.mge_case_create(C.byref(self._p), self.nlevel, C.byref(h)), "mge_case_create")
+        self.h = h
+        for li, g in enumerate(self.geoms):
+            s = g.spec
+            bct = (C.c_int * 6)(*s.bctype)
+            rat = (C.c_int * 3)(*s.ratio)
+            _ffi.check(self.lib.mge_level_define(
+                self.h, li + 1, s.n[0], s.n[1], s.n[2],
+                g.xv.ctypes.data_as(_dp), g.yv.ctypes.data_as(_dp), g.zv.ctypes.data_as(_dp),
+                s.cellvolume(), bct, rat, g.blank.ctypes.data_as(C.c_void_p), g.nmir,
+                g.recs, mxcyc), "mge_level_define")
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mge_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- reference call surface
+    def init_flow(self):
+        return _ffi.check(self.lib.mge_init_flow(self.h), "mge_init_flow")
+
+    def timestep(self, lvl):
+        return _ffi.check(self.lib.mge_timestep(self.h, lvl, 1), "mge_timestep")
+
+    def euler(self, lvl, ncyc, mgl, mlvl, lres, lfir, itime=0):
+        r = C.c_double(0.0)
+        _ffi.check(self.lib.mge_euler(self.h, lvl, 1, itime, ncyc, mgl, mlvl, int(lres), int(lfir),
+                                      C.byref(r)), "mge_euler")
+        return r.value
+
+    def bcond(self, lvl):
+        return _ffi.check(self.lib.mge_bcond(self.h, lvl, 1), "mge_bcond")
+
+    def bc_cutcell(self, lvl):
+        return _ffi.check(self.lib.mge_bc_cutcell(self.h, lvl, 1), "mge_bc_cutcell")
+
+    def bc_overlap(self, lvl):
+        return _ffi.check(self.lib.mge_bc_overlap(self.h, lvl), "mge_bc_overlap")
+
+    def movco(self, mgl):
+        return _ffi.check(self.lib.mge_movco(self.h, mgl), "mge_movco")
+
+    def movfin(self, mgl, mlvl):
+        return _ffi.check(self.lib.mge_movfin(self.h, mgl, mlvl), "mge_movfin")
+
+    def solve(self, mcyc: Sequence[int], cap: int = 100000):
+        mc = (C.c_int * self.nlevel)(*mcyc)
+        log = np.zeros(3 * cap, dtype=np.float64)
+        nlog = C.c_int(0)
+        cs = C.c_longlong(0)
+        _ffi.check(self.lib.mge_solve(self.h, mc, log.ctypes.data_as(_dp), cap, C.byref(nlog), C.byref(cs)),
+                   "mge_solve")
+        n = min(nlog.value, cap)
+        return log[: 3 * n].reshape(n, 3), cs.value
+
+    # ---- data
+    def dims(self, lvl):
+        ni, nj, nk = self.levels[lvl - 1].n
+        return ni + 4, nj + 4, nk + 4
+
+    def download(self, lvl, field):
+        ib1, jb1, kb1 = self.dims(lvl)
+        nc = 5 if field in ("w", "dw", "fw", "ws", "wr", "qf") else 1
+        out = np.empty((nc, kb1, jb1, ib1), dtype=np.float64)
+        _ffi.check(self.lib.mge_download(self.h, lvl, 1, field.encode(), out.ctypes.data_as(_dp)), "mge_download")
+        return out if nc == 5 else out[0]
+
+    def upload(self, lvl, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _ffi.check(self.lib.mge_upload(self.h, lvl, 1, field.encode(), a.ctypes.data_as(_dp)), "mge_upload")
+
+    def cuts(self, lvl):
+        g = self.geoms[lvl - 1]
+        out = (_ffi.MgeCutRec * max(g.nmir, 1))()
+        n = self.lib.mge_get_cuts(self.h, lvl, 1, out)
+        _ffi.check(n, "mge_get_cuts")
+        return out, n
+
+    def wall_force(self, lvl=None):
+        lvl = self.nlevel if lvl is None else lvl
+        out = (C.c_double * 3)()
+        _ffi.check(self.lib.mge_wall_force(self.h, lvl, out), "mge_wall_force")
+        return np.array(out[:])
+
+    def dtim(self, lvl):
+        return self.lib.mge_get_dtim(self.h, lvl, 1)
+
+    def sync(self):
+        return _ffi.check(self.lib.mge_synchronize(self.h), "mge_synchronize")
+
+    def timing(self, enable=True, name=None):
+        ms = C.c_double(0.0)
+        cnt = C.c_longlong(0)
+        _ffi.check(self.lib.mge_kernel_timing(self.h, int(enable), name.encode() if name else None,
+                                              C.byref(ms), C.byref(cnt)), "mge_kernel_timing")
+        return ms.value, cnt.value
+
+
+def force_coefficients(case: Case, ref_area: float):
+    """Cd, Cl from the FP64 wall force of the finest level: drag along
+    (cos a cos b, sin a cos b, sin b), lift along (-sin a, cos a, 0),
+    normalised by q_inf * ref_area."""
+    d = case.params.derived()
+    a, b = d["alpha"], d["beta"]
+    F = case.wall_force()
+    q = case.params.dynamic_pressure()
+    dd = np.array([np.cos(a) * np.cos(b), np.sin(a) * np.cos(b), np.sin(b)])
+    ll = np.array([-np.sin(a), np.cos(a), 0.0])
+    return float(F @ dd / (q * ref_area)), float(F @ ll / (q * ref_area))

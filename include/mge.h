@@ -1,0 +1,163 @@
+/*
+ * mge.h -- C ABI of the MI355X-native multigrid Jameson-RK solve step.
+ *
+ * The entry points keep the granularity of code_mge's Fortran call
+ * surface, so a Fortran driver binds them 1:1 with iso_c_binding:
+ *
+ *   mge_timestep   <- timestep(b, lwork)                       srcf90/timestep.f90:1
+ *   mge_euler      <- euler(b,itime,ncyc,mgl,mlvl,lres,lfir,…) srcf90/euler.f90:1
+ *   mge_bcond      <- bcond(b)                                 srcf90/bcond.f90:1
+ *   mge_bc_cutcell <- bc_cutcell(b)                            srcf90/bc_cutcell.f90:1
+ *   mge_bc_overlap <- bc_overlap(blk, lv)                      srcf90/bc_overlap.f90:265
+ *   mge_movco      <- movco(blk, mgl, nmesh, mblks)            srcf90/movco.f90:1
+ *   mge_movfin     <- movfin(blk, mgl, mlvl, nmesh, mblks)     srcf90/movfin.f90:1
+ *   mge_init_flow  <- init_flow(blk, lrflow, imodel)           srcf90/init_flow.f90:1
+ *   mge_solve      <- solver(blk, project, job, model)         srcf90/solver.f90:1
+ *   mge_level_define/mge_case_create  <- make_flow + init_metric
+ *                                       (srcf90/make_flow.f90:1, init_metric.f90:1)
+ *   mge_geom_level <- the cut-cell part of make_fsijob + solid_ipolcoef
+ *                     (srcf90/main.f90:44,57; their library is not in the
+ *                     reference tree, so the record layout follows the
+ *                     fields that bc_cutcell/efluxb/dfluxb read)
+ *
+ * Conventions
+ *  - Every function returns int: 0 = OK, < 0 = error (one code per
+ *    reference `stop` site), > 0 = warning (reference printed and went on).
+ *  - The case handle owns all device memory.  Host arrays are read or
+ *    written only during the call that receives them; none is retained.
+ *  - Calls on one handle are ordered on the handle's HIP stream and are not
+ *    re-entrant.  Different handles are independent.
+ *  - Levels are numbered 1..nlevel with 1 the coarsest (the reference's
+ *    mgl); each level is one Cartesian block (blk = 1).
+ *  - Field arrays crossing the ABI are Fortran-ordered (0:ib,0:jb,0:kb[,5])
+ *    with ib = ni+3: i fastest, component slowest, FP64.
+ */
+#ifndef MGE_H
+#define MGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MGE_OK               0
+#define MGE_ERR_NEG_C2     -10  /* c^2 < 0 or NaN in timestep   (srcf90/timestep.f90:45-49)      */
+#define MGE_ERR_WALL_RHO   -11  /* zero density at a wall face  (srcf90/efluxb.f90:246-249)      */
+#define MGE_ERR_WSOL_NAN   -12  /* NaN mirror state             (srcf90/bc_cutcell.f90:307-317)  */
+#define MGE_ERR_RES_NAN    -13  /* NaN density residual         (srcf90/solver.f90:180)          */
+#define MGE_ERR_PERIODIC   -14  /* unpaired periodic face       (srcf90/bcond.f90:26-31)         */
+#define MGE_ERR_DISTYPE    -15  /* bad distype / dissip         (srcf90/euler.f90:212-214)       */
+#define MGE_ERR_SMOO_DIM   -16  /* smoothing needs >= 2 cells   (srcf90/psmoo.f90:132,166)       */
+#define MGE_ERR_ARG        -20  /* bad argument                                                  */
+#define MGE_ERR_HIP        -21  /* HIP runtime failure                                           */
+#define MGE_ERR_NOMEM      -22  /* device allocation failed                                      */
+#define MGE_ERR_STATE      -23  /* call out of order (e.g. level not defined)                    */
+#define MGE_WARN_ZERO_CLOUD  1  /* empty reflected-node cloud: early return (bc_cutcell.f90:225-230) */
+#define MGE_WARN_BCTYPE      2  /* unknown bctype, face skipped (bcond.f90:35)                    */
+
+/* scheme and freestream parameters (typepara, filled as input_param.f90:13-115) */
+typedef struct {
+  int    mstage;              /* RK stages (<= 8)                                  */
+  double cstp[8], cdis[8];    /* stage and dissipation-blending coefficients       */
+  double cfl;                 /* |cfl|                                             */
+  int    vt;                  /* 0 local time step (jobsheet cfl < 0), 1 global    */
+  double vis2, vis4;          /* vis4 already divided by 32                        */
+  double smoo;                /* constant residual-smoothing coefficient           */
+  int    mgtype;              /* 1 V-cycle, 2 W-cycle                              */
+  double gamma;
+  double rho0, p0, c0, ei0, u0, v0, w0, h0;
+  int    eqntype;             /* 0 Euler, 1 laminar Navier-Stokes                  */
+  int    dissip;              /* 0 scalar, 1 matrix                                */
+  double viscl, prlmean, gas_const;
+  double drop;                /* solve exits when ini_res/res > drop (1e6 in the reference) */
+  int    distype;             /* 0 central (JST)                                   */
+  int    precision;           /* 64: FP64 throughout                               */
+} mge_params;
+
+/* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record
+ * that bc_cutcell.f90, efluxb.f90 and dfluxb.f90 read.  Indices follow the
+ * Fortran block numbering (interior cells 2..ncell+1). */
+typedef struct {
+  int    faceid;              /* 1..6: side of the cut cell holding the solid cell */
+  int    zz;                  /* direction 1..3                                    */
+  int    ccid[3], scid[3];    /* cut cell, solid cell                              */
+  double psurf[3], nvec[3], radcur[2], rr;
+  int    npoint;              /* reflected-node cloud size (<= 8)                  */
+  int    ijk[8][3];
+  double coef[8];
+  double psol, wsol[5];       /* mirror state (output of bc_cutcell)               */
+} mge_cutrec;
+
+/* analytic immersed bodies for the geometry preprocessor */
+#define MGE_BODY_NONE      0
+#define MGE_BODY_CYLINDER  1  /* circular cylinder, radius semi[0], axis `axis` */
+#define MGE_BODY_SPHERE    2  /* radius semi[0]                                  */
+#define MGE_BODY_ELLIPSOID 3  /* semi-axes semi[0..2] in body frame, rot = body->world (row-major) */
+typedef struct {
+  int    kind;
+  int    axis;
+  double center[3];
+  double semi[3];
+  double rot[9];
+} mge_body;
+
+typedef struct mge_case mge_case;
+
+/* ---- geometry preprocessor (host only, no GPU needed) ------------------ */
+/* Classifies the interior cells of one block (cell centre inside the body =
+ * solid) and builds one record per (fluid, solid) face neighbour with the
+ * surface point, normal, mirror distance and a trilinear reflected-node
+ * cloud over fluid cells.  xv/yv/zv: vertex coordinates x(0..ib).
+ * blank: ni*nj*nk bytes (i fastest, 1 fluid, 0 solid).  If recs is NULL or
+ * cap is too small only *nmir is set. */
+int mge_geom_level(const mge_body *body, int ni, int nj, int nk,
+                   const double *xv, const double *yv, const double *zv,
+                   uint8_t *blank, mge_cutrec *recs, int cap, int *nmir);
+
+/* ---- case lifetime (make_flow.f90 / init_metric.f90) -------------------- */
+int mge_case_create(const mge_params *prm, int nlevel, mge_case **out);
+int mge_case_destroy(mge_case *h);
+/* ratio: coarsening factor (1 or 2 per direction) from this level to lvl-1 */
+int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk,
+                     const double *xv, const double *yv, const double *zv,
+                     double cellvolume, const int bctype[6], const int ratio[3],
+                     const uint8_t *blank, int nmir, const mge_cutrec *recs,
+                     int mxcyc);
+
+/* ---- the solve step (reference call surface) ---------------------------- */
+int mge_init_flow(mge_case *h);
+int mge_timestep(mge_case *h, int lvl, int blk);
+/* rtrms: density-residual RMS of this call when lres == 0 (may be NULL) */
+int mge_euler(mge_case *h, int lvl, int blk, int itime, int ncyc, int mgl,
+              int mlvl, int lres, int lfir, double *rtrms);
+int mge_bcond(mge_case *h, int lvl, int blk);
+int mge_bc_cutcell(mge_case *h, int lvl, int blk);
+int mge_bc_overlap(mge_case *h, int lvl);
+int mge_movco(mge_case *h, int mgl);
+int mge_movfin(mge_case *h, int mgl, int mlvl);
+/* full FMG/FAS solve (solver.f90); reslog receives (level, ncyc, res)
+ * triples like <job>_res.dat; cell_stages counts residual+update passes. */
+int mge_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog,
+              long long *cell_stages);
+
+/* ---- data movement and results ------------------------------------------ */
+/* field: "w","p","dw","fw","ws","wr","qf","vdtim","radsum" (radi+radj+radk) */
+int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out);
+int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *in);
+int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out);
+double mge_get_dtim(mge_case *h, int lvl, int blk);
+/* FP64 wall force (Fx,Fy,Fz) on the body at level lvl: the momentum flux
+ * the scheme applies at each wall face (efluxb.f90:251-253), in record order */
+int mge_wall_force(mge_case *h, int lvl, double out[3]);
+int mge_synchronize(mge_case *h);
+/* mean device duration (ms) and launch count of the named kernel since the
+ * last reset, when timing is enabled; name NULL resets */
+int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count);
+const char *mge_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1516 @@
+/*
+ * oracle.c -- CPU restatement of code_mge's multigrid Jameson-RK solve step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Each routine cites the
+ * reference file:line it restates.  Expressions keep the reference's
+ * left-to-right evaluation order; x**2.0d0 is written x*x; Fortran
+ * sum() over a small array accumulates from 0 in index order.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  int ni, nj, nk;
+  int il, jl, kl, ie, je, ke, ib, jb, kb;
+  long E;
+  double *x;              /* vertices (0:ib,0:jb,0:kb,3) */
+  double *si, *sj, *sk;   /* face vectors (…,3) (init_metric.f90:208-274) */
+  double cellvolume;
+  int bctype[6];
+  double facenvec[6][3];
+  double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
+  double *radi, *radj, *radk, *vdtim;
+  double dtim;
+  int *blank, *marker;
+  int nmir;
+  orc_cut *z;
+  double *rtrms;
+  int mxcyc;
+  int ratio[3];
+  int defined;
+} blk_t;
+
+struct orc_case {
+  orc_params prm;
+  int nlevel;
+  blk_t *lv; /* lv[1..nlevel] */
+};
+
+#define IX(b, i, j, k) ((long)(i) + (long)((b)->ib + 1) * ((long)(j) + (long)((b)->jb + 1) * (long)(k)))
+#define W(b, n, c) ((b)->w[(long)(n) * (b)->E + (c)])
+
+static double dimf(double a, double b) { return a > b ? a - b : 0.0; }
+static double max2(double a, double b) { return a > b ? a : b; }
+static double min2(double a, double b) { return a < b ? a : b; }
+
+/* depvars_one (depvars.f90:37-52): p = (gamma-1)*dim(rhoE, 0.5*|m|^2/rho) */
+static double depvars1(double gamma, const double *cv5) {
+  double s = 0.0;
+  s = s + cv5[1] * cv5[1];
+  s = s + cv5[2] * cv5[2];
+  s = s + cv5[3] * cv5[3];
+  double qq = 0.5 * s / cv5[0];
+  return (gamma - 1.0) * dimf(cv5[4], qq);
+}
+
+static void get5(const blk_t *b, long c, double *v) {
+  for (int n = 0; n < 5; n++) v[n] = W(b, n, c);
+}
+static void put5(blk_t *b, long c, const double *v) {
+  for (int n = 0; n < 5; n++) W(b, n, c) = v[n];
+}
+
+/* depvars at one cell of the block */
+static void eos_cell(blk_t *b, double gamma, long c) {
+  double v[5];
+  get5(b, c, v);
+  b->p[c] = depvars1(gamma, v);
+}
+
+/* ------------------------------------------------------------------ setup */
+
+orc_case *orc_case_new(const orc_params *prm, int nlevel) {
+  orc_case *c = (orc_case *)calloc(1, sizeof(orc_case));
+  c->prm = *prm;
+  c->nlevel = nlevel;
+  c->lv = (blk_t *)calloc((size_t)nlevel + 1, sizeof(blk_t));
+  return c;
+}
+
+static void blk_free(blk_t *b) {
+  free(b->x); free(b->si); free(b->sj); free(b->sk);
+  free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
+  free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
+  free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
+}
+
+void orc_case_free(orc_case *c) {
+  if (!c) return;
+  for (int l = 1; l <= c->nlevel; l++) blk_free(&c->lv[l]);
+  free(c->lv);
+  free(c);
+}
+
+/* make_flow.f90:36-88 allocations + init_metric.f90:137-315 */
+int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
+                  const double *xv, const double *yv, const double *zv,
+                  double cellvolume, const int bctype[6],
+                  const uint8_t *blank_int, int nmir, const orc_cut *cuts,
+                  const int ratio[3], int mxcyc) {
+  if (lvl < 1 || lvl > c->nlevel) return ORC_ERR_ARG;
+  blk_t *b = &c->lv[lvl];
+  blk_free(b);
+  memset(b, 0, sizeof(*b));
+  b->ni = ni; b->nj = nj; b->nk = nk;
+  b->il = ni + 1; b->jl = nj + 1; b->kl = nk + 1;
+  b->ie = ni + 2; b->je = nj + 2; b->ke = nk + 2;
+  b->ib = ni + 3; b->jb = nj + 3; b->kb = nk + 3;
+  long E = (long)(b->ib + 1) * (b->jb + 1) * (b->kb + 1);
+  b->E = E;
+  b->cellvolume = cellvolume;
+  for (int f = 0; f < 6; f++) b->bctype[f] = bctype[f];
+  memset(b->facenvec, 0, sizeof(b->facenvec));
+  b->facenvec[0][0] = -1.0; b->facenvec[1][1] = -1.0; b->facenvec[2][2] = -1.0;
+  b->facenvec[3][0] = 1.0;  b->facenvec[4][1] = 1.0;  b->facenvec[5][2] = 1.0;
+  for (int d = 0; d < 3; d++) b->ratio[d] = ratio ? ratio[d] : 2;
+  b->x = (double *)calloc((size_t)E * 3, sizeof(double));
+  b->si = (double *)calloc((size_t)E * 3, sizeof(double));
+  b->sj = (double *)calloc((size_t)E * 3, sizeof(double));
+  b->sk = (double *)calloc((size_t)E * 3, sizeof(double));
+  b->w = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->ws = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->dw = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->fw = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->wn = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->wr = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->qf = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->p = (double *)calloc((size_t)E, sizeof(double));
+  b->radi = (double *)calloc((size_t)E, sizeof(double));
+  b->radj = (double *)calloc((size_t)E, sizeof(double));
+  b->radk = (double *)calloc((size_t)E, sizeof(double));
+  b->vdtim = (double *)calloc((size_t)E, sizeof(double));
+  b->blank = (int *)calloc((size_t)E, sizeof(int));
+  b->marker = (int *)calloc((size_t)E, sizeof(int));
+  b->mxcyc = mxcyc > 0 ? mxcyc : 1;
+  b->rtrms = (double *)calloc((size_t)b->mxcyc + 1, sizeof(double));
+  b->nmir = nmir;
+  b->z = (orc_cut *)calloc((size_t)(nmir > 0 ? nmir : 1), sizeof(orc_cut));
+  if (nmir > 0) memcpy(b->z, cuts, sizeof(orc_cut) * (size_t)nmir);
+
+  /* grid points (init_metric.f90:156-169): x(i,j,k) for i,j,k >= 1; index 0 stays 0 */
+  for (int k = 1; k <= b->kb; k++)
+    for (int j = 1; j <= b->jb; j++)
+      for (int i = 1; i <= b->ib; i++) {
+        long q = IX(b, i, j, k);
+        b->x[q] = xv[i];
+        b->x[E + q] = yv[j];
+        b->x[2 * E + q] = zv[k];
+      }
+  /* face vectors (init_metric.f90:216-274) */
+#define XV(i, j, k, d) (b->x[(long)(d) * E + IX(b, i, j, k)])
+  for (int k = 0; k <= b->ke; k++) {
+    int kp = k + 1;
+    for (int j = 0; j <= b->je; j++) {
+      int jp = j + 1;
+      for (int i = 0; i <= b->ie; i++) {
+        int ip = i + 1;
+        double x1[3], x2[3];
+        long q = IX(b, i, j, k);
+        for (int d = 0; d < 3; d++) {
+          x1[d] = XV(i, jp, k, d) - XV(i, j, kp, d);
+          x2[d] = XV(i, j, k, d) - XV(i, jp, kp, d);
+        }
+        b->si[q] = 0.5 * (x1[1] * x2[2] - x1[2] * x2[1]);
+        b->si[E + q] = 0.5 * (x1[2] * x2[0] - x1[0] * x2[2]);
+        b->si[2 * E + q] = 0.5 * (x1[0] * x2[1] - x1[1] * x2[0]);
+        for (int d = 0; d < 3; d++) {
+          x1[d] = XV(ip, j, k, d) - XV(i, j, kp, d);
+          x2[d] = XV(ip, j, kp, d) - XV(i, j, k, d);
+        }
+        b->sj[q] = 0.5 * (x1[1] * x2[2] - x1[2] * x2[1]);
+        b->sj[E + q] = 0.5 * (x1[2] * x2[0] - x1[0] * x2[2]);
+        b->sj[2 * E + q] = 0.5 * (x1[0] * x2[1] - x1[1] * x2[0]);
+        for (int d = 0; d < 3; d++) {
+          x1[d] = XV(ip, jp, k, d) - XV(i, j, k, d);
+          x2[d] = XV(ip, j, k, d) - XV(i, jp, k, d);
+        }
+        b->sk[q] = 0.5 * (x1[1] * x2[2] - x1[2] * x2[1]);
+        b->sk[E + q] = 0.5 * (x1[2] * x2[0] - x1[0] * x2[2]);
+        b->sk[2 * E + q] = 0.5 * (x1[0] * x2[1] - x1[1] * x2[0]);
+      }
+    }
+  }
+#undef XV
+  /* blank (init_metric.f90:278-315) */
+  for (long q = 0; q < E; q++) b->blank[q] = 1;
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++)
+        b->blank[IX(b, i, j, k)] =
+            blank_int[(long)(i - 2) + (long)ni * ((long)(j - 2) + (long)nj * (k - 2))] ? 1 : 0;
+  for (int k = 0; k <= b->kb; k++)
+    for (int j = 0; j <= b->jb; j++) {
+      b->blank[IX(b, 0, j, k)] = b->blank[IX(b, 2, j, k)];
+      b->blank[IX(b, 1, j, k)] = b->blank[IX(b, 2, j, k)];
+      b->blank[IX(b, b->ie, j, k)] = b->blank[IX(b, b->il, j, k)];
+      b->blank[IX(b, b->ib, j, k)] = b->blank[IX(b, b->il, j, k)];
+    }
+  for (int k = 0; k <= b->kb; k++)
+    for (int i = 0; i <= b->ib; i++) {
+      b->blank[IX(b, i, 0, k)] = b->blank[IX(b, i, 2, k)];
+      b->blank[IX(b, i, 1, k)] = b->blank[IX(b, i, 2, k)];
+      b->blank[IX(b, i, b->je, k)] = b->blank[IX(b, i, b->jl, k)];
+      b->blank[IX(b, i, b->jb, k)] = b->blank[IX(b, i, b->jl, k)];
+    }
+  for (int j = 0; j <= b->jb; j++)
+    for (int i = 0; i <= b->ib; i++) {
+      b->blank[IX(b, i, j, 0)] = b->blank[IX(b, i, j, 2)];
+      b->blank[IX(b, i, j, 1)] = b->blank[IX(b, i, j, 2)];
+      b->blank[IX(b, i, j, b->ke)] = b->blank[IX(b, i, j, b->kl)];
+      b->blank[IX(b, i, j, b->kb)] = b->blank[IX(b, i, j, b->kl)];
+    }
+  b->defined = 1;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ timestep */
+
+/* timestep.f90:1-282 (the smoothing coefficients of :286-309 feed only the
+ * unused psmoo and are not formed). */
+int orc_timestep(orc_case *c, int lvl) {
+  blk_t *b = &c->lv[lvl];
+  const orc_params *P = &c->prm;
+  const long E = b->E;
+  const int il = b->il, jl = b->jl, kl = b->kl, ie = b->ie, je = b->je, ke = b->ke;
+  const int ib = b->ib, jb = b->jb, kb = b->kb;
+  const double gamma = P->gamma, cfl = P->cfl;
+  const double plim = 0.001;
+  double cfac = 4.0;
+  b->dtim = 10.0;
+  for (int k = 2; k <= kl; k++)
+    for (int j = 2; j <= jl; j++)
+      for (int i = 2; i <= il; i++) {
+        long q = IX(b, i, j, k);
+        double cc = gamma * b->p[q] / W(b, 0, q);
+        if (cc < 0.0 || cc != cc) return ORC_ERR_NEG_C2;
+        cc = sqrt(max2(cc, 0.0));
+        double uu = W(b, 1, q) / W(b, 0, q);
+        double uv = W(b, 2, q) / W(b, 0, q);
+        double uw = W(b, 3, q) / W(b, 0, q);
+        double dtv = 0.0, radvi = 0.0, radvj = 0.0, radvk = 0.0;
+        if (P->eqntype == 1) { /* timestep.f90:141-145 */
+          double fmue = P->viscl / P->prlmean;
+          double f1 = 4.0 / W(b, 0, q) / 3.0;
+          double f2 = gamma / W(b, 0, q);
+          double fac = max2(f1, f2);
+          dtv = fac * fmue / b->cellvolume;
+        }
+        long qi = IX(b, i + 1, j, k), qj = IX(b, i, j + 1, k), qk = IX(b, i, j, k + 1);
+        double sx, sy, sz, qs, ds, cs;
+        sx = 0.5 * (b->si[q] + b->si[qi]);
+        sy = 0.5 * (b->si[E + q] + b->si[E + qi]);
+        sz = 0.5 * (b->si[2 * E + q] + b->si[2 * E + qi]);
+        qs = fabs(uu * sx + uv * sy + uw * sz);
+        ds = sx * sx + sy * sy + sz * sz;
+        cs = cc * sqrt(ds);
+        b->radi[q] = qs + cs;
+        radvi = dtv * ds;
+        sx = 0.5 * (b->sj[q] + b->sj[qj]);
+        sy = 0.5 * (b->sj[E + q] + b->sj[E + qj]);
+        sz = 0.5 * (b->sj[2 * E + q] + b->sj[2 * E + qj]);
+        qs = fabs(uu * sx + uv * sy + uw * sz);
+        ds = sx * sx + sy * sy + sz * sz;
+        cs = cc * sqrt(ds);
+        b->radj[q] = qs + cs;
+        radvj = dtv * ds;
+        sx = 0.5 * (b->sk[q] + b->sk[qk]);
+        sy = 0.5 * (b->sk[E + q] + b->sk[E + qk]);
+        sz = 0.5 * (b->sk[2 * E + q] + b->sk[2 * E + qk]);
+        qs = fabs(uu * sx + uv * sy + uw * sz);
+        ds = sx * sx + sy * sy + sz * sz;
+        cs = cc * sqrt(ds);
+        b->radk[q] = qs + cs;
+        radvk = dtv * ds;
+        double fict;
+        if (P->eqntype == 1)
+          fict = b->cellvolume / (b->radi[q] + b->radj[q] + b->radk[q] +
+                                  cfac * (radvi + radvj + radvk));
+        else
+          fict = b->cellvolume / (b->radi[q] + b->radj[q] + b->radk[q]);
+        b->vdtim[q] = fict;
+        double pc = b->p[q];
+        double pip = b->p[qi], pim = b->p[IX(b, i - 1, j, k)];
+        double pjp = b->p[qj], pjm = b->p[IX(b, i, j - 1, k)];
+        double pkp = b->p[qk], pkm = b->p[IX(b, i, j, k - 1)];
+        double dpi = fabs((pip - 2.0 * pc + pim) / (pip + 2.0 * pc + pim + plim));
+        double dpj = fabs((pjp - 2.0 * pc + pjm) / (pjp + 2.0 * pc + pjm + plim));
+        double dpk = fabs((pkp - 2.0 * pc + pkm) / (pkp + 2.0 * pc + pkm + plim));
+        double r = 1.0 / (1.0 + min2(dimf(cfl, 1.0), 2.0 * (dpi + dpj + dpk * (1.0 - 0))));
+        b->vdtim[q] = b->vdtim[q] * r;
+        if (b->vdtim[q] < b->dtim) b->dtim = b->vdtim[q];
+      }
+  /* zero on blank cells (timestep.f90:212-223) */
+  for (long q = 0; q < E; q++)
+    if (b->blank[q] == 0) {
+      b->vdtim[q] = 0.0; b->radi[q] = 0.0; b->radj[q] = 0.0; b->radk[q] = 0.0;
+    }
+  double *R[3] = {b->radi, b->radj, b->radk};
+  /* ghost copies in the reference order (timestep.f90:227-282) */
+  for (int i = 0; i <= ib; i++)
+    for (int j = 0; j <= jb; j++)
+      for (int r = 0; r < 3; r++) {
+        R[r][IX(b, i, j, 0)] = R[r][IX(b, i, j, 2)];
+        R[r][IX(b, i, j, 1)] = R[r][IX(b, i, j, 2)];
+        R[r][IX(b, i, j, ke)] = R[r][IX(b, i, j, kl)];
+        R[r][IX(b, i, j, kb)] = R[r][IX(b, i, j, kl)];
+      }
+  for (int j = 2; j <= jl; j++)
+    for (int k = 0; k <= kb; k++)
+      for (int r = 0; r < 3; r++) {
+        R[r][IX(b, 0, j, k)] = R[r][IX(b, 2, j, k)];
+        R[r][IX(b, 1, j, k)] = R[r][IX(b, 2, j, k)];
+        R[r][IX(b, ie, j, k)] = R[r][IX(b, il, j, k)];
+        R[r][IX(b, ib, j, k)] = R[r][IX(b, il, j, k)];
+      }
+  for (int i = 2; i <= il; i++)
+    for (int k = 2; k <= kl; k++)
+      for (int r = 0; r < 3; r++) {
+        R[r][IX(b, i, 0, k)] = R[r][IX(b, i, 2, k)];
+        R[r][IX(b, i, 1, k)] = R[r][IX(b, i, 2, k)];
+        R[r][IX(b, i, je, k)] = R[r][IX(b, i, jl, k)];
+        R[r][IX(b, i, jb, k)] = R[r][IX(b, i, jl, k)];
+      }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ fluxes */
+
+/* ceflux (ceflux.f90:1-25; eflux.f90:133-155) */
+static void ceflux(const double *w, double p, const double *ss, double *fs) {
+  double f1 = 0.0;
+  f1 = f1 + w[1] * ss[0];
+  f1 = f1 + w[2] * ss[1];
+  f1 = f1 + w[3] * ss[2];
+  fs[0] = f1;
+  fs[1] = f1 * w[1] / w[0] + p * ss[0];
+  fs[2] = f1 * w[2] / w[0] + p * ss[1];
+  fs[3] = f1 * w[3] / w[0] + p * ss[2];
+  fs[4] = f1 * (w[4] + p) / w[0];
+}
+
+static void svec(const double *S, long E, long q, double *s) {
+  s[0] = S[q]; s[1] = S[E + q]; s[2] = S[2 * E + q];
+}
+
+/* dflux_matrix (dflux.f90:283-339) */
+static void dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
+                         const double *ss, double *ff) {
+  double gam1 = gamma - 1.0, epsn = 1.0, epsl = 1.0;
+  double rr = 0.5 * (u1[0] + u2[0]);
+  double us[3];
+  for (int d = 0; d < 3; d++) us[d] = 0.5 * (u1[1 + d] + u2[1 + d]) / rr;
+  double ee = 0.5 * (u1[4] + u2[4]) / rr;
+  double su = 0.0;
+  su = su + us[0] * us[0]; su = su + us[1] * us[1]; su = su + us[2] * us[2];
+  double qq = 0.5 * su;
+  double c2 = gamma * gam1 * (ee - qq);
+  double hh = c2 / gam1 + qq;
+  double sss = 0.0;
+  sss = sss + ss[0] * ss[0]; sss = sss + ss[1] * ss[1]; sss = sss + ss[2] * ss[2];
+  double cs = sqrt(c2 * sss);
+  double qn = 0.0;
+  qn = qn + ss[0] * us[0]; qn = qn + ss[1] * us[1]; qn = qn + ss[2] * us[2];
+  double e0 = fabs(qn) + cs;
+  double e1 = max2(epsn * e0, fabs(qn + cs));
+  double e2 = max2(epsn * e0, fabs(qn - cs));
+  double e3 = max2(epsl * e0, fabs(qn));
+  double s1 = 0.5 * (e1 + e2), s2 = 0.5 * (e1 - e2), s3 = e3;
+  double ak1 = (s1 - s3) / c2 * gam1;
+  double ak2 = s2 / cs;
+  double ak3 = c2 / gam1 / sss;
+  double vd = qq * dw[0] - us[0] * dw[1] - us[1] * dw[2] - us[2] * dw[3] + dw[4];
+  double qd = qn * dw[0] - ss[0] * dw[1] - ss[1] * dw[2] - ss[2] * dw[3];
+  double f[5];
+  f[0] = ak1 * vd - ak2 * qd + s3 * dw[0];
+  f[1] = ak1 * (us[0] * vd - ss[0] * ak3 * qd) + ak2 * (gam1 * ss[0] * vd - us[0] * qd) + s3 * dw[1];
+  f[2] = ak1 * (us[1] * vd - ss[1] * ak3 * qd) + ak2 * (gam1 * ss[1] * vd - us[1] * qd) + s3 * dw[2];
+  f[3] = ak1 * (us[2] * vd - ss[2] * ak3 * qd) + ak2 * (gam1 * ss[2] * vd - us[2] * qd) + s3 * dw[3];
+  f[4] = ak1 * (hh * vd - qn * ak3 * qd) + ak2 * (gam1 * qn * vd - hh * qd) + s3 * dw[4];
+  for (int n = 0; n < 5; n++) ff[n] = f[n];
+}
+
+/* face dissipation vector of one line sweep (dflux.f90:70-111 and repeats).
+ * q: pointers into the line; returns d(0..5) for faces 2..ie of the line in
+ * dfs[face*5+n].  mode selects the pressure-sum order of the j sweep. */
+typedef struct {
+  const orc_params *P;
+  double cd;
+} dctx;
+
+/* generic sweep along direction dir for fixed (a,b) (the two other indices) */
+static void dflux_line(blk_t *b, const orc_params *P, double cdis, int dir, int a1, int a2,
+                       double *fp, double *diw, double *dfs) {
+  int n_e, n_b;
+  if (dir == 0) { n_e = b->ie; n_b = b->ib; }
+  else if (dir == 1) { n_e = b->je; n_b = b->jb; }
+  else { n_e = b->ke; n_b = b->kb; }
+  const long E = b->E;
+#define LQ(s) (dir == 0 ? IX(b, (s), a1, a2) : (dir == 1 ? IX(b, a1, (s), a2) : IX(b, a1, a2, (s))))
+  for (int s = 1; s <= n_e; s++) {
+    double pm = b->p[LQ(s - 1)], pc = b->p[LQ(s)], pp = b->p[LQ(s + 1)];
+    double pb, num;
+    if (dir == 1) { /* dflux.f90:131-133: p(j+1) + 2p(j) + p(j-1) */
+      pb = fabs(pp + 2.0 * pc + pm);
+      num = fabs(pp - 2.0 * pc + pm);
+    } else {        /* dflux.f90:71-73, 188-190: p(i-1) + 2p(i) + p(i+1) */
+      pb = fabs(pm + 2.0 * pc + pp);
+      num = fabs(pm - 2.0 * pc + pp);
+    }
+    if (fabs(pb) > 0.0) fp[s] = num / pb;
+    else fp[s] = 0.0;
+  }
+  fp[0] = 0.0;
+  fp[n_b] = 0.0;
+  for (int s = 2; s <= n_e; s++)
+    for (int n = 0; n < 5; n++) diw[s * 5 + n] = W(b, n, LQ(s)) - W(b, n, LQ(s - 1));
+  for (int n = 0; n < 5; n++) {
+    diw[1 * 5 + n] = diw[2 * 5 + n];
+    diw[n_b * 5 + n] = diw[n_e * 5 + n];
+  }
+  const double *S = dir == 0 ? b->si : (dir == 1 ? b->sj : b->sk);
+  for (int s = 2; s <= n_e; s++) {
+    double ep = max2(max2(max2(fp[s - 2], fp[s - 1]), fp[s]), fp[s + 1]);
+    long qp = LQ(s), qm = LQ(s - 1);
+    double lp = b->radi[qp] + b->radj[qp] + b->radk[qp];
+    double lm = b->radi[qm] + b->radj[qm] + b->radk[qm];
+    double lam = 0.5 * (lm + lp);
+    double dis2 = P->vis2 * ep;
+    double dis4 = P->vis4;
+    dis4 = dimf(dis4, dis2);
+    dis2 = dis2 * cdis;
+    dis4 = dis4 * cdis;
+    double d[5];
+    for (int n = 0; n < 5; n++) {
+      double t = diw[(s + 1) * 5 + n] - 2.0 * diw[s * 5 + n] + diw[(s - 1) * 5 + n];
+      d[n] = dis2 * diw[s * 5 + n] - dis4 * t;
+    }
+    if (P->dissip == 0) {
+      for (int n = 0; n < 5; n++) d[n] = d[n] * lam;
+    } else {
+      double um[5], uc[5], ss[3], fs[5];
+      get5(b, qm, um); get5(b, qp, uc); svec(S, E, qp, ss);
+      dflux_matrix(P->gamma, um, uc, d, ss, fs);
+      for (int n = 0; n < 5; n++) d[n] = fs[n];
+    }
+    for (int n = 0; n < 5; n++) dfs[s * 5 + n] = d[n];
+  }
+#undef LQ
+}
+
+/* dfluxb (dfluxb.f90:29-423): wall corrections of the dissipation */
+static void dfluxb(blk_t *b, const orc_params *P, double cdis,
+                   const double *dfsx, const double *dfsy, const double *dfsz) {
+  const long E = b->E;
+  for (int m = 0; m < b->nmir; m++) {
+    orc_cut *t = &b->z[m];
+    int id = t->faceid, zz = t->zz;
+    int i = t->ccid[0], j = t->ccid[1], k = t->ccid[2];
+    double psol = t->psol;
+    double wsol[5];
+    for (int n = 0; n < 5; n++) wsol[n] = t->wsol[n];
+    wsol[4] = wsol[4] + psol;
+    /* unify the three directions: s = index along zz, LQ(s) the cell */
+    int s0 = zz == 1 ? i : (zz == 2 ? j : k);
+    int nbmax = zz == 1 ? b->ib : (zz == 2 ? b->jb : b->kb);
+    const double *dfs = zz == 1 ? dfsx : (zz == 2 ? dfsy : dfsz);
+    const double *S = zz == 1 ? b->si : (zz == 2 ? b->sj : b->sk);
+#define LQ(s) (zz == 1 ? IX(b, (s), j, k) : (zz == 2 ? IX(b, i, (s), k) : IX(b, i, j, (s))))
+#define PP(s) (b->p[LQ(s)])
+#define RS(s) (b->radi[LQ(s)] + b->radj[LQ(s)] + b->radk[LQ(s)])
+#define FW(n, s) (b->fw[(long)(n) * E + LQ(s)])
+#define DF(n, s) (dfs[(long)(n) * E + LQ(s)])
+    double fp[3], ep, lm, lp, lam, dis2, dis4, diw[3][5], fs1[5], fs2[5];
+    int hi = (id == 4 || id == 5 || id == 6);
+    int lo = (id == 1 || id == 2 || id == 3);
+    if (hi) {
+      int s = s0;
+      for (int n = 0; n < 5; n++) {
+        FW(n, s) = FW(n, s) - (DF(n, s) - DF(n, s + 1));
+        FW(n, s - 1) = FW(n, s - 1) - (DF(n, s - 1) - DF(n, s));
+      }
+      if (zz == 2) { /* dfluxb.f90:177-180 */
+        fp[0] = fabs(PP(s) - 2.0 * PP(s - 1) + PP(s - 2)) / fabs(PP(s) + 2.0 * PP(s - 1) + PP(s - 2));
+        fp[1] = fabs(psol - 2.0 * PP(s) + PP(s - 1)) / fabs(psol + 2.0 * PP(s) + PP(s - 1));
+      } else if (zz == 3) { /* dfluxb.f90:301-304 */
+        fp[0] = fabs(PP(s) - 2.0 * PP(s - 1) + PP(s - 2)) / fabs(PP(s) + 2.0 * PP(s - 1) + PP(s - 2));
+        fp[1] = fabs(psol - 2.0 * PP(s) + PP(s - 1)) / fabs(psol + 2.0 * PP(s) + PP(s - 1));
+      } else { /* dfluxb.f90:49-52 */
+        fp[0] = fabs(PP(s - 2) - 2.0 * PP(s - 1) + PP(s)) / fabs(PP(s - 2) + 2.0 * PP(s - 1) + PP(s));
+        fp[1] = fabs(PP(s - 1) - 2.0 * PP(s) + psol) / fabs(PP(s - 1) + 2.0 * PP(s) + psol);
+      }
+      ep = max2(fp[0], fp[1]);
+      lm = RS(s);
+      lam = lm;
+      dis2 = P->vis2 * ep; dis4 = P->vis4; dis4 = dimf(dis4, dis2);
+      dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+      for (int n = 0; n < 5; n++) {
+        diw[0][n] = W(b, n, LQ(s)) - W(b, n, LQ(s - 1));
+        diw[1][n] = wsol[n] - W(b, n, LQ(s));
+        diw[2][n] = diw[1][n];
+        fs2[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+        fs2[n] = dis2 * diw[1][n] - dis4 * fs2[n];
+      }
+      if (P->dissip == 0) {
+        for (int n = 0; n < 5; n++) fs2[n] = fs2[n] * lam;
+      } else {
+        double uc[5], ss[3], o[5];
+        get5(b, LQ(s), uc); svec(S, E, LQ(s + 1), ss);
+        /* wsol here carries the enthalpy; the cell state too (dflux swapped it) */
+        dflux_matrix(P->gamma, uc, wsol, fs2, ss, o);
+        for (int n = 0; n < 5; n++) fs2[n] = o[n];
+      }
+      if ((s - 3) >= 0) {
+        if (zz == 1)
+          fp[2] = fabs(PP(s - 1) - 2.0 * PP(s - 2) + PP(s - 3)) / fabs(PP(s - 1) + 2.0 * PP(s - 2) + PP(s - 3));
+        else
+          fp[2] = fabs(PP(s - 1) - 2.0 * PP(s - 2) + PP(s - 3)) / fabs(PP(s - 1) + 2.0 * PP(s - 2) + PP(s - 3));
+      } else {
+        fp[2] = 0.0;
+      }
+      ep = max2(max2(fp[0], fp[1]), fp[2]);
+      lm = RS(s - 1);
+      lp = RS(s);
+      lam = 0.5 * (lm + lp);
+      dis2 = P->vis2 * ep; dis4 = P->vis4; dis4 = dimf(dis4, dis2);
+      dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+      for (int n = 0; n < 5; n++) {
+        diw[0][n] = W(b, n, LQ(s - 1)) - W(b, n, LQ(s - 2));
+        diw[1][n] = W(b, n, LQ(s)) - W(b, n, LQ(s - 1));
+        diw[2][n] = wsol[n] - W(b, n, LQ(s));
+        fs1[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+        fs1[n] = dis2 * diw[1][n] - dis4 * fs1[n];
+      }
+      if (P->dissip == 0) {
+        for (int n = 0; n < 5; n++) fs1[n] = fs1[n] * lam;
+      } else {
+        double um[5], uc[5], ss[3], o[5];
+        get5(b, LQ(s - 1), um); get5(b, LQ(s), uc); svec(S, E, LQ(s), ss);
+        dflux_matrix(P->gamma, um, uc, fs1, ss, o);
+        for (int n = 0; n < 5; n++) fs1[n] = o[n];
+      }
+      for (int n = 0; n < 5; n++) {
+        FW(n, s) = FW(n, s) + (fs1[n] - fs2[n]);
+        FW(n, s - 1) = FW(n, s - 1) + (DF(n, s - 1) - fs1[n]);
+      }
+    } else if (lo) {
+      int s = s0;
+      for (int n = 0; n < 5; n++) {
+        FW(n, s) = FW(n, s) - (DF(n, s) - DF(n, s + 1));
+        FW(n, s + 1) = FW(n, s + 1) - (DF(n, s + 1) - DF(n, s + 2));
+      }
+      /* dfluxb.f90:112-116 (same form in all three directions) */
+      fp[0] = fabs(PP(s + 1) - 2.0 * PP(s) + psol) / fabs(PP(s + 1) + 2.0 * PP(s) + psol);
+      fp[1] = fabs(PP(s + 2) - 2.0 * PP(s + 1) + PP(s)) / fabs(PP(s + 2) + 2.0 * PP(s + 1) + PP(s));
+      if ((s + 3) <= nbmax)
+        fp[2] = fabs(PP(s + 3) - 2.0 * PP(s + 2) + PP(s + 1)) / fabs(PP(s + 3) + 2.0 * PP(s + 2) + PP(s + 1));
+      else
+        fp[2] = 0.0;
+      ep = max2(max2(fp[0], fp[1]), fp[2]);
+      lm = RS(s);
+      lp = RS(s + 1);
+      lam = 0.5 * (lm + lp);
+      dis2 = P->vis2 * ep; dis4 = P->vis4; dis4 = dimf(dis4, dis2);
+      dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+      for (int n = 0; n < 5; n++) {
+        diw[0][n] = W(b, n, LQ(s)) - wsol[n];
+        diw[1][n] = W(b, n, LQ(s + 1)) - W(b, n, LQ(s));
+        diw[2][n] = W(b, n, LQ(s + 2)) - W(b, n, LQ(s + 1));
+        fs2[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+        fs2[n] = dis2 * diw[1][n] - dis4 * fs2[n];
+      }
+      if (P->dissip == 0) {
+        for (int n = 0; n < 5; n++) fs2[n] = fs2[n] * lam;
+      } else {
+        double uc[5], up[5], ss[3], o[5];
+        get5(b, LQ(s), uc); get5(b, LQ(s + 1), up); svec(S, E, LQ(s + 1), ss);
+        dflux_matrix(P->gamma, uc, up, fs2, ss, o);
+        for (int n = 0; n < 5; n++) fs2[n] = o[n];
+      }
+      ep = max2(fp[0], fp[1]);
+      lp = RS(s);
+      lam = lp;
+      dis2 = P->vis2 * ep; dis4 = P->vis4; dis4 = dimf(dis4, dis2);
+      dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+      for (int n = 0; n < 5; n++) {
+        diw[1][n] = W(b, n, LQ(s)) - wsol[n];
+        diw[0][n] = diw[1][n];
+        diw[2][n] = W(b, n, LQ(s + 1)) - W(b, n, LQ(s));
+        fs1[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+        fs1[n] = dis2 * diw[1][n] - dis4 * fs1[n];
+      }
+      if (P->dissip == 0) {
+        for (int n = 0; n < 5; n++) fs1[n] = fs1[n] * lam;
+      } else {
+        double uc[5], ss[3], o[5];
+        get5(b, LQ(s), uc); svec(S, E, LQ(s), ss);
+        dflux_matrix(P->gamma, wsol, uc, fs1, ss, o);
+        for (int n = 0; n < 5; n++) fs1[n] = o[n];
+      }
+      for (int n = 0; n < 5; n++) {
+        FW(n, s) = FW(n, s) + (fs1[n] - fs2[n]);
+        FW(n, s + 1) = FW(n, s + 1) + (fs2[n] - DF(n, s + 2));
+      }
+    }
+#undef LQ
+#undef PP
+#undef RS
+#undef FW
+#undef DF
+  }
+}
+
+/* dflux (dflux.f90:1-281) */
+static void dflux(blk_t *b, const orc_params *P, int istage) {
+  double cd = P->cdis[istage - 1];
+  if (cd <= 0.0) return;
+  const long E = b->E;
+  int mx = b->ib;
+  if (b->jb > mx) mx = b->jb;
+  if (b->kb > mx) mx = b->kb;
+  double *fp = (double *)calloc((size_t)mx + 1, sizeof(double));
+  double *diw = (double *)calloc(((size_t)mx + 1) * 5, sizeof(double));
+  double *dl = (double *)calloc(((size_t)mx + 1) * 5, sizeof(double));
+  double *dfs[3];
+  for (int d = 0; d < 3; d++) dfs[d] = (double *)calloc((size_t)E * 5, sizeof(double));
+  double *we = (double *)malloc((size_t)E * sizeof(double));
+  /* replace energy by enthalpy (dflux.f90:58-65) */
+  for (long q = 0; q < E; q++) {
+    we[q] = W(b, 4, q);
+    W(b, 4, q) = W(b, 4, q) + b->p[q];
+  }
+  /* i-direction (dflux.f90:68-124) */
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++) {
+      dflux_line(b, P, cd, 0, j, k, fp, diw, dl);
+      for (int s = 2; s <= b->ie; s++)
+        for (int n = 0; n < 5; n++) dfs[0][(long)n * E + IX(b, s, j, k)] = dl[s * 5 + n];
+    }
+  for (int n = 0; n < 5; n++)
+    for (int k = 2; k <= b->kl; k++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int i = 2; i <= b->il; i++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->fw[q] = b->fw[q] + (dfs[0][q] - dfs[0][(long)n * E + IX(b, i + 1, j, k)]);
+        }
+  /* j-direction (dflux.f90:127-182) */
+  for (int k = 2; k <= b->kl; k++)
+    for (int i = 2; i <= b->il; i++) {
+      dflux_line(b, P, cd, 1, i, k, fp, diw, dl);
+      for (int s = 2; s <= b->je; s++)
+        for (int n = 0; n < 5; n++) dfs[1][(long)n * E + IX(b, i, s, k)] = dl[s * 5 + n];
+    }
+  for (int n = 0; n < 5; n++)
+    for (int k = 2; k <= b->kl; k++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int i = 2; i <= b->il; i++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->fw[q] = b->fw[q] + (dfs[1][q] - dfs[1][(long)n * E + IX(b, i, j + 1, k)]);
+        }
+  /* k-direction (dflux.f90:185-238) */
+  for (int i = 2; i <= b->il; i++)
+    for (int j = 2; j <= b->jl; j++) {
+      dflux_line(b, P, cd, 2, i, j, fp, diw, dl);
+      for (int s = 2; s <= b->ke; s++)
+        for (int n = 0; n < 5; n++) dfs[2][(long)n * E + IX(b, i, j, s)] = dl[s * 5 + n];
+    }
+  for (int n = 0; n < 5; n++)
+    for (int k = 2; k <= b->kl; k++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int i = 2; i <= b->il; i++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->fw[q] = b->fw[q] + (dfs[2][q] - dfs[2][(long)n * E + IX(b, i, j, k + 1)]);
+        }
+  dfluxb(b, P, cd, dfs[0], dfs[1], dfs[2]);
+  /* restore energy (dflux.f90:257-263) */
+  for (long q = 0; q < E; q++) W(b, 4, q) = we[q];
+  free(fp); free(diw); free(dl); free(we);
+  for (int d = 0; d < 3; d++) free(dfs[d]);
+}
+
+/* efluxb (efluxb.f90:186-302) */
+static int efluxb(blk_t *b, const double *efs[3]) {
+  const long E = b->E;
+  for (int m = 0; m < b->nmir; m++) {
+    orc_cut *t = &b->z[m];
+    int id = t->faceid, zz = t->zz;
+    int i = t->ccid[0], j = t->ccid[1], k = t->ccid[2];
+    int is = t->scid[0], js = t->scid[1], ks = t->scid[2];
+    double psol = t->psol, wsol[5];
+    for (int n = 0; n < 5; n++) wsol[n] = t->wsol[n];
+    const double *S = zz == 1 ? b->si : (zz == 2 ? b->sj : b->sk);
+    long qc = IX(b, i, j, k), qs = IX(b, is, js, ks);
+    double sc[3], ss[3];
+    svec(S, E, qc, sc);
+    svec(S, E, qs, ss);
+    if (fabs(W(b, 0, qc)) < 1.0e-8f) return ORC_ERR_WALL_RHO;
+    double wc[5], fs1[5], fs2[5], fs3[5], fs[5], fa[5], wo[5], so[3];
+    get5(b, qc, wc);
+    ceflux(wc, b->p[qc], sc, fs1);
+    ceflux(wsol, psol, ss, fs2);
+    for (int n = 0; n < 5; n++) fs[n] = 0.5 * (fs1[n] + fs2[n]);
+    const double *ef = efs[zz - 1];
+    long qn1 = zz == 1 ? IX(b, i + 1, j, k) : (zz == 2 ? IX(b, i, j + 1, k) : IX(b, i, j, k + 1));
+    for (int n = 0; n < 5; n++)
+      b->dw[(long)n * E + qc] = b->dw[(long)n * E + qc] - (ef[(long)n * E + qc] - ef[(long)n * E + qn1]);
+    int hi = (id == 4 && zz == 1) || (id == 5 && zz == 2) || (id == 6 && zz == 3);
+    int lo = (id == 1 && zz == 1) || (id == 2 && zz == 2) || (id == 3 && zz == 3);
+    if (hi || lo) {
+      long qo;
+      if (hi) qo = zz == 1 ? IX(b, i - 1, j, k) : (zz == 2 ? IX(b, i, j - 1, k) : IX(b, i, j, k - 1));
+      else qo = qn1;
+      get5(b, qo, wo);
+      svec(S, E, qo, so);
+      ceflux(wo, b->p[qo], so, fs3);
+      for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
+      for (int n = 0; n < 5; n++) {
+        long q = (long)n * E + qc;
+        if (hi) b->dw[q] = b->dw[q] + fa[n] - fs[n];
+        else b->dw[q] = b->dw[q] + fs[n] - fa[n];
+      }
+    }
+  }
+  return ORC_OK;
+}
+
+/* eflux (eflux.f90:1-130) */
+static int eflux(blk_t *b) {
+  const long E = b->E;
+  double *efs[3];
+  for (int d = 0; d < 3; d++) efs[d] = (double *)calloc((size_t)E * 5, sizeof(double));
+  double wa[5], wb[5], sa[3], sb[3], f1[5], f2[5];
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->ie; i++) {
+        long q = IX(b, i, j, k), qm = IX(b, i - 1, j, k);
+        get5(b, q, wa); get5(b, qm, wb); svec(b->si, E, q, sa); svec(b->si, E, qm, sb);
+        ceflux(wa, b->p[q], sa, f1);
+        ceflux(wb, b->p[qm], sb, f2);
+        for (int n = 0; n < 5; n++) efs[0][(long)n * E + q] = 0.5 * (f1[n] + f2[n]);
+      }
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->dw[q] = efs[0][q] - efs[0][(long)n * E + IX(b, i + 1, j, k)];
+        }
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->je; j++)
+      for (int i = 2; i <= b->il; i++) {
+        long q = IX(b, i, j, k), qm = IX(b, i, j - 1, k);
+        get5(b, q, wa); get5(b, qm, wb); svec(b->sj, E, q, sa); svec(b->sj, E, qm, sb);
+        ceflux(wa, b->p[q], sa, f1);
+        ceflux(wb, b->p[qm], sb, f2);
+        for (int n = 0; n < 5; n++) efs[1][(long)n * E + q] = 0.5 * (f1[n] + f2[n]);
+      }
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->dw[q] = b->dw[q] + efs[1][q] - efs[1][(long)n * E + IX(b, i, j + 1, k)];
+        }
+  for (int k = 2; k <= b->ke; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++) {
+        long q = IX(b, i, j, k), qm = IX(b, i, j, k - 1);
+        get5(b, q, wa); get5(b, qm, wb); svec(b->sk, E, q, sa); svec(b->sk, E, qm, sb);
+        ceflux(wa, b->p[q], sa, f1);
+        ceflux(wb, b->p[qm], sb, f2);
+        for (int n = 0; n < 5; n++) efs[2][(long)n * E + q] = 0.5 * (f1[n] + f2[n]);
+      }
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->dw[q] = b->dw[q] + efs[2][q] - efs[2][(long)n * E + IX(b, i, j, k + 1)];
+        }
+  const double *ce[3] = {efs[0], efs[1], efs[2]};
+  int rc = efluxb(b, ce);
+  for (int d = 0; d < 3; d++) free(efs[d]);
+  return rc;
+}
+
+/* ------------------------------------------------------------ smoothing */
+
+/* psmoo_old (psmoo.f90:97-242) */
+static int psmoo_old(blk_t *b, const orc_params *P) {
+  if (P->smoo <= 0.0) return ORC_OK;
+  const int ilm = b->ni, jlm = b->nj, klm = b->nk;
+  const int il = b->il, jl = b->jl, kl = b->kl;
+  if (ilm < 2 || jlm < 2 || klm < 2) return ORC_ERR_SMOO_DIM;
+  const long E = b->E;
+  double aa = 1.0 + 2.0 * P->smoo;
+  double bb = -P->smoo;
+  int mx = il > jl ? il : jl;
+  if (kl > mx) mx = kl;
+  double *ex1 = (double *)calloc((size_t)mx + 2, sizeof(double)), *ex2 = (double *)calloc((size_t)mx + 2, sizeof(double));
+  double *ey1 = (double *)calloc((size_t)mx + 2, sizeof(double)), *ey2 = (double *)calloc((size_t)mx + 2, sizeof(double));
+  double *ez1 = (double *)calloc((size_t)mx + 2, sizeof(double)), *ez2 = (double *)calloc((size_t)mx + 2, sizeof(double));
+  ex1[2] = aa; ex2[2] = bb / aa;
+  for (int i = 3; i <= ilm; i++) { ex1[i] = aa - bb * ex2[i - 1]; ex2[i] = bb / ex1[i]; }
+  ex1[il] = aa - bb * ex2[ilm];
+  ey1[2] = aa; ey2[2] = bb / aa;
+  for (int j = 3; j <= jlm; j++) { ey1[j] = aa - bb * ey2[j - 1]; ey2[j] = bb / ey1[j]; }
+  ey1[jl] = aa - bb * ey2[jlm];
+  ez1[2] = aa; ez2[2] = bb / aa;
+  for (int k = 3; k <= klm; k++) { ez1[k] = aa - bb * ez2[k - 1]; ez2[k] = bb / ez1[k]; }
+  ez1[kl] = aa - bb * ez2[klm];
+#define DW(i, j, k) d[IX(b, i, j, k)]
+#define BL(i, j, k) (b->blank[IX(b, i, j, k)] != 0)
+  for (int n = 0; n < 5; n++) {
+    double *d = b->dw + (long)n * E;
+    for (int k = 2; k <= kl; k++) {
+      for (int j = 2; j <= jl; j++) {
+        if (BL(2, j, k)) DW(2, j, k) = DW(2, j, k) / ex1[2];
+        for (int i = 3; i <= il; i++)
+          if (BL(i, j, k)) DW(i, j, k) = (DW(i, j, k) - bb * DW(i - 1, j, k)) / ex1[i];
+        for (int i = 2; i <= ilm; i++) {
+          int ir = il + 1 - i;
+          if (BL(ir, j, k)) DW(ir, j, k) = DW(ir, j, k) - ex2[ir] * DW(ir + 1, j, k);
+        }
+      }
+      for (int i = 2; i <= il; i++) {
+        if (BL(i, 2, k)) DW(i, 2, k) = DW(i, 2, k) / ey1[2];
+        for (int j = 3; j <= jl; j++)
+          if (BL(i, j, k)) DW(i, j, k) = (DW(i, j, k) - bb * DW(i, j - 1, k)) / ey1[j];
+        for (int j = 2; j <= jlm; j++) {
+          int jr = jl + 1 - j;
+          if (BL(i, jr, k)) DW(i, jr, k) = DW(i, jr, k) - ey2[jr] * DW(i, jr + 1, k);
+        }
+      }
+    }
+    for (int j = 2; j <= jl; j++) {
+      for (int i = 2; i <= il; i++)
+        if (BL(i, j, 2)) DW(i, j, 2) = DW(i, j, 2) / ez1[2];
+      for (int k = 3; k <= kl; k++)
+        for (int i = 2; i <= il; i++)
+          if (BL(i, j, k)) DW(i, j, k) = (DW(i, j, k) - bb * DW(i, j, k - 1)) / ez1[k];
+      for (int k = 2; k <= klm; k++) {
+        int kr = kl + 1 - k;
+        for (int i = 2; i <= il; i++)
+          if (BL(i, j, kr)) DW(i, j, kr) = DW(i, j, kr) - ez2[kr] * DW(i, j, kr + 1);
+      }
+    }
+  }
+#undef DW
+#undef BL
+  free(ex1); free(ex2); free(ey1); free(ey2); free(ez1); free(ez2);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ boundaries */
+
+/* bc_far (bc_far.f90:46-331) for one face; dir = face axis */
+static void bc_far(blk_t *b, const orc_params *P, int face) {
+  const double gamma = P->gamma;
+  const double gm = 1.0 / (gamma - 1.0);
+  int dir = (face - 1) % 3;
+  int phy1, dum1, dum2;
+  int nl = dir == 0 ? b->il : (dir == 1 ? b->jl : b->kl);
+  int ne = nl + 1, nb = nl + 2;
+  if (face <= 3) { phy1 = 2; dum1 = 1; dum2 = 0; }
+  else { phy1 = nl; dum1 = ne; dum2 = nb; }
+  const double *fn = b->facenvec[face - 1];
+  double vinf[3] = {P->u0, P->v0, P->w0};
+  int a_max = dir == 0 ? b->jb : b->ib;
+  int c_max = dir == 2 ? b->jb : b->kb;
+  for (int cc2 = 0; cc2 <= c_max; cc2++)
+    for (int a = 0; a <= a_max; a++) {
+      long q1, qd1, qd2;
+      if (dir == 0) { q1 = IX(b, phy1, a, cc2); qd1 = IX(b, dum1, a, cc2); qd2 = IX(b, dum2, a, cc2); }
+      else if (dir == 1) { q1 = IX(b, a, phy1, cc2); qd1 = IX(b, a, dum1, cc2); qd2 = IX(b, a, dum2, cc2); }
+      else { q1 = IX(b, a, cc2, phy1); qd1 = IX(b, a, cc2, dum1); qd2 = IX(b, a, cc2, dum2); }
+      double s = 0.0;
+      s = s + W(b, 1, q1) * fn[0];
+      s = s + W(b, 2, q1) * fn[1];
+      s = s + W(b, 3, q1) * fn[2];
+      double vel_norm = s / W(b, 0, q1);
+      double r, vel[3], pr;
+      if (vel_norm <= 0.0) { /* inlet */
+        double rhoe = W(b, 0, q1);
+        double ue = W(b, 1 + dir, q1) / rhoe;
+        double pe = b->p[q1];
+        double ce = sqrt(gamma * pe / rhoe);
+        double rpo = ue * fn[dir] + 2.0 * ce / (gamma - 1.0);
+        double rmi = vinf[dir] * fn[dir] - 2.0 * P->c0 / (gamma - 1.0);
+        double un = 0.5 * (rpo + rmi) * fn[dir];
+        double ci = 0.25 * (gamma - 1.0) * (rpo - rmi);
+        vel[0] = P->u0; vel[1] = P->v0; vel[2] = P->w0;
+        vel[dir] = un;
+        double si = pow(P->rho0, gamma) / P->p0;
+        double cc = ci * ci / gamma;
+        r = pow(si * cc, gm);
+        pr = r * cc;
+      } else { /* outlet */
+        double rhoi = W(b, 0, q1);
+        double ui = W(b, 1, q1) / rhoi, vi = W(b, 2, q1) / rhoi, wi = W(b, 3, q1) / rhoi;
+        double prei = b->p[q1];
+        double si = pow(rhoi, gamma) / prei;
+        double ci = sqrt(gamma * prei / rhoi);
+        double vv[3] = {ui, vi, wi};
+        double rpo = vv[dir] * fn[dir] + 2.0 * ci / (gamma - 1.0);
+        double rmi = vinf[dir] * fn[dir] - 2.0 * P->c0 / (gamma - 1.0);
+        double un = 0.5 * (rpo + rmi) * fn[dir];
+        double ce = 0.25 * (gamma - 1.0) * (rpo - rmi);
+        vel[0] = ui; vel[1] = vi; vel[2] = wi;
+        vel[dir] = un;
+        double cc = ce * ce / gamma;
+        r = pow(si * cc, gm);
+        pr = r * cc;
+      }
+      W(b, 0, qd1) = r;
+      W(b, 1, qd1) = r * vel[0];
+      W(b, 2, qd1) = r * vel[1];
+      W(b, 3, qd1) = r * vel[2];
+      W(b, 4, qd1) = pr * gm + 0.5 * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+      b->p[qd1] = pr;
+      for (int n = 0; n < 5; n++) W(b, n, qd2) = W(b, n, qd1);
+      b->p[qd2] = b->p[qd1];
+    }
+}
+
+/* bc_symm (bc_symm.f90:1-85), bc_periodic (bc_periodic.f90:1-94), bc_wall (bc_wall.f90:1-92) */
+static void bc_copy(blk_t *b, int face, int kind) {
+  int dir = (face - 1) % 3;
+  int nl = dir == 0 ? b->il : (dir == 1 ? b->jl : b->kl);
+  int ne = nl + 1, nb = nl + 2;
+  int phy1, phy2, dum1, dum2;
+  if (kind == 1) { /* periodic */
+    if (face <= 3) { dum1 = 1; dum2 = 0; phy1 = nl; phy2 = nl - 1; }
+    else { phy1 = 2; phy2 = 3; dum1 = ne; dum2 = nb; }
+  } else {
+    if (face <= 3) { phy1 = 2; phy2 = 3; dum1 = 1; dum2 = 0; }
+    else { phy1 = nl; phy2 = nl - 1; dum1 = ne; dum2 = nb; }
+  }
+  int a_max = dir == 0 ? b->jb : b->ib;
+  int c_max = dir == 2 ? b->jb : b->kb;
+  for (int cc2 = 0; cc2 <= c_max; cc2++)
+    for (int a = 0; a <= a_max; a++) {
+      long q1, q2, qd1, qd2;
+#define L(s) (dir == 0 ? IX(b, (s), a, cc2) : (dir == 1 ? IX(b, a, (s), cc2) : IX(b, a, cc2, (s))))
+      q1 = L(phy1); q2 = L(phy2); qd1 = L(dum1); qd2 = L(dum2);
+#undef L
+      if (kind == 1) {
+        for (int n = 0; n < 5; n++) W(b, n, qd2) = W(b, n, q2);
+        for (int n = 0; n < 5; n++) W(b, n, qd1) = W(b, n, q1);
+        b->p[qd2] = b->p[q2];
+        b->p[qd1] = b->p[q1];
+      } else if (kind == 0) { /* symmetry */
+        for (int n = 0; n < 5; n++) W(b, n, qd1) = W(b, n, q1);
+        W(b, 1 + dir, qd1) = -W(b, 1 + dir, q1);
+        b->p[qd1] = b->p[q1];
+        for (int n = 0; n < 5; n++) W(b, n, qd2) = W(b, n, q2);
+        W(b, 1 + dir, qd2) = -W(b, 1 + dir, q2);
+        b->p[qd2] = b->p[q2];
+      } else { /* no-slip wall */
+        W(b, 0, qd1) = W(b, 0, q1);
+        W(b, 4, qd1) = W(b, 4, q1);
+        for (int n = 1; n < 4; n++) W(b, n, qd1) = -W(b, n, q1);
+        W(b, 0, qd2) = W(b, 0, q2);
+        W(b, 4, qd2) = W(b, 4, q2);
+        for (int n = 1; n < 4; n++) W(b, n, qd2) = -W(b, n, q2);
+        b->p[qd1] = b->p[q1];
+        b->p[qd2] = b->p[q1];
+      }
+    }
+}
+
+/* bc_corner (bc_corner.f90:1-160) */
+static void bc_corner(blk_t *b, double gamma) {
+  const int il = b->il, jl = b->jl, kl = b->kl, ie = b->ie, je = b->je, ke = b->ke, ib = b->ib;
+#define WQ(n, i, j, k) W(b, n, IX(b, i, j, k))
+  for (int k = 2; k <= kl; k++) {
+    for (int n = 0; n < 5; n++) WQ(n, 1, 1, k) = WQ(n, 2, 1, k) + WQ(n, 1, 2, k) - WQ(n, 2, 2, k);
+    eos_cell(b, gamma, IX(b, 1, 1, k));
+    for (int n = 0; n < 5; n++) WQ(n, ie, 1, k) = WQ(n, il, 1, k) + WQ(n, ie, 2, k) - WQ(n, il, 2, k);
+    eos_cell(b, gamma, IX(b, ie, 1, k));
+    for (int n = 0; n < 5; n++) WQ(n, 1, je, k) = WQ(n, 2, je, k) + WQ(n, 1, jl, k) - WQ(n, 2, jl, k);
+    eos_cell(b, gamma, IX(b, 1, je, k));
+    for (int n = 0; n < 5; n++) WQ(n, ie, je, k) = WQ(n, il, je, k) + WQ(n, ie, jl, k) - WQ(n, il, jl, k);
+    eos_cell(b, gamma, IX(b, ie, je, k));
+  }
+  for (int j = 2; j <= jl; j++) {
+    for (int n = 0; n < 5; n++) WQ(n, 1, j, 1) = WQ(n, 2, j, 1) + WQ(n, 1, j, 2) - WQ(n, 2, j, 2);
+    eos_cell(b, gamma, IX(b, 1, j, 1));
+    for (int n = 0; n < 5; n++) WQ(n, ie, j, 1) = WQ(n, il, j, 1) + WQ(n, ie, j, 2) - WQ(n, il, j, 2);
+    eos_cell(b, gamma, IX(b, ie, j, 1));
+    for (int n = 0; n < 5; n++) WQ(n, 1, j, ke) = WQ(n, 2, j, ke) + WQ(n, 1, j, kl) - WQ(n, 2, j, kl);
+    eos_cell(b, gamma, IX(b, 1, j, ke));
+    for (int n = 0; n < 5; n++) WQ(n, ie, j, ke) = WQ(n, il, j, ke) + WQ(n, ie, j, kl) - WQ(n, il, j, kl);
+    eos_cell(b, gamma, IX(b, ie, j, ke));
+  }
+  for (int i = 0; i <= ib; i++) {
+    for (int n = 0; n < 5; n++) WQ(n, i, 1, 1) = WQ(n, i, 1, 2) + WQ(n, i, 2, 1) - WQ(n, i, 2, 2);
+    eos_cell(b, gamma, IX(b, i, 1, 1));
+    for (int n = 0; n < 5; n++) WQ(n, i, 1, ke) = WQ(n, i, 2, ke) + WQ(n, i, 1, kl) - WQ(n, i, 2, kl);
+    eos_cell(b, gamma, IX(b, i, 1, ke));
+    /* bc_corner.f90:120 adds and subtracts the same term */
+    for (int n = 0; n < 5; n++) WQ(n, i, je, 1) = WQ(n, i, je, 1) + WQ(n, i, jl, 2) - WQ(n, i, jl, 2);
+    eos_cell(b, gamma, IX(b, i, je, 1));
+    for (int n = 0; n < 5; n++) WQ(n, i, je, ke) = WQ(n, i, jl, ke) + WQ(n, i, je, kl) - WQ(n, i, jl, kl);
+    eos_cell(b, gamma, IX(b, i, je, ke));
+  }
+  for (int n = 0; n < 5; n++) {
+    WQ(n, 1, 1, 1) = (WQ(n, 2, 1, 1) + WQ(n, 1, 2, 1) + WQ(n, 1, 1, 2)) * 2.0 / 3.0 - WQ(n, 2, 2, 2);
+    WQ(n, 1, 1, ke) = (WQ(n, 2, 1, ke) + WQ(n, 1, 2, ke) + WQ(n, 1, 1, kl)) * 2.0 / 3.0 - WQ(n, 2, 2, kl);
+    WQ(n, 1, je, 1) = (WQ(n, 2, je, 1) + WQ(n, 1, jl, 1) + WQ(n, 1, je, 2)) * 2.0 / 3.0 - WQ(n, 2, jl, 2);
+    WQ(n, ie, 1, 1) = (WQ(n, il, 1, 1) + WQ(n, ie, 2, 1) + WQ(n, ie, 1, 2)) * 2.0 / 3.0 - WQ(n, il, 2, 2);
+    WQ(n, ie, 1, ke) = (WQ(n, il, 1, ke) + WQ(n, ie, 2, ke) + WQ(n, ie, 1, kl)) * 2.0 / 3.0 - WQ(n, il, 2, kl);
+    WQ(n, ie, je, 1) = (WQ(n, il, je, 1) + WQ(n, ie, jl, 1) + WQ(n, ie, je, 2)) * 2.0 / 3.0 - WQ(n, il, jl, 2);
+    WQ(n, 1, je, ke) = (WQ(n, 2, je, ke) + WQ(n, 1, jl, ke) + WQ(n, 1, je, kl)) * 2.0 / 3.0 - WQ(n, 2, jl, kl);
+    WQ(n, ie, je, ke) = (WQ(n, il, je, ke) + WQ(n, ie, jl, ke) + WQ(n, ie, je, kl)) * 2.0 / 3.0 - WQ(n, il, jl, kl);
+  }
+  eos_cell(b, gamma, IX(b, 1, 1, 1));
+  eos_cell(b, gamma, IX(b, 1, 1, ke));
+  eos_cell(b, gamma, IX(b, 1, je, 1));
+  eos_cell(b, gamma, IX(b, ie, 1, 1));
+  eos_cell(b, gamma, IX(b, ie, 1, ke));
+  eos_cell(b, gamma, IX(b, ie, je, 1));
+  eos_cell(b, gamma, IX(b, 1, je, ke));
+  eos_cell(b, gamma, IX(b, ie, je, ke));
+#undef WQ
+}
+
+static int bcond_blk(blk_t *b, const orc_params *P) {
+  int rc = ORC_OK;
+  for (int f = 1; f <= 6; f++) {
+    int t = b->bctype[f - 1];
+    if (t == 0) continue;
+    if (t == -20) bc_far(b, P, f);
+    else if (t == -50) bc_copy(b, f, 0);
+    else if (t == -60) {
+      bc_copy(b, f, 1);
+      if (f <= 3 && b->bctype[f + 2] != -60) return ORC_ERR_PERIODIC;
+    } else if (t == -80) bc_copy(b, f, 2);
+    else rc = ORC_WARN_BCTYPE;
+  }
+  bc_corner(b, P->gamma);
+  return rc;
+}
+
+int orc_bcond(orc_case *c, int lvl) { return bcond_blk(&c->lv[lvl], &c->prm); }
+
+/* bc_cutcell (bc_cutcell.f90:161-432), steady bodies (approxbc off) */
+static int bc_cutcell_blk(blk_t *b, const orc_params *P) {
+  const double gamma = P->gamma;
+  for (int m = 0; m < b->nmir; m++) {
+    orc_cut *t = &b->z[m];
+    int ncl = t->npoint;
+    double nvec[3] = {t->nvec[0], t->nvec[1], t->nvec[2]};
+    double qb[3] = {0.0, 0.0, 0.0};
+    double s = 0.0;
+    s = s + nvec[0] * nvec[0]; s = s + nvec[1] * nvec[1]; s = s + nvec[2] * nvec[2];
+    double nn = sqrt(s);
+    for (int d = 0; d < 3; d++) nvec[d] = nvec[d] / nn;
+    double psol = 0.0, wsol[5] = {0, 0, 0, 0, 0};
+    if (ncl == 0) {
+      t->psol = psol;
+      for (int n = 0; n < 5; n++) t->wsol[n] = wsol[n];
+      return ORC_WARN_ZERO_CLOUD;
+    }
+    double var_refl[5] = {0, 0, 0, 0, 0}, p_refl = 0.0;
+    for (int n = 0; n < 5; n++)
+      for (int i = 0; i < ncl; i++)
+        var_refl[n] = var_refl[n] + t->coef[i] * W(b, n, IX(b, t->ijk[i][0], t->ijk[i][1], t->ijk[i][2]));
+    for (int i = 0; i < ncl; i++)
+      p_refl = p_refl + t->coef[i] * b->p[IX(b, t->ijk[i][0], t->ijk[i][1], t->ijk[i][2])];
+    psol = p_refl;
+    for (int n = 0; n < 5; n++) wsol[n] = var_refl[n];
+    if (P->eqntype == 0) {
+      double qn = 2.0 * (nvec[0] * (wsol[1] - wsol[0] * qb[0]) + nvec[1] * (wsol[2] - wsol[0] * qb[1]) +
+                         nvec[2] * (wsol[3] - wsol[0] * qb[2]));
+      for (int d = 0; d < 3; d++) wsol[1 + d] = wsol[1 + d] - qn * nvec[d];
+    } else {
+      for (int d = 0; d < 3; d++) wsol[1 + d] = 2.0 * qb[d] * var_refl[0] - var_refl[1 + d];
+    }
+    double s2 = 0.0;
+    s2 = s2 + wsol[1] * wsol[1]; s2 = s2 + wsol[2] * wsol[2]; s2 = s2 + wsol[3] * wsol[3];
+    wsol[4] = psol / (gamma - 1.0) + 0.5 * s2 / wsol[0];
+    if (P->eqntype == 0)
+      for (int n = 0; n < 5; n++)
+        if (wsol[n] != wsol[n]) return ORC_ERR_WSOL_NAN;
+    t->psol = psol;
+    for (int n = 0; n < 5; n++) t->wsol[n] = wsol[n];
+  }
+  return ORC_OK;
+}
+
+int orc_bc_cutcell(orc_case *c, int lvl) { return bc_cutcell_blk(&c->lv[lvl], &c->prm); }
+
+/* ------------------------------------------------------------ euler */
+
+/* euler.f90:1-462 (central scheme, steady, Euler equations) */
+int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int lfir) {
+  blk_t *b = &c->lv[lvl];
+  const orc_params *P = &c->prm;
+  const long E = b->E;
+  const int il = b->il, jl = b->jl, kl = b->kl;
+  const double fcoll = 1.0;
+  int rc = ORC_OK, warn = ORC_OK;
+  if (lfir) memcpy(b->ws, b->w, sizeof(double) * (size_t)E * 5);
+  for (int k = 2; k <= kl; k++)
+    for (int j = 2; j <= jl; j++)
+      for (int i = 2; i <= il; i++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          b->wn[q] = b->w[q];
+        }
+  memset(b->dw, 0, sizeof(double) * (size_t)E * 5);
+  memset(b->fw, 0, sizeof(double) * (size_t)E * 5);
+  for (int m = 1; m <= P->mstage; m++) {
+    double fn = P->cstp[m - 1] * P->cfl;
+    if (m > 1) {
+      for (int k = 2; k <= kl; k++)
+        for (int j = 2; j <= jl; j++)
+          for (int i = 2; i <= il; i++)
+            for (int n = 0; n < 5; n++) {
+              long q = (long)n * E + IX(b, i, j, k);
+              b->fw[q] = b->fw[q] * (1.0 - P->cdis[m - 1]);
+            }
+    }
+    if (P->cdis[m - 1] > 0.0) dflux(b, P, m);
+    rc = eflux(b);
+    if (rc) return rc;
+    for (int k = 2; k <= kl; k++)
+      for (int j = 2; j <= jl; j++)
+        for (int i = 2; i <= il; i++)
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * E + IX(b, i, j, k);
+            b->dw[q] = b->dw[q] + b->fw[q];
+          }
+    if (mgl < mlvl) { /* euler.f90:294-318 */
+      for (int k = 2; k <= kl; k++)
+        for (int j = 2; j <= jl; j++)
+          for (int i = 2; i <= il; i++) {
+            long q0 = IX(b, i, j, k);
+            for (int n = 0; n < 5; n++) {
+              long q = (long)n * E + q0;
+              if (lfir && m == 1) {
+                b->qf[q] = 0.0;
+                if (b->marker[q0] == 1) b->qf[q] = fcoll * b->wr[q] - b->dw[q];
+              }
+              b->dw[q] = b->dw[q] + b->qf[q];
+            }
+          }
+    }
+    for (int k = 2; k <= kl; k++)
+      for (int j = 2; j <= jl; j++)
+        for (int i = 2; i <= il; i++) {
+          long q0 = IX(b, i, j, k);
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * E + q0;
+            b->dw[q] = b->dw[q] * (double)b->blank[q0];
+          }
+        }
+    if (lres) return ORC_OK;
+    for (int k = 2; k <= kl; k++)
+      for (int j = 2; j <= jl; j++)
+        for (int i = 2; i <= il; i++) {
+          long q0 = IX(b, i, j, k);
+          double de = fn * b->vdtim[q0] / b->cellvolume;
+          if (P->vt == 1) de = fn * b->dtim / b->cellvolume;
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * E + q0;
+            b->dw[q] = de * b->dw[q];
+          }
+        }
+    if (m % 2 != 0) {
+      rc = psmoo_old(b, P);
+      if (rc) return rc;
+    }
+    for (int k = 2; k <= kl; k++)
+      for (int j = 2; j <= jl; j++)
+        for (int i = 2; i <= il; i++) {
+          long q0 = IX(b, i, j, k);
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * E + q0;
+            b->w[q] = b->wn[q] - b->dw[q] / 1.0;
+          }
+          eos_cell(b, P->gamma, q0);
+        }
+    rc = bc_cutcell_blk(b, P);
+    if (rc < 0) return rc;
+    if (rc > 0) warn = rc;
+    rc = bcond_blk(b, P);
+    if (rc < 0) return rc;
+    if (rc > 0) warn = rc;
+  }
+  double acc = 0.0;
+  for (int k = 2; k <= kl; k++)
+    for (int j = 2; j <= jl; j++)
+      for (int i = 2; i <= il; i++) {
+        long q0 = IX(b, i, j, k);
+        double de = P->cfl * b->vdtim[q0];
+        if (P->vt == 1) de = P->cfl * b->dtim;
+        double rt = 0.0;
+        if (b->blank[q0] != 0) rt = b->dw[q0] / de;
+        acc = acc + rt * rt;
+      }
+  if (ncyc >= 1 && ncyc <= b->mxcyc)
+    b->rtrms[ncyc] = sqrt(acc / (double)((long)b->ni * b->nj * b->nk));
+  return warn;
+}
+
+/* ------------------------------------------------------------ transfers */
+
+/* 1-D stencil of the cell-centre trilinear interpolation between aligned
+ * levels (the role of eval_x2cell/shft_cell/trilinear_factor in
+ * collc.f90:141-166 and addw.f90:386-411), written from integer indices. */
+static void restrict_stencil(int I, int ratio, int *s, double *f0, double *f1) {
+  if (ratio == 2) { *s = 2 * I - 2; *f0 = 0.5; *f1 = 0.5; }
+  else { *s = I; *f0 = 1.0; *f1 = 0.0; }
+}
+static void prolong_stencil(int i, int ratio, int *s, double *f0, double *f1) {
+  if (ratio == 2) {
+    if (i % 2 == 0) { int m = i / 2; *s = m; *f0 = 0.25; *f1 = 0.75; }
+    else { int m = (i - 1) / 2; *s = m + 1; *f0 = 0.75; *f1 = 0.25; }
+  } else { *s = i; *f0 = 1.0; *f1 = 0.0; }
+}
+
+/* trilinear_factor corner order: column-major over (s:s+1)^3 */
+static void tri_fac(const double fa[2], const double fb[2], const double fc[2], double fac[8]) {
+  for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
+}
+
+/* collc (collc.f90:53-220): restrict w and the residual from level mgl+1 */
+static void collc(orc_case *c, int mgl) {
+  blk_t *b = &c->lv[mgl], *f = &c->lv[mgl + 1];
+  const long Eb = b->E, Ef = f->E;
+  for (long q = 0; q < Eb; q++) b->marker[q] = 0;
+  for (int i = 2; i <= b->il; i++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int k = 2; k <= b->kl; k++) {
+        int s[3];
+        double fa[2], fb[2], fc[2], fac[8];
+        restrict_stencil(i, f->ratio[0], &s[0], &fa[0], &fa[1]);
+        restrict_stencil(j, f->ratio[1], &s[1], &fb[0], &fb[1]);
+        restrict_stencil(k, f->ratio[2], &s[2], &fc[0], &fc[1]);
+        tri_fac(fa, fb, fc, fac);
+        long qb = IX(b, i, j, k);
+        for (int n = 0; n < 5; n++) {
+          double acc = 0.0;
+          for (int c8 = 0; c8 < 8; c8++) {
+            long qf = IX(f, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+            double val = f->dw[(long)n * Ef + qf] / f->cellvolume;
+            acc = acc + val * fac[c8];
+          }
+          b->wr[(long)n * Eb + qb] = acc * b->cellvolume;
+        }
+        for (int n = 0; n < 5; n++) {
+          double acc = 0.0;
+          for (int c8 = 0; c8 < 8; c8++) {
+            long qf = IX(f, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+            acc = acc + f->w[(long)n * Ef + qf] * fac[c8];
+          }
+          b->w[(long)n * Eb + qb] = acc;
+        }
+        b->marker[qb] = 1;
+      }
+}
+
+/* movco (movco.f90:1-50) */
+int orc_movco(orc_case *c, int mgl) {
+  blk_t *b = &c->lv[mgl];
+  collc(c, mgl);
+  for (long q = 0; q < b->E; q++) eos_cell(b, c->prm.gamma, q);
+  for (int i = 2; i <= b->il; i++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int k = 2; k <= b->kl; k++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * b->E + IX(b, i, j, k);
+          b->wn[q] = b->w[q];
+        }
+  return bcond_blk(b, &c->prm);
+}
+
+/* addw (addw.f90:264-464): prolong to level mgl from mgl-1 */
+static void addw(orc_case *c, int mgl, int mlvl) {
+  blk_t *b = &c->lv[mgl], *pc = &c->lv[mgl - 1];
+  const long Eb = b->E, Ec = pc->E;
+  double *store = (double *)malloc(sizeof(double) * (size_t)Eb * 5);
+  memcpy(store, b->w, sizeof(double) * (size_t)Eb * 5);
+  for (int i = 2; i <= b->il; i++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int k = 2; k <= b->kl; k++) {
+        int s[3];
+        double fa[2], fb[2], fc[2], fac[8];
+        prolong_stencil(i, b->ratio[0], &s[0], &fa[0], &fa[1]);
+        prolong_stencil(j, b->ratio[1], &s[1], &fb[0], &fb[1]);
+        prolong_stencil(k, b->ratio[2], &s[2], &fc[0], &fc[1]);
+        tri_fac(fa, fb, fc, fac);
+        long qb = IX(b, i, j, k);
+        double dw[5];
+        for (int n = 0; n < 5; n++) {
+          double acc = 0.0;
+          for (int c8 = 0; c8 < 8; c8++) {
+            long qc = IX(pc, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+            double val = pc->w[(long)n * Ec + qc];
+            if (mgl - 1 != mlvl) val = val - pc->ws[(long)n * Ec + qc];
+            acc = acc + val * fac[c8];
+          }
+          dw[n] = acc;
+        }
+        if (b->blank[qb] != 0) {
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * Eb + qb;
+            if (mgl - 1 == mlvl) b->w[q] = dw[n];
+            else b->w[q] = store[q] + dw[n];
+          }
+        }
+      }
+  free(store);
+}
+
+/* movfin (movfin.f90:1-42); bc_embed/bc_overlap are no-ops when every
+ * level is one block covering the whole domain */
+int orc_movfin(orc_case *c, int mgl, int mlvl) {
+  blk_t *b = &c->lv[mgl];
+  addw(c, mgl, mlvl);
+  for (long q = 0; q < b->E; q++) eos_cell(b, c->prm.gamma, q);
+  return bcond_blk(b, &c->prm);
+}
+
+/* ------------------------------------------------------------ init, solver */
+
+/* init_flow.f90:324-424 (uniform freestream) */
+int orc_init_flow(orc_case *c) {
+  const orc_params *P = &c->prm;
+  int warn = ORC_OK;
+  for (int l = 1; l <= c->nlevel; l++) {
+    blk_t *b = &c->lv[l];
+    for (long q = 0; q < b->E; q++) {
+      W(b, 0, q) = P->rho0;
+      W(b, 1, q) = P->rho0 * P->u0;
+      W(b, 2, q) = P->rho0 * P->v0;
+      W(b, 3, q) = P->rho0 * P->w0;
+      W(b, 4, q) = P->rho0 * P->h0 - P->p0;
+    }
+  }
+  for (int l = 1; l <= c->nlevel; l++) {
+    blk_t *b = &c->lv[l];
+    for (long q = 0; q < b->E; q++) eos_cell(b, P->gamma, q);
+    int rc = bc_cutcell_blk(b, P);
+    if (rc < 0) return rc;
+    if (rc > 0) warn = rc;
+    rc = bcond_blk(b, P);
+    if (rc < 0) return rc;
+    if (rc > 0) warn = rc;
+  }
+  return warn;
+}
+
+#define CHK(x)                \
+  do {                        \
+    int _r = (x);             \
+    if (_r < 0) return _r;    \
+  } while (0)
+
+/* solver.f90:1-303 (steady: nperiod = nstep = 1) */
+int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
+              long long *cell_stages) {
+  const int nlevel = c->nlevel;
+  const orc_params *P = &c->prm;
+  int icyc[64];
+  int mgl = 1;
+  double ini_res = 0.0, max_res = 0.0;
+  int nl = 0;
+  long long cs = 0;
+  for (int mlvl = 1; mlvl <= nlevel; mlvl++) {
+    mgl = mlvl;
+    for (int ncyc = 1; ncyc <= mcyc[mlvl - 1]; ncyc++) {
+      for (int l = 0; l <= nlevel; l++) icyc[l] = 0;
+      int next_cycle = 0;
+      for (;;) { /* lopmg */
+        for (;;) { /* lopr */
+          icyc[mgl] = icyc[mgl] + 1;
+          int lfir = (mgl == mlvl) ? 0 : 1;
+          blk_t *b = &c->lv[mgl];
+          CHK(orc_timestep(c, mgl));
+          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 0, lfir));
+          cs += (long long)P->mstage * b->ni * b->nj * b->nk;
+          if (mgl == mlvl) {
+            max_res = 1000.0;
+            double r = b->rtrms[ncyc];
+            if (ncyc == 1) ini_res = r;
+            if (r < max_res) max_res = r;
+            if (r != r) return ORC_ERR_RES_NAN;
+            if (reslog && nl < cap) {
+              reslog[3 * nl] = mgl;
+              reslog[3 * nl + 1] = ncyc;
+              reslog[3 * nl + 2] = max_res;
+            }
+            nl++;
+          }
+          if (mlvl == 1) { next_cycle = 1; break; }
+          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 1, 0));
+          cs += (long long)b->ni * b->nj * b->nk;
+          mgl = mgl - 1;
+          CHK(orc_movco(c, mgl));
+          if (mgl == 1) break;
+        }
+        if (next_cycle) break;
+        int lfir = 1;
+        int exit_mg = 0;
+        for (;;) { /* lopp */
+          blk_t *b = &c->lv[mgl];
+          CHK(orc_timestep(c, mgl));
+          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 0, lfir));
+          cs += (long long)P->mstage * b->ni * b->nj * b->nk;
+          lfir = 0;
+          mgl = mgl + 1;
+          CHK(orc_movfin(c, mgl, mlvl));
+          if (mgl == mlvl) { exit_mg = 1; break; }
+          if (icyc[mgl] != P->mgtype) break; /* cycle lopmg */
+          icyc[mgl] = 0;
+        }
+        if (exit_mg) break;
+      }
+      if (next_cycle) continue;
+      if (mgl == mlvl) {
+        if (ini_res / max_res > P->drop) break;
+      }
+    }
+    if (mlvl < nlevel) {
+      mgl = mlvl + 1;
+      CHK(orc_movfin(c, mgl, mlvl));
+    }
+  }
+  if (nlog) *nlog = nl;
+  if (cell_stages) *cell_stages = cs;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ access */
+
+static double *field_ptr(blk_t *b, const char *name, long *n) {
+  long E = b->E;
+  *n = 5 * E;
+  if (!strcmp(name, "w")) return b->w;
+  if (!strcmp(name, "dw")) return b->dw;
+  if (!strcmp(name, "fw")) return b->fw;
+  if (!strcmp(name, "ws")) return b->ws;
+  if (!strcmp(name, "wn")) return b->wn;
+  if (!strcmp(name, "wr")) return b->wr;
+  if (!strcmp(name, "qf")) return b->qf;
+  *n = 3 * E;
+  if (!strcmp(name, "si")) return b->si;
+  if (!strcmp(name, "sj")) return b->sj;
+  if (!strcmp(name, "sk")) return b->sk;
+  *n = E;
+  if (!strcmp(name, "p")) return b->p;
+  if (!strcmp(name, "radi")) return b->radi;
+  if (!strcmp(name, "radj")) return b->radj;
+  if (!strcmp(name, "radk")) return b->radk;
+  if (!strcmp(name, "vdtim")) return b->vdtim;
+  return NULL;
+}
+
+int orc_get(orc_case *c, int lvl, const char *name, double *out) {
+  long n;
+  double *p = field_ptr(&c->lv[lvl], name, &n);
+  if (!p) return ORC_ERR_ARG;
+  memcpy(out, p, sizeof(double) * (size_t)n);
+  return ORC_OK;
+}
+
+int orc_set(orc_case *c, int lvl, const char *name, const double *in) {
+  long n;
+  double *p = field_ptr(&c->lv[lvl], name, &n);
+  if (!p) return ORC_ERR_ARG;
+  memcpy(p, in, sizeof(double) * (size_t)n);
+  return ORC_OK;
+}
+
+int orc_get_cuts(orc_case *c, int lvl, orc_cut *out) {
+  blk_t *b = &c->lv[lvl];
+  if (b->nmir > 0) memcpy(out, b->z, sizeof(orc_cut) * (size_t)b->nmir);
+  return b->nmir;
+}
+
+double orc_get_rtrms(orc_case *c, int lvl, int ncyc) {
+  blk_t *b = &c->lv[lvl];
+  if (ncyc < 1 || ncyc > b->mxcyc) return 0.0;
+  return b->rtrms[ncyc];
+}
+
+double orc_get_dtim(orc_case *c, int lvl) { return c->lv[lvl].dtim; }
+
+/* wall force: momentum flux through each wall face as the scheme applies it
+ * (efluxb.f90:251-253), outward from the fluid cell */
+int orc_wall_force(orc_case *c, int lvl, double out[3]) {
+  blk_t *b = &c->lv[lvl];
+  const long E = b->E;
+  double F[3] = {0, 0, 0};
+  for (int m = 0; m < b->nmir; m++) {
+    orc_cut *t = &b->z[m];
+    const double *S = t->zz == 1 ? b->si : (t->zz == 2 ? b->sj : b->sk);
+    long qc = IX(b, t->ccid[0], t->ccid[1], t->ccid[2]);
+    long qs = IX(b, t->scid[0], t->scid[1], t->scid[2]);
+    double sc[3], ss[3], wc[5], f1[5], f2[5];
+    svec(S, E, qc, sc);
+    svec(S, E, qs, ss);
+    get5(b, qc, wc);
+    ceflux(wc, b->p[qc], sc, f1);
+    ceflux(t->wsol, t->psol, ss, f2);
+    double sgn = (t->faceid >= 4) ? -1.0 : 1.0;
+    for (int d = 0; d < 3; d++) F[d] = F[d] + sgn * (0.5 * (f1[1 + d] + f2[1 + d]));
+  }
+  for (int d = 0; d < 3; d++) out[d] = F[d];
+  return ORC_OK;
+}

@@ -1,0 +1,107 @@
+/*
+ * oracle.h -- CPU restatement of code_mge's multigrid Jameson-RK solve step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle: tests/, the smoke
+ * check and the cpu_baseline leg of bench.py may load it, and only as the
+ * checker.  The product path (code_mge_amd/) never links, loads or calls it.
+ *
+ * Every routine follows the reference Fortran line by line, including the
+ * evaluation order of each expression (left to right, as gfortran evaluates
+ * it without -ffast-math), so that the HIP path can be compared at the
+ * last-bit level on single stages and at 1e-10 after many cycles.
+ * Build with -O2 -ffp-contract=off (no fused multiply-add).
+ *
+ * Layout: every field is a Fortran array (0:ib,0:jb,0:kb[,5]) stored with
+ * i fastest; component n of a 5-vector field lives at offset n*E.
+ * Level 1 is the coarsest level (the reference's mgl = 1).
+ */
+#ifndef MGE_ORACLE_H
+#define MGE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error / warning codes shared with the product's C ABI (include/mge.h) */
+#define ORC_OK               0
+#define ORC_ERR_NEG_C2     -10  /* timestep: c^2 < 0 or NaN   (srcf90/timestep.f90:45-49)  */
+#define ORC_ERR_WALL_RHO   -11  /* efluxb: zero wall density   (srcf90/efluxb.f90:246-249)  */
+#define ORC_ERR_WSOL_NAN   -12  /* bc_cutcell: NaN mirror state (srcf90/bc_cutcell.f90:307-317) */
+#define ORC_ERR_RES_NAN    -13  /* solver: NaN residual        (srcf90/solver.f90:180)      */
+#define ORC_ERR_PERIODIC   -14  /* bcond: unpaired periodic    (srcf90/bcond.f90:26-31)     */
+#define ORC_ERR_SMOO_DIM   -16  /* psmoo_old with < 2 cells    (srcf90/psmoo.f90:132,166)   */
+#define ORC_ERR_ARG        -20
+#define ORC_WARN_ZERO_CLOUD  1  /* bc_cutcell: empty cloud, early return (bc_cutcell.f90:225-230) */
+#define ORC_WARN_BCTYPE      2  /* bcond: unknown bctype (bcond.f90:35) */
+
+typedef struct {
+  int    mstage;
+  double cstp[8], cdis[8];
+  double cfl;          /* |cfl| (input_param.f90:461-463) */
+  int    vt;           /* 0: local time step, 1: global */
+  double vis2, vis4;   /* vis4 already divided by 32 (input_param.f90:464) */
+  double smoo;
+  int    mgtype;       /* 1 V-cycle, 2 W-cycle */
+  double gamma;
+  double rho0, p0, c0, ei0, u0, v0, w0, h0;
+  int    eqntype;      /* 0 Euler, 1 laminar Navier-Stokes */
+  int    dissip;       /* 0 scalar, 1 matrix */
+  double viscl, prlmean, gas_const;
+  double drop;         /* cycle exit when ini_res/res > drop (1e6 in solver.f90:238) */
+} orc_params;
+
+typedef struct {
+  int    faceid, zz;
+  int    ccid[3], scid[3];
+  double psurf[3], nvec[3], radcur[2], rr;
+  int    npoint;
+  int    ijk[8][3];
+  double coef[8];
+  double psol, wsol[5];
+} orc_cut;
+
+typedef struct orc_case orc_case;
+
+orc_case *orc_case_new(const orc_params *prm, int nlevel);
+void      orc_case_free(orc_case *c);
+
+/* Level lvl (1..nlevel, 1 = coarsest).  xv/yv/zv: vertex coordinates
+ * x(1..ib) (index 0 unused, kept 0 like init_metric.f90:156).
+ * blank_int: ni*nj*nk bytes, i fastest, 0 = solid.  ratio: coarsening
+ * ratio (1 or 2 per direction) from this level to lvl-1. */
+int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
+                  const double *xv, const double *yv, const double *zv,
+                  double cellvolume, const int bctype[6],
+                  const uint8_t *blank_int, int nmir, const orc_cut *cuts,
+                  const int ratio[3], int mxcyc);
+
+int  orc_init_flow(orc_case *c);                               /* init_flow.f90 */
+int  orc_timestep(orc_case *c, int lvl);                       /* timestep.f90  */
+int  orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl,
+               int lres, int lfir);                            /* euler.f90     */
+int  orc_bcond(orc_case *c, int lvl);                          /* bcond.f90     */
+int  orc_bc_cutcell(orc_case *c, int lvl);                     /* bc_cutcell.f90*/
+int  orc_movco(orc_case *c, int mgl);                          /* movco.f90     */
+int  orc_movfin(orc_case *c, int mgl, int mlvl);               /* movfin.f90    */
+/* solver.f90: runs the FMG ramp; writes (level, ncyc, res) triples into
+ * reslog (capacity cap entries) and returns the count in *nlog. */
+int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
+               long long *cell_stages);
+
+/* field access.  name: "w","p","dw","fw","ws","wn","wr","qf","radi","radj",
+ * "radk","vdtim","si","sj","sk".  Copies the full (0:ib,...) array. */
+int    orc_get(orc_case *c, int lvl, const char *name, double *out);
+int    orc_set(orc_case *c, int lvl, const char *name, const double *in);
+int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);
+double orc_get_rtrms(orc_case *c, int lvl, int ncyc);
+double orc_get_dtim(orc_case *c, int lvl);
+/* build-defined FP64 wall force (Fx,Fy,Fz) from the wall-face flux that the
+ * scheme applies (efluxb.f90:251-253), summed over records in order. */
+int    orc_wall_force(orc_case *c, int lvl, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

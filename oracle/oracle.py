@@ -1,0 +1,204 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+cpu_baseline leg of bench.py, always as the checker.  It mirrors
+code_mge_amd.solver.Case so parity tests read call for call.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("mstage", C.c_int), ("cstp", C.c_double * 8), ("cdis", C.c_double * 8),
+        ("cfl", C.c_double), ("vt", C.c_int), ("vis2", C.c_double), ("vis4", C.c_double),
+        ("smoo", C.c_double), ("mgtype", C.c_int), ("gamma", C.c_double),
+        ("rho0", C.c_double), ("p0", C.c_double), ("c0", C.c_double), ("ei0", C.c_double),
+        ("u0", C.c_double), ("v0", C.c_double), ("w0", C.c_double), ("h0", C.c_double),
+        ("eqntype", C.c_int), ("dissip", C.c_int),
+        ("viscl", C.c_double), ("prlmean", C.c_double), ("gas_const", C.c_double),
+        ("drop", C.c_double),
+    ]
+
+
+class OrcCut(C.Structure):
+    _fields_ = [
+        ("faceid", C.c_int), ("zz", C.c_int),
+        ("ccid", C.c_int * 3), ("scid", C.c_int * 3),
+        ("psurf", C.c_double * 3), ("nvec", C.c_double * 3),
+        ("radcur", C.c_double * 2), ("rr", C.c_double),
+        ("npoint", C.c_int), ("ijk", (C.c_int * 3) * 8), ("coef", C.c_double * 8),
+        ("psol", C.c_double), ("wsol", C.c_double * 5),
+    ]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "liboracle.so"], cwd=HERE, check=True)
+    L = C.CDLL(LIB_PATH)
+    P, I = C.c_void_p, C.c_int
+    L.orc_case_new.restype = P
+    L.orc_case_new.argtypes = [C.POINTER(OrcParams), I]
+    L.orc_case_free.argtypes = [P]
+    L.orc_set_level.argtypes = [P, I, I, I, I, _dp, _dp, _dp, C.c_double, C.POINTER(I), C.c_void_p, I,
+                                C.c_void_p, C.POINTER(I), I]
+    for n in ("orc_init_flow",):
+        getattr(L, n).argtypes = [P]
+    L.orc_timestep.argtypes = [P, I]
+    L.orc_euler.argtypes = [P, I, I, I, I, I, I]
+    L.orc_bcond.argtypes = [P, I]
+    L.orc_bc_cutcell.argtypes = [P, I]
+    L.orc_movco.argtypes = [P, I]
+    L.orc_movfin.argtypes = [P, I, I]
+    L.orc_solve.argtypes = [P, C.POINTER(I), _dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.orc_get.argtypes = [P, I, C.c_char_p, _dp]
+    L.orc_set.argtypes = [P, I, C.c_char_p, _dp]
+    L.orc_get_cuts.argtypes = [P, I, C.c_void_p]
+    L.orc_get_rtrms.argtypes = [P, I, I]
+    L.orc_get_rtrms.restype = C.c_double
+    L.orc_get_dtim.argtypes = [P, I]
+    L.orc_get_dtim.restype = C.c_double
+    L.orc_wall_force.argtypes = [P, I, _dp]
+    _lib = L
+    return L
+
+
+def params_from(prm):
+    """OrcParams from a code_mge_amd.params.Params (same derivations)."""
+    d = prm.derived()
+    P = OrcParams()
+    P.mstage = len(prm.cstp)
+    for i, v in enumerate(prm.cstp):
+        P.cstp[i] = v
+    for i, v in enumerate(prm.cdis):
+        P.cdis[i] = v
+    P.cfl = d["cfl"]
+    P.vt = d["vt"]
+    P.vis2 = prm.vis2
+    P.vis4 = d["vis4"]
+    P.smoo = prm.smoo
+    P.mgtype = prm.mgtype
+    P.gamma = prm.gamma
+    for k in ("rho0", "p0", "c0", "ei0", "u0", "v0", "w0", "h0"):
+        setattr(P, k, d[k])
+    P.eqntype = 1 if prm.eqntype.lower() == "n" else 0
+    P.dissip = 1 if prm.dissip.lower() == "matrix" else 0
+    P.viscl = d["viscl"]
+    P.prlmean = prm.prlmean
+    P.gas_const = 1.0 / prm.gamma
+    P.drop = prm.drop
+    return P
+
+
+class OracleCase:
+    """Same calls as code_mge_amd.solver.Case, on the CPU restatement."""
+
+    def __init__(self, prm, geoms, mxcyc=1000):
+        self.L = lib()
+        self.prm = prm
+        self.geoms = geoms
+        self.nlevel = len(geoms)
+        self._p = params_from(prm)
+        self.h = self.L.orc_case_new(C.byref(self._p), self.nlevel)
+        for li, g in enumerate(geoms):
+            s = g.spec
+            cuts = (OrcCut * max(g.nmir, 1))()
+            C.memmove(cuts, g.recs, C.sizeof(OrcCut) * g.nmir)
+            bct = (C.c_int * 6)(*s.bctype)
+            rat = (C.c_int * 3)(*s.ratio)
+            rc = self.L.orc_set_level(self.h, li + 1, s.n[0], s.n[1], s.n[2],
+                                      g.xv.ctypes.data_as(_dp), g.yv.ctypes.data_as(_dp),
+                                      g.zv.ctypes.data_as(_dp), s.cellvolume(), bct,
+                                      g.blank.ctypes.data_as(C.c_void_p), g.nmir, cuts, rat, mxcyc)
+            assert rc == 0, rc
+
+    def close(self):
+        if self.h:
+            self.L.orc_case_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise RuntimeError(f"oracle {what} failed: {rc}")
+        return rc
+
+    def init_flow(self):
+        return self._chk(self.L.orc_init_flow(self.h), "init_flow")
+
+    def timestep(self, lvl):
+        return self._chk(self.L.orc_timestep(self.h, lvl), "timestep")
+
+    def euler(self, lvl, ncyc, mgl, mlvl, lres, lfir):
+        self._chk(self.L.orc_euler(self.h, lvl, ncyc, mgl, mlvl, int(lres), int(lfir)), "euler")
+        return self.L.orc_get_rtrms(self.h, lvl, ncyc)
+
+    def bcond(self, lvl):
+        return self._chk(self.L.orc_bcond(self.h, lvl), "bcond")
+
+    def bc_cutcell(self, lvl):
+        return self._chk(self.L.orc_bc_cutcell(self.h, lvl), "bc_cutcell")
+
+    def movco(self, mgl):
+        return self._chk(self.L.orc_movco(self.h, mgl), "movco")
+
+    def movfin(self, mgl, mlvl):
+        return self._chk(self.L.orc_movfin(self.h, mgl, mlvl), "movfin")
+
+    def solve(self, mcyc, cap=100000):
+        mc = (C.c_int * self.nlevel)(*mcyc)
+        log = np.zeros(3 * cap)
+        nlog = C.c_int(0)
+        cs = C.c_longlong(0)
+        self._chk(self.L.orc_solve(self.h, mc, log.ctypes.data_as(_dp), cap, C.byref(nlog), C.byref(cs)),
+                  "solve")
+        n = min(nlog.value, cap)
+        return log[: 3 * n].reshape(n, 3), cs.value
+
+    def dims(self, lvl):
+        ni, nj, nk = self.geoms[lvl - 1].spec.n
+        return ni + 4, nj + 4, nk + 4
+
+    def download(self, lvl, field):
+        ib1, jb1, kb1 = self.dims(lvl)
+        nc = {"w": 5, "dw": 5, "fw": 5, "ws": 5, "wn": 5, "wr": 5, "qf": 5, "si": 3, "sj": 3, "sk": 3}.get(field, 1)
+        out = np.empty((nc, kb1, jb1, ib1))
+        self._chk(self.L.orc_get(self.h, lvl, field.encode(), out.ctypes.data_as(_dp)), "get")
+        return out if nc > 1 else out[0]
+
+    def upload(self, lvl, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.L.orc_set(self.h, lvl, field.encode(), a.ctypes.data_as(_dp)), "set")
+
+    def cuts(self, lvl):
+        g = self.geoms[lvl - 1]
+        out = (OrcCut * max(g.nmir, 1))()
+        n = self.L.orc_get_cuts(self.h, lvl, out)
+        return out, n
+
+    def wall_force(self, lvl=None):
+        lvl = self.nlevel if lvl is None else lvl
+        out = (C.c_double * 3)()
+        self.L.orc_wall_force(self.h, lvl, out)
+        return np.array(out[:])
+
+    def dtim(self, lvl):
+        return self.L.orc_get_dtim(self.h, lvl)
