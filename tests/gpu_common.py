@@ -55,3 +55,18 @@ def max_rel_to_field(a, b, comp_axis=0):
         return out
     s = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / s)
+
+
+def cell_rel_err(a, b, prm):
+    """per-cell relative error of a 5-component state: |a-b| / (|b| + s_n)
+    with s_n the freestream magnitude of component n (rho, rho|U| for the
+    three momenta, rho*E), so components that cross zero are measured
+    against their physical scale."""
+    d = prm.derived()
+    U = (d["u0"] ** 2 + d["v0"] ** 2 + d["w0"] ** 2) ** 0.5
+    scale = [d["rho0"], d["rho0"] * U, d["rho0"] * U, d["rho0"] * U, d["rho0"] * d["h0"] - d["p0"]]
+    out = 0.0
+    for n in range(5):
+        e = np.abs(a[n] - b[n]) / (np.abs(b[n]) + scale[n])
+        out = max(out, float(e.max()))
+    return out
