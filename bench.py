@@ -1,0 +1,163 @@
+#!/usr/bin/env python
+"""Benchmark of the multigrid Jameson-RK solve step on one MI355X.
+
+Workload: steady inviscid flow past a sphere (radius 0.5) in a 256^3
+Cartesian box [-8,8]^3, M = 0.5, alpha = 2 deg, cut-cell wall, 4-level
+W-cycle, FP64, JST central scheme (the Roe/MUSCL flux of that case is not
+built yet; the scheme is named in the output).  A step is one multigrid
+cycle on the finest level; the unit of work is a cell-stage (one RK-stage
+residual + update of one cell, a residual-only pass counting 1).
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (k_stage; HIP events on the library's stream over the timed region)
+and the CPU baseline (the FP64 CPU restatement, one core, bounded sample).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(n, prm_kw, nz_slab=32):
+    """FP64 CPU restatement on one core: timestep + one 5-stage relax on a
+    n x n x nz_slab slab (z in [-nz_slab*h/2, nz_slab*h/2]) of the same grid
+    and body; returns Mcell-stage/s."""
+    from code_mge_amd import configs
+    from code_mge_amd.grid import LevelSpec, level_geometry
+    from oracle.oracle import OracleCase
+    prm, levels, bdy, _ = configs.sphere3d(n=n, nlevel=1, **prm_kw)
+    h = 16.0 / n
+    spec = LevelSpec(n=(n, n, nz_slab), lo=(-8.0, -8.0, -nz_slab * h / 2), hi=(8.0, 8.0, nz_slab * h / 2),
+                     ratio=(2, 2, 2), bctype=(-20,) * 6)
+    geoms = [level_geometry(spec, bdy)]
+    oc = OracleCase(prm, geoms, mxcyc=4)
+    oc.init_flow()
+    t0 = time.perf_counter()
+    oc.timestep(1)
+    oc.euler(1, 1, 1, 1, 0, 0)
+    dt = time.perf_counter() - t0
+    cs = prm_stages(prm) * n * n * nz_slab
+    oc.close()
+    return cs / dt / 1e6, dt, f"timestep + one {prm_stages(prm)}-stage relax on a {n}x{n}x{nz_slab} slab " \
+                             f"through the sphere of the {n}^3 grid ({cs/1e6:.1f} M cell-stages, {dt:.1f} s)"
+
+
+def prm_stages(prm):
+    return len(prm.cstp)
+
+
+def load_traffic():
+    """HBM bytes per k_stage launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "k_stage_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solve step runs on one MI355X; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    from code_mge_amd.solver import Case, force_coefficients
+
+    prm_kw = dict(mach=0.5, alpha=2.0)
+    prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, **prm_kw)
+    print(f"[bench] building {args.n}^3 sphere case, {args.levels} levels", file=sys.stderr, flush=True)
+    geoms = [level_geometry(s, bdy) for s in levels]
+    case = Case(prm, levels, geoms=geoms, mxcyc=args.warmup + args.steps + 2)
+    case.init_flow()
+    nl = args.levels
+    print(f"[bench] warmup {args.warmup} cycles", file=sys.stderr, flush=True)
+    r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
+
+    timing = not args.no_timing
+    case.timing(enable=timing, name=None)
+    case.sync()
+    t0 = time.perf_counter()
+    r1, cs = case.cycles(nl, args.steps, ncyc0=args.warmup + 1)   # ends with a stream sync
+    t1 = time.perf_counter()
+    ms, cnt, by = case.timing(enable=False, name="k_stage")
+    dt = t1 - t0
+    value = cs / dt / 1e6
+    cd, cl = force_coefficients(case, aref)
+    print(f"[bench] {args.steps} cycles in {dt:.3f} s, {cs/1e6:.1f} M cell-stages, res {r0:.3e} -> {r1:.3e}",
+          file=sys.stderr, flush=True)
+
+    roofline = None
+    if timing and cnt > 0 and ms > 0:
+        achieved = by / (ms * 1e-3) / 1e9
+        roofline = {"kernel": "k_stage", "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                    "traffic": load_traffic(), "avg_launch_ms": round(ms, 5), "launches": cnt,
+                    "alg_bytes_per_launch": round(by)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle, 1 core)", file=sys.stderr, flush=True)
+        v, t, sample = cpu_baseline(args.n, prm_kw)
+        cpu = {"value": round(v, 3), "unit": "Mcell-stage/s", "cores": 1, "kind": "port",
+               "sample": sample, "cpu": _cpu_model()}
+
+    out = {
+        "metric": "Mcell-stage updates/s",
+        "value": round(value, 2),
+        "unit": "Mcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (freestream start, analytic sphere geometry)",
+        "config": {"workload": f"sphere {args.n}^3 in [-8,8]^3, M=0.5, alpha=2, {args.levels}-level W-cycle, "
+                               "5-stage RK, JST central scheme, FP64",
+                   "n": args.n, "levels": args.levels, "cell_stages_per_step": cs // args.steps,
+                   "cd": round(cd, 6), "cl": round(cl, 6)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    case.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -106,7 +106,8 @@ def lib():
     L.mge_get_dtim.restype = D
     L.mge_wall_force.argtypes = [P, I, dp]
     L.mge_synchronize.argtypes = [P]
-    L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong)]
+    L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong), dp]
+    L.mge_cycles.argtypes = [P, I, I, I, dp, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -124,5 +125,5 @@ EXPORTED = [
     "mge_init_flow", "mge_timestep", "mge_euler", "mge_bcond", "mge_bc_cutcell",
     "mge_bc_overlap", "mge_movco", "mge_movfin", "mge_solve", "mge_download",
     "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
-    "mge_kernel_timing", "mge_last_error",
+    "mge_kernel_timing", "mge_last_error", "mge_cycles",
 ]

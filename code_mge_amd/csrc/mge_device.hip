@@ -1198,6 +1198,13 @@ struct Level {
 struct Timer {
   std::string name;
   hipEvent_t a, b;
+  double bytes;
+};
+
+struct TStat {
+  double ms = 0.0;
+  long long count = 0;
+  double bytes = 0.0;
 };
 
 }  // namespace
@@ -1211,7 +1218,8 @@ struct mge_case {
   bool timing = false;
   std::vector<Timer> pending;
   std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, long long>> tstat;
+  std::map<std::string, TStat> tstat;
+  double next_bytes = 0.0;  // algorithmic bytes of the next timed launch
   double smoo_bb = 0.0;
 };
 
@@ -1245,6 +1253,8 @@ hipEvent_t ev_get(mge_case *h) {
 }
 
 void tic(mge_case *h, const char *name, Timer *t) {
+  t->bytes = h->next_bytes;
+  h->next_bytes = 0.0;
   if (!h->timing) return;
   t->name = name;
   t->a = ev_get(h);
@@ -1262,8 +1272,9 @@ void resolve_timers(mge_case *h) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, t.a, t.b);
     auto &s = h->tstat[t.name];
-    s.first += ms;
-    s.second += 1;
+    s.ms += ms;
+    s.count += 1;
+    s.bytes += t.bytes;
     h->pool.push_back(t.a);
     h->pool.push_back(t.b);
   }
@@ -1432,6 +1443,16 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     A.smoothed = (m % 2 != 0) && P.smoo > 0.0;
     A.store_fw = (m < P.mstage) && (A.has_diss || m == 1 || A.blend != 1.0);
     A.vt = P.vt;
+    {
+      // algorithmic bytes per cell: read W^(m-1); write W^(m) and read W^(0)
+      // when this kernel updates, else write dw; forcing on coarse levels;
+      // one flag byte
+      double s8 = 8.0, per = 5 * s8 + 1;
+      bool upd = !lres && !A.smoothed;
+      per += upd ? 10 * s8 : 5 * s8;
+      if (A.coarse) per += A.lfir1 ? 10 * s8 : 5 * s8;
+      h->next_bytes = per * (double)d.ni * d.nj * d.nk;
+    }
     LAUNCH(h, "k_stage", k_stage, cell_grid(d), cell_block(), A);
     if (L.ntouch > 0) {
       RecArgs R;
@@ -1586,80 +1607,79 @@ int do_init_flow(mge_case *h) {
     if (r_ < 0) return r_;  \
   } while (0)
 
-// solver.f90:119-261 for steady flow (nperiod = nstep = 1)
-int do_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, long long *cs_out) {
+// one multigrid cycle with finest active level mlvl: the body of lopmg in
+// solver.f90:127-234.  The residual of the mlvl relax is left in
+// L.rtrms[ncyc] (read by the caller after the cycle: the finest level is
+// relaxed once per cycle, so the value is final when the cycle ends).
+int do_cycle(mge_case *h, int mlvl, int ncyc, long long *cs) {
   const int nlevel = h->nlevel;
   const mge_params &P = h->prm;
   std::vector<int> icyc(nlevel + 1, 0);
-  int mgl = 1, nl = 0;
+  int mgl = mlvl;
+  for (;;) {
+    for (;;) {
+      icyc[mgl] += 1;
+      int lfir = (mgl == mlvl) ? 0 : 1;
+      Level &B = h->lv[mgl];
+      CHKRC(do_timestep(h, B));
+      CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 0, lfir));
+      *cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
+      if (mlvl == 1) return MGE_OK;
+      CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 1, 0));
+      *cs += (long long)B.d.ni * B.d.nj * B.d.nk;
+      mgl -= 1;
+      CHKRC(do_movco(h, mgl));
+      if (mgl == 1) break;
+    }
+    int lfir = 1;
+    for (;;) {
+      Level &B = h->lv[mgl];
+      CHKRC(do_timestep(h, B));
+      CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 0, lfir));
+      *cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
+      lfir = 0;
+      mgl += 1;
+      CHKRC(do_movfin(h, mgl, mlvl));
+      if (mgl == mlvl) return MGE_OK;
+      if (icyc[mgl] != P.mgtype) break;
+      icyc[mgl] = 0;
+    }
+  }
+}
+
+int read_rtrms(mge_case *h, int mlvl, int ncyc, double *r) {
+  Level &B = h->lv[mlvl];
+  int c = ncyc >= 1 && ncyc <= B.mxcyc ? ncyc : 0;
+  HIPCHK(hipMemcpyAsync(r, B.rtrms + c, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return check_err(h);
+}
+
+// solver.f90:119-261 for steady flow (nperiod = nstep = 1): FMG ramp over
+// the levels, cycles with the residual log and the drop test
+int do_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, long long *cs_out) {
+  const int nlevel = h->nlevel;
+  const mge_params &P = h->prm;
+  int nl = 0;
   double ini_res = 0.0, max_res = 0.0;
   long long cs = 0;
   for (int mlvl = 1; mlvl <= nlevel; mlvl++) {
-    mgl = mlvl;
     for (int ncyc = 1; ncyc <= mcyc[mlvl - 1]; ncyc++) {
-      std::fill(icyc.begin(), icyc.end(), 0);
-      bool next_cycle = false;
-      for (;;) {
-        for (;;) {
-          icyc[mgl] += 1;
-          int lfir = (mgl == mlvl) ? 0 : 1;
-          Level &B = h->lv[mgl];
-          CHKRC(do_timestep(h, B));
-          CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 0, lfir));
-          cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
-          if (mgl == mlvl) {
-            double r = 0.0;
-            int c = ncyc <= B.mxcyc ? ncyc : 0;
-            HIPCHK(hipMemcpyAsync(&r, B.rtrms + c, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-            CHKRC(check_err(h));
-            max_res = 1000.0;
-            if (ncyc == 1) ini_res = r;
-            if (r < max_res) max_res = r;
-            if (r != r) return set_err(MGE_ERR_RES_NAN, "solver: residual is NaN");
-            if (reslog && nl < cap) {
-              reslog[3 * nl] = mgl;
-              reslog[3 * nl + 1] = ncyc;
-              reslog[3 * nl + 2] = max_res;
-            }
-            nl++;
-          }
-          if (mlvl == 1) {
-            next_cycle = true;
-            break;
-          }
-          CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 1, 0));
-          cs += (long long)B.d.ni * B.d.nj * B.d.nk;
-          mgl -= 1;
-          CHKRC(do_movco(h, mgl));
-          if (mgl == 1) break;
-        }
-        if (next_cycle) break;
-        int lfir = 1;
-        bool exit_mg = false;
-        for (;;) {
-          Level &B = h->lv[mgl];
-          CHKRC(do_timestep(h, B));
-          CHKRC(do_euler(h, B, ncyc, mgl, mlvl, 0, lfir));
-          cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
-          lfir = 0;
-          mgl += 1;
-          CHKRC(do_movfin(h, mgl, mlvl));
-          if (mgl == mlvl) {
-            exit_mg = true;
-            break;
-          }
-          if (icyc[mgl] != P.mgtype) break;
-          icyc[mgl] = 0;
-        }
-        if (exit_mg) break;
+      CHKRC(do_cycle(h, mlvl, ncyc, &cs));
+      double r = 0.0;
+      CHKRC(read_rtrms(h, mlvl, ncyc, &r));
+      max_res = 1000.0;
+      if (ncyc == 1) ini_res = r;
+      if (r < max_res) max_res = r;
+      if (r != r) return set_err(MGE_ERR_RES_NAN, "solver: residual is NaN");
+      if (reslog && nl < cap) {
+        reslog[3 * nl] = mlvl;
+        reslog[3 * nl + 1] = ncyc;
+        reslog[3 * nl + 2] = max_res;
       }
-      if (next_cycle) continue;
-      if (mgl == mlvl && ini_res / max_res > P.drop) break;
+      nl++;
+      if (mlvl > 1 && ini_res / max_res > P.drop) break;
     }
-    if (mlvl < nlevel) {
-      mgl = mlvl + 1;
-      CHKRC(do_movfin(h, mgl, mlvl));
-    }
+    if (mlvl < nlevel) CHKRC(do_movfin(h, mlvl + 1, mlvl));
   }
   CHKRC(check_err(h));
   if (nlog) *nlog = nl;
@@ -2003,6 +2023,18 @@ int mge_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, 
   return do_solve(h, mcyc, reslog, cap, nlog, cell_stages);
 }
 
+int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long long *cell_stages) {
+  if (!h || mlvl < 1 || mlvl > h->nlevel || count < 0) return set_err(MGE_ERR_ARG, "bad cycle arguments");
+  for (int l = 1; l <= mlvl; l++)
+    if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
+  long long cs = 0;
+  for (int c = 0; c < count; c++) CHKRC(do_cycle(h, mlvl, ncyc0 + c, &cs));
+  if (res && count > 0) CHKRC(read_rtrms(h, mlvl, ncyc0 + count - 1, res));
+  else CHKRC(check_err(h));
+  if (cell_stages) *cell_stages = cs;
+  return MGE_OK;
+}
+
 static double *field_ptr(Level &L, const char *name, long *n) {
   const long E = L.d.E;
   *n = 5 * E;
@@ -2085,7 +2117,8 @@ int mge_synchronize(mge_case *h) {
   return check_err(h);
 }
 
-int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count) {
+int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count,
+                      double *bytes) {
   if (!h) return set_err(MGE_ERR_ARG, "null handle");
   if (h->timing) {
     hipStreamSynchronize(h->stream);
@@ -2100,10 +2133,13 @@ int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, lon
   if (it == h->tstat.end()) {
     if (ms) *ms = 0.0;
     if (count) *count = 0;
+    if (bytes) *bytes = 0.0;
     return MGE_OK;
   }
-  if (ms) *ms = it->second.second ? it->second.first / it->second.second : 0.0;
-  if (count) *count = it->second.second;
+  const TStat &t = it->second;
+  if (ms) *ms = t.count ? t.ms / t.count : 0.0;
+  if (count) *count = t.count;
+  if (bytes) *bytes = t.count ? t.bytes / t.count : 0.0;
   return MGE_OK;
 }
 

@@ -97,6 +97,13 @@ class Case:
         n = min(nlog.value, cap)
         return log[: 3 * n].reshape(n, 3), cs.value
 
+    def cycles(self, mlvl, count, ncyc0=1):
+        """`count` cycles at finest active level mlvl without the drop test"""
+        r = C.c_double(0.0)
+        cs = C.c_longlong(0)
+        _ffi.check(self.lib.mge_cycles(self.h, mlvl, ncyc0, count, C.byref(r), C.byref(cs)), "mge_cycles")
+        return r.value, cs.value
+
     # ---- data
     def dims(self, lvl):
         ni, nj, nk = self.levels[lvl - 1].n
@@ -133,11 +140,14 @@ class Case:
         return _ffi.check(self.lib.mge_synchronize(self.h), "mge_synchronize")
 
     def timing(self, enable=True, name=None):
+        """(mean ms, launches, mean algorithmic bytes) of kernel `name`;
+        name None resets the statistics."""
         ms = C.c_double(0.0)
         cnt = C.c_longlong(0)
+        by = C.c_double(0.0)
         _ffi.check(self.lib.mge_kernel_timing(self.h, int(enable), name.encode() if name else None,
-                                              C.byref(ms), C.byref(cnt)), "mge_kernel_timing")
-        return ms.value, cnt.value
+                                              C.byref(ms), C.byref(cnt), C.byref(by)), "mge_kernel_timing")
+        return ms.value, cnt.value, by.value
 
 
 def force_coefficients(case: Case, ref_area: float):

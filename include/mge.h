@@ -142,6 +142,11 @@ int mge_movfin(mge_case *h, int mgl, int mlvl);
 int mge_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog,
               long long *cell_stages);
 
+/* `count` multigrid cycles with finest active level mlvl and no drop test
+ * (the lop0 body of solver.f90:124-254 for ncyc = ncyc0..ncyc0+count-1);
+ * res receives the last cycle's residual (may be NULL). */
+int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long long *cell_stages);
+
 /* ---- data movement and results ------------------------------------------ */
 /* field: "w","p","dw","fw","ws","wr","qf","vdtim","radsum" (radi+radj+radk) */
 int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out);
@@ -152,9 +157,13 @@ double mge_get_dtim(mge_case *h, int lvl, int blk);
  * the scheme applies at each wall face (efluxb.f90:251-253), in record order */
 int mge_wall_force(mge_case *h, int lvl, double out[3]);
 int mge_synchronize(mge_case *h);
-/* mean device duration (ms) and launch count of the named kernel since the
- * last reset, when timing is enabled; name NULL resets */
-int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count);
+/* HIP-event timing of the library's launches on the case stream: mean
+ * duration (ms), launch count and mean algorithmic bytes per launch of the
+ * named kernel since the last reset; enable turns recording on or off;
+ * name NULL resets the statistics.  (No counterpart in the reference: it is
+ * the measurement hook of the benchmark.) */
+int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count,
+                      double *bytes);
 const char *mge_last_error(void);
 
 #ifdef __cplusplus
