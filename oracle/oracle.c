@@ -1514,3 +1514,10 @@ int orc_wall_force(orc_case *c, int lvl, double out[3]) {
   for (int d = 0; d < 3; d++) out[d] = F[d];
   return ORC_OK;
 }
+
+int orc_psmoo(orc_case *c, int lvl) { return psmoo_old(&c->lv[lvl], &c->prm); }
+
+void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
+                      const double *ss, double *ff) {
+  dflux_matrix(gamma, u1, u2, dw, ss, ff);
+}

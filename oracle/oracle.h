@@ -90,6 +90,12 @@ int  orc_movfin(orc_case *c, int mgl, int mlvl);               /* movfin.f90    
 int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
                long long *cell_stages);
 
+/* unit-level entry points for the tests: psmoo_old on the level's dw
+ * (psmoo.f90:97-242) and dflux_matrix (dflux.f90:283-339) */
+int  orc_psmoo(orc_case *c, int lvl);
+void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
+                      const double *ss, double *ff);
+
 /* field access.  name: "w","p","dw","fw","ws","wn","wr","qf","radi","radj",
  * "radk","vdtim","si","sj","sk".  Copies the full (0:ib,...) array. */
 int    orc_get(orc_case *c, int lvl, const char *name, double *out);

@@ -72,6 +72,8 @@ def lib():
     L.orc_get_dtim.argtypes = [P, I]
     L.orc_get_dtim.restype = C.c_double
     L.orc_wall_force.argtypes = [P, I, _dp]
+    L.orc_psmoo.argtypes = [P, I]
+    L.orc_dflux_matrix.argtypes = [C.c_double, _dp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -202,3 +204,14 @@ class OracleCase:
 
     def dtim(self, lvl):
         return self.L.orc_get_dtim(self.h, lvl)
+
+    def psmoo(self, lvl):
+        return self._chk(self.L.orc_psmoo(self.h, lvl), "psmoo")
+
+
+def dflux_matrix(gamma, u1, u2, dw, ss):
+    L = lib()
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u1, u2, dw, ss)]
+    ff = np.zeros(5)
+    L.orc_dflux_matrix(gamma, *[x.ctypes.data_as(_dp) for x in a], ff.ctypes.data_as(_dp))
+    return ff

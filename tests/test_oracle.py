@@ -1,0 +1,341 @@
+"""Pins of the CPU oracle (no GPU).  The reference ships no golden vectors
+or known-answer tests for this path (it has no test harness at all), so
+each oracle function is pinned by what the mathematics fixes: closed
+forms, invariants, brute force on tiny inputs.  Each check is chosen so a
+plausible slip (a dropped term, a wrong sign or index, a transposed
+operand) fails it."""
+import numpy as np
+import pytest
+
+from code_mge_amd import configs
+from code_mge_amd.grid import LevelSpec, hierarchy, level_geometry
+from code_mge_amd.params import Params
+from oracle.oracle import OracleCase, dflux_matrix
+
+
+def _oracle(cfg="cylinder2d", body=True, mxcyc=50, **kw):
+    prm, levels, bdy, aref = getattr(configs, cfg)(**kw)
+    geoms = [level_geometry(s, bdy if body else None) for s in levels]
+    return OracleCase(prm, geoms, mxcyc=mxcyc), prm, geoms, aref
+
+
+def _interior(a, n):
+    ni, nj, nk = n
+    return a[..., 2:nk + 2, 2:nj + 2, 2:ni + 2]
+
+
+def test_face_vectors_closed_form():
+    """init_metric's cross products (init_metric.f90:216-274) on an
+    axis-aligned grid are si=(-dy dz,0,0), sj=(0,-dx dz,0), sk=(0,0,-dx dy)
+    with the rounded vertex differences; the HIP path uses this form."""
+    spec = LevelSpec(n=(7, 5, 6), lo=(-1.3, 0.2, -2.0), hi=(2.1, 1.1, 0.7), ratio=(2, 2, 2),
+                     bctype=(-20,) * 6)
+    oc, prm, _, _ = _oracle(nlevel=1, n=8, nz=4, body=False)
+    g = level_geometry(spec, None)
+    oc = OracleCase(prm, [g])
+    si, sj, sk = (oc.download(1, s) for s in ("si", "sj", "sk"))
+    dx, dy, dz = (np.diff(v) for v in (g.xv, g.yv, g.zv))
+    ni, nj, nk = spec.n
+    for k in range(1, nk + 3):
+        for j in range(1, nj + 3):
+            for i in range(1, ni + 3):
+                assert si[0, k, j, i] == -(dy[j] * dz[k]) and si[1, k, j, i] == 0 and si[2, k, j, i] == 0
+                assert sj[1, k, j, i] == -(dx[i] * dz[k]) and sj[0, k, j, i] == 0 and sj[2, k, j, i] == 0
+                assert sk[2, k, j, i] == -(dx[i] * dy[j]) and sk[0, k, j, i] == 0 and sk[1, k, j, i] == 0
+    # face area = |S| and points toward -axis
+    assert np.allclose(-si[0, 2:nk + 2, 2:nj + 2, 2:ni + 2], (1.1 - 0.2) / nj * 2.7 / nk)
+
+
+def test_eos_and_freestream_init():
+    """depvars (depvars.f90:37-52) and init_flow: p = 1/(gamma M^2) and
+    |U| = 1 in the freestream; p = (gamma-1)(E - |m|^2/(2 rho)) with dim()
+    clamping a negative internal energy to 0 (checked on the coarse cells
+    movco recomputes)."""
+    oc, prm, geoms, _ = _oracle(cfg="sphere3d", n=8, nlevel=2, body=False, mach=0.3, alpha=7.0)
+    oc.init_flow()
+    d = prm.derived()
+    assert np.allclose(oc.download(2, "p"), 1.0 / (1.4 * 0.09), rtol=1e-14, atol=0)
+    assert abs(d["u0"] ** 2 + d["v0"] ** 2 + d["w0"] ** 2 - 1.0) < 1e-14
+    assert abs(prm.dynamic_pressure() - 0.5) < 1e-14
+    rng = np.random.default_rng(2)
+    w = oc.download(2, "w")
+    w *= 1 + 0.2 * rng.uniform(-1, 1, size=w.shape)
+    w[4, 2:4, 2:4, 2:4] = 1e-3          # one coarse cell with negative internal energy
+    oc.upload(2, "w", w)
+    oc.upload(2, "dw", np.zeros_like(w))
+    oc.movco(1)
+    wc = oc.download(1, "w")[:, 2:6, 2:6, 2:6]
+    pc = oc.download(1, "p")[2:6, 2:6, 2:6]
+    e = wc[4] - 0.5 * (wc[1] ** 2 + wc[2] ** 2 + wc[3] ** 2) / wc[0]
+    assert np.allclose(pc, 0.4 * np.maximum(e, 0.0), rtol=1e-14, atol=0)
+    assert pc[0, 0, 0] == 0.0 and np.all(pc.reshape(-1)[1:] > 0)
+
+
+def test_timestep_closed_form_uniform_state():
+    """timestep.f90:41-115 on a uniform state: lambda_d = |u.S_d| + c|S_d|,
+    dt = V/sum(lambda) (the adaptive factor is 1: no pressure curvature)."""
+    oc, prm, geoms, _ = _oracle(n=16, nz=4, nlevel=1, body=False, mach=0.4, alpha=20.0)
+    oc.init_flow()
+    oc.timestep(1)
+    d = prm.derived()
+    spec = geoms[0].spec
+    hx, hy, hz = spec.spacing()
+    c = np.sqrt(1.4 * d["p0"])
+    li = abs(d["u0"]) * hy * hz + c * hy * hz
+    lj = abs(d["v0"]) * hx * hz + c * hx * hz
+    lk = abs(d["w0"]) * hx * hy + c * hx * hy
+    n = spec.n
+    assert np.allclose(_interior(oc.download(1, "radi"), n), li, rtol=1e-13)
+    assert np.allclose(_interior(oc.download(1, "radj"), n), lj, rtol=1e-13)
+    assert np.allclose(_interior(oc.download(1, "radk"), n), lk, rtol=1e-13)
+    assert np.allclose(_interior(oc.download(1, "vdtim"), n), hx * hy * hz / (li + lj + lk), rtol=1e-13)
+
+
+def test_freestream_preserved_through_wcycle():
+    """No body: the JST residual of a uniform flow is zero, far-field ghosts
+    reproduce the freestream and the transfers keep it (size-independent)."""
+    oc, prm, _, _ = _oracle(n=32, nz=4, nlevel=3, body=False, alpha=10.0)
+    oc.init_flow()
+    w0 = oc.download(3, "w")
+    log, cs = oc.solve([1, 1, 3])
+    w1 = oc.download(3, "w")
+    assert np.max(np.abs(w1 - w0)) < 1e-12
+    assert np.all(log[:, 2] < 1e-13)
+
+
+def test_wcycle_work_count():
+    """solver.f90 W-cycle on 3 levels: 6 N3 + 17 N2 + 10 N1 cell-stage
+    passes per cycle; V-cycle 6/11/5 (the lopmg trace of the reference)."""
+    for mgtype, coef in ((2, (10, 17, 6)), (1, (5, 11, 6))):
+        oc, prm, geoms, _ = _oracle(n=32, nz=4, nlevel=3, body=False, mgtype=mgtype)
+        oc.init_flow()
+        _, cs = oc.solve([0, 0, 2])
+        N = [g.spec.ncell for g in geoms]
+        assert cs == 2 * sum(c * n for c, n in zip(coef, N)), (mgtype, cs)
+
+
+def test_y_symmetry_without_smoothing():
+    """Cylinder at alpha = 0 with smoothing off: the state stays mirror
+    symmetric in y and the lift is exactly zero.  (With smoothing the
+    reference's psmoo_old breaks the symmetry on purpose-free grounds: after
+    a blank cell it keeps the full-line pivots.)"""
+    oc, prm, geoms, _ = _oracle(n=32, nz=4, nlevel=1, smoo=0.0)
+    oc.init_flow()
+    for c in range(1, 4):
+        oc.timestep(1)
+        oc.euler(1, c, 1, 1, 0, 0)
+    w = oc.download(1, "w")
+    wm = w[:, :, ::-1, :].copy()
+    wm[2] *= -1
+    assert np.max(np.abs(_interior(w - wm, geoms[0].spec.n))) < 1e-13
+    F = oc.wall_force()
+    assert F[1] == 0.0 and F[0] > 0.0
+
+
+def test_psmoo_equals_tridiagonal_solve_without_blanks():
+    """psmoo_old without blanks = the product of the three constant
+    tridiagonal inverses (1+2e, -e), Dirichlet-zero ends (brute force)."""
+    oc, prm, geoms, _ = _oracle(n=8, nz=6, nlevel=1, body=False, smoo=0.6)
+    spec = geoms[0].spec
+    ni, nj, nk = spec.n
+    rng = np.random.default_rng(3)
+    dw = np.zeros((5, nk + 4, nj + 4, ni + 4))
+    dw[:, 2:nk + 2, 2:nj + 2, 2:ni + 2] = rng.normal(size=(5, nk, nj, ni))
+    oc.upload(1, "dw", dw)
+    assert oc.psmoo(1) == 0
+    out = _interior(oc.download(1, "dw"), spec.n)
+    e = 0.6
+
+    def T(n):
+        return np.diag(np.full(n, 1 + 2 * e)) + np.diag(np.full(n - 1, -e), 1) + np.diag(np.full(n - 1, -e), -1)
+
+    x = dw[:, 2:nk + 2, 2:nj + 2, 2:ni + 2]
+    x = np.linalg.solve(T(ni), np.moveaxis(x, 3, 0).reshape(ni, -1)).reshape((ni, 5, nk, nj))
+    x = np.moveaxis(x, 0, 3)
+    x = np.moveaxis(np.linalg.solve(T(nj), np.moveaxis(x, 2, 0).reshape(nj, -1)).reshape((nj, 5, nk, ni)), 0, 2)
+    x = np.moveaxis(np.linalg.solve(T(nk), np.moveaxis(x, 1, 0).reshape(nk, -1)).reshape((nk, 5, nj, ni)), 0, 1)
+    assert np.allclose(out, x, rtol=1e-12, atol=1e-14)
+
+
+def test_psmoo_blank_cells_stay_zero():
+    """blanks keep dw = 0 inside the recurrences (psmoo.f90:174-237)"""
+    oc, prm, geoms, _ = _oracle(n=32, nz=4, nlevel=1)
+    spec = geoms[0].spec
+    ni, nj, nk = spec.n
+    rng = np.random.default_rng(4)
+    blank = geoms[0].blank
+    dw = np.zeros((5, nk + 4, nj + 4, ni + 4))
+    dw[:, 2:nk + 2, 2:nj + 2, 2:ni + 2] = rng.normal(size=(5, nk, nj, ni)) * blank
+    oc.upload(1, "dw", dw)
+    oc.psmoo(1)
+    out = _interior(oc.download(1, "dw"), spec.n)
+    assert np.all(out[:, blank == 0] == 0.0)
+    assert np.count_nonzero(out[:, blank == 1]) > 0
+
+
+def test_convective_residual_telescopes_on_periodic_box():
+    """eflux assigns dw = sum of face-flux differences; with periodic faces
+    everywhere and no dissipation the flux sum over the box is zero."""
+    spec = LevelSpec(n=(12, 10, 8), lo=(0, 0, 0), hi=(1.2, 1.0, 0.8), ratio=(2, 2, 2), bctype=(-60,) * 6)
+    prm = Params(mach=0.5, alpha_deg=30.0, cdis=(0, 0, 0, 0, 0), smoo=0.0)
+    g = level_geometry(spec, None)
+    oc = OracleCase(prm, [g])
+    oc.init_flow()
+    rng = np.random.default_rng(5)
+    w = oc.download(1, "w")
+    w *= 1.0 + 0.05 * rng.uniform(-1, 1, size=w.shape)
+    oc.upload(1, "w", w)
+    s = w[1] ** 2 + w[2] ** 2 + w[3] ** 2
+    oc.upload(1, "p", 0.4 * (w[4] - 0.5 * s / w[0]))
+    oc.bcond(1)
+    oc.timestep(1)
+    oc.euler(1, 1, 1, 1, 1, 0)  # residual only
+    dw = _interior(oc.download(1, "dw"), spec.n)
+    tot = dw.reshape(5, -1).sum(axis=1)
+    assert np.all(np.abs(tot) < 1e-12 * np.abs(dw).max() * dw[0].size), tot
+    assert np.abs(dw).max() > 1e-6
+
+
+def test_restriction_sums_residual_and_averages_state():
+    """collc for ratio 2: w_c = mean of the 8 children, wr_c = sum of their
+    dw (V_c = 8 V_f); linear fields restrict exactly."""
+    oc, prm, geoms, _ = _oracle(cfg="sphere3d", n=16, nlevel=2, body=False)
+    oc.init_flow()
+    f = geoms[1].spec
+    ni, nj, nk = f.n
+    xc = [0.5 * (v[1:-1] + v[2:]) for v in (geoms[1].xv, geoms[1].yv, geoms[1].zv)]
+    Z, Y, X = np.meshgrid(xc[2], xc[1], xc[0], indexing="ij")
+    w = oc.download(2, "w")
+    w[0, 1:-1, 1:-1, 1:-1] = 1.0 + 0.01 * X + 0.02 * Y - 0.03 * Z
+    oc.upload(2, "w", w)
+    rng = np.random.default_rng(6)
+    dw = np.zeros_like(w)
+    dw[:, 2:nk + 2, 2:nj + 2, 2:ni + 2] = rng.normal(size=(5, nk, nj, ni))
+    oc.upload(2, "dw", dw)
+    oc.movco(1)
+    c = geoms[0].spec
+    wr = _interior(oc.download(1, "wr"), c.n)
+    assert np.allclose(wr.reshape(5, -1).sum(1), dw.reshape(5, -1).sum(1), rtol=1e-12, atol=1e-12)
+    blocks = dw[:, 2:nk + 2, 2:nj + 2, 2:ni + 2].reshape(5, nk // 2, 2, nj // 2, 2, ni // 2, 2).sum(axis=(2, 4, 6))
+    assert np.allclose(wr, blocks, rtol=1e-12, atol=1e-13)
+    xcc = [0.5 * (v[1:-1] + v[2:]) for v in (geoms[0].xv, geoms[0].yv, geoms[0].zv)]
+    Zc, Yc, Xc = np.meshgrid(xcc[2], xcc[1], xcc[0], indexing="ij")
+    lin = 1.0 + 0.01 * Xc + 0.02 * Yc - 0.03 * Zc
+    assert np.allclose(_interior(oc.download(1, "w")[0], c.n), _interior(lin, c.n), rtol=0, atol=1e-14)
+
+
+def test_prolongation_linear_exact_and_fmg_weights():
+    """addw on the FMG step interpolates the coarse solution trilinearly
+    (weights 27/64, 9/64, 3/64, 1/64): a linear field is reproduced at
+    fine cells whose stencil is interior."""
+    oc, prm, geoms, _ = _oracle(cfg="sphere3d", n=16, nlevel=2, body=False)
+    oc.init_flow()
+    xcc = [0.5 * (v[1:-1] + v[2:]) for v in (geoms[0].xv, geoms[0].yv, geoms[0].zv)]
+    Zc, Yc, Xc = np.meshgrid(xcc[2], xcc[1], xcc[0], indexing="ij")
+    w = oc.download(1, "w")
+    w[0, 1:-1, 1:-1, 1:-1] = 2.0 + 0.1 * Xc - 0.05 * Yc + 0.02 * Zc
+    oc.upload(1, "w", w)
+    oc.movfin(2, 1)
+    xc = [0.5 * (v[1:-1] + v[2:]) for v in (geoms[1].xv, geoms[1].yv, geoms[1].zv)]
+    Z, Y, X = np.meshgrid(xc[2], xc[1], xc[0], indexing="ij")
+    lin = 2.0 + 0.1 * X - 0.05 * Y + 0.02 * Z
+    wf = oc.download(2, "w")[0][1:-1, 1:-1, 1:-1]   # cells 1..n+2, like lin
+    sl = (slice(4, -4),) * 3
+    assert np.allclose(wf[sl], lin[sl], rtol=0, atol=1e-13)
+
+
+def test_far_field_reproduces_freestream():
+    """bc_far with a freestream interior gives freestream ghosts (Riemann
+    invariants of a uniform state), for inflow and outflow faces."""
+    oc, prm, geoms, _ = _oracle(cfg="sphere3d", n=8, nlevel=1, body=False, alpha=25.0, mach=0.6)
+    oc.init_flow()
+    w = oc.download(1, "w")
+    w0 = w.copy()
+    w[:, 0:2] = 0.0
+    w[:, -2:] = 0.0
+    w[0, 0:2] = 1.0
+    w[0, -2:] = 1.0
+    oc.upload(1, "w", w)
+    oc.bcond(1)
+    w1 = oc.download(1, "w")
+    assert np.max(np.abs(w1[:, 0:2, 2:-2, 2:-2] - w0[:, 0:2, 2:-2, 2:-2])) < 1e-13
+    assert np.max(np.abs(w1[:, -2:, 2:-2, 2:-2] - w0[:, -2:, 2:-2, 2:-2])) < 1e-13
+
+
+def test_matrix_dissipation_degenerates_to_scalar():
+    """dflux_matrix with epsn = epsl = 1 (dflux.f90:300-301) returns
+    (|q_n| + c|S|) dw with the face-mean state (closed form)."""
+    rng = np.random.default_rng(8)
+    g = 1.4
+    for _ in range(20):
+        u1 = np.array([1.0, 0.3, -0.2, 0.1, 3.0]) * (1 + 0.1 * rng.uniform(-1, 1, 5))
+        u2 = np.array([1.1, 0.2, -0.1, 0.05, 3.2]) * (1 + 0.1 * rng.uniform(-1, 1, 5))
+        dw = rng.normal(size=5)
+        ss = rng.normal(size=3)
+        ff = dflux_matrix(g, u1, u2, dw, ss)
+        rr = 0.5 * (u1[0] + u2[0])
+        us = 0.5 * (u1[1:4] + u2[1:4]) / rr
+        ee = 0.5 * (u1[4] + u2[4]) / rr
+        qq = 0.5 * us @ us
+        c2 = g * (g - 1) * (ee - qq)
+        e0 = abs(ss @ us) + np.sqrt(c2 * (ss @ ss))
+        assert np.allclose(ff, e0 * dw, rtol=1e-12, atol=1e-14)
+
+
+def test_rtrms_formula():
+    """euler.f90:431-447: rtrms = sqrt(sum over fluid cells of
+    (dw_1/(cfl dt))^2 / N), blanks counted in N."""
+    oc, prm, geoms, _ = _oracle(n=32, nz=4, nlevel=1)
+    oc.init_flow()
+    oc.timestep(1)
+    r = oc.euler(1, 1, 1, 1, 0, 0)
+    spec = geoms[0].spec
+    dw = _interior(oc.download(1, "dw")[0], spec.n)
+    vd = _interior(oc.download(1, "vdtim"), spec.n)
+    bl = geoms[0].blank
+    rt = np.where(bl == 1, dw / (prm.derived()["cfl"] * np.where(bl == 1, vd, 1.0)), 0.0)
+    assert abs(np.sqrt((rt ** 2).sum() / rt.size) - r) < 1e-13 * r
+
+
+def test_mirror_state_reflects_normal_momentum():
+    """bc_cutcell (Euler): rho and p of the reflected node are kept, the
+    normal momentum flips, the tangential momentum is kept; weights sum to 1."""
+    oc, prm, geoms, _ = _oracle(cfg="sphere3d", n=24, nlevel=1, box=3.0)
+    oc.init_flow()
+    rng = np.random.default_rng(9)
+    w = oc.download(1, "w")
+    w *= 1 + 0.05 * rng.uniform(-1, 1, size=w.shape)
+    oc.upload(1, "w", w)
+    s = w[1] ** 2 + w[2] ** 2 + w[3] ** 2
+    p = 0.4 * (w[4] - 0.5 * s / w[0])
+    oc.upload(1, "p", p)
+    oc.bc_cutcell(1)
+    cuts, n = oc.cuts(1)
+    assert n > 0
+    for m in range(n):
+        r = cuts[m]
+        co = np.array(r.coef[: r.npoint])
+        assert abs(co.sum() - 1) < 1e-14
+        idx = [tuple(r.ijk[c]) for c in range(r.npoint)]
+        wr = sum(co[c] * w[:, k, j, i] for c, (i, j, k) in enumerate(idx))
+        pr = sum(co[c] * p[k, j, i] for c, (i, j, k) in enumerate(idx))
+        nv = np.array(r.nvec[:])
+        nv /= np.linalg.norm(nv)
+        ws = np.array(r.wsol[:])
+        assert abs(ws[0] - wr[0]) < 1e-14 and abs(r.psol - pr) < 1e-13
+        assert abs(ws[1:4] @ nv + wr[1:4] @ nv) < 1e-13
+        t_s = ws[1:4] - (ws[1:4] @ nv) * nv
+        t_r = wr[1:4] - (wr[1:4] @ nv) * nv
+        assert np.allclose(t_s, t_r, atol=1e-13)
+        assert abs(ws[4] - (r.psol / 0.4 + 0.5 * (ws[1:4] @ ws[1:4]) / ws[0])) < 1e-12
+
+
+@pytest.mark.parametrize("dissip", ["scalar", "matrix"])
+def test_short_solve_converges(dissip):
+    """end to end: the residual of the 64x64x4 cylinder falls over 30
+    W-cycles and the drag is positive"""
+    oc, prm, geoms, aref = _oracle(n=64, nz=4, nlevel=3, dissip=dissip)
+    oc.init_flow()
+    log, _ = oc.solve([0, 0, 30])
+    assert log[-1, 2] < 0.2 * log[0, 2]
+    assert oc.wall_force()[0] > 0
