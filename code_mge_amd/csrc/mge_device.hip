@@ -115,9 +115,10 @@ struct RecArgs {
   const double *psol;
   int nmir;
   const long *tcell;        // touched cells
-  const int *fw_beg, *dw_beg;   // op ranges (ntouch+1)
-  const int2 *fw_ops, *dw_ops;  // (record, kind)
-  int ntouch;
+  const int *beg;           // op range of each touched cell (ntouch+1)
+  const int2 *ops;          // (record, kind), fw ops then dw ops per cell
+  double *opval;            // 10 doubles per op
+  int ntouch, nops;
   int *err;
 };
 
@@ -433,7 +434,77 @@ __device__ void efluxb_rec(const FluxCtx &c, const DevRec &r, const double *wsol
   for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
 }
 
-__global__ void __launch_bounds__(64) k_records(RecArgs R) {
+// the value each op contributes (dfluxb.f90 / efluxb.f90 terms), one
+// thread per op; k_rec_apply adds them in the reference order
+__global__ void __launch_bounds__(64) k_rec_ops(RecArgs R) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= R.nops) return;
+  const StageArgs &A = R.st;
+  int2 op = R.ops[o];
+  double *out = R.opval + (long)o * 10;
+  const DevRec &r = R.recs[op.x];
+  if (op.y < OP_E_UNDO && !A.has_diss) return;
+  int zz = r.zz;
+  int s0 = zz == 1 ? r.c[0] : (zz == 2 ? r.c[1] : r.c[2]);
+  double ws[5];
+  for (int n = 0; n < 5; n++) ws[n] = R.wsol[n * R.nmir + op.x];
+  double ps = R.psol[op.x];
+  double a[5], b[5], fs1[5], fs2[5];
+  int ci = r.c[0], cj = r.c[1], ck = r.c[2];
+  switch (op.y) {
+    case OP_HI_C_UNDO:
+    case OP_LO_C_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_HI_N_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_LO_N_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_HI_C_ADD:
+    case OP_LO_C_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      for (int n = 0; n < 5; n++) out[n] = fs1[n] - fs2[n];
+      break;
+    case OP_HI_N_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - fs1[n];
+      break;
+    case OP_LO_N_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, a);
+      for (int n = 0; n < 5; n++) out[n] = fs2[n] - a[n];
+      break;
+    case OP_E_UNDO:
+      rec_efs_any(A.fx, zz, ci, cj, ck, s0, a);
+      rec_efs_any(A.fx, zz, ci, cj, ck, s0 + 1, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_E_ADD: {
+      double fs[5], fa[5];
+      efluxb_rec(A.fx, r, ws, ps, fs, fa, R.err);
+      bool hi = r.faceid >= 4;
+      for (int n = 0; n < 5; n++) {
+        out[n] = hi ? fa[n] : fs[n];
+        out[5 + n] = hi ? fs[n] : fa[n];
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// per touched cell: apply the op values in record order, then finish
+__global__ void __launch_bounds__(64) k_rec_apply(RecArgs R) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= R.ntouch) return;
   const StageArgs &A = R.st;
@@ -448,74 +519,32 @@ __global__ void __launch_bounds__(64) k_records(RecArgs R) {
     f[n] = A.fw[n * E + q];
     e[n] = A.dw[n * E + q];
   }
-  if (A.has_diss) {
-    for (int o = R.fw_beg[t]; o < R.fw_beg[t + 1]; o++) {
-      int2 op = R.fw_ops[o];
-      const DevRec &r = R.recs[op.x];
-      int zz = r.zz;
-      int s0 = zz == 1 ? r.c[0] : (zz == 2 ? r.c[1] : r.c[2]);
-      double ws[5];
-      for (int n = 0; n < 5; n++) ws[n] = R.wsol[n * R.nmir + op.x];
-      double ps = R.psol[op.x];
-      double a[5], b[5], fs1[5], fs2[5];
-      int ci = r.c[0], cj = r.c[1], ck = r.c[2];
-      switch (op.y) {
-        case OP_HI_C_UNDO:
-        case OP_LO_C_UNDO:
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0, a);
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, b);
-          for (int n = 0; n < 5; n++) f[n] = f[n] - (a[n] - b[n]);
-          break;
-        case OP_HI_N_UNDO:
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0, b);
-          for (int n = 0; n < 5; n++) f[n] = f[n] - (a[n] - b[n]);
-          break;
-        case OP_LO_N_UNDO:
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, a);
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, b);
-          for (int n = 0; n < 5; n++) f[n] = f[n] - (a[n] - b[n]);
-          break;
-        case OP_HI_C_ADD:
-        case OP_LO_C_ADD:
-          dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
-          for (int n = 0; n < 5; n++) f[n] = f[n] + (fs1[n] - fs2[n]);
-          break;
-        case OP_HI_N_ADD:
-          dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
-          for (int n = 0; n < 5; n++) f[n] = f[n] + (a[n] - fs1[n]);
-          break;
-        case OP_LO_N_ADD:
-          dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
-          rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, a);
-          for (int n = 0; n < 5; n++) f[n] = f[n] + (fs2[n] - a[n]);
-          break;
-        default:
-          break;
-      }
-    }
-  }
-  for (int o = R.dw_beg[t]; o < R.dw_beg[t + 1]; o++) {
-    int2 op = R.dw_ops[o];
-    const DevRec &r = R.recs[op.x];
-    int zz = r.zz;
-    int s0 = zz == 1 ? r.c[0] : (zz == 2 ? r.c[1] : r.c[2]);
-    double ws[5];
-    for (int n = 0; n < 5; n++) ws[n] = R.wsol[n * R.nmir + op.x];
-    double ps = R.psol[op.x];
-    if (op.y == OP_E_UNDO) {
-      double a[5], b[5];
-      rec_efs_any(A.fx, zz, r.c[0], r.c[1], r.c[2], s0, a);
-      rec_efs_any(A.fx, zz, r.c[0], r.c[1], r.c[2], s0 + 1, b);
-      for (int n = 0; n < 5; n++) e[n] = e[n] - (a[n] - b[n]);
-    } else {
-      double fs[5], fa[5];
-      efluxb_rec(A.fx, r, ws, ps, fs, fa, R.err);
-      if (r.faceid >= 4)
-        for (int n = 0; n < 5; n++) e[n] = e[n] + fa[n] - fs[n];
-      else
-        for (int n = 0; n < 5; n++) e[n] = e[n] + fs[n] - fa[n];
+  for (int o = R.beg[t]; o < R.beg[t + 1]; o++) {
+    int kind = R.ops[o].y;
+    const double *v = R.opval + (long)o * 10;
+    switch (kind) {
+      case OP_HI_C_UNDO:
+      case OP_HI_N_UNDO:
+      case OP_LO_C_UNDO:
+      case OP_LO_N_UNDO:
+        if (A.has_diss)
+          for (int n = 0; n < 5; n++) f[n] = f[n] - v[n];
+        break;
+      case OP_HI_C_ADD:
+      case OP_HI_N_ADD:
+      case OP_LO_C_ADD:
+      case OP_LO_N_ADD:
+        if (A.has_diss)
+          for (int n = 0; n < 5; n++) f[n] = f[n] + v[n];
+        break;
+      case OP_E_UNDO:
+        for (int n = 0; n < 5; n++) e[n] = e[n] - v[n];
+        break;
+      case OP_E_ADD:
+        for (int n = 0; n < 5; n++) e[n] = e[n] + v[n] - v[5 + n];
+        break;
+      default:
+        break;
     }
   }
   finish_cell(A, q, i, j, k, f, e);
@@ -527,7 +556,8 @@ struct SmoothArgs {
   Dims d;
   double *dw;
   const uint8_t *flags;
-  const double *e1, *e2;  // pivots of this direction
+  const double *e1, *e2;  // pivots of this direction (psmoo.f90:134-166)
+  const double *fa;       // -bb/e1 (x lines: affine-map form of the forward step)
   double bb;
   // z pass fuses the update
   const double *wn;
@@ -535,8 +565,106 @@ struct SmoothArgs {
   double gamma;
 };
 
-// psmoo_old.f90:171-188 (x lines); one thread per (j,k) line and component
-__global__ void __launch_bounds__(256) k_smooth_x(SmoothArgs S) {
+// psmoo_old.f90:171-188 (x lines), one wavefront per (j,k) line and
+// component.  Lane l owns CPL consecutive cells, so loads and stores are
+// coalesced.  Both recurrences are first composed as affine maps
+// (forward y_i = A_i y_(i-1) + B_i, backward z_i = C_i z_(i+1) + y_i),
+// scanned across the wave, and then every lane re-evaluates its own cells
+// with the reference formula from the scanned boundary value.  Only the
+// lane-boundary values differ from the sequential evaluation (by a few
+// ulp); blank cells keep their value as in the reference.
+template <int CPL>
+__global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs S) {
+  const Dims &d = S.d;
+  const int lane = threadIdx.x & 63;
+  const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+  const long nl = (long)d.nj * d.nk;
+  if (wave >= nl * 5) return;  // uniform per wave
+  const int n = (int)(wave / nl);
+  const long l = wave - (long)n * nl;
+  const int j = 2 + (int)(l % d.nj), k = 2 + (int)(l / d.nj);
+  double *D = S.dw + n * d.E;
+  const long b0 = ix(d, 0, j, k);
+  const int il = d.il, ni = d.ni;
+  const int i0 = 2 + lane * CPL;
+  double r[CPL];
+  bool fl[CPL], ok[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    ok[c] = i <= il;
+    r[c] = ok[c] ? D[b0 + i] : 0.0;
+    fl[c] = ok[c] && (S.flags[b0 + i] & F_FLUID);
+  }
+  // forward: lane composite of y_i = A_i y_(i-1) + B_i
+  double Al = 1.0, Bl = 0.0;
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    double A, B;
+    if (!ok[c]) { A = 1.0; B = 0.0; }
+    else if (!fl[c]) { A = 0.0; B = r[c]; }
+    else if (i == 2) { A = 0.0; B = r[c] / S.e1[2]; }
+    else { A = S.fa[i]; B = r[c] / S.e1[i]; }
+    Bl = A * Bl + B;
+    Al = A * Al;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    double Ap = __shfl_up(Al, off, 64), Bp = __shfl_up(Bl, off, 64);
+    if (lane >= off) {
+      Bl = Al * Bp + Bl;
+      Al = Al * Ap;
+    }
+  }
+  double yin = __shfl_up(Bl, 1, 64);
+  if (lane == 0) yin = 0.0;
+  double y[CPL];
+  double yp = yin;
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    double v = r[c];
+    if (ok[c] && fl[c]) v = (i == 2) ? r[c] / S.e1[2] : (r[c] - S.bb * yp) / S.e1[i];
+    y[c] = v;
+    yp = v;
+  }
+  // backward: z_i = C_i z_(i+1) + y_i, C_i = -e2_i on fluid cells 2..ni
+  double Cl = 1.0, Dl = 0.0;
+#pragma unroll
+  for (int c = CPL - 1; c >= 0; c--) {
+    int i = i0 + c;
+    double C, Dv;
+    if (!ok[c]) { C = 1.0; Dv = 0.0; }
+    else if (fl[c] && i <= ni) { C = -S.e2[i]; Dv = y[c]; }
+    else { C = 0.0; Dv = y[c]; }
+    Dl = C * Dl + Dv;
+    Cl = C * Cl;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    double Cn = __shfl_down(Cl, off, 64), Dn = __shfl_down(Dl, off, 64);
+    if (lane + off < 64) {
+      Dl = Cl * Dn + Dl;
+      Cl = Cl * Cn;
+    }
+  }
+  double zin = __shfl_down(Dl, 1, 64);
+  if (lane == 63) zin = 0.0;
+  double zn = zin;
+#pragma unroll
+  for (int c = CPL - 1; c >= 0; c--) {
+    int i = i0 + c;
+    if (!ok[c]) continue;
+    double v = y[c];
+    if (fl[c] && i <= ni) v = y[c] - S.e2[i] * zn;
+    zn = v;
+    if (fl[c]) D[b0 + i] = v;
+  }
+}
+
+// sequential x lines for blocks wider than 512 cells
+__global__ void __launch_bounds__(256) k_smooth_x_seq(SmoothArgs S) {
   const Dims &d = S.d;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long nl = (long)d.nj * d.nk;
@@ -555,52 +683,125 @@ __global__ void __launch_bounds__(256) k_smooth_x(SmoothArgs S) {
   }
 }
 
-// psmoo_old.f90:190-206 (y lines); thread per (i,k), coalesced over i
-__global__ void __launch_bounds__(256) k_smooth_y(SmoothArgs S) {
+// One line of the exact sequential recurrence of psmoo_old
+// (psmoo.f90:190-237): cells s = 2..nl at stride st.  The line is read in
+// register chunks of CH cells, double-buffered, so one chunk's loads are in
+// flight while the previous chunk runs its dependent divisions (a single
+// prefetched cell cannot cover HBM latency).  Chunk buffers are named
+// arrays with compile-time indices so they stay in registers.
+#define SWEEP_LOAD(BUF, FB, c, s_of)                                 \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
+    int s_ = s_of(c, t);                                             \
+    if (s_ >= 2 && s_ <= nl) {                                       \
+      BUF[t] = D[b0 + (long)s_ * st];                                \
+      FB[t] = F[b0 + (long)s_ * st];                                 \
+    }                                                                \
+  }
+#define FWD_S(c, t) (2 + (c) * CH + (t))
+#define BWD_S(c, t) (lm - (c) * CH - (t))
+#define FWD_PROC(BUF, FB, c)                                         \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
+    int s_ = FWD_S(c, t);                                            \
+    if (s_ <= nl) {                                                  \
+      double r_ = BUF[t];                                            \
+      if (FB[t] & F_FLUID) {                                         \
+        y = (s_ == 2) ? r_ / e1[2] : (r_ - bb * y) / e1[s_];         \
+        D[b0 + (long)s_ * st] = y;                                   \
+      } else {                                                       \
+        y = r_;                                                      \
+      }                                                              \
+    }                                                                \
+  }
+#define BWD_PROC(BUF, FB, c)                                         \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
+    int s_ = BWD_S(c, t);                                            \
+    if (s_ >= 2) {                                                   \
+      double r_ = BUF[t];                                            \
+      if (FB[t] & F_FLUID) {                                         \
+        y = r_ - e2[s_] * y;                                         \
+        D[b0 + (long)s_ * st] = y;                                   \
+      } else {                                                       \
+        y = r_;                                                      \
+      }                                                              \
+    }                                                                \
+  }
+
+template <int CH>
+__device__ __attribute__((always_inline)) inline void line_sweep(double *D, const uint8_t *F, long b0, long st,
+                                                                 int nl, int lm,
+                                  const double *e1, const double *e2, double bb) {
+  double A[CH], B[CH];
+  uint8_t FA[CH], FBb[CH];
+  double y = 0.0;
+  // forward, s = 2..nl
+  const int nf = (nl - 1 + CH - 1) / CH;
+  SWEEP_LOAD(A, FA, 0, FWD_S);
+  for (int c = 0; c < nf; c += 2) {
+    if (c + 1 < nf) { SWEEP_LOAD(B, FBb, c + 1, FWD_S); }
+    FWD_PROC(A, FA, c);
+    if (c + 1 >= nf) break;
+    if (c + 2 < nf) { SWEEP_LOAD(A, FA, c + 2, FWD_S); }
+    FWD_PROC(B, FBb, c + 1);
+  }
+  // backward, s = lm..2 with z(nl) = y(nl)
+  const int nb = (lm - 1 + CH - 1) / CH;
+  if (nb <= 0) return;
+  SWEEP_LOAD(A, FA, 0, BWD_S);
+  for (int c = 0; c < nb; c += 2) {
+    if (c + 1 < nb) { SWEEP_LOAD(B, FBb, c + 1, BWD_S); }
+    BWD_PROC(A, FA, c);
+    if (c + 1 >= nb) break;
+    if (c + 2 < nb) { SWEEP_LOAD(A, FA, c + 2, BWD_S); }
+    BWD_PROC(B, FBb, c + 1);
+  }
+}
+#undef SWEEP_LOAD
+#undef FWD_S
+#undef BWD_S
+#undef FWD_PROC
+#undef BWD_PROC
+
+// psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
+__global__ void __launch_bounds__(64) k_smooth_y(SmoothArgs S) {
   const Dims &d = S.d;
   int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int k = 2 + blockIdx.y;
   int n = blockIdx.z;
   if (i > d.il) return;
-  double *D = S.dw + n * d.E;
-  long b0 = ix(d, i, 0, k);
-  const long sy = d.sy;
-  if (S.flags[b0 + 2 * sy] & F_FLUID) D[b0 + 2 * sy] = D[b0 + 2 * sy] / S.e1[2];
-  for (int j = 3; j <= d.jl; j++)
-    if (S.flags[b0 + j * sy] & F_FLUID) D[b0 + j * sy] = (D[b0 + j * sy] - S.bb * D[b0 + (j - 1) * sy]) / S.e1[j];
-  for (int j = 2; j <= d.nj; j++) {
-    int jr = d.jl + 1 - j;
-    if (S.flags[b0 + jr * sy] & F_FLUID) D[b0 + jr * sy] = D[b0 + jr * sy] - S.e2[jr] * D[b0 + (jr + 1) * sy];
-  }
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, d.nj, S.e1, S.e2, S.bb);
 }
 
-// psmoo_old.f90:211-237 (z lines) + the update w = wn - dw and the EOS
-// (euler.f90:381-399); thread per (i,j), all five components
-__global__ void __launch_bounds__(256) k_smooth_z(SmoothArgs S) {
+// psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
+__global__ void __launch_bounds__(64) k_smooth_z(SmoothArgs S) {
   const Dims &d = S.d;
   int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 2 + blockIdx.y;
+  int n = blockIdx.z;
   if (i > d.il) return;
-  const long sz = d.sz, E = d.E;
-  long b0 = ix(d, i, j, 0);
-  for (int n = 0; n < 5; n++) {
-    double *D = S.dw + n * E;
-    if (S.flags[b0 + 2 * sz] & F_FLUID) D[b0 + 2 * sz] = D[b0 + 2 * sz] / S.e1[2];
-    for (int k = 3; k <= d.kl; k++)
-      if (S.flags[b0 + k * sz] & F_FLUID) D[b0 + k * sz] = (D[b0 + k * sz] - S.bb * D[b0 + (k - 1) * sz]) / S.e1[k];
-    for (int k = 2; k <= d.nk; k++) {
-      int kr = d.kl + 1 - k;
-      if (S.flags[b0 + kr * sz] & F_FLUID) D[b0 + kr * sz] = D[b0 + kr * sz] - S.e2[kr] * D[b0 + (kr + 1) * sz];
-    }
-  }
-  for (int k = 2; k <= d.kl; k++) {
-    long q = b0 + k * sz;
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, d.nk, S.e1, S.e2, S.bb);
+}
+
+// after smoothing: w = wn - dw and the EOS on interior cells
+// (euler.f90:381-399); the ghost shell of the previous state is carried
+// over, as k_ghost_copy does on unsmoothed stages
+__global__ void __launch_bounds__(256) k_update_ghost(Dims d, const double *wn, const double *dw, StateC src,
+                                                      State dst, double gamma) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= d.E) return;
+  int k = (int)(q / d.sz);
+  int j = (int)((q - (long)k * d.sz) / d.sy);
+  int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
+  bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
+  if (inner) {
     double v[5];
     for (int n = 0; n < 5; n++) {
-      v[n] = S.wn[n * E + q] - S.dw[n * E + q] / 1.0;
-      S.wdst[n * E + q] = v[n];
+      v[n] = wn[n * d.E + q] - dw[n * d.E + q] / 1.0;
+      dst.w[n * d.E + q] = v[n];
     }
-    S.pdst[q] = eos(S.gamma, v);
+    dst.p[q] = eos(gamma, v);
+  } else {
+    for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
+    dst.p[q] = src.p[q];
   }
 }
 
@@ -953,62 +1154,74 @@ __device__ inline void corner_eos(const Dims &d, State s, double gamma, long q) 
   s.p[q] = eos(gamma, v);
 }
 
-__global__ void __launch_bounds__(256) k_bc_corner(Dims d, State s, double gamma) {
+// the four ordered phases of bc_corner.f90; within a phase every (edge,
+// index) item is independent, so one 1024-thread workgroup runs each phase
+// in parallel and synchronises between phases
+__device__ inline void edge_fill(const Dims &d, State s, double gamma, int ti, int tj, int tk, int ai, int aj,
+                                 int ak, int bi, int bj, int bk, int ci, int cj, int ck) {
+  const long E = d.E;
+  long qt = ix(d, ti, tj, tk), qa = ix(d, ai, aj, ak), qb = ix(d, bi, bj, bk), qc = ix(d, ci, cj, ck);
+  double v[5];
+  for (int n = 0; n < 5; n++) {
+    v[n] = s.w[n * E + qa] + s.w[n * E + qb] - s.w[n * E + qc];
+    s.w[n * E + qt] = v[n];
+  }
+  s.p[qt] = eos(gamma, v);
+}
+
+__global__ void __launch_bounds__(1024) k_bc_corner(Dims d, State s, double gamma) {
   const long E = d.E;
   double *W = s.w;
   const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke, ib = d.ib;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // phase 1: edges along k (bc_corner.f90:23-61)
+  for (int t = tid; t < 4 * (kl - 1); t += nt) {
+    int e = t & 3, k = 2 + (t >> 2);
+    if (e == 0) edge_fill(d, s, gamma, 1, 1, k, 2, 1, k, 1, 2, k, 2, 2, k);
+    else if (e == 1) edge_fill(d, s, gamma, ie, 1, k, il, 1, k, ie, 2, k, il, 2, k);
+    else if (e == 2) edge_fill(d, s, gamma, 1, je, k, 2, je, k, 1, jl, k, 2, jl, k);
+    else edge_fill(d, s, gamma, ie, je, k, il, je, k, ie, jl, k, il, jl, k);
+  }
+  __syncthreads();
+  // phase 2: edges along j (:63-99)
+  for (int t = tid; t < 4 * (jl - 1); t += nt) {
+    int e = t & 3, j = 2 + (t >> 2);
+    if (e == 0) edge_fill(d, s, gamma, 1, j, 1, 2, j, 1, 1, j, 2, 2, j, 2);
+    else if (e == 1) edge_fill(d, s, gamma, ie, j, 1, il, j, 1, ie, j, 2, il, j, 2);
+    else if (e == 2) edge_fill(d, s, gamma, 1, j, ke, 2, j, ke, 1, j, kl, 2, j, kl);
+    else edge_fill(d, s, gamma, ie, j, ke, il, j, ke, ie, j, kl, il, j, kl);
+  }
+  __syncthreads();
+  // phase 3: edges along i including the ghost i (:101-137); the third
+  // line adds and subtracts the same value (:120)
+  for (int t = tid; t < 4 * (ib + 1); t += nt) {
+    int e = t & 3, i = t >> 2;
+    if (e == 0) edge_fill(d, s, gamma, i, 1, 1, i, 1, 2, i, 2, 1, i, 2, 2);
+    else if (e == 1) edge_fill(d, s, gamma, i, 1, ke, i, 2, ke, i, 1, kl, i, 2, kl);
+    else if (e == 2) edge_fill(d, s, gamma, i, je, 1, i, je, 1, i, jl, 2, i, jl, 2);
+    else edge_fill(d, s, gamma, i, je, ke, i, jl, ke, i, je, kl, i, jl, kl);
+  }
+  __syncthreads();
 #define WQ(n, i, j, k) W[(n) * E + ix(d, i, j, k)]
-  for (int k = 2 + threadIdx.x; k <= kl; k += blockDim.x) {
-    for (int n = 0; n < 5; n++) WQ(n, 1, 1, k) = WQ(n, 2, 1, k) + WQ(n, 1, 2, k) - WQ(n, 2, 2, k);
-    corner_eos(d, s, gamma, ix(d, 1, 1, k));
-    for (int n = 0; n < 5; n++) WQ(n, ie, 1, k) = WQ(n, il, 1, k) + WQ(n, ie, 2, k) - WQ(n, il, 2, k);
-    corner_eos(d, s, gamma, ix(d, ie, 1, k));
-    for (int n = 0; n < 5; n++) WQ(n, 1, je, k) = WQ(n, 2, je, k) + WQ(n, 1, jl, k) - WQ(n, 2, jl, k);
-    corner_eos(d, s, gamma, ix(d, 1, je, k));
-    for (int n = 0; n < 5; n++) WQ(n, ie, je, k) = WQ(n, il, je, k) + WQ(n, ie, jl, k) - WQ(n, il, jl, k);
-    corner_eos(d, s, gamma, ix(d, ie, je, k));
+  // phase 4: the 8 corners (:141-157)
+  if (tid < 40) {
+    int n = tid % 5, c = tid / 5;
+    if (c == 0) WQ(n, 1, 1, 1) = (WQ(n, 2, 1, 1) + WQ(n, 1, 2, 1) + WQ(n, 1, 1, 2)) * 2.0 / 3.0 - WQ(n, 2, 2, 2);
+    if (c == 1) WQ(n, 1, 1, ke) = (WQ(n, 2, 1, ke) + WQ(n, 1, 2, ke) + WQ(n, 1, 1, kl)) * 2.0 / 3.0 - WQ(n, 2, 2, kl);
+    if (c == 2) WQ(n, 1, je, 1) = (WQ(n, 2, je, 1) + WQ(n, 1, jl, 1) + WQ(n, 1, je, 2)) * 2.0 / 3.0 - WQ(n, 2, jl, 2);
+    if (c == 3) WQ(n, ie, 1, 1) = (WQ(n, il, 1, 1) + WQ(n, ie, 2, 1) + WQ(n, ie, 1, 2)) * 2.0 / 3.0 - WQ(n, il, 2, 2);
+    if (c == 4) WQ(n, ie, 1, ke) = (WQ(n, il, 1, ke) + WQ(n, ie, 2, ke) + WQ(n, ie, 1, kl)) * 2.0 / 3.0 - WQ(n, il, 2, kl);
+    if (c == 5) WQ(n, ie, je, 1) = (WQ(n, il, je, 1) + WQ(n, ie, jl, 1) + WQ(n, ie, je, 2)) * 2.0 / 3.0 - WQ(n, il, jl, 2);
+    if (c == 6) WQ(n, 1, je, ke) = (WQ(n, 2, je, ke) + WQ(n, 1, jl, ke) + WQ(n, 1, je, kl)) * 2.0 / 3.0 - WQ(n, 2, jl, kl);
+    if (c == 7) WQ(n, ie, je, ke) = (WQ(n, il, je, ke) + WQ(n, ie, jl, ke) + WQ(n, ie, je, kl)) * 2.0 / 3.0 - WQ(n, il, jl, kl);
   }
   __syncthreads();
-  for (int j = 2 + threadIdx.x; j <= jl; j += blockDim.x) {
-    for (int n = 0; n < 5; n++) WQ(n, 1, j, 1) = WQ(n, 2, j, 1) + WQ(n, 1, j, 2) - WQ(n, 2, j, 2);
-    corner_eos(d, s, gamma, ix(d, 1, j, 1));
-    for (int n = 0; n < 5; n++) WQ(n, ie, j, 1) = WQ(n, il, j, 1) + WQ(n, ie, j, 2) - WQ(n, il, j, 2);
-    corner_eos(d, s, gamma, ix(d, ie, j, 1));
-    for (int n = 0; n < 5; n++) WQ(n, 1, j, ke) = WQ(n, 2, j, ke) + WQ(n, 1, j, kl) - WQ(n, 2, j, kl);
-    corner_eos(d, s, gamma, ix(d, 1, j, ke));
-    for (int n = 0; n < 5; n++) WQ(n, ie, j, ke) = WQ(n, il, j, ke) + WQ(n, ie, j, kl) - WQ(n, il, j, kl);
-    corner_eos(d, s, gamma, ix(d, ie, j, ke));
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i <= ib; i += blockDim.x) {
-    for (int n = 0; n < 5; n++) WQ(n, i, 1, 1) = WQ(n, i, 1, 2) + WQ(n, i, 2, 1) - WQ(n, i, 2, 2);
-    corner_eos(d, s, gamma, ix(d, i, 1, 1));
-    for (int n = 0; n < 5; n++) WQ(n, i, 1, ke) = WQ(n, i, 2, ke) + WQ(n, i, 1, kl) - WQ(n, i, 2, kl);
-    corner_eos(d, s, gamma, ix(d, i, 1, ke));
-    for (int n = 0; n < 5; n++) WQ(n, i, je, 1) = WQ(n, i, je, 1) + WQ(n, i, jl, 2) - WQ(n, i, jl, 2);
-    corner_eos(d, s, gamma, ix(d, i, je, 1));
-    for (int n = 0; n < 5; n++) WQ(n, i, je, ke) = WQ(n, i, jl, ke) + WQ(n, i, je, kl) - WQ(n, i, jl, kl);
-    corner_eos(d, s, gamma, ix(d, i, je, ke));
-  }
-  __syncthreads();
-  if (threadIdx.x < 5) {
-    int n = threadIdx.x;
-    WQ(n, 1, 1, 1) = (WQ(n, 2, 1, 1) + WQ(n, 1, 2, 1) + WQ(n, 1, 1, 2)) * 2.0 / 3.0 - WQ(n, 2, 2, 2);
-    WQ(n, 1, 1, ke) = (WQ(n, 2, 1, ke) + WQ(n, 1, 2, ke) + WQ(n, 1, 1, kl)) * 2.0 / 3.0 - WQ(n, 2, 2, kl);
-    WQ(n, 1, je, 1) = (WQ(n, 2, je, 1) + WQ(n, 1, jl, 1) + WQ(n, 1, je, 2)) * 2.0 / 3.0 - WQ(n, 2, jl, 2);
-    WQ(n, ie, 1, 1) = (WQ(n, il, 1, 1) + WQ(n, ie, 2, 1) + WQ(n, ie, 1, 2)) * 2.0 / 3.0 - WQ(n, il, 2, 2);
-    WQ(n, ie, 1, ke) = (WQ(n, il, 1, ke) + WQ(n, ie, 2, ke) + WQ(n, ie, 1, kl)) * 2.0 / 3.0 - WQ(n, il, 2, kl);
-    WQ(n, ie, je, 1) = (WQ(n, il, je, 1) + WQ(n, ie, jl, 1) + WQ(n, ie, je, 2)) * 2.0 / 3.0 - WQ(n, il, jl, 2);
-    WQ(n, 1, je, ke) = (WQ(n, 2, je, ke) + WQ(n, 1, jl, ke) + WQ(n, 1, je, kl)) * 2.0 / 3.0 - WQ(n, 2, jl, kl);
-    WQ(n, ie, je, ke) = (WQ(n, il, je, ke) + WQ(n, ie, jl, ke) + WQ(n, ie, je, kl)) * 2.0 / 3.0 - WQ(n, il, jl, kl);
-  }
-  __syncthreads();
-  if (threadIdx.x < 8) {
-    int c = threadIdx.x;
+#undef WQ
+  if (tid < 8) {
+    int c = tid;
     int ci = (c & 1) ? ie : 1, cj = (c & 2) ? je : 1, ck = (c & 4) ? ke : 1;
     corner_eos(d, s, gamma, ix(d, ci, cj, ck));
   }
-#undef WQ
 }
 
 // ------------------------------------------------------------------ misc kernels
@@ -1136,26 +1349,32 @@ __global__ void __launch_bounds__(256) k_prolong(XferArgs X) {
   }
 }
 
-// wall force: momentum flux of the scheme's wall faces, in record order
-__global__ void k_wall_force(Dims d, StateC s, Geo g, const DevRec *recs, const double *wsol,
-                             const double *psol, int nmir, double *out) {
+// wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253);
+// one thread per record writes its term, one thread adds them in record order
+__global__ void k_wall_terms(Dims d, StateC s, Geo g, const DevRec *recs, const double *wsol, const double *psol,
+                             int nmir, double *terms) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmir) return;
+  const DevRec &r = recs[m];
+  int zz = r.zz;
+  double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+  sc[zz - 1] = face_s_any(g, zz, r.c[0], r.c[1], r.c[2]);
+  ss[zz - 1] = face_s_any(g, zz, r.s[0], r.s[1], r.s[2]);
+  long qc = ix(d, r.c[0], r.c[1], r.c[2]);
+  double wc[5], ws[5], f1[5], f2[5];
+  load5(s.w, d.E, qc, wc);
+  for (int n = 0; n < 5; n++) ws[n] = wsol[n * nmir + m];
+  cflux3(wc, s.p[qc], sc, f1);
+  cflux3(ws, psol[m], ss, f2);
+  double sgn = (r.faceid >= 4) ? -1.0 : 1.0;
+  for (int dd = 0; dd < 3; dd++) terms[3 * m + dd] = sgn * (0.5 * (f1[1 + dd] + f2[1 + dd]));
+}
+
+__global__ void k_wall_sum(const double *terms, int nmir, double *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double F[3] = {0.0, 0.0, 0.0};
-  for (int m = 0; m < nmir; m++) {
-    const DevRec &r = recs[m];
-    int zz = r.zz;
-    double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
-    sc[zz - 1] = face_s_any(g, zz, r.c[0], r.c[1], r.c[2]);
-    ss[zz - 1] = face_s_any(g, zz, r.s[0], r.s[1], r.s[2]);
-    long qc = ix(d, r.c[0], r.c[1], r.c[2]);
-    double wc[5], ws[5], f1[5], f2[5];
-    load5(s.w, d.E, qc, wc);
-    for (int n = 0; n < 5; n++) ws[n] = wsol[n * nmir + m];
-    cflux3(wc, s.p[qc], sc, f1);
-    cflux3(ws, psol[m], ss, f2);
-    double sgn = (r.faceid >= 4) ? -1.0 : 1.0;
-    for (int dd = 0; dd < 3; dd++) F[dd] = F[dd] + sgn * (0.5 * (f1[1 + dd] + f2[1 + dd]));
-  }
+  for (int m = 0; m < nmir; m++)
+    for (int dd = 0; dd < 3; dd++) F[dd] = F[dd] + terms[3 * m + dd];
   for (int dd = 0; dd < 3; dd++) out[dd] = F[dd];
 }
 
@@ -1176,15 +1395,17 @@ struct Level {
   uint8_t *flags = nullptr, *marker = nullptr;
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;
   double *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z each (nmax+2)
+  double *fax = nullptr;  // -bb/e1x
   int npiv = 0;
   int nmir = 0, nupd = 0, zero_at = -1;
   DevRec *recs = nullptr;
   double *wsol = nullptr, *psol = nullptr;
   std::vector<mge_cutrec> hrecs;
-  int ntouch = 0;
+  int ntouch = 0, nops = 0;
   long *tcell = nullptr;
-  int *fw_beg = nullptr, *dw_beg = nullptr;
-  int2 *fw_ops = nullptr, *dw_ops = nullptr;
+  int *beg = nullptr;
+  int2 *ops = nullptr;
+  double *opval = nullptr;
   double *partial = nullptr;
   long npartial = 0;
   unsigned long long *dtim_bits = nullptr;
@@ -1348,7 +1569,7 @@ int do_bcond(mge_case *h, Level &L, int buf) {
     dim3 g((amax + 1 + 255) / 256, cmax + 1);
     LAUNCH(h, "k_bc_face", k_bc_face, g, dim3(256), B);
   }
-  LAUNCH(h, "k_bc_corner", k_bc_corner, dim3(1), dim3(256), L.d, stm(L, buf), P.gamma);
+  LAUNCH(h, "k_bc_corner", k_bc_corner, dim3(1), dim3(1024), L.d, stm(L, buf), P.gamma);
   return warn;
 }
 
@@ -1463,13 +1684,14 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       R.psol = L.psol;
       R.nmir = L.nmir;
       R.tcell = L.tcell;
-      R.fw_beg = L.fw_beg;
-      R.dw_beg = L.dw_beg;
-      R.fw_ops = L.fw_ops;
-      R.dw_ops = L.dw_ops;
+      R.beg = L.beg;
+      R.ops = L.ops;
+      R.opval = L.opval;
       R.ntouch = L.ntouch;
+      R.nops = L.nops;
       R.err = h->err;
-      LAUNCH(h, "k_records", k_records, dim3((L.ntouch + 63) / 64), dim3(64), R);
+      LAUNCH(h, "k_rec_ops", k_rec_ops, dim3((L.nops + 63) / 64), dim3(64), R);
+      LAUNCH(h, "k_rec_apply", k_rec_apply, dim3((L.ntouch + 63) / 64), dim3(64), R);
     }
     if (lres) return MGE_OK;
     if (A.smoothed) {
@@ -1484,17 +1706,26 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       S.gamma = P.gamma;
       S.e1 = L.piv;
       S.e2 = L.piv + L.npiv;
-      long nlx = (long)d.nj * d.nk * 5;
-      LAUNCH(h, "k_smooth_x", k_smooth_x, dim3((unsigned)((nlx + 255) / 256)), dim3(256), S);
+      S.fa = L.fax;
+      long nwave = (long)d.nj * d.nk * 5;
+      dim3 gx((unsigned)((nwave * 64 + 255) / 256));
+      if (d.ni <= 64) LAUNCH(h, "k_smooth_x", k_smooth_x_scan<1>, gx, dim3(256), S);
+      else if (d.ni <= 128) LAUNCH(h, "k_smooth_x", k_smooth_x_scan<2>, gx, dim3(256), S);
+      else if (d.ni <= 256) LAUNCH(h, "k_smooth_x", k_smooth_x_scan<4>, gx, dim3(256), S);
+      else if (d.ni <= 512) LAUNCH(h, "k_smooth_x", k_smooth_x_scan<8>, gx, dim3(256), S);
+      else LAUNCH(h, "k_smooth_x", k_smooth_x_seq, dim3((unsigned)((nwave + 255) / 256)), dim3(256), S);
       S.e1 = L.piv + 2 * L.npiv;
       S.e2 = L.piv + 3 * L.npiv;
       LAUNCH(h, "k_smooth_y", k_smooth_y, dim3((d.ni + 63) / 64, d.nk, 5), dim3(64), S);
       S.e1 = L.piv + 4 * L.npiv;
       S.e2 = L.piv + 5 * L.npiv;
-      LAUNCH(h, "k_smooth_z", k_smooth_z, dim3((d.ni + 63) / 64, d.nj), dim3(64), S);
+      LAUNCH(h, "k_smooth_z", k_smooth_z, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
+      LAUNCH(h, "k_update_ghost", k_update_ghost, dim3((unsigned)((E + 255) / 256)), dim3(256), d,
+             (const double *)L.S[w0], (const double *)L.dw, stc(L, src), stm(L, dst), P.gamma);
+    } else {
+      LAUNCH(h, "k_ghost_copy", k_ghost_copy, dim3((unsigned)((E + 255) / 256)), dim3(256), d, stc(L, src),
+             stm(L, dst));
     }
-    LAUNCH(h, "k_ghost_copy", k_ghost_copy, dim3((unsigned)((E + 255) / 256)), dim3(256), d, stc(L, src),
-           stm(L, dst));
     int rc = do_bc_cutcell(h, L, dst);
     if (rc > 0) warn = rc;
     rc = do_bcond(h, L, dst);
@@ -1757,30 +1988,27 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
     pc.second.push_back(make_int2(m, OP_E_ADD));
   }
   std::vector<long> tcell;
-  std::vector<int> fwb(1, 0), dwb(1, 0);
-  std::vector<int2> fwo, dwo;
+  std::vector<int> begv(1, 0);
+  std::vector<int2> opv;
   for (auto &kv : per) {
     tcell.push_back(kv.first);
-    for (auto &o : kv.second.first) fwo.push_back(o);
-    for (auto &o : kv.second.second) dwo.push_back(o);
-    fwb.push_back((int)fwo.size());
-    dwb.push_back((int)dwo.size());
+    for (auto &o : kv.second.first) opv.push_back(o);
+    for (auto &o : kv.second.second) opv.push_back(o);
+    begv.push_back((int)opv.size());
   }
   L.ntouch = (int)tcell.size();
+  L.nops = (int)opv.size();
   CHKRC(dalloc(L, &L.recs, nmir));
   CHKRC(dalloc(L, &L.wsol, (size_t)5 * nmir));
   CHKRC(dalloc(L, &L.psol, nmir));
   CHKRC(dalloc(L, &L.tcell, tcell.size()));
-  CHKRC(dalloc(L, &L.fw_beg, fwb.size()));
-  CHKRC(dalloc(L, &L.dw_beg, dwb.size()));
-  CHKRC(dalloc(L, &L.fw_ops, fwo.size() + 1));
-  CHKRC(dalloc(L, &L.dw_ops, dwo.size() + 1));
+  CHKRC(dalloc(L, &L.beg, begv.size()));
+  CHKRC(dalloc(L, &L.ops, opv.size() + 1));
+  CHKRC(dalloc(L, &L.opval, (opv.size() + 1) * 10));
   HIPCHK(hipMemcpy(L.recs, dr.data(), sizeof(DevRec) * nmir, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(L.tcell, tcell.data(), sizeof(long) * tcell.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(L.fw_beg, fwb.data(), sizeof(int) * fwb.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(L.dw_beg, dwb.data(), sizeof(int) * dwb.size(), hipMemcpyHostToDevice));
-  if (!fwo.empty()) HIPCHK(hipMemcpy(L.fw_ops, fwo.data(), sizeof(int2) * fwo.size(), hipMemcpyHostToDevice));
-  if (!dwo.empty()) HIPCHK(hipMemcpy(L.dw_ops, dwo.data(), sizeof(int2) * dwo.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L.beg, begv.data(), sizeof(int) * begv.size(), hipMemcpyHostToDevice));
+  if (!opv.empty()) HIPCHK(hipMemcpy(L.ops, opv.data(), sizeof(int2) * opv.size(), hipMemcpyHostToDevice));
   std::vector<uint8_t> fl(d.E);
   HIPCHK(hipMemcpy(fl.data(), L.flags, d.E, hipMemcpyDeviceToHost));
   for (long q : tcell) fl[q] |= F_TOUCHED;
@@ -1919,6 +2147,11 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
     }
   }
   HIPCHK(hipMemcpy(L.piv, piv.data(), sizeof(double) * piv.size(), hipMemcpyHostToDevice));
+  std::vector<double> fa(nmax, 0.0);
+  if (h->prm.smoo > 0.0)
+    for (int i = 2; i <= d.il && i < nmax; i++) fa[i] = h->prm.smoo / piv[i];  // -bb/e1, bb = -smoo
+  CHKRC(dalloc(L, &L.fax, nmax));
+  HIPCHK(hipMemcpy(L.fax, fa.data(), sizeof(double) * nmax, hipMemcpyHostToDevice));
   // flags: fluid bit with the ghost copies of init_metric.f90:304-315
   std::vector<uint8_t> bl(E, 1);
   auto X = [&](int i, int j, int k) { return ix(d, i, j, k); };
@@ -2106,8 +2339,9 @@ int mge_wall_force(mge_case *h, int lvl, double out[3]) {
     out[0] = out[1] = out[2] = 0.0;
     return MGE_OK;
   }
-  LAUNCH(h, "k_wall_force", k_wall_force, dim3(1), dim3(64), L->d, stc(*L, L->cur), geo(*L), L->recs, L->wsol,
-         L->psol, L->nmir, L->force);
+  LAUNCH(h, "k_wall_terms", k_wall_terms, dim3((L->nmir + 63) / 64), dim3(64), L->d, stc(*L, L->cur), geo(*L),
+         L->recs, L->wsol, L->psol, L->nmir, L->opval);
+  LAUNCH(h, "k_wall_sum", k_wall_sum, dim3(1), dim3(64), (const double *)L->opval, L->nmir, L->force);
   HIPCHK(hipMemcpyAsync(out, L->force, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
   return check_err(h);
 }
