@@ -237,6 +237,265 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs A) {
   finish_cell(A, q, i, j, k, f, e);
 }
 
+// ------------------------------------------------------------------ k_stage2
+//
+// LDS-tiled form of k_stage.  A workgroup owns a TI x TJ column tile of
+// (i,j) and marches through a chunk of k planes.  Per plane the i- and
+// j-direction quantities every face needs -- the convective cell fluxes
+// F (ceflux.f90), the pressure switches fp and the enthalpy differences diw
+// (dflux.f90:70-86) -- are computed once per cell into LDS (tile + halo);
+// each thread then forms its faces from LDS.  The k-direction face values
+// are carried from plane to plane in registers, so every k face is
+// evaluated once.  Each face value is computed with the same expression
+// tree as the reference, so results equal k_stage's bit for bit.
+constexpr int TI = 64, TJ = 4;
+
+struct StageSmem {
+  double fx[5][TJ][TI + 2];   // F_x at i = i0-1 .. i0+TI
+  double fy[5][TJ + 2][TI];   // F_y at j = j0-1 .. j0+TJ
+  double px[TJ][TI + 4];      // fp_x at i = i0-2 .. i0+TI+1
+  double py[TJ + 4][TI];      // fp_y at j = j0-2 .. j0+TJ+1
+  double dx[5][TJ][TI + 3];   // diw_x at i = i0-1 .. i0+TI+1
+  double dy[5][TJ + 3][TI];   // diw_y at j = j0-1 .. j0+TJ+1
+};
+
+// JST face coefficients from the switches fp(a-2..a+1) and the radii sums
+// on both sides (dflux.f90:88-96); the face value of component n is then
+// (dis2*diw(a) - dis4*(diw(a+1) - 2 diw(a) + diw(a-1))) * lam
+__device__ inline void face_coef(const FluxCtx &c, double f0, double f1, double f2, double f3, double lm, double lp,
+                                 double *dis2o, double *dis4o, double *lamo) {
+  double ep = maxd(maxd(maxd(f0, f1), f2), f3);
+  double lam = 0.5 * (lm + lp);
+  double dis2 = c.vis2 * ep;
+  double dis4 = c.vis4;
+  dis4 = dimd(dis4, dis2);
+  dis2 = dis2 * c.cdis;
+  dis4 = dis4 * c.cdis;
+  *dis2o = dis2;
+  *dis4o = dis4;
+  *lamo = lam;
+}
+
+template <int DIR>
+__device__ inline void cell_flux(const FluxCtx &c, int i, int j, int k, double *o) {
+  long q = ix(c.d, i, j, k);
+  double v[5];
+  load5(c.s.w, c.d.E, q, v);
+  cflux<DIR>(v, c.s.p[q], face_s<DIR>(c.g, i, j, k), o);
+}
+
+__global__ void __launch_bounds__(256, 2) k_stage2(StageArgs A, int kc) {
+  __shared__ StageSmem sm;
+  const FluxCtx &c = A.fx;
+  const Dims &d = c.d;
+  const long E = d.E;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
+  const int i0 = 2 + blockIdx.x * TI, j0 = 2 + blockIdx.y * TJ;
+  const int kb0 = 2 + blockIdx.z * kc;
+  const int kb1 = min(kb0 + kc - 1, d.kl);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = i <= d.il && j <= d.jl;
+  const bool diss = A.has_diss;
+  const long bz = ix(d, i, j, 0);
+  // k-direction carry (face k) and rings for face k+1
+  double Dk[5], Ek[5], fz[4], dzr[3][5], rz[2], Fz[2][5];
+  if (valid) {
+    int k = kb0;
+    eface<2>(c, bz, k, i, j, k, Ek);
+    cell_flux<2>(c, i, j, k, Fz[0]);
+    cell_flux<2>(c, i, j, k + 1, Fz[1]);
+    if (diss) {
+      dface<2>(c, bz, k, i, j, k, Dk);
+      for (int t = 0; t < 4; t++) fz[t] = fp_at<2>(c, bz, k - 1 + t);
+      for (int t = 0; t < 3; t++) diw_at<2>(c, bz, k + t, dzr[t]);
+      rz[0] = rad_at<2>(c, bz, k);
+      rz[1] = rad_at<2>(c, bz, k + 1);
+    }
+  }
+  for (int k = kb0; k <= kb1; k++) {
+    // ---- fill the plane's i/j quantities (cells within the arrays)
+    {
+      // own cell
+      if (i <= d.ib && j <= d.jb) {
+        long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
+        double o[5];
+        cell_flux<0>(c, i, j, k, o);
+        for (int n = 0; n < 5; n++) sm.fx[n][ty][tx + 1] = o[n];
+        cell_flux<1>(c, i, j, k, o);
+        for (int n = 0; n < 5; n++) sm.fy[n][ty + 1][tx] = o[n];
+        if (diss) {
+          sm.px[ty][tx + 2] = fp_at<0>(c, brow, i);
+          sm.py[ty + 2][tx] = fp_at<1>(c, bcol, j);
+          diw_at<0>(c, brow, i, o);
+          for (int n = 0; n < 5; n++) sm.dx[n][ty][tx + 1] = o[n];
+          diw_at<1>(c, bcol, j, o);
+          for (int n = 0; n < 5; n++) sm.dy[n][ty + 1][tx] = o[n];
+        }
+      }
+      // halo items
+      const int nF = 2 * TJ + 2 * TI;
+      const int nP = diss ? 4 * TJ + 4 * TI : 0;
+      const int nD = diss ? 3 * TJ + 3 * TI : 0;
+      for (int t = tid; t < nF + nP + nD; t += TI * TJ) {
+        double o[5];
+        if (t < 2 * TJ) {  // F_x at i0-1, i0+TI
+          int r = t % TJ, side = t / TJ;
+          int ii = side ? i0 + TI : i0 - 1, jj = j0 + r, col = side ? TI + 1 : 0;
+          if (ii <= d.ib && jj <= d.jb) {
+            cell_flux<0>(c, ii, jj, k, o);
+            for (int n = 0; n < 5; n++) sm.fx[n][r][col] = o[n];
+          }
+        } else if (t < nF) {  // F_y at j0-1, j0+TJ
+          int u = t - 2 * TJ;
+          int col = u % TI, side = u / TI;
+          int ii = i0 + col, jj = side ? j0 + TJ : j0 - 1, row = side ? TJ + 1 : 0;
+          if (ii <= d.ib && jj <= d.jb) {
+            cell_flux<1>(c, ii, jj, k, o);
+            for (int n = 0; n < 5; n++) sm.fy[n][row][col] = o[n];
+          }
+        } else if (t < nF + 4 * TJ) {  // fp_x at i0-2, i0-1, i0+TI, i0+TI+1
+          int u = t - nF;
+          int r = u % TJ, w4 = u / TJ;
+          int off = w4 < 2 ? w4 - 2 : TI + (w4 - 2);
+          int ii = i0 + off, jj = j0 + r, col = off + 2;
+          if (jj <= d.jb) sm.px[r][col] = (ii >= 0 && ii <= d.ib) ? fp_at<0>(c, ix(d, 0, jj, k), ii) : 0.0;
+        } else if (t < nF + nP) {  // fp_y at j0-2, j0-1, j0+TJ, j0+TJ+1
+          int u = t - nF - 4 * TJ;
+          int col = u % TI, w4 = u / TI;
+          int off = w4 < 2 ? w4 - 2 : TJ + (w4 - 2);
+          int ii = i0 + col, jj = j0 + off, row = off + 2;
+          if (ii <= d.ib) sm.py[row][col] = (jj >= 0 && jj <= d.jb) ? fp_at<1>(c, ix(d, ii, 0, k), jj) : 0.0;
+        } else if (t < nF + nP + 3 * TJ) {  // diw_x at i0-1, i0+TI, i0+TI+1
+          int u = t - nF - nP;
+          int r = u % TJ, w3 = u / TJ;
+          int off = w3 == 0 ? -1 : TI + (w3 - 1);
+          int ii = i0 + off, jj = j0 + r, col = off + 1;
+          if (jj <= d.jb && ii <= d.ib) {
+            diw_at<0>(c, ix(d, 0, jj, k), ii, o);
+            for (int n = 0; n < 5; n++) sm.dx[n][r][col] = o[n];
+          }
+        } else {  // diw_y at j0-1, j0+TJ, j0+TJ+1
+          int u = t - nF - nP - 3 * TJ;
+          int col = u % TI, w3 = u / TI;
+          int off = w3 == 0 ? -1 : TJ + (w3 - 1);
+          int ii = i0 + col, jj = j0 + off, row = off + 1;
+          if (ii <= d.ib && jj <= d.jb) {
+            diw_at<1>(c, ix(d, ii, 0, k), jj, o);
+            for (int n = 0; n < 5; n++) sm.dy[n][row][col] = o[n];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      const long q = ix(d, i, j, k);
+      double f[5];
+      if (A.m > 1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = 0.0;
+      }
+      double Dk1[5], Ek1[5];
+      if (diss) {
+        // face coefficients (dflux.f90:88-96), then per component the two
+        // faces of each direction straight from LDS
+        double da2, da4, dal, db2, db4, dbl;
+        const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
+        {
+          double r_m = rad_at<0>(c, brow, i - 1), r_0 = rad_at<0>(c, brow, i), r_p = rad_at<0>(c, brow, i + 1);
+          face_coef(c, sm.px[ty][tx], sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], r_m, r_0, &da2, &da4,
+                    &dal);
+          face_coef(c, sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.px[ty][tx + 4], r_0, r_p, &db2,
+                    &db4, &dbl);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            double d0 = sm.dx[n][ty][tx], d1 = sm.dx[n][ty][tx + 1], d2 = sm.dx[n][ty][tx + 2],
+                   d3 = sm.dx[n][ty][tx + 3];
+            double fa = (da2 * d1 - da4 * (d2 - 2.0 * d1 + d0)) * dal;
+            double fb = (db2 * d2 - db4 * (d3 - 2.0 * d2 + d1)) * dbl;
+            f[n] = f[n] + (fa - fb);
+          }
+        }
+        {
+          double r_m = rad_at<1>(c, bcol, j - 1), r_0 = rad_at<1>(c, bcol, j), r_p = rad_at<1>(c, bcol, j + 1);
+          face_coef(c, sm.py[ty][tx], sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], r_m, r_0, &da2, &da4,
+                    &dal);
+          face_coef(c, sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.py[ty + 4][tx], r_0, r_p, &db2,
+                    &db4, &dbl);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            double d0 = sm.dy[n][ty][tx], d1 = sm.dy[n][ty + 1][tx], d2 = sm.dy[n][ty + 2][tx],
+                   d3 = sm.dy[n][ty + 3][tx];
+            double fa = (da2 * d1 - da4 * (d2 - 2.0 * d1 + d0)) * dal;
+            double fb = (db2 * d2 - db4 * (d3 - 2.0 * d2 + d1)) * dbl;
+            f[n] = f[n] + (fa - fb);
+          }
+        }
+        face_coef(c, fz[0], fz[1], fz[2], fz[3], rz[0], rz[1], &da2, &da4, &dal);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          Dk1[n] = (da2 * dzr[1][n] - da4 * (dzr[2][n] - 2.0 * dzr[1][n] + dzr[0][n])) * dal;
+          f[n] = f[n] + (Dk[n] - Dk1[n]);
+        }
+      }
+      double e[5];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        double ea = 0.5 * (sm.fx[n][ty][tx + 1] + sm.fx[n][ty][tx]);
+        double eb = 0.5 * (sm.fx[n][ty][tx + 2] + sm.fx[n][ty][tx + 1]);
+        e[n] = ea - eb;
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        double ea = 0.5 * (sm.fy[n][ty + 1][tx] + sm.fy[n][ty][tx]);
+        double eb = 0.5 * (sm.fy[n][ty + 2][tx] + sm.fy[n][ty + 1][tx]);
+        e[n] = e[n] + ea - eb;
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        Ek1[n] = 0.5 * (Fz[1][n] + Fz[0][n]);
+        e[n] = e[n] + Ek[n] - Ek1[n];
+      }
+      if (A.flags[q] & F_TOUCHED) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          A.fw[n * E + q] = f[n];
+          A.dw[n * E + q] = e[n];
+        }
+      } else {
+        finish_cell(A, q, i, j, k, f, e);
+      }
+      // advance the k-direction carry to face k+2
+      if (k < kb1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          Ek[n] = Ek1[n];
+          Fz[0][n] = Fz[1][n];
+        }
+        cell_flux<2>(c, i, j, k + 2, Fz[1]);
+        if (diss) {
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            Dk[n] = Dk1[n];
+            dzr[0][n] = dzr[1][n];
+            dzr[1][n] = dzr[2][n];
+          }
+          fz[0] = fz[1];
+          fz[1] = fz[2];
+          fz[2] = fz[3];
+          fz[3] = fp_at<2>(c, bz, k + 3);
+          diw_at<2>(c, bz, k + 3, dzr[2]);
+          rz[0] = rz[1];
+          rz[1] = rad_at<2>(c, bz, k + 2);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ wall records
 
 template <int DIR>
@@ -684,82 +943,91 @@ __global__ void __launch_bounds__(256) k_smooth_x_seq(SmoothArgs S) {
 }
 
 // One line of the exact sequential recurrence of psmoo_old
-// (psmoo.f90:190-237): cells s = 2..nl at stride st.  The line is read in
-// register chunks of CH cells, double-buffered, so one chunk's loads are in
-// flight while the previous chunk runs its dependent divisions (a single
-// prefetched cell cannot cover HBM latency).  Chunk buffers are named
-// arrays with compile-time indices so they stay in registers.
-#define SWEEP_LOAD(BUF, FB, c, s_of)                                 \
-  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
-    int s_ = s_of(c, t);                                             \
-    if (s_ >= 2 && s_ <= nl) {                                       \
-      BUF[t] = D[b0 + (long)s_ * st];                                \
-      FB[t] = F[b0 + (long)s_ * st];                                 \
-    }                                                                \
-  }
-#define FWD_S(c, t) (2 + (c) * CH + (t))
-#define BWD_S(c, t) (lm - (c) * CH - (t))
-#define FWD_PROC(BUF, FB, c)                                         \
-  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
-    int s_ = FWD_S(c, t);                                            \
-    if (s_ <= nl) {                                                  \
-      double r_ = BUF[t];                                            \
-      if (FB[t] & F_FLUID) {                                         \
-        y = (s_ == 2) ? r_ / e1[2] : (r_ - bb * y) / e1[s_];         \
-        D[b0 + (long)s_ * st] = y;                                   \
-      } else {                                                       \
-        y = r_;                                                      \
-      }                                                              \
-    }                                                                \
-  }
-#define BWD_PROC(BUF, FB, c)                                         \
-  _Pragma("unroll") for (int t = 0; t < CH; t++) {                   \
-    int s_ = BWD_S(c, t);                                            \
-    if (s_ >= 2) {                                                   \
-      double r_ = BUF[t];                                            \
-      if (FB[t] & F_FLUID) {                                         \
-        y = r_ - e2[s_] * y;                                         \
-        D[b0 + (long)s_ * st] = y;                                   \
-      } else {                                                       \
-        y = r_;                                                      \
-      }                                                              \
-    }                                                                \
-  }
-
+// (psmoo.f90:190-237): cells s = 2..nl at stride st.  Full chunks of CH
+// cells are read into registers double-buffered with unconditional loads,
+// so the compiler can keep the next chunk's loads in flight (counted waits)
+// while the current chunk runs its dependent divisions.  The updates are
+// branch-free: blank cells hold dw = +-0 (blanked before smoothing), so
+// y = m*((r - bb*y)/e1) with m = 0 on blank cells gives the reference's
+// values on fluid cells and keeps the zeros.  The backward sweep starts
+// at s = nl with z(nl+1) = 0 and e2(nl) = 0, which returns y(nl) there.
 template <int CH>
-__device__ __attribute__((always_inline)) inline void line_sweep(double *D, const uint8_t *F, long b0, long st,
-                                                                 int nl, int lm,
-                                  const double *e1, const double *e2, double bb) {
-  double A[CH], B[CH];
-  uint8_t FA[CH], FBb[CH];
+__device__ __attribute__((always_inline)) inline void line_sweep(double *__restrict__ D,
+                                                                 const uint8_t *__restrict__ F, long b0, long st,
+                                                                 int nl, const double *__restrict__ e1,
+                                                                 const double *__restrict__ e2, double bb) {
+  const int ncell = nl - 1;
+  const int nfull = ncell / CH;
+  double A[CH], B[CH], MA[CH], MB[CH];
   double y = 0.0;
-  // forward, s = 2..nl
-  const int nf = (nl - 1 + CH - 1) / CH;
-  SWEEP_LOAD(A, FA, 0, FWD_S);
-  for (int c = 0; c < nf; c += 2) {
-    if (c + 1 < nf) { SWEEP_LOAD(B, FBb, c + 1, FWD_S); }
-    FWD_PROC(A, FA, c);
-    if (c + 1 >= nf) break;
-    if (c + 2 < nf) { SWEEP_LOAD(A, FA, c + 2, FWD_S); }
-    FWD_PROC(B, FBb, c + 1);
+#define LD_F(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    long q_ = b0 + (long)(2 + (c) * CH + t) * st;                     \
+    BUF[t] = D[q_];                                                   \
+    MK[t] = (F[q_] & F_FLUID) ? 1.0 : 0.0;                            \
   }
-  // backward, s = lm..2 with z(nl) = y(nl)
-  const int nb = (lm - 1 + CH - 1) / CH;
-  if (nb <= 0) return;
-  SWEEP_LOAD(A, FA, 0, BWD_S);
-  for (int c = 0; c < nb; c += 2) {
-    if (c + 1 < nb) { SWEEP_LOAD(B, FBb, c + 1, BWD_S); }
-    BWD_PROC(A, FA, c);
-    if (c + 1 >= nb) break;
-    if (c + 2 < nb) { SWEEP_LOAD(A, FA, c + 2, BWD_S); }
-    BWD_PROC(B, FBb, c + 1);
+#define PR_F(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    int s_ = 2 + (c) * CH + t;                                        \
+    y = MK[t] * ((BUF[t] - bb * y) / e1[s_]);                         \
+    D[b0 + (long)s_ * st] = y;                                        \
   }
+#define LD_B(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    long q_ = b0 + (long)(nl - (c) * CH - t) * st;                    \
+    BUF[t] = D[q_];                                                   \
+    MK[t] = (F[q_] & F_FLUID) ? 1.0 : 0.0;                            \
+  }
+#define PR_B(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    int s_ = nl - (c) * CH - t;                                       \
+    y = BUF[t] - MK[t] * (e2[s_] * y);                                \
+    D[b0 + (long)s_ * st] = y;                                        \
+  }
+  // forward
+  if (nfull > 0) {
+    LD_F(A, MA, 0);
+    for (int c = 0; c < nfull; c += 2) {
+      int c1 = c + 1 < nfull ? c + 1 : c;
+      LD_F(B, MB, c1);
+      PR_F(A, MA, c);
+      if (c + 1 >= nfull) break;
+      int c2 = c + 2 < nfull ? c + 2 : c + 1;
+      LD_F(A, MA, c2);
+      PR_F(B, MB, c + 1);
+    }
+  }
+  for (int s_ = 2 + nfull * CH; s_ <= nl; s_++) {
+    long q_ = b0 + (long)s_ * st;
+    double m_ = (F[q_] & F_FLUID) ? 1.0 : 0.0;
+    y = m_ * ((D[q_] - bb * y) / e1[s_]);
+    D[q_] = y;
+  }
+  // backward
+  y = 0.0;
+  if (nfull > 0) {
+    LD_B(A, MA, 0);
+    for (int c = 0; c < nfull; c += 2) {
+      int c1 = c + 1 < nfull ? c + 1 : c;
+      LD_B(B, MB, c1);
+      PR_B(A, MA, c);
+      if (c + 1 >= nfull) break;
+      int c2 = c + 2 < nfull ? c + 2 : c + 1;
+      LD_B(A, MA, c2);
+      PR_B(B, MB, c + 1);
+    }
+  }
+  for (int s_ = nl - nfull * CH; s_ >= 2; s_--) {
+    long q_ = b0 + (long)s_ * st;
+    double m_ = (F[q_] & F_FLUID) ? 1.0 : 0.0;
+    y = D[q_] - m_ * (e2[s_] * y);
+    D[q_] = y;
+  }
+#undef LD_F
+#undef PR_F
+#undef LD_B
+#undef PR_B
 }
-#undef SWEEP_LOAD
-#undef FWD_S
-#undef BWD_S
-#undef FWD_PROC
-#undef BWD_PROC
 
 // psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
 __global__ void __launch_bounds__(64) k_smooth_y(SmoothArgs S) {
@@ -768,7 +1036,7 @@ __global__ void __launch_bounds__(64) k_smooth_y(SmoothArgs S) {
   int k = 2 + blockIdx.y;
   int n = blockIdx.z;
   if (i > d.il) return;
-  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, d.nj, S.e1, S.e2, S.bb);
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
 }
 
 // psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
@@ -778,7 +1046,7 @@ __global__ void __launch_bounds__(64) k_smooth_z(SmoothArgs S) {
   int j = 2 + blockIdx.y;
   int n = blockIdx.z;
   if (i > d.il) return;
-  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, d.nk, S.e1, S.e2, S.bb);
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
 }
 
 // after smoothing: w = wn - dw and the EOS on interior cells
@@ -1674,7 +1942,15 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       if (A.coarse) per += A.lfir1 ? 10 * s8 : 5 * s8;
       h->next_bytes = per * (double)d.ni * d.nj * d.nk;
     }
-    LAUNCH(h, "k_stage", k_stage, cell_grid(d), cell_block(), A);
+    {
+      int kc = d.nk;
+      long tiles = (long)((d.ni + TI - 1) / TI) * ((d.nj + TJ - 1) / TJ);
+      // enough workgroups to fill 256 CUs: split k into chunks
+      while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
+      dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
+      if (P.dissip == 0) LAUNCH(h, "k_stage", k_stage2, g, dim3(TI, TJ), A, kc);
+      else LAUNCH(h, "k_stage", k_stage, cell_grid(d), cell_block(), A);  // matrix dissipation
+    }
     if (L.ntouch > 0) {
       RecArgs R;
       R.st = A;
