@@ -1,0 +1,604 @@
+// mge_bc.h -- local time step, residual norm, boundary conditions, level
+// transfers and the wall force, templated on the bulk type T.  The
+// residual norm, the global time step, the cut-cell mirror state and the
+// force integral are evaluated in FP64 whatever T is.
+#pragma once
+#include "mge_stage.h"
+
+namespace mge {
+
+// ------------------------------------------------------------------ timestep
+
+template <typename T>
+struct TsArgs {
+  Dims d;
+  StateC<T> s;
+  Geo<T> g;
+  const uint8_t *flags;
+  T *radsum, *vdtim;
+  unsigned long long *dtim_bits;
+  T gamma, cfl, cellvolume;
+  int eqntype;
+  T viscl, prlmean;
+  int *err;
+};
+
+// timestep.f90:36-115 (Euler) and :117-209 (NS): spectral radii, local
+// time step with the adaptive factor, global minimum; blank cells zeroed
+// (:212-223).  The ghost copies of the radii (:227-282) are realised by
+// index clamping where the radii are read.
+template <typename T>
+__global__ void __launch_bounds__(256) k_timestep(TsArgs<T> A) {
+  const Dims &d = A.d;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 2 + blockIdx.z;
+  const T one = T(1), two = T(2);
+  T mine = T(10);
+  if (i <= d.il && j <= d.jl && k <= d.kl) {
+    const long E = d.E;
+    long q = ix(d, i, j, k);
+    T rho = A.s.w[q];
+    T cc = A.gamma * A.s.p[q] / rho;
+    if (cc < T(0) || cc != cc) raise_err(A.err, MGE_ERR_NEG_C2);
+    cc = sqrt(maxd(cc, T(0)));
+    T uu = A.s.w[E + q] / rho;
+    T uv = A.s.w[2 * E + q] / rho;
+    T uw = A.s.w[3 * E + q] / rho;
+    T dtv = T(0);
+    if (A.eqntype == 1) {
+      T fmue = A.viscl / A.prlmean;
+      T f1 = T(4) / rho / T(3);
+      T f2 = A.gamma / rho;
+      T fac = maxd(f1, f2);
+      dtv = fac * fmue / A.cellvolume;
+    }
+    T sx = face_s<0>(A.g, i, j, k);
+    T sy = face_s<1>(A.g, i, j, k);
+    T sz = face_s<2>(A.g, i, j, k);
+    T qs, ds, cs;
+    qs = fabs(uu * sx);
+    ds = sx * sx;
+    cs = cc * sqrt(ds);
+    T ri = qs + cs, rvi = dtv * ds;
+    qs = fabs(uv * sy);
+    ds = sy * sy;
+    cs = cc * sqrt(ds);
+    T rj = qs + cs, rvj = dtv * ds;
+    qs = fabs(uw * sz);
+    ds = sz * sz;
+    cs = cc * sqrt(ds);
+    T rk = qs + cs, rvk = dtv * ds;
+    T fict;
+    if (A.eqntype == 1) fict = A.cellvolume / (ri + rj + rk + T(4) * (rvi + rvj + rvk));
+    else fict = A.cellvolume / (ri + rj + rk);
+    const T plim = T(0.001);
+    const T *P = A.s.p;
+    T pc = P[q];
+    T pip = P[q + 1], pim = P[q - 1];
+    T pjp = P[q + d.sy], pjm = P[q - d.sy];
+    T pkp = P[q + d.sz], pkm = P[q - d.sz];
+    T dpi = fabs((pip - two * pc + pim) / (pip + two * pc + pim + plim));
+    T dpj = fabs((pjp - two * pc + pjm) / (pjp + two * pc + pjm + plim));
+    T dpk = fabs((pkp - two * pc + pkm) / (pkp + two * pc + pkm + plim));
+    T cm1 = dimd(A.cfl, one);
+    T sw = two * (dpi + dpj + dpk * one);
+    T r = one / (one + (cm1 < sw ? cm1 : sw));
+    T vd = fict * r;
+    mine = vd;
+    if (A.flags[q] & F_FLUID) {
+      A.radsum[q] = ri + rj + rk;
+      A.vdtim[q] = vd;
+    } else {
+      A.radsum[q] = T(0) + T(0) + T(0);
+      A.vdtim[q] = T(0);
+    }
+  }
+  // block minimum of the time step -> one atomic per block on the FP64
+  // bit pattern (positive doubles order like their bits)
+  __shared__ T sm[256];
+  int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  sm[tid] = mine;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s && sm[tid + s] < sm[tid]) sm[tid] = sm[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double v = (double)sm[0];
+    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    atomicMin(A.dtim_bits, bits);
+  }
+}
+
+__global__ void k_dtim_decode(const unsigned long long *bits, double *dtim) {
+  dtim[0] = __longlong_as_double((long long)bits[0]);
+}
+
+// ------------------------------------------------------------------ residual norm
+
+template <typename T>
+struct RtArgs {
+  Dims d;
+  const T *dw;
+  const T *vdtim;
+  const double *dtim;
+  const uint8_t *flags;
+  double cfl;
+  int vt;
+  double *partial;
+};
+
+// euler.f90:431-447: sum of (dw_1/(cfl*dt))^2 over fluid cells in FP64;
+// per-block partials in a fixed tree order, then k_rtrms_final adds them
+// in order, so the norm is deterministic
+template <typename T>
+__global__ void __launch_bounds__(256) k_rtrms(RtArgs<T> R) {
+  const Dims &d = R.d;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 2 + blockIdx.z;
+  double v = 0.0;
+  if (i <= d.il && j <= d.jl && k <= d.kl) {
+    long q = ix(d, i, j, k);
+    double de = R.cfl * (double)R.vdtim[q];
+    if (R.vt == 1) de = R.cfl * R.dtim[0];
+    double rt = 0.0;
+    if (R.flags[q] & F_FLUID) rt = (double)R.dw[q] / de;
+    v = rt * rt;
+  }
+  __shared__ double sm[256];
+  int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  sm[tid] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) sm[tid] = sm[tid] + sm[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    R.partial[b] = sm[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rtrms_final(const double *partial, long nb, double ncell, double *out,
+                                                     int *err) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (long b = threadIdx.x; b < nb; b += 256) acc = acc + partial[b];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double r = sqrt(sm[0] / ncell);
+    out[0] = r;
+    if (r != r) raise_err(err, MGE_ERR_RES_NAN);
+  }
+}
+
+// ------------------------------------------------------------------ bc_cutcell
+
+template <typename T>
+struct CutArgs {
+  Dims d;
+  StateC<T> s;
+  const DevRec *recs;
+  double *wsol, *psol;
+  int nmir, nupd, zero_at;  // records < nupd are updated; zero_at gets zeros
+  double gamma;
+  int eqntype;
+  int *err;
+};
+
+// bc_cutcell.f90:200-428 for steady bodies (qb = 0), in FP64
+template <typename T>
+__global__ void __launch_bounds__(64) k_bc_cut(CutArgs<T> C) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == C.zero_at) {
+    for (int n = 0; n < 5; n++) C.wsol[n * C.nmir + m] = 0.0;
+    C.psol[m] = 0.0;
+    return;
+  }
+  if (m >= C.nupd) return;
+  const DevRec &r = C.recs[m];
+  const long E = C.d.E;
+  double nv[3] = {r.nvec[0], r.nvec[1], r.nvec[2]};
+  double qb[3] = {0.0, 0.0, 0.0};
+  double s = 0.0;
+  s = s + nv[0] * nv[0];
+  s = s + nv[1] * nv[1];
+  s = s + nv[2] * nv[2];
+  double nn = sqrt(s);
+  for (int dd = 0; dd < 3; dd++) nv[dd] = nv[dd] / nn;
+  double vr[5] = {0, 0, 0, 0, 0}, pr = 0.0;
+  for (int n = 0; n < 5; n++)
+    for (int c = 0; c < r.npoint; c++) vr[n] = vr[n] + r.coef[c] * (double)C.s.w[n * E + r.cloud[c]];
+  for (int c = 0; c < r.npoint; c++) pr = pr + r.coef[c] * (double)C.s.p[r.cloud[c]];
+  double psol = pr, ws[5];
+  for (int n = 0; n < 5; n++) ws[n] = vr[n];
+  if (C.eqntype == 0) {
+    double qn = 2.0 * (nv[0] * (ws[1] - ws[0] * qb[0]) + nv[1] * (ws[2] - ws[0] * qb[1]) +
+                       nv[2] * (ws[3] - ws[0] * qb[2]));
+    for (int dd = 0; dd < 3; dd++) ws[1 + dd] = ws[1 + dd] - qn * nv[dd];
+  } else {
+    for (int dd = 0; dd < 3; dd++) ws[1 + dd] = 2.0 * qb[dd] * vr[0] - vr[1 + dd];
+  }
+  double s2 = 0.0;
+  s2 = s2 + ws[1] * ws[1];
+  s2 = s2 + ws[2] * ws[2];
+  s2 = s2 + ws[3] * ws[3];
+  ws[4] = psol / (C.gamma - 1.0) + 0.5 * s2 / ws[0];
+  if (C.eqntype == 0)
+    for (int n = 0; n < 5; n++)
+      if (ws[n] != ws[n]) raise_err(C.err, MGE_ERR_WSOL_NAN);
+  for (int n = 0; n < 5; n++) C.wsol[n * C.nmir + m] = ws[n];
+  C.psol[m] = psol;
+}
+
+// ------------------------------------------------------------------ bcond
+
+template <typename T>
+struct BcArgs {
+  Dims d;
+  State<T> s;
+  int face;      // 1..6
+  int kind;      // -20 far, -50 symmetry, -60 periodic, -80 wall
+  T gamma, rho0, p0, c0, u0, v0, w0;
+};
+
+// one face of bcond.f90:16-39 -> bc_far.f90 / bc_symm.f90 / bc_periodic.f90 /
+// bc_wall.f90, over the whole (n+4)^2 face like the reference loops
+template <typename T>
+__global__ void __launch_bounds__(256) k_bc_face(BcArgs<T> B) {
+  const Dims &d = B.d;
+  int dir = (B.face - 1) % 3;
+  int amax = dir == 0 ? d.jb : d.ib;
+  int cmax = dir == 2 ? d.jb : d.kb;
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  int c2 = blockIdx.y;
+  if (a > amax || c2 > cmax) return;
+  int nl = dir == 0 ? d.il : (dir == 1 ? d.jl : d.kl);
+  int ne = nl + 1, nb = nl + 2;
+  auto L = [&](int s) -> long {
+    return dir == 0 ? ix(d, s, a, c2) : (dir == 1 ? ix(d, a, s, c2) : ix(d, a, c2, s));
+  };
+  const long E = d.E;
+  T *W = B.s.w, *P = B.s.p;
+  if (B.kind == -20) {
+    const T one = T(1), two = T(2), half = T(0.5), quarter = T(0.25);
+    int phy1, dum1, dum2;
+    if (B.face <= 3) { phy1 = 2; dum1 = 1; dum2 = 0; }
+    else { phy1 = nl; dum1 = ne; dum2 = nb; }
+    T fn[3] = {T(0), T(0), T(0)};
+    fn[dir] = B.face <= 3 ? -one : one;
+    T vinf[3] = {B.u0, B.v0, B.w0};
+    const T gamma = B.gamma, gm = one / (gamma - one);
+    long q1 = L(phy1), qd1 = L(dum1), qd2 = L(dum2);
+    T s = T(0);
+    s = s + W[E + q1] * fn[0];
+    s = s + W[2 * E + q1] * fn[1];
+    s = s + W[3 * E + q1] * fn[2];
+    T vel_norm = s / W[q1];
+    T r, vel[3], pr;
+    if (vel_norm <= T(0)) {
+      T rhoe = W[q1];
+      T ue = W[(1 + dir) * E + q1] / rhoe;
+      T pe = P[q1];
+      T ce = sqrt(gamma * pe / rhoe);
+      T rpo = ue * fn[dir] + two * ce / (gamma - one);
+      T rmi = vinf[dir] * fn[dir] - two * B.c0 / (gamma - one);
+      T un = half * (rpo + rmi) * fn[dir];
+      T ci = quarter * (gamma - one) * (rpo - rmi);
+      vel[0] = B.u0; vel[1] = B.v0; vel[2] = B.w0;
+      vel[dir] = un;
+      T si = pow(B.rho0, gamma) / B.p0;
+      T cc = ci * ci / gamma;
+      r = pow(si * cc, gm);
+      pr = r * cc;
+    } else {
+      T rhoi = W[q1];
+      T vv[3] = {W[E + q1] / rhoi, W[2 * E + q1] / rhoi, W[3 * E + q1] / rhoi};
+      T prei = P[q1];
+      T si = pow(rhoi, gamma) / prei;
+      T ci = sqrt(gamma * prei / rhoi);
+      T rpo = vv[dir] * fn[dir] + two * ci / (gamma - one);
+      T rmi = vinf[dir] * fn[dir] - two * B.c0 / (gamma - one);
+      T un = half * (rpo + rmi) * fn[dir];
+      T ce = quarter * (gamma - one) * (rpo - rmi);
+      vel[0] = vv[0]; vel[1] = vv[1]; vel[2] = vv[2];
+      vel[dir] = un;
+      T cc = ce * ce / gamma;
+      r = pow(si * cc, gm);
+      pr = r * cc;
+    }
+    T o[5];
+    o[0] = r;
+    o[1] = r * vel[0];
+    o[2] = r * vel[1];
+    o[3] = r * vel[2];
+    o[4] = pr * gm + half * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    for (int n = 0; n < 5; n++) {
+      W[n * E + qd1] = o[n];
+      W[n * E + qd2] = o[n];
+    }
+    P[qd1] = pr;
+    P[qd2] = pr;
+    return;
+  }
+  int phy1, phy2, dum1, dum2;
+  if (B.kind == -60) {
+    if (B.face <= 3) { dum1 = 1; dum2 = 0; phy1 = nl; phy2 = nl - 1; }
+    else { phy1 = 2; phy2 = 3; dum1 = ne; dum2 = nb; }
+  } else {
+    if (B.face <= 3) { phy1 = 2; phy2 = 3; dum1 = 1; dum2 = 0; }
+    else { phy1 = nl; phy2 = nl - 1; dum1 = ne; dum2 = nb; }
+  }
+  long q1 = L(phy1), q2 = L(phy2), qd1 = L(dum1), qd2 = L(dum2);
+  if (B.kind == -60) {
+    for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
+    for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
+    P[qd2] = P[q2];
+    P[qd1] = P[q1];
+  } else if (B.kind == -50) {
+    for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
+    W[(1 + dir) * E + qd1] = -W[(1 + dir) * E + q1];
+    P[qd1] = P[q1];
+    for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
+    W[(1 + dir) * E + qd2] = -W[(1 + dir) * E + q2];
+    P[qd2] = P[q2];
+  } else if (B.kind == -80) {
+    W[qd1] = W[q1];
+    W[4 * E + qd1] = W[4 * E + q1];
+    for (int n = 1; n < 4; n++) W[n * E + qd1] = -W[n * E + q1];
+    W[qd2] = W[q2];
+    W[4 * E + qd2] = W[4 * E + q2];
+    for (int n = 1; n < 4; n++) W[n * E + qd2] = -W[n * E + q2];
+    P[qd1] = P[q1];
+    P[qd2] = P[q1];
+  }
+}
+
+// the ordered edge extrapolation of bc_corner.f90 (one target cell)
+template <typename T>
+__device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, int ti, int tj, int tk, int ai, int aj, int ak,
+                                 int bi, int bj, int bk, int ci, int cj, int ck) {
+  const long E = d.E;
+  long qt = ix(d, ti, tj, tk), qa = ix(d, ai, aj, ak), qb = ix(d, bi, bj, bk), qc = ix(d, ci, cj, ck);
+  T v[5];
+  for (int n = 0; n < 5; n++) {
+    v[n] = s.w[n * E + qa] + s.w[n * E + qb] - s.w[n * E + qc];
+    s.w[n * E + qt] = v[n];
+  }
+  s.p[qt] = eos(gamma, v);
+}
+
+// bc_corner.f90:23-157: four ordered phases; within a phase every (edge,
+// index) item is independent, so one 1024-thread workgroup runs each phase
+// in parallel and synchronises between phases
+template <typename T>
+__global__ void __launch_bounds__(1024) k_bc_corner(Dims d, State<T> s, T gamma) {
+  const long E = d.E;
+  T *W = s.w;
+  const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke, ib = d.ib;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // phase 1: edges along k (bc_corner.f90:23-61)
+  for (int t = tid; t < 4 * (kl - 1); t += nt) {
+    int e = t & 3, k = 2 + (t >> 2);
+    if (e == 0) edge_fill(d, s, gamma, 1, 1, k, 2, 1, k, 1, 2, k, 2, 2, k);
+    else if (e == 1) edge_fill(d, s, gamma, ie, 1, k, il, 1, k, ie, 2, k, il, 2, k);
+    else if (e == 2) edge_fill(d, s, gamma, 1, je, k, 2, je, k, 1, jl, k, 2, jl, k);
+    else edge_fill(d, s, gamma, ie, je, k, il, je, k, ie, jl, k, il, jl, k);
+  }
+  __syncthreads();
+  // phase 2: edges along j (:63-99)
+  for (int t = tid; t < 4 * (jl - 1); t += nt) {
+    int e = t & 3, j = 2 + (t >> 2);
+    if (e == 0) edge_fill(d, s, gamma, 1, j, 1, 2, j, 1, 1, j, 2, 2, j, 2);
+    else if (e == 1) edge_fill(d, s, gamma, ie, j, 1, il, j, 1, ie, j, 2, il, j, 2);
+    else if (e == 2) edge_fill(d, s, gamma, 1, j, ke, 2, j, ke, 1, j, kl, 2, j, kl);
+    else edge_fill(d, s, gamma, ie, j, ke, il, j, ke, ie, j, kl, il, j, kl);
+  }
+  __syncthreads();
+  // phase 3: edges along i including the ghost i (:101-137); the third
+  // line adds and subtracts the same value (:120)
+  for (int t = tid; t < 4 * (ib + 1); t += nt) {
+    int e = t & 3, i = t >> 2;
+    if (e == 0) edge_fill(d, s, gamma, i, 1, 1, i, 1, 2, i, 2, 1, i, 2, 2);
+    else if (e == 1) edge_fill(d, s, gamma, i, 1, ke, i, 2, ke, i, 1, kl, i, 2, kl);
+    else if (e == 2) edge_fill(d, s, gamma, i, je, 1, i, je, 1, i, jl, 2, i, jl, 2);
+    else edge_fill(d, s, gamma, i, je, ke, i, jl, ke, i, je, kl, i, jl, kl);
+  }
+  __syncthreads();
+  const T two = T(2), three = T(3);
+#define WQ(n, i, j, k) W[(n) * E + ix(d, i, j, k)]
+  // phase 4: the 8 corners (:141-157)
+  if (tid < 40) {
+    int n = tid % 5, c = tid / 5;
+    if (c == 0) WQ(n, 1, 1, 1) = (WQ(n, 2, 1, 1) + WQ(n, 1, 2, 1) + WQ(n, 1, 1, 2)) * two / three - WQ(n, 2, 2, 2);
+    if (c == 1) WQ(n, 1, 1, ke) = (WQ(n, 2, 1, ke) + WQ(n, 1, 2, ke) + WQ(n, 1, 1, kl)) * two / three - WQ(n, 2, 2, kl);
+    if (c == 2) WQ(n, 1, je, 1) = (WQ(n, 2, je, 1) + WQ(n, 1, jl, 1) + WQ(n, 1, je, 2)) * two / three - WQ(n, 2, jl, 2);
+    if (c == 3) WQ(n, ie, 1, 1) = (WQ(n, il, 1, 1) + WQ(n, ie, 2, 1) + WQ(n, ie, 1, 2)) * two / three - WQ(n, il, 2, 2);
+    if (c == 4) WQ(n, ie, 1, ke) = (WQ(n, il, 1, ke) + WQ(n, ie, 2, ke) + WQ(n, ie, 1, kl)) * two / three - WQ(n, il, 2, kl);
+    if (c == 5) WQ(n, ie, je, 1) = (WQ(n, il, je, 1) + WQ(n, ie, jl, 1) + WQ(n, ie, je, 2)) * two / three - WQ(n, il, jl, 2);
+    if (c == 6) WQ(n, 1, je, ke) = (WQ(n, 2, je, ke) + WQ(n, 1, jl, ke) + WQ(n, 1, je, kl)) * two / three - WQ(n, 2, jl, kl);
+    if (c == 7) WQ(n, ie, je, ke) = (WQ(n, il, je, ke) + WQ(n, ie, jl, ke) + WQ(n, ie, je, kl)) * two / three - WQ(n, il, jl, kl);
+  }
+  __syncthreads();
+#undef WQ
+  if (tid < 8) {
+    int c = tid;
+    int ci = (c & 1) ? ie : 1, cj = (c & 2) ? je : 1, ck = (c & 4) ? ke : 1;
+    long q = ix(d, ci, cj, ck);
+    T v[5];
+    for (int n = 0; n < 5; n++) v[n] = s.w[n * E + q];
+    s.p[q] = eos(gamma, v);
+  }
+}
+
+// ------------------------------------------------------------------ misc kernels
+
+// ghost shell of src -> dst (w and p), so bcond sees the same ghost values
+// that the reference's single in-place array holds at that point
+template <typename T>
+__global__ void __launch_bounds__(256) k_ghost_copy(Dims d, StateC<T> src, State<T> dst) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= d.E) return;
+  int k = (int)(q / d.sz);
+  int j = (int)((q - (long)k * d.sz) / d.sy);
+  int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
+  bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
+  if (inner) return;
+  for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
+  dst.p[q] = src.p[q];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_eos_all(Dims d, State<T> s, T gamma) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= d.E) return;
+  T v[5];
+  for (int n = 0; n < 5; n++) v[n] = s.w[n * d.E + q];
+  s.p[q] = eos(gamma, v);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_freestream(Dims d, State<T> s, T r0, T ru, T rv, T rw, T re) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= d.E) return;
+  s.w[q] = r0;
+  s.w[d.E + q] = ru;
+  s.w[2 * d.E + q] = rv;
+  s.w[3 * d.E + q] = rw;
+  s.w[4 * d.E + q] = re;
+}
+
+// ------------------------------------------------------------------ transfers
+
+template <typename T>
+struct XferArgs {
+  Dims dc, df;           // coarse, fine
+  int rx, ry, rz;        // fine/coarse ratio per direction
+  const T *wf, *dwf;
+  T *wc;
+  const T *wsc;
+  T *wr;
+  uint8_t *marker;
+  T volc, volf;
+  int fmg;               // mgl-1 == mlvl: interpolate the solution itself
+  const uint8_t *flagsf;
+  T *wfo;                // fine state (prolongation, in place)
+};
+
+template <typename T>
+__device__ inline void rstencil(int I, int r, int *s, T *f) {
+  if (r == 2) { *s = 2 * I - 2; f[0] = T(0.5); f[1] = T(0.5); }
+  else { *s = I; f[0] = T(1); f[1] = T(0); }
+}
+template <typename T>
+__device__ inline void pstencil(int i, int r, int *s, T *f) {
+  if (r == 2) {
+    if ((i & 1) == 0) { *s = i / 2; f[0] = T(0.25); f[1] = T(0.75); }
+    else { *s = (i - 1) / 2 + 1; f[0] = T(0.75); f[1] = T(0.25); }
+  } else { *s = i; f[0] = T(1); f[1] = T(0); }
+}
+
+// collc.f90:110-198 for aligned ratio-1/2 levels: w and the residual
+// restricted with the cell-centre trilinear weights; marker = 1
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(XferArgs<T> X) {
+  const Dims &c = X.dc, &f = X.df;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 2 + blockIdx.z;
+  if (i > c.il || j > c.jl || k > c.kl) return;
+  int s[3];
+  T fa[2], fb[2], fc[2], fac[8];
+  rstencil(i, X.rx, &s[0], fa);
+  rstencil(j, X.ry, &s[1], fb);
+  rstencil(k, X.rz, &s[2], fc);
+  for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
+  long qc = ix(c, i, j, k);
+  long qf[8];
+  for (int c8 = 0; c8 < 8; c8++) qf[c8] = ix(f, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+  for (int n = 0; n < 5; n++) {
+    T acc = T(0);
+    for (int c8 = 0; c8 < 8; c8++) {
+      T val = X.dwf[n * f.E + qf[c8]] / X.volf;
+      acc = acc + val * fac[c8];
+    }
+    X.wr[n * c.E + qc] = acc * X.volc;
+  }
+  for (int n = 0; n < 5; n++) {
+    T acc = T(0);
+    for (int c8 = 0; c8 < 8; c8++) acc = acc + X.wf[n * f.E + qf[c8]] * fac[c8];
+    X.wc[n * c.E + qc] = acc;
+  }
+  X.marker[qc] = 1;
+}
+
+// addw.f90:369-438: trilinear prolongation of the correction w-ws (or of
+// the solution on the full-multigrid step) to fluid cells
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(XferArgs<T> X) {
+  const Dims &c = X.dc, &f = X.df;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 2 + blockIdx.z;
+  if (i > f.il || j > f.jl || k > f.kl) return;
+  int s[3];
+  T fa[2], fb[2], fc[2], fac[8];
+  pstencil(i, X.rx, &s[0], fa);
+  pstencil(j, X.ry, &s[1], fb);
+  pstencil(k, X.rz, &s[2], fc);
+  for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
+  long qf = ix(f, i, j, k);
+  if (!(X.flagsf[qf] & F_FLUID)) return;
+  long qc[8];
+  for (int c8 = 0; c8 < 8; c8++) qc[c8] = ix(c, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+  for (int n = 0; n < 5; n++) {
+    T acc = T(0);
+    for (int c8 = 0; c8 < 8; c8++) {
+      T val = X.wc[n * c.E + qc[c8]];
+      if (!X.fmg) val = val - X.wsc[n * c.E + qc[c8]];
+      acc = acc + val * fac[c8];
+    }
+    if (X.fmg) X.wfo[n * f.E + qf] = acc;
+    else X.wfo[n * f.E + qf] = X.wfo[n * f.E + qf] + acc;
+  }
+}
+
+// wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253)
+// in FP64; one thread per record writes its term, one thread adds them in
+// record order
+template <typename T>
+__global__ void k_wall_terms(Dims d, StateC<T> s, Geo<T> g, const DevRec *recs, const double *wsol,
+                             const double *psol, int nmir, double *terms) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmir) return;
+  const DevRec &r = recs[m];
+  int zz = r.zz;
+  double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+  sc[zz - 1] = face_s_any(g, zz, r.c[0], r.c[1], r.c[2]);
+  ss[zz - 1] = face_s_any(g, zz, r.s[0], r.s[1], r.s[2]);
+  long qc = ix(d, r.c[0], r.c[1], r.c[2]);
+  double wc[5], ws[5], f1[5], f2[5];
+  for (int n = 0; n < 5; n++) wc[n] = s.w[n * d.E + qc];
+  for (int n = 0; n < 5; n++) ws[n] = wsol[n * nmir + m];
+  cflux3(wc, (double)s.p[qc], sc, f1);
+  cflux3(ws, psol[m], ss, f2);
+  double sgn = (r.faceid >= 4) ? -1.0 : 1.0;
+  for (int dd = 0; dd < 3; dd++) terms[3 * m + dd] = sgn * (0.5 * (f1[1 + dd] + f2[1 + dd]));
+}
+
+__global__ void k_wall_sum(const double *terms, int nmir, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double F[3] = {0.0, 0.0, 0.0};
+  for (int m = 0; m < nmir; m++)
+    for (int dd = 0; dd < 3; dd++) F[dd] = F[dd] + terms[3 * m + dd];
+  for (int dd = 0; dd < 3; dd++) out[dd] = F[dd];
+}
+
+}  // namespace mge

@@ -1,14 +1,15 @@
 // mge_kernels.h -- device-side data views and the per-cell arithmetic of
-// the Jameson-RK stage, shared by every kernel of the FP64 path.
+// the Jameson-RK stage, templated on the bulk type T (double or float).
 //
 // Each helper restates one piece of the reference with the same expression
-// tree (the library is compiled with -ffp-contract=off), so the GPU path
-// reproduces the reference arithmetic operation for operation:
+// tree (the library is compiled with -ffp-contract=off), so with T = double
+// the GPU path reproduces the reference arithmetic operation for operation:
 //   eos           depvars.f90:37-52
 //   cflux         ceflux.f90:1-25 with the face vector of one direction
 //   fp_at/diw_at  dflux.f90:70-86 (pressure switch, enthalpy differences)
 //   dface         dflux.f90:87-111 (one face of the JST dissipation)
 //   dmatrix       dflux.f90:283-339 (matrix dissipation)
+// Every literal is written as a T value so that T = float computes in FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,52 +34,59 @@ __host__ __device__ inline long ix(const Dims &d, int i, int j, int k) {
 enum : uint8_t { F_FLUID = 1, F_TOUCHED = 2 };
 
 // read-only view of one state buffer: w (5 components) followed by p
+template <typename T>
 struct StateC {
-  const double *w;
-  const double *p;
+  const T *w;
+  const T *p;
 };
+template <typename T>
 struct State {
-  double *w;
-  double *p;
+  T *w;
+  T *p;
 };
 
+template <typename T>
 struct Geo {
-  const double *dx, *dy, *dz;  // vertex differences x(i+1)-x(i)
+  const T *dx, *dy, *dz;  // vertex differences x(i+1)-x(i)
 };
 
 // face vector of cell (i,j,k) in direction dir: the non-zero component of
 // si/sj/sk (init_metric.f90:216-274) on an axis-aligned grid.
 // si = (-dy*dz, 0, 0), sj = (0, -dx*dz, 0), sk = (0, 0, -dx*dy); the
 // cross products of init_metric reduce to exactly these values.
-template <int DIR>
-__device__ inline double face_s(const Geo &g, int i, int j, int k) {
+template <int DIR, typename T>
+__device__ inline T face_s(const Geo<T> &g, int i, int j, int k) {
   if (DIR == 0) return -(g.dy[j] * g.dz[k]);
   if (DIR == 1) return -(g.dx[i] * g.dz[k]);
   return -(g.dx[i] * g.dy[j]);
 }
 
-__device__ inline double dimd(double a, double b) { return a > b ? a - b : 0.0; }
-__device__ inline double maxd(double a, double b) { return a > b ? a : b; }
+template <typename T>
+__device__ inline T dimd(T a, T b) { return a > b ? a - b : T(0); }
+template <typename T>
+__device__ inline T maxd(T a, T b) { return a > b ? a : b; }
 
 // depvars_one: p = (gamma-1)*dim(rhoE, 0.5*|m|^2/rho)
-__device__ inline double eos(double gamma, const double *v) {
-  double s = 0.0;
+template <typename T>
+__device__ inline T eos(T gamma, const T *v) {
+  T s = T(0);
   s = s + v[1] * v[1];
   s = s + v[2] * v[2];
   s = s + v[3] * v[3];
-  double qq = 0.5 * s / v[0];
-  return (gamma - 1.0) * dimd(v[4], qq);
+  T qq = T(0.5) * s / v[0];
+  return (gamma - T(1)) * dimd(v[4], qq);
 }
 
-__device__ inline void load5(const double *w, long E, long q, double *v) {
+template <typename T>
+__device__ inline void load5(const T *w, long E, long q, T *v) {
 #pragma unroll
   for (int n = 0; n < 5; n++) v[n] = w[n * E + q];
 }
 
 // ceflux with ss having only component DIR non-zero (value S)
-template <int DIR>
-__device__ inline void cflux(const double *v, double p, double S, double *fs) {
-  double f1 = v[1 + DIR] * S;
+template <int DIR, typename T>
+__device__ inline void cflux(const T *v, T p, T S, T *fs) {
+  T f1 = v[1 + DIR] * S;
   fs[0] = f1;
   fs[1] = f1 * v[1] / v[0];
   fs[2] = f1 * v[2] / v[0];
@@ -88,8 +96,9 @@ __device__ inline void cflux(const double *v, double p, double S, double *fs) {
 }
 
 // general ceflux (full face vector), used by the wall records
-__device__ inline void cflux3(const double *v, double p, const double *ss, double *fs) {
-  double f1 = 0.0;
+template <typename T>
+__device__ inline void cflux3(const T *v, T p, const T *ss, T *fs) {
+  T f1 = T(0);
   f1 = f1 + v[1] * ss[0];
   f1 = f1 + v[2] * ss[1];
   f1 = f1 + v[3] * ss[2];
@@ -101,40 +110,41 @@ __device__ inline void cflux3(const double *v, double p, const double *ss, doubl
 }
 
 // dflux_matrix (dflux.f90:283-339); u1/u2 carry the enthalpy in slot 4
-__device__ inline void dmatrix(double gamma, const double *u1, const double *u2, const double *dw,
-                               const double *ss, double *ff) {
-  double gam1 = gamma - 1.0, epsn = 1.0, epsl = 1.0;
-  double rr = 0.5 * (u1[0] + u2[0]);
-  double us[3];
-  for (int d = 0; d < 3; d++) us[d] = 0.5 * (u1[1 + d] + u2[1 + d]) / rr;
-  double ee = 0.5 * (u1[4] + u2[4]) / rr;
-  double su = 0.0;
+template <typename T>
+__device__ inline void dmatrix(T gamma, const T *u1, const T *u2, const T *dw, const T *ss, T *ff) {
+  const T half = T(0.5);
+  T gam1 = gamma - T(1), epsn = T(1), epsl = T(1);
+  T rr = half * (u1[0] + u2[0]);
+  T us[3];
+  for (int d = 0; d < 3; d++) us[d] = half * (u1[1 + d] + u2[1 + d]) / rr;
+  T ee = half * (u1[4] + u2[4]) / rr;
+  T su = T(0);
   su = su + us[0] * us[0];
   su = su + us[1] * us[1];
   su = su + us[2] * us[2];
-  double qq = 0.5 * su;
-  double c2 = gamma * gam1 * (ee - qq);
-  double hh = c2 / gam1 + qq;
-  double sss = 0.0;
+  T qq = half * su;
+  T c2 = gamma * gam1 * (ee - qq);
+  T hh = c2 / gam1 + qq;
+  T sss = T(0);
   sss = sss + ss[0] * ss[0];
   sss = sss + ss[1] * ss[1];
   sss = sss + ss[2] * ss[2];
-  double cs = sqrt(c2 * sss);
-  double qn = 0.0;
+  T cs = sqrt(c2 * sss);
+  T qn = T(0);
   qn = qn + ss[0] * us[0];
   qn = qn + ss[1] * us[1];
   qn = qn + ss[2] * us[2];
-  double e0 = fabs(qn) + cs;
-  double e1 = maxd(epsn * e0, fabs(qn + cs));
-  double e2 = maxd(epsn * e0, fabs(qn - cs));
-  double e3 = maxd(epsl * e0, fabs(qn));
-  double s1 = 0.5 * (e1 + e2), s2 = 0.5 * (e1 - e2), s3 = e3;
-  double ak1 = (s1 - s3) / c2 * gam1;
-  double ak2 = s2 / cs;
-  double ak3 = c2 / gam1 / sss;
-  double vd = qq * dw[0] - us[0] * dw[1] - us[1] * dw[2] - us[2] * dw[3] + dw[4];
-  double qd = qn * dw[0] - ss[0] * dw[1] - ss[1] * dw[2] - ss[2] * dw[3];
-  double f[5];
+  T e0 = fabs(qn) + cs;
+  T e1 = maxd(epsn * e0, T(fabs(qn + cs)));
+  T e2 = maxd(epsn * e0, T(fabs(qn - cs)));
+  T e3 = maxd(epsl * e0, T(fabs(qn)));
+  T s1 = half * (e1 + e2), s2 = half * (e1 - e2), s3 = e3;
+  T ak1 = (s1 - s3) / c2 * gam1;
+  T ak2 = s2 / cs;
+  T ak3 = c2 / gam1 / sss;
+  T vd = qq * dw[0] - us[0] * dw[1] - us[1] * dw[2] - us[2] * dw[3] + dw[4];
+  T qd = qn * dw[0] - ss[0] * dw[1] - ss[1] * dw[2] - ss[2] * dw[3];
+  T f[5];
   f[0] = ak1 * vd - ak2 * qd + s3 * dw[0];
   f[1] = ak1 * (us[0] * vd - ss[0] * ak3 * qd) + ak2 * (gam1 * ss[0] * vd - us[0] * qd) + s3 * dw[1];
   f[2] = ak1 * (us[1] * vd - ss[1] * ak3 * qd) + ak2 * (gam1 * ss[1] * vd - us[1] * qd) + s3 * dw[2];
@@ -144,13 +154,14 @@ __device__ inline void dmatrix(double gamma, const double *u1, const double *u2,
 }
 
 // everything a flux evaluation needs
+template <typename T>
 struct FluxCtx {
   Dims d;
-  StateC s;
-  Geo g;
-  const double *radsum;  // radi+radj+radk at interior cells
-  double gamma, vis2, vis4, cdis;
-  int dissip;            // 0 scalar, 1 matrix
+  StateC<T> s;
+  Geo<T> g;
+  const T *radsum;  // radi+radj+radk at interior cells
+  T gamma, vis2, vis4, cdis;
+  int dissip;       // 0 scalar, 1 matrix
 };
 
 template <int DIR>
@@ -166,32 +177,35 @@ __device__ inline int dnl(const Dims &d) {
   return DIR == 0 ? d.il : (DIR == 1 ? d.jl : d.kl);
 }
 
-// pressure switch fp(s) of one line (dflux.f90:70-81); fp(0)=fp(ib)=0
-template <int DIR>
-__device__ inline double fp_at(const FluxCtx &c, long base, int s) {
+// pressure switch fp(s) of one line (dflux.f90:70-81); fp(0)=fp(ib)=0.
+// Branch-free: the loads use an index clamped into [1, ie] (always inside
+// the array) and the boundary convention is applied by a select, so the
+// compiler can batch the loads of neighbouring calls.
+template <int DIR, typename T>
+__device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
   int nb = dne<DIR>(c.d) + 1;
-  if (s <= 0 || s >= nb) return 0.0;
+  int sc = s < 1 ? 1 : (s > nb - 1 ? nb - 1 : s);
   long st = dstep<DIR>(c.d);
-  double pm = c.s.p[base + (s - 1) * st], pc = c.s.p[base + s * st], pp = c.s.p[base + (s + 1) * st];
-  double pb, num;
+  T pm = c.s.p[base + (sc - 1) * st], pc = c.s.p[base + sc * st], pp = c.s.p[base + (sc + 1) * st];
+  T pb, num;
   if (DIR == 1) {
-    pb = fabs(pp + 2.0 * pc + pm);
-    num = fabs(pp - 2.0 * pc + pm);
+    pb = fabs(pp + T(2) * pc + pm);
+    num = fabs(pp - T(2) * pc + pm);
   } else {
-    pb = fabs(pm + 2.0 * pc + pp);
-    num = fabs(pm - 2.0 * pc + pp);
+    pb = fabs(pm + T(2) * pc + pp);
+    num = fabs(pm - T(2) * pc + pp);
   }
-  if (fabs(pb) > 0.0) return num / pb;
-  return 0.0;
+  T v = num / pb;
+  v = fabs(pb) > T(0) ? v : T(0);
+  return (s <= 0 || s >= nb) ? T(0) : v;
 }
 
 // enthalpy-vector difference diw(s) = q(s)-q(s-1) with the end copies
 // diw(1)=diw(2), diw(ib)=diw(ie) (dflux.f90:82-86)
-template <int DIR>
-__device__ inline void diw_at(const FluxCtx &c, long base, int s, double *o) {
-  int ne = dne<DIR>(c.d), nb = ne + 1;
-  if (s <= 1) s = 2;
-  else if (s >= nb) s = ne;
+template <int DIR, typename T>
+__device__ inline void diw_at(const FluxCtx<T> &c, long base, int s, T *o) {
+  int ne = dne<DIR>(c.d);
+  s = s < 2 ? 2 : (s > ne ? ne : s);
   long st = dstep<DIR>(c.d);
   long qa = base + s * st, qb = qa - st;
   const long E = c.d.E;
@@ -200,43 +214,42 @@ __device__ inline void diw_at(const FluxCtx &c, long base, int s, double *o) {
   o[4] = (c.s.w[4 * E + qa] + c.s.p[qa]) - (c.s.w[4 * E + qb] + c.s.p[qb]);
 }
 
-template <int DIR>
-__device__ inline double rad_at(const FluxCtx &c, long base, int s) {
+template <int DIR, typename T>
+__device__ inline T rad_at(const FluxCtx<T> &c, long base, int s) {
   int nl = dnl<DIR>(c.d);
-  if (s < 2) s = 2;
-  if (s > nl) s = nl;
+  s = s < 2 ? 2 : (s > nl ? nl : s);
   return c.radsum[base + s * dstep<DIR>(c.d)];
 }
 
 // JST dissipation at face a (between a-1 and a) of the line `base`
 // (dflux.f90:87-111); zero outside 2..ie like the unset dfsx entries.
-template <int DIR>
-__device__ inline void dface(const FluxCtx &c, long base, int a, int li, int lj, int lk, double *out) {
+template <int DIR, typename T>
+__device__ inline void dface(const FluxCtx<T> &c, long base, int a, int li, int lj, int lk, T *out) {
   int ne = dne<DIR>(c.d);
   if (a < 2 || a > ne) {
 #pragma unroll
-    for (int n = 0; n < 5; n++) out[n] = 0.0;
+    for (int n = 0; n < 5; n++) out[n] = T(0);
     return;
   }
-  double f0 = fp_at<DIR>(c, base, a - 2), f1 = fp_at<DIR>(c, base, a - 1);
-  double f2 = fp_at<DIR>(c, base, a), f3 = fp_at<DIR>(c, base, a + 1);
-  double ep = maxd(maxd(maxd(f0, f1), f2), f3);
-  double lp = rad_at<DIR>(c, base, a);
-  double lm = rad_at<DIR>(c, base, a - 1);
-  double lam = 0.5 * (lm + lp);
-  double dis2 = c.vis2 * ep;
-  double dis4 = c.vis4;
+  T f0 = fp_at<DIR>(c, base, a - 2), f1 = fp_at<DIR>(c, base, a - 1);
+  T f2 = fp_at<DIR>(c, base, a), f3 = fp_at<DIR>(c, base, a + 1);
+  T ep = maxd(maxd(maxd(f0, f1), f2), f3);
+  T lp = rad_at<DIR>(c, base, a);
+  T lm = rad_at<DIR>(c, base, a - 1);
+  T lam = T(0.5) * (lm + lp);
+  T dis2 = c.vis2 * ep;
+  T dis4 = c.vis4;
   dis4 = dimd(dis4, dis2);
   dis2 = dis2 * c.cdis;
   dis4 = dis4 * c.cdis;
-  double dm[5], da[5], dp[5];
+  T dm[5], da[5], dp[5];
   diw_at<DIR>(c, base, a - 1, dm);
   diw_at<DIR>(c, base, a, da);
   diw_at<DIR>(c, base, a + 1, dp);
-  double d[5];
+  T d[5];
 #pragma unroll
   for (int n = 0; n < 5; n++) {
-    double t = dp[n] - 2.0 * da[n] + dm[n];
+    T t = dp[n] - T(2) * da[n] + dm[n];
     d[n] = dis2 * da[n] - dis4 * t;
   }
   if (c.dissip == 0) {
@@ -245,32 +258,32 @@ __device__ inline void dface(const FluxCtx &c, long base, int a, int li, int lj,
   } else {
     long st = dstep<DIR>(c.d);
     long qm = base + (a - 1) * st, qa = base + a * st;
-    double um[5], ua[5];
+    T um[5], ua[5];
     load5(c.s.w, c.d.E, qm, um);
     load5(c.s.w, c.d.E, qa, ua);
     um[4] = um[4] + c.s.p[qm];
     ua[4] = ua[4] + c.s.p[qa];
     int fi = DIR == 0 ? a : li, fj = DIR == 1 ? a : lj, fk = DIR == 2 ? a : lk;
-    double ss[3] = {0.0, 0.0, 0.0};
+    T ss[3] = {T(0), T(0), T(0)};
     ss[DIR] = face_s<DIR>(c.g, fi, fj, fk);
     dmatrix(c.gamma, um, ua, d, ss, out);
   }
 }
 
 // convective flux at face a: 0.5*(F(a)+F(a-1)) (eflux.f90:44-48)
-template <int DIR>
-__device__ inline void eface(const FluxCtx &c, long base, int a, int li, int lj, int lk, double *out) {
+template <int DIR, typename T>
+__device__ inline void eface(const FluxCtx<T> &c, long base, int a, int li, int lj, int lk, T *out) {
   long st = dstep<DIR>(c.d);
   long qa = base + a * st, qm = qa - st;
   int ai = DIR == 0 ? a : li, aj = DIR == 1 ? a : lj, ak = DIR == 2 ? a : lk;
   int mi = DIR == 0 ? a - 1 : li, mj = DIR == 1 ? a - 1 : lj, mk = DIR == 2 ? a - 1 : lk;
-  double va[5], vm[5], fa[5], fm[5];
+  T va[5], vm[5], fa[5], fm[5];
   load5(c.s.w, c.d.E, qa, va);
   load5(c.s.w, c.d.E, qm, vm);
   cflux<DIR>(va, c.s.p[qa], face_s<DIR>(c.g, ai, aj, ak), fa);
   cflux<DIR>(vm, c.s.p[qm], face_s<DIR>(c.g, mi, mj, mk), fm);
 #pragma unroll
-  for (int n = 0; n < 5; n++) out[n] = 0.5 * (fa[n] + fm[n]);
+  for (int n = 0; n < 5; n++) out[n] = T(0.5) * (fa[n] + fm[n]);
 }
 
 }  // namespace mge

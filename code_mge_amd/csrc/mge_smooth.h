@@ -1,0 +1,276 @@
+// mge_smooth.h -- implicit residual smoothing (psmoo_old.f90) and the
+// post-smoothing update, templated on the bulk type T.
+//
+// psmoo_old solves, for every grid line and each direction in turn, the
+// constant-coefficient tridiagonal system (1+2 eps) x_s - eps (x_(s-1) +
+// x_(s+1)) = dw_s with a Thomas recurrence whose pivots e1/e2 depend only
+// on the line length (psmoo.f90:134-166); blank cells are skipped (their
+// value is kept and they do not feed the recurrence).
+#pragma once
+#include "mge_kernels.h"
+
+namespace mge {
+
+template <typename T>
+struct SmoothArgs {
+  Dims d;
+  T *dw;
+  const uint8_t *flags;
+  const T *e1, *e2;  // pivots of this direction (psmoo.f90:134-166)
+  const T *fa;       // -bb/e1 (x lines: affine-map form of the forward step)
+  T bb;
+};
+
+// psmoo_old.f90:171-188 (x lines), one wavefront per (j,k) line and
+// component.  Lane l owns CPL consecutive cells, so loads and stores are
+// coalesced.  Both recurrences are first composed as affine maps
+// (forward y_i = A_i y_(i-1) + B_i, backward z_i = C_i z_(i+1) + y_i),
+// scanned across the wave, and then every lane re-evaluates its own cells
+// with the reference formula from the scanned boundary value.  Only the
+// lane-boundary values differ from the sequential evaluation (by a few
+// ulp); blank cells keep their value as in the reference.
+template <int CPL, typename T>
+__global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs<T> S) {
+  const Dims &d = S.d;
+  const int lane = threadIdx.x & 63;
+  const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+  const long nl = (long)d.nj * d.nk;
+  if (wave >= nl * 5) return;  // uniform per wave
+  const int n = (int)(wave / nl);
+  const long l = wave - (long)n * nl;
+  const int j = 2 + (int)(l % d.nj), k = 2 + (int)(l / d.nj);
+  T *D = S.dw + n * d.E;
+  const long b0 = ix(d, 0, j, k);
+  const int il = d.il, ni = d.ni;
+  const int i0 = 2 + lane * CPL;
+  T r[CPL];
+  bool fl[CPL], ok[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    ok[c] = i <= il;
+    r[c] = ok[c] ? D[b0 + i] : T(0);
+    fl[c] = ok[c] && (S.flags[b0 + i] & F_FLUID);
+  }
+  // forward: lane composite of y_i = A_i y_(i-1) + B_i
+  T Al = T(1), Bl = T(0);
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    T A, B;
+    if (!ok[c]) { A = T(1); B = T(0); }
+    else if (!fl[c]) { A = T(0); B = r[c]; }
+    else if (i == 2) { A = T(0); B = r[c] / S.e1[2]; }
+    else { A = S.fa[i]; B = r[c] / S.e1[i]; }
+    Bl = A * Bl + B;
+    Al = A * Al;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T Ap = __shfl_up(Al, off, 64), Bp = __shfl_up(Bl, off, 64);
+    if (lane >= off) {
+      Bl = Al * Bp + Bl;
+      Al = Al * Ap;
+    }
+  }
+  T yin = __shfl_up(Bl, 1, 64);
+  if (lane == 0) yin = T(0);
+  T y[CPL];
+  T yp = yin;
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    int i = i0 + c;
+    T v = r[c];
+    if (ok[c] && fl[c]) v = (i == 2) ? r[c] / S.e1[2] : (r[c] - S.bb * yp) / S.e1[i];
+    y[c] = v;
+    yp = v;
+  }
+  // backward: z_i = C_i z_(i+1) + y_i, C_i = -e2_i on fluid cells 2..ni
+  T Cl = T(1), Dl = T(0);
+#pragma unroll
+  for (int c = CPL - 1; c >= 0; c--) {
+    int i = i0 + c;
+    T C, Dv;
+    if (!ok[c]) { C = T(1); Dv = T(0); }
+    else if (fl[c] && i <= ni) { C = -S.e2[i]; Dv = y[c]; }
+    else { C = T(0); Dv = y[c]; }
+    Dl = C * Dl + Dv;
+    Cl = C * Cl;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T Cn = __shfl_down(Cl, off, 64), Dn = __shfl_down(Dl, off, 64);
+    if (lane + off < 64) {
+      Dl = Cl * Dn + Dl;
+      Cl = Cl * Cn;
+    }
+  }
+  T zin = __shfl_down(Dl, 1, 64);
+  if (lane == 63) zin = T(0);
+  T zn = zin;
+#pragma unroll
+  for (int c = CPL - 1; c >= 0; c--) {
+    int i = i0 + c;
+    if (!ok[c]) continue;
+    T v = y[c];
+    if (fl[c] && i <= ni) v = y[c] - S.e2[i] * zn;
+    zn = v;
+    if (fl[c]) D[b0 + i] = v;
+  }
+}
+
+// sequential x lines for blocks wider than 512 cells
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth_x_seq(SmoothArgs<T> S) {
+  const Dims &d = S.d;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long nl = (long)d.nj * d.nk;
+  if (t >= nl * 5) return;
+  int n = (int)(t / nl);
+  long l = t - (long)n * nl;
+  int j = 2 + (int)(l % d.nj), k = 2 + (int)(l / d.nj);
+  T *D = S.dw + n * d.E;
+  long b0 = ix(d, 0, j, k);
+  if (S.flags[b0 + 2] & F_FLUID) D[b0 + 2] = D[b0 + 2] / S.e1[2];
+  for (int i = 3; i <= d.il; i++)
+    if (S.flags[b0 + i] & F_FLUID) D[b0 + i] = (D[b0 + i] - S.bb * D[b0 + i - 1]) / S.e1[i];
+  for (int i = 2; i <= d.ni; i++) {
+    int ir = d.il + 1 - i;
+    if (S.flags[b0 + ir] & F_FLUID) D[b0 + ir] = D[b0 + ir] - S.e2[ir] * D[b0 + ir + 1];
+  }
+}
+
+// One line of the exact sequential recurrence of psmoo_old
+// (psmoo.f90:190-237): cells s = 2..nl at stride st.  Full chunks of CH
+// cells are read into registers double-buffered with unconditional loads,
+// so the compiler can keep the next chunk's loads in flight (counted waits)
+// while the current chunk runs its dependent divisions.  The updates are
+// branch-free: blank cells hold dw = +-0 (blanked before smoothing), so
+// y = m*((r - bb*y)/e1) with m = 0 on blank cells gives the reference's
+// values on fluid cells and keeps the zeros.  The backward sweep starts
+// at s = nl with z(nl+1) = 0 and e2(nl) = 0, which returns y(nl) there.
+template <int CH, typename T>
+__device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__ D, const uint8_t *__restrict__ F,
+                                                                 long b0, long st, int nl, const T *__restrict__ e1,
+                                                                 const T *__restrict__ e2, T bb) {
+  const int ncell = nl - 1;
+  const int nfull = ncell / CH;
+  T A[CH], B[CH], MA[CH], MB[CH];
+  T y = T(0);
+#define LD_F(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    long q_ = b0 + (long)(2 + (c) * CH + t) * st;                     \
+    BUF[t] = D[q_];                                                   \
+    MK[t] = (F[q_] & F_FLUID) ? T(1) : T(0);                          \
+  }
+#define PR_F(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    int s_ = 2 + (c) * CH + t;                                        \
+    y = MK[t] * ((BUF[t] - bb * y) / e1[s_]);                         \
+    D[b0 + (long)s_ * st] = y;                                        \
+  }
+#define LD_B(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    long q_ = b0 + (long)(nl - (c) * CH - t) * st;                    \
+    BUF[t] = D[q_];                                                   \
+    MK[t] = (F[q_] & F_FLUID) ? T(1) : T(0);                          \
+  }
+#define PR_B(BUF, MK, c)                                              \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    int s_ = nl - (c) * CH - t;                                       \
+    y = BUF[t] - MK[t] * (e2[s_] * y);                                \
+    D[b0 + (long)s_ * st] = y;                                        \
+  }
+  // forward
+  if (nfull > 0) {
+    LD_F(A, MA, 0);
+    for (int c = 0; c < nfull; c += 2) {
+      int c1 = c + 1 < nfull ? c + 1 : c;
+      LD_F(B, MB, c1);
+      PR_F(A, MA, c);
+      if (c + 1 >= nfull) break;
+      int c2 = c + 2 < nfull ? c + 2 : c + 1;
+      LD_F(A, MA, c2);
+      PR_F(B, MB, c + 1);
+    }
+  }
+  for (int s_ = 2 + nfull * CH; s_ <= nl; s_++) {
+    long q_ = b0 + (long)s_ * st;
+    T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
+    y = m_ * ((D[q_] - bb * y) / e1[s_]);
+    D[q_] = y;
+  }
+  // backward
+  y = T(0);
+  if (nfull > 0) {
+    LD_B(A, MA, 0);
+    for (int c = 0; c < nfull; c += 2) {
+      int c1 = c + 1 < nfull ? c + 1 : c;
+      LD_B(B, MB, c1);
+      PR_B(A, MA, c);
+      if (c + 1 >= nfull) break;
+      int c2 = c + 2 < nfull ? c + 2 : c + 1;
+      LD_B(A, MA, c2);
+      PR_B(B, MB, c + 1);
+    }
+  }
+  for (int s_ = nl - nfull * CH; s_ >= 2; s_--) {
+    long q_ = b0 + (long)s_ * st;
+    T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
+    y = D[q_] - m_ * (e2[s_] * y);
+    D[q_] = y;
+  }
+#undef LD_F
+#undef PR_F
+#undef LD_B
+#undef PR_B
+}
+
+// psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
+template <typename T>
+__global__ void __launch_bounds__(64) k_smooth_y(SmoothArgs<T> S) {
+  const Dims &d = S.d;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int k = 2 + blockIdx.y;
+  int n = blockIdx.z;
+  if (i > d.il) return;
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
+}
+
+// psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
+template <typename T>
+__global__ void __launch_bounds__(64) k_smooth_z(SmoothArgs<T> S) {
+  const Dims &d = S.d;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y;
+  int n = blockIdx.z;
+  if (i > d.il) return;
+  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
+}
+
+// after smoothing: w = wn - dw and the EOS on interior cells
+// (euler.f90:381-399); the ghost shell of the previous state is carried
+// over, as k_ghost_copy does on unsmoothed stages
+template <typename T>
+__global__ void __launch_bounds__(256) k_update_ghost(Dims d, const T *wn, const T *dw, StateC<T> src, State<T> dst,
+                                                      T gamma) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= d.E) return;
+  int k = (int)(q / d.sz);
+  int j = (int)((q - (long)k * d.sz) / d.sy);
+  int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
+  bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
+  if (inner) {
+    T v[5];
+    for (int n = 0; n < 5; n++) {
+      v[n] = wn[n * d.E + q] - dw[n * d.E + q] / T(1);
+      dst.w[n * d.E + q] = v[n];
+    }
+    dst.p[q] = eos(gamma, v);
+  } else {
+    for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
+    dst.p[q] = src.p[q];
+  }
+}
+
+}  // namespace mge

@@ -1,0 +1,792 @@
+// mge_stage.h -- the flux/update kernels of one Jameson-RK stage and the
+// wall-record corrections, templated on the bulk type T.
+//
+// Stage pipeline (euler.f90:118-423), per RK stage m:
+//   k_stage2 / k_stage  fluxes + dissipation + blending + forcing + dt
+//               scaling (+ the update when the stage is not smoothed) for
+//               every cell not touched by a wall record
+//   k_rec_ops / k_rec_apply  the wall corrections of dfluxb.f90/efluxb.f90
+//               in record order for the touched cells, then the same finish
+// The wall terms that involve the mirror state (wsol/psol, always FP64) are
+// evaluated in FP64 whatever T is.
+#pragma once
+#include "mge_kernels.h"
+
+namespace mge {
+
+// device error flag: the first code raised stays (the reference stops at
+// its first failing check)
+__device__ inline void raise_err(int *flag, int code) { atomicCAS(flag, 0, code); }
+
+struct DevRec {
+  int faceid, zz;
+  int c[3], s[3];
+  int npoint, pad;
+  long cloud[8];
+  double coef[8];
+  double nvec[3];
+};
+
+// op kinds applied to one touched cell, in the reference order
+enum OpKind : int {
+  OP_HI_C_UNDO = 0,  // fw(s)   -= dfs(s) - dfs(s+1)        dfluxb.f90:47
+  OP_HI_N_UNDO,      // fw(s-1) -= dfs(s-1) - dfs(s)        dfluxb.f90:48
+  OP_HI_C_ADD,       // fw(s)   += fs1 - fs2                dfluxb.f90:104
+  OP_HI_N_ADD,       // fw(s-1) += dfs(s-1) - fs1           dfluxb.f90:105
+  OP_LO_C_UNDO,      // fw(s)   -= dfs(s) - dfs(s+1)        dfluxb.f90:109
+  OP_LO_N_UNDO,      // fw(s+1) -= dfs(s+1) - dfs(s+2)      dfluxb.f90:110
+  OP_LO_C_ADD,       // fw(s)   += fs1 - fs2                dfluxb.f90:170
+  OP_LO_N_ADD,       // fw(s+1) += fs2 - dfs(s+2)           dfluxb.f90:171
+  OP_E_UNDO,         // dw(s)   -= efs(s) - efs(s+1)        efluxb.f90:257
+  OP_E_ADD           // dw(s)   += fa - fs  or  fs - fa     efluxb.f90:262,266
+};
+
+template <typename T>
+struct StageArgs {
+  FluxCtx<T> fx;
+  T *wdst, *pdst;           // output state (update)
+  const T *wn;              // W^(0)
+  T *dw, *fw;
+  T *qf;
+  const T *wr;
+  const uint8_t *marker;
+  const uint8_t *flags;
+  const T *vdtim;
+  const double *dtim;       // device scalar (global time step)
+  T cellvolume;
+  T blend;                  // 1 - cdis(m)
+  T fn;                     // cstp(m)*cfl
+  int m;                    // stage number (1-based)
+  int has_diss;             // cdis(m) > 0
+  int coarse;               // mgl < mlvl
+  int lfir1;                // lfir and m == 1
+  int lres;
+  int smoothed;             // psmoo_old follows (odd stage, smoo > 0)
+  int store_fw;             // a later stage reads fw
+  int vt;
+};
+
+template <typename T>
+struct RecArgs {
+  StageArgs<T> st;
+  const DevRec *recs;
+  const double *wsol;       // 5*nmir, component-major
+  const double *psol;
+  int nmir;
+  const long *tcell;        // touched cells
+  const int *beg;           // op range of each touched cell (ntouch+1)
+  const int2 *ops;          // (record, kind), fw ops then dw ops per cell
+  double *opval;            // 10 doubles per op
+  int ntouch, nops;
+  int *err;
+};
+
+// ------------------------------------------------------------------ finish of a cell
+
+// dw = dw + fw, forcing, blanking, time-step scaling and (unsmoothed) the
+// update with the EOS (euler.f90:217-223, 294-342, 363-399)
+template <typename T>
+__device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, const T *e) {
+  const long E = A.fx.d.E;
+  T r[5];
+#pragma unroll
+  for (int n = 0; n < 5; n++) r[n] = e[n] + f[n];
+  if (A.coarse) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      T qv;
+      if (A.lfir1) {
+        qv = T(0);
+        if (A.marker[q] == 1) qv = T(1) * A.wr[n * E + q] - r[n];
+        A.qf[n * E + q] = qv;
+      } else {
+        qv = A.qf[n * E + q];
+      }
+      r[n] = r[n] + qv;
+    }
+  }
+  T bl = (A.flags[q] & F_FLUID) ? T(1) : T(0);
+#pragma unroll
+  for (int n = 0; n < 5; n++) r[n] = r[n] * bl;
+  if (A.store_fw) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) A.fw[n * E + q] = f[n];
+  }
+  if (A.lres) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
+    return;
+  }
+  T de = A.fn * A.vdtim[q] / A.cellvolume;
+  if (A.vt == 1) de = A.fn * T(A.dtim[0]) / A.cellvolume;
+#pragma unroll
+  for (int n = 0; n < 5; n++) r[n] = de * r[n];
+#pragma unroll
+  for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
+  if (!A.smoothed) {
+    T v[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      v[n] = A.wn[n * E + q] - r[n] / T(1);
+      A.wdst[n * E + q] = v[n];
+    }
+    A.pdst[q] = eos(A.fx.gamma, v);
+  }
+}
+
+// ------------------------------------------------------------------ k_stage
+//
+// One thread per interior cell, every face evaluated from global memory.
+// Used for matrix dissipation (whose per-face state is too large for the
+// LDS-tiled kernel's register budget).
+template <typename T>
+__global__ void __launch_bounds__(256) k_stage(StageArgs<T> A) {
+  const Dims &d = A.fx.d;
+  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 2 + blockIdx.z;
+  if (i > d.il || j > d.jl || k > d.kl) return;
+  const long E = d.E;
+  long q = ix(d, i, j, k);
+  T f[5];
+  if (A.m > 1) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+  } else {
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = T(0);
+  }
+  if (A.has_diss) {
+    T a[5], b[5];
+    long bx = ix(d, 0, j, k), by = ix(d, i, 0, k), bz = ix(d, i, j, 0);
+    dface<0>(A.fx, bx, i, i, j, k, a);
+    dface<0>(A.fx, bx, i + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = f[n] + (a[n] - b[n]);
+    dface<1>(A.fx, by, j, i, j, k, a);
+    dface<1>(A.fx, by, j + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = f[n] + (a[n] - b[n]);
+    dface<2>(A.fx, bz, k, i, j, k, a);
+    dface<2>(A.fx, bz, k + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = f[n] + (a[n] - b[n]);
+  }
+  T e[5];
+  {
+    T a[5], b[5];
+    long bx = ix(d, 0, j, k), by = ix(d, i, 0, k), bz = ix(d, i, j, 0);
+    eface<0>(A.fx, bx, i, i, j, k, a);
+    eface<0>(A.fx, bx, i + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) e[n] = a[n] - b[n];
+    eface<1>(A.fx, by, j, i, j, k, a);
+    eface<1>(A.fx, by, j + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) e[n] = e[n] + a[n] - b[n];
+    eface<2>(A.fx, bz, k, i, j, k, a);
+    eface<2>(A.fx, bz, k + 1, i, j, k, b);
+#pragma unroll
+    for (int n = 0; n < 5; n++) e[n] = e[n] + a[n] - b[n];
+  }
+  if (A.flags[q] & F_TOUCHED) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      A.fw[n * E + q] = f[n];
+      A.dw[n * E + q] = e[n];
+    }
+    return;
+  }
+  finish_cell(A, q, f, e);
+}
+
+// ------------------------------------------------------------------ k_stage2
+//
+// LDS-tiled form of k_stage.  A workgroup owns a TI x TJ column tile of
+// (i,j) and marches through a chunk of k planes.  Per plane the i- and
+// j-direction quantities every face needs -- the convective cell fluxes
+// F (ceflux.f90), the pressure switches fp and the enthalpy differences diw
+// (dflux.f90:70-86) -- are computed once per cell into LDS (tile + halo);
+// each thread then forms its faces from LDS.  The k-direction face values
+// are carried from plane to plane in registers, so every k face is
+// evaluated once.  Each face value is computed with the same expression
+// tree as the reference, so results equal k_stage's bit for bit.
+constexpr int TI = 64, TJ = 4;
+
+template <typename T>
+struct StageSmem {
+  T fx[5][TJ][TI + 2];   // F_x at i = i0-1 .. i0+TI
+  T fy[5][TJ + 2][TI];   // F_y at j = j0-1 .. j0+TJ
+  T px[TJ][TI + 4];      // fp_x at i = i0-2 .. i0+TI+1
+  T py[TJ + 4][TI];      // fp_y at j = j0-2 .. j0+TJ+1
+  T dx[5][TJ][TI + 3];   // diw_x at i = i0-1 .. i0+TI+1
+  T dy[5][TJ + 3][TI];   // diw_y at j = j0-1 .. j0+TJ+1
+};
+
+// JST face coefficients from the switches fp(a-2..a+1) and the radii sums
+// on both sides (dflux.f90:88-96); the face value of component n is then
+// (dis2*diw(a) - dis4*(diw(a+1) - 2 diw(a) + diw(a-1))) * lam
+template <typename T>
+__device__ inline void face_coef(const FluxCtx<T> &c, T f0, T f1, T f2, T f3, T lm, T lp, T *dis2o, T *dis4o,
+                                 T *lamo) {
+  T ep = maxd(maxd(maxd(f0, f1), f2), f3);
+  T lam = T(0.5) * (lm + lp);
+  T dis2 = c.vis2 * ep;
+  T dis4 = c.vis4;
+  dis4 = dimd(dis4, dis2);
+  dis2 = dis2 * c.cdis;
+  dis4 = dis4 * c.cdis;
+  *dis2o = dis2;
+  *dis4o = dis4;
+  *lamo = lam;
+}
+
+template <int DIR, typename T>
+__device__ inline void cell_flux(const FluxCtx<T> &c, int i, int j, int k, T *o) {
+  long q = ix(c.d, i, j, k);
+  T v[5];
+  load5(c.s.w, c.d.E, q, v);
+  cflux<DIR>(v, c.s.p[q], face_s<DIR>(c.g, i, j, k), o);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256, 2) k_stage2(StageArgs<T> A, int kc) {
+  __shared__ StageSmem<T> sm;
+  const FluxCtx<T> &c = A.fx;
+  const Dims &d = c.d;
+  const long E = d.E;
+  const T two = T(2), half = T(0.5);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
+  const int i0 = 2 + blockIdx.x * TI, j0 = 2 + blockIdx.y * TJ;
+  const int kb0 = 2 + blockIdx.z * kc;
+  const int kb1 = min(kb0 + kc - 1, d.kl);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = i <= d.il && j <= d.jl;
+  const bool diss = A.has_diss;
+  const long bz = ix(d, i, j, 0);
+  // k-direction carry (face k) and rings for face k+1
+  T Dk[5], Ek[5], fz[4], dzr[3][5], rz[2], Fz[2][5];
+  if (valid) {
+    int k = kb0;
+    eface<2>(c, bz, k, i, j, k, Ek);
+    cell_flux<2>(c, i, j, k, Fz[0]);
+    cell_flux<2>(c, i, j, k + 1, Fz[1]);
+    if (diss) {
+      dface<2>(c, bz, k, i, j, k, Dk);
+      for (int t = 0; t < 4; t++) fz[t] = fp_at<2>(c, bz, k - 1 + t);
+      for (int t = 0; t < 3; t++) diw_at<2>(c, bz, k + t, dzr[t]);
+      rz[0] = rad_at<2>(c, bz, k);
+      rz[1] = rad_at<2>(c, bz, k + 1);
+    }
+  }
+  for (int k = kb0; k <= kb1; k++) {
+    // ---- fill the plane's i/j quantities (cells within the arrays)
+    {
+      // own cell
+      if (i <= d.ib && j <= d.jb) {
+        long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
+        T o[5];
+        cell_flux<0>(c, i, j, k, o);
+        for (int n = 0; n < 5; n++) sm.fx[n][ty][tx + 1] = o[n];
+        cell_flux<1>(c, i, j, k, o);
+        for (int n = 0; n < 5; n++) sm.fy[n][ty + 1][tx] = o[n];
+        if (diss) {
+          sm.px[ty][tx + 2] = fp_at<0>(c, brow, i);
+          sm.py[ty + 2][tx] = fp_at<1>(c, bcol, j);
+          diw_at<0>(c, brow, i, o);
+          for (int n = 0; n < 5; n++) sm.dx[n][ty][tx + 1] = o[n];
+          diw_at<1>(c, bcol, j, o);
+          for (int n = 0; n < 5; n++) sm.dy[n][ty + 1][tx] = o[n];
+        }
+      }
+      // halo items
+      const int nF = 2 * TJ + 2 * TI;
+      const int nP = diss ? 4 * TJ + 4 * TI : 0;
+      const int nD = diss ? 3 * TJ + 3 * TI : 0;
+      for (int t = tid; t < nF + nP + nD; t += TI * TJ) {
+        T o[5];
+        if (t < 2 * TJ) {  // F_x at i0-1, i0+TI
+          int r = t % TJ, side = t / TJ;
+          int ii = side ? i0 + TI : i0 - 1, jj = j0 + r, col = side ? TI + 1 : 0;
+          if (ii <= d.ib && jj <= d.jb) {
+            cell_flux<0>(c, ii, jj, k, o);
+            for (int n = 0; n < 5; n++) sm.fx[n][r][col] = o[n];
+          }
+        } else if (t < nF) {  // F_y at j0-1, j0+TJ
+          int u = t - 2 * TJ;
+          int col = u % TI, side = u / TI;
+          int ii = i0 + col, jj = side ? j0 + TJ : j0 - 1, row = side ? TJ + 1 : 0;
+          if (ii <= d.ib && jj <= d.jb) {
+            cell_flux<1>(c, ii, jj, k, o);
+            for (int n = 0; n < 5; n++) sm.fy[n][row][col] = o[n];
+          }
+        } else if (t < nF + 4 * TJ) {  // fp_x at i0-2, i0-1, i0+TI, i0+TI+1
+          int u = t - nF;
+          int r = u % TJ, w4 = u / TJ;
+          int off = w4 < 2 ? w4 - 2 : TI + (w4 - 2);
+          int ii = i0 + off, jj = j0 + r, col = off + 2;
+          if (jj <= d.jb) sm.px[r][col] = (ii >= 0 && ii <= d.ib) ? fp_at<0>(c, ix(d, 0, jj, k), ii) : T(0);
+        } else if (t < nF + nP) {  // fp_y at j0-2, j0-1, j0+TJ, j0+TJ+1
+          int u = t - nF - 4 * TJ;
+          int col = u % TI, w4 = u / TI;
+          int off = w4 < 2 ? w4 - 2 : TJ + (w4 - 2);
+          int ii = i0 + col, jj = j0 + off, row = off + 2;
+          if (ii <= d.ib) sm.py[row][col] = (jj >= 0 && jj <= d.jb) ? fp_at<1>(c, ix(d, ii, 0, k), jj) : T(0);
+        } else if (t < nF + nP + 3 * TJ) {  // diw_x at i0-1, i0+TI, i0+TI+1
+          int u = t - nF - nP;
+          int r = u % TJ, w3 = u / TJ;
+          int off = w3 == 0 ? -1 : TI + (w3 - 1);
+          int ii = i0 + off, jj = j0 + r, col = off + 1;
+          if (jj <= d.jb && ii <= d.ib) {
+            diw_at<0>(c, ix(d, 0, jj, k), ii, o);
+            for (int n = 0; n < 5; n++) sm.dx[n][r][col] = o[n];
+          }
+        } else {  // diw_y at j0-1, j0+TJ, j0+TJ+1
+          int u = t - nF - nP - 3 * TJ;
+          int col = u % TI, w3 = u / TI;
+          int off = w3 == 0 ? -1 : TJ + (w3 - 1);
+          int ii = i0 + col, jj = j0 + off, row = off + 1;
+          if (ii <= d.ib && jj <= d.jb) {
+            diw_at<1>(c, ix(d, ii, 0, k), jj, o);
+            for (int n = 0; n < 5; n++) sm.dy[n][row][col] = o[n];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      const long q = ix(d, i, j, k);
+      T f[5];
+      if (A.m > 1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = T(0);
+      }
+      T Dk1[5], Ek1[5];
+      if (diss) {
+        // face coefficients (dflux.f90:88-96), then per component the two
+        // faces of each direction straight from LDS
+        T da2, da4, dal, db2, db4, dbl;
+        const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
+        {
+          T r_m = rad_at<0>(c, brow, i - 1), r_0 = rad_at<0>(c, brow, i), r_p = rad_at<0>(c, brow, i + 1);
+          face_coef(c, sm.px[ty][tx], sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], r_m, r_0, &da2, &da4,
+                    &dal);
+          face_coef(c, sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.px[ty][tx + 4], r_0, r_p, &db2,
+                    &db4, &dbl);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            T d0 = sm.dx[n][ty][tx], d1 = sm.dx[n][ty][tx + 1], d2 = sm.dx[n][ty][tx + 2], d3 = sm.dx[n][ty][tx + 3];
+            T fa = (da2 * d1 - da4 * (d2 - two * d1 + d0)) * dal;
+            T fb = (db2 * d2 - db4 * (d3 - two * d2 + d1)) * dbl;
+            f[n] = f[n] + (fa - fb);
+          }
+        }
+        {
+          T r_m = rad_at<1>(c, bcol, j - 1), r_0 = rad_at<1>(c, bcol, j), r_p = rad_at<1>(c, bcol, j + 1);
+          face_coef(c, sm.py[ty][tx], sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], r_m, r_0, &da2, &da4,
+                    &dal);
+          face_coef(c, sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.py[ty + 4][tx], r_0, r_p, &db2,
+                    &db4, &dbl);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            T d0 = sm.dy[n][ty][tx], d1 = sm.dy[n][ty + 1][tx], d2 = sm.dy[n][ty + 2][tx], d3 = sm.dy[n][ty + 3][tx];
+            T fa = (da2 * d1 - da4 * (d2 - two * d1 + d0)) * dal;
+            T fb = (db2 * d2 - db4 * (d3 - two * d2 + d1)) * dbl;
+            f[n] = f[n] + (fa - fb);
+          }
+        }
+        face_coef(c, fz[0], fz[1], fz[2], fz[3], rz[0], rz[1], &da2, &da4, &dal);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          Dk1[n] = (da2 * dzr[1][n] - da4 * (dzr[2][n] - two * dzr[1][n] + dzr[0][n])) * dal;
+          f[n] = f[n] + (Dk[n] - Dk1[n]);
+        }
+      }
+      T e[5];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        T ea = half * (sm.fx[n][ty][tx + 1] + sm.fx[n][ty][tx]);
+        T eb = half * (sm.fx[n][ty][tx + 2] + sm.fx[n][ty][tx + 1]);
+        e[n] = ea - eb;
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        T ea = half * (sm.fy[n][ty + 1][tx] + sm.fy[n][ty][tx]);
+        T eb = half * (sm.fy[n][ty + 2][tx] + sm.fy[n][ty + 1][tx]);
+        e[n] = e[n] + ea - eb;
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        Ek1[n] = half * (Fz[1][n] + Fz[0][n]);
+        e[n] = e[n] + Ek[n] - Ek1[n];
+      }
+      if (A.flags[q] & F_TOUCHED) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          A.fw[n * E + q] = f[n];
+          A.dw[n * E + q] = e[n];
+        }
+      } else {
+        finish_cell(A, q, f, e);
+      }
+      // advance the k-direction carry to face k+2
+      if (k < kb1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          Ek[n] = Ek1[n];
+          Fz[0][n] = Fz[1][n];
+        }
+        cell_flux<2>(c, i, j, k + 2, Fz[1]);
+        if (diss) {
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            Dk[n] = Dk1[n];
+            dzr[0][n] = dzr[1][n];
+            dzr[1][n] = dzr[2][n];
+          }
+          fz[0] = fz[1];
+          fz[1] = fz[2];
+          fz[2] = fz[3];
+          fz[3] = fp_at<2>(c, bz, k + 3);
+          diw_at<2>(c, bz, k + 3, dzr[2]);
+          rz[0] = rz[1];
+          rz[1] = rad_at<2>(c, bz, k + 2);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ wall records
+
+template <int DIR, typename T>
+__device__ inline void rec_dfs(const FluxCtx<T> &c, int i, int j, int k, int a, T *o) {
+  long base = DIR == 0 ? ix(c.d, 0, j, k) : (DIR == 1 ? ix(c.d, i, 0, k) : ix(c.d, i, j, 0));
+  dface<DIR>(c, base, a, i, j, k, o);
+}
+template <typename T>
+__device__ inline void rec_dfs_any(const FluxCtx<T> &c, int zz, int i, int j, int k, int a, T *o) {
+  if (zz == 1) rec_dfs<0>(c, i, j, k, a, o);
+  else if (zz == 2) rec_dfs<1>(c, i, j, k, a, o);
+  else rec_dfs<2>(c, i, j, k, a, o);
+}
+template <int DIR, typename T>
+__device__ inline void rec_efs(const FluxCtx<T> &c, int i, int j, int k, int a, T *o) {
+  long base = DIR == 0 ? ix(c.d, 0, j, k) : (DIR == 1 ? ix(c.d, i, 0, k) : ix(c.d, i, j, 0));
+  eface<DIR>(c, base, a, i, j, k, o);
+}
+template <typename T>
+__device__ inline void rec_efs_any(const FluxCtx<T> &c, int zz, int i, int j, int k, int a, T *o) {
+  if (zz == 1) rec_efs<0>(c, i, j, k, a, o);
+  else if (zz == 2) rec_efs<1>(c, i, j, k, a, o);
+  else rec_efs<2>(c, i, j, k, a, o);
+}
+
+template <typename T>
+__device__ inline T face_s_any(const Geo<T> &g, int zz, int i, int j, int k) {
+  if (zz == 1) return face_s<0>(g, i, j, k);
+  if (zz == 2) return face_s<1>(g, i, j, k);
+  return face_s<2>(g, i, j, k);
+}
+
+// the two recomputed faces of dfluxb.f90 for record r: fs1 (inner face of
+// the pair) and fs2 (the wall face or the face behind it).  They involve the
+// FP64 mirror state, so they are evaluated in FP64 from the bulk values.
+template <typename T>
+__device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsolv, double psol, double *fs1,
+                           double *fs2) {
+  const Dims &d = c.d;
+  const long E = d.E;
+  int zz = r.zz, id = r.faceid;
+  int i = r.c[0], j = r.c[1], k = r.c[2];
+  int s0 = zz == 1 ? i : (zz == 2 ? j : k);
+  long st = zz == 1 ? 1L : (zz == 2 ? d.sy : d.sz);
+  long base = ix(d, i, j, k) - (long)s0 * st;
+  int nbmax = zz == 1 ? d.ib : (zz == 2 ? d.jb : d.kb);
+  const double vis2 = c.vis2, vis4 = c.vis4, cdis = c.cdis;
+  double wsol[5];
+  for (int n = 0; n < 5; n++) wsol[n] = wsolv[n];
+  wsol[4] = wsol[4] + psol;
+  auto P = [&](int s) { return (double)c.s.p[base + (long)s * st]; };
+  auto H = [&](int s, int n) {
+    long q = base + (long)s * st;
+    return n < 4 ? (double)c.s.w[n * E + q] : (double)c.s.w[4 * E + q] + (double)c.s.p[q];
+  };
+  auto RS = [&](int s) { return (double)c.radsum[base + (long)s * st]; };
+  auto SS = [&](int s, double *ss) {
+    ss[0] = ss[1] = ss[2] = 0.0;
+    int fi = zz == 1 ? s : i, fj = zz == 2 ? s : j, fk = zz == 3 ? s : k;
+    ss[zz - 1] = face_s_any(c.g, zz, fi, fj, fk);
+  };
+  double fp[3], ep, lm, lp, lam, dis2, dis4, diw[3][5];
+  int hi = (id >= 4);
+  int s = s0;
+  if (hi) {
+    if (zz == 1) {
+      fp[0] = fabs(P(s - 2) - 2.0 * P(s - 1) + P(s)) / fabs(P(s - 2) + 2.0 * P(s - 1) + P(s));
+      fp[1] = fabs(P(s - 1) - 2.0 * P(s) + psol) / fabs(P(s - 1) + 2.0 * P(s) + psol);
+    } else {
+      fp[0] = fabs(P(s) - 2.0 * P(s - 1) + P(s - 2)) / fabs(P(s) + 2.0 * P(s - 1) + P(s - 2));
+      fp[1] = fabs(psol - 2.0 * P(s) + P(s - 1)) / fabs(psol + 2.0 * P(s) + P(s - 1));
+    }
+    ep = maxd(fp[0], fp[1]);
+    lm = RS(s);
+    lam = lm;
+    dis2 = vis2 * ep; dis4 = vis4; dis4 = dimd(dis4, dis2);
+    dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+    for (int n = 0; n < 5; n++) {
+      diw[0][n] = H(s, n) - H(s - 1, n);
+      diw[1][n] = wsol[n] - H(s, n);
+      diw[2][n] = diw[1][n];
+      fs2[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+      fs2[n] = dis2 * diw[1][n] - dis4 * fs2[n];
+    }
+    if (c.dissip == 0) {
+      for (int n = 0; n < 5; n++) fs2[n] = fs2[n] * lam;
+    } else {
+      double uc[5], ss[3], o[5];
+      for (int n = 0; n < 5; n++) uc[n] = H(s, n);
+      SS(s + 1, ss);
+      dmatrix((double)c.gamma, uc, wsol, fs2, ss, o);
+      for (int n = 0; n < 5; n++) fs2[n] = o[n];
+    }
+    if ((s - 3) >= 0)
+      fp[2] = fabs(P(s - 1) - 2.0 * P(s - 2) + P(s - 3)) / fabs(P(s - 1) + 2.0 * P(s - 2) + P(s - 3));
+    else
+      fp[2] = 0.0;
+    ep = maxd(maxd(fp[0], fp[1]), fp[2]);
+    lm = RS(s - 1);
+    lp = RS(s);
+    lam = 0.5 * (lm + lp);
+    dis2 = vis2 * ep; dis4 = vis4; dis4 = dimd(dis4, dis2);
+    dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+    for (int n = 0; n < 5; n++) {
+      diw[0][n] = H(s - 1, n) - H(s - 2, n);
+      diw[1][n] = H(s, n) - H(s - 1, n);
+      diw[2][n] = wsol[n] - H(s, n);
+      fs1[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+      fs1[n] = dis2 * diw[1][n] - dis4 * fs1[n];
+    }
+    if (c.dissip == 0) {
+      for (int n = 0; n < 5; n++) fs1[n] = fs1[n] * lam;
+    } else {
+      double um[5], uc[5], ss[3], o[5];
+      for (int n = 0; n < 5; n++) { um[n] = H(s - 1, n); uc[n] = H(s, n); }
+      SS(s, ss);
+      dmatrix((double)c.gamma, um, uc, fs1, ss, o);
+      for (int n = 0; n < 5; n++) fs1[n] = o[n];
+    }
+  } else {
+    fp[0] = fabs(P(s + 1) - 2.0 * P(s) + psol) / fabs(P(s + 1) + 2.0 * P(s) + psol);
+    fp[1] = fabs(P(s + 2) - 2.0 * P(s + 1) + P(s)) / fabs(P(s + 2) + 2.0 * P(s + 1) + P(s));
+    if ((s + 3) <= nbmax)
+      fp[2] = fabs(P(s + 3) - 2.0 * P(s + 2) + P(s + 1)) / fabs(P(s + 3) + 2.0 * P(s + 2) + P(s + 1));
+    else
+      fp[2] = 0.0;
+    ep = maxd(maxd(fp[0], fp[1]), fp[2]);
+    lm = RS(s);
+    lp = RS(s + 1);
+    lam = 0.5 * (lm + lp);
+    dis2 = vis2 * ep; dis4 = vis4; dis4 = dimd(dis4, dis2);
+    dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+    for (int n = 0; n < 5; n++) {
+      diw[0][n] = H(s, n) - wsol[n];
+      diw[1][n] = H(s + 1, n) - H(s, n);
+      diw[2][n] = H(s + 2, n) - H(s + 1, n);
+      fs2[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+      fs2[n] = dis2 * diw[1][n] - dis4 * fs2[n];
+    }
+    if (c.dissip == 0) {
+      for (int n = 0; n < 5; n++) fs2[n] = fs2[n] * lam;
+    } else {
+      double uc[5], up[5], ss[3], o[5];
+      for (int n = 0; n < 5; n++) { uc[n] = H(s, n); up[n] = H(s + 1, n); }
+      SS(s + 1, ss);
+      dmatrix((double)c.gamma, uc, up, fs2, ss, o);
+      for (int n = 0; n < 5; n++) fs2[n] = o[n];
+    }
+    ep = maxd(fp[0], fp[1]);
+    lp = RS(s);
+    lam = lp;
+    dis2 = vis2 * ep; dis4 = vis4; dis4 = dimd(dis4, dis2);
+    dis2 = dis2 * cdis; dis4 = dis4 * cdis;
+    for (int n = 0; n < 5; n++) {
+      diw[1][n] = H(s, n) - wsol[n];
+      diw[0][n] = diw[1][n];
+      diw[2][n] = H(s + 1, n) - H(s, n);
+      fs1[n] = diw[2][n] - 2.0 * diw[1][n] + diw[0][n];
+      fs1[n] = dis2 * diw[1][n] - dis4 * fs1[n];
+    }
+    if (c.dissip == 0) {
+      for (int n = 0; n < 5; n++) fs1[n] = fs1[n] * lam;
+    } else {
+      double uc[5], ss[3], o[5];
+      for (int n = 0; n < 5; n++) uc[n] = H(s, n);
+      SS(s, ss);
+      dmatrix((double)c.gamma, wsol, uc, fs1, ss, o);
+      for (int n = 0; n < 5; n++) fs1[n] = o[n];
+    }
+  }
+}
+
+// the wall flux fs and the opposite-face flux fa of efluxb.f90:235-253,
+// in FP64 like dfluxb_rec
+template <typename T>
+__device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double *fs,
+                           double *fa, int *err) {
+  const Dims &d = c.d;
+  int zz = r.zz, id = r.faceid;
+  int i = r.c[0], j = r.c[1], k = r.c[2];
+  long qc = ix(d, i, j, k);
+  double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
+  sc[zz - 1] = face_s_any(c.g, zz, i, j, k);
+  ss[zz - 1] = face_s_any(c.g, zz, r.s[0], r.s[1], r.s[2]);
+  double wc[5];
+  for (int n = 0; n < 5; n++) wc[n] = c.s.w[n * d.E + qc];
+  if (fabs(wc[0]) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
+  double fs1[5], fs2[5], fs3[5];
+  cflux3(wc, (double)c.s.p[qc], sc, fs1);
+  cflux3(wsol, psol, ss, fs2);
+  for (int n = 0; n < 5; n++) fs[n] = 0.5 * (fs1[n] + fs2[n]);
+  int hi = (id >= 4);
+  int oi = i, oj = j, ok = k;
+  int dlt = hi ? -1 : 1;
+  if (zz == 1) oi += dlt;
+  else if (zz == 2) oj += dlt;
+  else ok += dlt;
+  long qo = ix(d, oi, oj, ok);
+  double wo[5], so[3] = {0, 0, 0};
+  for (int n = 0; n < 5; n++) wo[n] = c.s.w[n * d.E + qo];
+  so[zz - 1] = face_s_any(c.g, zz, oi, oj, ok);
+  cflux3(wo, (double)c.s.p[qo], so, fs3);
+  for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
+}
+
+// the value each op contributes (dfluxb.f90 / efluxb.f90 terms), one
+// thread per op; k_rec_apply adds them in the reference order.  The undo
+// terms repeat the bulk kernel's face values in T.
+template <typename T>
+__global__ void __launch_bounds__(64) k_rec_ops(RecArgs<T> R) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= R.nops) return;
+  const StageArgs<T> &A = R.st;
+  int2 op = R.ops[o];
+  double *out = R.opval + (long)o * 10;
+  const DevRec &r = R.recs[op.x];
+  if (op.y < OP_E_UNDO && !A.has_diss) return;
+  int zz = r.zz;
+  int s0 = zz == 1 ? r.c[0] : (zz == 2 ? r.c[1] : r.c[2]);
+  double ws[5];
+  for (int n = 0; n < 5; n++) ws[n] = R.wsol[n * R.nmir + op.x];
+  double ps = R.psol[op.x];
+  T a[5], b[5];
+  double fs1[5], fs2[5];
+  int ci = r.c[0], cj = r.c[1], ck = r.c[2];
+  switch (op.y) {
+    case OP_HI_C_UNDO:
+    case OP_LO_C_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_HI_N_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_LO_N_UNDO:
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 1, a);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_HI_C_ADD:
+    case OP_LO_C_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      for (int n = 0; n < 5; n++) out[n] = fs1[n] - fs2[n];
+      break;
+    case OP_HI_N_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 - 1, a);
+      for (int n = 0; n < 5; n++) out[n] = (double)a[n] - fs1[n];
+      break;
+    case OP_LO_N_ADD:
+      dfluxb_rec(A.fx, r, ws, ps, fs1, fs2);
+      rec_dfs_any(A.fx, zz, ci, cj, ck, s0 + 2, a);
+      for (int n = 0; n < 5; n++) out[n] = fs2[n] - (double)a[n];
+      break;
+    case OP_E_UNDO:
+      rec_efs_any(A.fx, zz, ci, cj, ck, s0, a);
+      rec_efs_any(A.fx, zz, ci, cj, ck, s0 + 1, b);
+      for (int n = 0; n < 5; n++) out[n] = a[n] - b[n];
+      break;
+    case OP_E_ADD: {
+      double fs[5], fa[5];
+      efluxb_rec(A.fx, r, ws, ps, fs, fa, R.err);
+      bool hi = r.faceid >= 4;
+      for (int n = 0; n < 5; n++) {
+        out[n] = hi ? fa[n] : fs[n];
+        out[5 + n] = hi ? fs[n] : fa[n];
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// per touched cell: apply the op values in record order (in FP64), then
+// finish in T
+template <typename T>
+__global__ void __launch_bounds__(64) k_rec_apply(RecArgs<T> R) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R.ntouch) return;
+  const StageArgs<T> &A = R.st;
+  const Dims &d = A.fx.d;
+  const long E = d.E;
+  long q = R.tcell[t];
+  double f[5], e[5];
+  for (int n = 0; n < 5; n++) {
+    f[n] = A.fw[n * E + q];
+    e[n] = A.dw[n * E + q];
+  }
+  for (int o = R.beg[t]; o < R.beg[t + 1]; o++) {
+    int kind = R.ops[o].y;
+    const double *v = R.opval + (long)o * 10;
+    switch (kind) {
+      case OP_HI_C_UNDO:
+      case OP_HI_N_UNDO:
+      case OP_LO_C_UNDO:
+      case OP_LO_N_UNDO:
+        if (A.has_diss)
+          for (int n = 0; n < 5; n++) f[n] = f[n] - v[n];
+        break;
+      case OP_HI_C_ADD:
+      case OP_HI_N_ADD:
+      case OP_LO_C_ADD:
+      case OP_LO_N_ADD:
+        if (A.has_diss)
+          for (int n = 0; n < 5; n++) f[n] = f[n] + v[n];
+        break;
+      case OP_E_UNDO:
+        for (int n = 0; n < 5; n++) e[n] = e[n] - v[n];
+        break;
+      case OP_E_ADD:
+        for (int n = 0; n < 5; n++) e[n] = e[n] + v[n] - v[5 + n];
+        break;
+      default:
+        break;
+    }
+  }
+  T ft[5], et[5];
+  for (int n = 0; n < 5; n++) {
+    ft[n] = (T)f[n];
+    et[n] = (T)e[n];
+  }
+  finish_cell(A, q, ft, et);
+}
+
+}  // namespace mge

@@ -30,7 +30,8 @@
  *  - Levels are numbered 1..nlevel with 1 the coarsest (the reference's
  *    mgl); each level is one Cartesian block (blk = 1).
  *  - Field arrays crossing the ABI are Fortran-ordered (0:ib,0:jb,0:kb[,5])
- *    with ib = ni+3: i fastest, component slowest, FP64.
+ *    with ib = ni+3: i fastest, component slowest, FP64 (an FP32 case
+ *    converts on download/upload).
  */
 #ifndef MGE_H
 #define MGE_H
@@ -73,7 +74,10 @@ typedef struct {
   double viscl, prlmean, gas_const;
   double drop;                /* solve exits when ini_res/res > drop (1e6 in the reference) */
   int    distype;             /* 0 central (JST)                                   */
-  int    precision;           /* 64: FP64 throughout                               */
+  int    precision;           /* 64 (or 0): FP64 throughout.  32: FP32 bulk state,
+                                 fluxes, smoothing and updates; the wall mirror
+                                 states, residual norms, global time step and
+                                 force integral stay FP64                          */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record
