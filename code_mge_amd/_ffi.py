@@ -41,6 +41,7 @@ class MgeParams(C.Structure):
         ("drop", C.c_double),
         ("distype", C.c_int),
         ("precision", C.c_int),
+        ("epsentr", C.c_double),
     ]
 
 

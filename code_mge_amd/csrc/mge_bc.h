@@ -20,6 +20,7 @@ struct TsArgs {
   T gamma, cfl, cellvolume;
   int eqntype;
   T viscl, prlmean;
+  T cfac;  // viscous factor of the scheme (timestep.f90:119-124)
   int *err;
 };
 
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(256) k_timestep(TsArgs<T> A) {
     cs = cc * sqrt(ds);
     T rk = qs + cs, rvk = dtv * ds;
     T fict;
-    if (A.eqntype == 1) fict = A.cellvolume / (ri + rj + rk + T(4) * (rvi + rvj + rvk));
+    if (A.eqntype == 1) fict = A.cellvolume / (ri + rj + rk + A.cfac * (rvi + rvj + rvk));
     else fict = A.cellvolume / (ri + rj + rk);
     const T plim = T(0.001);
     const T *P = A.s.p;
@@ -570,15 +571,26 @@ __global__ void __launch_bounds__(256) k_prolong(XferArgs<T> X) {
   }
 }
 
-// wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253)
+// wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253
+// for JST; the full Roe wall flux, central + upwind, for the Roe schemes)
 // in FP64; one thread per record writes its term, one thread adds them in
 // record order
 template <typename T>
-__global__ void k_wall_terms(Dims d, StateC<T> s, Geo<T> g, const DevRec *recs, const double *wsol,
-                             const double *psol, int nmir, double *terms) {
+__global__ void k_wall_terms(FluxCtx<T> fc, const DevRec *recs, const double *wsol, const double *psol, int nmir,
+                             int roe, double gamma, double eps, double *terms, int *err) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nmir) return;
+  const Dims &d = fc.d;
+  const StateC<T> &s = fc.s;
+  const Geo<T> &g = fc.g;
   const DevRec &r = recs[m];
+  if (roe) {
+    double ws5[5], ve[5], vd[5];
+    for (int n = 0; n < 5; n++) ws5[n] = wsol[n * nmir + m];
+    roe_wall_rec(fc, r, ws5, psol[m], gamma, eps, 1.0, ve, vd, err);
+    for (int dd = 0; dd < 3; dd++) terms[3 * m + dd] = ve[1 + dd] + vd[1 + dd];
+    return;
+  }
   int zz = r.zz;
   double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
   sc[zz - 1] = face_s_any(g, zz, r.c[0], r.c[1], r.c[2]);

@@ -1,8 +1,9 @@
 // mge_device.hip -- host side of the MI355X solve step and the C ABI of
-// include/mge.h.  The kernels live in mge_stage.h (RK stage, wall records),
-// mge_smooth.h (residual smoothing) and mge_bc.h (time step, norms,
-// boundary conditions, transfers, forces); all are templated on the bulk
-// type T and instantiated for double (FP64 case) and float (FP32 case).
+// include/mge.h.  The kernels live in mge_stage.h (JST stage, wall records),
+// mge_roe.h (Roe / MUSCL stage), mge_smooth.h (residual smoothing) and
+// mge_bc.h (time step, norms, boundary conditions, transfers, forces); all
+// are templated on the bulk type T and instantiated for double (FP64 case)
+// and float (FP32 case).
 //
 // One Cartesian block per multigrid level lives in HBM for the whole solve:
 //   S[3]     three state buffers (w: 5 components, then p), rotated across
@@ -27,6 +28,7 @@
 
 #include "../../include/mge.h"
 #include "mge_bc.h"
+#include "mge_roe.h"
 #include "mge_smooth.h"
 #include "mge_stage.h"
 
@@ -329,6 +331,7 @@ int do_timestep(mge_case *h, Level &L) {
   A.eqntype = P.eqntype;
   A.viscl = (T)P.viscl;
   A.prlmean = (T)P.prlmean;
+  A.cfac = (T)(P.distype == 0 ? 4.0 : (P.distype == 2 ? 1.0 : 2.0));
   A.err = h->err;
   LAUNCH(h, "k_timestep", k_timestep<T>, cell_grid(L.d), cell_block(), A);
   LAUNCH(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(1), L.dtim_bits, L.dtim);
@@ -384,6 +387,8 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     A.smoothed = (m % 2 != 0) && P.smoo > 0.0;
     A.store_fw = (m < P.mstage) && (A.has_diss || m == 1 || A.blend != T(1));
     A.vt = P.vt;
+    A.order = P.distype == 2 ? 2 : 1;
+    A.eps = (T)P.epsentr;
     {
       // algorithmic bytes per cell: read W^(m-1); write W^(m) and read W^(0)
       // when this kernel updates, else write dw; forcing on coarse levels;
@@ -400,7 +405,8 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       // enough workgroups to fill 256 CUs: split k into chunks
       while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
       dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
-      if (P.dissip == 0) LAUNCH(h, "k_stage", k_stage2<T>, g, dim3(TI, TJ), A, kc);
+      if (P.distype != 0) LAUNCH(h, "k_stage", k_stage_roe<T>, g, dim3(TI, TJ), A, kc);
+      else if (P.dissip == 0) LAUNCH(h, "k_stage", k_stage2<T>, g, dim3(TI, TJ), A, kc);
       else LAUNCH(h, "k_stage", k_stage<T>, cell_grid(d), cell_block(), A);  // matrix dissipation
     }
     if (L.ntouch > 0) {
@@ -640,15 +646,21 @@ int do_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, l
 
 template <class T>
 int do_wall_force(mge_case *h, Level &L) {
-  LAUNCH(h, "k_wall_terms", k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), L.d, stc<T>(L, L.cur), geo<T>(L),
-         L.recs, L.wsol, L.psol, L.nmir, L.opval);
+  FluxCtx<T> fc{};
+  fc.d = L.d;
+  fc.s = stc<T>(L, L.cur);
+  fc.g = geo<T>(L);
+  LAUNCH(h, "k_wall_terms", k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), fc, L.recs, L.wsol, L.psol,
+         L.nmir, h->prm.distype != 0 ? 1 : 0, h->prm.gamma, h->prm.epsentr, L.opval, h->err);
   LAUNCH(h, "k_wall_sum", k_wall_sum, dim3(1), dim3(64), (const double *)L.opval, L.nmir, L.force);
   return MGE_OK;
 }
 
-// host-side record bookkeeping: touched cells and their ordered op lists
+// host-side record bookkeeping: touched cells and their ordered op lists.
+// JST: the dfluxb/efluxb undo/add terms on the cut cell and its inner
+// neighbour; Roe: one wall-face term on the cut cell.
 int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
-  (void)h;
+  const bool roe = h->prm.distype != 0;
   const Dims &d = L.d;
   L.nmir = nmir;
   L.hrecs.assign(recs, recs + nmir);
@@ -701,6 +713,10 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
     bool hi = r.faceid >= 4;
     long qn = hi ? qc - st : qc + st;
     auto &pc = per[qc];
+    if (roe) {
+      pc.first.push_back(make_int2(m, OP_ROE_WALL));
+      continue;
+    }
     auto &pn = per[qn];
     if (hi) {
       pc.first.push_back(make_int2(m, OP_HI_C_UNDO));
@@ -775,7 +791,7 @@ const char *mge_last_error(void) { return g_err.c_str(); }
 int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
   if (!prm || !out || nlevel < 1 || nlevel > 16) return set_err(MGE_ERR_ARG, "bad case arguments");
   if (prm->mstage < 1 || prm->mstage > 8) return set_err(MGE_ERR_ARG, "mstage must be 1..8");
-  if (prm->distype != 0) return set_err(MGE_ERR_DISTYPE, "only the central (JST) scheme is available");
+  if (prm->distype < 0 || prm->distype > 2) return set_err(MGE_ERR_DISTYPE, "distype must be c (0), r1 (1) or r2 (2)");
   if (prm->dissip != 0 && prm->dissip != 1) return set_err(MGE_ERR_DISTYPE, "dissip must be scalar or matrix");
   if (prm->precision != 64 && prm->precision != 32 && prm->precision != 0)
     return set_err(MGE_ERR_ARG, "precision must be 64 or 32");

@@ -1,0 +1,458 @@
+// mge_roe.h -- Roe flux with MUSCL-minmod reconstruction (first or second
+// order), the stage kernel that applies it and its wall-record term.
+//
+// The reference calls var_diffs / dflux_roe2 / eflux_roe2 and dflux_roe1 /
+// eflux_roe1 from euler.f90:201-210; the roepak module has no source in the
+// reference tree, so the scheme is written from the mathematics:
+//  - face states from primitive variables q = (rho,u,v,w,p); second order
+//    q_L = q(a-1) + minmod(dq(a-1), dq(a))/2, q_R = q(a) - minmod(dq(a),
+//    dq(a+1))/2, with a slope that would reach a solid cell set to zero;
+//  - central part 0.5*(F_S(U_L) + F_S(U_R)) (accumulated like eflux into dw);
+//  - upwind part cdis*0.5*|A_roe(S)|(U_R - U_L) from Roe's averages and
+//    three wave families with Harten's entropy fix (accumulated like dflux
+//    into fw, so it is blended across stages as the JST dissipation is);
+//  - no bulk flux through a face between a fluid and a solid cell: each
+//    wall record adds the Roe flux between its cut cell (first order) and
+//    its FP64 mirror state.
+// The face vector S of the reference points from cell a to cell a-1, so
+// e + d is minus the outward flux through the face times |S|.
+#pragma once
+#include "mge_stage.h"
+
+namespace mge {
+
+// minmod(a, b): 0 when a*b <= 0, else the argument of smaller magnitude
+// (written as value selects so no stack slot is needed)
+template <typename T>
+__device__ inline T minmod(T a, T b) {
+  T m = fabs(a) < fabs(b) ? a : b;
+  return a * b <= T(0) ? T(0) : m;
+}
+
+template <typename T>
+__device__ inline void prim_of(const StateC<T> &s, long E, long q, T *o) {
+  T r = s.w[q];
+  o[0] = r;
+  o[1] = s.w[E + q] / r;
+  o[2] = s.w[2 * E + q] / r;
+  o[3] = s.w[3 * E + q] / r;
+  o[4] = s.p[q];
+}
+
+template <typename T>
+__device__ inline void cons_of(T gamma, const T *q, T *u) {
+  T q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  u[0] = q[0];
+  u[1] = q[0] * q[1];
+  u[2] = q[0] * q[2];
+  u[3] = q[0] * q[3];
+  u[4] = q[4] / (gamma - T(1)) + T(0.5) * q[0] * q2;
+}
+
+template <typename T>
+__device__ inline T fix_abs(T l, T del) {
+  T a = fabs(l);
+  if (a < del) a = (l * l + del * del) / (T(2) * del);
+  return a;
+}
+
+// face states of one face from the primitives of cells a-2..a+1 and the
+// fluid flags of the outer two
+template <typename T>
+__device__ inline void muscl(int order, const T *qm2, const T *qm1, const T *qp0, const T *qp1, bool fm2, bool fp1,
+                             T *qL, T *qR) {
+  if (order < 2) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      qL[n] = qm1[n];
+      qR[n] = qp0[n];
+    }
+    return;
+  }
+#pragma unroll
+  for (int n = 0; n < 5; n++) {
+    T sl = fm2 ? minmod(qm1[n] - qm2[n], qp0[n] - qm1[n]) : T(0);
+    T sr = fp1 ? minmod(qp0[n] - qm1[n], qp1[n] - qp0[n]) : T(0);
+    qL[n] = qm1[n] + T(0.5) * sl;
+    qR[n] = qp0[n] - T(0.5) * sr;
+  }
+}
+
+// one Roe face whose face vector has the single non-zero component S in
+// direction DIR (axis-aligned blocks); e central part, d upwind part (only
+// when diss)
+template <int DIR, typename T>
+__device__ inline void roe_face(T gamma, T eps, T cdis, bool diss, const T *qL, const T *qR, T S, T *e, T *d) {
+  T UL[5], UR[5], FL[5], FR[5];
+  cons_of(gamma, qL, UL);
+  cons_of(gamma, qR, UR);
+  cflux<DIR>(UL, qL[4], S, FL);
+  cflux<DIR>(UR, qR[4], S, FR);
+#pragma unroll
+  for (int n = 0; n < 5; n++) e[n] = T(0.5) * (FL[n] + FR[n]);
+  if (!diss) return;
+  T sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  T rh = sL * sR;
+  T uh[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
+  T HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
+  T Hh = (sL * HL + sR * HR) / den;
+  T q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
+  T c2 = (gamma - T(1)) * (Hh - T(0.5) * q2h);
+  if (c2 < T(1.0e-12)) c2 = T(1.0e-12);
+  T c = sqrt(c2);
+  T A = sqrt(S * S);
+  T nd = S / A;
+  T un = uh[DIR] * nd;
+  T l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
+  T del = eps * (fabs(un) + c) * A;
+  T a1 = fix_abs(l1, del), a2 = fix_abs(l2, del), a3 = fix_abs(l3, del);
+  T dr = qR[0] - qL[0], dp = qR[4] - qL[4];
+  T du[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) du[m] = qR[1 + m] - qL[1 + m];
+  T dun = du[DIR] * nd;
+  T al1 = (dp - rh * c * dun) / (T(2) * c2);
+  T al3 = (dp + rh * c * dun) / (T(2) * c2);
+  T al2 = dr - dp / c2;
+  T dut[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) dut[m] = du[m];
+  dut[DIR] = du[DIR] - dun * nd;
+  T udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
+  T dd[5];
+  dd[0] = a1 * al1 + a2 * al2 + a3 * al3;
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    if (m == DIR)
+      dd[1 + m] = a1 * al1 * (uh[m] - c * nd) + a2 * (al2 * uh[m] + rh * dut[m]) + a3 * al3 * (uh[m] + c * nd);
+    else
+      dd[1 + m] = a1 * al1 * uh[m] + a2 * (al2 * uh[m] + rh * dut[m]) + a3 * al3 * uh[m];
+  }
+  dd[4] = a1 * al1 * (Hh - c * un) + a2 * (al2 * T(0.5) * q2h + rh * udt) + a3 * al3 * (Hh + c * un);
+#pragma unroll
+  for (int n = 0; n < 5; n++) d[n] = cdis * (T(0.5) * dd[n]);
+}
+
+// the same face for a general face vector S (FP64 wall faces)
+__device__ inline void roe_face_vec(double gamma, double eps, double cdis, const double *qL, const double *qR,
+                                    const double *S, double *e, double *d) {
+  double UL[5], UR[5], FL[5], FR[5];
+  cons_of(gamma, qL, UL);
+  cons_of(gamma, qR, UR);
+  cflux3(UL, qL[4], S, FL);
+  cflux3(UR, qR[4], S, FR);
+  for (int n = 0; n < 5; n++) e[n] = 0.5 * (FL[n] + FR[n]);
+  double sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  double rh = sL * sR;
+  double uh[3];
+  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
+  double HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
+  double Hh = (sL * HL + sR * HR) / den;
+  double q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
+  double c2 = (gamma - 1.0) * (Hh - 0.5 * q2h);
+  if (c2 < 1.0e-12) c2 = 1.0e-12;
+  double c = sqrt(c2);
+  double A = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  double nv[3] = {S[0] / A, S[1] / A, S[2] / A};
+  double un = uh[0] * nv[0] + uh[1] * nv[1] + uh[2] * nv[2];
+  double l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
+  double del = eps * (fabs(un) + c) * A;
+  double a1 = fix_abs(l1, del), a2 = fix_abs(l2, del), a3 = fix_abs(l3, del);
+  double dr = qR[0] - qL[0], dp = qR[4] - qL[4];
+  double du[3] = {qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
+  double dun = du[0] * nv[0] + du[1] * nv[1] + du[2] * nv[2];
+  double al1 = (dp - rh * c * dun) / (2.0 * c2);
+  double al3 = (dp + rh * c * dun) / (2.0 * c2);
+  double al2 = dr - dp / c2;
+  double dut[3];
+  for (int m = 0; m < 3; m++) dut[m] = du[m] - dun * nv[m];
+  double udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
+  double dd[5];
+  dd[0] = a1 * al1 + a2 * al2 + a3 * al3;
+  for (int m = 0; m < 3; m++)
+    dd[1 + m] = a1 * al1 * (uh[m] - c * nv[m]) + a2 * (al2 * uh[m] + rh * dut[m]) + a3 * al3 * (uh[m] + c * nv[m]);
+  dd[4] = a1 * al1 * (Hh - c * un) + a2 * (al2 * 0.5 * q2h + rh * udt) + a3 * al3 * (Hh + c * un);
+  for (int n = 0; n < 5; n++) d[n] = cdis * (0.5 * dd[n]);
+}
+
+// wall face of record r: the signed contributions (ve, vd) to the cut cell,
+// in FP64 from the bulk state and the mirror state
+template <typename T>
+__device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
+                             double eps, double cdis, double *ve, double *vd, int *err) {
+  const Dims &d = c.d;
+  int zz = r.zz;
+  bool hi = r.faceid >= 4;
+  long qc = ix(d, r.c[0], r.c[1], r.c[2]);
+  if (fabs((double)c.s.w[qc]) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
+  int fi = r.c[0], fj = r.c[1], fk = r.c[2];
+  if (hi) {
+    if (zz == 1) fi++;
+    else if (zz == 2) fj++;
+    else fk++;
+  }
+  double qcv[5], qm[5], S[3] = {0.0, 0.0, 0.0}, e[5], dv[5];
+  double rc = c.s.w[qc];
+  qcv[0] = rc;
+  qcv[1] = (double)c.s.w[d.E + qc] / rc;
+  qcv[2] = (double)c.s.w[2 * d.E + qc] / rc;
+  qcv[3] = (double)c.s.w[3 * d.E + qc] / rc;
+  qcv[4] = c.s.p[qc];
+  qm[0] = wsol[0];
+  qm[1] = wsol[1] / wsol[0];
+  qm[2] = wsol[2] / wsol[0];
+  qm[3] = wsol[3] / wsol[0];
+  qm[4] = psol;
+  S[zz - 1] = face_s_any(c.g, zz, fi, fj, fk);
+  if (hi) roe_face_vec(gamma, eps, cdis, qcv, qm, S, e, dv);
+  else roe_face_vec(gamma, eps, cdis, qm, qcv, S, e, dv);
+  double sgn = hi ? -1.0 : 1.0;
+  for (int n = 0; n < 5; n++) {
+    ve[n] = sgn * e[n];
+    vd[n] = sgn * dv[n];
+  }
+}
+
+// ------------------------------------------------------------------ k_stage_roe
+//
+// Per k plane, the workgroup's TI x TJ tile plus a two-cell halo is
+// converted to primitive variables in LDS; every i and j face of the tile
+// is evaluated once (each cell its low face, plus the faces at the tile's
+// high edges) into LDS; k faces are carried from plane to plane in
+// registers with the primitives of the cell column k-1..k+2.
+template <typename T>
+struct RoeSmem {
+  T q[5][TJ + 4][TI + 4];
+  uint8_t fl[TJ + 4][TI + 4];
+  T xe[5][TJ][TI + 1], xd[5][TJ][TI + 1];
+  T ye[5][TJ + 1][TI], yd[5][TJ + 1][TI];
+};
+
+template <typename T>
+__device__ inline void roe_load(const FluxCtx<T> &c, const uint8_t *flags, long q, T *o, bool *f) {
+  prim_of(c.s, c.d.E, q, o);
+  *f = (flags[q] & F_FLUID) != 0;
+}
+
+// the face between two cells of the LDS tile at (row, col-1) | (row, col)
+// for DIR 0, or (row-1, col) | (row, col) for DIR 1
+template <int DIR, typename T>
+__device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem<T> &sm, const StageArgs<T> &A, int row, int col, T S, T *e,
+                                    T *dd) {
+  const bool diss = A.has_diss;
+  bool f[4];
+  T v[4][5];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    int r = DIR == 1 ? row + t - 2 : row, cc = DIR == 0 ? col + t - 2 : col;
+    f[t] = sm.fl[r][cc] != 0;
+#pragma unroll
+    for (int n = 0; n < 5; n++) v[t][n] = sm.q[n][r][cc];
+  }
+  if (!(f[1] && f[2])) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      e[n] = T(0);
+      dd[n] = T(0);
+    }
+    return;
+  }
+  T qL[5], qR[5];
+  muscl(A.order, v[0], v[1], v[2], v[3], f[0], f[3], qL, qR);
+  roe_face<DIR>(A.fx.gamma, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
+  if (!diss)
+#pragma unroll
+    for (int n = 0; n < 5; n++) dd[n] = T(0);
+}
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs<T> &A, T (*zq)[5], const bool *zf, T S, T *e, T *dd) {
+  const bool diss = A.has_diss;
+  if (!(zf[1] && zf[2])) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      e[n] = T(0);
+      dd[n] = T(0);
+    }
+    return;
+  }
+  T qL[5], qR[5];
+  muscl(A.order, zq[0], zq[1], zq[2], zq[3], zf[0], zf[3], qL, qR);
+  roe_face<2>(A.fx.gamma, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
+  if (!diss)
+#pragma unroll
+    for (int n = 0; n < 5; n++) dd[n] = T(0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256, 2) k_stage_roe(StageArgs<T> A, int kc) {
+  __shared__ RoeSmem<T> sm;
+  const FluxCtx<T> &c = A.fx;
+  const Dims &d = c.d;
+  const long E = d.E;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
+  const int i0 = 2 + blockIdx.x * TI, j0 = 2 + blockIdx.y * TJ;
+  const int kb0 = 2 + blockIdx.z * kc;
+  const int kb1 = min(kb0 + kc - 1, d.kl);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = i <= d.il && j <= d.jl;
+  const bool inarr = i <= d.ib && j <= d.jb;
+  const bool diss = A.has_diss;
+  const T Sz = face_s<2>(c.g, i, j, 0);
+  // column carry: primitives of k-1..k+2 and the bottom face of plane k
+  T zq[4][5], Ez[5], Dz[5];
+  bool zf[4];
+  if (valid) {
+    const long bz = ix(d, i, j, 0);
+    roe_load(c, A.flags, bz + (long)(kb0 - 2) * d.sz, zq[0], &zf[0]);
+    roe_load(c, A.flags, bz + (long)(kb0 - 1) * d.sz, zq[1], &zf[1]);
+    roe_load(c, A.flags, bz + (long)kb0 * d.sz, zq[2], &zf[2]);
+    roe_load(c, A.flags, bz + (long)(kb0 + 1) * d.sz, zq[3], &zf[3]);
+    roe_z_face(A, zq, zf, Sz, Ez, Dz);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) zq[t][n] = zq[t + 1][n];
+      zf[t] = zf[t + 1];
+    }
+    roe_load(c, A.flags, bz + (long)(kb0 + 2) * d.sz, zq[3], &zf[3]);
+  }
+  for (int k = kb0; k <= kb1; k++) {
+    // ---- primitives of the plane: own cell, then the halo ring
+    if (inarr) {
+      T o[5];
+      bool f;
+      if (valid) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) o[n] = zq[1][n];
+        f = zf[1];
+      } else {
+        roe_load(c, A.flags, ix(d, i, j, k), o, &f);
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) sm.q[n][ty + 2][tx + 2] = o[n];
+      sm.fl[ty + 2][tx + 2] = f;
+    }
+    for (int t = tid; t < 4 * (TI + 4) + 4 * TJ; t += TI * TJ) {
+      int r, col;
+      if (t < 4 * (TI + 4)) {
+        int rr = t / (TI + 4);
+        col = t % (TI + 4);
+        r = rr < 2 ? rr : TJ + rr;
+      } else {
+        int u = t - 4 * (TI + 4);
+        int cc = u / TJ;
+        r = 2 + u % TJ;
+        col = cc < 2 ? cc : TI + cc;
+      }
+      int ii = i0 - 2 + col, jj = j0 - 2 + r;
+      if (ii <= d.ib && jj <= d.jb) {
+        T o[5];
+        bool f;
+        roe_load(c, A.flags, ix(d, ii, jj, k), o, &f);
+#pragma unroll
+        for (int n = 0; n < 5; n++) sm.q[n][r][col] = o[n];
+        sm.fl[r][col] = f;
+      }
+    }
+    __syncthreads();
+    // ---- faces: own i face and j face, then the tile's high-edge faces
+    {
+      T e[5], dd[5];
+      if (j <= d.jl && i <= d.ie) {
+        roe_lds_face<0>(sm, A, ty + 2, tx + 2, face_s<0>(c.g, i, j, k), e, dd);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          sm.xe[n][ty][tx] = e[n];
+          sm.xd[n][ty][tx] = dd[n];
+        }
+      }
+      if (i <= d.il && j <= d.je) {
+        roe_lds_face<1>(sm, A, ty + 2, tx + 2, face_s<1>(c.g, i, j, k), e, dd);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          sm.ye[n][ty][tx] = e[n];
+          sm.yd[n][ty][tx] = dd[n];
+        }
+      }
+      if (tid < TJ) {
+        int jj = j0 + tid, a = i0 + TI;
+        if (jj <= d.jl && a <= d.ie) {
+          roe_lds_face<0>(sm, A, tid + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            sm.xe[n][tid][TI] = e[n];
+            sm.xd[n][tid][TI] = dd[n];
+          }
+        }
+      } else if (tid < TJ + TI) {
+        int cidx = tid - TJ, ii = i0 + cidx, a = j0 + TJ;
+        if (ii <= d.il && a <= d.je) {
+          roe_lds_face<1>(sm, A, TJ + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd);
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            sm.ye[n][TJ][cidx] = e[n];
+            sm.yd[n][TJ][cidx] = dd[n];
+          }
+        }
+      }
+    }
+    T Ez1[5], Dz1[5];
+    if (valid) roe_z_face(A, zq, zf, Sz, Ez1, Dz1);
+    __syncthreads();
+    // ---- accumulate and finish
+    if (valid) {
+      const long q = ix(d, i, j, k);
+      T f[5], e[5];
+      if (A.m > 1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = T(0);
+      }
+      if (diss) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          f[n] = f[n] + (sm.xd[n][ty][tx] - sm.xd[n][ty][tx + 1]);
+          f[n] = f[n] + (sm.yd[n][ty][tx] - sm.yd[n][ty + 1][tx]);
+          f[n] = f[n] + (Dz[n] - Dz1[n]);
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        e[n] = sm.xe[n][ty][tx] - sm.xe[n][ty][tx + 1];
+        e[n] = e[n] + (sm.ye[n][ty][tx] - sm.ye[n][ty + 1][tx]);
+        e[n] = e[n] + (Ez[n] - Ez1[n]);
+      }
+      if (A.flags[q] & F_TOUCHED) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          A.fw[n * E + q] = f[n];
+          A.dw[n * E + q] = e[n];
+        }
+      } else {
+        finish_cell(A, q, f, e);
+      }
+      // advance the column carry
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        Ez[n] = Ez1[n];
+        Dz[n] = Dz1[n];
+      }
+      if (k < kb1) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+#pragma unroll
+          for (int n = 0; n < 5; n++) zq[t][n] = zq[t + 1][n];
+          zf[t] = zf[t + 1];
+        }
+        roe_load(c, A.flags, ix(d, i, j, k + 3), zq[3], &zf[3]);
+      }
+    }
+  }
+}
+
+}  // namespace mge

@@ -38,7 +38,9 @@ enum OpKind : int {
   OP_LO_C_ADD,       // fw(s)   += fs1 - fs2                dfluxb.f90:170
   OP_LO_N_ADD,       // fw(s+1) += fs2 - dfs(s+2)           dfluxb.f90:171
   OP_E_UNDO,         // dw(s)   -= efs(s) - efs(s+1)        efluxb.f90:257
-  OP_E_ADD           // dw(s)   += fa - fs  or  fs - fa     efluxb.f90:262,266
+  OP_E_ADD,          // dw(s)   += fa - fs  or  fs - fa     efluxb.f90:262,266
+  OP_ROE_WALL        // Roe schemes: fw(s) += upwind part, dw(s) += central
+                     // part of the wall face between s and its mirror state
 };
 
 template <typename T>
@@ -64,6 +66,8 @@ struct StageArgs {
   int smoothed;             // psmoo_old follows (odd stage, smoo > 0)
   int store_fw;             // a later stage reads fw
   int vt;
+  int order;                // Roe schemes: 1 first order, 2 MUSCL-minmod
+  T eps;                    // Roe entropy-fix coefficient
 };
 
 template <typename T>
@@ -80,6 +84,11 @@ struct RecArgs {
   int ntouch, nops;
   int *err;
 };
+
+// the Roe wall-face term of a record (defined with the Roe scheme)
+template <typename T>
+__device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
+                             double eps, double cdis, double *ve, double *vd, int *err);
 
 // ------------------------------------------------------------------ finish of a cell
 
@@ -733,6 +742,15 @@ __global__ void __launch_bounds__(64) k_rec_ops(RecArgs<T> R) {
       }
       break;
     }
+    case OP_ROE_WALL: {
+      double ve[5], vd[5];
+      roe_wall_rec(A.fx, r, ws, ps, (double)A.fx.gamma, (double)A.eps, (double)A.fx.cdis, ve, vd, R.err);
+      for (int n = 0; n < 5; n++) {
+        out[n] = vd[n];
+        out[5 + n] = ve[n];
+      }
+      break;
+    }
     default:
       break;
   }
@@ -776,6 +794,12 @@ __global__ void __launch_bounds__(64) k_rec_apply(RecArgs<T> R) {
         break;
       case OP_E_ADD:
         for (int n = 0; n < 5; n++) e[n] = e[n] + v[n] - v[5 + n];
+        break;
+      case OP_ROE_WALL:
+        for (int n = 0; n < 5; n++) {
+          if (A.has_diss) f[n] = f[n] + v[n];
+          e[n] = e[n] + v[5 + n];
+        }
         break;
       default:
         break;

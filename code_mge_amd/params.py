@@ -34,6 +34,7 @@ class Params:
     ref_leng: float = 1.0
     drop: float = 1.0e6
     precision: int = 64
+    epsentr: float = 0.05             # Roe entropy fix (jobsheet r2_epsentr)
     extra: dict = field(default_factory=dict)
 
     def derived(self):
@@ -84,6 +85,7 @@ class Params:
         P.drop = self.drop
         P.distype = {"c": 0, "r1": 1, "r2": 2}[self.distype.lower()]
         P.precision = self.precision
+        P.epsentr = self.epsentr
         return P
 
     def dynamic_pressure(self):

@@ -73,11 +73,14 @@ typedef struct {
   int    dissip;              /* 0 scalar, 1 matrix                                */
   double viscl, prlmean, gas_const;
   double drop;                /* solve exits when ini_res/res > drop (1e6 in the reference) */
-  int    distype;             /* 0 central (JST)                                   */
+  int    distype;             /* 0 central (JST); 1 Roe first order (r1); 2 Roe with
+                                 MUSCL-minmod (r2)                                 */
   int    precision;           /* 64 (or 0): FP64 throughout.  32: FP32 bulk state,
                                  fluxes, smoothing and updates; the wall mirror
                                  states, residual norms, global time step and
                                  force integral stay FP64                          */
+  double epsentr;             /* Roe entropy-fix coefficient (jobsheet r2_epsentr,
+                                 input_param.f90:103)                              */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record

@@ -229,7 +229,8 @@ int orc_timestep(orc_case *c, int lvl) {
   const int ib = b->ib, jb = b->jb, kb = b->kb;
   const double gamma = P->gamma, cfl = P->cfl;
   const double plim = 0.001;
-  double cfac = 4.0;
+  /* viscous time-step factor by scheme (timestep.f90:119-124) */
+  double cfac = P->distype == 0 ? 4.0 : (P->distype == 2 ? 1.0 : 2.0);
   b->dtim = 10.0;
   for (int k = 2; k <= kl; k++)
     for (int j = 2; j <= jl; j++)
@@ -786,6 +787,239 @@ static int eflux(blk_t *b) {
   return rc;
 }
 
+/* ------------------------------------------------------------ Roe scheme
+ *
+ * The reference calls var_diffs / dflux_roe2 / eflux_roe2 (second order)
+ * and dflux_roe1 / eflux_roe1 (first order) from euler.f90:201-210, but the
+ * roepak module has no source in the reference tree.  This is an own FP64
+ * statement of the scheme from the mathematics (parity unpinned against
+ * the reference; pinned by closed-form checks in the tests):
+ *  - face states: primitive variables (rho,u,v,w,p); second order uses
+ *    MUSCL with the minmod limiter, q_L = q(a-1) + minmod(dq(a-1),dq(a))/2,
+ *    q_R = q(a) - minmod(dq(a),dq(a+1))/2; a slope that would reach a
+ *    solid cell is zero (minmod face states lie between the neighbouring
+ *    cell values, so positive density and pressure stay positive);
+ *  - central part (the role of eflux_roe): 0.5*(F_S(U_L) + F_S(U_R));
+ *  - upwind part (the role of dflux_roe, blended by cdis like dflux):
+ *    cdis*0.5*|A_roe(S)|(U_R - U_L) from Roe's averages, written as the
+ *    three wave families (acoustic -, entropy + shear, acoustic +) with
+ *    Harten's entropy fix |l| -> (l^2 + del^2)/(2 del) for |l| < del,
+ *    del = epsentr*(|u_n| + c)*|S|;
+ *  - faces between a fluid and a solid cell carry no bulk flux; each wall
+ *    record adds the Roe flux between the cut cell (first order) and its
+ *    mirror state (bc_cutcell) to the cut cell.
+ * In the face-vector orientation of the reference (S points to -x for an
+ * i face) the sum e + d is minus the outward Roe flux times |S|, matching
+ * the sign convention of eflux/dflux. */
+
+static double minmod(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  return fabs(a) < fabs(b) ? a : b;
+}
+
+static void prim_cell(const blk_t *b, long q, double *v) {
+  double r = W(b, 0, q);
+  v[0] = r;
+  v[1] = W(b, 1, q) / r;
+  v[2] = W(b, 2, q) / r;
+  v[3] = W(b, 3, q) / r;
+  v[4] = b->p[q];
+}
+
+static void cons_of(double gamma, const double *q, double *u) {
+  double q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  u[0] = q[0];
+  u[1] = q[0] * q[1];
+  u[2] = q[0] * q[2];
+  u[3] = q[0] * q[3];
+  u[4] = q[4] / (gamma - 1.0) + 0.5 * q[0] * q2;
+}
+
+static double fix_abs(double l, double del) {
+  double a = fabs(l);
+  if (a < del) a = (l * l + del * del) / (2.0 * del);
+  return a;
+}
+
+void orc_roe_face(double gamma, double eps, double cdis, const double *qL, const double *qR,
+                  const double *S, int signed_lambda, double *e, double *d) {
+  double UL[5], UR[5], FL[5], FR[5];
+  cons_of(gamma, qL, UL);
+  cons_of(gamma, qR, UR);
+  ceflux(UL, qL[4], S, FL);
+  ceflux(UR, qR[4], S, FR);
+  for (int n = 0; n < 5; n++) e[n] = 0.5 * (FL[n] + FR[n]);
+  double sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  double rh = sL * sR;
+  double uh[3];
+  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
+  double HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
+  double Hh = (sL * HL + sR * HR) / den;
+  double q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
+  double c2 = (gamma - 1.0) * (Hh - 0.5 * q2h);
+  if (c2 < 1.0e-12) c2 = 1.0e-12;
+  double c = sqrt(c2);
+  double A = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  double nv[3] = {S[0] / A, S[1] / A, S[2] / A};
+  double un = uh[0] * nv[0] + uh[1] * nv[1] + uh[2] * nv[2];
+  double l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
+  double a1, a2, a3;
+  if (signed_lambda) {
+    a1 = l1; a2 = l2; a3 = l3;
+  } else {
+    double del = eps * (fabs(un) + c) * A;
+    a1 = fix_abs(l1, del);
+    a2 = fix_abs(l2, del);
+    a3 = fix_abs(l3, del);
+  }
+  double dr = qR[0] - qL[0], dp = qR[4] - qL[4];
+  double du[3] = {qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
+  double dun = du[0] * nv[0] + du[1] * nv[1] + du[2] * nv[2];
+  double al1 = (dp - rh * c * dun) / (2.0 * c2);
+  double al3 = (dp + rh * c * dun) / (2.0 * c2);
+  double al2 = dr - dp / c2;
+  double dut[3];
+  for (int m = 0; m < 3; m++) dut[m] = du[m] - dun * nv[m];
+  double udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
+  double dd[5];
+  dd[0] = a1 * al1 + a2 * al2 + a3 * al3;
+  for (int m = 0; m < 3; m++)
+    dd[1 + m] = a1 * al1 * (uh[m] - c * nv[m]) + a2 * (al2 * uh[m] + rh * dut[m]) +
+                a3 * al3 * (uh[m] + c * nv[m]);
+  dd[4] = a1 * al1 * (Hh - c * un) + a2 * (al2 * 0.5 * q2h + rh * udt) + a3 * al3 * (Hh + c * un);
+  for (int n = 0; n < 5; n++) d[n] = cdis * (0.5 * dd[n]);
+}
+
+void orc_muscl(int order, const double *qm2, const double *qm1, const double *qp0, const double *qp1,
+               const int *fl, double *qL, double *qR) {
+  for (int n = 0; n < 5; n++) {
+    qL[n] = qm1[n];
+    qR[n] = qp0[n];
+  }
+  if (order < 2) return;
+  for (int n = 0; n < 5; n++) {
+    double sl = fl[0] ? minmod(qm1[n] - qm2[n], qp0[n] - qm1[n]) : 0.0;
+    double sr = fl[3] ? minmod(qp0[n] - qm1[n], qp1[n] - qp0[n]) : 0.0;
+    qL[n] = qm1[n] + 0.5 * sl;
+    qR[n] = qp0[n] - 0.5 * sr;
+  }
+}
+
+/* the Roe face of direction dir at face a of the line through (i,j,k);
+ * zero unless both neighbours are fluid */
+static void roe_bulk_face(const blk_t *b, const orc_params *P, double cdis, int dir, int i, int j, int k,
+                          double *e, double *d) {
+  int o[3] = {0, 0, 0};
+  o[dir] = 1;
+  long q[4];
+  int fl[4];
+  for (int t = 0; t < 4; t++) {
+    int s = t - 2;
+    q[t] = IX(b, i + s * o[0], j + s * o[1], k + s * o[2]);
+    fl[t] = b->blank[q[t]] != 0;
+  }
+  if (!(fl[1] && fl[2])) {
+    for (int n = 0; n < 5; n++) e[n] = d[n] = 0.0;
+    return;
+  }
+  double v[4][5], qL[5], qR[5], S[3];
+  for (int t = 0; t < 4; t++) prim_cell(b, q[t], v[t]);
+  orc_muscl(P->distype, v[0], v[1], v[2], v[3], fl, qL, qR);
+  const double *SV = dir == 0 ? b->si : (dir == 1 ? b->sj : b->sk);
+  svec(SV, b->E, q[2], S);
+  orc_roe_face(P->gamma, P->epsentr, cdis, qL, qR, S, 0, e, d);
+}
+
+/* the wall face of record t (Roe): contribution to the cut cell */
+static int roe_wall(const blk_t *b, const orc_params *P, const orc_cut *t, double cdis, double *ve,
+                    double *vd) {
+  int zz = t->zz;
+  int hi = t->faceid >= 4;
+  long qc = IX(b, t->ccid[0], t->ccid[1], t->ccid[2]);
+  if (fabs(W(b, 0, qc)) < 1.0e-8f) return ORC_ERR_WALL_RHO;
+  int fi = t->ccid[0], fj = t->ccid[1], fk = t->ccid[2];
+  if (hi) {
+    if (zz == 1) fi++;
+    else if (zz == 2) fj++;
+    else fk++;
+  }
+  double qcv[5], qm[5], S[3], e[5], d[5];
+  prim_cell(b, qc, qcv);
+  qm[0] = t->wsol[0];
+  qm[1] = t->wsol[1] / t->wsol[0];
+  qm[2] = t->wsol[2] / t->wsol[0];
+  qm[3] = t->wsol[3] / t->wsol[0];
+  qm[4] = t->psol;
+  const double *SV = zz == 1 ? b->si : (zz == 2 ? b->sj : b->sk);
+  svec(SV, b->E, IX(b, fi, fj, fk), S);
+  if (hi) orc_roe_face(P->gamma, P->epsentr, cdis, qcv, qm, S, 0, e, d);
+  else orc_roe_face(P->gamma, P->epsentr, cdis, qm, qcv, S, 0, e, d);
+  double sgn = hi ? -1.0 : 1.0;
+  for (int n = 0; n < 5; n++) {
+    ve[n] = sgn * e[n];
+    vd[n] = sgn * d[n];
+  }
+  return ORC_OK;
+}
+
+/* one stage's Roe fluxes: fw += upwind part (when cdis > 0), dw = central
+ * part, then the wall records in order */
+static int roe_fluxes(blk_t *b, const orc_params *P, int istage) {
+  const long E = b->E;
+  const double cd = P->cdis[istage - 1];
+  const int diss = cd > 0.0;
+  double *efs[3], *dfs[3];
+  for (int d = 0; d < 3; d++) {
+    efs[d] = (double *)calloc((size_t)E * 5, sizeof(double));
+    dfs[d] = (double *)calloc((size_t)E * 5, sizeof(double));
+  }
+  int lim[3] = {b->ie, b->je, b->ke};
+  for (int dir = 0; dir < 3; dir++)
+    for (int k = 2; k <= (dir == 2 ? lim[2] : b->kl); k++)
+      for (int j = 2; j <= (dir == 1 ? lim[1] : b->jl); j++)
+        for (int i = 2; i <= (dir == 0 ? lim[0] : b->il); i++) {
+          double e[5], d[5];
+          roe_bulk_face(b, P, cd, dir, i, j, k, e, d);
+          long q = IX(b, i, j, k);
+          for (int n = 0; n < 5; n++) {
+            efs[dir][(long)n * E + q] = e[n];
+            dfs[dir][(long)n * E + q] = d[n];
+          }
+        }
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++)
+        for (int n = 0; n < 5; n++) {
+          long q = (long)n * E + IX(b, i, j, k);
+          long qi = (long)n * E + IX(b, i + 1, j, k), qj = (long)n * E + IX(b, i, j + 1, k);
+          long qk = (long)n * E + IX(b, i, j, k + 1);
+          if (diss) {
+            b->fw[q] = b->fw[q] + (dfs[0][q] - dfs[0][qi]);
+            b->fw[q] = b->fw[q] + (dfs[1][q] - dfs[1][qj]);
+            b->fw[q] = b->fw[q] + (dfs[2][q] - dfs[2][qk]);
+          }
+          b->dw[q] = efs[0][q] - efs[0][qi];
+          b->dw[q] = b->dw[q] + (efs[1][q] - efs[1][qj]);
+          b->dw[q] = b->dw[q] + (efs[2][q] - efs[2][qk]);
+        }
+  for (int d = 0; d < 3; d++) {
+    free(efs[d]);
+    free(dfs[d]);
+  }
+  for (int m = 0; m < b->nmir; m++) {
+    double ve[5], vd[5];
+    int rc = roe_wall(b, P, &b->z[m], cd, ve, vd);
+    if (rc) return rc;
+    long qc = IX(b, b->z[m].ccid[0], b->z[m].ccid[1], b->z[m].ccid[2]);
+    for (int n = 0; n < 5; n++) {
+      long q = (long)n * E + qc;
+      if (diss) b->fw[q] = b->fw[q] + vd[n];
+      b->dw[q] = b->dw[q] + ve[n];
+    }
+  }
+  return ORC_OK;
+}
+
 /* ------------------------------------------------------------ smoothing */
 
 /* psmoo_old (psmoo.f90:97-242) */
@@ -1124,8 +1358,12 @@ int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int l
               b->fw[q] = b->fw[q] * (1.0 - P->cdis[m - 1]);
             }
     }
-    if (P->cdis[m - 1] > 0.0) dflux(b, P, m);
-    rc = eflux(b);
+    if (P->distype == 0) {
+      if (P->cdis[m - 1] > 0.0) dflux(b, P, m);
+      rc = eflux(b);
+    } else {
+      rc = roe_fluxes(b, P, m); /* euler.f90:201-210 */
+    }
     if (rc) return rc;
     for (int k = 2; k <= kl; k++)
       for (int j = 2; j <= jl; j++)
@@ -1497,6 +1735,17 @@ int orc_wall_force(orc_case *c, int lvl, double out[3]) {
   blk_t *b = &c->lv[lvl];
   const long E = b->E;
   double F[3] = {0, 0, 0};
+  if (c->prm.distype != 0) {
+    /* Roe: the full wall-face flux (central + upwind, cdis = 1) */
+    for (int m = 0; m < b->nmir; m++) {
+      double ve[5], vd[5];
+      int rc = roe_wall(b, &c->prm, &b->z[m], 1.0, ve, vd);
+      if (rc) return rc;
+      for (int d = 0; d < 3; d++) F[d] = F[d] + (ve[1 + d] + vd[1 + d]);
+    }
+    for (int d = 0; d < 3; d++) out[d] = F[d];
+    return ORC_OK;
+  }
   for (int m = 0; m < b->nmir; m++) {
     orc_cut *t = &b->z[m];
     const double *S = t->zz == 1 ? b->si : (t->zz == 2 ? b->sj : b->sk);

@@ -50,6 +50,8 @@ typedef struct {
   int    dissip;       /* 0 scalar, 1 matrix */
   double viscl, prlmean, gas_const;
   double drop;         /* cycle exit when ini_res/res > drop (1e6 in solver.f90:238) */
+  int    distype;      /* 0 central (JST), 1 Roe first order, 2 Roe + MUSCL-minmod */
+  double epsentr;      /* Roe entropy-fix coefficient */
 } orc_params;
 
 typedef struct {
@@ -93,6 +95,16 @@ int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
 /* unit-level entry points for the tests: psmoo_old on the level's dw
  * (psmoo.f90:97-242) and dflux_matrix (dflux.f90:283-339) */
 int  orc_psmoo(orc_case *c, int lvl);
+/* one Roe face (own design, no reference source): primitive states qL, qR
+ * (rho,u,v,w,p), face vector S; e = central part 0.5*(F(UL)+F(UR)),
+ * d = cdis*0.5*|A_roe|(UR-UL) with the Harten entropy fix eps.  With
+ * signed != 0 the eigenvalues keep their sign (no fix): d = cdis*0.5*A(UR-UL). */
+void orc_roe_face(double gamma, double eps, double cdis, const double *qL, const double *qR,
+                  const double *S, int signed_lambda, double *e, double *d);
+/* MUSCL-minmod face states from four cell primitives (q(a-2)..q(a+1)) and
+ * their fluid flags; order 1 or 2 */
+void orc_muscl(int order, const double *qm2, const double *qm1, const double *qp0, const double *qp1,
+               const int *fl, double *qL, double *qR);
 void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
                       const double *ss, double *ff);
 

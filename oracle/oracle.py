@@ -24,6 +24,7 @@ class OrcParams(C.Structure):
         ("eqntype", C.c_int), ("dissip", C.c_int),
         ("viscl", C.c_double), ("prlmean", C.c_double), ("gas_const", C.c_double),
         ("drop", C.c_double),
+        ("distype", C.c_int), ("epsentr", C.c_double),
     ]
 
 
@@ -102,7 +103,29 @@ def params_from(prm):
     P.prlmean = prm.prlmean
     P.gas_const = 1.0 / prm.gamma
     P.drop = prm.drop
+    P.distype = {"c": 0, "r1": 1, "r2": 2}[prm.distype.lower()]
+    P.epsentr = prm.epsentr
     return P
+
+
+def roe_face(gamma, eps, cdis, qL, qR, S, signed=False):
+    """one Roe face of the oracle: (central part e, upwind part d)"""
+    L = lib()
+    e = (C.c_double * 5)()
+    d = (C.c_double * 5)()
+    L.orc_roe_face(C.c_double(gamma), C.c_double(eps), C.c_double(cdis), (C.c_double * 5)(*qL),
+                   (C.c_double * 5)(*qR), (C.c_double * 3)(*S), int(signed), e, d)
+    return np.array(e[:]), np.array(d[:])
+
+
+def muscl(order, qm2, qm1, qp0, qp1, fl=(1, 1, 1, 1)):
+    """MUSCL-minmod face states (qL, qR) of the oracle"""
+    L = lib()
+    qL = (C.c_double * 5)()
+    qR = (C.c_double * 5)()
+    L.orc_muscl(int(order), (C.c_double * 5)(*qm2), (C.c_double * 5)(*qm1), (C.c_double * 5)(*qp0),
+                (C.c_double * 5)(*qp1), (C.c_int * 4)(*fl), qL, qR)
+    return np.array(qL[:]), np.array(qR[:])
 
 
 class OracleCase:
