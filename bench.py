@@ -2,15 +2,17 @@
 """Benchmark of the multigrid Jameson-RK solve step on one MI355X.
 
 Workload: steady inviscid flow past a sphere (radius 0.5) in a 256^3
-Cartesian box [-8,8]^3, M = 0.5, alpha = 2 deg, cut-cell wall, 4-level
-W-cycle, FP64, JST central scheme (the Roe/MUSCL flux of that case is not
-built yet; the scheme is named in the output).  A step is one multigrid
-cycle on the finest level; the unit of work is a cell-stage (one RK-stage
-residual + update of one cell, a residual-only pass counting 1).
+Cartesian box [-8,8]^3, M = 0.5, alpha = 2 deg, cut-cell wall, Roe flux
+with MUSCL-minmod reconstruction, 5-stage RK with residual smoothing,
+4-level W-cycle.  A step is one multigrid cycle on the finest level; the
+unit of work is a cell-stage (one RK-stage residual + update of one cell,
+a residual-only pass counting 1).
 
-Prints one JSON line with the throughput, the roofline of the dominant
-kernel (k_stage; HIP events on the library's stream over the timed region)
-and the CPU baseline (the FP64 CPU restatement, one core, bounded sample).
+The headline value is the plain FP64 path; the same workload is then run
+on the FP32 bulk path and reported beside it.  Prints one JSON line with
+the throughput, the roofline of the dominant kernel (k_stage; HIP events
+on the library's stream over the timed region) and the CPU baseline (the
+FP64 CPU restatement, one core, bounded sample).
 """
 import argparse
 import json
@@ -36,7 +38,7 @@ def _cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(n, prm_kw, nz_slab=32):
+def cpu_baseline(n, prm_kw, nz_slab=48):
     """FP64 CPU restatement on one core: timestep + one 5-stage relax on a
     n x n x nz_slab slab (z in [-nz_slab*h/2, nz_slab*h/2]) of the same grid
     and body; returns Mcell-stage/s."""
@@ -64,16 +66,48 @@ def prm_stages(prm):
     return len(prm.cstp)
 
 
-def load_traffic():
+def load_traffic(tag):
     """HBM bytes per k_stage launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
     p = os.path.join(ROOT, "profiles", "k_stage_traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)
-        return d.get("bytes_per_launch")
-    except (OSError, ValueError):
+        return d.get(tag, {}).get("bytes_per_launch")
+    except (OSError, ValueError, AttributeError):
         return None
+
+
+def run_leg(prm, levels, geoms, aref, args, precision):
+    """warmup + timed cycles of one precision; returns the measurements"""
+    import dataclasses
+    from code_mge_amd.solver import Case, force_coefficients
+    pp = dataclasses.replace(prm, precision=precision)
+    case = Case(pp, levels, geoms=geoms, mxcyc=args.warmup + args.steps + 2)
+    case.init_flow()
+    nl = args.levels
+    print(f"[bench] FP{precision}: warmup {args.warmup} cycles", file=sys.stderr, flush=True)
+    r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
+    timing = not args.no_timing
+    case.timing(enable=timing, name=None)
+    case.sync()
+    t0 = time.perf_counter()
+    r1, cs = case.cycles(nl, args.steps, ncyc0=args.warmup + 1)   # ends with a stream sync
+    t1 = time.perf_counter()
+    ms, cnt, by = case.timing(enable=False, name="k_stage")
+    dt = t1 - t0
+    cd, cl = force_coefficients(case, aref)
+    print(f"[bench] FP{precision}: {args.steps} cycles in {dt:.3f} s, {cs/1e6:.1f} M cell-stages, "
+          f"res {r0:.3e} -> {r1:.3e}", file=sys.stderr, flush=True)
+    roofline = None
+    if timing and cnt > 0 and ms > 0:
+        achieved = by / (ms * 1e-3) / 1e9
+        roofline = {"kernel": "k_stage", "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                    "traffic": load_traffic(f"fp{precision}"), "avg_launch_ms": round(ms, 5), "launches": cnt,
+                    "alg_bytes_per_launch": round(by)}
+    case.close()
+    return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1)
 
 
 def main():
@@ -85,6 +119,7 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -95,38 +130,21 @@ def main():
 
     from code_mge_amd import configs
     from code_mge_amd.grid import level_geometry
-    from code_mge_amd.solver import Case, force_coefficients
 
-    prm_kw = dict(mach=0.5, alpha=2.0)
+    prm_kw = dict(mach=0.5, alpha=2.0, distype="r2")
     prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, **prm_kw)
     print(f"[bench] building {args.n}^3 sphere case, {args.levels} levels", file=sys.stderr, flush=True)
     geoms = [level_geometry(s, bdy) for s in levels]
-    case = Case(prm, levels, geoms=geoms, mxcyc=args.warmup + args.steps + 2)
-    case.init_flow()
-    nl = args.levels
-    print(f"[bench] warmup {args.warmup} cycles", file=sys.stderr, flush=True)
-    r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
-
-    timing = not args.no_timing
-    case.timing(enable=timing, name=None)
-    case.sync()
-    t0 = time.perf_counter()
-    r1, cs = case.cycles(nl, args.steps, ncyc0=args.warmup + 1)   # ends with a stream sync
-    t1 = time.perf_counter()
-    ms, cnt, by = case.timing(enable=False, name="k_stage")
-    dt = t1 - t0
-    value = cs / dt / 1e6
-    cd, cl = force_coefficients(case, aref)
-    print(f"[bench] {args.steps} cycles in {dt:.3f} s, {cs/1e6:.1f} M cell-stages, res {r0:.3e} -> {r1:.3e}",
-          file=sys.stderr, flush=True)
-
-    roofline = None
-    if timing and cnt > 0 and ms > 0:
-        achieved = by / (ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_stage", "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                    "traffic": load_traffic(), "avg_launch_ms": round(ms, 5), "launches": cnt,
-                    "alg_bytes_per_launch": round(by)}
+    main_leg = run_leg(prm, levels, geoms, aref, args, 64)
+    fp32 = None
+    if not args.no_fp32:
+        leg = run_leg(prm, levels, geoms, aref, args, 32)
+        fp32 = {"value": round(leg["value"], 2), "unit": "Mcell-stage/s", "dtype": "f32",
+                "ms_per_step": round(leg["dt"] / args.steps * 1e3, 3), "roofline": leg["roofline"],
+                "cd": round(leg["cd"], 6), "cl": round(leg["cl"], 6),
+                "speedup_vs_fp64": round(leg["value"] / main_leg["value"], 3)}
+    value, dt, cs = main_leg["value"], main_leg["dt"], main_leg["cs"]
+    cd, cl, roofline = main_leg["cd"], main_leg["cl"], main_leg["roofline"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -149,14 +167,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic (freestream start, analytic sphere geometry)",
         "config": {"workload": f"sphere {args.n}^3 in [-8,8]^3, M=0.5, alpha=2, {args.levels}-level W-cycle, "
-                               "5-stage RK, JST central scheme, FP64",
+                               "5-stage RK with residual smoothing, Roe flux + MUSCL-minmod, FP64",
                    "n": args.n, "levels": args.levels, "cell_stages_per_step": cs // args.steps,
                    "cd": round(cd, 6), "cl": round(cl, 6)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32": fp32,
     }
     print(json.dumps(out))
-    case.close()
 
 
 if __name__ == "__main__":

@@ -389,6 +389,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     A.vt = P.vt;
     A.order = P.distype == 2 ? 2 : 1;
     A.eps = (T)P.epsentr;
+    A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
     {
       // algorithmic bytes per cell: read W^(m-1); write W^(m) and read W^(0)
       // when this kernel updates, else write dw; forcing on coarse levels;

@@ -7,7 +7,8 @@
 //  - face states from primitive variables q = (rho,u,v,w,p); second order
 //    q_L = q(a-1) + minmod(dq(a-1), dq(a))/2, q_R = q(a) - minmod(dq(a),
 //    dq(a+1))/2, with a slope that would reach a solid cell set to zero;
-//  - central part 0.5*(F_S(U_L) + F_S(U_R)) (accumulated like eflux into dw);
+//  - central part 0.5*(F_S(q_L) + F_S(q_R)), the physical fluxes written from
+//    the primitive face states (accumulated like eflux into dw);
 //  - upwind part cdis*0.5*|A_roe(S)|(U_R - U_L) from Roe's averages and
 //    three wave families with Harten's entropy fix (accumulated like dflux
 //    into fw, so it is blended across stages as the JST dissipation is);
@@ -32,28 +33,52 @@ __device__ inline T minmod(T a, T b) {
 template <typename T>
 __device__ inline void prim_of(const StateC<T> &s, long E, long q, T *o) {
   T r = s.w[q];
+  T rr = T(1) / r;
   o[0] = r;
-  o[1] = s.w[E + q] / r;
-  o[2] = s.w[2 * E + q] / r;
-  o[3] = s.w[3 * E + q] / r;
+  o[1] = s.w[E + q] * rr;
+  o[2] = s.w[2 * E + q] * rr;
+  o[3] = s.w[3 * E + q] * rr;
   o[4] = s.p[q];
 }
 
-template <typename T>
-__device__ inline void cons_of(T gamma, const T *q, T *u) {
+// physical flux of a primitive face state through the face vector whose
+// single non-zero component S lies in direction DIR; returns the total
+// enthalpy H = gamma/(gamma-1) p/rho + |u|^2/2 (g1 = gamma/(gamma-1))
+template <int DIR, typename T>
+__device__ inline T roe_state_flux(T g1, const T *q, T S, T *F) {
+  T rr = T(1) / q[0];
   T q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-  u[0] = q[0];
-  u[1] = q[0] * q[1];
-  u[2] = q[0] * q[2];
-  u[3] = q[0] * q[3];
-  u[4] = q[4] / (gamma - T(1)) + T(0.5) * q[0] * q2;
+  T H = g1 * (q[4] * rr) + T(0.5) * q2;
+  T m = q[0] * (q[1 + DIR] * S);
+  F[0] = m;
+#pragma unroll
+  for (int c = 0; c < 3; c++) F[1 + c] = m * q[1 + c];
+  F[1 + DIR] = m * q[1 + DIR] + q[4] * S;
+  F[4] = m * H;
+  return H;
 }
 
+// the same for a general face vector (FP64 wall faces)
+__device__ inline double roe_state_flux_vec(double g1, const double *q, const double *S, double *F) {
+  double rr = 1.0 / q[0];
+  double q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  double H = g1 * (q[4] * rr) + 0.5 * q2;
+  double unS = q[1] * S[0] + q[2] * S[1] + q[3] * S[2];
+  double m = q[0] * unS;
+  F[0] = m;
+  F[1] = m * q[1] + q[4] * S[0];
+  F[2] = m * q[2] + q[4] * S[1];
+  F[3] = m * q[3] + q[4] * S[2];
+  F[4] = m * H;
+  return H;
+}
+
+// Harten's fix: |l| below del becomes (l^2 + del^2)/(2 del); hd = 0.5/del
 template <typename T>
-__device__ inline T fix_abs(T l, T del) {
+__device__ inline T fix_abs(T l, T del, T hd) {
   T a = fabs(l);
-  if (a < del) a = (l * l + del * del) / (T(2) * del);
-  return a;
+  T f = (l * l + del * del) * hd;
+  return a < del ? f : a;
 }
 
 // face states of one face from the primitives of cells a-2..a+1 and the
@@ -80,42 +105,46 @@ __device__ inline void muscl(int order, const T *qm2, const T *qm1, const T *qp0
 
 // one Roe face whose face vector has the single non-zero component S in
 // direction DIR (axis-aligned blocks); e central part, d upwind part (only
-// when diss)
+// when diss).  |S| and S/|S| are formed as fabs(S) and sign(S), which equal
+// sqrt(S*S) and S/sqrt(S*S) exactly in IEEE arithmetic.
 template <int DIR, typename T>
-__device__ inline void roe_face(T gamma, T eps, T cdis, bool diss, const T *qL, const T *qR, T S, T *e, T *d) {
-  T UL[5], UR[5], FL[5], FR[5];
-  cons_of(gamma, qL, UL);
-  cons_of(gamma, qR, UR);
-  cflux<DIR>(UL, qL[4], S, FL);
-  cflux<DIR>(UR, qR[4], S, FR);
+__device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T *qL, const T *qR, T S, T *e,
+                                T *d) {
+  T FL[5], FR[5];
+  T HL = roe_state_flux<DIR>(g1, qL, S, FL);
+  T HR = roe_state_flux<DIR>(g1, qR, S, FR);
 #pragma unroll
   for (int n = 0; n < 5; n++) e[n] = T(0.5) * (FL[n] + FR[n]);
   if (!diss) return;
-  T sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  T sL = sqrt(qL[0]), sR = sqrt(qR[0]);
+  T rd = T(1) / (sL + sR);
+  T wL = sL * rd, wR = sR * rd;
   T rh = sL * sR;
   T uh[3];
 #pragma unroll
-  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
-  T HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
-  T Hh = (sL * HL + sR * HR) / den;
+  for (int m = 0; m < 3; m++) uh[m] = wL * qL[1 + m] + wR * qR[1 + m];
+  T Hh = wL * HL + wR * HR;
   T q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
   T c2 = (gamma - T(1)) * (Hh - T(0.5) * q2h);
   if (c2 < T(1.0e-12)) c2 = T(1.0e-12);
   T c = sqrt(c2);
-  T A = sqrt(S * S);
-  T nd = S / A;
+  T rc2 = T(1) / c2;
+  T A = fabs(S);
+  T nd = S < T(0) ? T(-1) : T(1);
   T un = uh[DIR] * nd;
   T l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
   T del = eps * (fabs(un) + c) * A;
-  T a1 = fix_abs(l1, del), a2 = fix_abs(l2, del), a3 = fix_abs(l3, del);
+  T hd = T(0.5) / del;
+  T a1 = fix_abs(l1, del, hd), a2 = fix_abs(l2, del, hd), a3 = fix_abs(l3, del, hd);
   T dr = qR[0] - qL[0], dp = qR[4] - qL[4];
   T du[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) du[m] = qR[1 + m] - qL[1 + m];
   T dun = du[DIR] * nd;
-  T al1 = (dp - rh * c * dun) / (T(2) * c2);
-  T al3 = (dp + rh * c * dun) / (T(2) * c2);
-  T al2 = dr - dp / c2;
+  T hc = T(0.5) * rc2;
+  T al1 = (dp - rh * c * dun) * hc;
+  T al3 = (dp + rh * c * dun) * hc;
+  T al2 = dr - dp * rc2;
   T dut[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) dut[m] = du[m];
@@ -138,34 +167,37 @@ __device__ inline void roe_face(T gamma, T eps, T cdis, bool diss, const T *qL, 
 // the same face for a general face vector S (FP64 wall faces)
 __device__ inline void roe_face_vec(double gamma, double eps, double cdis, const double *qL, const double *qR,
                                     const double *S, double *e, double *d) {
-  double UL[5], UR[5], FL[5], FR[5];
-  cons_of(gamma, qL, UL);
-  cons_of(gamma, qR, UR);
-  cflux3(UL, qL[4], S, FL);
-  cflux3(UR, qR[4], S, FR);
+  const double g1 = gamma / (gamma - 1.0);
+  double FL[5], FR[5];
+  double HL = roe_state_flux_vec(g1, qL, S, FL);
+  double HR = roe_state_flux_vec(g1, qR, S, FR);
   for (int n = 0; n < 5; n++) e[n] = 0.5 * (FL[n] + FR[n]);
-  double sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  double sL = sqrt(qL[0]), sR = sqrt(qR[0]);
+  double rd = 1.0 / (sL + sR);
+  double wL = sL * rd, wR = sR * rd;
   double rh = sL * sR;
   double uh[3];
-  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
-  double HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
-  double Hh = (sL * HL + sR * HR) / den;
+  for (int m = 0; m < 3; m++) uh[m] = wL * qL[1 + m] + wR * qR[1 + m];
+  double Hh = wL * HL + wR * HR;
   double q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
   double c2 = (gamma - 1.0) * (Hh - 0.5 * q2h);
   if (c2 < 1.0e-12) c2 = 1.0e-12;
   double c = sqrt(c2);
+  double rc2 = 1.0 / c2;
   double A = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
   double nv[3] = {S[0] / A, S[1] / A, S[2] / A};
   double un = uh[0] * nv[0] + uh[1] * nv[1] + uh[2] * nv[2];
   double l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
   double del = eps * (fabs(un) + c) * A;
-  double a1 = fix_abs(l1, del), a2 = fix_abs(l2, del), a3 = fix_abs(l3, del);
+  double hd = 0.5 / del;
+  double a1 = fix_abs(l1, del, hd), a2 = fix_abs(l2, del, hd), a3 = fix_abs(l3, del, hd);
   double dr = qR[0] - qL[0], dp = qR[4] - qL[4];
   double du[3] = {qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
   double dun = du[0] * nv[0] + du[1] * nv[1] + du[2] * nv[2];
-  double al1 = (dp - rh * c * dun) / (2.0 * c2);
-  double al3 = (dp + rh * c * dun) / (2.0 * c2);
-  double al2 = dr - dp / c2;
+  double hc = 0.5 * rc2;
+  double al1 = (dp - rh * c * dun) * hc;
+  double al3 = (dp + rh * c * dun) * hc;
+  double al2 = dr - dp * rc2;
   double dut[3];
   for (int m = 0; m < 3; m++) dut[m] = du[m] - dun * nv[m];
   double udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
@@ -195,15 +227,17 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
   }
   double qcv[5], qm[5], S[3] = {0.0, 0.0, 0.0}, e[5], dv[5];
   double rc = c.s.w[qc];
+  double rr = 1.0 / rc;
   qcv[0] = rc;
-  qcv[1] = (double)c.s.w[d.E + qc] / rc;
-  qcv[2] = (double)c.s.w[2 * d.E + qc] / rc;
-  qcv[3] = (double)c.s.w[3 * d.E + qc] / rc;
+  qcv[1] = (double)c.s.w[d.E + qc] * rr;
+  qcv[2] = (double)c.s.w[2 * d.E + qc] * rr;
+  qcv[3] = (double)c.s.w[3 * d.E + qc] * rr;
   qcv[4] = c.s.p[qc];
+  double rm = 1.0 / wsol[0];
   qm[0] = wsol[0];
-  qm[1] = wsol[1] / wsol[0];
-  qm[2] = wsol[2] / wsol[0];
-  qm[3] = wsol[3] / wsol[0];
+  qm[1] = wsol[1] * rm;
+  qm[2] = wsol[2] * rm;
+  qm[3] = wsol[3] * rm;
   qm[4] = psol;
   S[zz - 1] = face_s_any(c.g, zz, fi, fj, fk);
   if (hi) roe_face_vec(gamma, eps, cdis, qcv, qm, S, e, dv);
@@ -261,7 +295,7 @@ __device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem
   }
   T qL[5], qR[5];
   muscl(A.order, v[0], v[1], v[2], v[3], f[0], f[3], qL, qR);
-  roe_face<DIR>(A.fx.gamma, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
+  roe_face<DIR>(A.fx.gamma, A.g1, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
   if (!diss)
 #pragma unroll
     for (int n = 0; n < 5; n++) dd[n] = T(0);
@@ -280,7 +314,7 @@ __device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs
   }
   T qL[5], qR[5];
   muscl(A.order, zq[0], zq[1], zq[2], zq[3], zf[0], zf[3], qL, qR);
-  roe_face<2>(A.fx.gamma, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
+  roe_face<2>(A.fx.gamma, A.g1, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
   if (!diss)
 #pragma unroll
     for (int n = 0; n < 5; n++) dd[n] = T(0);

@@ -68,6 +68,7 @@ struct StageArgs {
   int vt;
   int order;                // Roe schemes: 1 first order, 2 MUSCL-minmod
   T eps;                    // Roe entropy-fix coefficient
+  T g1;                     // gamma/(gamma-1)
 };
 
 template <typename T>

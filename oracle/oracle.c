@@ -799,7 +799,8 @@ static int eflux(blk_t *b) {
  *    q_R = q(a) - minmod(dq(a),dq(a+1))/2; a slope that would reach a
  *    solid cell is zero (minmod face states lie between the neighbouring
  *    cell values, so positive density and pressure stay positive);
- *  - central part (the role of eflux_roe): 0.5*(F_S(U_L) + F_S(U_R));
+ *  - central part (the role of eflux_roe): 0.5*(F_S(q_L) + F_S(q_R)), the
+ *    physical fluxes written from the primitive face states;
  *  - upwind part (the role of dflux_roe, blended by cdis like dflux):
  *    cdis*0.5*|A_roe(S)|(U_R - U_L) from Roe's averages, written as the
  *    three wave families (acoustic -, entropy + shear, acoustic +) with
@@ -819,46 +820,56 @@ static double minmod(double a, double b) {
 
 static void prim_cell(const blk_t *b, long q, double *v) {
   double r = W(b, 0, q);
+  double rr = 1.0 / r;
   v[0] = r;
-  v[1] = W(b, 1, q) / r;
-  v[2] = W(b, 2, q) / r;
-  v[3] = W(b, 3, q) / r;
+  v[1] = W(b, 1, q) * rr;
+  v[2] = W(b, 2, q) * rr;
+  v[3] = W(b, 3, q) * rr;
   v[4] = b->p[q];
 }
 
-static void cons_of(double gamma, const double *q, double *u) {
+/* physical flux of a primitive face state through S, and its total
+ * enthalpy H = gamma/(gamma-1) p/rho + |u|^2/2 */
+static double roe_state_flux(double g1, const double *q, const double *S, double *F) {
+  double rr = 1.0 / q[0];
   double q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-  u[0] = q[0];
-  u[1] = q[0] * q[1];
-  u[2] = q[0] * q[2];
-  u[3] = q[0] * q[3];
-  u[4] = q[4] / (gamma - 1.0) + 0.5 * q[0] * q2;
+  double H = g1 * (q[4] * rr) + 0.5 * q2;
+  double unS = q[1] * S[0] + q[2] * S[1] + q[3] * S[2];
+  double m = q[0] * unS;
+  F[0] = m;
+  F[1] = m * q[1] + q[4] * S[0];
+  F[2] = m * q[2] + q[4] * S[1];
+  F[3] = m * q[3] + q[4] * S[2];
+  F[4] = m * H;
+  return H;
 }
 
-static double fix_abs(double l, double del) {
+/* Harten's fix: |l| below del becomes (l^2 + del^2)/(2 del); hd = 0.5/del */
+static double fix_abs(double l, double del, double hd) {
   double a = fabs(l);
-  if (a < del) a = (l * l + del * del) / (2.0 * del);
+  if (a < del) a = (l * l + del * del) * hd;
   return a;
 }
 
 void orc_roe_face(double gamma, double eps, double cdis, const double *qL, const double *qR,
                   const double *S, int signed_lambda, double *e, double *d) {
-  double UL[5], UR[5], FL[5], FR[5];
-  cons_of(gamma, qL, UL);
-  cons_of(gamma, qR, UR);
-  ceflux(UL, qL[4], S, FL);
-  ceflux(UR, qR[4], S, FR);
+  const double g1 = gamma / (gamma - 1.0);
+  double FL[5], FR[5];
+  double HL = roe_state_flux(g1, qL, S, FL);
+  double HR = roe_state_flux(g1, qR, S, FR);
   for (int n = 0; n < 5; n++) e[n] = 0.5 * (FL[n] + FR[n]);
-  double sL = sqrt(qL[0]), sR = sqrt(qR[0]), den = sL + sR;
+  double sL = sqrt(qL[0]), sR = sqrt(qR[0]);
+  double rd = 1.0 / (sL + sR);
+  double wL = sL * rd, wR = sR * rd;
   double rh = sL * sR;
   double uh[3];
-  for (int m = 0; m < 3; m++) uh[m] = (sL * qL[1 + m] + sR * qR[1 + m]) / den;
-  double HL = (UL[4] + qL[4]) / qL[0], HR = (UR[4] + qR[4]) / qR[0];
-  double Hh = (sL * HL + sR * HR) / den;
+  for (int m = 0; m < 3; m++) uh[m] = wL * qL[1 + m] + wR * qR[1 + m];
+  double Hh = wL * HL + wR * HR;
   double q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
   double c2 = (gamma - 1.0) * (Hh - 0.5 * q2h);
   if (c2 < 1.0e-12) c2 = 1.0e-12;
   double c = sqrt(c2);
+  double rc2 = 1.0 / c2;
   double A = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
   double nv[3] = {S[0] / A, S[1] / A, S[2] / A};
   double un = uh[0] * nv[0] + uh[1] * nv[1] + uh[2] * nv[2];
@@ -868,16 +879,18 @@ void orc_roe_face(double gamma, double eps, double cdis, const double *qL, const
     a1 = l1; a2 = l2; a3 = l3;
   } else {
     double del = eps * (fabs(un) + c) * A;
-    a1 = fix_abs(l1, del);
-    a2 = fix_abs(l2, del);
-    a3 = fix_abs(l3, del);
+    double hd = 0.5 / del;
+    a1 = fix_abs(l1, del, hd);
+    a2 = fix_abs(l2, del, hd);
+    a3 = fix_abs(l3, del, hd);
   }
   double dr = qR[0] - qL[0], dp = qR[4] - qL[4];
   double du[3] = {qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
   double dun = du[0] * nv[0] + du[1] * nv[1] + du[2] * nv[2];
-  double al1 = (dp - rh * c * dun) / (2.0 * c2);
-  double al3 = (dp + rh * c * dun) / (2.0 * c2);
-  double al2 = dr - dp / c2;
+  double hc = 0.5 * rc2;
+  double al1 = (dp - rh * c * dun) * hc;
+  double al3 = (dp + rh * c * dun) * hc;
+  double al2 = dr - dp * rc2;
   double dut[3];
   for (int m = 0; m < 3; m++) dut[m] = du[m] - dun * nv[m];
   double udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
@@ -945,10 +958,11 @@ static int roe_wall(const blk_t *b, const orc_params *P, const orc_cut *t, doubl
   }
   double qcv[5], qm[5], S[3], e[5], d[5];
   prim_cell(b, qc, qcv);
+  double rm = 1.0 / t->wsol[0];
   qm[0] = t->wsol[0];
-  qm[1] = t->wsol[1] / t->wsol[0];
-  qm[2] = t->wsol[2] / t->wsol[0];
-  qm[3] = t->wsol[3] / t->wsol[0];
+  qm[1] = t->wsol[1] * rm;
+  qm[2] = t->wsol[2] * rm;
+  qm[3] = t->wsol[3] * rm;
   qm[4] = t->psol;
   const double *SV = zz == 1 ? b->si : (zz == 2 ? b->sj : b->sk);
   svec(SV, b->E, IX(b, fi, fj, fk), S);
