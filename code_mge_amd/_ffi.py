@@ -109,6 +109,13 @@ def lib():
     L.mge_synchronize.argtypes = [P]
     L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong), dp]
     L.mge_cycles.argtypes = [P, I, I, I, dp, C.POINTER(C.c_longlong)]
+    L.mge_batch_create.argtypes = [I, I, C.POINTER(P)]
+    L.mge_batch_destroy.argtypes = [P]
+    L.mge_batch_add.argtypes = [P, P, D, I]
+    L.mge_batch_remove.argtypes = [P, I]
+    L.mge_batch_run.argtypes = [P, I, C.POINTER(I)]
+    L.mge_batch_status.argtypes = [P, I, C.POINTER(I), C.POINTER(I), dp, dp]
+    L.mge_batch_log.argtypes = [P, I, dp, I]
     _lib = L
     return L
 
@@ -127,4 +134,6 @@ EXPORTED = [
     "mge_bc_overlap", "mge_movco", "mge_movfin", "mge_solve", "mge_download",
     "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
     "mge_kernel_timing", "mge_last_error", "mge_cycles",
+    "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
+    "mge_batch_status", "mge_batch_log",
 ]

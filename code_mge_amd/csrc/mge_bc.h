@@ -29,11 +29,11 @@ struct TsArgs {
 // (:212-223).  The ghost copies of the radii (:227-282) are realised by
 // index clamping where the radii are read.
 template <typename T>
-__global__ void __launch_bounds__(256) k_timestep(TsArgs<T> A) {
+__device__ __attribute__((always_inline)) inline void k_timestep_body(const TsArgs<T> &A, int bx, int by, int bz) {
   const Dims &d = A.d;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 2 + blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by * blockDim.y + threadIdx.y;
+  int k = 2 + bz;
   const T one = T(1), two = T(2);
   T mine = T(10);
   if (i <= d.il && j <= d.jl && k <= d.kl) {
@@ -111,10 +111,21 @@ __global__ void __launch_bounds__(256) k_timestep(TsArgs<T> A) {
     atomicMin(A.dtim_bits, bits);
   }
 }
+MGE_ENTRY_T(k_timestep, TsArgs, 256, 1)
 
-__global__ void k_dtim_decode(const unsigned long long *bits, double *dtim) {
-  dtim[0] = __longlong_as_double((long long)bits[0]);
+// global time step: bits of 10.0 before k_timestep, decoded after it
+struct DtimArgs {
+  unsigned long long *bits;
+  double *dtim;
+};
+__device__ __attribute__((always_inline)) inline void k_dtim_init_body(const DtimArgs &A, int, int, int) {
+  if (threadIdx.x == 0) A.bits[0] = 0x4024000000000000ULL;
 }
+MGE_ENTRY(k_dtim_init, DtimArgs, 64, 1)
+__device__ __attribute__((always_inline)) inline void k_dtim_decode_body(const DtimArgs &A, int, int, int) {
+  if (threadIdx.x == 0) A.dtim[0] = __longlong_as_double((long long)A.bits[0]);
+}
+MGE_ENTRY(k_dtim_decode, DtimArgs, 64, 1)
 
 // ------------------------------------------------------------------ residual norm
 
@@ -128,17 +139,18 @@ struct RtArgs {
   double cfl;
   int vt;
   double *partial;
+  int gx, gy;  // grid of the launch (partial index)
 };
 
 // euler.f90:431-447: sum of (dw_1/(cfl*dt))^2 over fluid cells in FP64;
 // per-block partials in a fixed tree order, then k_rtrms_final adds them
 // in order, so the norm is deterministic
 template <typename T>
-__global__ void __launch_bounds__(256) k_rtrms(RtArgs<T> R) {
+__device__ __attribute__((always_inline)) inline void k_rtrms_body(const RtArgs<T> &R, int bx, int by, int bz) {
   const Dims &d = R.d;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 2 + blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by * blockDim.y + threadIdx.y;
+  int k = 2 + bz;
   double v = 0.0;
   if (i <= d.il && j <= d.jl && k <= d.kl) {
     long q = ix(d, i, j, k);
@@ -157,13 +169,23 @@ __global__ void __launch_bounds__(256) k_rtrms(RtArgs<T> R) {
     __syncthreads();
   }
   if (tid == 0) {
-    long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    long b = bx + (long)R.gx * (by + (long)R.gy * bz);
     R.partial[b] = sm[0];
   }
 }
+MGE_ENTRY_T(k_rtrms, RtArgs, 256, 1)
 
-__global__ void __launch_bounds__(256) k_rtrms_final(const double *partial, long nb, double ncell, double *out,
-                                                     int *err) {
+struct RfArgs {
+  const double *partial;
+  long nb;
+  double ncell;
+  double *out;
+  int *err;
+};
+
+__device__ __attribute__((always_inline)) inline void k_rtrms_final_body(const RfArgs &F, int, int, int) {
+  const double *partial = F.partial;
+  const long nb = F.nb;
   __shared__ double sm[256];
   double acc = 0.0;
   for (long b = threadIdx.x; b < nb; b += 256) acc = acc + partial[b];
@@ -174,11 +196,12 @@ __global__ void __launch_bounds__(256) k_rtrms_final(const double *partial, long
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double r = sqrt(sm[0] / ncell);
-    out[0] = r;
-    if (r != r) raise_err(err, MGE_ERR_RES_NAN);
+    double r = sqrt(sm[0] / F.ncell);
+    F.out[0] = r;
+    if (r != r) raise_err(F.err, MGE_ERR_RES_NAN);
   }
 }
+MGE_ENTRY(k_rtrms_final, RfArgs, 256, 1)
 
 // ------------------------------------------------------------------ bc_cutcell
 
@@ -196,8 +219,8 @@ struct CutArgs {
 
 // bc_cutcell.f90:200-428 for steady bodies (qb = 0), in FP64
 template <typename T>
-__global__ void __launch_bounds__(64) k_bc_cut(CutArgs<T> C) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __attribute__((always_inline)) inline void k_bc_cut_body(const CutArgs<T> &C, int bx, int, int) {
+  int m = bx * blockDim.x + threadIdx.x;
   if (m == C.zero_at) {
     for (int n = 0; n < 5; n++) C.wsol[n * C.nmir + m] = 0.0;
     C.psol[m] = 0.0;
@@ -238,6 +261,7 @@ __global__ void __launch_bounds__(64) k_bc_cut(CutArgs<T> C) {
   for (int n = 0; n < 5; n++) C.wsol[n * C.nmir + m] = ws[n];
   C.psol[m] = psol;
 }
+MGE_ENTRY_T(k_bc_cut, CutArgs, 64, 1)
 
 // ------------------------------------------------------------------ bcond
 
@@ -253,13 +277,13 @@ struct BcArgs {
 // one face of bcond.f90:16-39 -> bc_far.f90 / bc_symm.f90 / bc_periodic.f90 /
 // bc_wall.f90, over the whole (n+4)^2 face like the reference loops
 template <typename T>
-__global__ void __launch_bounds__(256) k_bc_face(BcArgs<T> B) {
+__device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArgs<T> &B, int bx, int by, int) {
   const Dims &d = B.d;
   int dir = (B.face - 1) % 3;
   int amax = dir == 0 ? d.jb : d.ib;
   int cmax = dir == 2 ? d.jb : d.kb;
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  int c2 = blockIdx.y;
+  int a = bx * blockDim.x + threadIdx.x;
+  int c2 = by;
   if (a > amax || c2 > cmax) return;
   int nl = dir == 0 ? d.il : (dir == 1 ? d.jl : d.kl);
   int ne = nl + 1, nb = nl + 2;
@@ -361,6 +385,7 @@ __global__ void __launch_bounds__(256) k_bc_face(BcArgs<T> B) {
     P[qd2] = P[q1];
   }
 }
+MGE_ENTRY_T(k_bc_face, BcArgs, 256, 1)
 
 // the ordered edge extrapolation of bc_corner.f90 (one target cell)
 template <typename T>
@@ -380,7 +405,17 @@ __device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, int ti, int
 // index) item is independent, so one 1024-thread workgroup runs each phase
 // in parallel and synchronises between phases
 template <typename T>
-__global__ void __launch_bounds__(1024) k_bc_corner(Dims d, State<T> s, T gamma) {
+struct CornerArgs {
+  Dims d;
+  State<T> s;
+  T gamma;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_bc_corner_body(const CornerArgs<T> &C, int, int, int) {
+  const Dims &d = C.d;
+  const State<T> s = C.s;
+  const T gamma = C.gamma;
   const long E = d.E;
   T *W = s.w;
   const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke, ib = d.ib;
@@ -438,14 +473,25 @@ __global__ void __launch_bounds__(1024) k_bc_corner(Dims d, State<T> s, T gamma)
     s.p[q] = eos(gamma, v);
   }
 }
+MGE_ENTRY_T(k_bc_corner, CornerArgs, 1024, 1)
 
 // ------------------------------------------------------------------ misc kernels
 
 // ghost shell of src -> dst (w and p), so bcond sees the same ghost values
 // that the reference's single in-place array holds at that point
 template <typename T>
-__global__ void __launch_bounds__(256) k_ghost_copy(Dims d, StateC<T> src, State<T> dst) {
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+struct GhostArgs {
+  Dims d;
+  StateC<T> src;
+  State<T> dst;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_ghost_copy_body(const GhostArgs<T> &G, int bx, int, int) {
+  const Dims &d = G.d;
+  const StateC<T> &src = G.src;
+  const State<T> &dst = G.dst;
+  long q = bx * (long)blockDim.x + threadIdx.x;
   if (q >= d.E) return;
   int k = (int)(q / d.sz);
   int j = (int)((q - (long)k * d.sz) / d.sy);
@@ -455,26 +501,56 @@ __global__ void __launch_bounds__(256) k_ghost_copy(Dims d, StateC<T> src, State
   for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
   dst.p[q] = src.p[q];
 }
+MGE_ENTRY_T(k_ghost_copy, GhostArgs, 256, 1)
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_eos_all(Dims d, State<T> s, T gamma) {
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+struct EosArgs {
+  Dims d;
+  State<T> s;
+  T gamma;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_eos_all_body(const EosArgs<T> &A, int bx, int, int) {
+  const Dims &d = A.d;
+  long q = bx * (long)blockDim.x + threadIdx.x;
   if (q >= d.E) return;
   T v[5];
-  for (int n = 0; n < 5; n++) v[n] = s.w[n * d.E + q];
-  s.p[q] = eos(gamma, v);
+  for (int n = 0; n < 5; n++) v[n] = A.s.w[n * d.E + q];
+  A.s.p[q] = eos(A.gamma, v);
 }
+MGE_ENTRY_T(k_eos_all, EosArgs, 256, 1)
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill_freestream(Dims d, State<T> s, T r0, T ru, T rv, T rw, T re) {
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+struct FillArgs {
+  Dims d;
+  State<T> s;
+  T v[5];
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_fill_body(const FillArgs<T> &A, int bx, int, int) {
+  const Dims &d = A.d;
+  long q = bx * (long)blockDim.x + threadIdx.x;
   if (q >= d.E) return;
-  s.w[q] = r0;
-  s.w[d.E + q] = ru;
-  s.w[2 * d.E + q] = rv;
-  s.w[3 * d.E + q] = rw;
-  s.w[4 * d.E + q] = re;
+  for (int n = 0; n < 5; n++) A.s.w[n * d.E + q] = A.v[n];
 }
+MGE_ENTRY_T(k_fill, FillArgs, 256, 1)
+
+// n values src -> dst (the multigrid saved state ws = w on a first visit)
+template <typename T>
+struct CopyArgs {
+  const T *src;
+  T *dst;
+  long n;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_copy_body(const CopyArgs<T> &A, int bx, int, int) {
+  long q = bx * (long)blockDim.x + threadIdx.x;
+  if (q < A.n) A.dst[q] = A.src[q];
+}
+MGE_ENTRY_T(k_copy, CopyArgs, 256, 1)
 
 // ------------------------------------------------------------------ transfers
 
@@ -509,11 +585,11 @@ __device__ inline void pstencil(int i, int r, int *s, T *f) {
 // collc.f90:110-198 for aligned ratio-1/2 levels: w and the residual
 // restricted with the cell-centre trilinear weights; marker = 1
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(XferArgs<T> X) {
+__device__ __attribute__((always_inline)) inline void k_restrict_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 2 + blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by * blockDim.y + threadIdx.y;
+  int k = 2 + bz;
   if (i > c.il || j > c.jl || k > c.kl) return;
   int s[3];
   T fa[2], fb[2], fc[2], fac[8];
@@ -539,15 +615,16 @@ __global__ void __launch_bounds__(256) k_restrict(XferArgs<T> X) {
   }
   X.marker[qc] = 1;
 }
+MGE_ENTRY_T(k_restrict, XferArgs, 256, 1)
 
 // addw.f90:369-438: trilinear prolongation of the correction w-ws (or of
 // the solution on the full-multigrid step) to fluid cells
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong(XferArgs<T> X) {
+__device__ __attribute__((always_inline)) inline void k_prolong_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 2 + blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by * blockDim.y + threadIdx.y;
+  int k = 2 + bz;
   if (i > f.il || j > f.jl || k > f.kl) return;
   int s[3];
   T fa[2], fb[2], fc[2], fac[8];
@@ -570,6 +647,7 @@ __global__ void __launch_bounds__(256) k_prolong(XferArgs<T> X) {
     else X.wfo[n * f.E + qf] = X.wfo[n * f.E + qf] + acc;
   }
 }
+MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
 
 // wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253
 // for JST; the full Roe wall flux, central + upwind, for the Roe schemes)

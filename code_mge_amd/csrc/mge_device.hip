@@ -105,6 +105,15 @@ struct Timer {
   double bytes;
 };
 
+// one recorded launch of a case (batch programs): both entry points of the
+// kernel, the case's own grid and its argument struct
+struct Rec {
+  const char *name;
+  const void *fs, *fb;
+  dim3 grid, block;
+  std::vector<char> args;
+};
+
 struct TStat {
   double ms = 0.0;
   long long count = 0;
@@ -127,6 +136,8 @@ struct mge_case {
   std::map<std::string, TStat> tstat;
   double next_bytes = 0.0;  // algorithmic bytes of the next timed launch
   double smoo_bb = 0.0;
+  std::vector<Rec> *rec = nullptr;  // when set, launches are recorded, not run
+  hipError_t launch_err = hipSuccess;  // first failed launch, reported by check_err
 };
 
 namespace {
@@ -197,14 +208,38 @@ void resolve_timers(mge_case *h) {
   h->pending.clear();
 }
 
-// kernel names with several template arguments are passed in parentheses
-#define LAUNCH(h, name, kern, grid, block, ...)                         \
-  do {                                                                  \
-    Timer t_;                                                           \
-    tic(h, name, &t_);                                                  \
-    hipLaunchKernelGGL(kern, grid, block, 0, (h)->stream, __VA_ARGS__); \
-    toc(h, &t_);                                                        \
-  } while (0)
+// Launch one kernel of the cycle for one case (fs: single-case entry point,
+// fb: batch entry point), or record it when the case is being recorded
+// into a batch program.  A grid with no blocks is a launch the case skips
+// (no wall records, an inactive face); it is recorded so that every case
+// of a batch records the same sequence.
+template <class A>
+void launch(mge_case *h, const char *name, const void *fs, const void *fb, dim3 g, dim3 b, const A &a) {
+  if (h->rec) {
+    Rec r;
+    r.name = name;
+    r.fs = fs;
+    r.fb = fb;
+    r.grid = g;
+    r.block = b;
+    r.args.assign((const char *)&a, (const char *)&a + sizeof(A));
+    h->rec->push_back(std::move(r));
+    h->next_bytes = 0.0;
+    return;
+  }
+  if ((long)g.x * g.y * g.z == 0) {
+    h->next_bytes = 0.0;
+    return;
+  }
+  Timer t;
+  tic(h, name, &t);
+  void *p[1] = {(void *)&a};
+  hipError_t e = hipLaunchKernel(fs, g, b, p, 0, h->stream);
+  if (e != hipSuccess && h->launch_err == hipSuccess) h->launch_err = e;
+  toc(h, &t);
+}
+#define LAUNCHT(h, name, K, T, g, b, a) launch(h, name, (const void *)&K<T>, (const void *)&K##_b<T>, g, b, a)
+#define LAUNCHN(h, name, K, g, b, a) launch(h, name, (const void *)&K, (const void *)&K##_b, g, b, a)
 
 // typed views of a level's bulk arrays
 template <class T>
@@ -241,6 +276,12 @@ Level *get_level(mge_case *h, int lvl) {
 }
 
 int check_err(mge_case *h) {
+  if (h->launch_err != hipSuccess) {
+    char b[256];
+    snprintf(b, sizeof b, "kernel launch failed: %s", hipGetErrorString(h->launch_err));
+    h->launch_err = hipSuccess;
+    return set_err(MGE_ERR_HIP, b);
+  }
   int e = 0;
   if (hipMemcpyAsync(&e, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
     return set_err(MGE_ERR_HIP, "error-flag copy failed");
@@ -267,9 +308,9 @@ int do_bcond(mge_case *h, Level &L, int buf) {
   int warn = MGE_OK;
   for (int f = 1; f <= 6; f++) {
     int t = L.bctype[f - 1];
-    if (t == 0) continue;
-    if (t != -20 && t != -50 && t != -60 && t != -80) {
-      warn = MGE_WARN_BCTYPE;
+    if (t == 0 || (t != -20 && t != -50 && t != -60 && t != -80)) {
+      if (t != 0) warn = MGE_WARN_BCTYPE;
+      LAUNCHT(h, "k_bc_face", k_bc_face, T, dim3(0), dim3(256), BcArgs<T>{});
       continue;
     }
     if (t == -60 && f <= 3 && L.bctype[f + 2] != -60)
@@ -285,15 +326,18 @@ int do_bcond(mge_case *h, Level &L, int buf) {
     int amax = dir == 0 ? L.d.jb : L.d.ib;
     int cmax = dir == 2 ? L.d.jb : L.d.kb;
     dim3 g((amax + 1 + 255) / 256, cmax + 1);
-    LAUNCH(h, "k_bc_face", k_bc_face<T>, g, dim3(256), B);
+    LAUNCHT(h, "k_bc_face", k_bc_face, T, g, dim3(256), B);
   }
-  LAUNCH(h, "k_bc_corner", k_bc_corner<T>, dim3(1), dim3(1024), L.d, stm<T>(L, buf), (T)P.gamma);
+  LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3(1), dim3(1024), (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma}));
   return warn;
 }
 
 template <class T>
 int do_bc_cutcell(mge_case *h, Level &L, int buf) {
-  if (L.nmir == 0) return MGE_OK;
+  if (L.nmir == 0) {
+    LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3(0), dim3(64), CutArgs<T>{});
+    return MGE_OK;
+  }
   CutArgs<T> C;
   C.d = L.d;
   C.s = stc<T>(L, buf);
@@ -306,7 +350,7 @@ int do_bc_cutcell(mge_case *h, Level &L, int buf) {
   C.gamma = h->prm.gamma;
   C.eqntype = h->prm.eqntype;
   C.err = h->err;
-  LAUNCH(h, "k_bc_cut", k_bc_cut<T>, dim3((L.nmir + 63) / 64), dim3(64), C);
+  LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3((L.nmir + 63) / 64), dim3(64), C);
   return L.zero_at >= 0 ? MGE_WARN_ZERO_CLOUD : MGE_OK;
 }
 
@@ -315,8 +359,8 @@ int do_bc_cutcell(mge_case *h, Level &L, int buf) {
 template <class T>
 int do_timestep(mge_case *h, Level &L) {
   const mge_params &P = h->prm;
-  unsigned long long init = (unsigned long long)0x4024000000000000ULL;  // 10.0
-  HIPCHK(hipMemcpyAsync(L.dtim_bits, &init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+  const DtimArgs D{L.dtim_bits, L.dtim};
+  LAUNCHN(h, "k_dtim_init", k_dtim_init, dim3(1), dim3(64), D);
   TsArgs<T> A;
   A.d = L.d;
   A.s = stc<T>(L, L.cur);
@@ -333,8 +377,8 @@ int do_timestep(mge_case *h, Level &L) {
   A.prlmean = (T)P.prlmean;
   A.cfac = (T)(P.distype == 0 ? 4.0 : (P.distype == 2 ? 1.0 : 2.0));
   A.err = h->err;
-  LAUNCH(h, "k_timestep", k_timestep<T>, cell_grid(L.d), cell_block(), A);
-  LAUNCH(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(1), L.dtim_bits, L.dtim);
+  LAUNCHT(h, "k_timestep", k_timestep, T, cell_grid(L.d), cell_block(), A);
+  LAUNCHN(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(64), D);
   return MGE_OK;
 }
 
@@ -347,7 +391,9 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   const long E = d.E;
   if (P.smoo > 0.0 && (d.ni < 2 || d.nj < 2 || d.nk < 2))
     return set_err(MGE_ERR_SMOO_DIM, "residual smoothing needs at least 2 cells per direction");
-  if (lfir) HIPCHK(hipMemcpyAsync(L.ws, L.S[L.cur], sizeof(T) * 5 * E, hipMemcpyDeviceToDevice, h->stream));
+  if (lfir)
+    LAUNCHT(h, "k_copy", k_copy, T, flat_grid(5 * E), dim3(256),
+            (CopyArgs<T>{(const T *)tp<T>(L.S[L.cur]), tp<T>(L.ws), 5 * E}));
   T *dwp = tp<T>(L.dw);
   const int w0 = L.cur;
   int src = w0;
@@ -406,11 +452,12 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       // enough workgroups to fill 256 CUs: split k into chunks
       while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
       dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
-      if (P.distype != 0) LAUNCH(h, "k_stage", k_stage_roe<T>, g, dim3(TI, TJ), A, kc);
-      else if (P.dissip == 0) LAUNCH(h, "k_stage", k_stage2<T>, g, dim3(TI, TJ), A, kc);
-      else LAUNCH(h, "k_stage", k_stage<T>, cell_grid(d), cell_block(), A);  // matrix dissipation
+      A.kc = kc;
+      if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, TJ), A);
+      else if (P.dissip == 0) LAUNCHT(h, "k_stage", k_stage2, T, g, dim3(TI, TJ), A);
+      else LAUNCHT(h, "k_stage", k_stage, T, cell_grid(d), cell_block(), A);  // matrix dissipation
     }
-    if (L.ntouch > 0) {
+    {
       RecArgs<T> R;
       R.st = A;
       R.st.store_fw = 1;
@@ -425,8 +472,8 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       R.ntouch = L.ntouch;
       R.nops = L.nops;
       R.err = h->err;
-      LAUNCH(h, "k_rec_ops", k_rec_ops<T>, dim3((L.nops + 63) / 64), dim3(64), R);
-      LAUNCH(h, "k_rec_apply", k_rec_apply<T>, dim3((L.ntouch + 63) / 64), dim3(64), R);
+      LAUNCHT(h, "k_rec_ops", k_rec_ops, T, dim3((L.nops + 63) / 64), dim3(64), R);
+      LAUNCHT(h, "k_rec_apply", k_rec_apply, T, dim3((L.ntouch + 63) / 64), dim3(64), R);
     }
     if (lres) return MGE_OK;
     if (A.smoothed) {
@@ -441,21 +488,25 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       S.fa = tp<T>(L.fax);
       long nwave = (long)d.nj * d.nk * 5;
       dim3 gx((unsigned)((nwave * 64 + 255) / 256));
-      if (d.ni <= 64) LAUNCH(h, "k_smooth_x", (k_smooth_x_scan<1, T>), gx, dim3(256), S);
-      else if (d.ni <= 128) LAUNCH(h, "k_smooth_x", (k_smooth_x_scan<2, T>), gx, dim3(256), S);
-      else if (d.ni <= 256) LAUNCH(h, "k_smooth_x", (k_smooth_x_scan<4, T>), gx, dim3(256), S);
-      else if (d.ni <= 512) LAUNCH(h, "k_smooth_x", (k_smooth_x_scan<8, T>), gx, dim3(256), S);
-      else LAUNCH(h, "k_smooth_x", k_smooth_x_seq<T>, flat_grid(nwave), dim3(256), S);
+#define XSCAN(C) launch(h, "k_smooth_x", (const void *)&k_smooth_x_scan<C, T>, (const void *)&k_smooth_x_scan_b<C, T>, gx, dim3(256), S)
+      if (d.ni <= 64) XSCAN(1);
+      else if (d.ni <= 128) XSCAN(2);
+      else if (d.ni <= 256) XSCAN(4);
+      else if (d.ni <= 512) XSCAN(8);
+      else LAUNCHT(h, "k_smooth_x", k_smooth_x_seq, T, flat_grid(nwave), dim3(256), S);
+#undef XSCAN
       S.e1 = piv + 2 * L.npiv;
       S.e2 = piv + 3 * L.npiv;
-      LAUNCH(h, "k_smooth_y", k_smooth_y<T>, dim3((d.ni + 63) / 64, d.nk, 5), dim3(64), S);
+      LAUNCHT(h, "k_smooth_y", k_smooth_y, T, dim3((d.ni + 63) / 64, d.nk, 5), dim3(64), S);
       S.e1 = piv + 4 * L.npiv;
       S.e2 = piv + 5 * L.npiv;
-      LAUNCH(h, "k_smooth_z", k_smooth_z<T>, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
-      LAUNCH(h, "k_update_ghost", k_update_ghost<T>, flat_grid(E), dim3(256), d, (const T *)tp<T>(L.S[w0]),
-             (const T *)dwp, stc<T>(L, src), stm<T>(L, dst), (T)P.gamma);
+      LAUNCHT(h, "k_smooth_z", k_smooth_z, T, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
+      LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256),
+              (UpdArgs<T>{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
+                          (T)P.gamma}));
     } else {
-      LAUNCH(h, "k_ghost_copy", k_ghost_copy<T>, flat_grid(E), dim3(256), d, stc<T>(L, src), stm<T>(L, dst));
+      LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, flat_grid(E), dim3(256),
+              (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
     }
     int rc = do_bc_cutcell<T>(h, L, dst);
     if (rc > 0) warn = rc;
@@ -475,11 +526,13 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   R.vt = P.vt;
   R.partial = L.partial;
   dim3 g = cell_grid(d);
-  LAUNCH(h, "k_rtrms", k_rtrms<T>, g, cell_block(), R);
+  R.gx = (int)g.x;
+  R.gy = (int)g.y;
+  LAUNCHT(h, "k_rtrms", k_rtrms, T, g, cell_block(), R);
   long nb = (long)g.x * g.y * g.z;
   int c = ncyc >= 1 && ncyc <= L.mxcyc ? ncyc : 0;
-  LAUNCH(h, "k_rtrms_final", k_rtrms_final, dim3(1), dim3(256), (const double *)L.partial, nb,
-         (double)((long)d.ni * d.nj * d.nk), L.rtrms + c, h->err);
+  LAUNCHN(h, "k_rtrms_final", k_rtrms_final, dim3(1), dim3(256),
+          (RfArgs{(const double *)L.partial, nb, (double)((long)d.ni * d.nj * d.nk), L.rtrms + c, h->err}));
   return warn;
 }
 
@@ -512,8 +565,8 @@ int do_movco(mge_case *h, int mgl) {
   X.fmg = 0;
   X.flagsf = f.flags;
   X.wfo = nullptr;
-  LAUNCH(h, "k_restrict", k_restrict<T>, cell_grid(c.d), cell_block(), X);
-  LAUNCH(h, "k_eos_all", k_eos_all<T>, flat_grid(c.d.E), dim3(256), c.d, stm<T>(c, c.cur), (T)h->prm.gamma);
+  LAUNCHT(h, "k_restrict", k_restrict, T, cell_grid(c.d), cell_block(), X);
+  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(c.d.E), dim3(256), (EosArgs<T>{c.d, stm<T>(c, c.cur), (T)h->prm.gamma}));
   return do_bcond<T>(h, c, c.cur);
 }
 
@@ -536,8 +589,8 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.fmg = (mgl - 1 == mlvl);
   X.flagsf = f.flags;
   X.wfo = tp<T>(f.S[f.cur]);
-  LAUNCH(h, "k_prolong", k_prolong<T>, cell_grid(f.d), cell_block(), X);
-  LAUNCH(h, "k_eos_all", k_eos_all<T>, flat_grid(f.d.E), dim3(256), f.d, stm<T>(f, f.cur), (T)h->prm.gamma);
+  LAUNCHT(h, "k_prolong", k_prolong, T, cell_grid(f.d), cell_block(), X);
+  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(f.d.E), dim3(256), (EosArgs<T>{f.d, stm<T>(f, f.cur), (T)h->prm.gamma}));
   return do_bcond<T>(h, f, f.cur);
 }
 
@@ -548,12 +601,14 @@ int do_init_flow(mge_case *h) {
   for (int l = 1; l <= h->nlevel; l++) {
     Level &L = h->lv[l];
     if (!L.defined) return set_err(MGE_ERR_STATE, "level not defined");
-    LAUNCH(h, "k_fill", k_fill_freestream<T>, flat_grid(L.d.E), dim3(256), L.d, stm<T>(L, L.cur), (T)P.rho0,
-           (T)(P.rho0 * P.u0), (T)(P.rho0 * P.v0), (T)(P.rho0 * P.w0), (T)(P.rho0 * P.h0 - P.p0));
+    LAUNCHT(h, "k_fill", k_fill, T, flat_grid(L.d.E), dim3(256),
+            (FillArgs<T>{L.d, stm<T>(L, L.cur),
+                         {(T)P.rho0, (T)(P.rho0 * P.u0), (T)(P.rho0 * P.v0), (T)(P.rho0 * P.w0),
+                          (T)(P.rho0 * P.h0 - P.p0)}}));
   }
   for (int l = 1; l <= h->nlevel; l++) {
     Level &L = h->lv[l];
-    LAUNCH(h, "k_eos_all", k_eos_all<T>, flat_grid(L.d.E), dim3(256), L.d, stm<T>(L, L.cur), (T)P.gamma);
+    LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(L.d.E), dim3(256), (EosArgs<T>{L.d, stm<T>(L, L.cur), (T)P.gamma}));
     int rc = do_bc_cutcell<T>(h, L, L.cur);
     if (rc > 0) warn = rc;
     rc = do_bcond<T>(h, L, L.cur);
@@ -651,9 +706,10 @@ int do_wall_force(mge_case *h, Level &L) {
   fc.d = L.d;
   fc.s = stc<T>(L, L.cur);
   fc.g = geo<T>(L);
-  LAUNCH(h, "k_wall_terms", k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), fc, L.recs, L.wsol, L.psol,
-         L.nmir, h->prm.distype != 0 ? 1 : 0, h->prm.gamma, h->prm.epsentr, L.opval, h->err);
-  LAUNCH(h, "k_wall_sum", k_wall_sum, dim3(1), dim3(64), (const double *)L.opval, L.nmir, L.force);
+  hipLaunchKernelGGL(k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), 0, h->stream, fc, (const DevRec *)L.recs,
+                     (const double *)L.wsol, (const double *)L.psol, L.nmir, h->prm.distype != 0 ? 1 : 0,
+                     h->prm.gamma, h->prm.epsentr, L.opval, h->err);
+  hipLaunchKernelGGL(k_wall_sum, dim3(1), dim3(64), 0, h->stream, (const double *)L.opval, L.nmir, L.force);
   return MGE_OK;
 }
 
@@ -778,6 +834,286 @@ int put_bulk(mge_case *h, void *dev, const double *src, size_t n) {
   std::vector<float> t(n);
   for (size_t q = 0; q < n; q++) t[q] = (float)src[q];
   HIPCHK(hipMemcpy(dev, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  return MGE_OK;
+}
+
+
+// ================================================================== batches
+//
+// A batch advances many cases together.  Every case of a batch follows the
+// same cycle (same number of levels, scheme, stage count, smoothing and
+// precision), so the launches its cycle makes line up one to one across
+// the cases.  The batch records two cycles of each active case (two cycles
+// bring every level's state buffers back to the same rotation), merges the
+// recorded launches position by position into one launch of the kernel's
+// batch entry point over all cases, uploads the argument structs once, and
+// replays the merged program -- as a hipGraph or as a launch loop.  After
+// each cycle k_converge updates every case's cycle count and residual and
+// clears the case's active flag when its residual has dropped by `drop`
+// (the drop test of solver.f90:238), when it fails, or at its cycle limit;
+// the remaining launches of the replay skip an inactive case.  Each case
+// computes exactly what its own single-case run computes.
+
+struct BCtl {
+  const double *rtrms;  // residual slots of the case's finest level
+  const int *err;       // the case's device error flag
+  double *log;          // residual history (cap entries)
+  double ini, last, drop;
+  int ncyc, maxcyc, status, cap;
+};
+
+__global__ void k_converge(BCtl *ctl, int *act, int n, int slot) {
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    if (!act[c]) continue;
+    BCtl &C = ctl[c];
+    double r = C.rtrms[slot];
+    C.ncyc += 1;
+    if (C.ncyc == 1) C.ini = r;
+    C.last = r;
+    if (C.ncyc - 1 < C.cap) C.log[C.ncyc - 1] = r;
+    int e = *C.err;
+    if (r != r || e < 0) {
+      act[c] = 0;
+      C.status = e < 0 ? e : MGE_ERR_RES_NAN;
+    } else if (C.ini / r > C.drop) {
+      act[c] = 0;
+      C.status = 1;
+    } else if (C.ncyc >= C.maxcyc) {
+      act[c] = 0;
+      C.status = 2;
+    }
+  }
+}
+
+}  // namespace
+
+struct mge_batch {
+  struct Step {
+    const void *fb;      // batch entry point; null: k_converge of cycle `slot`
+    dim3 block;
+    unsigned nblocks;
+    const void *args;    // device: nc argument structs
+    const int4 *cg;      // device: nc {first block, gx, gy, slot}
+    int nc, slot;
+  };
+  std::vector<mge_case *> cs;          // slot -> case (null after removal)
+  std::vector<BCtl> hctl;
+  std::vector<int> hact;
+  BCtl *d_ctl = nullptr;
+  int *d_act = nullptr;
+  double *d_log = nullptr;
+  int nslot_cap = 0, cap = 0;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  bool dirty = true;
+  std::vector<Step> prog;
+  std::vector<int> prog_slots;         // slots the program was built for
+  std::vector<std::vector<int>> cur1;  // per program case: level buffers after cycle 1
+  char *d_arena = nullptr;
+  size_t arena_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nlevel = 0, fp32 = -1, distype = -1, dissip = -1, mstage = -1, mgtype = -1, smoothed = -1;
+};
+
+namespace {
+
+int batch_grow(mge_batch *b, int need) {
+  if (need <= b->nslot_cap) return MGE_OK;
+  int cap = std::max(need, 2 * b->nslot_cap + 8);
+  BCtl *nc = nullptr;
+  int *na = nullptr;
+  double *nl = nullptr;
+  if (hipMalloc(&nc, sizeof(BCtl) * cap) != hipSuccess || hipMalloc(&na, sizeof(int) * cap) != hipSuccess ||
+      hipMalloc(&nl, sizeof(double) * (size_t)cap * b->cap) != hipSuccess)
+    return set_err(MGE_ERR_NOMEM, "batch control allocation failed");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->hctl.resize(cap);
+  b->hact.resize(cap, 0);
+  // move the residual logs of existing slots to the new log array
+  std::vector<double> lg((size_t)cap * b->cap, 0.0);
+  if (b->nslot_cap) HIPCHK(hipMemcpy(lg.data(), b->d_log, sizeof(double) * (size_t)b->nslot_cap * b->cap,
+                                     hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nl, lg.data(), sizeof(double) * lg.size(), hipMemcpyHostToDevice));
+  for (int c = 0; c < cap; c++) b->hctl[c].log = nl + (size_t)c * b->cap;
+  if (b->d_ctl) hipFree(b->d_ctl);
+  if (b->d_act) hipFree(b->d_act);
+  if (b->d_log) hipFree(b->d_log);
+  b->d_ctl = nc;
+  b->d_act = na;
+  b->d_log = nl;
+  b->nslot_cap = cap;
+  HIPCHK(hipMemcpy(b->d_ctl, b->hctl.data(), sizeof(BCtl) * cap, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_act, b->hact.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  return MGE_OK;
+}
+
+void batch_drop_graph(mge_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int batch_enqueue(mge_batch *b) {
+  for (const auto &st : b->prog) {
+    if (!st.fb) {
+      hipLaunchKernelGGL(k_converge, dim3(1), dim3(256), 0, b->stream, b->d_ctl, b->d_act, (int)b->cs.size(),
+                         st.slot);
+      continue;
+    }
+    const void *a = st.args;
+    const int4 *cg = st.cg;
+    int nc = st.nc;
+    const int *act = b->d_act;
+    void *p[4] = {(void *)&a, (void *)&cg, (void *)&nc, (void *)&act};
+    HIPCHK(hipLaunchKernel(st.fb, dim3(st.nblocks), st.block, p, 0, b->stream));
+  }
+  return MGE_OK;
+}
+
+// record two cycles of every active case and merge them into the program
+int batch_build(mge_batch *b) {
+  HIPCHK(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  b->prog.clear();
+  b->prog_slots.clear();
+  b->cur1.clear();
+  for (int c = 0; c < (int)b->cs.size(); c++)
+    if (b->cs[c] && b->hact[c]) b->prog_slots.push_back(c);
+  b->dirty = false;
+  const int nc = (int)b->prog_slots.size();
+  if (nc == 0) return MGE_OK;
+  std::vector<std::vector<Rec>> R(nc);
+  b->cur1.resize(nc);
+  for (int i = 0; i < nc; i++) {
+    mge_case *h = b->cs[b->prog_slots[i]];
+    h->rec = &R[i];
+    long long cs = 0;
+    int rc = MGE_OK;
+    for (int cyc = 1; cyc <= 2 && rc >= 0; cyc++) {
+      rc = by_type(h, [&](auto z) { return do_cycle<decltype(z)>(h, b->nlevel, cyc, &cs); });
+      Rec m;
+      m.name = "k_converge";
+      m.fs = m.fb = nullptr;
+      m.grid = dim3(cyc);
+      R[i].push_back(std::move(m));
+      if (cyc == 1) {
+        for (int l = 1; l <= h->nlevel; l++) b->cur1[i].push_back(h->lv[l].cur);
+      }
+    }
+    h->rec = nullptr;
+    if (rc < 0) return rc;
+  }
+  const size_t np = R[0].size();
+  for (int i = 1; i < nc; i++)
+    if (R[i].size() != np) return set_err(MGE_ERR_STATE, "cases of a batch must follow the same cycle");
+  // per position, the cases grouped by entry point (cases whose line
+  // lengths pick different scan widths launch different kernels there)
+  struct Group {
+    size_t p;
+    std::vector<int> cases;
+  };
+  std::vector<Group> groups;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t total = 0;
+  for (size_t p = 0; p < np; p++) {
+    for (int i = 1; i < nc; i++)
+      if (std::strcmp(R[i][p].name, R[0][p].name) != 0 || (R[i][p].fb == nullptr) != (R[0][p].fb == nullptr))
+        return set_err(MGE_ERR_STATE, "cases of a batch must follow the same cycle");
+    if (!R[0][p].fb) {
+      groups.push_back(Group{p, {}});
+      continue;
+    }
+    size_t first_group = groups.size();
+    for (int i = 0; i < nc; i++) {
+      const Rec &r = R[i][p];
+      size_t gsel = groups.size();
+      for (size_t g = first_group; g < groups.size(); g++) {
+        const Rec &q = R[groups[g].cases[0]][p];
+        if (q.fb == r.fb && q.block.x == r.block.x && q.block.y == r.block.y && q.block.z == r.block.z &&
+            q.args.size() == r.args.size()) {
+          gsel = g;
+          break;
+        }
+      }
+      if (gsel == groups.size()) groups.push_back(Group{p, {}});
+      groups[gsel].cases.push_back(i);
+    }
+    for (size_t g = first_group; g < groups.size(); g++)
+      total += al(R[groups[g].cases[0]][p].args.size() * groups[g].cases.size()) +
+               al(sizeof(int4) * groups[g].cases.size());
+  }
+  if (total > b->arena_cap) {
+    if (b->d_arena) hipFree(b->d_arena);
+    b->d_arena = nullptr;
+    b->arena_cap = 0;
+    if (hipMalloc(&b->d_arena, total) != hipSuccess) return set_err(MGE_ERR_NOMEM, "batch arena allocation failed");
+    b->arena_cap = total;
+  }
+  std::vector<char> host(total + 1, 0);
+  size_t off = 0;
+  for (const Group &G : groups) {
+    const size_t p = G.p;
+    if (G.cases.empty()) {
+      mge_batch::Step st{};
+      st.fb = nullptr;
+      st.slot = (int)R[0][p].grid.x;
+      b->prog.push_back(st);
+      continue;
+    }
+    const int ng = (int)G.cases.size();
+    const Rec &r0 = R[G.cases[0]][p];
+    const size_t as = r0.args.size();
+    const size_t aoff = off, coff = off + al(as * ng);
+    off = coff + al(sizeof(int4) * ng);
+    long first = 0;
+    for (int k = 0; k < ng; k++) {
+      const Rec &r = R[G.cases[k]][p];
+      std::memcpy(&host[aoff + as * k], r.args.data(), as);
+      long n = (long)r.grid.x * r.grid.y * r.grid.z;
+      int4 g = make_int4((int)first, (int)r.grid.x, (int)r.grid.y, b->prog_slots[G.cases[k]]);
+      std::memcpy(&host[coff + sizeof(int4) * k], &g, sizeof(int4));
+      first += n;
+    }
+    if (first == 0) continue;  // no case of the group launches here
+    if (first > 0x7fffffffL) return set_err(MGE_ERR_ARG, "batch launch too large");
+    mge_batch::Step st{};
+    st.fb = r0.fb;
+    st.block = r0.block;
+    st.nblocks = (unsigned)first;
+    st.args = b->d_arena + aoff;
+    st.cg = (const int4 *)(b->d_arena + coff);
+    st.nc = ng;
+    b->prog.push_back(st);
+  }
+  if (total) HIPCHK(hipMemcpy(b->d_arena, host.data(), total, hipMemcpyHostToDevice));
+  if (b->use_graph) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc < 0) return rc;
+    if (e != hipSuccess) return set_err(MGE_ERR_HIP, "hipStreamEndCapture failed");
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  return MGE_OK;
+}
+
+int batch_pull(mge_batch *b) {
+  const int n = (int)b->cs.size();
+  HIPCHK(hipMemcpyAsync(b->hctl.data(), b->d_ctl, sizeof(BCtl) * n, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(b->hact.data(), b->d_act, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return MGE_OK;
+}
+
+int batch_push(mge_batch *b) {
+  const int n = (int)b->cs.size();
+  HIPCHK(hipMemcpyAsync(b->d_ctl, b->hctl.data(), sizeof(BCtl) * n, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->d_act, b->hact.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return MGE_OK;
 }
 
@@ -1111,6 +1447,147 @@ int mge_wall_force(mge_case *h, int lvl, double out[3]) {
   by_type(h, [&](auto z) { return do_wall_force<decltype(z)>(h, *L); });
   HIPCHK(hipMemcpyAsync(out, L->force, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
   return check_err(h);
+}
+
+// ---- batches ---------------------------------------------------------------
+
+int mge_batch_create(int use_graph, int log_cap, mge_batch **out) {
+  if (!out || log_cap < 0) return set_err(MGE_ERR_ARG, "bad batch arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(MGE_ERR_HIP, "no HIP device");
+  mge_batch *b = new mge_batch();
+  b->use_graph = use_graph != 0;
+  b->cap = log_cap > 0 ? log_cap : 1;
+  if (hipStreamCreate(&b->stream) != hipSuccess) {
+    delete b;
+    return set_err(MGE_ERR_HIP, "hipStreamCreate failed");
+  }
+  *out = b;
+  return MGE_OK;
+}
+
+int mge_batch_destroy(mge_batch *b) {
+  if (!b) return MGE_OK;
+  hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_arena) hipFree(b->d_arena);
+  if (b->d_ctl) hipFree(b->d_ctl);
+  if (b->d_act) hipFree(b->d_act);
+  if (b->d_log) hipFree(b->d_log);
+  hipStreamDestroy(b->stream);
+  delete b;
+  return MGE_OK;
+}
+
+int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc) {
+  if (!b || !h || maxcyc < 1) return set_err(MGE_ERR_ARG, "bad batch arguments");
+  for (int l = 1; l <= h->nlevel; l++)
+    if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
+  if (h->lv[h->nlevel].mxcyc < 2) return set_err(MGE_ERR_ARG, "a batched case needs mxcyc >= 2");
+  const int sm = h->prm.smoo > 0.0;
+  if (b->fp32 < 0) {
+    b->nlevel = h->nlevel;
+    b->fp32 = h->fp32;
+    b->distype = h->prm.distype;
+    b->dissip = h->prm.dissip;
+    b->mstage = h->prm.mstage;
+    b->mgtype = h->prm.mgtype;
+    b->smoothed = sm;
+  } else if (b->nlevel != h->nlevel || b->fp32 != (int)h->fp32 || b->distype != h->prm.distype ||
+             b->dissip != h->prm.dissip || b->mstage != h->prm.mstage || b->mgtype != h->prm.mgtype ||
+             b->smoothed != sm) {
+    return set_err(MGE_ERR_ARG, "cases of a batch need the same levels, scheme, stages, cycle and precision");
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  int slot = (int)b->cs.size();
+  CHKRC(batch_grow(b, slot + 1));
+  b->cs.push_back(h);
+  BCtl &C = b->hctl[slot];
+  double *log = C.log;
+  C = BCtl{};
+  C.log = log;
+  C.rtrms = h->lv[h->nlevel].rtrms;
+  C.err = h->err;
+  C.drop = drop;
+  C.maxcyc = maxcyc;
+  C.cap = b->cap;
+  b->hact[slot] = 1;
+  CHKRC(batch_push(b));
+  b->dirty = true;
+  return slot;
+}
+
+int mge_batch_remove(mge_batch *b, int slot) {
+  if (!b || slot < 0 || slot >= (int)b->cs.size()) return set_err(MGE_ERR_ARG, "bad batch slot");
+  CHKRC(batch_pull(b));
+  b->hact[slot] = 0;
+  b->cs[slot] = nullptr;
+  CHKRC(batch_push(b));
+  b->dirty = true;
+  return MGE_OK;
+}
+
+int mge_batch_run(mge_batch *b, int periods, int *nactive) {
+  if (!b || periods < 0) return set_err(MGE_ERR_ARG, "bad batch arguments");
+  for (int p = 0; p < periods; p++) {
+    // rebuild when members changed or a quarter of the program has stopped
+    int stopped = 0;
+    for (int s : b->prog_slots)
+      if (!b->hact[s]) stopped++;
+    if (b->dirty || (stopped > 0 && 4 * stopped >= (int)b->prog_slots.size())) CHKRC(batch_build(b));
+    if (b->prog.empty()) break;
+    const int np = (int)b->prog_slots.size();
+    std::vector<int> n0(np), a0(np);
+    for (int i = 0; i < np; i++) {
+      n0[i] = b->hctl[b->prog_slots[i]].ncyc;
+      a0[i] = b->hact[b->prog_slots[i]];
+    }
+    if (b->use_graph) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    else CHKRC(batch_enqueue(b));
+    CHKRC(batch_pull(b));
+    // every case active at the start of a period runs at least one cycle
+    for (int i = 0; i < np; i++)
+      if (a0[i] && b->hctl[b->prog_slots[i]].ncyc == n0[i])
+        return set_err(MGE_ERR_STATE, "batch period made no progress");
+    int alive = 0;
+    for (int i = 0; i < np; i++) {
+      int s = b->prog_slots[i];
+      if (a0[i] && !b->hact[s] && b->hctl[s].ncyc - n0[i] == 1) {
+        // stopped after the first cycle of the period: its buffers are
+        // in the rotation recorded after cycle 1
+        mge_case *h = b->cs[s];
+        for (int l = 1; l <= h->nlevel; l++) h->lv[l].cur = b->cur1[i][l - 1];
+      }
+      alive += b->hact[s];
+    }
+    if (!alive) break;
+  }
+  if (nactive) {
+    int a = 0;
+    for (int c = 0; c < (int)b->cs.size(); c++) a += b->cs[c] && b->hact[c];
+    *nactive = a;
+  }
+  return MGE_OK;
+}
+
+int mge_batch_status(mge_batch *b, int slot, int *status, int *ncyc, double *ini, double *last) {
+  if (!b || slot < 0 || slot >= (int)b->cs.size()) return set_err(MGE_ERR_ARG, "bad batch slot");
+  const BCtl &C = b->hctl[slot];
+  if (status) *status = b->hact[slot] ? 0 : (C.status ? C.status : 3);
+  if (ncyc) *ncyc = C.ncyc;
+  if (ini) *ini = C.ini;
+  if (last) *last = C.last;
+  return MGE_OK;
+}
+
+int mge_batch_log(mge_batch *b, int slot, double *out, int cap) {
+  if (!b || slot < 0 || slot >= (int)b->cs.size() || !out) return set_err(MGE_ERR_ARG, "bad batch slot");
+  int n = std::min(std::min(b->hctl[slot].ncyc, b->cap), cap);
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(out, b->hctl[slot].log, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  return n;
 }
 
 int mge_synchronize(mge_case *h) {

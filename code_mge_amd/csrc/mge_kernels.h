@@ -61,6 +61,58 @@ __device__ inline T face_s(const Geo<T> &g, int i, int j, int k) {
   return -(g.dx[i] * g.dy[j]);
 }
 
+// ------------------------------------------------------------------ entry points
+//
+// Every kernel of the multigrid cycle is written as NAME_body(args, bx, by,
+// bz) and gets two entry points:
+//   NAME    one case; the block coordinates are blockIdx
+//   NAME_b  a batch of cases in one launch: the 1-D grid is the
+//           concatenation of the cases' own grids; cg[c] = {first block,
+//           grid x, grid y, batch slot} of case c (sorted by first block)
+//           and a block finds its case by binary search.  A case whose
+//           slot is inactive (act[slot] == 0) returns at once, so a batch
+//           stops each case exactly at its own convergence.
+// Each case's blocks compute exactly what its single-case launch computes.
+__device__ inline int batch_case(const int4 *cg, int nc, int b) {
+  int lo = 0, hi = nc - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (cg[mid].x <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define MGE_ENTRY_T(NAME, ARGS, NT, MINB)                                                              \
+  template <typename T>                                                                                \
+  __global__ void __launch_bounds__(NT, MINB) NAME(ARGS<T> A) {                                        \
+    NAME##_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);                                 \
+  }                                                                                                    \
+  template <typename T>                                                                                \
+  __global__ void __launch_bounds__(NT, MINB) NAME##_b(const ARGS<T> *As, const int4 *cg, int nc,      \
+                                                       const int *act) {                               \
+    const int b = (int)blockIdx.x;                                                                     \
+    const int c = batch_case(cg, nc, b);                                                               \
+    const int4 g = cg[c];                                                                              \
+    if (!act[g.w]) return;                                                                             \
+    const int l = b - g.x;                                                                             \
+    NAME##_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));                                     \
+  }
+
+#define MGE_ENTRY(NAME, ARGS, NT, MINB)                                                                \
+  __global__ void __launch_bounds__(NT, MINB) NAME(ARGS A) {                                           \
+    NAME##_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);                                 \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(NT, MINB) NAME##_b(const ARGS *As, const int4 *cg, int nc,         \
+                                                       const int *act) {                               \
+    const int b = (int)blockIdx.x;                                                                     \
+    const int c = batch_case(cg, nc, b);                                                               \
+    const int4 g = cg[c];                                                                              \
+    if (!act[g.w]) return;                                                                             \
+    const int l = b - g.x;                                                                             \
+    NAME##_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));                                     \
+  }
+
 template <typename T>
 __device__ inline T dimd(T a, T b) { return a > b ? a - b : T(0); }
 template <typename T>

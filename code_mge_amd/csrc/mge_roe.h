@@ -321,14 +321,15 @@ __device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256, 2) k_stage_roe(StageArgs<T> A, int kc) {
+__device__ __attribute__((always_inline)) inline void k_stage_roe_body(const StageArgs<T> &A, int bx, int by, int bz) {
   __shared__ RoeSmem<T> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
   const long E = d.E;
+  const int kc = A.kc;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
-  const int i0 = 2 + blockIdx.x * TI, j0 = 2 + blockIdx.y * TJ;
-  const int kb0 = 2 + blockIdx.z * kc;
+  const int i0 = 2 + bx * TI, j0 = 2 + by * TJ;
+  const int kb0 = 2 + bz * kc;
   const int kb1 = min(kb0 + kc - 1, d.kl);
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = i <= d.il && j <= d.jl;
@@ -488,5 +489,6 @@ __global__ void __launch_bounds__(256, 2) k_stage_roe(StageArgs<T> A, int kc) {
     }
   }
 }
+MGE_ENTRY_T(k_stage_roe, StageArgs, 256, 2)
 
 }  // namespace mge

@@ -30,10 +30,10 @@ struct SmoothArgs {
 // lane-boundary values differ from the sequential evaluation (by a few
 // ulp); blank cells keep their value as in the reference.
 template <int CPL, typename T>
-__global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs<T> S) {
+__device__ __attribute__((always_inline)) inline void smooth_x_scan(const SmoothArgs<T> &S, int bx) {
   const Dims &d = S.d;
   const int lane = threadIdx.x & 63;
-  const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+  const long wave = (bx * (long)blockDim.x + threadIdx.x) >> 6;
   const long nl = (long)d.nj * d.nk;
   if (wave >= nl * 5) return;  // uniform per wave
   const int n = (int)(wave / nl);
@@ -121,9 +121,9 @@ __global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs<T> S) {
 
 // sequential x lines for blocks wider than 512 cells
 template <typename T>
-__global__ void __launch_bounds__(256) k_smooth_x_seq(SmoothArgs<T> S) {
+__device__ __attribute__((always_inline)) inline void smooth_x_seq(const SmoothArgs<T> &S, int bx) {
   const Dims &d = S.d;
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long t = bx * (long)blockDim.x + threadIdx.x;
   long nl = (long)d.nj * d.nk;
   if (t >= nl * 5) return;
   int n = (int)(t / nl);
@@ -139,6 +139,28 @@ __global__ void __launch_bounds__(256) k_smooth_x_seq(SmoothArgs<T> S) {
     if (S.flags[b0 + ir] & F_FLUID) D[b0 + ir] = D[b0 + ir] - S.e2[ir] * D[b0 + ir + 1];
   }
 }
+
+// x-line entry points: one per scan width (cells per lane, chosen from
+// the line length) and the sequential form; a batch launches each width
+// once over the cases that use it
+template <int CPL, typename T>
+__global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs<T> S) {
+  smooth_x_scan<CPL>(S, (int)blockIdx.x);
+}
+template <int CPL, typename T>
+__global__ void __launch_bounds__(256) k_smooth_x_scan_b(const SmoothArgs<T> *As, const int4 *cg, int nc,
+                                                         const int *act) {
+  const int b = (int)blockIdx.x;
+  const int c = batch_case(cg, nc, b);
+  const int4 g = cg[c];
+  if (!act[g.w]) return;
+  smooth_x_scan<CPL>(As[c], b - g.x);
+}
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_smooth_x_seq_body(const SmoothArgs<T> &S, int bx, int, int) {
+  smooth_x_seq(S, bx);
+}
+MGE_ENTRY_T(k_smooth_x_seq, SmoothArgs, 256, 1)
 
 // One line of the exact sequential recurrence of psmoo_old
 // (psmoo.f90:190-237): cells s = 2..nl at stride st.  Full chunks of CH
@@ -228,33 +250,48 @@ __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__
 
 // psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
 template <typename T>
-__global__ void __launch_bounds__(64) k_smooth_y(SmoothArgs<T> S) {
+__device__ __attribute__((always_inline)) inline void k_smooth_y_body(const SmoothArgs<T> &S, int bx, int by, int bz) {
   const Dims &d = S.d;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int k = 2 + blockIdx.y;
-  int n = blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int k = 2 + by;
+  int n = bz;
   if (i > d.il) return;
   line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
 }
+MGE_ENTRY_T(k_smooth_y, SmoothArgs, 64, 1)
 
 // psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
 template <typename T>
-__global__ void __launch_bounds__(64) k_smooth_z(SmoothArgs<T> S) {
+__device__ __attribute__((always_inline)) inline void k_smooth_z_body(const SmoothArgs<T> &S, int bx, int by, int bz) {
   const Dims &d = S.d;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y;
-  int n = blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by;
+  int n = bz;
   if (i > d.il) return;
   line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
 }
+MGE_ENTRY_T(k_smooth_z, SmoothArgs, 64, 1)
 
 // after smoothing: w = wn - dw and the EOS on interior cells
 // (euler.f90:381-399); the ghost shell of the previous state is carried
 // over, as k_ghost_copy does on unsmoothed stages
 template <typename T>
-__global__ void __launch_bounds__(256) k_update_ghost(Dims d, const T *wn, const T *dw, StateC<T> src, State<T> dst,
-                                                      T gamma) {
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+struct UpdArgs {
+  Dims d;
+  const T *wn, *dw;
+  StateC<T> src;
+  State<T> dst;
+  T gamma;
+};
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_update_ghost_body(const UpdArgs<T> &U, int bx, int, int) {
+  const Dims &d = U.d;
+  const T *wn = U.wn, *dw = U.dw;
+  const StateC<T> &src = U.src;
+  const State<T> &dst = U.dst;
+  const T gamma = U.gamma;
+  long q = bx * (long)blockDim.x + threadIdx.x;
   if (q >= d.E) return;
   int k = (int)(q / d.sz);
   int j = (int)((q - (long)k * d.sz) / d.sy);
@@ -272,5 +309,6 @@ __global__ void __launch_bounds__(256) k_update_ghost(Dims d, const T *wn, const
     dst.p[q] = src.p[q];
   }
 }
+MGE_ENTRY_T(k_update_ghost, UpdArgs, 256, 1)
 
 }  // namespace mge

@@ -69,6 +69,7 @@ struct StageArgs {
   int order;                // Roe schemes: 1 first order, 2 MUSCL-minmod
   T eps;                    // Roe entropy-fix coefficient
   T g1;                     // gamma/(gamma-1)
+  int kc;                   // k planes per workgroup (LDS-tiled kernels)
 };
 
 template <typename T>
@@ -150,11 +151,11 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
 // Used for matrix dissipation (whose per-face state is too large for the
 // LDS-tiled kernel's register budget).
 template <typename T>
-__global__ void __launch_bounds__(256) k_stage(StageArgs<T> A) {
+__device__ __attribute__((always_inline)) inline void k_stage_body(const StageArgs<T> &A, int bx, int by, int bz) {
   const Dims &d = A.fx.d;
-  int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 2 + blockIdx.z;
+  int i = 2 + bx * blockDim.x + threadIdx.x;
+  int j = 2 + by * blockDim.y + threadIdx.y;
+  int k = 2 + bz;
   if (i > d.il || j > d.jl || k > d.kl) return;
   const long E = d.E;
   long q = ix(d, i, j, k);
@@ -209,6 +210,7 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs<T> A) {
   }
   finish_cell(A, q, f, e);
 }
+MGE_ENTRY_T(k_stage, StageArgs, 256, 1)
 
 // ------------------------------------------------------------------ k_stage2
 //
@@ -260,15 +262,16 @@ __device__ inline void cell_flux(const FluxCtx<T> &c, int i, int j, int k, T *o)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256, 2) k_stage2(StageArgs<T> A, int kc) {
+__device__ __attribute__((always_inline)) inline void k_stage2_body(const StageArgs<T> &A, int gbx, int gby, int gbz) {
   __shared__ StageSmem<T> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
   const long E = d.E;
+  const int kc = A.kc;
   const T two = T(2), half = T(0.5);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
-  const int i0 = 2 + blockIdx.x * TI, j0 = 2 + blockIdx.y * TJ;
-  const int kb0 = 2 + blockIdx.z * kc;
+  const int i0 = 2 + gbx * TI, j0 = 2 + gby * TJ;
+  const int kb0 = 2 + gbz * kc;
   const int kb1 = min(kb0 + kc - 1, d.kl);
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = i <= d.il && j <= d.jl;
@@ -470,6 +473,7 @@ __global__ void __launch_bounds__(256, 2) k_stage2(StageArgs<T> A, int kc) {
     __syncthreads();
   }
 }
+MGE_ENTRY_T(k_stage2, StageArgs, 256, 2)
 
 // ------------------------------------------------------------------ wall records
 
@@ -680,8 +684,8 @@ __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
 // thread per op; k_rec_apply adds them in the reference order.  The undo
 // terms repeat the bulk kernel's face values in T.
 template <typename T>
-__global__ void __launch_bounds__(64) k_rec_ops(RecArgs<T> R) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __attribute__((always_inline)) inline void k_rec_ops_body(const RecArgs<T> &R, int bx, int, int) {
+  int o = bx * blockDim.x + threadIdx.x;
   if (o >= R.nops) return;
   const StageArgs<T> &A = R.st;
   int2 op = R.ops[o];
@@ -756,12 +760,13 @@ __global__ void __launch_bounds__(64) k_rec_ops(RecArgs<T> R) {
       break;
   }
 }
+MGE_ENTRY_T(k_rec_ops, RecArgs, 64, 1)
 
 // per touched cell: apply the op values in record order (in FP64), then
 // finish in T
 template <typename T>
-__global__ void __launch_bounds__(64) k_rec_apply(RecArgs<T> R) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __attribute__((always_inline)) inline void k_rec_apply_body(const RecArgs<T> &R, int bx, int, int) {
+  int t = bx * blockDim.x + threadIdx.x;
   if (t >= R.ntouch) return;
   const StageArgs<T> &A = R.st;
   const Dims &d = A.fx.d;
@@ -813,5 +818,6 @@ __global__ void __launch_bounds__(64) k_rec_apply(RecArgs<T> R) {
   }
   finish_cell(A, q, ft, et);
 }
+MGE_ENTRY_T(k_rec_apply, RecArgs, 64, 1)
 
 }  // namespace mge

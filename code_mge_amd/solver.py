@@ -161,3 +161,63 @@ def force_coefficients(case: Case, ref_area: float):
     dd = np.array([np.cos(a) * np.cos(b), np.sin(a) * np.cos(b), np.sin(b)])
     ll = np.array([-np.sin(a), np.cos(a), 0.0])
     return float(F @ dd / (q * ref_area)), float(F @ ll / (q * ref_area))
+
+
+class Batch:
+    """Many cases advanced together (mge_batch_*): one launch per operation
+    for all cases, replayed as a hipGraph.  Each added case (initialised,
+    same levels/scheme/stages/cycle/precision as the others) runs multigrid
+    cycles on its finest level until its residual has dropped by `drop`,
+    or maxcyc cycles; it then holds the state its own run would hold."""
+
+    STATUS = {0: "running", 1: "converged", 2: "cycle limit", 3: "removed"}
+
+    def __init__(self, use_graph=True, log_cap=2000):
+        self.lib = _ffi.lib()
+        h = C.c_void_p()
+        _ffi.check(self.lib.mge_batch_create(int(use_graph), log_cap, C.byref(h)), "mge_batch_create")
+        self.h = h
+        self.cases = {}
+
+    def close(self):
+        if self.h:
+            self.lib.mge_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, case: Case, drop: float, maxcyc: int) -> int:
+        slot = _ffi.check(self.lib.mge_batch_add(self.h, case.h, float(drop), int(maxcyc)), "mge_batch_add")
+        self.cases[slot] = case
+        return slot
+
+    def remove(self, slot: int):
+        _ffi.check(self.lib.mge_batch_remove(self.h, slot), "mge_batch_remove")
+        self.cases.pop(slot, None)
+
+    def run(self, periods: int = 1) -> int:
+        n = C.c_int(0)
+        _ffi.check(self.lib.mge_batch_run(self.h, int(periods), C.byref(n)), "mge_batch_run")
+        return n.value
+
+    def status(self, slot: int):
+        st, nc = C.c_int(0), C.c_int(0)
+        ini, last = C.c_double(0.0), C.c_double(0.0)
+        _ffi.check(self.lib.mge_batch_status(self.h, slot, C.byref(st), C.byref(nc), C.byref(ini), C.byref(last)),
+                   "mge_batch_status")
+        return dict(status=st.value, ncyc=nc.value, ini=ini.value, last=last.value)
+
+    def log(self, slot: int, cap: int = 100000):
+        out = np.zeros(cap)
+        n = _ffi.check(self.lib.mge_batch_log(self.h, slot, out.ctypes.data_as(_dp), cap), "mge_batch_log")
+        return out[:n]

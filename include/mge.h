@@ -173,6 +173,32 @@ int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, lon
                       double *bytes);
 const char *mge_last_error(void);
 
+/* ---- batches of cases --------------------------------------------------
+ * No counterpart in the reference (it advances one case per process); for
+ * callers that solve many independent cases: throughput runs and the
+ * serving loop.  A batch advances its cases together, two multigrid cycles
+ * (finest level, no full-multigrid ramp) per period, with one launch per
+ * operation for all cases, replayed as a hipGraph when use_graph != 0.
+ * Its cases must share the number of levels, scheme, stages, cycle type,
+ * smoothing on/off and precision; they may differ in grid, body and flow
+ * parameters.  Each case stops at the first cycle whose finest-level
+ * residual has dropped by more than `drop` from its first cycle (the drop
+ * test of srcf90/solver.f90:238), at maxcyc cycles, or on an error; it then
+ * holds exactly the state its own single-case run of those cycles holds.
+ * The cases stay owned by the caller: remove a case before destroying it. */
+typedef struct mge_batch mge_batch;
+int mge_batch_create(int use_graph, int log_cap, mge_batch **out);
+int mge_batch_destroy(mge_batch *b);
+/* returns the case's slot (>= 0); the case must have been initialised */
+int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc);
+int mge_batch_remove(mge_batch *b, int slot);
+/* run up to `periods` periods (2 cycles each) or until no case is active */
+int mge_batch_run(mge_batch *b, int periods, int *nactive);
+/* status: 0 running, 1 converged, 2 cycle limit, 3 removed, < 0 error code */
+int mge_batch_status(mge_batch *b, int slot, int *status, int *ncyc, double *ini, double *last);
+/* residual history of a slot (one value per cycle); returns the count */
+int mge_batch_log(mge_batch *b, int slot, double *out, int cap);
+
 #ifdef __cplusplus
 }
 #endif
