@@ -6,6 +6,7 @@ cylinder2d: inviscid flow past a circular cylinder (radius 0.5, axis z) in
   time step); smoothing eps = 0.6; W-cycle.  n = 128 is the 128x128x4
   three-level case of the measurement plan.
 sphere3d: sphere radius 0.5 in [-8,8]^3 with far field on all faces.
+ellipsoid_sweep: the batched-throughput sweep of seeded random ellipsoids.
 """
 import math
 
@@ -57,3 +58,35 @@ def random_rotation(rng):
         [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
         [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
     ])
+
+
+def quaternion_rotation(q):
+    """rotation matrix of the unit quaternion q = (w, x, y, z)"""
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def ellipsoid_sweep(count=64, seed=20261202, nlevel=3, box=8.0, **kw):
+    """`count` Euler cases from numpy PCG64(seed), drawn per case in the
+    order: semi-axes a, b, c ~ U[0.2, 1.0]; a uniformly random rotation
+    (unit quaternion of 4 normals); N ~ exp U[ln 48, ln 160] rounded to a
+    multiple of 8 cells per edge; M ~ U[0.1, 0.8]; alpha ~ U[-5, 15] deg.
+    Box [-box, box]^3, far field on every face, `nlevel` levels.  Returns a
+    list of (params, levels, body, ref_area, description)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        semi = tuple(float(v) for v in rng.uniform(0.2, 1.0, size=3))
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        n = int(8 * round(math.exp(rng.uniform(math.log(48.0), math.log(160.0))) / 8.0))
+        mach = float(rng.uniform(0.1, 0.8))
+        alpha = float(rng.uniform(-5.0, 15.0))
+        prm, levels, bdy, aref = ellipsoid3d(n=n, nlevel=nlevel, semi=semi, rot=quaternion_rotation(q), mach=mach,
+                                             alpha=alpha, box=box, **kw)
+        out.append((prm, levels, bdy, aref, dict(n=n, semi=semi, mach=mach, alpha=alpha)))
+    return out

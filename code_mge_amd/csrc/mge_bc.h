@@ -22,6 +22,7 @@ struct TsArgs {
   T viscl, prlmean;
   T cfac;  // viscous factor of the scheme (timestep.f90:119-124)
   int *err;
+  Inf<T> inf;
 };
 
 // timestep.f90:36-115 (Euler) and :117-209 (NS): spectral radii, local
@@ -39,13 +40,20 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
   if (i <= d.il && j <= d.jl && k <= d.kl) {
     const long E = d.E;
     long q = ix(d, i, j, k);
-    T rho = A.s.w[q];
-    T cc = A.gamma * A.s.p[q] / rho;
+    T rho = A.s.w[q], pq = A.s.p[q], m1 = A.s.w[E + q], m2 = A.s.w[2 * E + q], m3 = A.s.w[3 * E + q];
+    if constexpr (kDelta<T>) {
+      rho = rho + A.inf.w[0];
+      pq = pq + A.inf.p;
+      m1 = m1 + A.inf.w[1];
+      m2 = m2 + A.inf.w[2];
+      m3 = m3 + A.inf.w[3];
+    }
+    T cc = A.gamma * pq / rho;
     if (cc < T(0) || cc != cc) raise_err(A.err, MGE_ERR_NEG_C2);
     cc = sqrt(maxd(cc, T(0)));
-    T uu = A.s.w[E + q] / rho;
-    T uv = A.s.w[2 * E + q] / rho;
-    T uw = A.s.w[3 * E + q] / rho;
+    T uu = m1 / rho;
+    T uv = m2 / rho;
+    T uw = m3 / rho;
     T dtv = T(0);
     if (A.eqntype == 1) {
       T fmue = A.viscl / A.prlmean;
@@ -79,9 +87,11 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
     T pip = P[q + 1], pim = P[q - 1];
     T pjp = P[q + d.sy], pjm = P[q - d.sy];
     T pkp = P[q + d.sz], pkm = P[q - d.sz];
-    T dpi = fabs((pip - two * pc + pim) / (pip + two * pc + pim + plim));
-    T dpj = fabs((pjp - two * pc + pjm) / (pjp + two * pc + pjm + plim));
-    T dpk = fabs((pkp - two * pc + pkm) / (pkp + two * pc + pkm + plim));
+    T four_p = T(0);
+    if constexpr (kDelta<T>) four_p = T(4) * A.inf.p;
+    T dpi = fabs((pip - two * pc + pim) / (pip + two * pc + pim + four_p + plim));
+    T dpj = fabs((pjp - two * pc + pjm) / (pjp + two * pc + pjm + four_p + plim));
+    T dpk = fabs((pkp - two * pc + pkm) / (pkp + two * pc + pkm + four_p + plim));
     T cm1 = dimd(A.cfl, one);
     T sw = two * (dpi + dpj + dpk * one);
     T r = one / (one + (cm1 < sw ? cm1 : sw));
@@ -209,6 +219,7 @@ template <typename T>
 struct CutArgs {
   Dims d;
   StateC<T> s;
+  Inf<T> inf;
   const DevRec *recs;
   double *wsol, *psol;
   int nmir, nupd, zero_at;  // records < nupd are updated; zero_at gets zeros
@@ -239,8 +250,8 @@ __device__ __attribute__((always_inline)) inline void k_bc_cut_body(const CutArg
   for (int dd = 0; dd < 3; dd++) nv[dd] = nv[dd] / nn;
   double vr[5] = {0, 0, 0, 0, 0}, pr = 0.0;
   for (int n = 0; n < 5; n++)
-    for (int c = 0; c < r.npoint; c++) vr[n] = vr[n] + r.coef[c] * (double)C.s.w[n * E + r.cloud[c]];
-  for (int c = 0; c < r.npoint; c++) pr = pr + r.coef[c] * (double)C.s.p[r.cloud[c]];
+    for (int c = 0; c < r.npoint; c++) vr[n] = vr[n] + r.coef[c] * full(C.inf, n, C.s.w[n * E + r.cloud[c]]);
+  for (int c = 0; c < r.npoint; c++) pr = pr + r.coef[c] * full(C.inf, 5, C.s.p[r.cloud[c]]);
   double psol = pr, ws[5];
   for (int n = 0; n < 5; n++) ws[n] = vr[n];
   if (C.eqntype == 0) {
@@ -271,14 +282,17 @@ struct BcArgs {
   State<T> s;
   int face;      // 1..6
   int kind;      // -20 far, -50 symmetry, -60 periodic, -80 wall
-  T gamma, rho0, p0, c0, u0, v0, w0;
+  double gamma, rho0, p0, c0, u0, v0, w0;
+  Inf<T> inf;
 };
 
 // one face of bcond.f90:16-39 -> bc_far.f90 / bc_symm.f90 / bc_periodic.f90 /
-// bc_wall.f90, over the whole (n+4)^2 face like the reference loops
+// bc_wall.f90, over the whole (n+4)^2 face like the reference loops.
+// Evaluated in FP64 on full state values (FP32 stores deviations).
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArgs<T> &B, int bx, int by, int) {
   const Dims &d = B.d;
+  const Inf<T> &f = B.inf;
   int dir = (B.face - 1) % 3;
   int amax = dir == 0 ? d.jb : d.ib;
   int cmax = dir == 2 ? d.jb : d.kb;
@@ -292,65 +306,67 @@ __device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArg
   };
   const long E = d.E;
   T *W = B.s.w, *P = B.s.p;
+  auto Wf = [&](int n, long q) { return full(f, n, W[n * E + q]); };
   if (B.kind == -20) {
-    const T one = T(1), two = T(2), half = T(0.5), quarter = T(0.25);
     int phy1, dum1, dum2;
     if (B.face <= 3) { phy1 = 2; dum1 = 1; dum2 = 0; }
     else { phy1 = nl; dum1 = ne; dum2 = nb; }
-    T fn[3] = {T(0), T(0), T(0)};
-    fn[dir] = B.face <= 3 ? -one : one;
-    T vinf[3] = {B.u0, B.v0, B.w0};
-    const T gamma = B.gamma, gm = one / (gamma - one);
+    double fn[3] = {0.0, 0.0, 0.0};
+    fn[dir] = B.face <= 3 ? -1.0 : 1.0;
+    double vinf[3] = {B.u0, B.v0, B.w0};
+    const double gamma = B.gamma, gm = 1.0 / (gamma - 1.0);
     long q1 = L(phy1), qd1 = L(dum1), qd2 = L(dum2);
-    T s = T(0);
-    s = s + W[E + q1] * fn[0];
-    s = s + W[2 * E + q1] * fn[1];
-    s = s + W[3 * E + q1] * fn[2];
-    T vel_norm = s / W[q1];
-    T r, vel[3], pr;
-    if (vel_norm <= T(0)) {
-      T rhoe = W[q1];
-      T ue = W[(1 + dir) * E + q1] / rhoe;
-      T pe = P[q1];
-      T ce = sqrt(gamma * pe / rhoe);
-      T rpo = ue * fn[dir] + two * ce / (gamma - one);
-      T rmi = vinf[dir] * fn[dir] - two * B.c0 / (gamma - one);
-      T un = half * (rpo + rmi) * fn[dir];
-      T ci = quarter * (gamma - one) * (rpo - rmi);
+    double s = 0.0;
+    s = s + Wf(1, q1) * fn[0];
+    s = s + Wf(2, q1) * fn[1];
+    s = s + Wf(3, q1) * fn[2];
+    double vel_norm = s / Wf(0, q1);
+    double r, vel[3], pr;
+    if (vel_norm <= 0.0) {
+      double rhoe = Wf(0, q1);
+      double ue = Wf(1 + dir, q1) / rhoe;
+      double pe = full(f, 5, P[q1]);
+      double ce = sqrt(gamma * pe / rhoe);
+      double rpo = ue * fn[dir] + 2.0 * ce / (gamma - 1.0);
+      double rmi = vinf[dir] * fn[dir] - 2.0 * B.c0 / (gamma - 1.0);
+      double un = 0.5 * (rpo + rmi) * fn[dir];
+      double ci = 0.25 * (gamma - 1.0) * (rpo - rmi);
       vel[0] = B.u0; vel[1] = B.v0; vel[2] = B.w0;
       vel[dir] = un;
-      T si = pow(B.rho0, gamma) / B.p0;
-      T cc = ci * ci / gamma;
+      double si = pow(B.rho0, gamma) / B.p0;
+      double cc = ci * ci / gamma;
       r = pow(si * cc, gm);
       pr = r * cc;
     } else {
-      T rhoi = W[q1];
-      T vv[3] = {W[E + q1] / rhoi, W[2 * E + q1] / rhoi, W[3 * E + q1] / rhoi};
-      T prei = P[q1];
-      T si = pow(rhoi, gamma) / prei;
-      T ci = sqrt(gamma * prei / rhoi);
-      T rpo = vv[dir] * fn[dir] + two * ci / (gamma - one);
-      T rmi = vinf[dir] * fn[dir] - two * B.c0 / (gamma - one);
-      T un = half * (rpo + rmi) * fn[dir];
-      T ce = quarter * (gamma - one) * (rpo - rmi);
+      double rhoi = Wf(0, q1);
+      double vv[3] = {Wf(1, q1) / rhoi, Wf(2, q1) / rhoi, Wf(3, q1) / rhoi};
+      double prei = full(f, 5, P[q1]);
+      double si = pow(rhoi, gamma) / prei;
+      double ci = sqrt(gamma * prei / rhoi);
+      double rpo = vv[dir] * fn[dir] + 2.0 * ci / (gamma - 1.0);
+      double rmi = vinf[dir] * fn[dir] - 2.0 * B.c0 / (gamma - 1.0);
+      double un = 0.5 * (rpo + rmi) * fn[dir];
+      double ce = 0.25 * (gamma - 1.0) * (rpo - rmi);
       vel[0] = vv[0]; vel[1] = vv[1]; vel[2] = vv[2];
       vel[dir] = un;
-      T cc = ce * ce / gamma;
+      double cc = ce * ce / gamma;
       r = pow(si * cc, gm);
       pr = r * cc;
     }
-    T o[5];
+    double o[5];
     o[0] = r;
     o[1] = r * vel[0];
     o[2] = r * vel[1];
     o[3] = r * vel[2];
-    o[4] = pr * gm + half * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    o[4] = pr * gm + 0.5 * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
     for (int n = 0; n < 5; n++) {
-      W[n * E + qd1] = o[n];
-      W[n * E + qd2] = o[n];
+      T v = stored(f, n, o[n]);
+      W[n * E + qd1] = v;
+      W[n * E + qd2] = v;
     }
-    P[qd1] = pr;
-    P[qd2] = pr;
+    T pv = stored(f, 5, pr);
+    P[qd1] = pv;
+    P[qd2] = pv;
     return;
   }
   int phy1, phy2, dum1, dum2;
@@ -362,6 +378,8 @@ __device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArg
     else { phy1 = nl; phy2 = nl - 1; dum1 = ne; dum2 = nb; }
   }
   long q1 = L(phy1), q2 = L(phy2), qd1 = L(dum1), qd2 = L(dum2);
+  // a mirrored (negated) component of the full state
+  auto neg = [&](int n, long q) { return stored(f, n, -Wf(n, q)); };
   if (B.kind == -60) {
     for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
     for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
@@ -369,18 +387,18 @@ __device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArg
     P[qd1] = P[q1];
   } else if (B.kind == -50) {
     for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
-    W[(1 + dir) * E + qd1] = -W[(1 + dir) * E + q1];
+    W[(1 + dir) * E + qd1] = neg(1 + dir, q1);
     P[qd1] = P[q1];
     for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
-    W[(1 + dir) * E + qd2] = -W[(1 + dir) * E + q2];
+    W[(1 + dir) * E + qd2] = neg(1 + dir, q2);
     P[qd2] = P[q2];
   } else if (B.kind == -80) {
     W[qd1] = W[q1];
     W[4 * E + qd1] = W[4 * E + q1];
-    for (int n = 1; n < 4; n++) W[n * E + qd1] = -W[n * E + q1];
+    for (int n = 1; n < 4; n++) W[n * E + qd1] = neg(n, q1);
     W[qd2] = W[q2];
     W[4 * E + qd2] = W[4 * E + q2];
-    for (int n = 1; n < 4; n++) W[n * E + qd2] = -W[n * E + q2];
+    for (int n = 1; n < 4; n++) W[n * E + qd2] = neg(n, q2);
     P[qd1] = P[q1];
     P[qd2] = P[q1];
   }
@@ -389,8 +407,8 @@ MGE_ENTRY_T(k_bc_face, BcArgs, 256, 1)
 
 // the ordered edge extrapolation of bc_corner.f90 (one target cell)
 template <typename T>
-__device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, int ti, int tj, int tk, int ai, int aj, int ak,
-                                 int bi, int bj, int bk, int ci, int cj, int ck) {
+__device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, const Inf<T> &f, int ti, int tj, int tk, int ai,
+                                 int aj, int ak, int bi, int bj, int bk, int ci, int cj, int ck) {
   const long E = d.E;
   long qt = ix(d, ti, tj, tk), qa = ix(d, ai, aj, ak), qb = ix(d, bi, bj, bk), qc = ix(d, ci, cj, ck);
   T v[5];
@@ -398,7 +416,7 @@ __device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, int ti, int
     v[n] = s.w[n * E + qa] + s.w[n * E + qb] - s.w[n * E + qc];
     s.w[n * E + qt] = v[n];
   }
-  s.p[qt] = eos(gamma, v);
+  s.p[qt] = eos_st(gamma, f, v);
 }
 
 // bc_corner.f90:23-157: four ordered phases; within a phase every (edge,
@@ -409,6 +427,7 @@ struct CornerArgs {
   Dims d;
   State<T> s;
   T gamma;
+  Inf<T> inf;
 };
 
 template <typename T>
@@ -423,29 +442,29 @@ __device__ __attribute__((always_inline)) inline void k_bc_corner_body(const Cor
   // phase 1: edges along k (bc_corner.f90:23-61)
   for (int t = tid; t < 4 * (kl - 1); t += nt) {
     int e = t & 3, k = 2 + (t >> 2);
-    if (e == 0) edge_fill(d, s, gamma, 1, 1, k, 2, 1, k, 1, 2, k, 2, 2, k);
-    else if (e == 1) edge_fill(d, s, gamma, ie, 1, k, il, 1, k, ie, 2, k, il, 2, k);
-    else if (e == 2) edge_fill(d, s, gamma, 1, je, k, 2, je, k, 1, jl, k, 2, jl, k);
-    else edge_fill(d, s, gamma, ie, je, k, il, je, k, ie, jl, k, il, jl, k);
+    if (e == 0) edge_fill(d, s, gamma, C.inf, 1, 1, k, 2, 1, k, 1, 2, k, 2, 2, k);
+    else if (e == 1) edge_fill(d, s, gamma, C.inf, ie, 1, k, il, 1, k, ie, 2, k, il, 2, k);
+    else if (e == 2) edge_fill(d, s, gamma, C.inf, 1, je, k, 2, je, k, 1, jl, k, 2, jl, k);
+    else edge_fill(d, s, gamma, C.inf, ie, je, k, il, je, k, ie, jl, k, il, jl, k);
   }
   __syncthreads();
   // phase 2: edges along j (:63-99)
   for (int t = tid; t < 4 * (jl - 1); t += nt) {
     int e = t & 3, j = 2 + (t >> 2);
-    if (e == 0) edge_fill(d, s, gamma, 1, j, 1, 2, j, 1, 1, j, 2, 2, j, 2);
-    else if (e == 1) edge_fill(d, s, gamma, ie, j, 1, il, j, 1, ie, j, 2, il, j, 2);
-    else if (e == 2) edge_fill(d, s, gamma, 1, j, ke, 2, j, ke, 1, j, kl, 2, j, kl);
-    else edge_fill(d, s, gamma, ie, j, ke, il, j, ke, ie, j, kl, il, j, kl);
+    if (e == 0) edge_fill(d, s, gamma, C.inf, 1, j, 1, 2, j, 1, 1, j, 2, 2, j, 2);
+    else if (e == 1) edge_fill(d, s, gamma, C.inf, ie, j, 1, il, j, 1, ie, j, 2, il, j, 2);
+    else if (e == 2) edge_fill(d, s, gamma, C.inf, 1, j, ke, 2, j, ke, 1, j, kl, 2, j, kl);
+    else edge_fill(d, s, gamma, C.inf, ie, j, ke, il, j, ke, ie, j, kl, il, j, kl);
   }
   __syncthreads();
   // phase 3: edges along i including the ghost i (:101-137); the third
   // line adds and subtracts the same value (:120)
   for (int t = tid; t < 4 * (ib + 1); t += nt) {
     int e = t & 3, i = t >> 2;
-    if (e == 0) edge_fill(d, s, gamma, i, 1, 1, i, 1, 2, i, 2, 1, i, 2, 2);
-    else if (e == 1) edge_fill(d, s, gamma, i, 1, ke, i, 2, ke, i, 1, kl, i, 2, kl);
-    else if (e == 2) edge_fill(d, s, gamma, i, je, 1, i, je, 1, i, jl, 2, i, jl, 2);
-    else edge_fill(d, s, gamma, i, je, ke, i, jl, ke, i, je, kl, i, jl, kl);
+    if (e == 0) edge_fill(d, s, gamma, C.inf, i, 1, 1, i, 1, 2, i, 2, 1, i, 2, 2);
+    else if (e == 1) edge_fill(d, s, gamma, C.inf, i, 1, ke, i, 2, ke, i, 1, kl, i, 2, kl);
+    else if (e == 2) edge_fill(d, s, gamma, C.inf, i, je, 1, i, je, 1, i, jl, 2, i, jl, 2);
+    else edge_fill(d, s, gamma, C.inf, i, je, ke, i, jl, ke, i, je, kl, i, jl, kl);
   }
   __syncthreads();
   const T two = T(2), three = T(3);
@@ -470,7 +489,7 @@ __device__ __attribute__((always_inline)) inline void k_bc_corner_body(const Cor
     long q = ix(d, ci, cj, ck);
     T v[5];
     for (int n = 0; n < 5; n++) v[n] = s.w[n * E + q];
-    s.p[q] = eos(gamma, v);
+    s.p[q] = eos_st(gamma, C.inf, v);
   }
 }
 MGE_ENTRY_T(k_bc_corner, CornerArgs, 1024, 1)
@@ -508,6 +527,7 @@ struct EosArgs {
   Dims d;
   State<T> s;
   T gamma;
+  Inf<T> inf;
 };
 
 template <typename T>
@@ -517,7 +537,7 @@ __device__ __attribute__((always_inline)) inline void k_eos_all_body(const EosAr
   if (q >= d.E) return;
   T v[5];
   for (int n = 0; n < 5; n++) v[n] = A.s.w[n * d.E + q];
-  A.s.p[q] = eos(A.gamma, v);
+  A.s.p[q] = eos_st(A.gamma, A.inf, v);
 }
 MGE_ENTRY_T(k_eos_all, EosArgs, 256, 1)
 
@@ -675,9 +695,9 @@ __global__ void k_wall_terms(FluxCtx<T> fc, const DevRec *recs, const double *ws
   ss[zz - 1] = face_s_any(g, zz, r.s[0], r.s[1], r.s[2]);
   long qc = ix(d, r.c[0], r.c[1], r.c[2]);
   double wc[5], ws[5], f1[5], f2[5];
-  for (int n = 0; n < 5; n++) wc[n] = s.w[n * d.E + qc];
+  for (int n = 0; n < 5; n++) wc[n] = full(fc.inf, n, s.w[n * d.E + qc]);
   for (int n = 0; n < 5; n++) ws[n] = wsol[n * nmir + m];
-  cflux3(wc, (double)s.p[qc], sc, f1);
+  cflux3(wc, full(fc.inf, 5, s.p[qc]), sc, f1);
   cflux3(ws, psol[m], ss, f2);
   double sgn = (r.faceid >= 4) ? -1.0 : 1.0;
   for (int dd = 0; dd < 3; dd++) terms[3 * m + dd] = sgn * (0.5 * (f1[1 + dd] + f2[1 + dd]));

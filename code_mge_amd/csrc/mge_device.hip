@@ -259,6 +259,20 @@ State<T> stm(const Level &L, int b) {
   return State<T>{tp<T>(L.S[b]), tp<T>(L.S[b]) + 5 * L.d.E};
 }
 
+// freestream of the case in the bulk type (the FP32 state is stored as the
+// deviation from it)
+template <class T>
+Inf<T> make_inf(const mge_params &P) {
+  Inf<T> f;
+  f.w[0] = (T)P.rho0;
+  f.w[1] = (T)(P.rho0 * P.u0);
+  f.w[2] = (T)(P.rho0 * P.v0);
+  f.w[3] = (T)(P.rho0 * P.w0);
+  f.w[4] = (T)(P.rho0 * P.h0 - P.p0);
+  f.p = (T)P.p0;
+  return f;
+}
+
 // run f with the case's bulk type: f(double{}) or f(float{})
 template <class F>
 int by_type(mge_case *h, F f) {
@@ -320,15 +334,17 @@ int do_bcond(mge_case *h, Level &L, int buf) {
     B.s = stm<T>(L, buf);
     B.face = f;
     B.kind = t;
-    B.gamma = (T)P.gamma; B.rho0 = (T)P.rho0; B.p0 = (T)P.p0; B.c0 = (T)P.c0;
-    B.u0 = (T)P.u0; B.v0 = (T)P.v0; B.w0 = (T)P.w0;
+    B.gamma = P.gamma; B.rho0 = P.rho0; B.p0 = P.p0; B.c0 = P.c0;
+    B.u0 = P.u0; B.v0 = P.v0; B.w0 = P.w0;
+    B.inf = make_inf<T>(P);
     int dir = (f - 1) % 3;
     int amax = dir == 0 ? L.d.jb : L.d.ib;
     int cmax = dir == 2 ? L.d.jb : L.d.kb;
     dim3 g((amax + 1 + 255) / 256, cmax + 1);
     LAUNCHT(h, "k_bc_face", k_bc_face, T, g, dim3(256), B);
   }
-  LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3(1), dim3(1024), (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma}));
+  LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3(1), dim3(1024),
+          (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma, make_inf<T>(P)}));
   return warn;
 }
 
@@ -341,6 +357,7 @@ int do_bc_cutcell(mge_case *h, Level &L, int buf) {
   CutArgs<T> C;
   C.d = L.d;
   C.s = stc<T>(L, buf);
+  C.inf = make_inf<T>(h->prm);
   C.recs = L.recs;
   C.wsol = L.wsol;
   C.psol = L.psol;
@@ -376,6 +393,7 @@ int do_timestep(mge_case *h, Level &L) {
   A.viscl = (T)P.viscl;
   A.prlmean = (T)P.prlmean;
   A.cfac = (T)(P.distype == 0 ? 4.0 : (P.distype == 2 ? 1.0 : 2.0));
+  A.inf = make_inf<T>(P);
   A.err = h->err;
   LAUNCHT(h, "k_timestep", k_timestep, T, cell_grid(L.d), cell_block(), A);
   LAUNCHN(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(64), D);
@@ -411,6 +429,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     A.fx.vis4 = (T)P.vis4;
     A.fx.cdis = (T)P.cdis[m - 1];
     A.fx.dissip = P.dissip;
+    A.fx.inf = make_inf<T>(P);
     A.wdst = tp<T>(L.S[dst]);
     A.pdst = tp<T>(L.S[dst]) + 5 * E;
     A.wn = tp<T>(L.S[w0]);
@@ -503,7 +522,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       LAUNCHT(h, "k_smooth_z", k_smooth_z, T, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
       LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256),
               (UpdArgs<T>{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
-                          (T)P.gamma}));
+                          (T)P.gamma, make_inf<T>(P)}));
     } else {
       LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, flat_grid(E), dim3(256),
               (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
@@ -566,7 +585,7 @@ int do_movco(mge_case *h, int mgl) {
   X.flagsf = f.flags;
   X.wfo = nullptr;
   LAUNCHT(h, "k_restrict", k_restrict, T, cell_grid(c.d), cell_block(), X);
-  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(c.d.E), dim3(256), (EosArgs<T>{c.d, stm<T>(c, c.cur), (T)h->prm.gamma}));
+  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(c.d.E), dim3(256), (EosArgs<T>{c.d, stm<T>(c, c.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
   return do_bcond<T>(h, c, c.cur);
 }
 
@@ -590,7 +609,7 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.flagsf = f.flags;
   X.wfo = tp<T>(f.S[f.cur]);
   LAUNCHT(h, "k_prolong", k_prolong, T, cell_grid(f.d), cell_block(), X);
-  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(f.d.E), dim3(256), (EosArgs<T>{f.d, stm<T>(f, f.cur), (T)h->prm.gamma}));
+  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(f.d.E), dim3(256), (EosArgs<T>{f.d, stm<T>(f, f.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
   return do_bcond<T>(h, f, f.cur);
 }
 
@@ -601,14 +620,16 @@ int do_init_flow(mge_case *h) {
   for (int l = 1; l <= h->nlevel; l++) {
     Level &L = h->lv[l];
     if (!L.defined) return set_err(MGE_ERR_STATE, "level not defined");
-    LAUNCHT(h, "k_fill", k_fill, T, flat_grid(L.d.E), dim3(256),
-            (FillArgs<T>{L.d, stm<T>(L, L.cur),
-                         {(T)P.rho0, (T)(P.rho0 * P.u0), (T)(P.rho0 * P.v0), (T)(P.rho0 * P.w0),
-                          (T)(P.rho0 * P.h0 - P.p0)}}));
+    // the freestream state itself (FP64) or a zero deviation from it (FP32)
+    const Inf<T> fi = make_inf<T>(P);
+    FillArgs<T> F{L.d, stm<T>(L, L.cur), {fi.w[0], fi.w[1], fi.w[2], fi.w[3], fi.w[4]}};
+    if constexpr (kDelta<T>)
+      for (int n = 0; n < 5; n++) F.v[n] = T(0);
+    LAUNCHT(h, "k_fill", k_fill, T, flat_grid(L.d.E), dim3(256), F);
   }
   for (int l = 1; l <= h->nlevel; l++) {
     Level &L = h->lv[l];
-    LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(L.d.E), dim3(256), (EosArgs<T>{L.d, stm<T>(L, L.cur), (T)P.gamma}));
+    LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(L.d.E), dim3(256), (EosArgs<T>{L.d, stm<T>(L, L.cur), (T)P.gamma, make_inf<T>(P)}));
     int rc = do_bc_cutcell<T>(h, L, L.cur);
     if (rc > 0) warn = rc;
     rc = do_bcond<T>(h, L, L.cur);
@@ -706,6 +727,7 @@ int do_wall_force(mge_case *h, Level &L) {
   fc.d = L.d;
   fc.s = stc<T>(L, L.cur);
   fc.g = geo<T>(L);
+  fc.inf = make_inf<T>(h->prm);
   hipLaunchKernelGGL(k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), 0, h->stream, fc, (const DevRec *)L.recs,
                      (const double *)L.wsol, (const double *)L.psol, L.nmir, h->prm.distype != 0 ? 1 : 0,
                      h->prm.gamma, h->prm.epsentr, L.opval, h->err);
@@ -1367,6 +1389,14 @@ int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long lo
   return MGE_OK;
 }
 
+// fields holding a state (FP32 stores them as the deviation from the
+// freestream): "w" and "ws" (5 components), "p" (1)
+static int state_field(const char *name) {
+  if (!strcmp(name, "w") || !strcmp(name, "ws")) return 5;
+  if (!strcmp(name, "p")) return 1;
+  return 0;
+}
+
 static void *field_ptr(Level &L, size_t es, const char *name, long *n) {
   const long E = L.d.E;
   *n = 5 * E;
@@ -1396,7 +1426,13 @@ int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out) 
   std::vector<float> t(n);
   HIPCHK(hipMemcpyAsync(t.data(), p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
   int e = check_err(h);
-  for (long q = 0; q < n; q++) out[q] = t[q];
+  const int nc = state_field(field);
+  const Inf<float> f = make_inf<float>(h->prm);
+  const long E = L->d.E;
+  for (long q = 0; q < n; q++) {
+    double off = nc == 5 ? (double)f.w[q / E] : (nc == 1 ? (double)f.p : 0.0);
+    out[q] = (double)t[q] + off;
+  }
   return e;
 }
 
@@ -1407,7 +1443,17 @@ int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *i
   void *p = field_ptr(*L, h->es, field, &n);
   if (!p) return set_err(MGE_ERR_ARG, "unknown field");
   HIPCHK(hipStreamSynchronize(h->stream));
-  return put_bulk(h, p, in, n);
+  const int nc = state_field(field);
+  if (!h->fp32 || nc == 0) return put_bulk(h, p, in, n);
+  const Inf<float> f = make_inf<float>(h->prm);
+  const long E = L->d.E;
+  std::vector<float> t(n);
+  for (long q = 0; q < n; q++) {
+    double off = nc == 5 ? (double)f.w[q / E] : (double)f.p;
+    t[q] = (float)(in[q] - off);
+  }
+  HIPCHK(hipMemcpy(p, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  return MGE_OK;
 }
 
 int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out) {

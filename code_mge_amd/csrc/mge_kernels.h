@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace mge {
 
 // Block dimensions in the reference numbering: interior 2..il, ghosts
@@ -49,6 +51,34 @@ template <typename T>
 struct Geo {
   const T *dx, *dy, *dz;  // vertex differences x(i+1)-x(i)
 };
+
+// The FP32 bulk path stores the state as its deviation from the freestream
+// (w' = w - w_inf, p' = p - p_inf) and evaluates fluxes and the EOS in delta
+// form, so that the freestream parts -- pressure and energy terms far larger
+// than the changes that drive convergence -- cancel exactly instead of being
+// rounded at the scale of the freestream.  FP64 stores the state itself and
+// follows the reference expression trees.
+template <typename T>
+constexpr bool kDelta = std::is_same<T, float>::value;
+
+template <typename T>
+struct Inf {
+  T w[5];  // freestream conservative state
+  T p;     // freestream pressure
+};
+
+// full value of component n (5 = p) of a stored state value, in FP64
+template <typename T>
+__device__ inline double full(const Inf<T> &f, int n, T v) {
+  if constexpr (kDelta<T>) return (double)v + (double)(n < 5 ? f.w[n] : f.p);
+  else return v;
+}
+// stored value of a full FP64 state value
+template <typename T>
+__device__ inline T stored(const Inf<T> &f, int n, double v) {
+  if constexpr (kDelta<T>) return (T)(v - (double)(n < 5 ? f.w[n] : f.p));
+  else return (T)v;
+}
 
 // face vector of cell (i,j,k) in direction dir: the non-zero component of
 // si/sj/sk (init_metric.f90:216-274) on an axis-aligned grid.
@@ -129,6 +159,26 @@ __device__ inline T eos(T gamma, const T *v) {
   return (gamma - T(1)) * dimd(v[4], qq);
 }
 
+// EOS of a stored state: the reference form for FP64; for FP32 the delta
+// form p' = (gamma-1)(E' - KE'), KE' = KE - KE_inf written from the
+// deviations, KE' = (rho_inf (2 m_inf.m' + |m'|^2) - rho' |m_inf|^2) /
+// (2 rho rho_inf), with the reference's clamp p >= 0
+template <typename T>
+__device__ inline T eos_st(T gamma, const Inf<T> &f, const T *v) {
+  if constexpr (kDelta<T>) {
+    const T r0 = f.w[0];
+    T mm = f.w[1] * v[1] + f.w[2] * v[2] + f.w[3] * v[3];
+    T m2 = v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+    T mi2 = f.w[1] * f.w[1] + f.w[2] * f.w[2] + f.w[3] * f.w[3];
+    T rho = r0 + v[0];
+    T ke = (r0 * (T(2) * mm + m2) - v[0] * mi2) / (T(2) * rho * r0);
+    T pd = (gamma - T(1)) * (v[4] - ke);
+    return f.p + pd < T(0) ? -f.p : pd;
+  } else {
+    return eos(gamma, v);
+  }
+}
+
 template <typename T>
 __device__ inline void load5(const T *w, long E, long q, T *v) {
 #pragma unroll
@@ -145,6 +195,32 @@ __device__ inline void cflux(const T *v, T p, T S, T *fs) {
   fs[3] = f1 * v[3] / v[0];
   fs[1 + DIR] = fs[1 + DIR] + p * S;
   fs[4] = f1 * (v[4] + p) / v[0];
+}
+
+// ceflux of a stored state: the reference form for FP64; for FP32 the flux
+// deviation F - F_inf in delta form (f1 = f1_inf + f1', rho = rho_inf +
+// rho'), e.g. F_c' = (rho_inf (m_inf,c f1' + m_c' f1_inf + m_c' f1') -
+// rho' m_inf,c f1_inf) / (rho rho_inf) + p' S delta_c,DIR.  F_inf is the
+// same for every cell of a grid line (S does not vary along it), so face
+// differences of F' equal those of F.
+template <int DIR, typename T>
+__device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
+  if constexpr (kDelta<T>) {
+    const T r0 = f.w[0];
+    T f1i = f.w[1 + DIR] * S, f1d = v[1 + DIR] * S;
+    T den = (r0 + v[0]) * r0;
+    fs[0] = f1d;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T mi = f.w[1 + c], md = v[1 + c];
+      fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) / den;
+    }
+    fs[1 + DIR] = fs[1 + DIR] + p * S;
+    T hi = f.w[4] + f.p, hd = v[4] + p;
+    fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) / den;
+  } else {
+    cflux<DIR>(v, p, S, fs);
+  }
 }
 
 // general ceflux (full face vector), used by the wall records
@@ -214,6 +290,7 @@ struct FluxCtx {
   const T *radsum;  // radi+radj+radk at interior cells
   T gamma, vis2, vis4, cdis;
   int dissip;       // 0 scalar, 1 matrix
+  Inf<T> inf;       // freestream (FP32: the state is stored as deviation)
 };
 
 template <int DIR>
@@ -241,12 +318,14 @@ __device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
   T pm = c.s.p[base + (sc - 1) * st], pc = c.s.p[base + sc * st], pp = c.s.p[base + (sc + 1) * st];
   T pb, num;
   if (DIR == 1) {
-    pb = fabs(pp + T(2) * pc + pm);
+    pb = pp + T(2) * pc + pm;
     num = fabs(pp - T(2) * pc + pm);
   } else {
-    pb = fabs(pm + T(2) * pc + pp);
+    pb = pm + T(2) * pc + pp;
     num = fabs(pm - T(2) * pc + pp);
   }
+  if constexpr (kDelta<T>) pb = pb + T(4) * c.inf.p;
+  pb = fabs(pb);
   T v = num / pb;
   v = fabs(pb) > T(0) ? v : T(0);
   return (s <= 0 || s >= nb) ? T(0) : v;
@@ -313,8 +392,18 @@ __device__ inline void dface(const FluxCtx<T> &c, long base, int a, int li, int 
     T um[5], ua[5];
     load5(c.s.w, c.d.E, qm, um);
     load5(c.s.w, c.d.E, qa, ua);
-    um[4] = um[4] + c.s.p[qm];
-    ua[4] = ua[4] + c.s.p[qa];
+    if constexpr (kDelta<T>) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        um[n] = um[n] + c.inf.w[n];
+        ua[n] = ua[n] + c.inf.w[n];
+      }
+      um[4] = um[4] + (c.s.p[qm] + c.inf.p);
+      ua[4] = ua[4] + (c.s.p[qa] + c.inf.p);
+    } else {
+      um[4] = um[4] + c.s.p[qm];
+      ua[4] = ua[4] + c.s.p[qa];
+    }
     int fi = DIR == 0 ? a : li, fj = DIR == 1 ? a : lj, fk = DIR == 2 ? a : lk;
     T ss[3] = {T(0), T(0), T(0)};
     ss[DIR] = face_s<DIR>(c.g, fi, fj, fk);
@@ -332,8 +421,8 @@ __device__ inline void eface(const FluxCtx<T> &c, long base, int a, int li, int 
   T va[5], vm[5], fa[5], fm[5];
   load5(c.s.w, c.d.E, qa, va);
   load5(c.s.w, c.d.E, qm, vm);
-  cflux<DIR>(va, c.s.p[qa], face_s<DIR>(c.g, ai, aj, ak), fa);
-  cflux<DIR>(vm, c.s.p[qm], face_s<DIR>(c.g, mi, mj, mk), fm);
+  cflux_st<DIR>(c.inf, va, c.s.p[qa], face_s<DIR>(c.g, ai, aj, ak), fa);
+  cflux_st<DIR>(c.inf, vm, c.s.p[qm], face_s<DIR>(c.g, mi, mj, mk), fm);
 #pragma unroll
   for (int n = 0; n < 5; n++) out[n] = T(0.5) * (fa[n] + fm[n]);
 }

@@ -30,15 +30,24 @@ __device__ inline T minmod(T a, T b) {
   return a * b <= T(0) ? T(0) : m;
 }
 
+// primitive variables of a cell from the stored state (FP32: the full
+// values rebuilt from the deviations)
 template <typename T>
-__device__ inline void prim_of(const StateC<T> &s, long E, long q, T *o) {
-  T r = s.w[q];
+__device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long q, T *o) {
+  T r = s.w[q], m1 = s.w[E + q], m2 = s.w[2 * E + q], m3 = s.w[3 * E + q], p = s.p[q];
+  if constexpr (kDelta<T>) {
+    r = r + f.w[0];
+    m1 = m1 + f.w[1];
+    m2 = m2 + f.w[2];
+    m3 = m3 + f.w[3];
+    p = p + f.p;
+  }
   T rr = T(1) / r;
   o[0] = r;
-  o[1] = s.w[E + q] * rr;
-  o[2] = s.w[2 * E + q] * rr;
-  o[3] = s.w[3 * E + q] * rr;
-  o[4] = s.p[q];
+  o[1] = m1 * rr;
+  o[2] = m2 * rr;
+  o[3] = m3 * rr;
+  o[4] = p;
 }
 
 // physical flux of a primitive face state through the face vector whose
@@ -218,7 +227,7 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
   int zz = r.zz;
   bool hi = r.faceid >= 4;
   long qc = ix(d, r.c[0], r.c[1], r.c[2]);
-  if (fabs((double)c.s.w[qc]) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
+  if (fabs(full(c.inf, 0, c.s.w[qc])) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
   int fi = r.c[0], fj = r.c[1], fk = r.c[2];
   if (hi) {
     if (zz == 1) fi++;
@@ -226,13 +235,13 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
     else fk++;
   }
   double qcv[5], qm[5], S[3] = {0.0, 0.0, 0.0}, e[5], dv[5];
-  double rc = c.s.w[qc];
+  double rc = full(c.inf, 0, c.s.w[qc]);
   double rr = 1.0 / rc;
   qcv[0] = rc;
-  qcv[1] = (double)c.s.w[d.E + qc] * rr;
-  qcv[2] = (double)c.s.w[2 * d.E + qc] * rr;
-  qcv[3] = (double)c.s.w[3 * d.E + qc] * rr;
-  qcv[4] = c.s.p[qc];
+  qcv[1] = full(c.inf, 1, c.s.w[d.E + qc]) * rr;
+  qcv[2] = full(c.inf, 2, c.s.w[2 * d.E + qc]) * rr;
+  qcv[3] = full(c.inf, 3, c.s.w[3 * d.E + qc]) * rr;
+  qcv[4] = full(c.inf, 5, c.s.p[qc]);
   double rm = 1.0 / wsol[0];
   qm[0] = wsol[0];
   qm[1] = wsol[1] * rm;
@@ -266,7 +275,7 @@ struct RoeSmem {
 
 template <typename T>
 __device__ inline void roe_load(const FluxCtx<T> &c, const uint8_t *flags, long q, T *o, bool *f) {
-  prim_of(c.s, c.d.E, q, o);
+  prim_of(c.s, c.inf, c.d.E, q, o);
   *f = (flags[q] & F_FLUID) != 0;
 }
 

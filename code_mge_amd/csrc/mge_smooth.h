@@ -282,6 +282,7 @@ struct UpdArgs {
   StateC<T> src;
   State<T> dst;
   T gamma;
+  Inf<T> inf;
 };
 
 template <typename T>
@@ -303,7 +304,7 @@ __device__ __attribute__((always_inline)) inline void k_update_ghost_body(const 
       v[n] = wn[n * d.E + q] - dw[n * d.E + q] / T(1);
       dst.w[n * d.E + q] = v[n];
     }
-    dst.p[q] = eos(gamma, v);
+    dst.p[q] = eos_st(gamma, U.inf, v);
   } else {
     for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
     dst.p[q] = src.p[q];

@@ -141,7 +141,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
       v[n] = A.wn[n * E + q] - r[n] / T(1);
       A.wdst[n * E + q] = v[n];
     }
-    A.pdst[q] = eos(A.fx.gamma, v);
+    A.pdst[q] = eos_st(A.fx.gamma, A.fx.inf, v);
   }
 }
 
@@ -258,7 +258,7 @@ __device__ inline void cell_flux(const FluxCtx<T> &c, int i, int j, int k, T *o)
   long q = ix(c.d, i, j, k);
   T v[5];
   load5(c.s.w, c.d.E, q, v);
-  cflux<DIR>(v, c.s.p[q], face_s<DIR>(c.g, i, j, k), o);
+  cflux_st<DIR>(c.inf, v, c.s.p[q], face_s<DIR>(c.g, i, j, k), o);
 }
 
 template <typename T>
@@ -525,10 +525,10 @@ __device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   double wsol[5];
   for (int n = 0; n < 5; n++) wsol[n] = wsolv[n];
   wsol[4] = wsol[4] + psol;
-  auto P = [&](int s) { return (double)c.s.p[base + (long)s * st]; };
+  auto P = [&](int s) { return full(c.inf, 5, c.s.p[base + (long)s * st]); };
   auto H = [&](int s, int n) {
     long q = base + (long)s * st;
-    return n < 4 ? (double)c.s.w[n * E + q] : (double)c.s.w[4 * E + q] + (double)c.s.p[q];
+    return n < 4 ? full(c.inf, n, c.s.w[n * E + q]) : full(c.inf, 4, c.s.w[4 * E + q]) + full(c.inf, 5, c.s.p[q]);
   };
   auto RS = [&](int s) { return (double)c.radsum[base + (long)s * st]; };
   auto SS = [&](int s, double *ss) {
@@ -660,10 +660,10 @@ __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   sc[zz - 1] = face_s_any(c.g, zz, i, j, k);
   ss[zz - 1] = face_s_any(c.g, zz, r.s[0], r.s[1], r.s[2]);
   double wc[5];
-  for (int n = 0; n < 5; n++) wc[n] = c.s.w[n * d.E + qc];
+  for (int n = 0; n < 5; n++) wc[n] = full(c.inf, n, c.s.w[n * d.E + qc]);
   if (fabs(wc[0]) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
   double fs1[5], fs2[5], fs3[5];
-  cflux3(wc, (double)c.s.p[qc], sc, fs1);
+  cflux3(wc, full(c.inf, 5, c.s.p[qc]), sc, fs1);
   cflux3(wsol, psol, ss, fs2);
   for (int n = 0; n < 5; n++) fs[n] = 0.5 * (fs1[n] + fs2[n]);
   int hi = (id >= 4);
@@ -674,9 +674,9 @@ __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   else ok += dlt;
   long qo = ix(d, oi, oj, ok);
   double wo[5], so[3] = {0, 0, 0};
-  for (int n = 0; n < 5; n++) wo[n] = c.s.w[n * d.E + qo];
+  for (int n = 0; n < 5; n++) wo[n] = full(c.inf, n, c.s.w[n * d.E + qo]);
   so[zz - 1] = face_s_any(c.g, zz, oi, oj, ok);
-  cflux3(wo, (double)c.s.p[qo], so, fs3);
+  cflux3(wo, full(c.inf, 5, c.s.p[qo]), so, fs3);
   for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
 }
 
