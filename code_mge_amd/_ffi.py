@@ -109,6 +109,8 @@ def lib():
     L.mge_synchronize.argtypes = [P]
     L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong), dp]
     L.mge_cycles.argtypes = [P, I, I, I, dp, C.POINTER(C.c_longlong)]
+    L.mge_case_bytes.argtypes = [P]
+    L.mge_case_bytes.restype = C.c_longlong
     L.mge_batch_create.argtypes = [I, I, C.POINTER(P)]
     L.mge_batch_destroy.argtypes = [P]
     L.mge_batch_add.argtypes = [P, P, D, I]
@@ -135,5 +137,5 @@ EXPORTED = [
     "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
     "mge_kernel_timing", "mge_last_error", "mge_cycles",
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
-    "mge_batch_status", "mge_batch_log",
+    "mge_batch_status", "mge_batch_log", "mge_case_bytes",
 ]

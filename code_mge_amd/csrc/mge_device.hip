@@ -97,6 +97,7 @@ struct Level {
   double *force = nullptr;
   int mxcyc = 1;
   std::vector<void *> allocs;
+  size_t bytes = 0;  // device bytes of the level
 };
 
 struct Timer {
@@ -150,6 +151,7 @@ int dalloc_bytes(Level &L, void **p, size_t bytes) {
     return set_err(MGE_ERR_HIP, "hipMemset failed");
   }
   L.allocs.push_back(q);
+  L.bytes += bytes + 64;
   *p = q;
   return MGE_OK;
 }
@@ -1634,6 +1636,13 @@ int mge_batch_log(mge_batch *b, int slot, double *out, int cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
   }
   return n;
+}
+
+long long mge_case_bytes(mge_case *h) {
+  if (!h) return set_err(MGE_ERR_ARG, "null handle");
+  long long b = 0;
+  for (auto &L : h->lv) b += (long long)L.bytes;
+  return b;
 }
 
 int mge_synchronize(mge_case *h) {

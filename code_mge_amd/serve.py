@@ -1,0 +1,196 @@
+"""Serving loop for steady solves: continuous batching on one GPU.
+
+Requests (a case description plus an arrival time) queue in arrival order.
+A request is admitted as soon as its device memory fits the HBM budget: its
+geometry is built, its multigrid state is allocated and initialised on the
+GPU and it joins the batch of its class (cases of one batch share levels,
+scheme, stages, cycle type, smoothing and precision).  Every tick advances
+each batch by a few periods (two multigrid cycles each, all cases in one
+launch per operation); a case leaves the batch at the cycle its residual
+has dropped by `drop` (or at maxcyc), its Cd/Cl are read, its memory is
+freed and waiting requests are admitted in its place.  Requests that arrive
+while others run are admitted between ticks and overlap with them on the
+device.  A case computes exactly what it computes alone.
+
+No counterpart in the reference (it runs one case per process).
+"""
+import dataclasses
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional
+
+import numpy as np
+
+from .grid import level_geometry
+from .params import Params
+from .solver import Batch, Case, force_coefficients
+
+DEVREC_BYTES = 192
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    params: Params
+    levels: list              # LevelSpec, coarsest first
+    body: Any                 # analytic body or None
+    ref_area: float
+    info: Dict = field(default_factory=dict)
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    completed: float
+    status: int               # 1 converged, 2 cycle limit, < 0 error
+    ncyc: int
+    residual_ratio: float
+    cd: float
+    cl: float
+    device_bytes: int
+
+    @property
+    def time_to_convergence(self):
+        return self.completed - self.arrival
+
+
+def level_bytes(n, es, nmir=0, mxcyc=4):
+    """device bytes of one level as the library allocates them (every array
+    padded by 64 bytes); wall-record arrays at their upper bound"""
+    ni, nj, nk = n
+    E = (ni + 4) * (nj + 4) * (nk + 4)
+    nmax = max(ni, nj, nk) + 5
+    ngrid = -(-ni // 64) * -(-nj // 4) * nk
+    sizes = [es * 6 * E] * 3 + [es * 5 * E] * 5 + [es * E] * 2 + [E] * 2
+    sizes += [es * nmax] * 3 + [es * 6 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
+    if nmir:
+        ntouch, nops = 2 * nmir, 6 * nmir
+        sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
+                  80 * (nops + 1)]
+    return sum(s + 64 for s in sizes)
+
+
+class Server:
+    def __init__(self, hbm_budget: float, drop: float = 1e5, maxcyc: int = 3000, precision: int = 32,
+                 use_graph: bool = True, periods: int = 4, log=None):
+        self.budget = float(hbm_budget)
+        self.drop = drop
+        self.maxcyc = maxcyc
+        self.precision = precision
+        self.use_graph = use_graph
+        self.periods = periods
+        self.log = log
+        self.batches: Dict[tuple, Batch] = {}
+        self.used = 0
+
+    def _class(self, prm: Params, levels) -> tuple:
+        return (prm.eqntype.lower(), prm.distype.lower(), prm.dissip.lower(), len(levels), len(prm.cstp),
+                prm.mgtype, prm.smoo > 0.0, self.precision)
+
+    def estimate(self, req: Request, geoms=None) -> int:
+        es = 4 if self.precision == 32 else 8
+        total = 0
+        for li, spec in enumerate(req.levels):
+            nmir = geoms[li].nmir if geoms is not None else 0
+            total += level_bytes(spec.n, es, nmir)
+        return total
+
+    def run(self, requests: List[Request], clock=time.monotonic) -> List[Result]:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        queue: List[Request] = []
+        active = {}            # (class, slot) -> (req, case, admitted, bytes)
+        results: List[Result] = []
+        geom_cache = {}
+        t0 = clock()
+        while pending or queue or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.pop(0))
+            # admission in arrival order while the budget allows
+            while queue:
+                req = queue[0]
+                if req.rid not in geom_cache:
+                    geom_cache[req.rid] = [level_geometry(s, req.body) for s in req.levels]
+                geoms = geom_cache[req.rid]
+                need = self.estimate(req, geoms)
+                if self.used + need > self.budget and active:
+                    break
+                queue.pop(0)
+                geom_cache.pop(req.rid, None)
+                prm = dataclasses.replace(req.params, precision=self.precision)
+                case = Case(prm, req.levels, geoms=geoms, mxcyc=4)
+                case.init_flow()
+                got = case.device_bytes()
+                key = self._class(prm, req.levels)
+                b = self.batches.get(key)
+                if b is None:
+                    b = Batch(use_graph=self.use_graph, log_cap=8)
+                    self.batches[key] = b
+                slot = b.add(case, self.drop, self.maxcyc)
+                active[(key, slot)] = (req, case, clock() - t0, got)
+                self.used += got
+                if self.log:
+                    self.log(f"[serve] t={clock() - t0:7.2f}s admit {req.rid} ({got / 2**20:.0f} MiB, "
+                             f"{len(active)} active, {len(queue)} waiting)")
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            for key, b in list(self.batches.items()):
+                if any(k == key for (k, _) in active):
+                    b.run(self.periods)
+            for (key, slot) in list(active):
+                b = self.batches[key]
+                st = b.status(slot)
+                if st["status"] == 0:
+                    continue
+                req, case, adm, got = active.pop((key, slot))
+                cd, cl = force_coefficients(case, req.ref_area) if st["status"] > 0 else (math.nan, math.nan)
+                b.remove(slot)
+                case.close()
+                self.used -= got
+                done = clock() - t0
+                ratio = st["last"] / st["ini"] if st["ini"] else math.nan
+                results.append(Result(req.rid, req.arrival, adm, done, st["status"], st["ncyc"], ratio, cd, cl, got))
+                if self.log:
+                    self.log(f"[serve] t={done:7.2f}s done {req.rid} after {st['ncyc']} cycles "
+                             f"(status {st['status']}, {done - req.arrival:.2f}s)")
+        for b in self.batches.values():
+            b.close()
+        self.batches.clear()
+        return results
+
+
+def percentiles(results: List[Result]):
+    t = np.array([r.time_to_convergence for r in results])
+    return float(np.percentile(t, 50)), float(np.percentile(t, 99))
+
+
+def poisson_requests(count=256, rate=4.0, seed=20261203, nlevel=3, box=8.0, euler_only=True):
+    """Requests with exponential inter-arrival gaps (rate per second) from
+    numpy PCG64(seed); per request in order: type ~ Bernoulli(0.5)
+    Euler/Navier-Stokes, grid N ~ exp U[ln 32, ln 192] rounded to a multiple
+    of 8 cells per edge, M ~ U[0.1, 0.8], Re ~ exp U[ln 20, ln 200] for the
+    Navier-Stokes type; a sphere of radius 0.5 in [-box, box]^3.  With
+    euler_only the Navier-Stokes draws are kept (same stream) but served as
+    Euler requests."""
+    from . import configs
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        ns = bool(rng.uniform() < 0.5)
+        n = int(8 * round(math.exp(rng.uniform(math.log(32.0), math.log(192.0))) / 8.0))
+        mach = float(rng.uniform(0.1, 0.8))
+        re = float(math.exp(rng.uniform(math.log(20.0), math.log(200.0)))) if ns else None
+        kw = {}
+        if ns and not euler_only:
+            kw = dict(eqntype="n", reynolds=re)
+        prm, levels, bdy, aref = configs.sphere3d(n=n, nlevel=nlevel, mach=mach, alpha=0.0, box=box, **kw)
+        out.append(Request(rid, t, prm, levels, bdy, aref, dict(n=n, mach=mach, ns=ns, re=re)))
+    return out

@@ -136,6 +136,9 @@ class Case:
     def dtim(self, lvl):
         return self.lib.mge_get_dtim(self.h, lvl, 1)
 
+    def device_bytes(self):
+        return int(self.lib.mge_case_bytes(self.h))
+
     def sync(self):
         return _ffi.check(self.lib.mge_synchronize(self.h), "mge_synchronize")
 

@@ -186,6 +186,10 @@ const char *mge_last_error(void);
  * test of srcf90/solver.f90:238), at maxcyc cycles, or on an error; it then
  * holds exactly the state its own single-case run of those cycles holds.
  * The cases stay owned by the caller: remove a case before destroying it. */
+/* device bytes held by a case (every level's arrays); no counterpart in
+ * the reference -- for callers that admit cases by a memory budget */
+long long mge_case_bytes(mge_case *h);
+
 typedef struct mge_batch mge_batch;
 int mge_batch_create(int use_graph, int log_cap, mge_batch **out);
 int mge_batch_destroy(mge_batch *b);
