@@ -110,6 +110,55 @@ def run_leg(prm, levels, geoms, aref, args, precision):
     return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1)
 
 
+def sweep_leg(k, drop=1e5, maxcyc=3000):
+    """Converged solves per hour on the first k cases of the seeded ellipsoid
+    sweep: FP32 served (continuous batching, all requests at t = 0) against
+    the plain FP64 path one case at a time (solver.f90's loop with the drop
+    test), both to a residual drop of `drop`; per-case setup (geometry,
+    device state) is inside both timings."""
+    import dataclasses
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    from code_mge_amd.serve import Request, Server, percentiles
+    from code_mge_amd.solver import Case, force_coefficients
+    sweep = configs.ellipsoid_sweep()[:k]
+    reqs = [Request(i, 0.0, prm, levels, bdy, aref, info) for i, (prm, levels, bdy, aref, info) in enumerate(sweep)]
+    print(f"[bench] sweep: {k} cases, FP32 served", file=sys.stderr, flush=True)
+    srv = Server(hbm_budget=200e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4)
+    t0 = time.perf_counter()
+    served = srv.run(reqs)
+    t32 = time.perf_counter() - t0
+    served = {r.rid: r for r in served}
+    print(f"[bench] sweep: FP32 served in {t32:.1f} s; FP64 one at a time", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    alone = {}
+    for i, (prm, levels, bdy, aref, info) in enumerate(sweep):
+        p64 = dataclasses.replace(prm, precision=64, drop=drop)
+        geoms = [level_geometry(s, bdy) for s in levels]
+        with Case(p64, levels, geoms=geoms, mxcyc=maxcyc + 2) as c:
+            c.init_flow()
+            log, _ = c.solve([0] * (len(levels) - 1) + [maxcyc])
+            alone[i] = (len(log), force_coefficients(c, aref), log[-1, 2] / log[0, 2])
+    t64 = time.perf_counter() - t0
+    print(f"[bench] sweep: FP64 one at a time in {t64:.1f} s", file=sys.stderr, flush=True)
+    conv32 = sum(1 for r in served.values() if r.status == 1)
+    conv64 = sum(1 for v in alone.values() if v[2] < 1.0 / drop)
+    dcd = max(abs(served[i].cd - alone[i][1][0]) for i in alone)
+    dcl = max(abs(served[i].cl - alone[i][1][1]) for i in alone)
+    p50, p99 = percentiles(list(served.values()))
+    s32 = conv32 / t32 * 3600.0
+    s64 = conv64 / t64 * 3600.0
+    return {"sample": f"first {k} cases of the seeded ellipsoid sweep (PCG64(20261202), N 48..160, 3 levels), "
+                      f"JST, residual drop {drop:g}",
+            "fp32_served_solves_per_h": round(s32, 1), "fp64_one_at_a_time_solves_per_h": round(s64, 1),
+            "ratio": round(s32 / s64, 2) if s64 > 0 else None,
+            "fp32_converged": conv32, "fp64_converged": conv64, "cases": k,
+            "fp32_wall_s": round(t32, 2), "fp64_wall_s": round(t64, 2),
+            "max_cd_err_fp32_vs_fp64": dcd, "max_cl_err_fp32_vs_fp64": dcl,
+            "ttc_p50_s": round(p50, 3), "ttc_p99_s": round(p99, 3),
+            "cycles_fp32": [served[i].ncyc for i in sorted(served)], "cycles_fp64": [alone[i][0] for i in sorted(alone)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +169,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -145,6 +195,8 @@ def main():
                 "speedup_vs_fp64": round(leg["value"] / main_leg["value"], 3)}
     value, dt, cs = main_leg["value"], main_leg["dt"], main_leg["cs"]
     cd, cl, roofline = main_leg["cd"], main_leg["cl"], main_leg["roofline"]
+
+    solves = sweep_leg(args.sweep) if args.sweep > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -173,6 +225,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32": fp32,
+        "solves": solves,
     }
     print(json.dumps(out))
 
