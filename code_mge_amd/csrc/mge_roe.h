@@ -17,6 +17,9 @@
 //    its FP64 mirror state.
 // The face vector S of the reference points from cell a to cell a-1, so
 // e + d is minus the outward flux through the face times |S|.
+// The bulk-face arithmetic is compiled with FMA contraction (the FP64 wall
+// term and the CPU restatement are not), so the GPU Roe path agrees with
+// the restatement to rounding rather than bit for bit.
 #pragma once
 #include "mge_stage.h"
 
@@ -55,6 +58,7 @@ __device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long
 // enthalpy H = gamma/(gamma-1) p/rho + |u|^2/2 (g1 = gamma/(gamma-1))
 template <int DIR, typename T>
 __device__ inline T roe_state_flux(T g1, const T *q, T S, T *F) {
+#pragma clang fp contract(fast)
   T rr = T(1) / q[0];
   T q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   T H = g1 * (q[4] * rr) + T(0.5) * q2;
@@ -95,6 +99,7 @@ __device__ inline T fix_abs(T l, T del, T hd) {
 template <typename T>
 __device__ inline void muscl(int order, const T *qm2, const T *qm1, const T *qp0, const T *qp1, bool fm2, bool fp1,
                              T *qL, T *qR) {
+#pragma clang fp contract(fast)
   if (order < 2) {
 #pragma unroll
     for (int n = 0; n < 5; n++) {
@@ -119,6 +124,7 @@ __device__ inline void muscl(int order, const T *qm2, const T *qm1, const T *qp0
 template <int DIR, typename T>
 __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T *qL, const T *qR, T S, T *e,
                                 T *d) {
+#pragma clang fp contract(fast)
   T FL[5], FR[5];
   T HL = roe_state_flux<DIR>(g1, qL, S, FL);
   T HR = roe_state_flux<DIR>(g1, qR, S, FR);
@@ -343,6 +349,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = i <= d.il && j <= d.jl;
   const bool inarr = i <= d.ib && j <= d.jb;
+  static_assert(TJ >= 3, "the high-edge faces are spread over waves 1 and 2");
   const bool diss = A.has_diss;
   const T Sz = face_s<2>(c.g, i, j, 0);
   // column carry: primitives of k-1..k+2 and the bottom face of plane k
@@ -421,18 +428,20 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           sm.yd[n][ty][tx] = dd[n];
         }
       }
-      if (tid < TJ) {
-        int jj = j0 + tid, a = i0 + TI;
+      // the tile's high-edge faces go to two different waves (x edge:
+      // wave 2, y edge: wave 1) so no wave evaluates more than one extra face
+      if (tid >= 2 * TI && tid < 2 * TI + TJ) {
+        int jj = j0 + tid - 2 * TI, a = i0 + TI;
         if (jj <= d.jl && a <= d.ie) {
-          roe_lds_face<0>(sm, A, tid + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd);
+          roe_lds_face<0>(sm, A, tid - 2 * TI + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd);
 #pragma unroll
           for (int n = 0; n < 5; n++) {
-            sm.xe[n][tid][TI] = e[n];
-            sm.xd[n][tid][TI] = dd[n];
+            sm.xe[n][tid - 2 * TI][TI] = e[n];
+            sm.xd[n][tid - 2 * TI][TI] = dd[n];
           }
         }
-      } else if (tid < TJ + TI) {
-        int cidx = tid - TJ, ii = i0 + cidx, a = j0 + TJ;
+      } else if (tid >= TI && tid < 2 * TI) {
+        int cidx = tid - TI, ii = i0 + cidx, a = j0 + TJ;
         if (ii <= d.il && a <= d.je) {
           roe_lds_face<1>(sm, A, TJ + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd);
 #pragma unroll
