@@ -10,9 +10,11 @@ a residual-only pass counting 1).
 
 The headline value is the plain FP64 path; the same workload is then run
 on the FP32 bulk path and reported beside it.  Prints one JSON line with
-the throughput, the roofline of the dominant kernel (k_stage; HIP events
-on the library's stream over the timed region) and the CPU baseline (the
-FP64 CPU restatement, one core, bounded sample).
+the throughput (timed without instrumentation), the roofline of the
+dominant kernel (the finest-level stage kernel; HIP events recorded on the
+library's stream around each launch in a separate pass of a few cycles
+after the timed ones) and the CPU baseline (the FP64 CPU restatement, one
+core, bounded sample).
 """
 import argparse
 import json
@@ -79,33 +81,44 @@ def load_traffic(tag):
 
 
 def run_leg(prm, levels, geoms, aref, args, precision):
-    """warmup + timed cycles of one precision; returns the measurements"""
+    """warmup, the timed cycles (no instrumentation), then a separate pass of
+    a few cycles with HIP events around every launch for the roofline of
+    the finest-level stage kernel; returns the measurements"""
     import dataclasses
     from code_mge_amd.solver import Case, force_coefficients
     pp = dataclasses.replace(prm, precision=precision)
-    case = Case(pp, levels, geoms=geoms, mxcyc=args.warmup + args.steps + 2)
+    nroof = 0 if args.no_timing else args.roofline_steps
+    case = Case(pp, levels, geoms=geoms, mxcyc=args.warmup + args.steps + nroof + 2)
     case.init_flow()
     nl = args.levels
     print(f"[bench] FP{precision}: warmup {args.warmup} cycles", file=sys.stderr, flush=True)
     r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
-    timing = not args.no_timing
-    case.timing(enable=timing, name=None)
     case.sync()
     t0 = time.perf_counter()
     r1, cs = case.cycles(nl, args.steps, ncyc0=args.warmup + 1)   # ends with a stream sync
     t1 = time.perf_counter()
-    ms, cnt, by = case.timing(enable=False, name="k_stage")
     dt = t1 - t0
     cd, cl = force_coefficients(case, aref)
     print(f"[bench] FP{precision}: {args.steps} cycles in {dt:.3f} s, {cs/1e6:.1f} M cell-stages, "
           f"res {r0:.3e} -> {r1:.3e}", file=sys.stderr, flush=True)
     roofline = None
-    if timing and cnt > 0 and ms > 0:
-        achieved = by / (ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_stage", "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                    "traffic": load_traffic(f"fp{precision}"), "avg_launch_ms": round(ms, 5), "launches": cnt,
-                    "alg_bytes_per_launch": round(by)}
+    if nroof > 0:
+        case.timing(enable=True, name=None)
+        case.cycles(nl, nroof, ncyc0=args.warmup + args.steps + 1)
+        ms, cnt, by = case.timing(enable=False, name=f"k_stage@{nl}")
+        ms_all, cnt_all, by_all = case.timing(enable=False, name="k_stage")
+        if cnt > 0 and ms > 0:
+            achieved = by / (ms * 1e-3) / 1e9
+            kname = {"r1": "k_stage_roe", "r2": "k_stage_roe"}.get(prm.distype.lower(), "k_stage2")
+            roofline = {"kernel": f"{kname}<{'double' if precision == 64 else 'float'}> on the finest level "
+                                  f"({levels[-1].n[0]}x{levels[-1].n[1]}x{levels[-1].n[2]})",
+                        "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(f"fp{precision}"),
+                        "avg_launch_ms": round(ms, 5), "launches": cnt, "alg_bytes_per_launch": round(by),
+                        "timed_cycles": nroof,
+                        "all_levels": {"avg_launch_ms": round(ms_all, 5), "launches": cnt_all,
+                                       "frac": round(by_all / (ms_all * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                                       if ms_all > 0 else None}}
     case.close()
     return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1)
 
@@ -159,6 +172,30 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
             "cycles_fp32": [served[i].ncyc for i in sorted(served)], "cycles_fp64": [alone[i][0] for i in sorted(alone)]}
 
 
+def traffic_probe(args, precision):
+    """A short program for rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE):
+    a 1 GiB device-to-device copy (the calibration dispatch: 1 GiB read,
+    1 GiB written) and then one cycle of the bench workload, whose
+    finest-level stage-kernel dispatches give the HBM bytes per launch."""
+    import dataclasses
+    import torch
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    from code_mge_amd.solver import Case
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, mach=0.5, alpha=2.0, distype="r2")
+    geoms = [level_geometry(s, bdy) for s in levels]
+    with Case(dataclasses.replace(prm, precision=precision), levels, geoms=geoms, mxcyc=4) as c:
+        c.init_flow()
+        c.cycles(args.levels, 1, ncyc0=1)
+        c.sync()
+    print(f"[bench] traffic probe FP{precision} done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,10 +204,16 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel HIP event timing")
+    ap.add_argument("--no-timing", action="store_true", help="skip the HIP-event roofline pass")
+    ap.add_argument("--roofline-steps", type=int, default=2, help="cycles of the HIP-event roofline pass")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
+    ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
+                    help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args, args.traffic_probe)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

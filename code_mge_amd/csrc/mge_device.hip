@@ -104,6 +104,7 @@ struct Timer {
   std::string name;
   hipEvent_t a, b;
   double bytes;
+  int level;  // level of the launch (0: not level-specific)
 };
 
 // one recorded launch of a case (batch programs): both entry points of the
@@ -136,6 +137,7 @@ struct mge_case {
   std::vector<hipEvent_t> pool;
   std::map<std::string, TStat> tstat;
   double next_bytes = 0.0;  // algorithmic bytes of the next timed launch
+  int cur_level = 0;        // level the current operation works on (timer statistics)
   double smoo_bb = 0.0;
   std::vector<Rec> *rec = nullptr;  // when set, launches are recorded, not run
   hipError_t launch_err = hipSuccess;  // first failed launch, reported by check_err
@@ -186,6 +188,7 @@ void tic(mge_case *h, const char *name, Timer *t) {
   h->next_bytes = 0.0;
   if (!h->timing) return;
   t->name = name;
+  t->level = h->cur_level;
   t->a = ev_get(h);
   t->b = ev_get(h);
   hipEventRecord(t->a, h->stream);
@@ -200,10 +203,14 @@ void resolve_timers(mge_case *h) {
     hipEventSynchronize(t.b);
     float ms = 0.f;
     hipEventElapsedTime(&ms, t.a, t.b);
-    auto &s = h->tstat[t.name];
-    s.ms += ms;
-    s.count += 1;
-    s.bytes += t.bytes;
+    // per kernel, and per kernel and level ("name@level")
+    for (int v = 0; v < 2; v++) {
+      if (v == 1 && t.level <= 0) break;
+      auto &s = h->tstat[v == 0 ? t.name : t.name + "@" + std::to_string(t.level)];
+      s.ms += ms;
+      s.count += 1;
+      s.bytes += t.bytes;
+    }
     h->pool.push_back(t.a);
     h->pool.push_back(t.b);
   }
@@ -408,6 +415,11 @@ template <class T>
 int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir) {
   const mge_params &P = h->prm;
   const Dims &d = L.d;
+  struct LevelTag {  // timer statistics of this call carry the level number
+    mge_case *h;
+    ~LevelTag() { h->cur_level = 0; }
+  } tag{h};
+  h->cur_level = (int)(&L - h->lv.data());  // lv[1..nlevel]
   const long E = d.E;
   if (P.smoo > 0.0 && (d.ni < 2 || d.nj < 2 || d.nk < 2))
     return set_err(MGE_ERR_SMOO_DIM, "residual smoothing needs at least 2 cells per direction");

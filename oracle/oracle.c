@@ -976,6 +976,196 @@ static int roe_wall(const blk_t *b, const orc_params *P, const orc_cut *t, doubl
   return ORC_OK;
 }
 
+/* ------------------------------------------------------------ viscous fluxes
+ *
+ * euler.f90:137-181 computes vel = m/rho and tem = p/(rho R) on every cell
+ * (zero on blank cells) and calls fgradi/fgradj/fgradk and vflux of the
+ * viscpak module when cdis(m) > 0; the result "is added to dissipation and
+ * applied with blending".  viscpak has no source in the reference tree, so
+ * the laminar flux below is this build's own (parity unpinned against the
+ * reference; pinned by the Couette and Poiseuille closed forms):
+ *  - face gradient: normal derivative (phi_a - phi_(a-1)) / hn with hn the
+ *    centre distance; tangential derivatives the mean of the two cells'
+ *    centred differences, one-sided where a neighbour is blank, zero where
+ *    both are;
+ *  - tau = mu (G + G^T) - 2/3 mu div(u) I, q = -kappa dT/dn with kappa =
+ *    mu gamma R / ((gamma-1) Pr); face flux (0, tau_n, u_f . tau_n - q_n)
+ *    times the face vector S (u_f the mean of the two cells);
+ *  - no bulk flux through a face with a blank side; each wall record adds
+ *    the flux of the face between its cut cell and its mirror state
+ *    (no-slip ghost from bc_cutcell), normal derivatives only;
+ *  - per cell fw += cdis * ((gx_hi - gx_lo) + (gy_hi - gy_lo) + (gz_hi -
+ *    gz_lo)), after the blending and before the dissipation; the wall
+ *    records then add cdis * (+-g) to their cut cells in record order. */
+static void visc_face(int dir, const double *pL, const double *pR, const double *t1L, const double *t1R,
+                      const double *t2L, const double *t2R, double hn, double S, double mu, double kap,
+                      double *g) {
+  const int t1 = dir == 0 ? 1 : 0, t2 = dir == 2 ? 1 : 2;
+  double G[3][3];
+  for (int a = 0; a < 3; a++) {
+    G[a][dir] = (pR[a] - pL[a]) / hn;
+    G[a][t1] = 0.5 * (t1L[a] + t1R[a]);
+    G[a][t2] = 0.5 * (t2L[a] + t2R[a]);
+  }
+  double dT = (pR[3] - pL[3]) / hn;
+  double div = (G[0][0] + G[1][1]) + G[2][2];
+  double tau[3];
+  for (int a = 0; a < 3; a++) tau[a] = mu * (G[a][dir] + G[dir][a]);
+  tau[dir] = mu * (2.0 * G[dir][dir] - (2.0 / 3.0) * div);
+  double uf[3];
+  for (int a = 0; a < 3; a++) uf[a] = 0.5 * (pL[a] + pR[a]);
+  double qn = -kap * dT;
+  double F[5];
+  F[0] = 0.0;
+  F[1] = tau[0];
+  F[2] = tau[1];
+  F[3] = tau[2];
+  F[4] = ((uf[0] * tau[0] + uf[1] * tau[1]) + uf[2] * tau[2]) - qn;
+  for (int n = 0; n < 5; n++) g[n] = F[n] * S;
+}
+
+/* centred (one-sided next to a blank cell) difference of the velocity of
+ * cell q along axis t; c = the cell's index along t, st the stride, h the
+ * axis' cell widths */
+static void visc_dt(const blk_t *b, const double *prim, long q, long st, int c, const double *h, double *o) {
+  const long E = b->E;
+  int fp = b->blank[q + st] != 0, fm = b->blank[q - st] != 0;
+  for (int a = 0; a < 3; a++) {
+    const double *v = prim + (long)a * E;
+    if (fp && fm)
+      o[a] = (v[q + st] - v[q - st]) / (0.5 * (h[c - 1] + h[c]) + 0.5 * (h[c] + h[c + 1]));
+    else if (fp)
+      o[a] = (v[q + st] - v[q]) / (0.5 * (h[c] + h[c + 1]));
+    else if (fm)
+      o[a] = (v[q] - v[q - st]) / (0.5 * (h[c - 1] + h[c]));
+    else
+      o[a] = 0.0;
+  }
+}
+
+typedef struct {
+  double *prim;       /* u, v, w, T (4E) */
+  double *h[3];       /* cell widths along each axis */
+  long st[3];
+} visc_ws;
+
+static void visc_setup(const blk_t *b, const orc_params *P, visc_ws *V) {
+  const long E = b->E;
+  V->prim = (double *)malloc(sizeof(double) * (size_t)E * 4);
+  int nmax = b->ib + 2;
+  if (b->jb + 2 > nmax) nmax = b->jb + 2;
+  if (b->kb + 2 > nmax) nmax = b->kb + 2;
+  for (int a = 0; a < 3; a++) V->h[a] = (double *)calloc((size_t)nmax, sizeof(double));
+  for (int i = 1; i < b->ib; i++) V->h[0][i] = b->x[IX(b, i + 1, 1, 1)] - b->x[IX(b, i, 1, 1)];
+  for (int j = 1; j < b->jb; j++) V->h[1][j] = b->x[E + IX(b, 1, j + 1, 1)] - b->x[E + IX(b, 1, j, 1)];
+  for (int k = 1; k < b->kb; k++) V->h[2][k] = b->x[2 * E + IX(b, 1, 1, k + 1)] - b->x[2 * E + IX(b, 1, 1, k)];
+  V->st[0] = 1;
+  V->st[1] = b->ib + 1;
+  V->st[2] = (long)(b->ib + 1) * (b->jb + 1);
+  /* euler.f90:140-150 */
+  for (long q = 0; q < E; q++) {
+    double r = W(b, 0, q);
+    V->prim[q] = W(b, 1, q) / r;
+    V->prim[E + q] = W(b, 2, q) / r;
+    V->prim[2 * E + q] = W(b, 3, q) / r;
+    V->prim[3 * E + q] = b->p[q] / (r * P->gas_const);
+    if (b->blank[q] == 0)
+      for (int a = 0; a < 4; a++) V->prim[(long)a * E + q] = 0.0;
+  }
+}
+
+static void visc_free(visc_ws *V) {
+  free(V->prim);
+  for (int a = 0; a < 3; a++) free(V->h[a]);
+}
+
+/* bulk flux of the face in direction dir whose higher cell is q (index a
+ * along dir); zero when either side is blank */
+static void visc_bulk_face(const blk_t *b, const orc_params *P, const visc_ws *V, int dir, long q, const int *ijk,
+                           double *g) {
+  const long E = b->E;
+  const long sd = V->st[dir];
+  for (int n = 0; n < 5; n++) g[n] = 0.0;
+  if (b->blank[q] == 0 || b->blank[q - sd] == 0) return;
+  const int t1 = dir == 0 ? 1 : 0, t2 = dir == 2 ? 1 : 2;
+  double pL[4], pR[4], t1L[3], t1R[3], t2L[3], t2R[3];
+  for (int a = 0; a < 4; a++) {
+    pL[a] = V->prim[(long)a * E + q - sd];
+    pR[a] = V->prim[(long)a * E + q];
+  }
+  visc_dt(b, V->prim, q - sd, V->st[t1], ijk[t1], V->h[t1], t1L);
+  visc_dt(b, V->prim, q, V->st[t1], ijk[t1], V->h[t1], t1R);
+  visc_dt(b, V->prim, q - sd, V->st[t2], ijk[t2], V->h[t2], t2L);
+  visc_dt(b, V->prim, q, V->st[t2], ijk[t2], V->h[t2], t2R);
+  const double *hd = V->h[dir];
+  double hn = 0.5 * (hd[ijk[dir] - 1] + hd[ijk[dir]]);
+  const double *Sv = dir == 0 ? b->si : (dir == 1 ? b->sj : b->sk);
+  double S = Sv[(long)dir * E + q];
+  double mu = P->viscl;
+  double kap = mu * P->gamma * P->gas_const / ((P->gamma - 1.0) * P->prlmean);
+  visc_face(dir, pL, pR, t1L, t1R, t2L, t2R, hn, S, mu, kap, g);
+}
+
+/* wall-record flux: the face between the cut cell and its mirror state */
+static void visc_wall_face(const blk_t *b, const orc_params *P, const visc_ws *V, const orc_cut *t, double *g,
+                           int *hi_out) {
+  const long E = b->E;
+  const int dir = t->zz - 1;
+  const int hi = t->faceid >= 4;
+  long qc = IX(b, t->ccid[0], t->ccid[1], t->ccid[2]);
+  long qs = IX(b, t->scid[0], t->scid[1], t->scid[2]);
+  double pc[4], pg[4], zero[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 4; a++) pc[a] = V->prim[(long)a * E + qc];
+  double rs = t->wsol[0];
+  pg[0] = t->wsol[1] / rs;
+  pg[1] = t->wsol[2] / rs;
+  pg[2] = t->wsol[3] / rs;
+  pg[3] = t->psol / (rs * P->gas_const);
+  const double *hd = V->h[dir];
+  int c = t->ccid[dir];
+  const double *Sv = dir == 0 ? b->si : (dir == 1 ? b->sj : b->sk);
+  double mu = P->viscl;
+  double kap = mu * P->gamma * P->gas_const / ((P->gamma - 1.0) * P->prlmean);
+  if (hi)
+    visc_face(dir, pc, pg, zero, zero, zero, zero, 0.5 * (hd[c] + hd[c + 1]), Sv[(long)dir * E + qs], mu, kap, g);
+  else
+    visc_face(dir, pg, pc, zero, zero, zero, zero, 0.5 * (hd[c - 1] + hd[c]), Sv[(long)dir * E + qc], mu, kap, g);
+  *hi_out = hi;
+}
+
+static void visc_fluxes(blk_t *b, const orc_params *P, double cdis) {
+  const long E = b->E;
+  visc_ws V;
+  visc_setup(b, P, &V);
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++) {
+        long q = IX(b, i, j, k);
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int dir = 0; dir < 3; dir++) {
+          int lo[3] = {i, j, k}, up[3] = {i, j, k};
+          up[dir] += 1;
+          double glo[5], ghi[5];
+          visc_bulk_face(b, P, &V, dir, q, lo, glo);
+          visc_bulk_face(b, P, &V, dir, q + V.st[dir], up, ghi);
+          for (int n = 0; n < 5; n++) acc[n] = acc[n] + (ghi[n] - glo[n]);
+        }
+        for (int n = 0; n < 5; n++) b->fw[(long)n * E + q] = b->fw[(long)n * E + q] + cdis * acc[n];
+      }
+  for (int m = 0; m < b->nmir; m++) {
+    const orc_cut *t = &b->z[m];
+    double g[5];
+    int hi;
+    visc_wall_face(b, P, &V, t, g, &hi);
+    long qc = IX(b, t->ccid[0], t->ccid[1], t->ccid[2]);
+    for (int n = 0; n < 5; n++) {
+      double v = hi ? g[n] : -g[n];
+      b->fw[(long)n * E + qc] = b->fw[(long)n * E + qc] + cdis * v;
+    }
+  }
+  visc_free(&V);
+}
+
 /* one stage's Roe fluxes: fw += upwind part (when cdis > 0), dw = central
  * part, then the wall records in order */
 static int roe_fluxes(blk_t *b, const orc_params *P, int istage) {
@@ -1372,6 +1562,7 @@ int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int l
               b->fw[q] = b->fw[q] * (1.0 - P->cdis[m - 1]);
             }
     }
+    if (P->eqntype == 1 && P->cdis[m - 1] > 0.0) visc_fluxes(b, P, P->cdis[m - 1]); /* euler.f90:137-181 */
     if (P->distype == 0) {
       if (P->cdis[m - 1] > 0.0) dflux(b, P, m);
       rc = eflux(b);
@@ -1745,19 +1936,37 @@ double orc_get_dtim(orc_case *c, int lvl) { return c->lv[lvl].dtim; }
 
 /* wall force: momentum flux through each wall face as the scheme applies it
  * (efluxb.f90:251-253), outward from the fluid cell */
+/* viscous part of a record's wall force: minus the viscous momentum flux
+ * leaving the fluid through the wall face (zero for Euler) */
+static void visc_wall_force(const blk_t *b, const orc_params *P, const visc_ws *V, const orc_cut *t, double *fv) {
+  for (int d = 0; d < 3; d++) fv[d] = 0.0;
+  if (P->eqntype != 1) return;
+  double g[5];
+  int hi;
+  visc_wall_face(b, P, V, t, g, &hi);
+  for (int d = 0; d < 3; d++) fv[d] = hi ? g[1 + d] : -g[1 + d];
+}
+
 int orc_wall_force(orc_case *c, int lvl, double out[3]) {
   blk_t *b = &c->lv[lvl];
   const long E = b->E;
   double F[3] = {0, 0, 0};
+  visc_ws V;
+  visc_setup(b, &c->prm, &V);
   if (c->prm.distype != 0) {
     /* Roe: the full wall-face flux (central + upwind, cdis = 1) */
     for (int m = 0; m < b->nmir; m++) {
-      double ve[5], vd[5];
+      double ve[5], vd[5], fv[3];
       int rc = roe_wall(b, &c->prm, &b->z[m], 1.0, ve, vd);
-      if (rc) return rc;
-      for (int d = 0; d < 3; d++) F[d] = F[d] + (ve[1 + d] + vd[1 + d]);
+      if (rc) {
+        visc_free(&V);
+        return rc;
+      }
+      visc_wall_force(b, &c->prm, &V, &b->z[m], fv);
+      for (int d = 0; d < 3; d++) F[d] = F[d] + ((ve[1 + d] + vd[1 + d]) + fv[d]);
     }
     for (int d = 0; d < 3; d++) out[d] = F[d];
+    visc_free(&V);
     return ORC_OK;
   }
   for (int m = 0; m < b->nmir; m++) {
@@ -1771,10 +1980,12 @@ int orc_wall_force(orc_case *c, int lvl, double out[3]) {
     get5(b, qc, wc);
     ceflux(wc, b->p[qc], sc, f1);
     ceflux(t->wsol, t->psol, ss, f2);
-    double sgn = (t->faceid >= 4) ? -1.0 : 1.0;
-    for (int d = 0; d < 3; d++) F[d] = F[d] + sgn * (0.5 * (f1[1 + d] + f2[1 + d]));
+    double sgn = (t->faceid >= 4) ? -1.0 : 1.0, fv[3];
+    visc_wall_force(b, &c->prm, &V, t, fv);
+    for (int d = 0; d < 3; d++) F[d] = F[d] + (sgn * (0.5 * (f1[1 + d] + f2[1 + d])) + fv[d]);
   }
   for (int d = 0; d < 3; d++) out[d] = F[d];
+  visc_free(&V);
   return ORC_OK;
 }
 
@@ -1783,4 +1994,18 @@ int orc_psmoo(orc_case *c, int lvl) { return psmoo_old(&c->lv[lvl], &c->prm); }
 void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
                       const double *ss, double *ff) {
   dflux_matrix(gamma, u1, u2, dw, ss, ff);
+}
+
+/* test hooks for the viscous flux: one face, and one full pass into fw
+ * (fw zeroed first) */
+void orc_visc_face(int dir, const double *pL, const double *pR, const double *t1L, const double *t1R,
+                   const double *t2L, const double *t2R, double hn, double S, double mu, double kap, double *g) {
+  visc_face(dir, pL, pR, t1L, t1R, t2L, t2R, hn, S, mu, kap, g);
+}
+
+int orc_visc(orc_case *c, int lvl, double cdis) {
+  blk_t *b = &c->lv[lvl];
+  memset(b->fw, 0, sizeof(double) * (size_t)b->E * 5);
+  visc_fluxes(b, &c->prm, cdis);
+  return ORC_OK;
 }

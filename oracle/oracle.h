@@ -119,6 +119,11 @@ double orc_get_dtim(orc_case *c, int lvl);
  * scheme applies (efluxb.f90:251-253), summed over records in order. */
 int    orc_wall_force(orc_case *c, int lvl, double out[3]);
 
+/* viscous-flux test hooks (own laminar flux; see oracle.c) */
+void orc_visc_face(int dir, const double *pL, const double *pR, const double *t1L, const double *t1R,
+                   const double *t2L, const double *t2R, double hn, double S, double mu, double kap, double *g);
+int  orc_visc(orc_case *c, int lvl, double cdis);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,9 @@ def lib():
     L.orc_wall_force.argtypes = [P, I, _dp]
     L.orc_psmoo.argtypes = [P, I]
     L.orc_dflux_matrix.argtypes = [C.c_double, _dp, _dp, _dp, _dp, _dp]
+    L.orc_visc_face.argtypes = [I, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                C.c_double, _dp]
+    L.orc_visc.argtypes = [P, I, C.c_double]
     _lib = L
     return L
 
@@ -231,6 +234,10 @@ class OracleCase:
     def psmoo(self, lvl):
         return self._chk(self.L.orc_psmoo(self.h, lvl), "psmoo")
 
+    def visc(self, lvl, cdis=1.0):
+        """one viscous pass into fw (zeroed first)"""
+        return self._chk(self.L.orc_visc(self.h, lvl, cdis), "visc")
+
 
 def dflux_matrix(gamma, u1, u2, dw, ss):
     L = lib()
@@ -238,3 +245,12 @@ def dflux_matrix(gamma, u1, u2, dw, ss):
     ff = np.zeros(5)
     L.orc_dflux_matrix(gamma, *[x.ctypes.data_as(_dp) for x in a], ff.ctypes.data_as(_dp))
     return ff
+
+
+def visc_face(direction, pL, pR, t1L, t1R, t2L, t2R, hn, S, mu, kap):
+    """one viscous face flux of the oracle (5 components)"""
+    L = lib()
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (pL, pR, t1L, t1R, t2L, t2R)]
+    g = np.zeros(5)
+    L.orc_visc_face(direction, *[x.ctypes.data_as(_dp) for x in a], hn, S, mu, kap, g.ctypes.data_as(_dp))
+    return g
