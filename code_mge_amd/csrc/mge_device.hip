@@ -31,6 +31,7 @@
 #include "mge_roe.h"
 #include "mge_smooth.h"
 #include "mge_stage.h"
+#include "mge_visc.h"
 
 using namespace mge;
 
@@ -95,6 +96,10 @@ struct Level {
   double *dtim = nullptr;
   double *rtrms = nullptr;
   double *force = nullptr;
+  void *vprim = nullptr;        // Navier-Stokes: velocity and temperature per cell (4E)
+  int nvcell = 0;               // cut cells with wall records (viscous wall terms)
+  long *vcell = nullptr;
+  int *vbeg = nullptr, *vrec = nullptr;
   int mxcyc = 1;
   std::vector<void *> allocs;
   size_t bytes = 0;  // device bytes of the level
@@ -411,6 +416,31 @@ int do_timestep(mge_case *h, Level &L) {
 
 // ---- euler -------------------------------------------------------------------
 
+// arguments of the viscous kernels on state buffer buf
+template <class T>
+ViscArgs<T> visc_args(mge_case *h, Level &L, int buf, T cdis, T blend, int m) {
+  const mge_params &P = h->prm;
+  ViscArgs<T> V{};
+  V.d = L.d;
+  V.s = stc<T>(L, buf);
+  V.flags = L.flags;
+  V.g = geo<T>(L);
+  V.prim = tp<T>(L.vprim);
+  V.fw = tp<T>(L.fw);
+  V.inf = make_inf<T>(P);
+  V.uinf[0] = (T)P.u0;
+  V.uinf[1] = (T)P.v0;
+  V.uinf[2] = (T)P.w0;
+  V.tinf = (T)(P.p0 / (P.rho0 * P.gas_const));
+  V.rgas = (T)P.gas_const;
+  V.mu = (T)P.viscl;
+  V.kap = (T)(P.viscl * P.gamma * P.gas_const / ((P.gamma - 1.0) * P.prlmean));
+  V.cdis = cdis;
+  V.blend = blend;
+  V.m = m;
+  return V;
+}
+
 template <class T>
 int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir) {
   const mge_params &P = h->prm;
@@ -469,6 +499,16 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     A.order = P.distype == 2 ? 2 : 1;
     A.eps = (T)P.epsentr;
     A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
+    // laminar viscous fluxes (euler.f90:137-181): fw = blend fw + cdis V
+    // before the stage kernel adds its dissipation
+    A.fw_in = P.eqntype == 1 && A.has_diss;
+    if (A.fw_in) {
+      ViscArgs<T> V = visc_args<T>(h, L, src, A.fx.cdis, A.blend, m);
+      LAUNCHT(h, "k_visc_prim", k_visc_prim, T, flat_grid(E), dim3(256), V);
+      LAUNCHT(h, "k_visc", k_visc, T, cell_grid(d), cell_block(), V);
+      LAUNCHT(h, "k_visc_wall", k_visc_wall, T, dim3((L.nvcell + 63) / 64), dim3(64),
+              (ViscWallArgs<T>{V, L.recs, L.wsol, L.psol, L.nmir, L.vcell, L.vbeg, L.vrec, L.nvcell}));
+    }
     {
       // algorithmic bytes per cell: read W^(m-1); write W^(m) and read W^(0)
       // when this kernel updates, else write dw; forcing on coarse levels;
@@ -745,6 +785,11 @@ int do_wall_force(mge_case *h, Level &L) {
   hipLaunchKernelGGL(k_wall_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), 0, h->stream, fc, (const DevRec *)L.recs,
                      (const double *)L.wsol, (const double *)L.psol, L.nmir, h->prm.distype != 0 ? 1 : 0,
                      h->prm.gamma, h->prm.epsentr, L.opval, h->err);
+  if (h->prm.eqntype == 1 && L.nmir > 0) {
+    ViscArgs<T> V = visc_args<T>(h, L, L.cur, T(1), T(0), 1);
+    hipLaunchKernelGGL(k_wall_visc_terms<T>, dim3((L.nmir + 63) / 64), dim3(64), 0, h->stream, V,
+                       (const DevRec *)L.recs, (const double *)L.wsol, (const double *)L.psol, L.nmir, L.opval);
+  }
   hipLaunchKernelGGL(k_wall_sum, dim3(1), dim3(64), 0, h->stream, (const double *)L.opval, L.nmir, L.force);
   return MGE_OK;
 }
@@ -844,6 +889,39 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
   CHKRC(dalloc(L, &L.ops, opv.size() + 1));
   CHKRC(dalloc(L, &L.opval, (opv.size() + 1) * 10));
   HIPCHK(hipMemcpy(L.recs, dr.data(), sizeof(DevRec) * nmir, hipMemcpyHostToDevice));
+  {
+    // cut cells with wall records, in order of their first record, and
+    // their records in record order (viscous wall terms)
+    std::map<long, int> slot;
+    std::vector<long> vc;
+    std::vector<std::vector<int>> lists;
+    for (int m = 0; m < nmir; m++) {
+      long qc = ix(d, recs[m].ccid[0], recs[m].ccid[1], recs[m].ccid[2]);
+      auto it = slot.find(qc);
+      int sl;
+      if (it == slot.end()) {
+        sl = (int)vc.size();
+        slot[qc] = sl;
+        vc.push_back(qc);
+        lists.emplace_back();
+      } else {
+        sl = it->second;
+      }
+      lists[sl].push_back(m);
+    }
+    std::vector<int> vb(1, 0), vr;
+    for (auto &l : lists) {
+      vr.insert(vr.end(), l.begin(), l.end());
+      vb.push_back((int)vr.size());
+    }
+    L.nvcell = (int)vc.size();
+    CHKRC(dalloc(L, &L.vcell, vc.size()));
+    CHKRC(dalloc(L, &L.vbeg, vb.size()));
+    CHKRC(dalloc(L, &L.vrec, vr.size()));
+    HIPCHK(hipMemcpy(L.vcell, vc.data(), sizeof(long) * vc.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.vbeg, vb.data(), sizeof(int) * vb.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.vrec, vr.data(), sizeof(int) * vr.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemcpy(L.tcell, tcell.data(), sizeof(long) * tcell.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(L.beg, begv.data(), sizeof(int) * begv.size(), hipMemcpyHostToDevice));
   if (!opv.empty()) HIPCHK(hipMemcpy(L.ops, opv.data(), sizeof(int2) * opv.size(), hipMemcpyHostToDevice));
@@ -1255,6 +1333,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc(L, &L.dtim, 1));
   CHKRC(dalloc(L, &L.rtrms, L.mxcyc + 1));
   CHKRC(dalloc(L, &L.force, 3));
+  if (h->prm.eqntype == 1) CHKRC(dalloc_bytes(L, &L.vprim, es * 4 * E));
   // vertex differences (ghost entries of index 0 stay unused)
   std::vector<double> hx(nmax, 0.0), hy(nmax, 0.0), hz(nmax, 0.0);
   for (int i = 1; i < d.ib; i++) hx[i] = xv[i + 1] - xv[i];

@@ -459,7 +459,10 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     if (valid) {
       const long q = ix(d, i, j, k);
       T f[5], e[5];
-      if (A.m > 1) {
+      if (A.fw_in) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q];
+      } else if (A.m > 1) {
 #pragma unroll
         for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
       } else {

@@ -70,6 +70,7 @@ struct StageArgs {
   T eps;                    // Roe entropy-fix coefficient
   T g1;                     // gamma/(gamma-1)
   int kc;                   // k planes per workgroup (LDS-tiled kernels)
+  int fw_in;                // fw already holds this stage's blend and viscous part
 };
 
 template <typename T>
@@ -160,7 +161,10 @@ __device__ __attribute__((always_inline)) inline void k_stage_body(const StageAr
   const long E = d.E;
   long q = ix(d, i, j, k);
   T f[5];
-  if (A.m > 1) {
+  if (A.fw_in) {
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q];
+  } else if (A.m > 1) {
 #pragma unroll
     for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
   } else {
@@ -370,7 +374,10 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     if (valid) {
       const long q = ix(d, i, j, k);
       T f[5];
-      if (A.m > 1) {
+      if (A.fw_in) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q];
+      } else if (A.m > 1) {
 #pragma unroll
         for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
       } else {
