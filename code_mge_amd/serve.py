@@ -58,7 +58,7 @@ class Result:
         return self.completed - self.arrival
 
 
-def level_bytes(n, es, nmir=0, mxcyc=4):
+def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False):
     """device bytes of one level as the library allocates them (every array
     padded by 64 bytes); wall-record arrays at their upper bound"""
     ni, nj, nk = n
@@ -67,10 +67,13 @@ def level_bytes(n, es, nmir=0, mxcyc=4):
     ngrid = -(-ni // 64) * -(-nj // 4) * nk
     sizes = [es * 6 * E] * 3 + [es * 5 * E] * 5 + [es * E] * 2 + [E] * 2
     sizes += [es * nmax] * 3 + [es * 6 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
+    if viscous:
+        sizes += [es * 4 * E]           # velocity and temperature
     if nmir:
         ntouch, nops = 2 * nmir, 6 * nmir
         sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
                   80 * (nops + 1)]
+        sizes += [8 * nmir, 4 * (nmir + 1), 4 * nmir]   # cut cells with their record lists
     return sum(s + 64 for s in sizes)
 
 
@@ -96,7 +99,7 @@ class Server:
         total = 0
         for li, spec in enumerate(req.levels):
             nmir = geoms[li].nmir if geoms is not None else 0
-            total += level_bytes(spec.n, es, nmir)
+            total += level_bytes(spec.n, es, nmir, viscous=req.params.eqntype.lower() == "n")
         return total
 
     def run(self, requests: List[Request], clock=time.monotonic) -> List[Result]:
@@ -170,14 +173,16 @@ def percentiles(results: List[Result]):
     return float(np.percentile(t, 50)), float(np.percentile(t, 99))
 
 
-def poisson_requests(count=256, rate=4.0, seed=20261203, nlevel=3, box=8.0, euler_only=True):
+def poisson_requests(count=256, rate=4.0, seed=20261203, nlevel=3, box=8.0, euler_only=False):
     """Requests with exponential inter-arrival gaps (rate per second) from
     numpy PCG64(seed); per request in order: type ~ Bernoulli(0.5)
     Euler/Navier-Stokes, grid N ~ exp U[ln 32, ln 192] rounded to a multiple
-    of 8 cells per edge, M ~ U[0.1, 0.8], Re ~ exp U[ln 20, ln 200] for the
-    Navier-Stokes type; a sphere of radius 0.5 in [-box, box]^3.  With
-    euler_only the Navier-Stokes draws are kept (same stream) but served as
-    Euler requests."""
+    of 8 cells per edge, M ~ U[0.1, 0.8], Re ~ exp U[ln 20, ln 200] (drawn
+    for the Navier-Stokes type only), then the body as in the ellipsoid
+    sweep: semi-axes ~ U[0.2, 1.0], a uniformly random rotation (unit
+    quaternion of 4 normals), alpha ~ U[-5, 15] deg; box [-box, box]^3.
+    With euler_only the Navier-Stokes draws are kept (same stream) but
+    served as Euler requests."""
     from . import configs
     rng = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
@@ -188,9 +193,16 @@ def poisson_requests(count=256, rate=4.0, seed=20261203, nlevel=3, box=8.0, eule
         n = int(8 * round(math.exp(rng.uniform(math.log(32.0), math.log(192.0))) / 8.0))
         mach = float(rng.uniform(0.1, 0.8))
         re = float(math.exp(rng.uniform(math.log(20.0), math.log(200.0)))) if ns else None
+        semi = tuple(float(v) for v in rng.uniform(0.2, 1.0, size=3))
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        alpha = float(rng.uniform(-5.0, 15.0))
         kw = {}
         if ns and not euler_only:
             kw = dict(eqntype="n", reynolds=re)
-        prm, levels, bdy, aref = configs.sphere3d(n=n, nlevel=nlevel, mach=mach, alpha=0.0, box=box, **kw)
-        out.append(Request(rid, t, prm, levels, bdy, aref, dict(n=n, mach=mach, ns=ns, re=re)))
+        prm, levels, bdy, aref = configs.ellipsoid3d(n=n, nlevel=nlevel, semi=semi,
+                                                     rot=configs.quaternion_rotation(q), mach=mach, alpha=alpha,
+                                                     box=box, **kw)
+        out.append(Request(rid, t, prm, levels, bdy, aref,
+                           dict(n=n, mach=mach, ns=ns and not euler_only, re=re, semi=semi, alpha=alpha)))
     return out
