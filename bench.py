@@ -90,6 +90,8 @@ def run_leg(prm, levels, geoms, aref, args, precision):
     nroof = 0 if args.no_timing else args.roofline_steps
     case = Case(pp, levels, geoms=geoms, mxcyc=args.warmup + args.steps + nroof + 2)
     case.init_flow()
+    tuned = case.autotune()   # k-chunk length of the stage kernel per level
+    print(f"[bench] FP{precision}: autotuned k-chunks {tuned}", file=sys.stderr, flush=True)
     nl = args.levels
     print(f"[bench] FP{precision}: warmup {args.warmup} cycles", file=sys.stderr, flush=True)
     r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
@@ -120,7 +122,8 @@ def run_leg(prm, levels, geoms, aref, args, precision):
                                        "frac": round(by_all / (ms_all * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
                                        if ms_all > 0 else None}}
     case.close()
-    return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1)
+    return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1,
+                kc={lvl: kc for lvl, (kc, us) in tuned.items()})
 
 
 def sweep_leg(k, drop=1e5, maxcyc=3000):
@@ -264,7 +267,7 @@ def main():
         "config": {"workload": f"sphere {args.n}^3 in [-8,8]^3, M=0.5, alpha=2, {args.levels}-level W-cycle, "
                                "5-stage RK with residual smoothing, Roe flux + MUSCL-minmod, FP64",
                    "n": args.n, "levels": args.levels, "cell_stages_per_step": cs // args.steps,
-                   "cd": round(cd, 6), "cl": round(cl, 6)},
+                   "cd": round(cd, 6), "cl": round(cl, 6), "stage_kchunk_per_level": main_leg["kc"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32": fp32,

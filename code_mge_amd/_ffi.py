@@ -111,6 +111,8 @@ def lib():
     L.mge_cycles.argtypes = [P, I, I, I, dp, C.POINTER(C.c_longlong)]
     L.mge_case_bytes.argtypes = [P]
     L.mge_case_bytes.restype = C.c_longlong
+    L.mge_autotune.argtypes = [P, I, C.POINTER(I), dp]
+    L.mge_set_kc.argtypes = [P, I, I]
     L.mge_batch_create.argtypes = [I, I, C.POINTER(P)]
     L.mge_batch_destroy.argtypes = [P]
     L.mge_batch_add.argtypes = [P, P, D, I]
@@ -137,5 +139,5 @@ EXPORTED = [
     "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
     "mge_kernel_timing", "mge_last_error", "mge_cycles",
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
-    "mge_batch_status", "mge_batch_log", "mge_case_bytes",
+    "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc",
 ]

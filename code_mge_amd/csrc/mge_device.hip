@@ -23,6 +23,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -96,6 +98,7 @@ struct Level {
   double *dtim = nullptr;
   double *rtrms = nullptr;
   double *force = nullptr;
+  int kc = 0;                   // autotuned k planes per stage workgroup (0: fill rule)
   void *vprim = nullptr;        // Navier-Stokes: velocity and temperature per cell (4E)
   int nvcell = 0;               // cut cells with wall records (viscous wall terms)
   long *vcell = nullptr;
@@ -441,6 +444,66 @@ ViscArgs<T> visc_args(mge_case *h, Level &L, int buf, T cdis, T blend, int m) {
   return V;
 }
 
+// the stage kernel's arguments for stage m reading state buffer src and
+// writing dst (w0: the stage-0 buffer)
+template <class T>
+StageArgs<T> stage_args(mge_case *h, Level &L, int m, int src, int dst, int w0, int mgl, int mlvl, int lres,
+                        int lfir) {
+  const mge_params &P = h->prm;
+  const Dims &d = L.d;
+  const long E = d.E;
+  T *dwp = tp<T>(L.dw);
+  StageArgs<T> A{};
+  A.fx.d = d;
+  A.fx.s = stc<T>(L, src);
+  A.fx.g = geo<T>(L);
+  A.fx.radsum = tp<T>(L.radsum);
+  A.fx.gamma = (T)P.gamma;
+  A.fx.vis2 = (T)P.vis2;
+  A.fx.vis4 = (T)P.vis4;
+  A.fx.cdis = (T)P.cdis[m - 1];
+  A.fx.dissip = P.dissip;
+  A.fx.inf = make_inf<T>(P);
+  A.wdst = tp<T>(L.S[dst]);
+  A.pdst = tp<T>(L.S[dst]) + 5 * E;
+  A.wn = tp<T>(L.S[w0]);
+  A.dw = dwp;
+  A.fw = tp<T>(L.fw);
+  A.qf = tp<T>(L.qf);
+  A.wr = tp<T>(L.wr);
+  A.marker = L.marker;
+  A.flags = L.flags;
+  A.vdtim = tp<T>(L.vdtim);
+  A.dtim = L.dtim;
+  A.cellvolume = (T)L.cellvolume;
+  A.blend = (T)(1.0 - P.cdis[m - 1]);
+  A.fn = (T)(P.cstp[m - 1] * P.cfl);
+  A.m = m;
+  A.has_diss = P.cdis[m - 1] > 0.0;
+  A.coarse = mgl < mlvl;
+  A.lfir1 = lfir && m == 1;
+  A.lres = lres;
+  A.smoothed = (m % 2 != 0) && P.smoo > 0.0;
+  A.store_fw = (m < P.mstage) && (A.has_diss || m == 1 || A.blend != T(1));
+  A.vt = P.vt;
+  A.order = P.distype == 2 ? 2 : 1;
+  A.eps = (T)P.epsentr;
+  A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
+  return A;
+}
+
+// k planes per workgroup of the LDS-tiled stage kernels: the level's
+// autotuned value when set, else halve from nk while the launch has fewer
+// than 2048 workgroups (enough to fill 256 CUs)
+int stage_kc(const Level &L) {
+  const Dims &d = L.d;
+  if (L.kc > 0) return std::min(L.kc, d.nk);
+  int kc = d.nk;
+  long tiles = (long)((d.ni + TI - 1) / TI) * ((d.nj + TJ - 1) / TJ);
+  while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
+  return kc;
+}
+
 template <class T>
 int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir) {
   const mge_params &P = h->prm;
@@ -463,42 +526,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   for (int m = 1; m <= P.mstage; m++) {
     int dst = 0;
     while (dst == w0 || dst == src) dst++;
-    StageArgs<T> A;
-    A.fx.d = d;
-    A.fx.s = stc<T>(L, src);
-    A.fx.g = geo<T>(L);
-    A.fx.radsum = tp<T>(L.radsum);
-    A.fx.gamma = (T)P.gamma;
-    A.fx.vis2 = (T)P.vis2;
-    A.fx.vis4 = (T)P.vis4;
-    A.fx.cdis = (T)P.cdis[m - 1];
-    A.fx.dissip = P.dissip;
-    A.fx.inf = make_inf<T>(P);
-    A.wdst = tp<T>(L.S[dst]);
-    A.pdst = tp<T>(L.S[dst]) + 5 * E;
-    A.wn = tp<T>(L.S[w0]);
-    A.dw = dwp;
-    A.fw = tp<T>(L.fw);
-    A.qf = tp<T>(L.qf);
-    A.wr = tp<T>(L.wr);
-    A.marker = L.marker;
-    A.flags = L.flags;
-    A.vdtim = tp<T>(L.vdtim);
-    A.dtim = L.dtim;
-    A.cellvolume = (T)L.cellvolume;
-    A.blend = (T)(1.0 - P.cdis[m - 1]);
-    A.fn = (T)(P.cstp[m - 1] * P.cfl);
-    A.m = m;
-    A.has_diss = P.cdis[m - 1] > 0.0;
-    A.coarse = mgl < mlvl;
-    A.lfir1 = lfir && m == 1;
-    A.lres = lres;
-    A.smoothed = (m % 2 != 0) && P.smoo > 0.0;
-    A.store_fw = (m < P.mstage) && (A.has_diss || m == 1 || A.blend != T(1));
-    A.vt = P.vt;
-    A.order = P.distype == 2 ? 2 : 1;
-    A.eps = (T)P.epsentr;
-    A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
+    StageArgs<T> A = stage_args<T>(h, L, m, src, dst, w0, mgl, mlvl, lres, lfir);
     // laminar viscous fluxes (euler.f90:137-181): fw = blend fw + cdis V
     // before the stage kernel adds its dissipation
     A.fw_in = P.eqntype == 1 && A.has_diss;
@@ -520,10 +548,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       h->next_bytes = per * (double)d.ni * d.nj * d.nk;
     }
     {
-      int kc = d.nk;
-      long tiles = (long)((d.ni + TI - 1) / TI) * ((d.nj + TJ - 1) / TJ);
-      // enough workgroups to fill 256 CUs: split k into chunks
-      while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
+      const int kc = stage_kc(L);
       dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
       A.kc = kc;
       if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, TJ), A);
@@ -772,6 +797,96 @@ int do_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, l
   CHKRC(check_err(h));
   if (nlog) *nlog = nl;
   if (cs_out) *cs_out = cs;
+  return MGE_OK;
+}
+
+// ---- autotuner for the LDS-tiled stage kernels ---------------------------
+//
+// The one launch parameter of k_stage2/k_stage_roe that changes neither the
+// arithmetic nor the results is kc, the number of k planes a workgroup
+// marches through (the column carry restarts at every chunk): few planes
+// give many workgroups, many planes amortise the chunk start.  The tuner
+// times stage 1 of the finest-level configuration for kc = 1, 2, 4, ... nk
+// (one warm-up launch, then the best of three) on the level's own arrays,
+// writing only a state buffer that is not current, dw, fw and the time-step
+// arrays (all rewritten by the next timestep/euler call).  Results are
+// cached per (shape, precision, scheme) for the process.
+struct TuneKey {
+  int ni, nj, nk, fp32, distype, smoothed;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(ni, nj, nk, fp32, distype, smoothed) < std::tie(o.ni, o.nj, o.nk, o.fp32, o.distype, o.smoothed);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, double>> g_tune;
+
+template <class T>
+int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
+  const mge_params &P = h->prm;
+  const Dims &d = L.d;
+  if (h->rec) return set_err(MGE_ERR_STATE, "autotune while a batch is being recorded");
+  if (P.distype == 0 && P.dissip != 0) {  // matrix dissipation: one thread per cell, nothing to tune
+    L.kc = 0;
+    if (kc_out) *kc_out = 0;
+    if (us_out) *us_out = 0.0;
+    return MGE_OK;
+  }
+  TuneKey key{d.ni, d.nj, d.nk, h->fp32 ? 1 : 0, P.distype, P.smoo > 0.0 ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      L.kc = it->second.first;
+      if (kc_out) *kc_out = L.kc;
+      if (us_out) *us_out = it->second.second;
+      return MGE_OK;
+    }
+  }
+  CHKRC(do_timestep<T>(h, L));
+  const int w0 = L.cur;
+  int dst = 0;
+  while (dst == w0) dst++;
+  StageArgs<T> A = stage_args<T>(h, L, 1, w0, dst, w0, 1, 1, 0, 0);
+  A.fw_in = 0;
+  const void *fn = P.distype != 0 ? (const void *)&k_stage_roe<T> : (const void *)&k_stage2<T>;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = 0;
+  double best_us = 1e300;
+  hipError_t err = hipSuccess;
+  for (int kc = 1;; kc = std::min(2 * kc, d.nk)) {
+    A.kc = kc;
+    dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
+    void *args[1] = {(void *)&A};
+    double t = 1e300;
+    for (int rep = 0; rep < 4 && err == hipSuccess; rep++) {
+      err = hipEventRecord(e0, h->stream);
+      if (err == hipSuccess) err = hipLaunchKernel(fn, g, dim3(TI, TJ), args, 0, h->stream);
+      if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0 && err == hipSuccess) t = std::min(t, 1e3 * (double)ms);
+    }
+    if (err != hipSuccess) break;
+    if (t < best_us) {
+      best_us = t;
+      best = kc;
+    }
+    if (kc == d.nk) break;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (err != hipSuccess) return set_err(MGE_ERR_HIP, hipGetErrorString(err));
+  CHKRC(check_err(h));
+  L.kc = best;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = {best, best_us};
+  }
+  if (kc_out) *kc_out = best;
+  if (us_out) *us_out = best_us;
   return MGE_OK;
 }
 
@@ -1739,6 +1854,19 @@ long long mge_case_bytes(mge_case *h) {
 int mge_synchronize(mge_case *h) {
   if (!h) return set_err(MGE_ERR_ARG, "null handle");
   return check_err(h);
+}
+
+int mge_autotune(mge_case *h, int lvl, int *kc, double *us) {
+  Level *L = get_level(h, lvl);
+  if (!L) return set_err(MGE_ERR_ARG, "bad level");
+  return by_type(h, [&](auto z) { return do_autotune<decltype(z)>(h, *L, kc, us); });
+}
+
+int mge_set_kc(mge_case *h, int lvl, int kc) {
+  Level *L = get_level(h, lvl);
+  if (!L || kc < 0) return set_err(MGE_ERR_ARG, "bad level or kc");
+  L->kc = kc;
+  return MGE_OK;
 }
 
 int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count,

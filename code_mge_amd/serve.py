@@ -79,7 +79,7 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False):
 
 class Server:
     def __init__(self, hbm_budget: float, drop: float = 1e5, maxcyc: int = 3000, precision: int = 32,
-                 use_graph: bool = True, periods: int = 4, log=None):
+                 use_graph: bool = True, periods: int = 4, log=None, autotune: bool = True):
         self.budget = float(hbm_budget)
         self.drop = drop
         self.maxcyc = maxcyc
@@ -87,6 +87,7 @@ class Server:
         self.use_graph = use_graph
         self.periods = periods
         self.log = log
+        self.autotune = autotune
         self.batches: Dict[tuple, Batch] = {}
         self.used = 0
 
@@ -127,6 +128,8 @@ class Server:
                 prm = dataclasses.replace(req.params, precision=self.precision)
                 case = Case(prm, req.levels, geoms=geoms, mxcyc=4)
                 case.init_flow()
+                if self.autotune:
+                    case.autotune()   # cached per level shape after the first request
                 got = case.device_bytes()
                 key = self._class(prm, req.levels)
                 b = self.batches.get(key)

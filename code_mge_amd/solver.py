@@ -139,6 +139,20 @@ class Case:
     def device_bytes(self):
         return int(self.lib.mge_case_bytes(self.h))
 
+    def autotune(self, levels=None):
+        """pick the stage kernels' k-chunk length per level by timing
+        (mge_autotune); returns {level: (kc, best launch us)}"""
+        out = {}
+        for lvl in (levels or range(1, len(self.levels) + 1)):
+            kc = C.c_int(0)
+            us = C.c_double(0.0)
+            _ffi.check(self.lib.mge_autotune(self.h, lvl, C.byref(kc), C.byref(us)), "mge_autotune")
+            out[lvl] = (kc.value, us.value)
+        return out
+
+    def set_kc(self, lvl, kc):
+        _ffi.check(self.lib.mge_set_kc(self.h, lvl, kc), "mge_set_kc")
+
     def sync(self):
         return _ffi.check(self.lib.mge_synchronize(self.h), "mge_synchronize")
 

@@ -171,6 +171,18 @@ int mge_synchronize(mge_case *h);
  * the measurement hook of the benchmark.) */
 int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, long long *count,
                       double *bytes);
+
+/* Autotuner of the LDS-tiled stage kernels (no counterpart in the
+ * reference): times stage 1 of level lvl for every k-chunk length kc = 1,
+ * 2, 4, ... nk and keeps the fastest for the level; results are cached per
+ * (shape, precision, scheme) for the process.  The chunk length changes no
+ * arithmetic.  Overwrites only scratch state (a non-current state buffer,
+ * dw, fw, the time-step arrays); call it between solves, not while a batch
+ * is recording.  *kc (0 for the matrix-dissipation kernel, which has no
+ * chunks) and *us (best launch time, microseconds) may be NULL. */
+int mge_autotune(mge_case *h, int lvl, int *kc, double *us);
+/* force the k-chunk length of level lvl (0: the built-in fill rule) */
+int mge_set_kc(mge_case *h, int lvl, int kc);
 const char *mge_last_error(void);
 
 /* ---- batches of cases --------------------------------------------------
