@@ -45,6 +45,27 @@ class MgeParams(C.Structure):
     ]
 
 
+class MgePinGrid(C.Structure):
+    _fields_ = [("name", C.c_int), ("points", C.c_int * 3), ("origin", C.c_double * 3), ("length", C.c_double * 3)]
+
+
+class MgeJob(C.Structure):
+    _fields_ = [
+        ("job", C.c_char * 64), ("model", C.c_char * 64), ("mesh", C.c_char * 512), ("surface", C.c_char * 512),
+        ("nlevel", C.c_int), ("mcyc", C.c_int * 16),
+        ("mach", C.c_double), ("reynolds", C.c_double), ("gamma", C.c_double), ("prlmean", C.c_double),
+        ("alphad", C.c_double), ("betad", C.c_double),
+        ("cflf", C.c_double), ("vis2", C.c_double), ("vis4", C.c_double), ("smoo", C.c_double),
+        ("epsentr", C.c_double),
+        ("mgtype", C.c_int), ("mstage", C.c_int),
+        ("cstp", C.c_double * 8), ("cdis", C.c_double * 8),
+        ("eqntype", C.c_int), ("distype", C.c_int), ("dissip", C.c_int),
+        ("n", C.c_int * 3),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("dom_lo", C.c_double * 3), ("dom_hi", C.c_double * 3),
+        ("bctype", C.c_int * 6),
+    ]
+
+
 class MgeCutRec(C.Structure):
     _fields_ = [
         ("faceid", C.c_int), ("zz", C.c_int),
@@ -87,6 +108,8 @@ def lib():
     L.mge_last_error.restype = C.c_char_p
     L.mge_geom_level.argtypes = [C.POINTER(MgeBody), I, I, I, dp, dp, dp, C.c_void_p,
                                  C.POINTER(MgeCutRec), I, C.POINTER(I)]
+    L.mge_geom_level_tri.argtypes = [dp, I, I, I, I, dp, dp, dp, C.c_void_p, C.POINTER(MgeCutRec), I,
+                                     C.POINTER(I)]
     L.mge_case_create.argtypes = [C.POINTER(MgeParams), I, C.POINTER(P)]
     L.mge_case_destroy.argtypes = [P]
     L.mge_level_define.argtypes = [P, I, I, I, I, dp, dp, dp, D, C.POINTER(I), C.POINTER(I),
@@ -112,6 +135,12 @@ def lib():
     L.mge_case_bytes.argtypes = [P]
     L.mge_case_bytes.restype = C.c_longlong
     L.mge_autotune.argtypes = [P, I, C.POINTER(I), dp]
+    L.mge_read_job.argtypes = [C.c_char_p, C.POINTER(MgeJob)]
+    L.mge_job_params.argtypes = [C.POINTER(MgeJob), I, C.POINTER(MgeParams)]
+    L.mge_job_level.argtypes = [C.POINTER(MgeJob), I, C.POINTER(I), C.POINTER(I)]
+    L.mge_read_pin.argtypes = [C.c_char_p, I, C.POINTER(MgePinGrid), C.POINTER(I)]
+    L.mge_read_p3d.argtypes = [C.c_char_p, I, dp, C.POINTER(I)]
+    L.mge_input_error.restype = C.c_char_p
     L.mge_set_kc.argtypes = [P, I, I]
     L.mge_batch_create.argtypes = [I, I, C.POINTER(P)]
     L.mge_batch_destroy.argtypes = [P]
@@ -139,5 +168,6 @@ EXPORTED = [
     "mge_upload", "mge_get_cuts", "mge_get_dtim", "mge_wall_force", "mge_synchronize",
     "mge_kernel_timing", "mge_last_error", "mge_cycles",
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
-    "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc",
+    "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc", "mge_geom_level_tri",
+    "mge_read_job", "mge_job_params", "mge_job_level", "mge_read_pin", "mge_read_p3d", "mge_input_error",
 ]

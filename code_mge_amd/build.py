@@ -29,7 +29,8 @@ def _run(cmd, cwd=None):
 
 def build_library(verbose=True):
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, "mge_device.hip"), os.path.join(CSRC, "mge_geom.cpp")]
+    srcs = [os.path.join(CSRC, "mge_device.hip"), os.path.join(CSRC, "mge_geom.cpp"),
+            os.path.join(CSRC, "mge_input.cpp")]
     tmp = LIB + ".tmp"
     _run(["hipcc", *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs])
     os.replace(tmp, LIB)

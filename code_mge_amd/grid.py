@@ -84,6 +84,19 @@ def body(kind, center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2, rot=None):
     return b
 
 
+class TriBody:
+    """a closed triangulated surface: tri is (ntri, 9) float64, the three
+    vertices of every triangle (e.g. the quads of PLOT3D surface patches,
+    each split in two)"""
+
+    def __init__(self, tri):
+        self.tri = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(-1, 9))
+
+    @property
+    def ntri(self):
+        return self.tri.shape[0]
+
+
 @dataclass
 class LevelGeom:
     spec: LevelSpec
@@ -95,18 +108,28 @@ class LevelGeom:
     nmir: int
 
 
-def level_geometry(spec: LevelSpec, bdy: Optional[_ffi.MgeBody]) -> LevelGeom:
-    """Run the geometry preprocessor (host code of libmge) for one level."""
+def level_geometry(spec: LevelSpec, bdy) -> LevelGeom:
+    """Run the geometry preprocessor (host code of libmge) for one level;
+    bdy is an analytic body (MgeBody), a TriBody or None."""
     L = _ffi.lib()
     xv, yv, zv = spec.vertices()
     ni, nj, nk = spec.n
     blank = np.ones((nk, nj, ni), dtype=np.uint8)
     nm = C.c_int(0)
     dp = C.POINTER(C.c_double)
-    bptr = C.byref(bdy) if bdy is not None else None
     args = (ni, nj, nk, xv.ctypes.data_as(dp), yv.ctypes.data_as(dp), zv.ctypes.data_as(dp),
             blank.ctypes.data_as(C.c_void_p))
-    _ffi.check(L.mge_geom_level(bptr, *args, None, 0, C.byref(nm)), "mge_geom_level")
+    if isinstance(bdy, TriBody):
+        def call(recs, cap):
+            return L.mge_geom_level_tri(bdy.tri.ctypes.data_as(dp), bdy.ntri, *args, recs, cap, C.byref(nm))
+        what = "mge_geom_level_tri"
+    else:
+        bptr = C.byref(bdy) if bdy is not None else None
+
+        def call(recs, cap):
+            return L.mge_geom_level(bptr, *args, recs, cap, C.byref(nm))
+        what = "mge_geom_level"
+    _ffi.check(call(None, 0), what)
     recs = (_ffi.MgeCutRec * max(nm.value, 1))()
-    _ffi.check(L.mge_geom_level(bptr, *args, recs, nm.value, C.byref(nm)), "mge_geom_level")
+    _ffi.check(call(recs, nm.value), what)
     return LevelGeom(spec, xv, yv, zv, blank, recs, nm.value)

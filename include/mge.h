@@ -112,6 +112,47 @@ typedef struct {
 
 typedef struct mge_case mge_case;
 
+/* ---- input files (the role of make_fsijob's readers, srcf90/main.f90:44) */
+/* one grid of a .pin file: vertex counts, origin, extent */
+typedef struct {
+  int    name;
+  int    points[3];
+  double origin[3], length[3];
+} mge_pin_grid;
+
+/* what a jobsheet (with its .pin and .p3d) defines for the solve step */
+typedef struct {
+  char   job[64], model[64];
+  char   mesh[512], surface[512];   /* paths of the .pin and .p3d files       */
+  int    nlevel;                    /* entries of mcyc                        */
+  int    mcyc[16];                  /* cycles per level of the FMG ramp       */
+  double mach, reynolds, gamma, prlmean, alphad, betad;
+  double cflf, vis2, vis4, smoo, epsentr;
+  int    mgtype, mstage;
+  double cstp[8], cdis[8];
+  int    eqntype, distype, dissip;
+  int    n[3];                      /* finest-level cells                     */
+  double lo[3], hi[3];              /* block extent                           */
+  double dom_lo[3], dom_hi[3];      /* DOMAIN bounds                          */
+  int    bctype[6];                 /* faces i-lo, j-lo, k-lo, i-hi, j-hi, k-hi */
+} mge_job;
+
+/* Read a jobsheet and the single-block .pin it names (make_fsijob's job
+ * and grid input; the key mapping is this library's, ctrlib being absent
+ * from the reference tree).  Paths in the jobsheet are relative to it. */
+int mge_read_job(const char *jobsheet, mge_job *job);
+/* the scheme parameters the job defines, derived as input_param.f90:13-115 */
+int mge_job_params(const mge_job *job, int precision, mge_params *prm);
+/* cells and coarsening ratio (to lvl-1) of level lvl, 1 = coarsest */
+int mge_job_level(const mge_job *job, int lvl, int n[3], int ratio[3]);
+/* grids of a .pin file (cap entries written, *ngrid = count) */
+int mge_read_pin(const char *path, int cap, mge_pin_grid *grids, int *ngrid);
+/* triangles (9 values each) of the surface patches of a PLOT3D .p3d file
+ * (cap triangles written, *ntri = count) */
+int mge_read_p3d(const char *path, int cap, double *tri, int *ntri);
+/* message of the last failed input call */
+const char *mge_input_error(void);
+
 /* ---- geometry preprocessor (host only, no GPU needed) ------------------ */
 /* Classifies the interior cells of one block (cell centre inside the body =
  * solid) and builds one record per (fluid, solid) face neighbour with the
@@ -122,6 +163,14 @@ typedef struct mge_case mge_case;
 int mge_geom_level(const mge_body *body, int ni, int nj, int nk,
                    const double *xv, const double *yv, const double *zv,
                    uint8_t *blank, mge_cutrec *recs, int cap, int *nmir);
+/* The same for a closed triangulated surface (tri: 9 values per triangle,
+ * the three vertices; e.g. the quads of the PLOT3D surface patches that
+ * make_fsijob reads, srcf90/main.f90:44, split in two): a cell centre is
+ * inside when the grid line through it crosses the surface an odd number
+ * of times before it; the record normal is the crossed triangle's. */
+int mge_geom_level_tri(const double *tri, int ntri, int ni, int nj, int nk,
+                       const double *xv, const double *yv, const double *zv,
+                       uint8_t *blank, mge_cutrec *recs, int cap, int *nmir);
 
 /* ---- case lifetime (make_flow.f90 / init_metric.f90) -------------------- */
 int mge_case_create(const mge_params *prm, int nlevel, mge_case **out);

@@ -1,0 +1,64 @@
+"""The Fortran driver (code_mge_amd/fortran/mgsol, iso_c_binding over
+libmge) on the reference's cylinder input files: it reads the jobsheet,
+.pin and .p3d, builds the cut-cell levels and runs solver.f90's FMG/V-cycle
+control through per-block mge_timestep/mge_euler/mge_movco/mge_movfin
+calls.  Its residual log and wall force must equal, bit for bit, the
+Python host running the same job through mge_solve (the library's own
+control) -- the binding, the struct layouts and the restated control are
+all exercised.  The job is the reference's with fewer cycles per level."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from code_mge_amd.grid import level_geometry
+from code_mge_amd.inputs import load_job
+from code_mge_amd.solver import Case
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "cylinder")
+EXE = os.path.join(os.path.dirname(HERE), "code_mge_amd", "fortran", "mgsol")
+MCYC = "2:2:2:3:4"
+
+
+def _job_dir(tmp_path):
+    for f in ("cylinder05_1.pin", "cylinder.p3d"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / f)
+    txt = open(os.path.join(GOLD, "jobsheet")).read()
+    txt = re.sub(r"(?m)^(\s*mcyc\s*=\s*)100:100:120:150:180", r"\g<1>" + MCYC, txt)
+    assert MCYC in txt
+    (tmp_path / "jobsheet").write_text(txt)
+    return tmp_path
+
+
+def test_fortran_driver_equals_library_solve(require_gpu, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (code_mge_amd/build.py fortran)")
+    d = _job_dir(tmp_path)
+    print("running the Fortran driver", flush=True)
+    p = subprocess.run([EXE, "jobsheet", "64"], cwd=d, capture_output=True, text=True, timeout=600)
+    print(p.stdout[-3000:], p.stderr[-2000:])
+    assert p.returncode == 0
+    rows = [l.split() for l in open(d / "cylinder_ibm_re20_test_res_full.dat") if not l.startswith(" force")]
+    flog = np.array([[float(x) for x in r] for r in rows])
+    force_line = [l for l in open(d / "cylinder_ibm_re20_test_res_full.dat") if l.startswith(" force")][0]
+    ff = np.array([float(x) for x in force_line.split()[1:]])
+    # the reference's fixed-width log next to it
+    assert len(open(d / "cylinder_ibm_re20_test_res.dat").read().splitlines()) == len(flog)
+
+    job = load_job(str(d / "jobsheet"))
+    geoms = [level_geometry(s, job.body) for s in job.levels]
+    with Case(job.params, job.levels, geoms=geoms, mxcyc=max(job.mcyc) + 2) as c:
+        c.init_flow()
+        log, _ = c.solve(job.mcyc)
+        fp = c.wall_force()
+    print("Fortran log", flog[-3:], "library log", log[-3:])
+    print("Fortran force", ff, "library force", fp)
+    assert flog.shape == log.shape
+    assert np.array_equal(flog, log)
+    assert np.array_equal(ff, fp)
