@@ -199,6 +199,64 @@ def traffic_probe(args, precision):
     print(f"[bench] traffic probe FP{precision} done", file=sys.stderr, flush=True)
 
 
+def ns_leg(cycles):
+    """Laminar Navier-Stokes cylinder (Re = 40 on the diameter, M = 0.1,
+    Pr = 0.72, constant viscosity) on a uniform 512 x 512 x 4 grid over
+    [-20, 20]^2, 4-level W-cycle, JST + viscous fluxes, no smoothing:
+    `cycles` W-cycles of the plain FP64 path and of the FP32 bulk path from
+    freestream; throughput, residual drop and Cd/Cl of each, and the FP32
+    error against FP64."""
+    import dataclasses
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    from code_mge_amd.solver import Case, force_coefficients
+    prm, levels, bdy, aref = configs.cylinder2d(n=512, nz=4, nlevel=4, mach=0.1, half=20.0, eqntype="n",
+                                                reynolds=40.0, prlmean=0.72, smoo=0.0, cfl=-2.0)
+    geoms = [level_geometry(s, bdy) for s in levels]
+    out = {"workload": "NS cylinder Re=40 M=0.1 Pr=0.72, 512x512x4 on [-20,20]^2 (uniform), 4-level W-cycle, "
+                       "JST + laminar viscous fluxes", "cycles": cycles}
+    for prec in (64, 32):
+        with Case(dataclasses.replace(prm, precision=prec), levels, geoms=geoms, mxcyc=cycles + 2) as c:
+            c.init_flow()
+            c.autotune()
+            r0, _ = c.cycles(len(levels), 1, ncyc0=1)
+            c.sync()
+            t0 = time.perf_counter()
+            r1, cs = c.cycles(len(levels), cycles - 1, ncyc0=2)
+            dt = time.perf_counter() - t0
+            cd, cl = force_coefficients(c, aref)
+        out[f"fp{prec}"] = {"mcell_stage_per_s": round(cs / dt / 1e6, 1), "s": round(dt, 2),
+                            "residual_drop": r0 / r1, "cd": cd, "cl": cl}
+        print(f"[bench] NS cylinder FP{prec}: {cs / dt / 1e6:.0f} Mcs/s, drop {r0 / r1:.3e}, Cd {cd:.6f}",
+              file=sys.stderr, flush=True)
+    out["cd_err_fp32"] = abs(out["fp32"]["cd"] - out["fp64"]["cd"])
+    out["cl_err_fp32"] = abs(out["fp32"]["cl"] - out["fp64"]["cl"])
+    return out
+
+
+def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000):
+    """Poisson serving stream (PCG64(20261203), `rate` requests/s, half of
+    them laminar Navier-Stokes): ellipsoids, grids 32^3..192^3, 3 levels,
+    FP32 served under an HBM budget to a residual drop of `drop`;
+    p50/p99 time-to-convergence (completion - arrival) and solves/h."""
+    from code_mge_amd.serve import Server, percentiles, poisson_requests
+    reqs = poisson_requests(count=count, rate=rate)
+    srv = Server(hbm_budget=budget_gb * 1e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4,
+                 log=lambda m: print(m, file=sys.stderr, flush=True))
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    p50, p99 = percentiles(res)
+    conv = sum(1 for r in res if r.status == 1)
+    ns = [r for r in res if reqs[r.rid].info["ns"]]
+    return {"workload": f"{count} Poisson requests at {rate}/s (PCG64(20261203)), Euler/NS ellipsoids "
+                        f"32^3..192^3, 3 levels, FP32, drop {drop:g}, HBM budget {budget_gb:g} GB",
+            "wall_s": round(wall, 2), "converged": conv, "requests": count, "ns_requests": len(ns),
+            "solves_per_h": round(conv / wall * 3600.0, 1), "ttc_p50_s": round(p50, 3), "ttc_p99_s": round(p99, 3),
+            "last_arrival_s": round(reqs[-1].arrival, 2),
+            "cycles_max": max(r.ncyc for r in res)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +269,8 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=2, help="cycles of the HIP-event roofline pass")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
+    ap.add_argument("--ns-leg", type=int, default=0, help="W-cycles of the NS cylinder leg (0: skip)")
+    ap.add_argument("--serve-leg", type=int, default=0, help="requests of the Poisson serving leg (0: skip)")
     ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
                     help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
     args = ap.parse_args()
@@ -243,6 +303,8 @@ def main():
     cd, cl, roofline = main_leg["cd"], main_leg["cl"], main_leg["roofline"]
 
     solves = sweep_leg(args.sweep) if args.sweep > 0 else None
+    ns = ns_leg(args.ns_leg) if args.ns_leg > 0 else None
+    served = serve_leg(args.serve_leg) if args.serve_leg > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -273,6 +335,10 @@ def main():
         "fp32": fp32,
         "solves": solves,
     }
+    if ns is not None:
+        out["ns_cylinder"] = ns
+    if served is not None:
+        out["serving"] = served
     print(json.dumps(out))
 
 

@@ -18,10 +18,10 @@ from .params import Params
 FAR, SYM, PER, WALL = -20, -50, -60, -80
 
 
-def cylinder2d(n=128, nz=4, nlevel=3, mach=0.2, alpha=0.0, **kw):
-    h = 20.0 / n
-    lo = (-10.0, -10.0, 0.0)
-    hi = (10.0, 10.0, nz * h)
+def cylinder2d(n=128, nz=4, nlevel=3, mach=0.2, alpha=0.0, half=10.0, **kw):
+    h = 2.0 * half / n
+    lo = (-half, -half, 0.0)
+    hi = (half, half, nz * h)
     levels = hierarchy((n, n, nz), lo, hi, nlevel, bctype=(FAR, FAR, PER, FAR, FAR, PER))
     prm = Params(mach=mach, alpha_deg=alpha, **kw)
     bdy = body("cylinder", center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2)

@@ -105,20 +105,26 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
       A.vdtim[q] = T(0);
     }
   }
-  // block minimum of the time step -> one atomic per block on the FP64
-  // bit pattern (positive doubles order like their bits)
-  __shared__ T sm[256];
-  int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  sm[tid] = mine;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s && sm[tid + s] < sm[tid]) sm[tid] = sm[tid + s];
-    __syncthreads();
+  // block minimum of the time step (wavefront shuffles, then the block's
+  // waves) -> one atomic per block on the FP64 bit pattern (positive
+  // doubles order like their bits), skipped when the block cannot lower the
+  // value already there, so the single address sees few atomics
+  T m = mine;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    T o = __shfl_xor(m, off, 64);
+    m = o < m ? o : m;
   }
+  __shared__ T wmin[16];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+  if ((tid & 63) == 0) wmin[tid >> 6] = m;
+  __syncthreads();
   if (tid == 0) {
-    double v = (double)sm[0];
-    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-    atomicMin(A.dtim_bits, bits);
+    T b = wmin[0];
+    for (int w = 1; w < nw; w++) b = wmin[w] < b ? wmin[w] : b;
+    unsigned long long bits = (unsigned long long)__double_as_longlong((double)b);
+    if (bits < __atomic_load_n(A.dtim_bits, __ATOMIC_RELAXED)) atomicMin(A.dtim_bits, bits);
   }
 }
 MGE_ENTRY_T(k_timestep, TsArgs, 256, 1)
@@ -419,9 +425,33 @@ __device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, const Inf<T
   s.p[qt] = eos_st(gamma, f, v);
 }
 
-// bc_corner.f90:23-157: four ordered phases; within a phase every (edge,
-// index) item is independent, so one 1024-thread workgroup runs each phase
-// in parallel and synchronises between phases
+// bc_corner.f90:23-157.  Every edge cell of the first ghost layer is
+// t = a + b - c of three cells that no edge writes (face ghosts and interior
+// cells; the reference's third i-edge line reads its own target and adds and
+// subtracts the same value, :120): edges along k (:23-61), along j (:63-99)
+// and along i (:101-137).  The i-edge cells at i = 1 and i = ie are the
+// eight corners, which :141-157 overwrite without reading them; each corner
+// reads three edge cells, which its thread evaluates itself with the same
+// expression.  So every item is independent: one launch over all of them.
+__device__ inline void edge_cells(const Dims &d, int ph, int e, int x, long *q) {
+  const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke;
+  int c[12];
+  if (ph == 1) {  // edges along k, x = k
+    const int T[4][12] = {{1, 1, 0, 2, 1, 0, 1, 2, 0, 2, 2, 0}, {ie, 1, 0, il, 1, 0, ie, 2, 0, il, 2, 0},
+                          {1, je, 0, 2, je, 0, 1, jl, 0, 2, jl, 0}, {ie, je, 0, il, je, 0, ie, jl, 0, il, jl, 0}};
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 2) ? x : T[e][m];
+  } else if (ph == 2) {  // edges along j, x = j
+    const int T[4][12] = {{1, 0, 1, 2, 0, 1, 1, 0, 2, 2, 0, 2}, {ie, 0, 1, il, 0, 1, ie, 0, 2, il, 0, 2},
+                          {1, 0, ke, 2, 0, ke, 1, 0, kl, 2, 0, kl}, {ie, 0, ke, il, 0, ke, ie, 0, kl, il, 0, kl}};
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 1) ? x : T[e][m];
+  } else {  // edges along i, x = i
+    const int T[4][12] = {{0, 1, 1, 0, 1, 2, 0, 2, 1, 0, 2, 2}, {0, 1, ke, 0, 2, ke, 0, 1, kl, 0, 2, kl},
+                          {0, je, 1, 0, je, 1, 0, jl, 2, 0, jl, 2}, {0, je, ke, 0, jl, ke, 0, je, kl, 0, jl, kl}};
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 0) ? x : T[e][m];
+  }
+  for (int m = 0; m < 4; m++) q[m] = ix(d, c[3 * m], c[3 * m + 1], c[3 * m + 2]);
+}
+
 template <typename T>
 struct CornerArgs {
   Dims d;
@@ -431,68 +461,72 @@ struct CornerArgs {
 };
 
 template <typename T>
-__device__ __attribute__((always_inline)) inline void k_bc_corner_body(const CornerArgs<T> &C, int, int, int) {
+__device__ __attribute__((always_inline)) inline void k_bc_corner_body(const CornerArgs<T> &C, int bx, int, int) {
   const Dims &d = C.d;
   const State<T> s = C.s;
-  const T gamma = C.gamma;
   const long E = d.E;
   T *W = s.w;
   const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke, ib = d.ib;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  // phase 1: edges along k (bc_corner.f90:23-61)
-  for (int t = tid; t < 4 * (kl - 1); t += nt) {
-    int e = t & 3, k = 2 + (t >> 2);
-    if (e == 0) edge_fill(d, s, gamma, C.inf, 1, 1, k, 2, 1, k, 1, 2, k, 2, 2, k);
-    else if (e == 1) edge_fill(d, s, gamma, C.inf, ie, 1, k, il, 1, k, ie, 2, k, il, 2, k);
-    else if (e == 2) edge_fill(d, s, gamma, C.inf, 1, je, k, 2, je, k, 1, jl, k, 2, jl, k);
-    else edge_fill(d, s, gamma, C.inf, ie, je, k, il, je, k, ie, jl, k, il, jl, k);
+  const int n1 = 4 * (kl - 1), n2 = 4 * (jl - 1), n3 = 4 * (ib + 1);
+  const int t = bx * blockDim.x + threadIdx.x;
+  T v[5];
+  long qt;
+  if (t < n1 + n2 + n3) {
+    int ph, u;
+    if (t < n1) {
+      ph = 1;
+      u = t;
+    } else if (t < n1 + n2) {
+      ph = 2;
+      u = t - n1;
+    } else {
+      ph = 3;
+      u = t - n1 - n2;
+    }
+    const int e = u & 3, x = ph == 3 ? (u >> 2) : 2 + (u >> 2);
+    if (ph == 3 && (x == 1 || x == ie)) return;  // corners, overwritten below
+    // the i-edge line of :120 reads its own target; the two of them next to
+    // a corner are that corner's input and are written by its thread
+    if (ph == 3 && e == 2 && (x == 2 || x == il)) return;
+    long q[4];
+    edge_cells(d, ph, e, x, q);
+    for (int n = 0; n < 5; n++) {
+      v[n] = W[n * E + q[1]] + W[n * E + q[2]] - W[n * E + q[3]];
+      W[n * E + q[0]] = v[n];
+    }
+    qt = q[0];
+  } else if (t < n1 + n2 + n3 + 8) {
+    const int c = t - (n1 + n2 + n3);
+    const bool hi = c & 1, hj = (c & 2) != 0, hk = (c & 4) != 0;
+    const int ci = hi ? ie : 1, cj = hj ? je : 1, ck = hk ? ke : 1;
+    const int ai = hi ? il : 2, aj = hj ? jl : 2, ak = hk ? kl : 2;
+    long qi[4], qj[4], qk[4];
+    edge_cells(d, 3, (hk ? 1 : 0) + (hj ? 2 : 0), ai, qi);  // (ai, cj, ck)
+    edge_cells(d, 2, (hi ? 1 : 0) + (hk ? 2 : 0), aj, qj);  // (ci, aj, ck)
+    edge_cells(d, 1, (hi ? 1 : 0) + (hj ? 2 : 0), ak, qk);  // (ci, cj, ak)
+    const T two = T(2), three = T(3);
+    const long qa = ix(d, ai, aj, ak);
+    const bool own = hj && !hk;  // the i-edge neighbour is a self-reading line of :120
+    T ve[5];
+    for (int n = 0; n < 5; n++) {
+      T ei = W[n * E + qi[1]] + W[n * E + qi[2]] - W[n * E + qi[3]];
+      T ej = W[n * E + qj[1]] + W[n * E + qj[2]] - W[n * E + qj[3]];
+      T ek = W[n * E + qk[1]] + W[n * E + qk[2]] - W[n * E + qk[3]];
+      v[n] = (ei + ej + ek) * two / three - W[n * E + qa];
+      ve[n] = ei;
+    }
+    if (own) {
+      for (int n = 0; n < 5; n++) W[n * E + qi[0]] = ve[n];
+      s.p[qi[0]] = eos_st(C.gamma, C.inf, ve);
+    }
+    qt = ix(d, ci, cj, ck);
+    for (int n = 0; n < 5; n++) W[n * E + qt] = v[n];
+  } else {
+    return;
   }
-  __syncthreads();
-  // phase 2: edges along j (:63-99)
-  for (int t = tid; t < 4 * (jl - 1); t += nt) {
-    int e = t & 3, j = 2 + (t >> 2);
-    if (e == 0) edge_fill(d, s, gamma, C.inf, 1, j, 1, 2, j, 1, 1, j, 2, 2, j, 2);
-    else if (e == 1) edge_fill(d, s, gamma, C.inf, ie, j, 1, il, j, 1, ie, j, 2, il, j, 2);
-    else if (e == 2) edge_fill(d, s, gamma, C.inf, 1, j, ke, 2, j, ke, 1, j, kl, 2, j, kl);
-    else edge_fill(d, s, gamma, C.inf, ie, j, ke, il, j, ke, ie, j, kl, il, j, kl);
-  }
-  __syncthreads();
-  // phase 3: edges along i including the ghost i (:101-137); the third
-  // line adds and subtracts the same value (:120)
-  for (int t = tid; t < 4 * (ib + 1); t += nt) {
-    int e = t & 3, i = t >> 2;
-    if (e == 0) edge_fill(d, s, gamma, C.inf, i, 1, 1, i, 1, 2, i, 2, 1, i, 2, 2);
-    else if (e == 1) edge_fill(d, s, gamma, C.inf, i, 1, ke, i, 2, ke, i, 1, kl, i, 2, kl);
-    else if (e == 2) edge_fill(d, s, gamma, C.inf, i, je, 1, i, je, 1, i, jl, 2, i, jl, 2);
-    else edge_fill(d, s, gamma, C.inf, i, je, ke, i, jl, ke, i, je, kl, i, jl, kl);
-  }
-  __syncthreads();
-  const T two = T(2), three = T(3);
-#define WQ(n, i, j, k) W[(n) * E + ix(d, i, j, k)]
-  // phase 4: the 8 corners (:141-157)
-  if (tid < 40) {
-    int n = tid % 5, c = tid / 5;
-    if (c == 0) WQ(n, 1, 1, 1) = (WQ(n, 2, 1, 1) + WQ(n, 1, 2, 1) + WQ(n, 1, 1, 2)) * two / three - WQ(n, 2, 2, 2);
-    if (c == 1) WQ(n, 1, 1, ke) = (WQ(n, 2, 1, ke) + WQ(n, 1, 2, ke) + WQ(n, 1, 1, kl)) * two / three - WQ(n, 2, 2, kl);
-    if (c == 2) WQ(n, 1, je, 1) = (WQ(n, 2, je, 1) + WQ(n, 1, jl, 1) + WQ(n, 1, je, 2)) * two / three - WQ(n, 2, jl, 2);
-    if (c == 3) WQ(n, ie, 1, 1) = (WQ(n, il, 1, 1) + WQ(n, ie, 2, 1) + WQ(n, ie, 1, 2)) * two / three - WQ(n, il, 2, 2);
-    if (c == 4) WQ(n, ie, 1, ke) = (WQ(n, il, 1, ke) + WQ(n, ie, 2, ke) + WQ(n, ie, 1, kl)) * two / three - WQ(n, il, 2, kl);
-    if (c == 5) WQ(n, ie, je, 1) = (WQ(n, il, je, 1) + WQ(n, ie, jl, 1) + WQ(n, ie, je, 2)) * two / three - WQ(n, il, jl, 2);
-    if (c == 6) WQ(n, 1, je, ke) = (WQ(n, 2, je, ke) + WQ(n, 1, jl, ke) + WQ(n, 1, je, kl)) * two / three - WQ(n, 2, jl, kl);
-    if (c == 7) WQ(n, ie, je, ke) = (WQ(n, il, je, ke) + WQ(n, ie, jl, ke) + WQ(n, ie, je, kl)) * two / three - WQ(n, il, jl, kl);
-  }
-  __syncthreads();
-#undef WQ
-  if (tid < 8) {
-    int c = tid;
-    int ci = (c & 1) ? ie : 1, cj = (c & 2) ? je : 1, ck = (c & 4) ? ke : 1;
-    long q = ix(d, ci, cj, ck);
-    T v[5];
-    for (int n = 0; n < 5; n++) v[n] = s.w[n * E + q];
-    s.p[q] = eos_st(gamma, C.inf, v);
-  }
+  s.p[qt] = eos_st(C.gamma, C.inf, v);
 }
-MGE_ENTRY_T(k_bc_corner, CornerArgs, 1024, 1)
+MGE_ENTRY_T(k_bc_corner, CornerArgs, 256, 1)
 
 // ------------------------------------------------------------------ misc kernels
 

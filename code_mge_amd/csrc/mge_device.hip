@@ -360,8 +360,12 @@ int do_bcond(mge_case *h, Level &L, int buf) {
     dim3 g((amax + 1 + 255) / 256, cmax + 1);
     LAUNCHT(h, "k_bc_face", k_bc_face, T, g, dim3(256), B);
   }
-  LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3(1), dim3(1024),
-          (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma, make_inf<T>(P)}));
+  {
+    const Dims &d = L.d;
+    const int items = 4 * (d.kl - 1) + 4 * (d.jl - 1) + 4 * (d.ib + 1) + 8;
+    LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3((items + 255) / 256), dim3(256),
+            (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma, make_inf<T>(P)}));
+  }
   return warn;
 }
 

@@ -18,7 +18,7 @@ DROP = 300.0
 MAXCYC = 60
 
 
-def _cases(precision=64, distype="c"):
+def _cases(precision=64, distype="c", **kw):
     """mixed shapes: ellipsoids of three sizes, a cylinder (semi-coarsened
     z) and a box without a body (no wall records)"""
     out = []
@@ -27,13 +27,13 @@ def _cases(precision=64, distype="c"):
                                  (48, (1.0, 0.5, 0.6), 0.6, -3.0)):
         prm, levels, bdy, aref = configs.ellipsoid3d(n=n, nlevel=3, semi=semi, box=4.0, mach=mach, alpha=alpha,
                                                      rot=configs.random_rotation(rng), precision=precision,
-                                                     distype=distype)
+                                                     distype=distype, **kw)
         out.append((prm, levels, [level_geometry(s, bdy) for s in levels]))
-    prm, levels, bdy, aref = configs.cylinder2d(n=64, nz=4, nlevel=3, precision=precision, distype=distype)
+    prm, levels, bdy, aref = configs.cylinder2d(n=64, nz=4, nlevel=3, precision=precision, distype=distype, **kw)
     out.append((prm, levels, [level_geometry(s, bdy) for s in levels]))
     spec = [LevelSpec(n=(8 * m, 6 * m, 4 * m), lo=(0, 0, 0), hi=(1.6, 1.2, 0.8), ratio=(2, 2, 2), bctype=(-20,) * 6)
             for m in (1, 2, 4)]
-    prm = Params(mach=0.4, alpha_deg=5.0, precision=precision, distype=distype)
+    prm = Params(mach=0.4, alpha_deg=5.0, precision=precision, distype=distype, **kw)
     out.append((prm, spec, [level_geometry(s, None) for s in spec]))
     return out
 
@@ -98,6 +98,18 @@ def test_batch_equals_alone_fp64(require_gpu, use_graph):
 
 def test_batch_equals_alone_fp32_roe(require_gpu):
     specs = _cases(precision=32, distype="r2")
+    ref = [_alone(*s) for s in specs]
+    got = _batch(specs, True)
+    for i, ((res, w, p), (st, log, wb, pb)) in enumerate(zip(ref, got)):
+        print(f"case {i}: alone {len(res)} cycles, batch {st}")
+        assert st["ncyc"] == len(res)
+        assert np.array_equal(log, res)
+        assert np.array_equal(wb, w) and np.array_equal(pb, p)
+
+
+def test_batch_equals_alone_ns_fp32(require_gpu):
+    """laminar Navier-Stokes cases (viscous kernels, no-slip mirror) batched"""
+    specs = _cases(precision=32, eqntype="n", reynolds=50.0)
     ref = [_alone(*s) for s in specs]
     got = _batch(specs, True)
     for i, ((res, w, p), (st, log, wb, pb)) in enumerate(zip(ref, got)):
