@@ -248,6 +248,125 @@ __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__
 #undef PR_B
 }
 
+// The same recurrences with three passes over the line instead of four:
+// the forward sweep keeps only the value y at the end of every chunk of CH
+// cells (in LDS, one slot per thread and chunk) and writes nothing; the
+// backward sweep, chunk by chunk from the end, re-reads the chunk's input,
+// recomputes its forward values from the checkpoint (the same operations,
+// so the same bits) and writes the final values.  The next chunk's loads
+// are issued before the current chunk's recurrence in both sweeps.  Lines
+// of up to CH * MAXC cells.
+template <int CH, int MAXC, typename T, typename CK>
+__device__ __attribute__((always_inline)) inline void line_sweep_ck(T *__restrict__ D, const uint8_t *__restrict__ F,
+                                                                    long b0, long st, int nl, const T *__restrict__ e1,
+                                                                    const T *__restrict__ e2, T bb, CK ck) {
+  const int ncell = nl - 1;
+  const int nfull = ncell / CH;
+  T A[CH], B[CH], Y[CH];
+  unsigned ma = 0, mb = 0;
+  // full chunks: unconditional loads (the chunk index is clamped by the
+  // caller), so the next chunk's loads stay in flight under counted waits
+#define LDC(BUF, MK, c)                                               \
+  {                                                                   \
+    MK = 0;                                                           \
+    _Pragma("unroll") for (int t = 0; t < CH; t++) {                  \
+      long q_ = b0 + (long)(2 + (c) * CH + t) * st;                   \
+      BUF[t] = D[q_];                                                 \
+      MK |= (F[q_] & F_FLUID) ? (1u << t) : 0u;                       \
+    }                                                                 \
+  }
+#define FWD(BUF, MK, c, OUT)                                          \
+  _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
+    T m_ = ((MK >> t) & 1u) ? T(1) : T(0);                            \
+    y = m_ * ((BUF[t] - bb * y) / e1[2 + (c) * CH + t]);              \
+    OUT                                                               \
+  }
+#define BWD(MK, c)                                                    \
+  _Pragma("unroll") for (int t = CH - 1; t >= 0; t--) {               \
+    int s_ = 2 + (c) * CH + t;                                        \
+    T m_ = ((MK >> t) & 1u) ? T(1) : T(0);                            \
+    z = Y[t] - m_ * (e2[s_] * z);                                     \
+    D[b0 + (long)s_ * st] = z;                                        \
+  }
+  // ---- forward: checkpoints after every full chunk, then the tail
+  T y = T(0);
+  if (nfull > 0) {
+    LDC(A, ma, 0);
+    for (int c = 0; c < nfull; c += 2) {
+      int c1 = c + 1 < nfull ? c + 1 : c;
+      LDC(B, mb, c1);
+      FWD(A, ma, c, ;);
+      ck[c] = y;
+      if (c + 1 >= nfull) break;
+      int c2 = c + 2 < nfull ? c + 2 : c + 1;
+      LDC(A, ma, c2);
+      FWD(B, mb, c + 1, ;);
+      ck[c + 1] = y;
+    }
+  }
+  // ---- backward: the tail (recomputed from the last checkpoint) ...
+  T z = T(0);
+  {
+    T yt = nfull > 0 ? ck[nfull - 1] : T(0);
+    const int s0 = 2 + nfull * CH;
+    T Yt[CH];
+    y = yt;
+    for (int s_ = s0; s_ <= nl; s_++) {
+      long q_ = b0 + (long)s_ * st;
+      T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
+      y = m_ * ((D[q_] - bb * y) / e1[s_]);
+      Yt[s_ - s0] = y;
+    }
+    for (int s_ = nl; s_ >= s0; s_--) {
+      long q_ = b0 + (long)s_ * st;
+      T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
+      z = Yt[s_ - s0] - m_ * (e2[s_] * z);
+      D[q_] = z;
+    }
+  }
+  // ... then the full chunks from the end, each recomputed from the
+  // checkpoint before it
+  if (nfull > 0) {
+    LDC(A, ma, nfull - 1);
+    for (int c = nfull - 1; c >= 0; c -= 2) {
+      int c1 = c >= 1 ? c - 1 : 0;
+      LDC(B, mb, c1);
+      y = c > 0 ? ck[c - 1] : T(0);
+      FWD(A, ma, c, Y[t] = y;);
+      BWD(ma, c);
+      if (c < 1) break;
+      int c2 = c >= 2 ? c - 2 : 0;
+      LDC(A, ma, c2);
+      y = c - 1 > 0 ? ck[c - 2] : T(0);
+      FWD(B, mb, c - 1, Y[t] = y;);
+      BWD(mb, c - 1);
+    }
+  }
+#undef LDC
+#undef FWD
+#undef BWD
+}
+
+// y/z line: the three-pass sweep when the line fits the checkpoints, else
+// the four-pass one
+template <typename T>
+__device__ __attribute__((always_inline)) inline void smooth_line(T *D, const uint8_t *F, long b0, long st, int nl,
+                                                                  const T *e1, const T *e2, T bb) {
+  constexpr int CH = 16, MAXC = 32;
+  __shared__ T ck[MAXC][64];
+  if (nl - 1 <= CH * MAXC && blockDim.x * blockDim.y <= 64) {
+    // ck[c][lane]: the lanes of one chunk index sit side by side
+    T *mine = &ck[0][threadIdx.x + blockDim.x * threadIdx.y];
+    struct Col {
+      T *p;
+      __device__ T &operator[](int c) const { return p[c * 64]; }
+    } col{mine};
+    line_sweep_ck<CH, MAXC>(D, F, b0, st, nl, e1, e2, bb, col);
+  } else {
+    line_sweep<16>(D, F, b0, st, nl, e1, e2, bb);
+  }
+}
+
 // psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const SmoothArgs<T> &S, int bx, int by, int bz) {
@@ -256,7 +375,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const Smoo
   int k = 2 + by;
   int n = bz;
   if (i > d.il) return;
-  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
+  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
 }
 MGE_ENTRY_T(k_smooth_y, SmoothArgs, 64, 1)
 
@@ -268,7 +387,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_z_body(const Smoo
   int j = 2 + by;
   int n = bz;
   if (i > d.il) return;
-  line_sweep<16>(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
+  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
 }
 MGE_ENTRY_T(k_smooth_z, SmoothArgs, 64, 1)
 
