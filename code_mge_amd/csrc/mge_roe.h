@@ -351,10 +351,10 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
   const bool inarr = i <= d.ib && j <= d.jb;
   static_assert(TJ >= 3, "the high-edge faces are spread over waves 1 and 2");
   const bool diss = A.has_diss;
-  const T Sz = face_s<2>(c.g, i, j, 0);
+  const T Sz = face_s<2>(c.g, min(i, d.ib), min(j, d.jb), 0);  // clamped: read only for valid cells
   // column carry: primitives of k-1..k+2 and the bottom face of plane k
-  T zq[4][5], Ez[5], Dz[5];
-  bool zf[4];
+  T zq[4][5] = {}, Ez[5], Dz[5];
+  bool zf[4] = {false, false, false, false};
   if (valid) {
     const long bz = ix(d, i, j, 0);
     roe_load(c, A.flags, bz + (long)(kb0 - 2) * d.sz, zq[0], &zf[0]);
