@@ -199,6 +199,35 @@ def traffic_probe(args, precision):
     print(f"[bench] traffic probe FP{precision} done", file=sys.stderr, flush=True)
 
 
+def converge_leg(prm, levels, geoms, aref, maxcyc, drop64=1e8, drop32=1e5):
+    """The bench workload run from freestream to its stopping residual drop
+    (1e-8 FP64, 1e-5 FP32; the W-cycle on the finest level after the FMG
+    ramp, 2 cycles per coarser level) or maxcyc cycles; time, cycles and
+    Cd/Cl of each, and the FP32 error against FP64."""
+    import dataclasses
+    from code_mge_amd.solver import Case, force_coefficients
+    out = {"workload": "the bench workload to convergence", "maxcyc": maxcyc}
+    nl = len(levels)
+    for prec, drop in ((64, drop64), (32, drop32)):
+        pp = dataclasses.replace(prm, precision=prec, drop=drop)
+        with Case(pp, levels, geoms=geoms, mxcyc=maxcyc + 8) as c:
+            c.init_flow()
+            c.autotune()
+            c.sync()
+            t0 = time.perf_counter()
+            log, cs = c.solve([2] * (nl - 1) + [maxcyc])
+            dt = time.perf_counter() - t0
+            cd, cl = force_coefficients(c, aref)
+        fine = log[log[:, 0] == nl]
+        out[f"fp{prec}"] = {"s": round(dt, 2), "cycles": int(len(fine)), "residual_drop": float(fine[0, 2] / fine[-1, 2]),
+                            "mcell_stage_per_s": round(cs / dt / 1e6, 1), "cd": cd, "cl": cl}
+        print(f"[bench] converge FP{prec}: {len(fine)} cycles, {dt:.1f} s, drop {fine[0, 2] / fine[-1, 2]:.2e}, "
+              f"Cd {cd:.8f}", file=sys.stderr, flush=True)
+    out["cd_err_fp32"] = abs(out["fp32"]["cd"] - out["fp64"]["cd"])
+    out["cl_err_fp32"] = abs(out["fp32"]["cl"] - out["fp64"]["cl"])
+    return out
+
+
 def ns_leg(cycles):
     """Laminar Navier-Stokes cylinder (Re = 40 on the diameter, M = 0.1,
     Pr = 0.72, constant viscosity) on a uniform 512 x 512 x 4 grid over
@@ -270,6 +299,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
     ap.add_argument("--ns-leg", type=int, default=0, help="W-cycles of the NS cylinder leg (0: skip)")
+    ap.add_argument("--converge", type=int, default=0,
+                    help="run the workload to its residual drop, at most this many cycles (0: skip)")
     ap.add_argument("--serve-leg", type=int, default=0, help="requests of the Poisson serving leg (0: skip)")
     ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
                     help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
@@ -304,6 +335,7 @@ def main():
 
     solves = sweep_leg(args.sweep) if args.sweep > 0 else None
     ns = ns_leg(args.ns_leg) if args.ns_leg > 0 else None
+    conv = converge_leg(prm, levels, geoms, aref, args.converge) if args.converge > 0 else None
     served = serve_leg(args.serve_leg) if args.serve_leg > 0 else None
 
     cpu = None
@@ -337,6 +369,8 @@ def main():
     }
     if ns is not None:
         out["ns_cylinder"] = ns
+    if conv is not None:
+        out["converged"] = conv
     if served is not None:
         out["serving"] = served
     print(json.dumps(out))

@@ -14,7 +14,7 @@ ERRORS = {
     -10: "MGE_ERR_NEG_C2", -11: "MGE_ERR_WALL_RHO", -12: "MGE_ERR_WSOL_NAN",
     -13: "MGE_ERR_RES_NAN", -14: "MGE_ERR_PERIODIC", -15: "MGE_ERR_DISTYPE",
     -16: "MGE_ERR_SMOO_DIM", -20: "MGE_ERR_ARG", -21: "MGE_ERR_HIP",
-    -22: "MGE_ERR_NOMEM", -23: "MGE_ERR_STATE",
+    -22: "MGE_ERR_NOMEM", -23: "MGE_ERR_STATE", -24: "MGE_ERR_RESTART",
 }
 WARNINGS = {1: "MGE_WARN_ZERO_CLOUD", 2: "MGE_WARN_BCTYPE"}
 
@@ -63,6 +63,7 @@ class MgeJob(C.Structure):
         ("n", C.c_int * 3),
         ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("dom_lo", C.c_double * 3), ("dom_hi", C.c_double * 3),
         ("bctype", C.c_int * 6),
+        ("iflow", C.c_int),
     ]
 
 
@@ -136,6 +137,8 @@ def lib():
     L.mge_case_bytes.restype = C.c_longlong
     L.mge_autotune.argtypes = [P, I, C.POINTER(I), dp]
     L.mge_read_job.argtypes = [C.c_char_p, C.POINTER(MgeJob)]
+    L.mge_write_solut.argtypes = [P, C.c_char_p, I]
+    L.mge_read_solut.argtypes = [P, C.c_char_p, I]
     L.mge_job_params.argtypes = [C.POINTER(MgeJob), I, C.POINTER(MgeParams)]
     L.mge_job_level.argtypes = [C.POINTER(MgeJob), I, C.POINTER(I), C.POINTER(I)]
     L.mge_read_pin.argtypes = [C.c_char_p, I, C.POINTER(MgePinGrid), C.POINTER(I)]
@@ -170,4 +173,5 @@ EXPORTED = [
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
     "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc", "mge_geom_level_tri",
     "mge_read_job", "mge_job_params", "mge_job_level", "mge_read_pin", "mge_read_p3d", "mge_input_error",
+    "mge_write_solut", "mge_read_solut",
 ]

@@ -1668,6 +1668,163 @@ int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *i
   return MGE_OK;
 }
 
+// ---- restart files (srcf90/wsolut.f90:47-93, srcf90/rsolut.f90:1-41) -------
+//
+// Text (format 0), as write_solut/read_solut: per block, levels 1..nlevel
+// with the block code 100*lvl + 1 (the .pin's GRID.NAME convention LNN),
+// a "(5I5)" header code, low(4), upp(4) with low = (0,0,0,1) and
+// upp = (ib,jb,kb,5), then w(0:ib,0:jb,0:kb,1:5) i fastest in "(5f16.7)".
+// That file carries w only: reading it recomputes p from w on every cell
+// and the wall mirror states from the restored state (bc_cutcell).
+// Binary (format 1), the full-precision variant: "MGESOL2\0", nlevel, then
+// per block code, ib, jb, kb, nmir (int32), w (5E doubles, same order), p
+// (E doubles) and the mirror states of the wall records (5 nmir + nmir
+// doubles): a run restarted from it continues exactly as the run that wrote
+// it.  Reading checks code and dimensions (the reference's "INCONSISTENT
+// FILE" stop).
+}  // extern "C"
+namespace {
+template <class T>
+int restore_level(mge_case *h, Level &L, bool mirror) {
+  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(L.d.E), dim3(256),
+          (EosArgs<T>{L.d, stm<T>(L, L.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
+  if (mirror) {
+    int rc = do_bc_cutcell<T>(h, L, L.cur);
+    if (rc < 0) return rc;
+  }
+  return check_err(h);
+}
+}  // namespace
+extern "C" {
+
+int mge_write_solut(mge_case *h, const char *path, int format) {
+  if (!h || !path || (format != 0 && format != 1)) return set_err(MGE_ERR_ARG, "bad write_solut arguments");
+  FILE *f = std::fopen(path, format ? "wb" : "w");
+  if (!f) return set_err(MGE_ERR_ARG, "cannot open the restart file for writing");
+  int rc = MGE_OK;
+  if (format == 1) {
+    const char magic[8] = {'M', 'G', 'E', 'S', 'O', 'L', '2', 0};
+    std::fwrite(magic, 1, 8, f);
+    int32_t nl = h->nlevel;
+    std::fwrite(&nl, 4, 1, f);
+  }
+  for (int lvl = 1; lvl <= h->nlevel && rc == MGE_OK; lvl++) {
+    Level &L = h->lv[lvl];
+    const Dims &d = L.d;
+    std::vector<double> w((size_t)5 * d.E);
+    rc = mge_download(h, lvl, 1, "w", w.data());
+    if (rc < 0) break;
+    const int code = 100 * lvl + 1;
+    if (format == 1) {
+      std::vector<double> p(d.E), ws((size_t)5 * L.nmir), ps(L.nmir);
+      rc = mge_download(h, lvl, 1, "p", p.data());
+      if (rc < 0) break;
+      if (L.nmir > 0) {
+        HIPCHK(hipMemcpyAsync(ws.data(), L.wsol, sizeof(double) * ws.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(ps.data(), L.psol, sizeof(double) * ps.size(), hipMemcpyDeviceToHost, h->stream));
+        rc = check_err(h);
+        if (rc < 0) break;
+      }
+      int32_t hd[5] = {code, d.ib, d.jb, d.kb, L.nmir};
+      std::fwrite(hd, 4, 5, f);
+      std::fwrite(w.data(), sizeof(double), w.size(), f);
+      std::fwrite(p.data(), sizeof(double), p.size(), f);
+      std::fwrite(ws.data(), sizeof(double), ws.size(), f);
+      std::fwrite(ps.data(), sizeof(double), ps.size(), f);
+    } else {
+      const int hdr[9] = {code, 0, 0, 0, 1, d.ib, d.jb, d.kb, 5};
+      for (int m = 0; m < 9; m++) {
+        std::fprintf(f, "%5d", hdr[m]);
+        if (m == 4 || m == 8) std::fputc('\n', f);
+      }
+      for (size_t q = 0; q < w.size(); q++) {
+        std::fprintf(f, "%16.7f", w[q]);
+        if (q % 5 == 4 || q + 1 == w.size()) std::fputc('\n', f);
+      }
+    }
+  }
+  if (std::fclose(f) != 0 && rc == MGE_OK) rc = set_err(MGE_ERR_ARG, "writing the restart file failed");
+  return rc;
+}
+
+int mge_read_solut(mge_case *h, const char *path, int format) {
+  if (!h || !path || (format != 0 && format != 1)) return set_err(MGE_ERR_ARG, "bad read_solut arguments");
+  FILE *f = std::fopen(path, format ? "rb" : "r");
+  if (!f) return set_err(MGE_ERR_ARG, "cannot open the restart file");
+  // init_flow(lrflow = .true.) (srcf90/init_flow.f90:25-31): everything the
+  // freestream initialisation sets up, then the state from the file
+  int rc = mge_init_flow(h);
+  if (rc < 0) {
+    std::fclose(f);
+    return rc;
+  }
+  if (format == 1) {
+    char magic[8];
+    int32_t nl = 0;
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "MGESOL2", 8) != 0 || std::fread(&nl, 4, 1, f) != 1 ||
+        nl != h->nlevel)
+      rc = set_err(MGE_ERR_RESTART, "restart file: not a binary solution of this case");
+  }
+  for (int lvl = 1; lvl <= h->nlevel && rc >= 0; lvl++) {
+    Level &L = h->lv[lvl];
+    const Dims &d = L.d;
+    std::vector<double> w((size_t)5 * d.E);
+    int hdr[9] = {0};
+    int nmir = L.nmir;
+    bool ok;
+    if (format == 1) {
+      int32_t hd[5];
+      ok = std::fread(hd, 4, 5, f) == 5;
+      hdr[0] = hd[0];
+      hdr[5] = hd[1];
+      hdr[6] = hd[2];
+      hdr[7] = hd[3];
+      hdr[8] = 5;
+      nmir = hd[4];
+    } else {
+      ok = true;
+      for (int m = 0; m < 9 && ok; m++) ok = std::fscanf(f, "%d", &hdr[m]) == 1;
+    }
+    // rsolut.f90:28-32
+    if (!ok || hdr[0] != 100 * lvl + 1 || hdr[5] != d.ib || hdr[6] != d.jb || hdr[7] != d.kb || hdr[8] != 5 ||
+        nmir != L.nmir) {
+      rc = set_err(MGE_ERR_RESTART, "INCONSISTENT FILE DOF & GRID DIMENSION");
+      break;
+    }
+    if (format == 1) {
+      ok = std::fread(w.data(), sizeof(double), w.size(), f) == w.size();
+    } else {
+      for (size_t q = 0; q < w.size() && ok; q++) ok = std::fscanf(f, "%lf", &w[q]) == 1;
+    }
+    if (!ok) {
+      rc = set_err(MGE_ERR_RESTART, "restart file: short data");
+      break;
+    }
+    rc = mge_upload(h, lvl, 1, "w", w.data());
+    if (rc < 0) break;
+    if (format == 1) {
+      std::vector<double> p(d.E), ws((size_t)5 * nmir), ps(nmir);
+      ok = std::fread(p.data(), sizeof(double), p.size(), f) == p.size() &&
+           std::fread(ws.data(), sizeof(double), ws.size(), f) == ws.size() &&
+           std::fread(ps.data(), sizeof(double), ps.size(), f) == ps.size();
+      if (!ok) {
+        rc = set_err(MGE_ERR_RESTART, "restart file: short data");
+        break;
+      }
+      rc = mge_upload(h, lvl, 1, "p", p.data());
+      if (rc < 0) break;
+      if (nmir > 0) {
+        HIPCHK(hipMemcpy(L.wsol, ws.data(), sizeof(double) * ws.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.psol, ps.data(), sizeof(double) * ps.size(), hipMemcpyHostToDevice));
+      }
+    } else {
+      rc = by_type(h, [&](auto z) { return restore_level<decltype(z)>(h, L, true); });
+    }
+  }
+  std::fclose(f);
+  return rc < 0 ? rc : MGE_OK;
+}
+
 int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1 || !out) return set_err(MGE_ERR_ARG, "bad arguments");

@@ -6,7 +6,8 @@
 // variable mapping below is therefore this library's own:
 //  - jobsheet: sections "NAME { key = value ... }", names and keys
 //    case-insensitive, '!' starts a comment, list values separated by ':'.
-//    CONTROL: job, model, mcyc (one entry per multigrid level, coarsest
+//    CONTROL: job, model, iflow (1: start from <job>_restart.dat,
+//    srcf90/main.f90:78), mcyc (one entry per multigrid level, coarsest
 //    first: the cycles of the FMG ramp, srcf90/solver.f90:119-261);
 //    FLOW: mach, reynolds, gamma, prlmean, alphad, betad;
 //    NUMERICS: cflf (negative: local time step), vis2, vis4, smoo, mgtype,
@@ -308,6 +309,11 @@ int mge_read_job(const char *path, mge_job *job) {
   if (!ctl || !flow || !nm || !mesh) return fail(MGE_ERR_ARG, "jobsheet: CONTROL, FLOW, NUMERICS and MESH are required");
   if (const std::string *v = ctl->get("job")) copy_str(job->job, sizeof(job->job), *v);
   if (const std::string *v = ctl->get("model")) copy_str(job->model, sizeof(job->model), *v);
+  {
+    double fl = 0.0;
+    num(ctl, "iflow", &fl);
+    job->iflow = (int)fl;
+  }
   const std::string *mc = ctl->get("mcyc");
   if (!mc) return fail(MGE_ERR_ARG, "jobsheet: mcyc missing");
   std::vector<double> m = list_of(*mc);

@@ -33,6 +33,7 @@ module mgeapi
     integer(c_int)  :: n(3)
     real(c_double)  :: lo(3), hi(3), dom_lo(3), dom_hi(3)
     integer(c_int)  :: bctype(6)
+    integer(c_int)  :: iflow
   end type
 
   ! one wall record (mge_cutrec): the txcut fields bc_cutcell/efluxb/dfluxb read
@@ -164,6 +165,18 @@ module mgeapi
       integer(c_int), value :: lvl, blk
       character(kind=c_char), intent(in) :: field(*)
       real(c_double) :: out(*)
+    end function
+    integer(c_int) function mge_write_solut(h, path, format) bind(C, name="mge_write_solut")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: format
+    end function
+    integer(c_int) function mge_read_solut(h, path, format) bind(C, name="mge_read_solut")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: format
     end function
     integer(c_int) function mge_wall_force(h, lvl, out) bind(C, name="mge_wall_force")
       import :: c_int, c_ptr, c_double

@@ -7,7 +7,9 @@
 ! the library's per-block calls (timestep, euler, movco, movfin), exactly
 ! as the reference sequences them: an FMG ramp over the levels with mcyc
 ! cycles each, V or W cycles (mgtype), the residual log <job>_res.dat
-! (level, cycle, residual in (2i6,e18.10)) and the drop test.
+! (level, cycle, residual in (2i6,e18.10)) and the drop test.  The restart
+! file <job>_restart.dat is read when the jobsheet sets iflow = 1 and
+! written at the end (main.f90:78,107), with a full-precision .bin beside it.
 !
 ! usage: mgsol [jobsheet [precision]]    precision 64 (default) or 32
 program mgsol
@@ -71,7 +73,12 @@ program mgsol
     print '(a,i2,a,3i6,a,i7,a)', ' level ', lvl, ':', n, ' cells,', nmir, ' wall records'
     deallocate (xv, yv, zv, blank, recs)
   end do
+  ! init_flow (main.f90:78): freestream, or the restart file when iflow = 1
   call check(mge_init_flow(h), 'init_flow')
+  if (job%iflow == 1) then
+    call check(mge_read_solut(h, cstr(trim(jobname)//'_restart.dat'), 0), 'read_solut')
+    print '(a,a)', ' restart from ', trim(jobname)//'_restart.dat'
+  end if
 
   ! ---- solver
   allocate (rlog(3, sum(job%mcyc(1:job%nlevel))))
@@ -106,6 +113,9 @@ program mgsol
     print '(a,f12.8,a,f12.8,a,es10.3)', ' Cd = ', (F(1)*ca*cb + F(2)*sa*cb + F(3)*sb)/(q*aref), &
       '   Cl = ', (-F(1)*sa + F(2)*ca)/(q*aref), '   reference area ', aref
   end if
+  ! write_solut (main.f90:107), and the full-precision variant
+  call check(mge_write_solut(h, cstr(trim(jobname)//'_restart.dat'), 0), 'write_solut')
+  call check(mge_write_solut(h, cstr(trim(jobname)//'_restart.bin'), 1), 'write_solut')
   call check(mge_case_destroy(h), 'destroying the case')
 
 contains

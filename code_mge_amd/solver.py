@@ -139,6 +139,16 @@ class Case:
     def device_bytes(self):
         return int(self.lib.mge_case_bytes(self.h))
 
+    def write_solut(self, path, binary=False):
+        """restart file of every level (write_solut: the reference's text
+        format, or raw doubles with binary)"""
+        _ffi.check(self.lib.mge_write_solut(self.h, str(path).encode(), 1 if binary else 0), "mge_write_solut")
+
+    def read_solut(self, path, binary=False):
+        """state of every level from a restart file (read_solut); p is
+        recomputed from w"""
+        _ffi.check(self.lib.mge_read_solut(self.h, str(path).encode(), 1 if binary else 0), "mge_read_solut")
+
     def autotune(self, levels=None):
         """pick the stage kernels' k-chunk length per level by timing
         (mge_autotune); returns {level: (kc, best launch us)}"""

@@ -55,6 +55,7 @@ extern "C" {
 #define MGE_ERR_HIP        -21  /* HIP runtime failure                                           */
 #define MGE_ERR_NOMEM      -22  /* device allocation failed                                      */
 #define MGE_ERR_STATE      -23  /* call out of order (e.g. level not defined)                    */
+#define MGE_ERR_RESTART    -24  /* restart file does not match the case (srcf90/rsolut.f90:28-32) */
 #define MGE_WARN_ZERO_CLOUD  1  /* empty reflected-node cloud: early return (bc_cutcell.f90:225-230) */
 #define MGE_WARN_BCTYPE      2  /* unknown bctype, face skipped (bcond.f90:35)                    */
 
@@ -135,6 +136,7 @@ typedef struct {
   double lo[3], hi[3];              /* block extent                           */
   double dom_lo[3], dom_hi[3];      /* DOMAIN bounds                          */
   int    bctype[6];                 /* faces i-lo, j-lo, k-lo, i-hi, j-hi, k-hi */
+  int    iflow;                     /* CONTROL iflow: 1 = start from <job>_restart.dat */
 } mge_job;
 
 /* Read a jobsheet and the single-block .pin it names (make_fsijob's job
@@ -212,6 +214,16 @@ double mge_get_dtim(mge_case *h, int lvl, int blk);
 /* FP64 wall force (Fx,Fy,Fz) on the body at level lvl: the momentum flux
  * the scheme applies at each wall face (efluxb.f90:251-253), in record order */
 int mge_wall_force(mge_case *h, int lvl, double out[3]);
+/* restart files of every level: format 0 the reference's text file
+ * (write_solut, srcf90/wsolut.f90:47-93: per block a "(5I5)" header code,
+ * low(4), upp(4), then w(0:ib,0:jb,0:kb,1:5) in "(5f16.7)"; block code
+ * 100*lvl + 1); reading it (read_solut, srcf90/rsolut.f90:1-41) checks code
+ * and dimensions, initialises the flow, takes w from the file and rebuilds
+ * p and the wall mirror states from it.  Format 1, the full-precision
+ * variant, also holds p and the mirror states, so a run restarted from it
+ * continues exactly as the run that wrote it. */
+int mge_write_solut(mge_case *h, const char *path, int format);
+int mge_read_solut(mge_case *h, const char *path, int format);
 int mge_synchronize(mge_case *h);
 /* HIP-event timing of the library's launches on the case stream: mean
  * duration (ms), launch count and mean algorithmic bytes per launch of the

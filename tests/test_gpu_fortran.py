@@ -62,3 +62,28 @@ def test_fortran_driver_equals_library_solve(require_gpu, tmp_path):
     assert flog.shape == log.shape
     assert np.array_equal(flog, log)
     assert np.array_equal(ff, fp)
+    # write_solut at the end of the run (main.f90:107): the text file and
+    # the full-precision variant
+    assert (d / "cylinder_ibm_re20_test_restart.dat").exists()
+    assert (d / "cylinder_ibm_re20_test_restart.bin").exists()
+
+
+def test_fortran_driver_restart(require_gpu, tmp_path):
+    """iflow = 1 starts from <job>_restart.dat (main.f90:78)"""
+    d = _job_dir(tmp_path)
+    p = subprocess.run([EXE, "jobsheet", "64"], cwd=d, capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stdout[-2000:]
+    first = [l.split() for l in open(d / "cylinder_ibm_re20_test_res_full.dat") if not l.startswith(" force")]
+    txt = (d / "jobsheet").read_text()
+    txt = re.sub(r"(?m)^(\s*iflow\s*=\s*)0", r"\g<1>1", txt)
+    assert re.search(r"(?m)^\s*iflow\s*=\s*1", txt)
+    (d / "jobsheet").write_text(txt)
+    p = subprocess.run([EXE, "jobsheet", "64"], cwd=d, capture_output=True, text=True, timeout=600)
+    print(p.stdout[-1500:])
+    assert p.returncode == 0 and "restart from" in p.stdout
+    second = [l.split() for l in open(d / "cylinder_ibm_re20_test_res_full.dat") if not l.startswith(" force")]
+    # the restarted run begins from the state the first run ended with (the
+    # coarsest level holds no body cell, so the fresh start's first residual
+    # there is exactly zero and the restarted one is not)
+    assert float(first[0][2]) == 0.0 and float(second[0][2]) > 0.0
+    assert second != first
