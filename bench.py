@@ -140,7 +140,8 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
     sweep = configs.ellipsoid_sweep()[:k]
     reqs = [Request(i, 0.0, prm, levels, bdy, aref, info) for i, (prm, levels, bdy, aref, info) in enumerate(sweep)]
     print(f"[bench] sweep: {k} cases, FP32 served", file=sys.stderr, flush=True)
-    srv = Server(hbm_budget=200e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4)
+    srv = Server(hbm_budget=200e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4,
+                 log=lambda m: print(m, file=sys.stderr, flush=True) if " done " in m else None)
     t0 = time.perf_counter()
     served = srv.run(reqs)
     t32 = time.perf_counter() - t0
@@ -155,6 +156,8 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
             c.init_flow()
             log, _ = c.solve([0] * (len(levels) - 1) + [maxcyc])
             alone[i] = (len(log), force_coefficients(c, aref), log[-1, 2] / log[0, 2])
+        print(f"[bench] sweep: FP64 case {i} ({levels[-1].n[0]}^3) {len(log)} cycles, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t64 = time.perf_counter() - t0
     print(f"[bench] sweep: FP64 one at a time in {t64:.1f} s", file=sys.stderr, flush=True)
     conv32 = sum(1 for r in served.values() if r.status == 1)

@@ -21,9 +21,27 @@
 // term and the CPU restatement are not), so the GPU Roe path agrees with
 // the restatement to rounding rather than bit for bit.
 #pragma once
+#include <type_traits>
+
 #include "mge_stage.h"
 
 namespace mge {
+
+// 1/x for positive normal x: in FP64 the hardware reciprocal estimate
+// refined by two Newton-Raphson steps r <- r + r (1 - x r) (within an ulp of
+// the rounded quotient), in FP32 the division
+template <typename T>
+__device__ inline T rcp_pos(T x) {
+  if constexpr (std::is_same<T, double>::value) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+  } else {
+    return T(1) / x;
+  }
+}
 
 // minmod(a, b): 0 when a*b <= 0, else the argument of smaller magnitude
 // (written as value selects so no stack slot is needed)
@@ -45,7 +63,7 @@ __device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long
     m3 = m3 + f.w[3];
     p = p + f.p;
   }
-  T rr = T(1) / r;
+  T rr = rcp_pos(r);
   o[0] = r;
   o[1] = m1 * rr;
   o[2] = m2 * rr;
@@ -59,7 +77,7 @@ __device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long
 template <int DIR, typename T>
 __device__ inline T roe_state_flux(T g1, const T *q, T S, T *F) {
 #pragma clang fp contract(fast)
-  T rr = T(1) / q[0];
+  T rr = rcp_pos(q[0]);
   T q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   T H = g1 * (q[4] * rr) + T(0.5) * q2;
   T m = q[0] * (q[1 + DIR] * S);
@@ -132,7 +150,7 @@ __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T
   for (int n = 0; n < 5; n++) e[n] = T(0.5) * (FL[n] + FR[n]);
   if (!diss) return;
   T sL = sqrt(qL[0]), sR = sqrt(qR[0]);
-  T rd = T(1) / (sL + sR);
+  T rd = rcp_pos(sL + sR);
   T wL = sL * rd, wR = sR * rd;
   T rh = sL * sR;
   T uh[3];
@@ -143,13 +161,13 @@ __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T
   T c2 = (gamma - T(1)) * (Hh - T(0.5) * q2h);
   if (c2 < T(1.0e-12)) c2 = T(1.0e-12);
   T c = sqrt(c2);
-  T rc2 = T(1) / c2;
+  T rc2 = rcp_pos(c2);
   T A = fabs(S);
   T nd = S < T(0) ? T(-1) : T(1);
   T un = uh[DIR] * nd;
   T l1 = (un - c) * A, l2 = un * A, l3 = (un + c) * A;
   T del = eps * (fabs(un) + c) * A;
-  T hd = T(0.5) / del;
+  T hd = T(0.5) * rcp_pos(del);
   T a1 = fix_abs(l1, del, hd), a2 = fix_abs(l2, del, hd), a3 = fix_abs(l3, del, hd);
   T dr = qR[0] - qL[0], dp = qR[4] - qL[4];
   T du[3];
@@ -165,16 +183,16 @@ __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T
   for (int m = 0; m < 3; m++) dut[m] = du[m];
   dut[DIR] = du[DIR] - dun * nd;
   T udt = uh[0] * dut[0] + uh[1] * dut[1] + uh[2] * dut[2];
+  // |A| dU = sum of the three wave families, grouped: with b_k = a_k al_k,
+  // s = b1 + b2 + b3, the acoustic difference c (b3 - b1) and g = a2 rh
+  const T b1 = a1 * al1, b2 = a2 * al2, b3 = a3 * al3;
+  const T s13 = b1 + b3, sall = s13 + b2, dl = c * (b3 - b1), g = a2 * rh;
   T dd[5];
-  dd[0] = a1 * al1 + a2 * al2 + a3 * al3;
+  dd[0] = sall;
 #pragma unroll
-  for (int m = 0; m < 3; m++) {
-    if (m == DIR)
-      dd[1 + m] = a1 * al1 * (uh[m] - c * nd) + a2 * (al2 * uh[m] + rh * dut[m]) + a3 * al3 * (uh[m] + c * nd);
-    else
-      dd[1 + m] = a1 * al1 * uh[m] + a2 * (al2 * uh[m] + rh * dut[m]) + a3 * al3 * uh[m];
-  }
-  dd[4] = a1 * al1 * (Hh - c * un) + a2 * (al2 * T(0.5) * q2h + rh * udt) + a3 * al3 * (Hh + c * un);
+  for (int m = 0; m < 3; m++) dd[1 + m] = sall * uh[m] + g * dut[m];
+  dd[1 + DIR] = dd[1 + DIR] + dl * nd;
+  dd[4] = s13 * Hh + dl * un + b2 * (T(0.5) * q2h) + g * udt;
 #pragma unroll
   for (int n = 0; n < 5; n++) d[n] = cdis * (T(0.5) * dd[n]);
 }
