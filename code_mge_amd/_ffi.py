@@ -42,6 +42,8 @@ class MgeParams(C.Structure):
         ("distype", C.c_int),
         ("precision", C.c_int),
         ("epsentr", C.c_double),
+        ("execmode", C.c_int),
+        ("dtstep", C.c_double),
     ]
 
 
@@ -64,6 +66,8 @@ class MgeJob(C.Structure):
         ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("dom_lo", C.c_double * 3), ("dom_hi", C.c_double * 3),
         ("bctype", C.c_int * 6),
         ("iflow", C.c_int),
+        ("execmode", C.c_int), ("nperiod", C.c_int), ("nstep", C.c_int),
+        ("dt", C.c_double), ("pmean", C.c_double), ("tmean", C.c_double), ("rgasmean", C.c_double),
     ]
 
 
@@ -133,6 +137,8 @@ def lib():
     L.mge_synchronize.argtypes = [P]
     L.mge_kernel_timing.argtypes = [P, I, C.c_char_p, dp, C.POINTER(C.c_longlong), dp]
     L.mge_cycles.argtypes = [P, I, I, I, dp, C.POINTER(C.c_longlong)]
+    L.mge_solve_step.argtypes = [P, C.POINTER(I), I, dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.mge_shift_time_levels.argtypes = [P]
     L.mge_case_bytes.argtypes = [P]
     L.mge_case_bytes.restype = C.c_longlong
     L.mge_autotune.argtypes = [P, I, C.POINTER(I), dp]
@@ -173,5 +179,5 @@ EXPORTED = [
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
     "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc", "mge_geom_level_tri",
     "mge_read_job", "mge_job_params", "mge_job_level", "mge_read_pin", "mge_read_p3d", "mge_input_error",
-    "mge_write_solut", "mge_read_solut",
+    "mge_write_solut", "mge_read_solut", "mge_solve_step", "mge_shift_time_levels",
 ]

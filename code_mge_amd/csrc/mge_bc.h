@@ -23,6 +23,8 @@ struct TsArgs {
   T cfac;  // viscous factor of the scheme (timestep.f90:119-124)
   int *err;
   Inf<T> inf;
+  int execmode;  // 1: the pseudo step is capped by dtime
+  T dtime;       // 2*dtstep/(3*cfl)
 };
 
 // timestep.f90:36-115 (Euler) and :117-209 (NS): spectral radii, local
@@ -81,6 +83,8 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
     T fict;
     if (A.eqntype == 1) fict = A.cellvolume / (ri + rj + rk + A.cfac * (rvi + rvj + rvk));
     else fict = A.cellvolume / (ri + rj + rk);
+    // unsteady: capped by the physical step (timestep.f90:86-99, 180-192)
+    if (A.execmode == 1) fict = fict < A.dtime ? fict : A.dtime;
     const T plim = T(0.001);
     const T *P = A.s.p;
     T pc = P[q];

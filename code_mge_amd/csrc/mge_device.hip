@@ -78,6 +78,7 @@ struct Level {
   int cur = 0;
   void *ws = nullptr, *wr = nullptr, *qf = nullptr, *dw = nullptr, *fw = nullptr;
   void *radsum = nullptr, *vdtim = nullptr;
+  void *w1 = nullptr, *w2 = nullptr;  // unsteady: physical time levels n, n-1 (5E each)
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
   void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z each npiv
@@ -149,6 +150,7 @@ struct mge_case {
   double smoo_bb = 0.0;
   std::vector<Rec> *rec = nullptr;  // when set, launches are recorded, not run
   hipError_t launch_err = hipSuccess;  // first failed launch, reported by check_err
+  int itime = 0;            // physical time level of the current euler call (unsteady)
 };
 
 namespace {
@@ -416,6 +418,8 @@ int do_timestep(mge_case *h, Level &L) {
   A.cfac = (T)(P.distype == 0 ? 4.0 : (P.distype == 2 ? 1.0 : 2.0));
   A.inf = make_inf<T>(P);
   A.err = h->err;
+  A.execmode = P.execmode;
+  A.dtime = (T)(2.0 * P.dtstep / (3.0 * P.cfl));
   LAUNCHT(h, "k_timestep", k_timestep, T, cell_grid(L.d), cell_block(), A);
   LAUNCHN(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(64), D);
   return MGE_OK;
@@ -493,6 +497,14 @@ StageArgs<T> stage_args(mge_case *h, Level &L, int m, int src, int dst, int w0, 
   A.order = P.distype == 2 ? 2 : 1;
   A.eps = (T)P.epsentr;
   A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
+  if (P.execmode == 1) {
+    A.tsrc = h->itime > 1 ? 2 : (h->itime == 1 ? 1 : 0);
+    A.ulam = A.coarse ? 1 : 2;
+    A.w1 = tp<T>(L.w1);
+    A.w2 = tp<T>(L.w2);
+    A.tcoef = (T)(0.5 * L.cellvolume / P.dtstep);
+    A.dtstep = (T)P.dtstep;
+  }
   return A;
 }
 
@@ -603,9 +615,15 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       S.e1 = piv + 4 * L.npiv;
       S.e2 = piv + 5 * L.npiv;
       LAUNCHT(h, "k_smooth_z", k_smooth_z, T, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
-      LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256),
-              (UpdArgs<T>{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
-                          (T)P.gamma, make_inf<T>(P)}));
+      UpdArgs<T> U{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
+                   (T)P.gamma, make_inf<T>(P)};
+      U.ulam = A.ulam == 2;
+      U.vdtim = A.vdtim;
+      U.dtim = A.dtim;
+      U.fn = A.fn;
+      U.dtstep = A.dtstep;
+      U.vt = A.vt;
+      LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256), U);
     } else {
       LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, flat_grid(E), dim3(256),
               (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
@@ -696,6 +714,26 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   return do_bcond<T>(h, f, f.cur);
 }
 
+// unsteady: both earlier physical time levels start from the current state
+// (init_flow.f90:79-86)
+int init_time_levels(mge_case *h, Level &L) {
+  const size_t n = h->es * 5 * (size_t)L.d.E;
+  HIPCHK(hipMemcpyAsync(L.w1, L.S[L.cur], n, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(L.w2, L.S[L.cur], n, hipMemcpyDeviceToDevice, h->stream));
+  return MGE_OK;
+}
+
+// solver.f90:263-276: w2 = w1, w1 = w on every level (whole arrays)
+int shift_time_levels(mge_case *h) {
+  for (int l = 1; l <= h->nlevel; l++) {
+    Level &L = h->lv[l];
+    const size_t n = h->es * 5 * (size_t)L.d.E;
+    HIPCHK(hipMemcpyAsync(L.w2, L.w1, n, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(L.w1, L.S[L.cur], n, hipMemcpyDeviceToDevice, h->stream));
+  }
+  return MGE_OK;
+}
+
 template <class T>
 int do_init_flow(mge_case *h) {
   const mge_params &P = h->prm;
@@ -713,6 +751,7 @@ int do_init_flow(mge_case *h) {
   for (int l = 1; l <= h->nlevel; l++) {
     Level &L = h->lv[l];
     LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(L.d.E), dim3(256), (EosArgs<T>{L.d, stm<T>(L, L.cur), (T)P.gamma, make_inf<T>(P)}));
+    if (P.execmode == 1) CHKRC(init_time_levels(h, L));
     int rc = do_bc_cutcell<T>(h, L, L.cur);
     if (rc > 0) warn = rc;
     rc = do_bcond<T>(h, L, L.cur);
@@ -770,11 +809,17 @@ int read_rtrms(mge_case *h, int mlvl, int ncyc, double *r) {
   return check_err(h);
 }
 
-// solver.f90:119-261 for steady flow (nperiod = nstep = 1): FMG ramp over
-// the levels, cycles with the residual log and the drop test
+// solver.f90:119-261, one physical time step at level itime (steady flow:
+// nperiod = nstep = 1, itime = 0): FMG ramp over the levels, cycles with the
+// residual log and the drop test
 template <class T>
-int do_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, long long *cs_out) {
+int do_solve(mge_case *h, const int *mcyc, int itime, double *reslog, int cap, int *nlog, long long *cs_out) {
   const int nlevel = h->nlevel;
+  struct TimeTag {  // euler calls of this solve run at time level itime
+    mge_case *h;
+    ~TimeTag() { h->itime = 0; }
+  } ttag{h};
+  h->itime = itime;
   const mge_params &P = h->prm;
   int nl = 0;
   double ini_res = 0.0, max_res = 0.0;
@@ -1365,6 +1410,8 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
   if (prm->dissip != 0 && prm->dissip != 1) return set_err(MGE_ERR_DISTYPE, "dissip must be scalar or matrix");
   if (prm->precision != 64 && prm->precision != 32 && prm->precision != 0)
     return set_err(MGE_ERR_ARG, "precision must be 64 or 32");
+  if (prm->execmode != 0 && prm->execmode != 1) return set_err(MGE_ERR_ARG, "execmode must be 0 or 1");
+  if (prm->execmode == 1 && !(prm->dtstep > 0.0)) return set_err(MGE_ERR_ARG, "unsteady flow needs dtstep > 0");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(MGE_ERR_HIP, "no HIP device");
   mge_case *h = new mge_case();
@@ -1453,6 +1500,10 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc(L, &L.rtrms, L.mxcyc + 1));
   CHKRC(dalloc(L, &L.force, 3));
   if (h->prm.eqntype == 1) CHKRC(dalloc_bytes(L, &L.vprim, es * 4 * E));
+  if (h->prm.execmode == 1) {
+    CHKRC(dalloc_bytes(L, &L.w1, es * 5 * E));
+    CHKRC(dalloc_bytes(L, &L.w2, es * 5 * E));
+  }
   // vertex differences (ghost entries of index 0 stay unused)
   std::vector<double> hx(nmax, 0.0), hy(nmax, 0.0), hz(nmax, 0.0);
   for (int i = 1; i < d.ib; i++) hx[i] = xv[i + 1] - xv[i];
@@ -1539,10 +1590,11 @@ int mge_timestep(mge_case *h, int lvl, int blk) {
 
 int mge_euler(mge_case *h, int lvl, int blk, int itime, int ncyc, int mgl, int mlvl, int lres, int lfir,
               double *rtrms) {
-  (void)itime;
   Level *L = get_level(h, lvl);
-  if (!L || blk != 1) return set_err(MGE_ERR_ARG, "bad level/block");
+  if (!L || blk != 1 || itime < 0) return set_err(MGE_ERR_ARG, "bad level/block");
+  h->itime = itime;
   int rc = by_type(h, [&](auto z) { return do_euler<decltype(z)>(h, *L, ncyc, mgl, mlvl, lres, lfir); });
+  h->itime = 0;
   if (rc < 0) return rc;
   if (rtrms && !lres) {
     int c = ncyc >= 1 && ncyc <= L->mxcyc ? ncyc : 0;
@@ -1585,7 +1637,27 @@ int mge_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog, 
   if (!h || !mcyc) return set_err(MGE_ERR_ARG, "bad solve arguments");
   for (int l = 1; l <= h->nlevel; l++)
     if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
-  return by_type(h, [&](auto z) { return do_solve<decltype(z)>(h, mcyc, reslog, cap, nlog, cell_stages); });
+  return by_type(h, [&](auto z) { return do_solve<decltype(z)>(h, mcyc, 0, reslog, cap, nlog, cell_stages); });
+}
+
+int mge_solve_step(mge_case *h, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                   long long *cell_stages) {
+  if (!h || !mcyc || itime < 0) return set_err(MGE_ERR_ARG, "bad solve arguments");
+  for (int l = 1; l <= h->nlevel; l++)
+    if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
+  CHKRC(by_type(h, [&](auto z) { return do_solve<decltype(z)>(h, mcyc, itime, reslog, cap, nlog, cell_stages); }));
+  if (h->prm.execmode == 1) {
+    CHKRC(shift_time_levels(h));
+    CHKRC(check_err(h));
+  }
+  return MGE_OK;
+}
+
+int mge_shift_time_levels(mge_case *h) {
+  if (!h) return set_err(MGE_ERR_ARG, "null handle");
+  if (h->prm.execmode != 1) return set_err(MGE_ERR_STATE, "time levels exist for unsteady cases only");
+  CHKRC(shift_time_levels(h));
+  return check_err(h);
 }
 
 int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long long *cell_stages) {
@@ -1604,7 +1676,7 @@ int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long lo
 // fields holding a state (FP32 stores them as the deviation from the
 // freestream): "w" and "ws" (5 components), "p" (1)
 static int state_field(const char *name) {
-  if (!strcmp(name, "w") || !strcmp(name, "ws")) return 5;
+  if (!strcmp(name, "w") || !strcmp(name, "ws") || !strcmp(name, "w1") || !strcmp(name, "w2")) return 5;
   if (!strcmp(name, "p")) return 1;
   return 0;
 }
@@ -1618,6 +1690,8 @@ static void *field_ptr(Level &L, size_t es, const char *name, long *n) {
   if (!strcmp(name, "ws")) return L.ws;
   if (!strcmp(name, "wr")) return L.wr;
   if (!strcmp(name, "qf")) return L.qf;
+  if (!strcmp(name, "w1")) return L.w1;
+  if (!strcmp(name, "w2")) return L.w2;
   *n = E;
   if (!strcmp(name, "p")) return (char *)L.S[L.cur] + es * 5 * E;
   if (!strcmp(name, "vdtim")) return L.vdtim;
@@ -1819,7 +1893,10 @@ int mge_read_solut(mge_case *h, const char *path, int format) {
       }
     } else {
       rc = by_type(h, [&](auto z) { return restore_level<decltype(z)>(h, L, true); });
+      if (rc < 0) break;
     }
+    // unsteady: the earlier time levels start from the restart state (init_flow.f90:79-86)
+    if (h->prm.execmode == 1) rc = init_time_levels(h, L);
   }
   std::fclose(f);
   return rc < 0 ? rc : MGE_OK;
@@ -1899,6 +1976,7 @@ int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc) {
   for (int l = 1; l <= h->nlevel; l++)
     if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
   if (h->lv[h->nlevel].mxcyc < 2) return set_err(MGE_ERR_ARG, "a batched case needs mxcyc >= 2");
+  if (h->prm.execmode != 0) return set_err(MGE_ERR_ARG, "unsteady cases are solved one at a time, not batched");
   const int sm = h->prm.smoo > 0.0;
   if (b->fp32 < 0) {
     b->nlevel = h->nlevel;

@@ -8,8 +8,13 @@
 //    case-insensitive, '!' starts a comment, list values separated by ':'.
 //    CONTROL: job, model, iflow (1: start from <job>_restart.dat,
 //    srcf90/main.f90:78), mcyc (one entry per multigrid level, coarsest
-//    first: the cycles of the FMG ramp, srcf90/solver.f90:119-261);
-//    FLOW: mach, reynolds, gamma, prlmean, alphad, betad;
+//    first: the cycles of the FMG ramp, srcf90/solver.f90:119-261),
+//    runtype (execmode: 0 steady, 1 unsteady), period and step (the
+//    physical time loop of solver.f90:110-112), dt (the physical time step
+//    in seconds: the reference's fsi_dt, set by its absent FSI job library);
+//    FLOW: mach, reynolds, gamma, prlmean, alphad, betad, pmean, tmean,
+//    rgasmean (the freestream speed that makes dt nondimensional,
+//    input_param.f90:51,92);
 //    NUMERICS: cflf (negative: local time step), vis2, vis4, smoo, mgtype,
 //    cstp, cdis, eqntype, distype, dissip, epsentr;
 //    CFD: file (the .p3d surface); MESH: file (the .pin);
@@ -310,9 +315,16 @@ int mge_read_job(const char *path, mge_job *job) {
   if (const std::string *v = ctl->get("job")) copy_str(job->job, sizeof(job->job), *v);
   if (const std::string *v = ctl->get("model")) copy_str(job->model, sizeof(job->model), *v);
   {
-    double fl = 0.0;
+    double fl = 0.0, rt = 0.0, per = 1.0, stp = 1.0;
     num(ctl, "iflow", &fl);
+    num(ctl, "runtype", &rt);
+    num(ctl, "period", &per);
+    num(ctl, "step", &stp);
+    num(ctl, "dt", &job->dt);
     job->iflow = (int)fl;
+    job->execmode = (int)rt;
+    job->nperiod = (int)per;
+    job->nstep = (int)stp;
   }
   const std::string *mc = ctl->get("mcyc");
   if (!mc) return fail(MGE_ERR_ARG, "jobsheet: mcyc missing");
@@ -331,6 +343,12 @@ int mge_read_job(const char *path, mge_job *job) {
   num(flow, "prlmean", &job->prlmean);
   num(flow, "alphad", &job->alphad);
   num(flow, "betad", &job->betad);
+  job->pmean = 101325.0;
+  job->tmean = 288.15;
+  job->rgasmean = 287.0;
+  num(flow, "pmean", &job->pmean);
+  num(flow, "tmean", &job->tmean);
+  num(flow, "rgasmean", &job->rgasmean);
   job->cflf = -2.5;
   job->vis2 = 0.5;
   job->vis4 = 1.0;
@@ -450,6 +468,14 @@ int mge_job_params(const mge_job *job, int precision, mge_params *P) {
   P->distype = job->distype;
   P->precision = precision;
   P->epsentr = job->epsentr;
+  // physical time step made nondimensional by ref_leng/vel_fs (ref_leng = 1),
+  // vel_fs = M sqrt(gamma pmean/rhomean), rhomean = pmean/(R tmean); laminar
+  // unsteady flow takes u0/ref_leng/5 instead (input_param.f90:51,81-96)
+  P->execmode = job->execmode;
+  const double rhomean = job->pmean / (job->rgasmean * job->tmean);
+  const double vel_fs = rm * std::sqrt(g * job->pmean / rhomean);
+  P->dtstep = job->dt * vel_fs / 1.0;
+  if (job->eqntype == 1 && job->execmode == 1) P->dtstep = P->u0 / 1.0 / 5.0;
   return MGE_OK;
 }
 

@@ -427,4 +427,14 @@ __device__ inline void eface(const FluxCtx<T> &c, long base, int a, int li, int 
   for (int n = 0; n < 5; n++) out[n] = T(0.5) * (fa[n] + fm[n]);
 }
 
+// the point-implicit factor of the physical time derivative in unsteady
+// dual time stepping, lam = 1 + de*3/2/dtstep with de = fn*vdtim
+// (euler.f90:322-326, 388-391)
+template <typename T>
+__device__ inline T unsteady_lam(T fn, const T *vdtim, const double *dtim, int vt, T dtstep, long q) {
+  T de = fn * vdtim[q];
+  if (vt == 1) de = fn * T(dtim[0]);
+  return T(1) + de * T(3) / T(2) / dtstep;
+}
+
 }  // namespace mge

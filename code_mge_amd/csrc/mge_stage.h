@@ -71,7 +71,14 @@ struct StageArgs {
   T g1;                     // gamma/(gamma-1)
   int kc;                   // k planes per workgroup (LDS-tiled kernels)
   int fw_in;                // fw already holds this stage's blend and viscous part
+  // unsteady dual time stepping (euler.f90:249-275, 320-331, 383-393)
+  int tsrc;                 // transient term: 0 none, 1 first step (w - w1), 2 BDF2
+  int ulam;                 // 1: coarse level divides dw by lam; 2: the update does
+  const T *w1, *w2;         // physical time levels n and n-1
+  T tcoef;                  // 0.5*cellvolume/dtstep
+  T dtstep;
 };
+
 
 template <typename T>
 struct RecArgs {
@@ -103,6 +110,18 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
   T r[5];
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = e[n] + f[n];
+  if (A.tsrc) {
+    // physical time derivative of the stage state (euler.f90:249-275)
+    const T *w = A.fx.s.w;
+    if (A.tsrc == 2) {
+#pragma unroll
+      for (int n = 0; n < 5; n++)
+        r[n] = r[n] + A.tcoef * (T(3) * w[n * E + q] - T(4) * A.w1[n * E + q] + A.w2[n * E + q]);
+    } else {
+#pragma unroll
+      for (int n = 0; n < 5; n++) r[n] = r[n] + A.tcoef * (w[n * E + q] - A.w1[n * E + q]);
+    }
+  }
   if (A.coarse) {
 #pragma unroll
     for (int n = 0; n < 5; n++) {
@@ -115,6 +134,11 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
         qv = A.qf[n * E + q];
       }
       r[n] = r[n] + qv;
+    }
+    if (A.ulam == 1) {
+      const T lam = unsteady_lam(A.fn, A.vdtim, A.dtim, A.vt, A.dtstep, q);
+#pragma unroll
+      for (int n = 0; n < 5; n++) r[n] = r[n] / lam;
     }
   }
   T bl = (A.flags[q] & F_FLUID) ? T(1) : T(0);
@@ -136,10 +160,12 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
 #pragma unroll
   for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
   if (!A.smoothed) {
+    T lam = T(1);
+    if (A.ulam == 2) lam = unsteady_lam(A.fn, A.vdtim, A.dtim, A.vt, A.dtstep, q);
     T v[5];
 #pragma unroll
     for (int n = 0; n < 5; n++) {
-      v[n] = A.wn[n * E + q] - r[n] / T(1);
+      v[n] = A.wn[n * E + q] - r[n] / lam;
       A.wdst[n * E + q] = v[n];
     }
     A.pdst[q] = eos_st(A.fx.gamma, A.fx.inf, v);

@@ -19,6 +19,8 @@ module mgeapi
     real(c_double)  :: viscl, prlmean, gas_const, drop
     integer(c_int)  :: distype, precision
     real(c_double)  :: epsentr
+    integer(c_int)  :: execmode
+    real(c_double)  :: dtstep
   end type
 
   ! what a jobsheet defines (mge_job)
@@ -34,6 +36,8 @@ module mgeapi
     real(c_double)  :: lo(3), hi(3), dom_lo(3), dom_hi(3)
     integer(c_int)  :: bctype(6)
     integer(c_int)  :: iflow
+    integer(c_int)  :: execmode, nperiod, nstep
+    real(c_double)  :: dt, pmean, tmean, rgasmean
   end type
 
   ! one wall record (mge_cutrec): the txcut fields bc_cutcell/efluxb/dfluxb read
@@ -148,6 +152,10 @@ module mgeapi
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: mgl, mlvl
+    end function
+    integer(c_int) function mge_shift_time_levels(h) bind(C, name="mge_shift_time_levels")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
     end function
     integer(c_int) function mge_solve(h, mcyc, reslog, cap, nlog, cell_stages) bind(C, name="mge_solve")
       import :: c_int, c_ptr, c_double, c_long_long

@@ -1,13 +1,16 @@
-! mgsol.f90 -- the steady single-rank mgsol driver on the MI355X library.
+! mgsol.f90 -- the single-rank mgsol driver on the MI355X library.
 !
 ! Follows srcf90/main.f90: read the job (make_fsijob: jobsheet, .pin,
 ! .p3d surface), build the cut-cell geometry and the multigrid blocks,
 ! initialise the flow, run the solver and report.  The solver is the
-! control of srcf90/solver.f90:119-261 for steady flow, restated here on
-! the library's per-block calls (timestep, euler, movco, movfin), exactly
-! as the reference sequences them: an FMG ramp over the levels with mcyc
-! cycles each, V or W cycles (mgtype), the residual log <job>_res.dat
-! (level, cycle, residual in (2i6,e18.10)) and the drop test.  The restart
+! control of srcf90/solver.f90:110-290, restated here on the library's
+! per-block calls (timestep, euler, movco, movfin), exactly as the reference
+! sequences them: an FMG ramp over the levels with mcyc cycles each, V or W
+! cycles (mgtype), the residual log <job>_res.dat (level, cycle, residual in
+! (2i6,e18.10)) and the drop test; for unsteady jobs (runtype = 1) that ramp
+! once per physical time step (period x step of them, time level itime
+! from 0), each followed by the time-level shift and the step's "(I6)" line
+! in the log.  The restart
 ! file <job>_restart.dat is read when the jobsheet sets iflow = 1 and
 ! written at the end (main.f90:78,107), with a full-precision .bin beside it.
 !
@@ -20,7 +23,7 @@ program mgsol
   type(mge_job_t) :: job
   type(mge_params_t) :: prm
   type(c_ptr) :: h
-  integer(c_int) :: lvl, n(3), ratio(3), ntri, nmir, prec, mxc, nlog, i
+  integer(c_int) :: lvl, n(3), ratio(3), ntri, nmir, prec, mxc, nlog, i, itime, nsteps
   real(c_double), allocatable, target :: tri(:), rlog(:, :)
   real(c_double), allocatable :: xv(:), yv(:), zv(:)
   integer(c_int8_t), allocatable, target :: blank(:)
@@ -52,6 +55,8 @@ program mgsol
   print '(a,a)', ' job      : ', trim(jobname)
   print '(a,3i6,a,i3,a,i7,a)', ' grid     : ', job%n, ' cells,', job%nlevel, ' levels,', ntri, ' surface triangles'
   print '(a,f8.4,a,f9.2,a,i3)', ' mach     : ', job%mach, '  reynolds: ', job%reynolds, '  precision: ', prec
+  if (prm%execmode == 1) print '(a,i5,a,i5,a,es12.5)', ' unsteady : ', job%nperiod, ' periods x', job%nstep, &
+    ' steps, dtstep ', prm%dtstep
 
   ! ---- make_flow / init_metric / solid_ipolcoef per level
   call check(mge_case_create(prm, job%nlevel, h), 'creating the case')
@@ -80,22 +85,41 @@ program mgsol
     print '(a,a)', ' restart from ', trim(jobname)//'_restart.dat'
   end if
 
-  ! ---- solver
-  allocate (rlog(3, sum(job%mcyc(1:job%nlevel))))
+  ! ---- solver: the physical time loop (solver.f90:110-112, 263-290); one
+  ! pass for steady flow
+  nsteps = 1
+  if (prm%execmode == 1) nsteps = max(job%nperiod, 1)*max(job%nstep, 1)
+  allocate (rlog(3, (sum(job%mcyc(1:job%nlevel)) + 1)*nsteps))
   call cpu_time(t0)
-  call solver(h, job%nlevel, job%mcyc, prm%mgtype, prm%drop, rlog, nlog)
+  nlog = 0
+  do itime = 0, nsteps - 1
+    call solver(h, job%nlevel, job%mcyc, prm%mgtype, prm%drop, itime, rlog, nlog)
+    if (prm%execmode == 1) then
+      call check(mge_shift_time_levels(h), 'shift_time_levels')
+      nlog = nlog + 1
+      rlog(:, nlog) = [dble(job%nlevel), -1.0d0, 0.0d0]
+    end if
+  end do
   call check(mge_synchronize(h), 'synchronising')
   call cpu_time(t1)
   print '("Time = ",f16.8," seconds  SOLVER ")', t1 - t0
 
   open (unit=11, file=trim(jobname)//'_res.dat', status='replace')
   do i = 1, nlog
-    write (11, '(2i6,e18.10)') int(rlog(1, i)), int(rlog(2, i)), rlog(3, i)
+    if (rlog(2, i) < 0.0d0) then
+      write (11, '(I6)') int(rlog(1, i))
+    else
+      write (11, '(2i6,e18.10)') int(rlog(1, i)), int(rlog(2, i)), rlog(3, i)
+    end if
   end do
   close (11)
   open (unit=12, file=trim(jobname)//'_res_full.dat', status='replace')
   do i = 1, nlog
-    write (12, '(2i6,es26.17)') int(rlog(1, i)), int(rlog(2, i)), rlog(3, i)
+    if (rlog(2, i) < 0.0d0) then
+      write (12, '(I6)') int(rlog(1, i))
+    else
+      write (12, '(2i6,es26.17)') int(rlog(1, i)), int(rlog(2, i)), rlog(3, i)
+    end if
   end do
 
   ! ---- forces on the finest level (FP64 wall-face momentum flux)
@@ -152,22 +176,22 @@ contains
     a = (ymax - ymin)*(zmax - zmin)
   end function
 
-  ! solver.f90:119-261 (steady: nperiod = nstep = 1): FMG ramp, the residual
-  ! of the finest level after each cycle, max_res reset to 1000 per cycle as
-  ! :166-176 do, the NaN stop (:180) and the drop test (:238)
-  subroutine solver(h, nlevel, mcyc, mgtype, drop, rlog, nlog)
+  ! solver.f90:119-261, one physical time step at time level itime: FMG
+  ! ramp, the residual of the finest level after each cycle, max_res reset
+  ! to 1000 per cycle as :166-176 do, the NaN stop (:180) and the drop test
+  ! (:238); log rows are appended after nlog
+  subroutine solver(h, nlevel, mcyc, mgtype, drop, itime, rlog, nlog)
     type(c_ptr), intent(in) :: h
-    integer(c_int), intent(in) :: nlevel, mcyc(:), mgtype
+    integer(c_int), intent(in) :: nlevel, mcyc(:), mgtype, itime
     real(c_double), intent(in) :: drop
-    real(c_double), intent(out) :: rlog(:, :)
-    integer(c_int), intent(out) :: nlog
+    real(c_double), intent(inout) :: rlog(:, :)
+    integer(c_int), intent(inout) :: nlog
     integer :: mlvl, ncyc
     real(c_double) :: r, ini_res, max_res
-    nlog = 0
     ini_res = 0.0d0
     do mlvl = 1, nlevel
       do ncyc = 1, mcyc(mlvl)
-        call cycle(h, nlevel, mlvl, ncyc, mgtype, r)
+        call cycle(h, nlevel, mlvl, ncyc, mgtype, itime, r)
         max_res = 1000.0d0
         if (ncyc == 1) ini_res = r
         if (r < max_res) max_res = r
@@ -187,9 +211,9 @@ contains
   ! one multigrid cycle with finest level mlvl (solver.f90:187-233): relax
   ! and residual pass going down with movco, relax going up with movfin;
   ! W cycles revisit a level mgtype times
-  subroutine cycle(h, nlevel, mlvl, ncyc, mgtype, r)
+  subroutine cycle(h, nlevel, mlvl, ncyc, mgtype, itime, r)
     type(c_ptr), intent(in) :: h
-    integer(c_int), intent(in) :: nlevel, mlvl, ncyc, mgtype
+    integer(c_int), intent(in) :: nlevel, mlvl, ncyc, mgtype, itime
     real(c_double), intent(out), target :: r
     integer :: icyc(nlevel), mgl, lfir
     icyc = 0
@@ -201,12 +225,12 @@ contains
         if (mgl == mlvl) lfir = 0
         call check(mge_timestep(h, mgl, 1), 'timestep')
         if (mgl == mlvl) then
-          call check(mge_euler(h, mgl, 1, 1, ncyc, mgl, mlvl, 0, lfir, c_loc(r)), 'euler')
+          call check(mge_euler(h, mgl, 1, itime, ncyc, mgl, mlvl, 0, lfir, c_loc(r)), 'euler')
         else
-          call check(mge_euler(h, mgl, 1, 1, ncyc, mgl, mlvl, 0, lfir, c_null_ptr), 'euler')
+          call check(mge_euler(h, mgl, 1, itime, ncyc, mgl, mlvl, 0, lfir, c_null_ptr), 'euler')
         end if
         if (mlvl == 1) return
-        call check(mge_euler(h, mgl, 1, 1, ncyc, mgl, mlvl, 1, 0, c_null_ptr), 'euler (residual)')
+        call check(mge_euler(h, mgl, 1, itime, ncyc, mgl, mlvl, 1, 0, c_null_ptr), 'euler (residual)')
         mgl = mgl - 1
         call check(mge_movco(h, mgl), 'movco')
         if (mgl == 1) exit
@@ -214,7 +238,7 @@ contains
       lfir = 1
       do
         call check(mge_timestep(h, mgl, 1), 'timestep')
-        call check(mge_euler(h, mgl, 1, 1, ncyc, mgl, mlvl, 0, lfir, c_null_ptr), 'euler')
+        call check(mge_euler(h, mgl, 1, itime, ncyc, mgl, mlvl, 0, lfir, c_null_ptr), 'euler')
         lfir = 0
         mgl = mgl + 1
         call check(mge_movfin(h, mgl, mlvl), 'movfin')

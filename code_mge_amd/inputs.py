@@ -21,6 +21,7 @@ class Job:
     levels: List[LevelSpec]     # coarsest first
     body: object                # TriBody or None
     mcyc: List[int]             # cycles per level of the FMG ramp
+    nsteps: int                 # physical time steps (period x step; 1 for steady jobs)
     ref_area: float             # projected bounding-box area of the surface normal to x
     raw: object                 # the mge_job structure
 
@@ -67,11 +68,13 @@ def load_job(path, precision=64) -> Job:
     j = _ffi.MgeJob()
     _err(L.mge_read_job(path.encode(), C.byref(j)), "mge_read_job")
     ns = j.mstage
+    Pc = _ffi.MgeParams()
+    _err(L.mge_job_params(C.byref(j), precision, C.byref(Pc)), "mge_job_params")
     prm = Params(mach=j.mach, alpha_deg=j.alphad, beta_deg=j.betad, gamma=j.gamma, cfl=j.cflf, vis2=j.vis2,
                  vis4=j.vis4, smoo=j.smoo, mgtype=j.mgtype, cstp=tuple(j.cstp[:ns]), cdis=tuple(j.cdis[:ns]),
                  eqntype="n" if j.eqntype == 1 else "e", dissip="matrix" if j.dissip == 1 else "scalar",
                  distype={0: "c", 1: "r1", 2: "r2"}[j.distype], reynolds=j.reynolds, prlmean=j.prlmean,
-                 epsentr=j.epsentr, precision=precision)
+                 epsentr=j.epsentr, precision=precision, execmode=j.execmode, dtstep=Pc.dtstep)
     levels = []
     for lvl in range(1, j.nlevel + 1):
         n = (C.c_int * 3)()
@@ -82,7 +85,8 @@ def load_job(path, precision=64) -> Job:
     surface = j.surface.decode()
     tri = read_p3d(surface) if surface else None
     return Job(j.job.decode(), prm, levels, TriBody(tri) if tri is not None else None,
-               list(j.mcyc[:j.nlevel]), frontal_area(tri), j)
+               list(j.mcyc[:j.nlevel]), max(j.nperiod, 1) * max(j.nstep, 1) if j.execmode == 1 else 1,
+               frontal_area(tri), j)
 
 
 def job_params_c(job: Job, precision=64):

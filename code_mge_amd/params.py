@@ -35,6 +35,8 @@ class Params:
     drop: float = 1.0e6
     precision: int = 64
     epsentr: float = 0.05             # Roe entropy fix (jobsheet r2_epsentr)
+    execmode: int = 0                 # 0 steady, 1 unsteady dual time stepping
+    dtstep: float = 0.0               # physical time step in units of ref_leng/|U_inf|
     extra: dict = field(default_factory=dict)
 
     def derived(self):
@@ -57,8 +59,13 @@ class Params:
         viscl = 0.0
         if self.eqntype.lower() == "n":
             viscl = rho0 * u0 * self.ref_leng / self.reynolds
+        # laminar unsteady flow takes a fixed physical step (input_param.f90:93-96)
+        dtstep = self.dtstep
+        if self.eqntype.lower() == "n" and self.execmode == 1:
+            dtstep = u0 / self.ref_leng / 5.0
         return dict(vt=vt, cfl=cfl, vis4=vis4, rho0=rho0, p0=p0, c0=c0, ei0=ei0,
-                    u0=u0, v0=v0, w0=w0, h0=h0, viscl=viscl, alpha=alpha, beta=beta)
+                    u0=u0, v0=v0, w0=w0, h0=h0, viscl=viscl, alpha=alpha, beta=beta,
+                    execmode=self.execmode, dtstep=dtstep)
 
     def to_c(self):
         d = self.derived()
@@ -86,6 +93,8 @@ class Params:
         P.distype = {"c": 0, "r1": 1, "r2": 2}[self.distype.lower()]
         P.precision = self.precision
         P.epsentr = self.epsentr
+        P.execmode = d["execmode"]
+        P.dtstep = d["dtstep"]
         return P
 
     def dynamic_pressure(self):

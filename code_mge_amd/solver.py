@@ -97,6 +97,31 @@ class Case:
         n = min(nlog.value, cap)
         return log[: 3 * n].reshape(n, 3), cs.value
 
+    def solve_step(self, mcyc: Sequence[int], itime: int, cap: int = 100000):
+        """one physical time step (solver.f90's alop/rstep body): the FMG ramp
+        at time level itime, then (unsteady cases) w2 = w1, w1 = w"""
+        mc = (C.c_int * self.nlevel)(*mcyc)
+        log = np.zeros(3 * cap, dtype=np.float64)
+        nlog = C.c_int(0)
+        cs = C.c_longlong(0)
+        _ffi.check(self.lib.mge_solve_step(self.h, mc, itime, log.ctypes.data_as(_dp), cap, C.byref(nlog),
+                                           C.byref(cs)), "mge_solve_step")
+        n = min(nlog.value, cap)
+        return log[: 3 * n].reshape(n, 3), cs.value
+
+    def solve_unsteady(self, mcyc: Sequence[int], nsteps: int, itime0: int = 0):
+        """nsteps physical time steps from time level itime0 (solver.f90's
+        period x step loop); the residual log of each step"""
+        logs, cs = [], 0
+        for it in range(itime0, itime0 + nsteps):
+            log, c = self.solve_step(mcyc, it)
+            logs.append(log)
+            cs += c
+        return logs, cs
+
+    def shift_time_levels(self):
+        return _ffi.check(self.lib.mge_shift_time_levels(self.h), "mge_shift_time_levels")
+
     def cycles(self, mlvl, count, ncyc0=1):
         """`count` cycles at finest active level mlvl without the drop test"""
         r = C.c_double(0.0)
@@ -111,7 +136,7 @@ class Case:
 
     def download(self, lvl, field):
         ib1, jb1, kb1 = self.dims(lvl)
-        nc = 5 if field in ("w", "dw", "fw", "ws", "wr", "qf") else 1
+        nc = 5 if field in ("w", "dw", "fw", "ws", "wr", "qf", "w1", "w2") else 1
         out = np.empty((nc, kb1, jb1, ib1), dtype=np.float64)
         _ffi.check(self.lib.mge_download(self.h, lvl, 1, field.encode(), out.ctypes.data_as(_dp)), "mge_download")
         return out if nc == 5 else out[0]

@@ -82,6 +82,11 @@ typedef struct {
                                  force integral stay FP64                          */
   double epsentr;             /* Roe entropy-fix coefficient (jobsheet r2_epsentr,
                                  input_param.f90:103)                              */
+  int    execmode;            /* 0 steady; 1 unsteady dual time stepping: BDF2
+                                 physical time derivative in euler, pseudo step
+                                 capped by the physical one (jobsheet runtype)     */
+  double dtstep;              /* nondimensional physical time step (> 0 when
+                                 execmode = 1; input_param.f90:81-96)              */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record
@@ -137,6 +142,10 @@ typedef struct {
   double dom_lo[3], dom_hi[3];      /* DOMAIN bounds                          */
   int    bctype[6];                 /* faces i-lo, j-lo, k-lo, i-hi, j-hi, k-hi */
   int    iflow;                     /* CONTROL iflow: 1 = start from <job>_restart.dat */
+  int    execmode;                  /* CONTROL runtype: 0 steady, 1 unsteady    */
+  int    nperiod, nstep;            /* CONTROL period, step: physical steps = period*step */
+  double dt;                        /* CONTROL dt: physical time step [s]      */
+  double pmean, tmean, rgasmean;    /* FLOW: freestream p [Pa], T [K], R [J/(kg K)] */
 } mge_job;
 
 /* Read a jobsheet and the single-block .pin it names (make_fsijob's job
@@ -200,13 +209,24 @@ int mge_movfin(mge_case *h, int mgl, int mlvl);
 int mge_solve(mge_case *h, const int *mcyc, double *reslog, int cap, int *nlog,
               long long *cell_stages);
 
+/* one physical time step of solver.f90's time loop (the alop/rstep body,
+ * solver.f90:110-290): the FMG ramp of mge_solve with euler at time level
+ * itime (0 for the first step; euler.f90:249-275 adds the BDF2 term from
+ * itime 2 on), then, when execmode = 1, the time-level shift of
+ * mge_shift_time_levels.  With execmode = 0 and itime = 0 this is mge_solve. */
+int mge_solve_step(mge_case *h, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                   long long *cell_stages);
+/* w2 = w1, w1 = w on every level (solver.f90:263-276); execmode = 1 only */
+int mge_shift_time_levels(mge_case *h);
+
 /* `count` multigrid cycles with finest active level mlvl and no drop test
  * (the lop0 body of solver.f90:124-254 for ncyc = ncyc0..ncyc0+count-1);
  * res receives the last cycle's residual (may be NULL). */
 int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long long *cell_stages);
 
 /* ---- data movement and results ------------------------------------------ */
-/* field: "w","p","dw","fw","ws","wr","qf","vdtim","radsum" (radi+radj+radk) */
+/* field: "w","p","dw","fw","ws","wr","qf","vdtim","radsum" (radi+radj+radk);
+ * unsteady cases also "w1","w2" (physical time levels n and n-1) */
 int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out);
 int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *in);
 int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out);

@@ -23,6 +23,7 @@ typedef struct {
   int bctype[6];
   double facenvec[6][3];
   double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
+  double *w1, *w2;        /* physical time levels n and n-1 (unsteady, make_flow.f90:38-39) */
   double *radi, *radj, *radk, *vdtim;
   double dtim;
   int *blank, *marker;
@@ -84,6 +85,7 @@ orc_case *orc_case_new(const orc_params *prm, int nlevel) {
 static void blk_free(blk_t *b) {
   free(b->x); free(b->si); free(b->sj); free(b->sk);
   free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
+  free(b->w1); free(b->w2);
   free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
   free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
 }
@@ -128,6 +130,8 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->wn = (double *)calloc((size_t)E * 5, sizeof(double));
   b->wr = (double *)calloc((size_t)E * 5, sizeof(double));
   b->qf = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->w1 = (double *)calloc((size_t)E * 5, sizeof(double));
+  b->w2 = (double *)calloc((size_t)E * 5, sizeof(double));
   b->p = (double *)calloc((size_t)E, sizeof(double));
   b->radi = (double *)calloc((size_t)E, sizeof(double));
   b->radj = (double *)calloc((size_t)E, sizeof(double));
@@ -282,7 +286,11 @@ int orc_timestep(orc_case *c, int lvl) {
                                   cfac * (radvi + radvj + radvk));
         else
           fict = b->cellvolume / (b->radi[q] + b->radj[q] + b->radk[q]);
-        b->vdtim[q] = fict;
+        /* unsteady: the pseudo step is capped by the physical one
+         * (timestep.f90:86-99, :180-192) */
+        double dtime = 2.0 * P->dtstep / (3.0 * cfl);
+        if (P->execmode == 0) b->vdtim[q] = fict;
+        else b->vdtim[q] = min2(fict, dtime);
         double pc = b->p[q];
         double pip = b->p[qi], pim = b->p[IX(b, i - 1, j, k)];
         double pjp = b->p[qj], pjm = b->p[IX(b, i, j - 1, k)];
@@ -1534,7 +1542,7 @@ int orc_bc_cutcell(orc_case *c, int lvl) { return bc_cutcell_blk(&c->lv[lvl], &c
 /* ------------------------------------------------------------ euler */
 
 /* euler.f90:1-462 (central scheme, steady, Euler equations) */
-int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int lfir) {
+int orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl, int lres, int lfir) {
   blk_t *b = &c->lv[lvl];
   const orc_params *P = &c->prm;
   const long E = b->E;
@@ -1577,6 +1585,23 @@ int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int l
             long q = (long)n * E + IX(b, i, j, k);
             b->dw[q] = b->dw[q] + b->fw[q];
           }
+    if (P->execmode == 1) {
+      /* transient term of the physical time derivative: second-order
+       * backward difference after the first two physical steps, and the
+       * reference's first-step form with its 0.5 factor at itime = 1; none at
+       * itime = 0 (euler.f90:249-275) */
+      for (int k = 2; k <= kl; k++)
+        for (int j = 2; j <= jl; j++)
+          for (int i = 2; i <= il; i++)
+            for (int n = 0; n < 5; n++) {
+              long q = (long)n * E + IX(b, i, j, k);
+              if (itime > 1)
+                b->dw[q] = b->dw[q] + 0.5 * b->cellvolume / P->dtstep *
+                                          (3.0 * b->w[q] - 4.0 * b->w1[q] + b->w2[q]);
+              else if (itime == 1)
+                b->dw[q] = b->dw[q] + 0.5 * b->cellvolume / P->dtstep * (b->w[q] - b->w1[q]);
+            }
+    }
     if (mgl < mlvl) { /* euler.f90:294-318 */
       for (int k = 2; k <= kl; k++)
         for (int j = 2; j <= jl; j++)
@@ -1591,6 +1616,20 @@ int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int l
               b->dw[q] = b->dw[q] + b->qf[q];
             }
           }
+      if (P->execmode == 1) { /* euler.f90:320-331 */
+        for (int k = 2; k <= kl; k++)
+          for (int j = 2; j <= jl; j++)
+            for (int i = 2; i <= il; i++) {
+              long q0 = IX(b, i, j, k);
+              double de = fn * b->vdtim[q0];
+              if (P->vt == 1) de = fn * b->dtim;
+              double lam = 1.0 + de * 3.0 / 2.0 / P->dtstep;
+              for (int n = 0; n < 5; n++) {
+                long q = (long)n * E + q0;
+                b->dw[q] = b->dw[q] / lam;
+              }
+            }
+      }
     }
     for (int k = 2; k <= kl; k++)
       for (int j = 2; j <= jl; j++)
@@ -1621,9 +1660,15 @@ int orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl, int lres, int l
       for (int j = 2; j <= jl; j++)
         for (int i = 2; i <= il; i++) {
           long q0 = IX(b, i, j, k);
+          double lam = 1.0;
+          if (P->execmode == 1 && mgl == mlvl) { /* euler.f90:383-393 */
+            double de = fn * b->vdtim[q0];
+            if (P->vt == 1) de = fn * b->dtim;
+            lam = 1.0 + de * 3.0 / 2.0 / P->dtstep;
+          }
           for (int n = 0; n < 5; n++) {
             long q = (long)n * E + q0;
-            b->w[q] = b->wn[q] - b->dw[q] / 1.0;
+            b->w[q] = b->wn[q] - b->dw[q] / lam;
           }
           eos_cell(b, P->gamma, q0);
         }
@@ -1788,6 +1833,10 @@ int orc_init_flow(orc_case *c) {
   for (int l = 1; l <= c->nlevel; l++) {
     blk_t *b = &c->lv[l];
     for (long q = 0; q < b->E; q++) eos_cell(b, P->gamma, q);
+    if (P->execmode == 1) { /* init_flow.f90:79-86 */
+      memcpy(b->w1, b->w, sizeof(double) * (size_t)b->E * 5);
+      memcpy(b->w2, b->w, sizeof(double) * (size_t)b->E * 5);
+    }
     int rc = bc_cutcell_blk(b, P);
     if (rc < 0) return rc;
     if (rc > 0) warn = rc;
@@ -1804,9 +1853,9 @@ int orc_init_flow(orc_case *c) {
     if (_r < 0) return _r;    \
   } while (0)
 
-/* solver.f90:1-303 (steady: nperiod = nstep = 1) */
-int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
-              long long *cell_stages) {
+/* solver.f90:127-254: the FMG ramp of one physical time step at level itime */
+static int solve_ramp(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                      long long *cell_stages) {
   const int nlevel = c->nlevel;
   const orc_params *P = &c->prm;
   int icyc[64];
@@ -1825,7 +1874,7 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
           int lfir = (mgl == mlvl) ? 0 : 1;
           blk_t *b = &c->lv[mgl];
           CHK(orc_timestep(c, mgl));
-          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 0, lfir));
+          CHK(orc_euler(c, mgl, itime, ncyc, mgl, mlvl, 0, lfir));
           cs += (long long)P->mstage * b->ni * b->nj * b->nk;
           if (mgl == mlvl) {
             max_res = 1000.0;
@@ -1841,7 +1890,7 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
             nl++;
           }
           if (mlvl == 1) { next_cycle = 1; break; }
-          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 1, 0));
+          CHK(orc_euler(c, mgl, itime, ncyc, mgl, mlvl, 1, 0));
           cs += (long long)b->ni * b->nj * b->nk;
           mgl = mgl - 1;
           CHK(orc_movco(c, mgl));
@@ -1853,7 +1902,7 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
         for (;;) { /* lopp */
           blk_t *b = &c->lv[mgl];
           CHK(orc_timestep(c, mgl));
-          CHK(orc_euler(c, mgl, ncyc, mgl, mlvl, 0, lfir));
+          CHK(orc_euler(c, mgl, itime, ncyc, mgl, mlvl, 0, lfir));
           cs += (long long)P->mstage * b->ni * b->nj * b->nk;
           lfir = 0;
           mgl = mgl + 1;
@@ -1879,6 +1928,30 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
   return ORC_OK;
 }
 
+/* solver.f90:1-303 for steady flow (nperiod = nstep = 1, itime = 0) */
+int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
+              long long *cell_stages) {
+  return solve_ramp(c, mcyc, 0, reslog, cap, nlog, cell_stages);
+}
+
+/* solver.f90:263-276: w2 = w1, w1 = w on every level (whole arrays) */
+int orc_shift_time_levels(orc_case *c) {
+  for (int l = 1; l <= c->nlevel; l++) {
+    blk_t *b = &c->lv[l];
+    memcpy(b->w2, b->w1, sizeof(double) * (size_t)b->E * 5);
+    memcpy(b->w1, b->w, sizeof(double) * (size_t)b->E * 5);
+  }
+  return ORC_OK;
+}
+
+/* one pass of solver.f90's physical time loop (alop/rstep body) */
+int orc_solve_step(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                   long long *cell_stages) {
+  CHK(solve_ramp(c, mcyc, itime, reslog, cap, nlog, cell_stages));
+  if (c->prm.execmode == 1) CHK(orc_shift_time_levels(c));
+  return ORC_OK;
+}
+
 /* ------------------------------------------------------------ access */
 
 static double *field_ptr(blk_t *b, const char *name, long *n) {
@@ -1891,6 +1964,8 @@ static double *field_ptr(blk_t *b, const char *name, long *n) {
   if (!strcmp(name, "wn")) return b->wn;
   if (!strcmp(name, "wr")) return b->wr;
   if (!strcmp(name, "qf")) return b->qf;
+  if (!strcmp(name, "w1")) return b->w1;
+  if (!strcmp(name, "w2")) return b->w2;
   *n = 3 * E;
   if (!strcmp(name, "si")) return b->si;
   if (!strcmp(name, "sj")) return b->sj;

@@ -52,6 +52,8 @@ typedef struct {
   double drop;         /* cycle exit when ini_res/res > drop (1e6 in solver.f90:238) */
   int    distype;      /* 0 central (JST), 1 Roe first order, 2 Roe + MUSCL-minmod */
   double epsentr;      /* Roe entropy-fix coefficient */
+  int    execmode;     /* 0 steady, 1 unsteady dual time stepping (BDF2 physical time) */
+  double dtstep;       /* nondimensional physical time step (input_param.f90:81-96) */
 } orc_params;
 
 typedef struct {
@@ -81,7 +83,7 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
 
 int  orc_init_flow(orc_case *c);                               /* init_flow.f90 */
 int  orc_timestep(orc_case *c, int lvl);                       /* timestep.f90  */
-int  orc_euler(orc_case *c, int lvl, int ncyc, int mgl, int mlvl,
+int  orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl,
                int lres, int lfir);                            /* euler.f90     */
 int  orc_bcond(orc_case *c, int lvl);                          /* bcond.f90     */
 int  orc_bc_cutcell(orc_case *c, int lvl);                     /* bc_cutcell.f90*/
@@ -91,6 +93,12 @@ int  orc_movfin(orc_case *c, int mgl, int mlvl);               /* movfin.f90    
  * reslog (capacity cap entries) and returns the count in *nlog. */
 int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
                long long *cell_stages);
+/* one physical time step of solver.f90's alop loop at time level itime (the
+ * FMG ramp of orc_solve with euler's transient terms), then, for execmode 1,
+ * the time-level shift w2 = w1, w1 = w on every level (solver.f90:263-276) */
+int  orc_solve_step(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                    long long *cell_stages);
+int  orc_shift_time_levels(orc_case *c);                       /* solver.f90:263-276 */
 
 /* unit-level entry points for the tests: psmoo_old on the level's dw
  * (psmoo.f90:97-242) and dflux_matrix (dflux.f90:283-339) */
@@ -109,7 +117,7 @@ void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const do
                       const double *ss, double *ff);
 
 /* field access.  name: "w","p","dw","fw","ws","wn","wr","qf","radi","radj",
- * "radk","vdtim","si","sj","sk".  Copies the full (0:ib,...) array. */
+ * "radk","vdtim","si","sj","sk","w1","w2".  Copies the full (0:ib,...) array. */
 int    orc_get(orc_case *c, int lvl, const char *name, double *out);
 int    orc_set(orc_case *c, int lvl, const char *name, const double *in);
 int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);

@@ -25,6 +25,7 @@ class OrcParams(C.Structure):
         ("viscl", C.c_double), ("prlmean", C.c_double), ("gas_const", C.c_double),
         ("drop", C.c_double),
         ("distype", C.c_int), ("epsentr", C.c_double),
+        ("execmode", C.c_int), ("dtstep", C.c_double),
     ]
 
 
@@ -59,12 +60,14 @@ def lib():
     for n in ("orc_init_flow",):
         getattr(L, n).argtypes = [P]
     L.orc_timestep.argtypes = [P, I]
-    L.orc_euler.argtypes = [P, I, I, I, I, I, I]
+    L.orc_euler.argtypes = [P, I, I, I, I, I, I, I]
     L.orc_bcond.argtypes = [P, I]
     L.orc_bc_cutcell.argtypes = [P, I]
     L.orc_movco.argtypes = [P, I]
     L.orc_movfin.argtypes = [P, I, I]
     L.orc_solve.argtypes = [P, C.POINTER(I), _dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.orc_solve_step.argtypes = [P, C.POINTER(I), I, _dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.orc_shift_time_levels.argtypes = [P]
     L.orc_get.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_set.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_get_cuts.argtypes = [P, I, C.c_void_p]
@@ -108,6 +111,8 @@ def params_from(prm):
     P.drop = prm.drop
     P.distype = {"c": 0, "r1": 1, "r2": 2}[prm.distype.lower()]
     P.epsentr = prm.epsentr
+    P.execmode = d["execmode"]
+    P.dtstep = d["dtstep"]
     return P
 
 
@@ -175,8 +180,8 @@ class OracleCase:
     def timestep(self, lvl):
         return self._chk(self.L.orc_timestep(self.h, lvl), "timestep")
 
-    def euler(self, lvl, ncyc, mgl, mlvl, lres, lfir):
-        self._chk(self.L.orc_euler(self.h, lvl, ncyc, mgl, mlvl, int(lres), int(lfir)), "euler")
+    def euler(self, lvl, ncyc, mgl, mlvl, lres, lfir, itime=0):
+        self._chk(self.L.orc_euler(self.h, lvl, itime, ncyc, mgl, mlvl, int(lres), int(lfir)), "euler")
         return self.L.orc_get_rtrms(self.h, lvl, ncyc)
 
     def bcond(self, lvl):
@@ -201,13 +206,27 @@ class OracleCase:
         n = min(nlog.value, cap)
         return log[: 3 * n].reshape(n, 3), cs.value
 
+    def solve_step(self, mcyc, itime, cap=100000):
+        mc = (C.c_int * self.nlevel)(*mcyc)
+        log = np.zeros(3 * cap)
+        nlog = C.c_int(0)
+        cs = C.c_longlong(0)
+        self._chk(self.L.orc_solve_step(self.h, mc, itime, log.ctypes.data_as(_dp), cap, C.byref(nlog),
+                                        C.byref(cs)), "solve_step")
+        n = min(nlog.value, cap)
+        return log[: 3 * n].reshape(n, 3), cs.value
+
+    def shift_time_levels(self):
+        return self._chk(self.L.orc_shift_time_levels(self.h), "shift_time_levels")
+
     def dims(self, lvl):
         ni, nj, nk = self.geoms[lvl - 1].spec.n
         return ni + 4, nj + 4, nk + 4
 
     def download(self, lvl, field):
         ib1, jb1, kb1 = self.dims(lvl)
-        nc = {"w": 5, "dw": 5, "fw": 5, "ws": 5, "wn": 5, "wr": 5, "qf": 5, "si": 3, "sj": 3, "sk": 3}.get(field, 1)
+        nc = {"w": 5, "dw": 5, "fw": 5, "ws": 5, "wn": 5, "wr": 5, "qf": 5, "w1": 5, "w2": 5,
+              "si": 3, "sj": 3, "sk": 3}.get(field, 1)
         out = np.empty((nc, kb1, jb1, ib1))
         self._chk(self.L.orc_get(self.h, lvl, field.encode(), out.ctypes.data_as(_dp)), "get")
         return out if nc > 1 else out[0]
