@@ -30,6 +30,7 @@
 
 #include "../../include/mge.h"
 #include "mge_bc.h"
+#include "mge_output.h"
 #include "mge_roe.h"
 #include "mge_smooth.h"
 #include "mge_stage.h"
@@ -1939,6 +1940,46 @@ int mge_wall_force(mge_case *h, int lvl, double out[3]) {
   by_type(h, [&](auto z) { return do_wall_force<decltype(z)>(h, *L); });
   HIPCHK(hipMemcpyAsync(out, L->force, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
   return check_err(h);
+}
+
+// ---- solution fields (upload_solut.f90:97-241) -------------------------------
+}  // extern "C"
+namespace {
+// freestream Mach number from the nondimensional state (|U| / c0)
+double freestream_mach(const mge_params &P) {
+  return std::sqrt(P.u0 * P.u0 + P.v0 * P.v0 + P.w0 * P.w0) / P.c0;
+}
+
+template <class T>
+int do_upload_solut(mge_case *h, Level &L, double ptot, double *host, int nval) {
+  const long nc = (long)L.d.ni * L.d.nj * L.d.nk;
+  double *dev = nullptr;
+  HIPCHK(hipMallocAsync((void **)&dev, sizeof(double) * nval * nc, h->stream));
+  SolutArgs<T> A{L.d, stc<T>(L, L.cur), make_inf<T>(h->prm), h->prm.gamma, h->prm.p0,
+                 freestream_mach(h->prm), ptot, nval == 11, nc, dev};
+  hipLaunchKernelGGL(k_upload_solut<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, h->stream, A);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host, dev, sizeof(double) * nval * nc, hipMemcpyDeviceToHost, h->stream);
+  hipError_t f = hipFreeAsync(dev, h->stream);
+  if (e == hipSuccess) e = f;
+  if (e != hipSuccess) return set_err(MGE_ERR_HIP, hipGetErrorString(e));
+  return check_err(h);
+}
+}  // namespace
+extern "C" {
+
+int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval) {
+  Level *L = get_level(h, lvl);
+  if (!L || blk != 1) return set_err(MGE_ERR_ARG, "bad level/block");
+  const int nv = h->prm.eqntype == 1 ? 11 : 7;
+  if (nval) *nval = nv;
+  if (!solut) return MGE_OK;
+  const mge_params &P = h->prm;
+  if (!(p_tot > 0.0)) {
+    const double m = freestream_mach(P);
+    p_tot = P.p0 * std::pow(1.0 + 0.2 * m * m, 3.5);
+  }
+  return by_type(h, [&](auto z) { return do_upload_solut<decltype(z)>(h, *L, p_tot, solut, nv); });
 }
 
 // ---- batches ---------------------------------------------------------------

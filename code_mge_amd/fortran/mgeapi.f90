@@ -186,6 +186,14 @@ module mgeapi
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int), value :: format
     end function
+    integer(c_int) function mge_upload_solut(h, lvl, blk, p_tot, solut, nval) bind(C, name="mge_upload_solut")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: lvl, blk
+      real(c_double), value :: p_tot
+      type(c_ptr), value :: solut
+      integer(c_int), intent(out) :: nval
+    end function
     integer(c_int) function mge_wall_force(h, lvl, out) bind(C, name="mge_wall_force")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

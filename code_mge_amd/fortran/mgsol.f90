@@ -23,8 +23,8 @@ program mgsol
   type(mge_job_t) :: job
   type(mge_params_t) :: prm
   type(c_ptr) :: h
-  integer(c_int) :: lvl, n(3), ratio(3), ntri, nmir, prec, mxc, nlog, i, itime, nsteps
-  real(c_double), allocatable, target :: tri(:), rlog(:, :)
+  integer(c_int) :: lvl, n(3), ratio(3), ntri, nmir, prec, mxc, nlog, i, itime, nsteps, nval
+  real(c_double), allocatable, target :: tri(:), rlog(:, :), solut(:, :)
   real(c_double), allocatable :: xv(:), yv(:), zv(:)
   integer(c_int8_t), allocatable, target :: blank(:)
   type(mge_cutrec_t), allocatable, target :: recs(:)
@@ -137,6 +137,14 @@ program mgsol
     print '(a,f12.8,a,f12.8,a,es10.3)', ' Cd = ', (F(1)*ca*cb + F(2)*sa*cb + F(3)*sb)/(q*aref), &
       '   Cl = ', (-F(1)*sa + F(2)*ca)/(q*aref), '   reference area ', aref
   end if
+  ! upload_solut (solver.f90:278): the output fields of the finest level,
+  ! reported as their ranges
+  call check(mge_upload_solut(h, job%nlevel, 1, 0.0d0, c_null_ptr, nval), 'upload_solut')
+  call check_in(mge_job_level(job, job%nlevel, n, ratio), 'level dimensions')
+  allocate (solut(n(1)*n(2)*n(3), nval))
+  call check(mge_upload_solut(h, job%nlevel, 1, 0.0d0, c_loc(solut), nval), 'upload_solut')
+  print '(a,2f10.5,a,f8.5,a,es11.3)', ' cp range     : ', minval(solut(:, 5)), maxval(solut(:, 5)), &
+    '   max Mach ', maxval(solut(:, 6)), '   max pt-loss ', maxval(abs(solut(:, 7)))
   ! write_solut (main.f90:107), and the full-precision variant
   call check(mge_write_solut(h, cstr(trim(jobname)//'_restart.dat'), 0), 'write_solut')
   call check(mge_write_solut(h, cstr(trim(jobname)//'_restart.bin'), 1), 'write_solut')

@@ -17,6 +17,9 @@ from .params import Params
 
 _dp = C.POINTER(C.c_double)
 
+# upload_solut's variable names (upload_solut.f90:170-180)
+SOLUT_NAMES = ("rho", "u", "v", "w", "cp", "mach", "pt-loss", "energy", "gradn1", "gradn2", "gradn3")
+
 
 class Case:
     def __init__(self, params: Params, levels: Sequence[LevelSpec], body=None,
@@ -144,6 +147,18 @@ class Case:
     def upload(self, lvl, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         _ffi.check(self.lib.mge_upload(self.h, lvl, 1, field.encode(), a.ctypes.data_as(_dp)), "mge_upload")
+
+    def upload_solut(self, lvl, p_tot=0.0):
+        """upload_solut's fields of level lvl (upload_solut.f90:97-241): a dict
+        name -> (nk, nj, ni) array over the interior cells; p_tot <= 0 takes
+        the freestream total pressure"""
+        nv = C.c_int(0)
+        _ffi.check(self.lib.mge_upload_solut(self.h, lvl, 1, p_tot, None, C.byref(nv)), "mge_upload_solut")
+        ni, nj, nk = self.levels[lvl - 1].n
+        out = np.empty((nv.value, nk, nj, ni), dtype=np.float64)
+        _ffi.check(self.lib.mge_upload_solut(self.h, lvl, 1, p_tot, out.ctypes.data_as(_dp), C.byref(nv)),
+                   "mge_upload_solut")
+        return dict(zip(SOLUT_NAMES[:nv.value], out))
 
     def cuts(self, lvl):
         g = self.geoms[lvl - 1]

@@ -231,6 +231,15 @@ int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out);
 int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *in);
 int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out);
 double mge_get_dtim(mge_case *h, int lvl, int blk);
+/* upload_solut (srcf90/upload_solut.f90:97-241): the output fields of every
+ * interior cell in FP64, field-major with i fastest over ni*nj*nk cells --
+ * rho, u, v, w, cp = (p/p0 - 1)*2/(gamma*M^2), Mach, total-pressure loss
+ * p*(1 + 0.2*Mach^2)^3.5/p_tot - 1, and for laminar flow rho*E/rho and three
+ * zero gradient slots.  *nval receives 7 (Euler) or 11 (Navier-Stokes);
+ * solut may be NULL to query it.  p_tot <= 0 takes the freestream total
+ * pressure p0*(1 + 0.2*M^2)^3.5 (the reference passes gas_ptot). */
+int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval);
+
 /* FP64 wall force (Fx,Fy,Fz) on the body at level lvl: the momentum flux
  * the scheme applies at each wall face (efluxb.f90:251-253), in record order */
 int mge_wall_force(mge_case *h, int lvl, double out[3]);

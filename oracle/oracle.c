@@ -1934,6 +1934,49 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
   return solve_ramp(c, mcyc, 0, reslog, cap, nlog, cell_stages);
 }
 
+/* upload_solut.f90:97-241: per interior cell rho, u, v, w, cp, Mach,
+ * total-pressure loss (and for laminar flow rho E / rho and three zero
+ * slots), field-major with i fastest; M from the nondimensional freestream
+ * (|U| / c0) */
+int orc_upload_solut(orc_case *c, int lvl, double p_tot, double *out) {
+  const orc_params *P = &c->prm;
+  blk_t *b = &c->lv[lvl];
+  const long nc = (long)b->ni * b->nj * b->nk;
+  const double gamma = P->gamma;
+  const double rm = sqrt(P->u0 * P->u0 + P->v0 * P->v0 + P->w0 * P->w0) / P->c0;
+  if (!(p_tot > 0.0)) p_tot = P->p0 * pow(1.0 + 0.2 * rm * rm, 3.5);
+  long cidx = 0;
+  for (int k = 2; k <= b->kl; k++)
+    for (int j = 2; j <= b->jl; j++)
+      for (int i = 2; i <= b->il; i++, cidx++) {
+        long q = IX(b, i, j, k);
+        double rr = W(b, 0, q);
+        double uu = W(b, 1, q) / rr;
+        double vv = W(b, 2, q) / rr;
+        double ww = W(b, 3, q) / rr;
+        double re = W(b, 4, q);
+        double qq = 0.5 * (uu * uu + vv * vv + ww * ww) * rr;
+        double pp = (gamma - 1.0) * dimf(re, qq);
+        double mm = sqrt((uu * uu + vv * vv + ww * ww) / gamma / pp * rr);
+        double ptloss = pp * pow(1.0 + 0.2 * mm * mm, 3.5) / p_tot - 1.0;
+        pp = (pp / P->p0 - 1.0) * 2.0 / (gamma * rm * rm);
+        out[cidx] = rr;
+        out[nc + cidx] = uu;
+        out[2 * nc + cidx] = vv;
+        out[3 * nc + cidx] = ww;
+        out[4 * nc + cidx] = pp;
+        out[5 * nc + cidx] = mm;
+        out[6 * nc + cidx] = ptloss;
+        if (P->eqntype == 1) {
+          out[7 * nc + cidx] = re / rr;
+          out[8 * nc + cidx] = 0.0;
+          out[9 * nc + cidx] = 0.0;
+          out[10 * nc + cidx] = 0.0;
+        }
+      }
+  return ORC_OK;
+}
+
 /* solver.f90:263-276: w2 = w1, w1 = w on every level (whole arrays) */
 int orc_shift_time_levels(orc_case *c) {
   for (int l = 1; l <= c->nlevel; l++) {

@@ -99,6 +99,9 @@ int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
 int  orc_solve_step(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
                     long long *cell_stages);
 int  orc_shift_time_levels(orc_case *c);                       /* solver.f90:263-276 */
+/* upload_solut.f90:97-241: 7 (Euler) or 11 (NS) fields per interior cell,
+ * field-major, i fastest; p_tot <= 0: freestream total pressure */
+int  orc_upload_solut(orc_case *c, int lvl, double p_tot, double *out);
 
 /* unit-level entry points for the tests: psmoo_old on the level's dw
  * (psmoo.f90:97-242) and dflux_matrix (dflux.f90:283-339) */

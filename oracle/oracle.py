@@ -68,6 +68,7 @@ def lib():
     L.orc_solve.argtypes = [P, C.POINTER(I), _dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
     L.orc_solve_step.argtypes = [P, C.POINTER(I), I, _dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
     L.orc_shift_time_levels.argtypes = [P]
+    L.orc_upload_solut.argtypes = [P, I, C.c_double, _dp]
     L.orc_get.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_set.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_get_cuts.argtypes = [P, I, C.c_void_p]
@@ -215,6 +216,14 @@ class OracleCase:
                                         C.byref(cs)), "solve_step")
         n = min(nlog.value, cap)
         return log[: 3 * n].reshape(n, 3), cs.value
+
+    def upload_solut(self, lvl, p_tot=0.0):
+        ni, nj, nk = self.geoms[lvl - 1].spec.n
+        nv = 11 if self.prm.eqntype.lower() == "n" else 7
+        out = np.empty((nv, nk, nj, ni))
+        self._chk(self.L.orc_upload_solut(self.h, lvl, p_tot, out.ctypes.data_as(_dp)), "upload_solut")
+        names = ("rho", "u", "v", "w", "cp", "mach", "pt-loss", "energy", "gradn1", "gradn2", "gradn3")
+        return dict(zip(names[:nv], out))
 
     def shift_time_levels(self):
         return self._chk(self.L.orc_shift_time_levels(self.h), "shift_time_levels")
