@@ -43,6 +43,29 @@ __device__ inline T rcp_pos(T x) {
   }
 }
 
+// sqrt(x) for positive normal x: in FP64 the hardware reciprocal square
+// root estimate refined by one Goldschmidt step and two Newton corrections
+// (the iteration of the compiler's own expansion, without its range scaling
+// and special-value selects, which positive normal arguments never need),
+// in FP32 the library square root
+template <typename T>
+__device__ inline T sqrt_pos(T x) {
+  if constexpr (std::is_same<T, double>::value) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  } else {
+    return sqrt(x);
+  }
+}
+
 // minmod(a, b): 0 when a*b <= 0, else the argument of smaller magnitude
 // (written as value selects so no stack slot is needed)
 template <typename T>
@@ -75,9 +98,9 @@ __device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long
 // single non-zero component S lies in direction DIR; returns the total
 // enthalpy H = gamma/(gamma-1) p/rho + |u|^2/2 (g1 = gamma/(gamma-1))
 template <int DIR, typename T>
-__device__ inline T roe_state_flux(T g1, const T *q, T S, T *F) {
+__device__ inline T roe_state_flux(T g1, const T *q, T S, T *F, T &rr) {
 #pragma clang fp contract(fast)
-  T rr = rcp_pos(q[0]);
+  rr = rcp_pos(q[0]);
   T q2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   T H = g1 * (q[4] * rr) + T(0.5) * q2;
   T m = q[0] * (q[1 + DIR] * S);
@@ -143,16 +166,17 @@ template <int DIR, typename T>
 __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T *qL, const T *qR, T S, T *e,
                                 T *d) {
 #pragma clang fp contract(fast)
-  T FL[5], FR[5];
-  T HL = roe_state_flux<DIR>(g1, qL, S, FL);
-  T HR = roe_state_flux<DIR>(g1, qR, S, FR);
+  T FL[5], FR[5], rrL, rrR;
+  T HL = roe_state_flux<DIR>(g1, qL, S, FL, rrL);
+  T HR = roe_state_flux<DIR>(g1, qR, S, FR, rrR);
 #pragma unroll
   for (int n = 0; n < 5; n++) e[n] = T(0.5) * (FL[n] + FR[n]);
   if (!diss) return;
-  T sL = sqrt(qL[0]), sR = sqrt(qR[0]);
-  T rd = rcp_pos(sL + sR);
-  T wL = sL * rd, wR = sR * rd;
-  T rh = sL * sR;
+  // Roe weights from t = sqrt(rho_R/rho_L): w_L = 1/(1+t), w_R = t w_L and
+  // the averaged density rho_L t (one square root per face)
+  T t = sqrt_pos(qR[0] * rrL);
+  T wL = rcp_pos(T(1) + t), wR = t * wL;
+  T rh = qL[0] * t;
   T uh[3];
 #pragma unroll
   for (int m = 0; m < 3; m++) uh[m] = wL * qL[1 + m] + wR * qR[1 + m];
@@ -160,7 +184,7 @@ __device__ inline void roe_face(T gamma, T g1, T eps, T cdis, bool diss, const T
   T q2h = uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2];
   T c2 = (gamma - T(1)) * (Hh - T(0.5) * q2h);
   if (c2 < T(1.0e-12)) c2 = T(1.0e-12);
-  T c = sqrt(c2);
+  T c = sqrt_pos(c2);
   T rc2 = rcp_pos(c2);
   T A = fabs(S);
   T nd = S < T(0) ? T(-1) : T(1);

@@ -425,13 +425,14 @@ __device__ __attribute__((always_inline)) inline void k_update_ghost_body(const 
   int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
   bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
   if (inner) {
-    T lam = T(1);
-    if (U.ulam) lam = unsteady_lam(U.fn, U.vdtim, U.dtim, U.vt, U.dtstep, q);
     T v[5];
-    for (int n = 0; n < 5; n++) {
-      v[n] = wn[n * d.E + q] - dw[n * d.E + q] / lam;
-      dst.w[n * d.E + q] = v[n];
+    if (U.ulam) {
+      const T lam = unsteady_lam(U.fn, U.vdtim, U.dtim, U.vt, U.dtstep, q);
+      for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q] / lam;
+    } else {
+      for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q];
     }
+    for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = v[n];
     dst.p[q] = eos_st(gamma, U.inf, v);
   } else {
     for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];

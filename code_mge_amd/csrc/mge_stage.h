@@ -153,19 +153,23 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
     for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
     return;
   }
-  T de = A.fn * A.vdtim[q] / A.cellvolume;
-  if (A.vt == 1) de = A.fn * T(A.dtim[0]) / A.cellvolume;
+  // de = fn*vdtim/V, or fn*dtim/V with the global step: one division
+  const T dt = A.vt == 1 ? T(A.dtim[0]) : A.vdtim[q];
+  const T de = A.fn * dt / A.cellvolume;
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = de * r[n];
 #pragma unroll
   for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
   if (!A.smoothed) {
-    T lam = T(1);
-    if (A.ulam == 2) lam = unsteady_lam(A.fn, A.vdtim, A.dtim, A.vt, A.dtstep, q);
     T v[5];
+    if (A.ulam == 2) {
+      const T lam = unsteady_lam(A.fn, A.vdtim, A.dtim, A.vt, A.dtstep, q);
+#pragma unroll
+      for (int n = 0; n < 5; n++) r[n] = r[n] / lam;
+    }
 #pragma unroll
     for (int n = 0; n < 5; n++) {
-      v[n] = A.wn[n * E + q] - r[n] / lam;
+      v[n] = A.wn[n * E + q] - r[n];
       A.wdst[n * E + q] = v[n];
     }
     A.pdst[q] = eos_st(A.fx.gamma, A.fx.inf, v);
