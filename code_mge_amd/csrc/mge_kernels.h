@@ -2,8 +2,10 @@
 // the Jameson-RK stage, templated on the bulk type T (double or float).
 //
 // Each helper restates one piece of the reference with the same expression
-// tree (the library is compiled with -ffp-contract=off), so with T = double
-// the GPU path reproduces the reference arithmetic operation for operation:
+// tree (the library is compiled with -ffp-contract=off); the quotients of
+// the convective flux and of the pressure switch are products with a
+// reciprocal refined to within an ulp, so with T = double the GPU path
+// agrees with the reference arithmetic to rounding:
 //   eos           depvars.f90:37-52
 //   cflux         ceflux.f90:1-25 with the face vector of one direction
 //   fp_at/diw_at  dflux.f90:70-86 (pressure switch, enthalpy differences)
@@ -185,16 +187,58 @@ __device__ inline void load5(const T *w, long E, long q, T *v) {
   for (int n = 0; n < 5; n++) v[n] = w[n * E + q];
 }
 
+// 1/x for positive normal x: in FP64 the hardware reciprocal estimate
+// refined by two Newton-Raphson steps r <- r + r (1 - x r) (within an ulp of
+// the rounded quotient), in FP32 the division
+template <typename T>
+__device__ inline T rcp_pos(T x) {
+  if constexpr (std::is_same<T, double>::value) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+  } else {
+    return T(1) / x;
+  }
+}
+
+// sqrt(x) for positive normal x: in FP64 the hardware reciprocal square
+// root estimate refined by one Goldschmidt step and two Newton corrections
+// (the iteration of the compiler's own expansion, without its range scaling
+// and special-value selects, which positive normal arguments never need),
+// in FP32 the library square root
+template <typename T>
+__device__ inline T sqrt_pos(T x) {
+  if constexpr (std::is_same<T, double>::value) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  } else {
+    return sqrt(x);
+  }
+}
+
 // ceflux with ss having only component DIR non-zero (value S)
+// (the quotients by rho as products with the refined reciprocal, which
+// agree with the divisions to rounding)
 template <int DIR, typename T>
 __device__ inline void cflux(const T *v, T p, T S, T *fs) {
+  const T rr = rcp_pos(v[0]);
   T f1 = v[1 + DIR] * S;
   fs[0] = f1;
-  fs[1] = f1 * v[1] / v[0];
-  fs[2] = f1 * v[2] / v[0];
-  fs[3] = f1 * v[3] / v[0];
+  fs[1] = f1 * v[1] * rr;
+  fs[2] = f1 * v[2] * rr;
+  fs[3] = f1 * v[3] * rr;
   fs[1 + DIR] = fs[1 + DIR] + p * S;
-  fs[4] = f1 * (v[4] + p) / v[0];
+  fs[4] = f1 * (v[4] + p) * rr;
 }
 
 // ceflux of a stored state: the reference form for FP64; for FP32 the flux
@@ -208,16 +252,16 @@ __device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
   if constexpr (kDelta<T>) {
     const T r0 = f.w[0];
     T f1i = f.w[1 + DIR] * S, f1d = v[1 + DIR] * S;
-    T den = (r0 + v[0]) * r0;
+    const T rden = rcp_pos((r0 + v[0]) * r0);
     fs[0] = f1d;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       T mi = f.w[1 + c], md = v[1 + c];
-      fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) / den;
+      fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) * rden;
     }
     fs[1 + DIR] = fs[1 + DIR] + p * S;
     T hi = f.w[4] + f.p, hd = v[4] + p;
-    fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) / den;
+    fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) * rden;
   } else {
     cflux<DIR>(v, p, S, fs);
   }
@@ -326,7 +370,7 @@ __device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
   }
   if constexpr (kDelta<T>) pb = pb + T(4) * c.inf.p;
   pb = fabs(pb);
-  T v = num / pb;
+  T v = num * rcp_pos(pb);   // pb = 0 (never for physical pressures) is selected away
   v = fabs(pb) > T(0) ? v : T(0);
   return (s <= 0 || s >= nb) ? T(0) : v;
 }

@@ -27,45 +27,6 @@
 
 namespace mge {
 
-// 1/x for positive normal x: in FP64 the hardware reciprocal estimate
-// refined by two Newton-Raphson steps r <- r + r (1 - x r) (within an ulp of
-// the rounded quotient), in FP32 the division
-template <typename T>
-__device__ inline T rcp_pos(T x) {
-  if constexpr (std::is_same<T, double>::value) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-  } else {
-    return T(1) / x;
-  }
-}
-
-// sqrt(x) for positive normal x: in FP64 the hardware reciprocal square
-// root estimate refined by one Goldschmidt step and two Newton corrections
-// (the iteration of the compiler's own expansion, without its range scaling
-// and special-value selects, which positive normal arguments never need),
-// in FP32 the library square root
-template <typename T>
-__device__ inline T sqrt_pos(T x) {
-  if constexpr (std::is_same<T, double>::value) {
-    double y = __builtin_amdgcn_rsq(x);
-    double g = x * y;
-    double h = 0.5 * y;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
-  } else {
-    return sqrt(x);
-  }
-}
-
 // minmod(a, b): 0 when a*b <= 0, else the argument of smaller magnitude
 // (written as value selects so no stack slot is needed)
 template <typename T>
@@ -413,6 +374,29 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     roe_load(c, A.flags, bz + (long)(kb0 + 2) * d.sz, zq[3], &zf[3]);
   }
   for (int k = kb0; k <= kb1; k++) {
+    // ---- FP64: the finish inputs of this plane's cell, loaded first so
+    // that their latency overlaps the face evaluation (two waves per SIMD
+    // hide little); FP32 runs four waves per SIMD and keeps its registers
+    constexpr bool PRE = !kDelta<T>;
+    const long qk = ix(d, i, j, k);
+    T pfw[5] = {}, pwn[5] = {}, pvd = T(0);
+    if (PRE && valid) {
+      if (A.fw_in || A.m > 1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pfw[n] = A.fw[n * E + qk];
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pfw[n] = T(0);
+      }
+      if (!A.lres && !A.smoothed) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pwn[n] = A.wn[n * E + qk];
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pwn[n] = T(0);
+      }
+      pvd = A.vdtim[qk];
+    }
     // ---- primitives of the plane: own cell, then the halo ring
     if (inarr) {
       T o[5];
@@ -499,14 +483,14 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     __syncthreads();
     // ---- accumulate and finish
     if (valid) {
-      const long q = ix(d, i, j, k);
+      const long q = qk;
       T f[5], e[5];
       if (A.fw_in) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q];
+        for (int n = 0; n < 5; n++) f[n] = PRE ? pfw[n] : A.fw[n * E + q];
       } else if (A.m > 1) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+        for (int n = 0; n < 5; n++) f[n] = (PRE ? pfw[n] : A.fw[n * E + q]) * A.blend;
       } else {
 #pragma unroll
         for (int n = 0; n < 5; n++) f[n] = T(0);
@@ -532,7 +516,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           A.dw[n * E + q] = e[n];
         }
       } else {
-        finish_cell(A, q, f, e);
+        finish_cell<T, PRE>(A, q, f, e, pwn, pvd);
       }
       // advance the column carry
 #pragma unroll

@@ -103,9 +103,12 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 // ------------------------------------------------------------------ finish of a cell
 
 // dw = dw + fw, forcing, blanking, time-step scaling and (unsmoothed) the
-// update with the EOS (euler.f90:217-223, 294-342, 363-399)
-template <typename T>
-__device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, const T *e) {
+// update with the EOS (euler.f90:217-223, 294-342, 363-399).  With PRE the
+// caller has already loaded the cell's W(0) (wnp) and local time step (vdt),
+// so that their latency overlapped its face evaluation.
+template <typename T, bool PRE = false>
+__device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, const T *e, const T *wnp = nullptr,
+                                   T vdt = T(0)) {
   const long E = A.fx.d.E;
   T r[5];
 #pragma unroll
@@ -154,7 +157,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
     return;
   }
   // de = fn*vdtim/V, or fn*dtim/V with the global step: one division
-  const T dt = A.vt == 1 ? T(A.dtim[0]) : A.vdtim[q];
+  const T dt = A.vt == 1 ? T(A.dtim[0]) : (PRE ? vdt : A.vdtim[q]);
   const T de = A.fn * dt / A.cellvolume;
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = de * r[n];
@@ -169,7 +172,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
     }
 #pragma unroll
     for (int n = 0; n < 5; n++) {
-      v[n] = A.wn[n * E + q] - r[n];
+      v[n] = (PRE ? wnp[n] : A.wn[n * E + q]) - r[n];
       A.wdst[n * E + q] = v[n];
     }
     A.pdst[q] = eos_st(A.fx.gamma, A.fx.inf, v);
@@ -327,6 +330,23 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     }
   }
   for (int k = kb0; k <= kb1; k++) {
+    // ---- FP64: the finish inputs of this plane's cell, loaded first so
+    // that their latency overlaps the plane's fill and faces (two waves per
+    // SIMD hide little); FP32 keeps its registers for occupancy
+    constexpr bool PRE = !kDelta<T>;
+    const long qk = ix(d, i, j, k);
+    T pfw[5] = {}, pwn[5] = {}, pvd = T(0);
+    if (PRE && valid) {
+      if (A.fw_in || A.m > 1) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pfw[n] = A.fw[n * E + qk];
+      }
+      if (!A.lres && !A.smoothed) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) pwn[n] = A.wn[n * E + qk];
+      }
+      pvd = A.vdtim[qk];
+    }
     // ---- fill the plane's i/j quantities (cells within the arrays)
     {
       // own cell
@@ -402,14 +422,14 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     }
     __syncthreads();
     if (valid) {
-      const long q = ix(d, i, j, k);
+      const long q = qk;
       T f[5];
       if (A.fw_in) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q];
+        for (int n = 0; n < 5; n++) f[n] = PRE ? pfw[n] : A.fw[n * E + q];
       } else if (A.m > 1) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = A.fw[n * E + q] * A.blend;
+        for (int n = 0; n < 5; n++) f[n] = (PRE ? pfw[n] : A.fw[n * E + q]) * A.blend;
       } else {
 #pragma unroll
         for (int n = 0; n < 5; n++) f[n] = T(0);
@@ -480,7 +500,7 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
           A.dw[n * E + q] = e[n];
         }
       } else {
-        finish_cell(A, q, f, e);
+        finish_cell<T, PRE>(A, q, f, e, pwn, pvd);
       }
       // advance the k-direction carry to face k+2
       if (k < kb1) {
