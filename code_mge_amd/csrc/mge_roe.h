@@ -38,8 +38,18 @@ __device__ inline T minmod(T a, T b) {
 // primitive variables of a cell from the stored state (FP32: the full
 // values rebuilt from the deviations)
 template <typename T>
+__device__ inline void prim_raw(const T *raw, const Inf<T> &f, T *o);
+
+template <typename T>
 __device__ inline void prim_of(const StateC<T> &s, const Inf<T> &f, long E, long q, T *o) {
-  T r = s.w[q], m1 = s.w[E + q], m2 = s.w[2 * E + q], m3 = s.w[3 * E + q], p = s.p[q];
+  const T raw[5] = {s.w[q], s.w[E + q], s.w[2 * E + q], s.w[3 * E + q], s.p[q]};
+  prim_raw(raw, f, o);
+}
+
+// the same from the loaded values rho, m1, m2, m3, p of a cell
+template <typename T>
+__device__ inline void prim_raw(const T *raw, const Inf<T> &f, T *o) {
+  T r = raw[0], m1 = raw[1], m2 = raw[2], m3 = raw[3], p = raw[4];
   if constexpr (kDelta<T>) {
     r = r + f.w[0];
     m1 = m1 + f.w[1];
@@ -355,6 +365,48 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
   static_assert(TJ >= 3, "the high-edge faces are spread over waves 1 and 2");
   const bool diss = A.has_diss;
   const T Sz = face_s<2>(c.g, min(i, d.ib), min(j, d.jb), 0);  // clamped: read only for valid cells
+  // halo ring items of this thread (t = tid and tid + TI*TJ): their LDS
+  // slot and cell do not change with k.  FP64 loads the next plane's items
+  // during the current plane's faces (two waves per SIMD hide little
+  // latency); FP32 loads them in place
+  constexpr bool PRE = !kDelta<T>;
+  constexpr int NH = 4 * (TI + 4) + 4 * TJ;
+  int hr[2] = {0, 0}, hc[2] = {0, 0};
+  long hq[2] = {0, 0};
+  bool hok[2] = {false, false};
+  T hraw[2][5] = {};
+  uint8_t hfl[2] = {0, 0};
+  if constexpr (PRE) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int t = tid + h * TI * TJ;
+      int r, col;
+      if (t < 4 * (TI + 4)) {
+        int rr = t / (TI + 4);
+        col = t % (TI + 4);
+        r = rr < 2 ? rr : TJ + rr;
+      } else {
+        int u = t - 4 * (TI + 4);
+        int cc = u / TJ;
+        r = 2 + u % TJ;
+        col = cc < 2 ? cc : TI + cc;
+      }
+      const int ii = i0 - 2 + col, jj = j0 - 2 + r;
+      hr[h] = r;
+      hc[h] = col;
+      hok[h] = t < NH && ii <= d.ib && jj <= d.jb;
+      hq[h] = hok[h] ? ix(d, ii, jj, 0) : 0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      if (hok[h]) {
+        const long q = hq[h] + (long)kb0 * d.sz;
+#pragma unroll
+        for (int n = 0; n < 4; n++) hraw[h][n] = c.s.w[n * E + q];
+        hraw[h][4] = c.s.p[q];
+        hfl[h] = A.flags[q];
+      }
+  }
   // column carry: primitives of k-1..k+2 and the bottom face of plane k
   T zq[4][5] = {}, Ez[5], Dz[5];
   bool zf[4] = {false, false, false, false};
@@ -412,29 +464,53 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       for (int n = 0; n < 5; n++) sm.q[n][ty + 2][tx + 2] = o[n];
       sm.fl[ty + 2][tx + 2] = f;
     }
-    for (int t = tid; t < 4 * (TI + 4) + 4 * TJ; t += TI * TJ) {
-      int r, col;
-      if (t < 4 * (TI + 4)) {
-        int rr = t / (TI + 4);
-        col = t % (TI + 4);
-        r = rr < 2 ? rr : TJ + rr;
-      } else {
-        int u = t - 4 * (TI + 4);
-        int cc = u / TJ;
-        r = 2 + u % TJ;
-        col = cc < 2 ? cc : TI + cc;
-      }
-      int ii = i0 - 2 + col, jj = j0 - 2 + r;
-      if (ii <= d.ib && jj <= d.jb) {
-        T o[5];
-        bool f;
-        roe_load(c, A.flags, ix(d, ii, jj, k), o, &f);
+    if constexpr (PRE) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) sm.q[n][r][col] = o[n];
-        sm.fl[r][col] = f;
+      for (int h = 0; h < 2; h++)
+        if (hok[h]) {
+          T o[5];
+          prim_raw(hraw[h], c.inf, o);
+#pragma unroll
+          for (int n = 0; n < 5; n++) sm.q[n][hr[h]][hc[h]] = o[n];
+          sm.fl[hr[h]][hc[h]] = (hfl[h] & F_FLUID) != 0;
+        }
+    } else {
+      for (int t = tid; t < NH; t += TI * TJ) {
+        int r, col;
+        if (t < 4 * (TI + 4)) {
+          int rr = t / (TI + 4);
+          col = t % (TI + 4);
+          r = rr < 2 ? rr : TJ + rr;
+        } else {
+          int u = t - 4 * (TI + 4);
+          int cc = u / TJ;
+          r = 2 + u % TJ;
+          col = cc < 2 ? cc : TI + cc;
+        }
+        int ii = i0 - 2 + col, jj = j0 - 2 + r;
+        if (ii <= d.ib && jj <= d.jb) {
+          T o[5];
+          bool f;
+          roe_load(c, A.flags, ix(d, ii, jj, k), o, &f);
+#pragma unroll
+          for (int n = 0; n < 5; n++) sm.q[n][r][col] = o[n];
+          sm.fl[r][col] = f;
+        }
       }
     }
     __syncthreads();
+    // FP64: the next plane's halo items, in flight during this plane's faces
+    if constexpr (PRE) if (k < kb1) {
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        if (hok[h]) {
+          const long q = hq[h] + (long)(k + 1) * d.sz;
+#pragma unroll
+          for (int n = 0; n < 4; n++) hraw[h][n] = c.s.w[n * E + q];
+          hraw[h][4] = c.s.p[q];
+          hfl[h] = A.flags[q];
+        }
+    }
     // ---- faces: own i face and j face, then the tile's high-edge faces
     {
       T e[5], dd[5];
