@@ -432,6 +432,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     constexpr bool PRE = !kDelta<T>;
     const long qk = ix(d, i, j, k);
     T pfw[5] = {}, pwn[5] = {}, pvd = T(0);
+    uint8_t pfl = 0;
     if (PRE && valid) {
       if (A.fw_in || A.m > 1) {
 #pragma unroll
@@ -448,6 +449,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
         for (int n = 0; n < 5; n++) pwn[n] = T(0);
       }
       pvd = A.vdtim[qk];
+      pfl = A.flags[qk];
     }
     // ---- primitives of the plane: own cell, then the halo ring
     if (inarr) {
@@ -585,14 +587,14 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
         e[n] = e[n] + (sm.ye[n][ty][tx] - sm.ye[n][ty + 1][tx]);
         e[n] = e[n] + (Ez[n] - Ez1[n]);
       }
-      if (A.flags[q] & F_TOUCHED) {
+      if ((PRE ? pfl : A.flags[q]) & F_TOUCHED) {
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           A.fw[n * E + q] = f[n];
           A.dw[n * E + q] = e[n];
         }
       } else {
-        finish_cell<T, PRE>(A, q, f, e, pwn, pvd);
+        finish_cell<T, PRE>(A, q, f, e, pwn, pvd, pfl);
       }
       // advance the column carry
 #pragma unroll

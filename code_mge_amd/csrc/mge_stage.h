@@ -104,11 +104,11 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 
 // dw = dw + fw, forcing, blanking, time-step scaling and (unsmoothed) the
 // update with the EOS (euler.f90:217-223, 294-342, 363-399).  With PRE the
-// caller has already loaded the cell's W(0) (wnp) and local time step (vdt),
-// so that their latency overlapped its face evaluation.
+// caller has already loaded the cell's W(0) (wnp), local time step (vdt) and
+// flags (fl), so that their latency overlapped its face evaluation.
 template <typename T, bool PRE = false>
 __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, const T *e, const T *wnp = nullptr,
-                                   T vdt = T(0)) {
+                                   T vdt = T(0), uint8_t fl = 0) {
   const long E = A.fx.d.E;
   T r[5];
 #pragma unroll
@@ -144,7 +144,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
       for (int n = 0; n < 5; n++) r[n] = r[n] / lam;
     }
   }
-  T bl = (A.flags[q] & F_FLUID) ? T(1) : T(0);
+  T bl = ((PRE ? fl : A.flags[q]) & F_FLUID) ? T(1) : T(0);
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = r[n] * bl;
   if (A.store_fw) {
@@ -336,6 +336,7 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     constexpr bool PRE = !kDelta<T>;
     const long qk = ix(d, i, j, k);
     T pfw[5] = {}, pwn[5] = {}, pvd = T(0);
+    uint8_t pfl = 0;
     if (PRE && valid) {
       if (A.fw_in || A.m > 1) {
 #pragma unroll
@@ -346,6 +347,7 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
         for (int n = 0; n < 5; n++) pwn[n] = A.wn[n * E + qk];
       }
       pvd = A.vdtim[qk];
+      pfl = A.flags[qk];
     }
     // ---- fill the plane's i/j quantities (cells within the arrays)
     {
@@ -493,14 +495,14 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
         Ek1[n] = half * (Fz[1][n] + Fz[0][n]);
         e[n] = e[n] + Ek[n] - Ek1[n];
       }
-      if (A.flags[q] & F_TOUCHED) {
+      if ((PRE ? pfl : A.flags[q]) & F_TOUCHED) {
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           A.fw[n * E + q] = f[n];
           A.dw[n * E + q] = e[n];
         }
       } else {
-        finish_cell<T, PRE>(A, q, f, e, pwn, pvd);
+        finish_cell<T, PRE>(A, q, f, e, pwn, pvd, pfl);
       }
       // advance the k-direction carry to face k+2
       if (k < kb1) {
