@@ -121,9 +121,56 @@ def run_leg(prm, levels, geoms, aref, args, precision):
                         "all_levels": {"avg_launch_ms": round(ms_all, 5), "launches": cnt_all,
                                        "frac": round(by_all / (ms_all * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
                                        if ms_all > 0 else None}}
+    kernels = finest_kernels(case, levels[-1].n, nl, precision) if nroof > 0 else None
     case.close()
     return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1,
-                kc={lvl: kc for lvl, (kc, us) in tuned.items()})
+                kc={lvl: kc for lvl, (kc, us) in tuned.items()}, kernels=kernels)
+
+
+def finest_kernels(case, n, lvl, precision):
+    """HIP-event mean time and HBM fraction of the finest level's kernels in
+    the roofline pass, from their algorithmic bytes per launch (s bytes per
+    value, N interior cells, E cells with the two ghost layers):
+    stage kernel as counted by the library; x/y/z smoothing read and write
+    dw (10 s N) and read the flags (N); the post-smoothing update reads W0
+    and dw and writes W and p on interior cells (16 s N) and copies W and p
+    on the ghost shell (12 s (E - N))"""
+    s = 8 if precision == 64 else 4
+    N = n[0] * n[1] * n[2]
+    E = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
+    out = {}
+    for name, by in (("k_stage", None), ("k_smooth_x", (10 * s + 1) * N), ("k_smooth_y", (10 * s + 1) * N),
+                     ("k_smooth_z", (10 * s + 1) * N), ("k_update_ghost", 16 * s * N + 12 * s * (E - N))):
+        ms, cnt, b = case.timing(enable=False, name=f"{name}@{lvl}")
+        if cnt == 0 or ms <= 0:
+            continue
+        b = b if by is None else by
+        gbs = b / (ms * 1e-3) / 1e9
+        out[name] = {"avg_us": round(ms * 1e3, 1), "launches": cnt, "alg_bytes": int(b),
+                     "achieved_gbs": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return out
+
+
+def jst_leg(args):
+    """The same sphere with the reference's own JST scheme (cfg2's grid,
+    levels and flow; distype c): Mcell-stage/s in FP64 and FP32 and the
+    finest level's per-kernel HBM fractions (the JST stage kernel is
+    memory-bound, unlike the Roe one)"""
+    import copy
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    a = copy.copy(args)
+    a.steps, a.warmup = min(args.steps, 5), min(args.warmup, 2)
+    prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, mach=0.5, alpha=2.0, distype="c")
+    geoms = [level_geometry(s, bdy) for s in levels]
+    out = {"workload": f"sphere {args.n}^3, M=0.5, alpha=2, {args.levels}-level W-cycle, JST (k2=0.5, k4=1/32)"}
+    for precision in (64, 32):
+        print(f"[bench] JST leg FP{precision}", file=sys.stderr, flush=True)
+        leg = run_leg(prm, levels, geoms, aref, a, precision)
+        out[f"fp{precision}"] = {"value": round(leg["value"], 2), "unit": "Mcell-stage/s",
+                                 "ms_per_step": round(leg["dt"] / a.steps * 1e3, 3),
+                                 "kernels": leg["kernels"], "cd": round(leg["cd"], 6)}
+    return out
 
 
 def sweep_leg(k, drop=1e5, maxcyc=3000):
@@ -193,7 +240,8 @@ def traffic_probe(args, precision):
     y.copy_(x)
     torch.cuda.synchronize()
     del x, y
-    prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, mach=0.5, alpha=2.0, distype="r2")
+    prm, levels, bdy, aref = configs.sphere3d(n=args.n, nlevel=args.levels, mach=0.5, alpha=2.0,
+                                              distype=args.probe_scheme)
     geoms = [level_geometry(s, bdy) for s in levels]
     with Case(dataclasses.replace(prm, precision=precision), levels, geoms=geoms, mxcyc=4) as c:
         c.init_flow()
@@ -301,12 +349,15 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=2, help="cycles of the HIP-event roofline pass")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
+    ap.add_argument("--no-jst", action="store_true", help="skip the JST sphere leg")
     ap.add_argument("--ns-leg", type=int, default=0, help="W-cycles of the NS cylinder leg (0: skip)")
     ap.add_argument("--converge", type=int, default=0,
                     help="run the workload to its residual drop, at most this many cycles (0: skip)")
     ap.add_argument("--serve-leg", type=int, default=0, help="requests of the Poisson serving leg (0: skip)")
     ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
                     help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
+    ap.add_argument("--probe-scheme", default="r2", choices=("r2", "c"),
+                    help="scheme of the --traffic-probe cycle: r2 (Roe/MUSCL, the headline) or c (JST)")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe(args, args.traffic_probe)
@@ -332,10 +383,11 @@ def main():
         fp32 = {"value": round(leg["value"], 2), "unit": "Mcell-stage/s", "dtype": "f32",
                 "ms_per_step": round(leg["dt"] / args.steps * 1e3, 3), "roofline": leg["roofline"],
                 "cd": round(leg["cd"], 6), "cl": round(leg["cl"], 6),
-                "speedup_vs_fp64": round(leg["value"] / main_leg["value"], 3)}
+                "speedup_vs_fp64": round(leg["value"] / main_leg["value"], 3), "kernels": leg["kernels"]}
     value, dt, cs = main_leg["value"], main_leg["dt"], main_leg["cs"]
     cd, cl, roofline = main_leg["cd"], main_leg["cl"], main_leg["roofline"]
 
+    jst = jst_leg(args) if not args.no_jst else None
     solves = sweep_leg(args.sweep) if args.sweep > 0 else None
     ns = ns_leg(args.ns_leg) if args.ns_leg > 0 else None
     conv = converge_leg(prm, levels, geoms, aref, args.converge) if args.converge > 0 else None
@@ -366,9 +418,11 @@ def main():
                    "n": args.n, "levels": args.levels, "cell_stages_per_step": cs // args.steps,
                    "cd": round(cd, 6), "cl": round(cl, 6), "stage_kchunk_per_level": main_leg["kc"]},
         "roofline": roofline,
+        "kernels": main_leg["kernels"],
         "cpu_baseline": cpu,
         "fp32": fp32,
         "solves": solves,
+        "jst": jst,
     }
     if ns is not None:
         out["ns_cylinder"] = ns
