@@ -349,6 +349,16 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
       pvd = A.vdtim[qk];
       pfl = A.flags[qk];
     }
+    // the radii sums of the dissipation faces: i-1, i, i+1 and j-1, j+1
+    T prad[5] = {};
+    if (PRE && valid && diss) {
+      const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
+      prad[0] = rad_at<0>(c, brow, i - 1);
+      prad[1] = rad_at<0>(c, brow, i);
+      prad[2] = rad_at<0>(c, brow, i + 1);
+      prad[3] = rad_at<1>(c, bcol, j - 1);
+      prad[4] = rad_at<1>(c, bcol, j + 1);
+    }
     // ---- fill the plane's i/j quantities (cells within the arrays)
     {
       // own cell
@@ -443,7 +453,8 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
         T da2, da4, dal, db2, db4, dbl;
         const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
         {
-          T r_m = rad_at<0>(c, brow, i - 1), r_0 = rad_at<0>(c, brow, i), r_p = rad_at<0>(c, brow, i + 1);
+          T r_m = PRE ? prad[0] : rad_at<0>(c, brow, i - 1), r_0 = PRE ? prad[1] : rad_at<0>(c, brow, i),
+            r_p = PRE ? prad[2] : rad_at<0>(c, brow, i + 1);
           face_coef(c, sm.px[ty][tx], sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], r_m, r_0, &da2, &da4,
                     &dal);
           face_coef(c, sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.px[ty][tx + 4], r_0, r_p, &db2,
@@ -457,7 +468,8 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
           }
         }
         {
-          T r_m = rad_at<1>(c, bcol, j - 1), r_0 = rad_at<1>(c, bcol, j), r_p = rad_at<1>(c, bcol, j + 1);
+          T r_m = PRE ? prad[3] : rad_at<1>(c, bcol, j - 1), r_0 = PRE ? prad[1] : rad_at<1>(c, bcol, j),
+            r_p = PRE ? prad[4] : rad_at<1>(c, bcol, j + 1);
           face_coef(c, sm.py[ty][tx], sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], r_m, r_0, &da2, &da4,
                     &dal);
           face_coef(c, sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.py[ty + 4][tx], r_0, r_p, &db2,
