@@ -2,10 +2,10 @@
 // the Jameson-RK stage, templated on the bulk type T (double or float).
 //
 // Each helper restates one piece of the reference with the same expression
-// tree (the library is compiled with -ffp-contract=off); the quotients of
-// the convective flux and of the pressure switch are products with a
-// reciprocal refined to within an ulp, so with T = double the GPU path
-// agrees with the reference arithmetic to rounding:
+// tree (the library is compiled with -ffp-contract=off); in FP64 the
+// quotients of the convective flux and of the pressure switch are products
+// with a reciprocal refined to within an ulp, so with T = double the GPU
+// path agrees with the reference arithmetic to rounding:
 //   eos           depvars.f90:37-52
 //   cflux         ceflux.f90:1-25 with the face vector of one direction
 //   fp_at/diw_at  dflux.f90:70-86 (pressure switch, enthalpy differences)
@@ -252,16 +252,16 @@ __device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
   if constexpr (kDelta<T>) {
     const T r0 = f.w[0];
     T f1i = f.w[1 + DIR] * S, f1d = v[1 + DIR] * S;
-    const T rden = rcp_pos((r0 + v[0]) * r0);
+    T den = (r0 + v[0]) * r0;
     fs[0] = f1d;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       T mi = f.w[1 + c], md = v[1 + c];
-      fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) * rden;
+      fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) / den;
     }
     fs[1 + DIR] = fs[1 + DIR] + p * S;
     T hi = f.w[4] + f.p, hd = v[4] + p;
-    fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) * rden;
+    fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) / den;
   } else {
     cflux<DIR>(v, p, S, fs);
   }
@@ -370,7 +370,11 @@ __device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
   }
   if constexpr (kDelta<T>) pb = pb + T(4) * c.inf.p;
   pb = fabs(pb);
-  T v = num * rcp_pos(pb);   // pb = 0 (never for physical pressures) is selected away
+  // FP64: times the refined reciprocal (pb = 0, never for physical
+  // pressures, is selected away); FP32 divides
+  T v;
+  if constexpr (kDelta<T>) v = num / pb;
+  else v = num * rcp_pos(pb);
   v = fabs(pb) > T(0) ? v : T(0);
   return (s <= 0 || s >= nb) ? T(0) : v;
 }
