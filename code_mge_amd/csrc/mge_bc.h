@@ -625,7 +625,20 @@ struct XferArgs {
   int fmg;               // mgl-1 == mlvl: interpolate the solution itself
   const uint8_t *flagsf;
   T *wfo;                // fine state (prolongation, in place)
+  T *corr;               // coarse correction w - ws (5 components, coarse cells)
 };
+
+// the coarse-level correction w - ws of every coarse cell, formed once so
+// that the prolongation reads one value per coarse neighbour and component
+// instead of two (addw.f90:369-438 forms the same difference per use)
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_corr_body(const XferArgs<T> &X, int bx, int, int) {
+  const long q = bx * (long)blockDim.x + threadIdx.x;
+  const long E = X.dc.E;
+  if (q >= 5 * E) return;
+  X.corr[q] = X.wc[q] - X.wsc[q];
+}
+MGE_ENTRY_T(k_corr, XferArgs, 256, 1)
 
 template <typename T>
 __device__ inline void rstencil(int I, int r, int *s, T *f) {
@@ -694,13 +707,10 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   if (!(X.flagsf[qf] & F_FLUID)) return;
   long qc[8];
   for (int c8 = 0; c8 < 8; c8++) qc[c8] = ix(c, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+  const T *src = X.fmg ? X.wc : X.corr;
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
-    for (int c8 = 0; c8 < 8; c8++) {
-      T val = X.wc[n * c.E + qc[c8]];
-      if (!X.fmg) val = val - X.wsc[n * c.E + qc[c8]];
-      acc = acc + val * fac[c8];
-    }
+    for (int c8 = 0; c8 < 8; c8++) acc = acc + src[n * c.E + qc[c8]] * fac[c8];
     if (X.fmg) X.wfo[n * f.E + qf] = acc;
     else X.wfo[n * f.E + qf] = X.wfo[n * f.E + qf] + acc;
   }

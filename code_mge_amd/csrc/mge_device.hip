@@ -80,6 +80,7 @@ struct Level {
   void *ws = nullptr, *wr = nullptr, *qf = nullptr, *dw = nullptr, *fw = nullptr;
   void *radsum = nullptr, *vdtim = nullptr;
   void *w1 = nullptr, *w2 = nullptr;  // unsteady: physical time levels n, n-1 (5E each)
+  void *corr = nullptr;               // levels below the finest: correction w - ws for movfin (5E)
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
   void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z each npiv
@@ -671,7 +672,7 @@ template <class T>
 int do_movco(mge_case *h, int mgl) {
   Level &c = h->lv[mgl], &f = h->lv[mgl + 1];
   if (!levels_nest(c, f)) return set_err(MGE_ERR_ARG, "levels do not nest with the given ratio");
-  XferArgs<T> X;
+  XferArgs<T> X{};
   X.dc = c.d;
   X.df = f.d;
   X.rx = f.ratio[0]; X.ry = f.ratio[1]; X.rz = f.ratio[2];
@@ -695,7 +696,7 @@ template <class T>
 int do_movfin(mge_case *h, int mgl, int mlvl) {
   Level &f = h->lv[mgl], &c = h->lv[mgl - 1];
   if (!levels_nest(c, f)) return set_err(MGE_ERR_ARG, "levels do not nest with the given ratio");
-  XferArgs<T> X;
+  XferArgs<T> X{};
   X.dc = c.d;
   X.df = f.d;
   X.rx = f.ratio[0]; X.ry = f.ratio[1]; X.rz = f.ratio[2];
@@ -710,6 +711,8 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.fmg = (mgl - 1 == mlvl);
   X.flagsf = f.flags;
   X.wfo = tp<T>(f.S[f.cur]);
+  X.corr = tp<T>(c.corr);
+  if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, flat_grid(5 * c.d.E), dim3(256), X);
   LAUNCHT(h, "k_prolong", k_prolong, T, cell_grid(f.d), cell_block(), X);
   LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(f.d.E), dim3(256), (EosArgs<T>{f.d, stm<T>(f, f.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
   return do_bcond<T>(h, f, f.cur);
@@ -1505,6 +1508,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
     CHKRC(dalloc_bytes(L, &L.w1, es * 5 * E));
     CHKRC(dalloc_bytes(L, &L.w2, es * 5 * E));
   }
+  if (lvl < h->nlevel) CHKRC(dalloc_bytes(L, &L.corr, es * 5 * E));
   // vertex differences (ghost entries of index 0 stay unused)
   std::vector<double> hx(nmax, 0.0), hy(nmax, 0.0), hz(nmax, 0.0);
   for (int i = 1; i < d.ib; i++) hx[i] = xv[i + 1] - xv[i];

@@ -58,9 +58,10 @@ class Result:
         return self.completed - self.arrival
 
 
-def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False):
+def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False):
     """device bytes of one level as the library allocates them (every array
-    padded by 64 bytes); wall-record arrays at their upper bound"""
+    padded by 64 bytes); wall-record arrays at their upper bound; a level
+    below the finest also holds the prolongation's correction"""
     ni, nj, nk = n
     E = (ni + 4) * (nj + 4) * (nk + 4)
     nmax = max(ni, nj, nk) + 5
@@ -69,6 +70,8 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False):
     sizes += [es * nmax] * 3 + [es * 6 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
     if viscous:
         sizes += [es * 4 * E]           # velocity and temperature
+    if coarse:
+        sizes += [es * 5 * E]           # correction w - ws
     if nmir:
         ntouch, nops = 2 * nmir, 6 * nmir
         sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
@@ -100,7 +103,8 @@ class Server:
         total = 0
         for li, spec in enumerate(req.levels):
             nmir = geoms[li].nmir if geoms is not None else 0
-            total += level_bytes(spec.n, es, nmir, viscous=req.params.eqntype.lower() == "n")
+            total += level_bytes(spec.n, es, nmir, viscous=req.params.eqntype.lower() == "n",
+                                 coarse=li < len(req.levels) - 1)
         return total
 
     def run(self, requests: List[Request], clock=time.monotonic) -> List[Result]:
