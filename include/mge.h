@@ -13,6 +13,10 @@
  *   mge_movfin     <- movfin(blk, mgl, mlvl, nmesh, mblks)     srcf90/movfin.f90:1
  *   mge_init_flow  <- init_flow(blk, lrflow, imodel)           srcf90/init_flow.f90:1
  *   mge_solve      <- solver(blk, project, job, model)         srcf90/solver.f90:1
+ *   mge_solve_step, mge_shift_time_levels
+ *                  <- solver's physical time loop              srcf90/solver.f90:110-290
+ *   mge_upload_solut <- upload_solut(blk, p_tot, atcellcenter) srcf90/upload_solut.f90:97
+ *   mge_write_solut/mge_read_solut <- write_solut/read_solut   srcf90/wsolut.f90:47, rsolut.f90:1
  *   mge_level_define/mge_case_create  <- make_flow + init_metric
  *                                       (srcf90/make_flow.f90:1, init_metric.f90:1)
  *   mge_geom_level <- the cut-cell part of make_fsijob + solid_ipolcoef
