@@ -567,10 +567,11 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     }
     {
       const int kc = stage_kc(L);
-      dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
+      const int tj = P.distype != 0 ? TJ : Stage2Tile<T>::TJ;  // tile rows of the LDS-tiled kernel
+      dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
       A.kc = kc;
-      if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, TJ), A);
-      else if (P.dissip == 0) LAUNCHT(h, "k_stage", k_stage2, T, g, dim3(TI, TJ), A);
+      if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, tj), A);
+      else if (P.dissip == 0) LAUNCHT(h, "k_stage", k_stage2, T, g, dim3(TI, tj), A);
       else LAUNCHT(h, "k_stage", k_stage, T, cell_grid(d), cell_block(), A);  // matrix dissipation
     }
     {
@@ -902,6 +903,7 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   StageArgs<T> A = stage_args<T>(h, L, 1, w0, dst, w0, 1, 1, 0, 0);
   A.fw_in = 0;
   const void *fn = P.distype != 0 ? (const void *)&k_stage_roe<T> : (const void *)&k_stage2<T>;
+  const int tj = P.distype != 0 ? TJ : Stage2Tile<T>::TJ;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -910,12 +912,12 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   hipError_t err = hipSuccess;
   for (int kc = 1;; kc = std::min(2 * kc, d.nk)) {
     A.kc = kc;
-    dim3 g((d.ni + TI - 1) / TI, (d.nj + TJ - 1) / TJ, (d.nk + kc - 1) / kc);
+    dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
     void *args[1] = {(void *)&A};
     double t = 1e300;
     for (int rep = 0; rep < 4 && err == hipSuccess; rep++) {
       err = hipEventRecord(e0, h->stream);
-      if (err == hipSuccess) err = hipLaunchKernel(fn, g, dim3(TI, TJ), args, 0, h->stream);
+      if (err == hipSuccess) err = hipLaunchKernel(fn, g, dim3(TI, tj), args, 0, h->stream);
       if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
       if (err == hipSuccess) err = hipEventSynchronize(e1);
       float ms = 0.f;

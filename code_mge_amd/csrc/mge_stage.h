@@ -262,14 +262,23 @@ MGE_ENTRY_T(k_stage, StageArgs, 256, 1)
 // tree as the reference, so results equal k_stage's bit for bit.
 constexpr int TI = 64, TJ = 4;
 
+// rows of the JST tile: FP64 uses 8 (512 threads, 107 KB LDS, one workgroup
+// per CU -- the same two waves per SIMD its registers allow anyway, with half
+// the y-halo work per cell); FP32 keeps 4 rows so that three waves per SIMD
+// stay resident
 template <typename T>
+struct Stage2Tile {
+  static constexpr int TJ = kDelta<T> ? 4 : 8;
+};
+
+template <typename T, int TJ_>
 struct StageSmem {
-  T fx[5][TJ][TI + 2];   // F_x at i = i0-1 .. i0+TI
-  T fy[5][TJ + 2][TI];   // F_y at j = j0-1 .. j0+TJ
-  T px[TJ][TI + 4];      // fp_x at i = i0-2 .. i0+TI+1
-  T py[TJ + 4][TI];      // fp_y at j = j0-2 .. j0+TJ+1
-  T dx[5][TJ][TI + 3];   // diw_x at i = i0-1 .. i0+TI+1
-  T dy[5][TJ + 3][TI];   // diw_y at j = j0-1 .. j0+TJ+1
+  T fx[5][TJ_][TI + 2];   // F_x at i = i0-1 .. i0+TI
+  T fy[5][TJ_ + 2][TI];   // F_y at j = j0-1 .. j0+TJ
+  T px[TJ_][TI + 4];      // fp_x at i = i0-2 .. i0+TI+1
+  T py[TJ_ + 4][TI];      // fp_y at j = j0-2 .. j0+TJ+1
+  T dx[5][TJ_][TI + 3];   // diw_x at i = i0-1 .. i0+TI+1
+  T dy[5][TJ_ + 3][TI];   // diw_y at j = j0-1 .. j0+TJ+1
 };
 
 // JST face coefficients from the switches fp(a-2..a+1) and the radii sums
@@ -300,14 +309,15 @@ __device__ inline void cell_flux(const FluxCtx<T> &c, int i, int j, int k, T *o)
 
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageArgs<T> &A, int gbx, int gby, int gbz) {
-  __shared__ StageSmem<T> sm;
+  constexpr int TJs = Stage2Tile<T>::TJ;
+  __shared__ StageSmem<T, TJs> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
   const long E = d.E;
   const int kc = A.kc;
   const T two = T(2), half = T(0.5);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
-  const int i0 = 2 + gbx * TI, j0 = 2 + gby * TJ;
+  const int i0 = 2 + gbx * TI, j0 = 2 + gby * TJs;
   const int kb0 = 2 + gbz * kc;
   const int kb1 = min(kb0 + kc - 1, d.kl);
   const int i = i0 + tx, j = j0 + ty;
@@ -379,51 +389,51 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
         }
       }
       // halo items
-      const int nF = 2 * TJ + 2 * TI;
-      const int nP = diss ? 4 * TJ + 4 * TI : 0;
-      const int nD = diss ? 3 * TJ + 3 * TI : 0;
-      for (int t = tid; t < nF + nP + nD; t += TI * TJ) {
+      const int nF = 2 * TJs + 2 * TI;
+      const int nP = diss ? 4 * TJs + 4 * TI : 0;
+      const int nD = diss ? 3 * TJs + 3 * TI : 0;
+      for (int t = tid; t < nF + nP + nD; t += TI * TJs) {
         T o[5];
-        if (t < 2 * TJ) {  // F_x at i0-1, i0+TI
-          int r = t % TJ, side = t / TJ;
+        if (t < 2 * TJs) {  // F_x at i0-1, i0+TI
+          int r = t % TJs, side = t / TJs;
           int ii = side ? i0 + TI : i0 - 1, jj = j0 + r, col = side ? TI + 1 : 0;
           if (ii <= d.ib && jj <= d.jb) {
             cell_flux<0>(c, ii, jj, k, o);
             for (int n = 0; n < 5; n++) sm.fx[n][r][col] = o[n];
           }
-        } else if (t < nF) {  // F_y at j0-1, j0+TJ
-          int u = t - 2 * TJ;
+        } else if (t < nF) {  // F_y at j0-1, j0+TJs
+          int u = t - 2 * TJs;
           int col = u % TI, side = u / TI;
-          int ii = i0 + col, jj = side ? j0 + TJ : j0 - 1, row = side ? TJ + 1 : 0;
+          int ii = i0 + col, jj = side ? j0 + TJs : j0 - 1, row = side ? TJs + 1 : 0;
           if (ii <= d.ib && jj <= d.jb) {
             cell_flux<1>(c, ii, jj, k, o);
             for (int n = 0; n < 5; n++) sm.fy[n][row][col] = o[n];
           }
-        } else if (t < nF + 4 * TJ) {  // fp_x at i0-2, i0-1, i0+TI, i0+TI+1
+        } else if (t < nF + 4 * TJs) {  // fp_x at i0-2, i0-1, i0+TI, i0+TI+1
           int u = t - nF;
-          int r = u % TJ, w4 = u / TJ;
+          int r = u % TJs, w4 = u / TJs;
           int off = w4 < 2 ? w4 - 2 : TI + (w4 - 2);
           int ii = i0 + off, jj = j0 + r, col = off + 2;
           if (jj <= d.jb) sm.px[r][col] = (ii >= 0 && ii <= d.ib) ? fp_at<0>(c, ix(d, 0, jj, k), ii) : T(0);
-        } else if (t < nF + nP) {  // fp_y at j0-2, j0-1, j0+TJ, j0+TJ+1
-          int u = t - nF - 4 * TJ;
+        } else if (t < nF + nP) {  // fp_y at j0-2, j0-1, j0+TJs, j0+TJs+1
+          int u = t - nF - 4 * TJs;
           int col = u % TI, w4 = u / TI;
-          int off = w4 < 2 ? w4 - 2 : TJ + (w4 - 2);
+          int off = w4 < 2 ? w4 - 2 : TJs + (w4 - 2);
           int ii = i0 + col, jj = j0 + off, row = off + 2;
           if (ii <= d.ib) sm.py[row][col] = (jj >= 0 && jj <= d.jb) ? fp_at<1>(c, ix(d, ii, 0, k), jj) : T(0);
-        } else if (t < nF + nP + 3 * TJ) {  // diw_x at i0-1, i0+TI, i0+TI+1
+        } else if (t < nF + nP + 3 * TJs) {  // diw_x at i0-1, i0+TI, i0+TI+1
           int u = t - nF - nP;
-          int r = u % TJ, w3 = u / TJ;
+          int r = u % TJs, w3 = u / TJs;
           int off = w3 == 0 ? -1 : TI + (w3 - 1);
           int ii = i0 + off, jj = j0 + r, col = off + 1;
           if (jj <= d.jb && ii <= d.ib) {
             diw_at<0>(c, ix(d, 0, jj, k), ii, o);
             for (int n = 0; n < 5; n++) sm.dx[n][r][col] = o[n];
           }
-        } else {  // diw_y at j0-1, j0+TJ, j0+TJ+1
-          int u = t - nF - nP - 3 * TJ;
+        } else {  // diw_y at j0-1, j0+TJs, j0+TJs+1
+          int u = t - nF - nP - 3 * TJs;
           int col = u % TI, w3 = u / TI;
-          int off = w3 == 0 ? -1 : TJ + (w3 - 1);
+          int off = w3 == 0 ? -1 : TJs + (w3 - 1);
           int ii = i0 + col, jj = j0 + off, row = off + 1;
           if (ii <= d.ib && jj <= d.jb) {
             diw_at<1>(c, ix(d, ii, 0, k), jj, o);
@@ -544,7 +554,7 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     __syncthreads();
   }
 }
-MGE_ENTRY_T(k_stage2, StageArgs, 256, 2)
+MGE_ENTRY_T(k_stage2, StageArgs, 512, 1)
 
 // ------------------------------------------------------------------ wall records
 
