@@ -567,7 +567,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     }
     {
       const int kc = stage_kc(L);
-      const int tj = P.distype != 0 ? TJ : Stage2Tile<T>::TJ;  // tile rows of the LDS-tiled kernel
+      const int tj = P.distype != 0 ? RoeTile<T>::TJ : Stage2Tile<T>::TJ;  // tile rows of the LDS-tiled kernel
       dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
       A.kc = kc;
       if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, tj), A);
@@ -903,7 +903,7 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   StageArgs<T> A = stage_args<T>(h, L, 1, w0, dst, w0, 1, 1, 0, 0);
   A.fw_in = 0;
   const void *fn = P.distype != 0 ? (const void *)&k_stage_roe<T> : (const void *)&k_stage2<T>;
-  const int tj = P.distype != 0 ? TJ : Stage2Tile<T>::TJ;
+  const int tj = P.distype != 0 ? RoeTile<T>::TJ : Stage2Tile<T>::TJ;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));

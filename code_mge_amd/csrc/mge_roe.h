@@ -283,13 +283,22 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 // converted to primitive variables in LDS; every i and j face of the tile
 // is evaluated once (each cell its low face, plus the faces at the tile's
 // high edges) into LDS; k faces are carried from plane to plane in
-// registers with the primitives of the cell column k-1..k+2.
+// registers with the primitives of the cell column k-1..k+2.  FP64 uses an
+// 8-row tile (512 threads, 121 KB LDS, one workgroup per CU: the two waves
+// per SIMD its registers allow anyway, with half the halo per cell); FP32
+// keeps 4 rows for its four waves per SIMD.
 template <typename T>
+struct RoeTile {
+  static constexpr int TJ = kDelta<T> ? 4 : 8;
+  static constexpr int MINB = kDelta<T> ? 2 : 1;   // workgroups per CU the launch bounds ask for
+};
+
+template <typename T, int TJ_>
 struct RoeSmem {
-  T q[5][TJ + 4][TI + 4];
-  uint8_t fl[TJ + 4][TI + 4];
-  T xe[5][TJ][TI + 1], xd[5][TJ][TI + 1];
-  T ye[5][TJ + 1][TI], yd[5][TJ + 1][TI];
+  T q[5][TJ_ + 4][TI + 4];
+  uint8_t fl[TJ_ + 4][TI + 4];
+  T xe[5][TJ_][TI + 1], xd[5][TJ_][TI + 1];
+  T ye[5][TJ_ + 1][TI], yd[5][TJ_ + 1][TI];
 };
 
 template <typename T>
@@ -300,8 +309,8 @@ __device__ inline void roe_load(const FluxCtx<T> &c, const uint8_t *flags, long 
 
 // the face between two cells of the LDS tile at (row, col-1) | (row, col)
 // for DIR 0, or (row-1, col) | (row, col) for DIR 1
-template <int DIR, typename T>
-__device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem<T> &sm, const StageArgs<T> &A, int row, int col, T S, T *e,
+template <int DIR, typename T, int TJ_>
+__device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem<T, TJ_> &sm, const StageArgs<T> &A, int row, int col, T S, T *e,
                                     T *dd) {
   const bool diss = A.has_diss;
   bool f[4];
@@ -350,45 +359,47 @@ __device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs
 
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const StageArgs<T> &A, int bx, int by, int bz) {
-  __shared__ RoeSmem<T> sm;
+  constexpr int TJr = RoeTile<T>::TJ;
+  __shared__ RoeSmem<T, TJr> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
   const long E = d.E;
   const int kc = A.kc;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
-  const int i0 = 2 + bx * TI, j0 = 2 + by * TJ;
+  const int i0 = 2 + bx * TI, j0 = 2 + by * TJr;
   const int kb0 = 2 + bz * kc;
   const int kb1 = min(kb0 + kc - 1, d.kl);
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = i <= d.il && j <= d.jl;
   const bool inarr = i <= d.ib && j <= d.jb;
-  static_assert(TJ >= 3, "the high-edge faces are spread over waves 1 and 2");
+  static_assert(TJr >= 3, "the high-edge faces are spread over waves 1 and 2");
   const bool diss = A.has_diss;
   const T Sz = face_s<2>(c.g, min(i, d.ib), min(j, d.jb), 0);  // clamped: read only for valid cells
-  // halo ring items of this thread (t = tid and tid + TI*TJ): their LDS
+  // halo ring items of this thread (t = tid, tid + TI*TJr, ...): their LDS
   // slot and cell do not change with k.  FP64 loads the next plane's items
   // during the current plane's faces (two waves per SIMD hide little
   // latency); FP32 loads them in place
   constexpr bool PRE = !kDelta<T>;
-  constexpr int NH = 4 * (TI + 4) + 4 * TJ;
-  int hr[2] = {0, 0}, hc[2] = {0, 0};
-  long hq[2] = {0, 0};
-  bool hok[2] = {false, false};
-  T hraw[2][5] = {};
-  uint8_t hfl[2] = {0, 0};
+  constexpr int NH = 4 * (TI + 4) + 4 * TJr;
+  constexpr int NS = (NH + TI * TJr - 1) / (TI * TJr);   // halo items per thread (2 for 4 rows, 1 for 8)
+  int hr[NS] = {}, hc[NS] = {};
+  long hq[NS] = {};
+  bool hok[NS] = {};
+  T hraw[NS][5] = {};
+  uint8_t hfl[NS] = {};
   if constexpr (PRE) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int t = tid + h * TI * TJ;
+    for (int h = 0; h < NS; h++) {
+      const int t = tid + h * TI * TJr;
       int r, col;
       if (t < 4 * (TI + 4)) {
         int rr = t / (TI + 4);
         col = t % (TI + 4);
-        r = rr < 2 ? rr : TJ + rr;
+        r = rr < 2 ? rr : TJr + rr;
       } else {
         int u = t - 4 * (TI + 4);
-        int cc = u / TJ;
-        r = 2 + u % TJ;
+        int cc = u / TJr;
+        r = 2 + u % TJr;
         col = cc < 2 ? cc : TI + cc;
       }
       const int ii = i0 - 2 + col, jj = j0 - 2 + r;
@@ -398,7 +409,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       hq[h] = hok[h] ? ix(d, ii, jj, 0) : 0;
     }
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < NS; h++)
       if (hok[h]) {
         const long q = hq[h] + (long)kb0 * d.sz;
 #pragma unroll
@@ -468,7 +479,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     }
     if constexpr (PRE) {
 #pragma unroll
-      for (int h = 0; h < 2; h++)
+      for (int h = 0; h < NS; h++)
         if (hok[h]) {
           T o[5];
           prim_raw(hraw[h], c.inf, o);
@@ -477,16 +488,16 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           sm.fl[hr[h]][hc[h]] = (hfl[h] & F_FLUID) != 0;
         }
     } else {
-      for (int t = tid; t < NH; t += TI * TJ) {
+      for (int t = tid; t < NH; t += TI * TJr) {
         int r, col;
         if (t < 4 * (TI + 4)) {
           int rr = t / (TI + 4);
           col = t % (TI + 4);
-          r = rr < 2 ? rr : TJ + rr;
+          r = rr < 2 ? rr : TJr + rr;
         } else {
           int u = t - 4 * (TI + 4);
-          int cc = u / TJ;
-          r = 2 + u % TJ;
+          int cc = u / TJr;
+          r = 2 + u % TJr;
           col = cc < 2 ? cc : TI + cc;
         }
         int ii = i0 - 2 + col, jj = j0 - 2 + r;
@@ -504,7 +515,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     // FP64: the next plane's halo items, in flight during this plane's faces
     if constexpr (PRE) if (k < kb1) {
 #pragma unroll
-      for (int h = 0; h < 2; h++)
+      for (int h = 0; h < NS; h++)
         if (hok[h]) {
           const long q = hq[h] + (long)(k + 1) * d.sz;
 #pragma unroll
@@ -534,7 +545,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       }
       // the tile's high-edge faces go to two different waves (x edge:
       // wave 2, y edge: wave 1) so no wave evaluates more than one extra face
-      if (tid >= 2 * TI && tid < 2 * TI + TJ) {
+      if (tid >= 2 * TI && tid < 2 * TI + TJr) {
         int jj = j0 + tid - 2 * TI, a = i0 + TI;
         if (jj <= d.jl && a <= d.ie) {
           roe_lds_face<0>(sm, A, tid - 2 * TI + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd);
@@ -545,13 +556,13 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           }
         }
       } else if (tid >= TI && tid < 2 * TI) {
-        int cidx = tid - TI, ii = i0 + cidx, a = j0 + TJ;
+        int cidx = tid - TI, ii = i0 + cidx, a = j0 + TJr;
         if (ii <= d.il && a <= d.je) {
-          roe_lds_face<1>(sm, A, TJ + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd);
+          roe_lds_face<1>(sm, A, TJr + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd);
 #pragma unroll
           for (int n = 0; n < 5; n++) {
-            sm.ye[n][TJ][cidx] = e[n];
-            sm.yd[n][TJ][cidx] = dd[n];
+            sm.ye[n][TJr][cidx] = e[n];
+            sm.yd[n][TJr][cidx] = dd[n];
           }
         }
       }
@@ -614,6 +625,21 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     }
   }
 }
-MGE_ENTRY_T(k_stage_roe, StageArgs, 256, 2)
+// entry points as MGE_ENTRY_T makes them, with launch bounds that follow the
+// precision's tile
+template <typename T>
+__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINB) k_stage_roe(StageArgs<T> A) {
+  k_stage_roe_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+template <typename T>
+__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINB)
+    k_stage_roe_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
+  const int b = (int)blockIdx.x;
+  const int c = batch_case(cg, nc, b);
+  const int4 g = cg[c];
+  if (!act[g.w]) return;
+  const int l = b - g.x;
+  k_stage_roe_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+}
 
 }  // namespace mge
