@@ -286,11 +286,12 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 // registers with the primitives of the cell column k-1..k+2.  FP64 uses an
 // 8-row tile (512 threads, 121 KB LDS, one workgroup per CU: the two waves
 // per SIMD its registers allow anyway, with half the halo per cell); FP32
-// keeps 4 rows for its four waves per SIMD.
+// keeps 4 rows (8 rows would need more than the 128 registers of its four
+// waves per SIMD).
 template <typename T>
 struct RoeTile {
   static constexpr int TJ = kDelta<T> ? 4 : 8;
-  static constexpr int MINB = kDelta<T> ? 2 : 1;   // workgroups per CU the launch bounds ask for
+  static constexpr int MINW = 2;   // waves per SIMD the launch bounds ask for
 };
 
 template <typename T, int TJ_>
@@ -628,11 +629,11 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 // entry points as MGE_ENTRY_T makes them, with launch bounds that follow the
 // precision's tile
 template <typename T>
-__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINB) k_stage_roe(StageArgs<T> A) {
+__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW) k_stage_roe(StageArgs<T> A) {
   k_stage_roe_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 template <typename T>
-__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINB)
+__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW)
     k_stage_roe_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
   const int b = (int)blockIdx.x;
   const int c = batch_case(cg, nc, b);

@@ -269,6 +269,7 @@ constexpr int TI = 64, TJ = 4;
 template <typename T>
 struct Stage2Tile {
   static constexpr int TJ = kDelta<T> ? 4 : 8;
+  static constexpr int MINW = 2;   // waves per SIMD the launch bounds ask for
 };
 
 template <typename T, int TJ_>
@@ -554,7 +555,22 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     __syncthreads();
   }
 }
-MGE_ENTRY_T(k_stage2, StageArgs, 512, 1)
+// entry points as MGE_ENTRY_T makes them, with launch bounds that follow the
+// precision's tile
+template <typename T>
+__global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW) k_stage2(StageArgs<T> A) {
+  k_stage2_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+template <typename T>
+__global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW)
+    k_stage2_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
+  const int b = (int)blockIdx.x;
+  const int c = batch_case(cg, nc, b);
+  const int4 g = cg[c];
+  if (!act[g.w]) return;
+  const int l = b - g.x;
+  k_stage2_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+}
 
 // ------------------------------------------------------------------ wall records
 
