@@ -152,10 +152,9 @@ def finest_kernels(case, n, lvl, precision):
 
 
 def jst_leg(args):
-    """The same sphere with the reference's own JST scheme (cfg2's grid,
-    levels and flow; distype c): Mcell-stage/s in FP64 and FP32 and the
-    finest level's per-kernel HBM fractions (the JST stage kernel is
-    memory-bound, unlike the Roe one)"""
+    """The same sphere with the reference's own JST scheme (the 256^3 grid,
+    levels and flow of the headline legs; distype c): Mcell-stage/s in FP64
+    and FP32 and the finest level's per-kernel HBM fractions"""
     import copy
     from code_mge_amd import configs
     from code_mge_amd.grid import level_geometry

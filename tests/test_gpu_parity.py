@@ -2,7 +2,7 @@
 inputs).  Single calls are compared at 1e-13 relative (the two sides
 evaluate the same expression trees; only pow() in the far-field boundary
 and the residual-norm summation order differ); multi-cycle solves at the
-north-star tolerance of 1e-10 per cell."""
+stated tolerance of 1e-10 relative per cell after 50 cycles."""
 import numpy as np
 import pytest
 
