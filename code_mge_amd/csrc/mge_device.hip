@@ -83,7 +83,7 @@ struct Level {
   void *corr = nullptr;               // levels below the finest: correction w - ws for movfin (5E)
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z each npiv
+  void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z,1/e1x,1/e1y,1/e1z each npiv
   void *fax = nullptr;  // -bb/e1x
   int npiv = 0;
   int nmir = 0, nupd = 0, zero_at = -1;
@@ -603,6 +603,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       S.e1 = piv;
       S.e2 = piv + L.npiv;
       S.fa = tp<T>(L.fax);
+      S.r1 = piv + 6 * L.npiv;
       long nwave = (long)d.nj * d.nk * 5;
       dim3 gx((unsigned)((nwave * 64 + 255) / 256));
 #define XSCAN(C) launch(h, "k_smooth_x", (const void *)&k_smooth_x_scan<C, T>, (const void *)&k_smooth_x_scan_b<C, T>, gx, dim3(256), S)
@@ -614,9 +615,11 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
 #undef XSCAN
       S.e1 = piv + 2 * L.npiv;
       S.e2 = piv + 3 * L.npiv;
+      S.r1 = piv + 7 * L.npiv;
       LAUNCHT(h, "k_smooth_y", k_smooth_y, T, dim3((d.ni + 63) / 64, d.nk, 5), dim3(64), S);
       S.e1 = piv + 4 * L.npiv;
       S.e2 = piv + 5 * L.npiv;
+      S.r1 = piv + 8 * L.npiv;
       LAUNCHT(h, "k_smooth_z", k_smooth_z, T, dim3((d.ni + 63) / 64, d.nj, 5), dim3(64), S);
       UpdArgs<T> U{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
                    (T)P.gamma, make_inf<T>(P)};
@@ -1496,7 +1499,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc_bytes(L, &L.dy, es * nmax));
   CHKRC(dalloc_bytes(L, &L.dz, es * nmax));
   L.npiv = nmax;
-  CHKRC(dalloc_bytes(L, &L.piv, es * 6 * nmax));
+  CHKRC(dalloc_bytes(L, &L.piv, es * 9 * nmax));
   CHKRC(dalloc_bytes(L, &L.fax, es * nmax));
   dim3 g = cell_grid(d);
   L.npartial = (long)g.x * g.y * g.z;
@@ -1520,7 +1523,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(put_bulk(h, L.dy, hy.data(), nmax));
   CHKRC(put_bulk(h, L.dz, hz.data(), nmax));
   // smoothing pivots (psmoo.f90:134-166), computed once on the host in FP64
-  std::vector<double> piv((size_t)6 * nmax, 0.0);
+  std::vector<double> piv((size_t)9 * nmax, 0.0);
   if (h->prm.smoo > 0.0) {
     double aa = 1.0 + 2.0 * h->prm.smoo, bb = -h->prm.smoo;
     int nn[3] = {ni, nj, nk};
@@ -1535,6 +1538,8 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
         e2[s] = bb / e1[s];
       }
       e1[l] = aa - bb * e2[lm];
+      double *r1 = &piv[(size_t)(6 + a) * nmax];
+      for (int s = 2; s <= l; s++) r1[s] = 1.0 / e1[s];
     }
   }
   CHKRC(put_bulk(h, L.piv, piv.data(), piv.size()));

@@ -18,8 +18,20 @@ struct SmoothArgs {
   const uint8_t *flags;
   const T *e1, *e2;  // pivots of this direction (psmoo.f90:134-166)
   const T *fa;       // -bb/e1 (x lines: affine-map form of the forward step)
+  const T *r1;       // 1/e1, rounded once from the FP64 pivots
   T bb;
 };
+
+// The forward step's division by the pivot e1_s.  FP64 multiplies by the
+// reciprocal pivot instead (one rounding more per cell, a few ulp over a
+// line, and no FP64 division on the recurrence's critical path); FP32
+// divides, as the reference does.
+template <typename T>
+__device__ __attribute__((always_inline)) inline T piv_div(T x, const T *__restrict__ e1,
+                                                           const T *__restrict__ r1, int s) {
+  if constexpr (kDelta<T>) return x / e1[s];
+  else return x * r1[s];
+}
 
 // psmoo_old.f90:171-188 (x lines), one wavefront per (j,k) line and
 // component.  Lane l owns CPL consecutive cells, so loads and stores are
@@ -60,8 +72,8 @@ __device__ __attribute__((always_inline)) inline void smooth_x_scan(const Smooth
     T A, B;
     if (!ok[c]) { A = T(1); B = T(0); }
     else if (!fl[c]) { A = T(0); B = r[c]; }
-    else if (i == 2) { A = T(0); B = r[c] / S.e1[2]; }
-    else { A = S.fa[i]; B = r[c] / S.e1[i]; }
+    else if (i == 2) { A = T(0); B = piv_div(r[c], S.e1, S.r1, 2); }
+    else { A = S.fa[i]; B = piv_div(r[c], S.e1, S.r1, i); }
     Bl = A * Bl + B;
     Al = A * Al;
   }
@@ -81,7 +93,7 @@ __device__ __attribute__((always_inline)) inline void smooth_x_scan(const Smooth
   for (int c = 0; c < CPL; c++) {
     int i = i0 + c;
     T v = r[c];
-    if (ok[c] && fl[c]) v = (i == 2) ? r[c] / S.e1[2] : (r[c] - S.bb * yp) / S.e1[i];
+    if (ok[c] && fl[c]) v = (i == 2) ? piv_div(r[c], S.e1, S.r1, 2) : piv_div(r[c] - S.bb * yp, S.e1, S.r1, i);
     y[c] = v;
     yp = v;
   }
@@ -131,9 +143,9 @@ __device__ __attribute__((always_inline)) inline void smooth_x_seq(const SmoothA
   int j = 2 + (int)(l % d.nj), k = 2 + (int)(l / d.nj);
   T *D = S.dw + n * d.E;
   long b0 = ix(d, 0, j, k);
-  if (S.flags[b0 + 2] & F_FLUID) D[b0 + 2] = D[b0 + 2] / S.e1[2];
+  if (S.flags[b0 + 2] & F_FLUID) D[b0 + 2] = piv_div(D[b0 + 2], S.e1, S.r1, 2);
   for (int i = 3; i <= d.il; i++)
-    if (S.flags[b0 + i] & F_FLUID) D[b0 + i] = (D[b0 + i] - S.bb * D[b0 + i - 1]) / S.e1[i];
+    if (S.flags[b0 + i] & F_FLUID) D[b0 + i] = piv_div(D[b0 + i] - S.bb * D[b0 + i - 1], S.e1, S.r1, i);
   for (int i = 2; i <= d.ni; i++) {
     int ir = d.il + 1 - i;
     if (S.flags[b0 + ir] & F_FLUID) D[b0 + ir] = D[b0 + ir] - S.e2[ir] * D[b0 + ir + 1];
@@ -174,7 +186,8 @@ MGE_ENTRY_T(k_smooth_x_seq, SmoothArgs, 256, 1)
 template <int CH, typename T>
 __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__ D, const uint8_t *__restrict__ F,
                                                                  long b0, long st, int nl, const T *__restrict__ e1,
-                                                                 const T *__restrict__ e2, T bb) {
+                                                                 const T *__restrict__ r1, const T *__restrict__ e2,
+                                                                 T bb) {
   const int ncell = nl - 1;
   const int nfull = ncell / CH;
   T A[CH], B[CH], MA[CH], MB[CH];
@@ -188,7 +201,7 @@ __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__
 #define PR_F(BUF, MK, c)                                              \
   _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
     int s_ = 2 + (c) * CH + t;                                        \
-    y = MK[t] * ((BUF[t] - bb * y) / e1[s_]);                         \
+    y = MK[t] * piv_div(BUF[t] - bb * y, e1, r1, s_);                 \
     D[b0 + (long)s_ * st] = y;                                        \
   }
 #define LD_B(BUF, MK, c)                                              \
@@ -219,7 +232,7 @@ __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__
   for (int s_ = 2 + nfull * CH; s_ <= nl; s_++) {
     long q_ = b0 + (long)s_ * st;
     T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
-    y = m_ * ((D[q_] - bb * y) / e1[s_]);
+    y = m_ * piv_div(D[q_] - bb * y, e1, r1, s_);
     D[q_] = y;
   }
   // backward
@@ -259,7 +272,8 @@ __device__ __attribute__((always_inline)) inline void line_sweep(T *__restrict__
 template <int CH, int MAXC, typename T, typename CK>
 __device__ __attribute__((always_inline)) inline void line_sweep_ck(T *__restrict__ D, const uint8_t *__restrict__ F,
                                                                     long b0, long st, int nl, const T *__restrict__ e1,
-                                                                    const T *__restrict__ e2, T bb, CK ck) {
+                                                                    const T *__restrict__ r1, const T *__restrict__ e2,
+                                                                    T bb, CK ck) {
   const int ncell = nl - 1;
   const int nfull = ncell / CH;
   T A[CH], B[CH], Y[CH];
@@ -278,7 +292,7 @@ __device__ __attribute__((always_inline)) inline void line_sweep_ck(T *__restric
 #define FWD(BUF, MK, c, OUT)                                          \
   _Pragma("unroll") for (int t = 0; t < CH; t++) {                    \
     T m_ = ((MK >> t) & 1u) ? T(1) : T(0);                            \
-    y = m_ * ((BUF[t] - bb * y) / e1[2 + (c) * CH + t]);              \
+    y = m_ * piv_div(BUF[t] - bb * y, e1, r1, 2 + (c) * CH + t);      \
     OUT                                                               \
   }
 #define BWD(MK, c)                                                    \
@@ -314,7 +328,7 @@ __device__ __attribute__((always_inline)) inline void line_sweep_ck(T *__restric
     for (int s_ = s0; s_ <= nl; s_++) {
       long q_ = b0 + (long)s_ * st;
       T m_ = (F[q_] & F_FLUID) ? T(1) : T(0);
-      y = m_ * ((D[q_] - bb * y) / e1[s_]);
+      y = m_ * piv_div(D[q_] - bb * y, e1, r1, s_);
       Yt[s_ - s0] = y;
     }
     for (int s_ = nl; s_ >= s0; s_--) {
@@ -347,12 +361,15 @@ __device__ __attribute__((always_inline)) inline void line_sweep_ck(T *__restric
 #undef BWD
 }
 
-// y/z line: the three-pass sweep when the line fits the checkpoints, else
-// the four-pass one
+// y/z line: the three-pass sweep for lines of up to 256 cells, else the
+// four-pass one.  16 checkpoints of one wave take 8 KB of LDS in FP64 and
+// the four-pass form runs 8-cell chunks, so neither limits the waves per
+// CU below what the three-pass registers allow (measured at 256^3: 4% faster
+// in FP64 and 17% in FP32 than 32 checkpoints and 16-cell fallback chunks).
 template <typename T>
 __device__ __attribute__((always_inline)) inline void smooth_line(T *D, const uint8_t *F, long b0, long st, int nl,
-                                                                  const T *e1, const T *e2, T bb) {
-  constexpr int CH = 16, MAXC = 32;
+                                                                  const T *e1, const T *r1, const T *e2, T bb) {
+  constexpr int CH = 16, MAXC = 16;
   __shared__ T ck[MAXC][64];
   if (nl - 1 <= CH * MAXC && blockDim.x * blockDim.y <= 64) {
     // ck[c][lane]: the lanes of one chunk index sit side by side
@@ -361,9 +378,9 @@ __device__ __attribute__((always_inline)) inline void smooth_line(T *D, const ui
       T *p;
       __device__ T &operator[](int c) const { return p[c * 64]; }
     } col{mine};
-    line_sweep_ck<CH, MAXC>(D, F, b0, st, nl, e1, e2, bb, col);
+    line_sweep_ck<CH, MAXC>(D, F, b0, st, nl, e1, r1, e2, bb, col);
   } else {
-    line_sweep<16>(D, F, b0, st, nl, e1, e2, bb);
+    line_sweep<8>(D, F, b0, st, nl, e1, r1, e2, bb);
   }
 }
 
@@ -375,7 +392,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const Smoo
   int k = 2 + by;
   int n = bz;
   if (i > d.il) return;
-  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.e2, S.bb);
+  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.r1, S.e2, S.bb);
 }
 MGE_ENTRY_T(k_smooth_y, SmoothArgs, 64, 1)
 
@@ -387,7 +404,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_z_body(const Smoo
   int j = 2 + by;
   int n = bz;
   if (i > d.il) return;
-  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.e2, S.bb);
+  smooth_line(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.r1, S.e2, S.bb);
 }
 MGE_ENTRY_T(k_smooth_z, SmoothArgs, 64, 1)
 

@@ -67,7 +67,7 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False):
     nmax = max(ni, nj, nk) + 5
     ngrid = -(-ni // 64) * -(-nj // 4) * nk
     sizes = [es * 6 * E] * 3 + [es * 5 * E] * 5 + [es * E] * 2 + [E] * 2
-    sizes += [es * nmax] * 3 + [es * 6 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
+    sizes += [es * nmax] * 3 + [es * 9 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
     if viscous:
         sizes += [es * 4 * E]           # velocity and temperature
     if coarse:

@@ -73,6 +73,33 @@ def test_symmetry_wall_and_periodic_faces(require_gpu):
     gpu.close()
 
 
+def test_long_smoothing_lines(require_gpu):
+    """y lines of 257 cells (beyond the 256 the checkpointed three-pass
+    sweep takes: the four-pass sweep runs) and z lines of exactly 256 (16
+    full checkpoint chunks, no tail): one smoothed stage against the oracle
+    at 1e-13 relative to each field's largest value"""
+    from tests.gpu_common import max_rel_to_field, perturbed_state
+    n = (8, 257, 256)
+    levels = hierarchy(n, (0.0, 0.0, 0.0), (0.08, 2.57, 2.56), 1, bctype=(FAR,) * 6)
+    prm = Params(mach=0.4, alpha_deg=3.0, beta_deg=1.0, smoo=0.6)
+    geoms = [level_geometry(s, None) for s in levels]
+    orc = OracleCase(prm, geoms, mxcyc=4)
+    with Case(prm, levels, geoms=geoms, mxcyc=4) as gpu:
+        gpu.init_flow()
+        orc.init_flow()
+        w, p = perturbed_state(gpu, 1, prm, seed=11)
+        for c in (gpu, orc):
+            c.upload(1, "w", w)
+            c.upload(1, "p", p)
+            c.bcond(1)
+            c.timestep(1)
+            c.euler(1, 1, 1, 1, 0, 0)
+        for f in ("w", "dw", "p"):
+            e = max_rel_to_field(gpu.download(1, f), orc.download(1, f))
+            print(f"  {f}: {e:.3e}")
+            assert e < 1e-13, f
+
+
 def test_single_cell_direction_with_smoothing_is_an_error(require_gpu):
     """psmoo_old needs two cells per line (psmoo.f90:132,166): the GPU
     reports MGE_ERR_SMOO_DIM, as the restatement does"""

@@ -1,8 +1,11 @@
 """Parity of the HIP path against the CPU oracle (same geometry, same
 inputs).  Single calls are compared at 1e-13 relative (the two sides
-evaluate the same expression trees; only pow() in the far-field boundary
-and the residual-norm summation order differ); multi-cycle solves at the
-stated tolerance of 1e-10 relative per cell after 50 cycles."""
+evaluate the same expression trees, except that the FP64 kernels multiply
+by reciprocals where the restatement divides -- density and the pressure
+switch's denominator in the fluxes, the smoothing pivots -- and that pow()
+in the far-field boundary and the residual-norm summation order differ);
+multi-cycle solves at the stated tolerance of 1e-10 relative per cell after
+50 cycles."""
 import numpy as np
 import pytest
 
