@@ -605,8 +605,10 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       S.fa = tp<T>(L.fax);
       S.r1 = piv + 6 * L.npiv;
       long nwave = (long)d.nj * d.nk * 5;
-      dim3 gx((unsigned)((nwave * 64 + 255) / 256));
-#define XSCAN(C) launch(h, "k_smooth_x", (const void *)&k_smooth_x_scan<C, T>, (const void *)&k_smooth_x_scan_b<C, T>, gx, dim3(256), S)
+      auto gxk = [&](int K) { return dim3((unsigned)(((nwave + K - 1) / K * 64 + 255) / 256)); };
+#define XSCAN(C)                                                                                            \
+  launch(h, "k_smooth_x", (const void *)&k_smooth_x_scan<C, T>, (const void *)&k_smooth_x_scan_b<C, T>,     \
+         gxk(xscan_lines<C>()), dim3(256), S)
       if (d.ni <= 64) XSCAN(1);
       else if (d.ni <= 128) XSCAN(2);
       else if (d.ni <= 256) XSCAN(4);

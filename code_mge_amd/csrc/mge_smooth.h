@@ -33,103 +33,152 @@ __device__ __attribute__((always_inline)) inline T piv_div(T x, const T *__restr
   else return x * r1[s];
 }
 
-// psmoo_old.f90:171-188 (x lines), one wavefront per (j,k) line and
-// component.  Lane l owns CPL consecutive cells, so loads and stores are
-// coalesced.  Both recurrences are first composed as affine maps
-// (forward y_i = A_i y_(i-1) + B_i, backward z_i = C_i z_(i+1) + y_i),
-// scanned across the wave, and then every lane re-evaluates its own cells
-// with the reference formula from the scanned boundary value.  Only the
-// lane-boundary values differ from the sequential evaluation (by a few
-// ulp); blank cells keep their value as in the reference.
-template <int CPL, typename T>
+// psmoo_old.f90:171-188 (x lines): one wavefront per K consecutive (j,k)
+// lines of a component.  Lane l owns CPL consecutive cells of each line, so
+// loads and stores are coalesced.  Both recurrences are first composed as
+// affine maps (forward y_i = A_i y_(i-1) + B_i, backward z_i = C_i z_(i+1)
+// + y_i), scanned across the wave, and then every lane re-evaluates its own
+// cells with the reference formula from the scanned boundary value.  Only
+// the lane-boundary values differ from the sequential evaluation (by a few
+// ulp); blank cells keep their value as in the reference.  The K lines are
+// independent: their loads are in flight together and their shuffles
+// interleave, and the pivots of the lane's cells are loaded once for all K.
+template <int CPL, int K, typename T>
 __device__ __attribute__((always_inline)) inline void smooth_x_scan(const SmoothArgs<T> &S, int bx) {
   const Dims &d = S.d;
   const int lane = threadIdx.x & 63;
-  const long wave = (bx * (long)blockDim.x + threadIdx.x) >> 6;
-  const long nl = (long)d.nj * d.nk;
-  if (wave >= nl * 5) return;  // uniform per wave
-  const int n = (int)(wave / nl);
-  const long l = wave - (long)n * nl;
-  const int j = 2 + (int)(l % d.nj), k = 2 + (int)(l / d.nj);
-  T *D = S.dw + n * d.E;
-  const long b0 = ix(d, 0, j, k);
+  // the wave index is uniform: keep the line arithmetic scalar
+  const int wave = __builtin_amdgcn_readfirstlane((int)((bx * (long)blockDim.x + threadIdx.x) >> 6));
+  const int nl = d.nj * d.nk;
+  const int nlines = nl * 5;
+  const int l0 = wave * K;
+  if (l0 >= nlines) return;  // uniform per wave
   const int il = d.il, ni = d.ni;
   const int i0 = 2 + lane * CPL;
-  T r[CPL];
-  bool fl[CPL], ok[CPL];
+  // pivots: e1 (FP32 divides by it) or 1/e1 (FP64 multiplies), -bb/e1, e2
+  T pv[CPL], pa[CPL], pe2[CPL];
+  bool ok[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; c++) {
-    int i = i0 + c;
+    const int i = i0 + c;
     ok[c] = i <= il;
-    r[c] = ok[c] ? D[b0 + i] : T(0);
-    fl[c] = ok[c] && (S.flags[b0 + i] & F_FLUID);
+    const int ic = ok[c] ? i : 2;
+    pv[c] = kDelta<T> ? S.e1[ic] : S.r1[ic];
+    pa[c] = S.fa[ic];
+    pe2[c] = S.e2[ic];
+  }
+  auto pdiv = [](T x, T p) { if constexpr (kDelta<T>) return x / p; else return x * p; };
+  T r[K][CPL];
+  bool fl[K][CPL], lv[K];
+  long base[K], fbase[K];
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    lv[t] = l0 + t < nlines;
+    const int lc = lv[t] ? l0 + t : l0;  // a missing line repeats the first; it is never stored
+    const int n = lc / nl;
+    const int l = lc - n * nl;
+    const int j = 2 + l % d.nj, k = 2 + l / d.nj;
+    fbase[t] = ix(d, 0, j, k);
+    base[t] = (long)n * d.E + fbase[t];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+      const int i = i0 + c;
+      r[t][c] = ok[c] ? S.dw[base[t] + i] : T(0);
+      fl[t][c] = ok[c] && (S.flags[fbase[t] + i] & F_FLUID);
+    }
   }
   // forward: lane composite of y_i = A_i y_(i-1) + B_i
-  T Al = T(1), Bl = T(0);
+  T Al[K], Bl[K];
 #pragma unroll
-  for (int c = 0; c < CPL; c++) {
-    int i = i0 + c;
-    T A, B;
-    if (!ok[c]) { A = T(1); B = T(0); }
-    else if (!fl[c]) { A = T(0); B = r[c]; }
-    else if (i == 2) { A = T(0); B = piv_div(r[c], S.e1, S.r1, 2); }
-    else { A = S.fa[i]; B = piv_div(r[c], S.e1, S.r1, i); }
-    Bl = A * Bl + B;
-    Al = A * Al;
+  for (int t = 0; t < K; t++) {
+    Al[t] = T(1);
+    Bl[t] = T(0);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+      const int i = i0 + c;
+      T A, B;
+      if (!ok[c]) { A = T(1); B = T(0); }
+      else if (!fl[t][c]) { A = T(0); B = r[t][c]; }
+      else if (i == 2) { A = T(0); B = pdiv(r[t][c], pv[c]); }
+      else { A = pa[c]; B = pdiv(r[t][c], pv[c]); }
+      Bl[t] = A * Bl[t] + B;
+      Al[t] = A * Al[t];
+    }
   }
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    T Ap = __shfl_up(Al, off, 64), Bp = __shfl_up(Bl, off, 64);
-    if (lane >= off) {
-      Bl = Al * Bp + Bl;
-      Al = Al * Ap;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      T Ap = __shfl_up(Al[t], off, 64), Bp = __shfl_up(Bl[t], off, 64);
+      if (lane >= off) {
+        Bl[t] = Al[t] * Bp + Bl[t];
+        Al[t] = Al[t] * Ap;
+      }
     }
   }
-  T yin = __shfl_up(Bl, 1, 64);
-  if (lane == 0) yin = T(0);
-  T y[CPL];
-  T yp = yin;
+  // re-evaluate the lane's cells from the incoming value (y kept in r)
 #pragma unroll
-  for (int c = 0; c < CPL; c++) {
-    int i = i0 + c;
-    T v = r[c];
-    if (ok[c] && fl[c]) v = (i == 2) ? piv_div(r[c], S.e1, S.r1, 2) : piv_div(r[c] - S.bb * yp, S.e1, S.r1, i);
-    y[c] = v;
-    yp = v;
+  for (int t = 0; t < K; t++) {
+    T yp = __shfl_up(Bl[t], 1, 64);
+    if (lane == 0) yp = T(0);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+      const int i = i0 + c;
+      T v = r[t][c];
+      if (ok[c] && fl[t][c]) v = (i == 2) ? pdiv(r[t][c], pv[c]) : pdiv(r[t][c] - S.bb * yp, pv[c]);
+      r[t][c] = v;
+      yp = v;
+    }
   }
   // backward: z_i = C_i z_(i+1) + y_i, C_i = -e2_i on fluid cells 2..ni
-  T Cl = T(1), Dl = T(0);
+  T Cl[K], Dl[K];
 #pragma unroll
-  for (int c = CPL - 1; c >= 0; c--) {
-    int i = i0 + c;
-    T C, Dv;
-    if (!ok[c]) { C = T(1); Dv = T(0); }
-    else if (fl[c] && i <= ni) { C = -S.e2[i]; Dv = y[c]; }
-    else { C = T(0); Dv = y[c]; }
-    Dl = C * Dl + Dv;
-    Cl = C * Cl;
+  for (int t = 0; t < K; t++) {
+    Cl[t] = T(1);
+    Dl[t] = T(0);
+#pragma unroll
+    for (int c = CPL - 1; c >= 0; c--) {
+      const int i = i0 + c;
+      T C, Dv;
+      if (!ok[c]) { C = T(1); Dv = T(0); }
+      else if (fl[t][c] && i <= ni) { C = -pe2[c]; Dv = r[t][c]; }
+      else { C = T(0); Dv = r[t][c]; }
+      Dl[t] = C * Dl[t] + Dv;
+      Cl[t] = C * Cl[t];
+    }
   }
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    T Cn = __shfl_down(Cl, off, 64), Dn = __shfl_down(Dl, off, 64);
-    if (lane + off < 64) {
-      Dl = Cl * Dn + Dl;
-      Cl = Cl * Cn;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      T Cn = __shfl_down(Cl[t], off, 64), Dn = __shfl_down(Dl[t], off, 64);
+      if (lane + off < 64) {
+        Dl[t] = Cl[t] * Dn + Dl[t];
+        Cl[t] = Cl[t] * Cn;
+      }
     }
   }
-  T zin = __shfl_down(Dl, 1, 64);
-  if (lane == 63) zin = T(0);
-  T zn = zin;
 #pragma unroll
-  for (int c = CPL - 1; c >= 0; c--) {
-    int i = i0 + c;
-    if (!ok[c]) continue;
-    T v = y[c];
-    if (fl[c] && i <= ni) v = y[c] - S.e2[i] * zn;
-    zn = v;
-    if (fl[c]) D[b0 + i] = v;
+  for (int t = 0; t < K; t++) {
+    T zn = __shfl_down(Dl[t], 1, 64);
+    if (lane == 63) zn = T(0);
+#pragma unroll
+    for (int c = CPL - 1; c >= 0; c--) {
+      const int i = i0 + c;
+      if (!ok[c]) continue;
+      T v = r[t][c];
+      if (fl[t][c] && i <= ni) v = r[t][c] - pe2[c] * zn;
+      zn = v;
+      if (lv[t] && fl[t][c]) S.dw[base[t] + i] = v;
+    }
   }
 }
+
+// lines per wave of the x scan: four lines of up to 256 cells share a
+// wave, 512-cell lines take one (measured at 256^3: one, two and four lines
+// per wave give 527, 512 and 477 us in FP64, 409, 339 and 324 us in FP32)
+template <int CPL>
+constexpr int xscan_lines() { return CPL >= 8 ? 1 : 4; }
 
 // sequential x lines for blocks wider than 512 cells
 template <typename T>
@@ -157,7 +206,7 @@ __device__ __attribute__((always_inline)) inline void smooth_x_seq(const SmoothA
 // once over the cases that use it
 template <int CPL, typename T>
 __global__ void __launch_bounds__(256) k_smooth_x_scan(SmoothArgs<T> S) {
-  smooth_x_scan<CPL>(S, (int)blockIdx.x);
+  smooth_x_scan<CPL, xscan_lines<CPL>()>(S, (int)blockIdx.x);
 }
 template <int CPL, typename T>
 __global__ void __launch_bounds__(256) k_smooth_x_scan_b(const SmoothArgs<T> *As, const int4 *cg, int nc,
@@ -166,7 +215,7 @@ __global__ void __launch_bounds__(256) k_smooth_x_scan_b(const SmoothArgs<T> *As
   const int c = batch_case(cg, nc, b);
   const int4 g = cg[c];
   if (!act[g.w]) return;
-  smooth_x_scan<CPL>(As[c], b - g.x);
+  smooth_x_scan<CPL, xscan_lines<CPL>()>(As[c], b - g.x);
 }
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_smooth_x_seq_body(const SmoothArgs<T> &S, int bx, int, int) {

@@ -73,14 +73,17 @@ def test_symmetry_wall_and_periodic_faces(require_gpu):
     gpu.close()
 
 
-def test_long_smoothing_lines(require_gpu):
-    """y lines of 257 cells (beyond the 256 the checkpointed three-pass
-    sweep takes: the four-pass sweep runs) and z lines of exactly 256 (16
-    full checkpoint chunks, no tail): one smoothed stage against the oracle
-    at 1e-13 relative to each field's largest value"""
+@pytest.mark.parametrize("n", [(8, 257, 256), (256, 3, 3), (300, 5, 3), (60, 3, 3)])
+def test_smoothing_line_shapes(require_gpu, n):
+    """One smoothed stage against the oracle at 1e-13 relative to each
+    field's largest value, for the line shapes the sweeps treat apart:
+    y lines of 257 cells (beyond the 256 the checkpointed three-pass sweep
+    takes: the four-pass sweep runs) and z lines of exactly 256 (16 full
+    checkpoint chunks, no tail); x lines of 256 cells (4 per lane, four
+    lines per wave, 45 lines: the last wave has one), of 300 (8 per lane,
+    one line per wave) and of 60 (1 per lane, four lines per wave)"""
     from tests.gpu_common import max_rel_to_field, perturbed_state
-    n = (8, 257, 256)
-    levels = hierarchy(n, (0.0, 0.0, 0.0), (0.08, 2.57, 2.56), 1, bctype=(FAR,) * 6)
+    levels = hierarchy(n, (0.0, 0.0, 0.0), tuple(0.01 * m for m in n), 1, bctype=(FAR,) * 6)
     prm = Params(mach=0.4, alpha_deg=3.0, beta_deg=1.0, smoo=0.6)
     geoms = [level_geometry(s, None) for s in levels]
     orc = OracleCase(prm, geoms, mxcyc=4)
