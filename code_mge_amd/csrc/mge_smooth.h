@@ -33,6 +33,21 @@ __device__ __attribute__((always_inline)) inline T piv_div(T x, const T *__restr
   else return x * r1[s];
 }
 
+// two-element vectors for the x scan's loads and stores
+template <typename T> struct PairOf;
+template <> struct PairOf<double> { using type = double2; };
+template <> struct PairOf<float> { using type = float2; };
+template <typename T> using Pair = typename PairOf<T>::type;
+
+// a lane moves its CPL cells as pairs when CPL is even, the line's first
+// cell (i = 2) sits on a pair boundary (so does every lane's first cell,
+// i0 - 2 being a multiple of CPL) and all its cells are inside the line
+template <int CPL, typename T>
+__device__ __attribute__((always_inline)) inline bool xscan_pairs(const T *row, int i0, int il) {
+  if constexpr (CPL % 2 != 0) return false;
+  else return ((reinterpret_cast<uintptr_t>(row + 2) & (2 * sizeof(T) - 1)) == 0) && i0 + CPL - 1 <= il;
+}
+
 // psmoo_old.f90:171-188 (x lines): one wavefront per K consecutive (j,k)
 // lines of a component.  Lane l owns CPL consecutive cells of each line, so
 // loads and stores are coalesced.  Both recurrences are first composed as
@@ -80,12 +95,20 @@ __device__ __attribute__((always_inline)) inline void smooth_x_scan(const Smooth
     const int j = 2 + l % d.nj, k = 2 + l / d.nj;
     fbase[t] = ix(d, 0, j, k);
     base[t] = (long)n * d.E + fbase[t];
+    if (xscan_pairs<CPL>(S.dw + base[t], i0, il)) {
+      const Pair<T> *pp = reinterpret_cast<const Pair<T> *>(S.dw + base[t] + i0);
 #pragma unroll
-    for (int c = 0; c < CPL; c++) {
-      const int i = i0 + c;
-      r[t][c] = ok[c] ? S.dw[base[t] + i] : T(0);
-      fl[t][c] = ok[c] && (S.flags[fbase[t] + i] & F_FLUID);
+      for (int h = 0; h < CPL / 2; h++) {
+        const Pair<T> v = pp[h];
+        r[t][2 * h] = v.x;
+        r[t][2 * h + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; c++) r[t][c] = ok[c] ? S.dw[base[t] + i0 + c] : T(0);
     }
+#pragma unroll
+    for (int c = 0; c < CPL; c++) fl[t][c] = ok[c] && (S.flags[fbase[t] + i0 + c] & F_FLUID);
   }
   // forward: lane composite of y_i = A_i y_(i-1) + B_i
   T Al[K], Bl[K];
@@ -166,17 +189,27 @@ __device__ __attribute__((always_inline)) inline void smooth_x_scan(const Smooth
     for (int c = CPL - 1; c >= 0; c--) {
       const int i = i0 + c;
       if (!ok[c]) continue;
-      T v = r[t][c];
-      if (fl[t][c] && i <= ni) v = r[t][c] - pe2[c] * zn;
-      zn = v;
-      if (lv[t] && fl[t][c]) S.dw[base[t] + i] = v;
+      if (fl[t][c] && i <= ni) r[t][c] = r[t][c] - pe2[c] * zn;
+      zn = r[t][c];
+    }
+    if (!lv[t]) continue;
+    if (xscan_pairs<CPL>(S.dw + base[t], i0, il)) {
+      // blank cells are written back with the value they were read with
+      Pair<T> *pp = reinterpret_cast<Pair<T> *>(S.dw + base[t] + i0);
+#pragma unroll
+      for (int h = 0; h < CPL / 2; h++) pp[h] = Pair<T>{r[t][2 * h], r[t][2 * h + 1]};
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; c++)
+        if (fl[t][c]) S.dw[base[t] + i0 + c] = r[t][c];
     }
   }
 }
 
 // lines per wave of the x scan: four lines of up to 256 cells share a
-// wave, 512-cell lines take one (measured at 256^3: one, two and four lines
-// per wave give 527, 512 and 477 us in FP64, 409, 339 and 324 us in FP32)
+// wave, 512-cell lines take one (measured at 256^3 with single loads: one,
+// two and four lines per wave give 527, 512 and 477 us in FP64, 409, 339
+// and 324 us in FP32; with pairs, four lines 408 us FP64, 298 us FP32)
 template <int CPL>
 constexpr int xscan_lines() { return CPL >= 8 ? 1 : 4; }
 

@@ -73,7 +73,7 @@ def test_symmetry_wall_and_periodic_faces(require_gpu):
     gpu.close()
 
 
-@pytest.mark.parametrize("n", [(8, 257, 256), (256, 3, 3), (300, 5, 3), (60, 3, 3)])
+@pytest.mark.parametrize("n", [(8, 257, 256), (256, 3, 3), (300, 5, 3), (60, 3, 3), (129, 3, 3)])
 def test_smoothing_line_shapes(require_gpu, n):
     """One smoothed stage against the oracle at 1e-13 relative to each
     field's largest value, for the line shapes the sweeps treat apart:
@@ -81,7 +81,9 @@ def test_smoothing_line_shapes(require_gpu, n):
     takes: the four-pass sweep runs) and z lines of exactly 256 (16 full
     checkpoint chunks, no tail); x lines of 256 cells (4 per lane, four
     lines per wave, 45 lines: the last wave has one), of 300 (8 per lane,
-    one line per wave) and of 60 (1 per lane, four lines per wave)"""
+    one line per wave), of 60 (1 per lane, four lines per wave) and of 129
+    (rows of odd length: every other line starts off a pair boundary and
+    takes single loads, and the last lane holds one cell)"""
     from tests.gpu_common import max_rel_to_field, perturbed_state
     levels = hierarchy(n, (0.0, 0.0, 0.0), tuple(0.01 * m for m in n), 1, bctype=(FAR,) * 6)
     prm = Params(mach=0.4, alpha_deg=3.0, beta_deg=1.0, smoo=0.6)
