@@ -27,12 +27,14 @@
 
 namespace mge {
 
-// minmod(a, b): 0 when a*b <= 0, else the argument of smaller magnitude
-// (written as value selects so no stack slot is needed)
+// minmod(a, b): 0 when a and b differ in sign or one is zero, else the
+// argument of smaller magnitude.  Written with min/max only (no selects, no
+// product): max(min(a,b),0) + min(max(a,b),0) is the smaller of two positive
+// arguments, the larger (nearer zero) of two negative ones and 0 otherwise,
+// and the sum adds an exact zero, so the value is that argument itself.
 template <typename T>
 __device__ inline T minmod(T a, T b) {
-  T m = fabs(a) < fabs(b) ? a : b;
-  return a * b <= T(0) ? T(0) : m;
+  return fmax(fmin(a, b), T(0)) + fmin(fmax(a, b), T(0));
 }
 
 // primitive variables of a cell from the stored state (FP32: the full
