@@ -571,7 +571,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       }
     }
     T Ez1[5], Dz1[5];
-    if (valid) roe_z_face(A, zq, zf, Sz, Ez1, Dz1);
+    roe_z_face(A, zq, zf, Sz, Ez1, Dz1);  // a lane outside the block computes from zeros and stores nothing
     __syncthreads();
     // ---- accumulate and finish
     if (valid) {
