@@ -131,6 +131,40 @@ __device__ inline int batch_case(const int4 *cg, int nc, int b) {
     NAME##_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));                                     \
   }
 
+// Block coordinates of a single-case launch with each XCD given a
+// contiguous range of blocks: workgroups are dealt to the 8 XCDs in turn
+// (block w runs on XCD w % 8), so block w takes the logical index
+// (w % 8) * (W / 8) + w / 8.  Neighbouring tiles (the segments of one row, a
+// tile and its halo neighbours) then run on one XCD and share its L2.  Grids
+// whose block count is not a multiple of 8 keep the plain order.
+__device__ inline void xcd_block(int &bx, int &by, int &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, W = gx * gy * gridDim.z;
+  const unsigned w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned l = (W % 8u == 0u) ? (w % 8u) * (W / 8u) + w / 8u : w;
+  bx = (int)(l % gx);
+  by = (int)((l / gx) % gy);
+  bz = (int)(l / (gx * gy));
+}
+
+// MGE_ENTRY_T with the XCD-contiguous block order in the single-case entry
+#define MGE_ENTRY_T_XCD(NAME, ARGS, NT, MINB)                                                          \
+  template <typename T>                                                                                \
+  __global__ void __launch_bounds__(NT, MINB) NAME(ARGS<T> A) {                                        \
+    int bx, by, bz;                                                                                    \
+    xcd_block(bx, by, bz);                                                                             \
+    NAME##_body(A, bx, by, bz);                                                                        \
+  }                                                                                                    \
+  template <typename T>                                                                                \
+  __global__ void __launch_bounds__(NT, MINB) NAME##_b(const ARGS<T> *As, const int4 *cg, int nc,      \
+                                                       const int *act) {                               \
+    const int b = (int)blockIdx.x;                                                                     \
+    const int c = batch_case(cg, nc, b);                                                               \
+    const int4 g = cg[c];                                                                              \
+    if (!act[g.w]) return;                                                                             \
+    const int l = b - g.x;                                                                             \
+    NAME##_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));                                     \
+  }
+
 #define MGE_ENTRY(NAME, ARGS, NT, MINB)                                                                \
   __global__ void __launch_bounds__(NT, MINB) NAME(ARGS A) {                                           \
     NAME##_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);                                 \

@@ -628,11 +628,13 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
     }
   }
 }
-// entry points as MGE_ENTRY_T makes them, with launch bounds that follow the
-// precision's tile
+// entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
+// the single-case entry), with launch bounds that follow the precision's tile
 template <typename T>
 __global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW) k_stage_roe(StageArgs<T> A) {
-  k_stage_roe_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  k_stage_roe_body(A, bx, by, bz);
 }
 template <typename T>
 __global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW)

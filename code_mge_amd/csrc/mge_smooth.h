@@ -476,7 +476,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const Smoo
   if (i > d.il) return;
   smooth_line(S.dw + n * d.E, S.flags, ix(d, i, 0, k), d.sy, d.jl, S.e1, S.r1, S.e2, S.bb);
 }
-MGE_ENTRY_T(k_smooth_y, SmoothArgs, 64, 1)
+MGE_ENTRY_T_XCD(k_smooth_y, SmoothArgs, 64, 1)
 
 // psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
 template <typename T>
@@ -488,7 +488,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_z_body(const Smoo
   if (i > d.il) return;
   smooth_line(S.dw + n * d.E, S.flags, ix(d, i, j, 0), d.sz, d.kl, S.e1, S.r1, S.e2, S.bb);
 }
-MGE_ENTRY_T(k_smooth_z, SmoothArgs, 64, 1)
+MGE_ENTRY_T_XCD(k_smooth_z, SmoothArgs, 64, 1)
 
 // after smoothing: w = wn - dw and the EOS on interior cells
 // (euler.f90:381-399); the ghost shell of the previous state is carried

@@ -555,11 +555,13 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
     __syncthreads();
   }
 }
-// entry points as MGE_ENTRY_T makes them, with launch bounds that follow the
-// precision's tile
+// entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
+// the single-case entry), with launch bounds that follow the precision's tile
 template <typename T>
 __global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW) k_stage2(StageArgs<T> A) {
-  k_stage2_body(A, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  k_stage2_body(A, bx, by, bz);
 }
 template <typename T>
 __global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW)
