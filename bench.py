@@ -181,7 +181,7 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
     import dataclasses
     from code_mge_amd import configs
     from code_mge_amd.grid import level_geometry
-    from code_mge_amd.serve import Request, Server, percentiles
+    from code_mge_amd.serve import Request, Server, outcome_counts, percentiles
     from code_mge_amd.solver import Case, force_coefficients
     sweep = configs.ellipsoid_sweep()[:k]
     reqs = [Request(i, 0.0, prm, levels, bdy, aref, info) for i, (prm, levels, bdy, aref, info) in enumerate(sweep)]
@@ -221,6 +221,7 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
             "fp32_wall_s": round(t32, 2), "fp64_wall_s": round(t64, 2),
             "max_cd_err_fp32_vs_fp64": dcd, "max_cl_err_fp32_vs_fp64": dcl,
             "ttc_p50_s": round(p50, 3), "ttc_p99_s": round(p99, 3),
+            "ttc_over": "converged requests only", "fp32_outcomes": outcome_counts(list(served.values())),
             "cycles_fp32": [served[i].ncyc for i in sorted(served)], "cycles_fp64": [alone[i][0] for i in sorted(alone)]}
 
 
@@ -318,7 +319,7 @@ def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000):
     them laminar Navier-Stokes): ellipsoids, grids 32^3..192^3, 3 levels,
     FP32 served under an HBM budget to a residual drop of `drop`;
     p50/p99 time-to-convergence (completion - arrival) and solves/h."""
-    from code_mge_amd.serve import Server, percentiles, poisson_requests
+    from code_mge_amd.serve import Server, outcome_counts, percentiles, poisson_requests
     reqs = poisson_requests(count=count, rate=rate)
     srv = Server(hbm_budget=budget_gb * 1e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4,
                  log=lambda m: print(m, file=sys.stderr, flush=True))
@@ -332,6 +333,7 @@ def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000):
                         f"32^3..192^3, 3 levels, FP32, drop {drop:g}, HBM budget {budget_gb:g} GB",
             "wall_s": round(wall, 2), "converged": conv, "requests": count, "ns_requests": len(ns),
             "solves_per_h": round(conv / wall * 3600.0, 1), "ttc_p50_s": round(p50, 3), "ttc_p99_s": round(p99, 3),
+            "ttc_over": "converged requests only", "outcomes": outcome_counts(res),
             "last_arrival_s": round(reqs[-1].arrival, 2),
             "cycles_max": max(r.ncyc for r in res)}
 

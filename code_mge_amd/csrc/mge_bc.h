@@ -238,7 +238,7 @@ struct CutArgs {
   int *err;
 };
 
-// bc_cutcell.f90:200-428 for steady bodies (qb = 0), in FP64
+// bc_cutcell.f90:40-219 for steady bodies (qb = 0), in FP64
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_bc_cut_body(const CutArgs<T> &C, int bx, int, int) {
   int m = bx * blockDim.x + threadIdx.x;
@@ -630,7 +630,7 @@ struct XferArgs {
 
 // the coarse-level correction w - ws of every coarse cell, formed once so
 // that the prolongation reads one value per coarse neighbour and component
-// instead of two (addw.f90:369-438 forms the same difference per use)
+// instead of two (addw.f90:156-173 forms the same difference per use)
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_corr_body(const XferArgs<T> &X, int bx, int, int) {
   const long q = bx * (long)blockDim.x + threadIdx.x;
@@ -653,7 +653,7 @@ __device__ inline void pstencil(int i, int r, int *s, T *f) {
   } else { *s = i; f[0] = T(1); f[1] = T(0); }
 }
 
-// collc.f90:110-198 for aligned ratio-1/2 levels: w and the residual
+// collc.f90:112-143 for aligned ratio-1/2 levels: w and the residual
 // restricted with the cell-centre trilinear weights; marker = 1
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_restrict_body(const XferArgs<T> &X, int bx, int by, int bz) {
@@ -688,7 +688,7 @@ __device__ __attribute__((always_inline)) inline void k_restrict_body(const Xfer
 }
 MGE_ENTRY_T(k_restrict, XferArgs, 256, 1)
 
-// addw.f90:369-438: trilinear prolongation of the correction w-ws (or of
+// addw.f90:156-173: trilinear prolongation of the correction w-ws (or of
 // the solution on the full-multigrid step) to fluid cells
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferArgs<T> &X, int bx, int by, int bz) {
@@ -717,7 +717,7 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
 }
 MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
 
-// wall force: momentum flux of the scheme's wall faces (efluxb.f90:251-253
+// wall force: momentum flux of the scheme's wall faces (efluxb.f90:66-68
 // for JST; the full Roe wall flux, central + upwind, for the Roe schemes)
 // in FP64; one thread per record writes its term, one thread adds them in
 // record order

@@ -539,11 +539,16 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
             (CopyArgs<T>{(const T *)tp<T>(L.S[L.cur]), tp<T>(L.ws), 5 * E}));
   T *dwp = tp<T>(L.dw);
   const int w0 = L.cur;
+  // the last stage always lands in `target` (0 <-> 1 from a start in 0 or
+  // 1), whatever the stage count, so the state buffer toggles with period
+  // two per call; earlier stages alternate between target and the third
+  // buffer, never writing the buffer they read
+  const int target = w0 == 0 ? 1 : 0;
+  const int other = 3 - w0 - target;
   int src = w0;
   int warn = MGE_OK;
   for (int m = 1; m <= P.mstage; m++) {
-    int dst = 0;
-    while (dst == w0 || dst == src) dst++;
+    const int dst = ((P.mstage - m) % 2 == 0) ? target : other;
     StageArgs<T> A = stage_args<T>(h, L, m, src, dst, w0, mgl, mlvl, lres, lfir);
     // laminar viscous fluxes (euler.f90:137-181): fw = blend fw + cdis V
     // before the stage kernel adds its dissipation
@@ -1280,6 +1285,9 @@ int batch_build(mge_batch *b) {
     h->rec = &R[i];
     long long cs = 0;
     int rc = MGE_OK;
+    // the program replays these two cycles from the buffers they start in
+    std::vector<int> cur0;
+    for (int l = 1; l <= h->nlevel; l++) cur0.push_back(h->lv[l].cur);
     for (int cyc = 1; cyc <= 2 && rc >= 0; cyc++) {
       rc = by_type(h, [&](auto z) { return do_cycle<decltype(z)>(h, b->nlevel, cyc, &cs); });
       Rec m;
@@ -1293,6 +1301,9 @@ int batch_build(mge_batch *b) {
     }
     h->rec = nullptr;
     if (rc < 0) return rc;
+    for (int l = 1; l <= h->nlevel; l++)
+      if (h->lv[l].cur != cur0[l - 1])
+        return set_err(MGE_ERR_STATE, "two recorded cycles do not return a level's state to its buffer");
   }
   const size_t np = R[0].size();
   for (int i = 1; i < nc; i++)
@@ -1549,7 +1560,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   if (h->prm.smoo > 0.0)
     for (int i = 2; i <= d.il && i < nmax; i++) fa[i] = h->prm.smoo / piv[i];  // -bb/e1, bb = -smoo
   CHKRC(put_bulk(h, L.fax, fa.data(), nmax));
-  // flags: fluid bit with the ghost copies of init_metric.f90:304-315
+  // flags: fluid bit with the ghost copies of init_metric.f90:207-218
   std::vector<uint8_t> bl(E, 1);
   auto X = [&](int i, int j, int k) { return ix(d, i, j, k); };
   for (int k = 2; k <= d.kl; k++)
@@ -1632,7 +1643,7 @@ int mge_bc_cutcell(mge_case *h, int lvl, int blk) {
 }
 
 int mge_bc_overlap(mge_case *h, int lvl) {
-  // one block per level: no same-level overlaps (bc_overlap.f90:296-354 loops over none)
+  // one block per level: no same-level overlaps (bc_overlap.f90:32-90 loops over none)
   if (!get_level(h, lvl)) return set_err(MGE_ERR_ARG, "bad level");
   return MGE_OK;
 }
@@ -1955,7 +1966,7 @@ int mge_wall_force(mge_case *h, int lvl, double out[3]) {
   return check_err(h);
 }
 
-// ---- solution fields (upload_solut.f90:97-241) -------------------------------
+// ---- solution fields (upload_solut.f90:89-241) -------------------------------
 }  // extern "C"
 namespace {
 // freestream Mach number from the nondimensional state (|U| / c0)

@@ -9,7 +9,7 @@
 //  - a cell is solid when its centre is strictly inside the body (surfaces:
 //    odd number of surface crossings of the grid line through the centre
 //    before it, the line nudged off the grid planes by 1e-9 of a cell)
-//    (blank = 0, the role of celltype==csolid in init_metric.f90:282-290);
+//    (blank = 0, the role of celltype==csolid in init_metric.f90:185-193);
 //  - every (fluid, solid) face-neighbour pair of interior cells becomes one
 //    record with the fields bc_cutcell.f90/efluxb.f90/dfluxb.f90 read:
 //    faceid/zz (side and direction of the solid cell), ccid/scid,

@@ -83,7 +83,7 @@ __device__ inline T stored(const Inf<T> &f, int n, double v) {
 }
 
 // face vector of cell (i,j,k) in direction dir: the non-zero component of
-// si/sj/sk (init_metric.f90:216-274) on an axis-aligned grid.
+// si/sj/sk (init_metric.f90:109-177) on an axis-aligned grid.
 // si = (-dy*dz, 0, 0), sj = (0, -dx*dz, 0), sk = (0, 0, -dx*dy); the
 // cross products of init_metric reduce to exactly these values.
 template <int DIR, typename T>

@@ -1,5 +1,5 @@
 // mge_output.h -- the solution fields code_mge hands to its output and FSI
-// layers (upload_solut, srcf90/upload_solut.f90:97-241).
+// layers (upload_solut, srcf90/upload_solut.f90:89-241).
 //
 // Per interior cell, in FP64 whatever the bulk precision: rho, u, v, w,
 // cp = (p/p0 - 1) 2/(gamma M^2), Mach, total-pressure loss

@@ -48,7 +48,7 @@ __device__ __attribute__((always_inline)) inline bool xscan_pairs(const T *row, 
   else return ((reinterpret_cast<uintptr_t>(row + 2) & (2 * sizeof(T) - 1)) == 0) && i0 + CPL - 1 <= il;
 }
 
-// psmoo_old.f90:171-188 (x lines): one wavefront per K consecutive (j,k)
+// psmoo.f90:171-188 (x lines): one wavefront per K consecutive (j,k)
 // lines of a component.  Lane l owns CPL consecutive cells of each line, so
 // loads and stores are coalesced.  Both recurrences are first composed as
 // affine maps (forward y_i = A_i y_(i-1) + B_i, backward z_i = C_i z_(i+1)
@@ -466,7 +466,7 @@ __device__ __attribute__((always_inline)) inline void smooth_line(T *D, const ui
   }
 }
 
-// psmoo_old.f90:190-206 (y lines); thread per (i,k) and component
+// psmoo.f90:190-206 (y lines); thread per (i,k) and component
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const SmoothArgs<T> &S, int bx, int by, int bz) {
   const Dims &d = S.d;
@@ -478,7 +478,7 @@ __device__ __attribute__((always_inline)) inline void k_smooth_y_body(const Smoo
 }
 MGE_ENTRY_T_XCD(k_smooth_y, SmoothArgs, 64, 1)
 
-// psmoo_old.f90:211-237 (z lines); thread per (i,j) and component
+// psmoo.f90:209-237 (z lines); thread per (i,j) and component
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_smooth_z_body(const SmoothArgs<T> &S, int bx, int by, int bz) {
   const Dims &d = S.d;

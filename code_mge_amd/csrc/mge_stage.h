@@ -37,8 +37,8 @@ enum OpKind : int {
   OP_LO_N_UNDO,      // fw(s+1) -= dfs(s+1) - dfs(s+2)      dfluxb.f90:110
   OP_LO_C_ADD,       // fw(s)   += fs1 - fs2                dfluxb.f90:170
   OP_LO_N_ADD,       // fw(s+1) += fs2 - dfs(s+2)           dfluxb.f90:171
-  OP_E_UNDO,         // dw(s)   -= efs(s) - efs(s+1)        efluxb.f90:257
-  OP_E_ADD,          // dw(s)   += fa - fs  or  fs - fa     efluxb.f90:262,266
+  OP_E_UNDO,         // dw(s)   -= efs(s) - efs(s+1)        efluxb.f90:72,86,100
+  OP_E_ADD,          // dw(s)   += fa - fs  or  fs - fa     efluxb.f90:77,81
   OP_ROE_WALL        // Roe schemes: fw(s) += upwind part, dw(s) += central
                      // part of the wall face between s and its mirror state
 };
@@ -746,7 +746,7 @@ __device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   }
 }
 
-// the wall flux fs and the opposite-face flux fa of efluxb.f90:235-253,
+// the wall flux fs and the opposite-face flux fa of efluxb.f90:50-80,
 // in FP64 like dfluxb_rec
 template <typename T>
 __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double *fs,

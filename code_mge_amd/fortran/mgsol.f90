@@ -137,7 +137,7 @@ program mgsol
     print '(a,f12.8,a,f12.8,a,es10.3)', ' Cd = ', (F(1)*ca*cb + F(2)*sa*cb + F(3)*sb)/(q*aref), &
       '   Cl = ', (-F(1)*sa + F(2)*ca)/(q*aref), '   reference area ', aref
   end if
-  ! upload_solut (solver.f90:278): the output fields of the finest level,
+  ! upload_solut (solver.f90:279): the output fields of the finest level,
   ! reported as their ranges
   call check(mge_upload_solut(h, job%nlevel, 1, 0.0d0, c_null_ptr, nval), 'upload_solut')
   call check_in(mge_job_level(job, job%nlevel, n, ratio), 'level dimensions')

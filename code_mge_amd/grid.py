@@ -2,7 +2,7 @@
 
 Each multigrid level is one block covering the whole domain (the
 configuration every benchmark case uses).  Vertices follow
-init_metric.f90:156-169: x(i) = h*(i-2) + origin for i = 1..ib, with
+init_metric.f90:59-72: x(i) = h*(i-2) + origin for i = 1..ib, with
 x(0) left at 0.  Coarsening halves a direction while it keeps >= 2 cells
 (psmoo_old needs two cells per line, psmoo.f90:132,166), otherwise the
 direction is kept (semi-coarsening).

@@ -3,7 +3,7 @@
 Restates srcf90/input_param.f90:13-115 (non-dimensionalisation with
 rho0 = 1, |U_inf| = 1, p0 = 1/(gamma M^2)) with two recorded deviations:
  - the laminar Reynolds number is taken from the input instead of the
-   hard-coded 500 of input_param.f90:535-539;
+   hard-coded 500 of input_param.f90:111-115;
  - angles are given in degrees (the jobsheet's alphad/betad) and converted.
 """
 import math
@@ -21,7 +21,7 @@ class Params:
     gamma: float = 1.4
     cfl: float = -2.5                 # jobsheet sign: negative = local time step
     vis2: float = 0.5
-    vis4: float = 1.0                 # divided by 32 below (input_param.f90:464)
+    vis4: float = 1.0                 # divided by 32 below (input_param.f90:40)
     smoo: float = 0.6
     mgtype: int = 2
     cstp: Sequence[float] = (0.25, 0.16667, 0.375, 0.5, 1.0)

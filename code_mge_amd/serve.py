@@ -176,8 +176,20 @@ class Server:
 
 
 def percentiles(results: List[Result]):
-    t = np.array([r.time_to_convergence for r in results])
+    """p50 and p99 time-to-convergence over the requests that converged
+    (status 1); NaN when none did.  Requests stopped at the cycle limit or
+    by an error never converged and are counted by outcome_counts."""
+    t = np.array([r.time_to_convergence for r in results if r.status == 1])
+    if t.size == 0:
+        return math.nan, math.nan
     return float(np.percentile(t, 50)), float(np.percentile(t, 99))
+
+
+def outcome_counts(results: List[Result]):
+    """requests per outcome: converged, stopped at the cycle limit, failed"""
+    return {"converged": sum(1 for r in results if r.status == 1),
+            "cycle_limit": sum(1 for r in results if r.status == 2),
+            "errors": sum(1 for r in results if r.status < 0)}
 
 
 def poisson_requests(count=256, rate=4.0, seed=20261203, nlevel=3, box=8.0, euler_only=False):

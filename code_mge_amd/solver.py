@@ -17,7 +17,7 @@ from .params import Params
 
 _dp = C.POINTER(C.c_double)
 
-# upload_solut's variable names (upload_solut.f90:170-180)
+# upload_solut's variable names (upload_solut.f90:206-234)
 SOLUT_NAMES = ("rho", "u", "v", "w", "cp", "mach", "pt-loss", "energy", "gradn1", "gradn2", "gradn3")
 
 
@@ -149,7 +149,7 @@ class Case:
         _ffi.check(self.lib.mge_upload(self.h, lvl, 1, field.encode(), a.ctypes.data_as(_dp)), "mge_upload")
 
     def upload_solut(self, lvl, p_tot=0.0):
-        """upload_solut's fields of level lvl (upload_solut.f90:97-241): a dict
+        """upload_solut's fields of level lvl (upload_solut.f90:89-241): a dict
         name -> (nk, nj, ni) array over the interior cells; p_tot <= 0 takes
         the freestream total pressure"""
         nv = C.c_int(0)

@@ -8,14 +8,14 @@
  *   mge_euler      <- euler(b,itime,ncyc,mgl,mlvl,lres,lfir,…) srcf90/euler.f90:1
  *   mge_bcond      <- bcond(b)                                 srcf90/bcond.f90:1
  *   mge_bc_cutcell <- bc_cutcell(b)                            srcf90/bc_cutcell.f90:1
- *   mge_bc_overlap <- bc_overlap(blk, lv)                      srcf90/bc_overlap.f90:265
+ *   mge_bc_overlap <- bc_overlap(blk, lv)                      srcf90/bc_overlap.f90:1
  *   mge_movco      <- movco(blk, mgl, nmesh, mblks)            srcf90/movco.f90:1
  *   mge_movfin     <- movfin(blk, mgl, mlvl, nmesh, mblks)     srcf90/movfin.f90:1
  *   mge_init_flow  <- init_flow(blk, lrflow, imodel)           srcf90/init_flow.f90:1
  *   mge_solve      <- solver(blk, project, job, model)         srcf90/solver.f90:1
  *   mge_solve_step, mge_shift_time_levels
  *                  <- solver's physical time loop              srcf90/solver.f90:110-290
- *   mge_upload_solut <- upload_solut(blk, p_tot, atcellcenter) srcf90/upload_solut.f90:97
+ *   mge_upload_solut <- upload_solut(blk, p_tot, atcellcenter) srcf90/upload_solut.f90:89
  *   mge_write_solut/mge_read_solut <- write_solut/read_solut   srcf90/wsolut.f90:47, rsolut.f90:1
  *   mge_level_define/mge_case_create  <- make_flow + init_metric
  *                                       (srcf90/make_flow.f90:1, init_metric.f90:1)
@@ -49,8 +49,8 @@ extern "C" {
 /* status codes */
 #define MGE_OK               0
 #define MGE_ERR_NEG_C2     -10  /* c^2 < 0 or NaN in timestep   (srcf90/timestep.f90:45-49)      */
-#define MGE_ERR_WALL_RHO   -11  /* zero density at a wall face  (srcf90/efluxb.f90:246-249)      */
-#define MGE_ERR_WSOL_NAN   -12  /* NaN mirror state             (srcf90/bc_cutcell.f90:307-317)  */
+#define MGE_ERR_WALL_RHO   -11  /* zero density at a wall face  (srcf90/efluxb.f90:61-64)      */
+#define MGE_ERR_WSOL_NAN   -12  /* NaN mirror state             (srcf90/bc_cutcell.f90:147-157)  */
 #define MGE_ERR_RES_NAN    -13  /* NaN density residual         (srcf90/solver.f90:180)          */
 #define MGE_ERR_PERIODIC   -14  /* unpaired periodic face       (srcf90/bcond.f90:26-31)         */
 #define MGE_ERR_DISTYPE    -15  /* bad distype / dissip         (srcf90/euler.f90:212-214)       */
@@ -60,7 +60,7 @@ extern "C" {
 #define MGE_ERR_NOMEM      -22  /* device allocation failed                                      */
 #define MGE_ERR_STATE      -23  /* call out of order (e.g. level not defined)                    */
 #define MGE_ERR_RESTART    -24  /* restart file does not match the case (srcf90/rsolut.f90:28-32) */
-#define MGE_WARN_ZERO_CLOUD  1  /* empty reflected-node cloud: early return (bc_cutcell.f90:225-230) */
+#define MGE_WARN_ZERO_CLOUD  1  /* empty reflected-node cloud: early return (bc_cutcell.f90:65-70) */
 #define MGE_WARN_BCTYPE      2  /* unknown bctype, face skipped (bcond.f90:35)                    */
 
 /* scheme and freestream parameters (typepara, filled as input_param.f90:13-115) */
@@ -85,7 +85,7 @@ typedef struct {
                                  states, residual norms, global time step and
                                  force integral stay FP64                          */
   double epsentr;             /* Roe entropy-fix coefficient (jobsheet r2_epsentr,
-                                 input_param.f90:103)                              */
+                                 input_param.f90:102)                              */
   int    execmode;            /* 0 steady; 1 unsteady dual time stepping: BDF2
                                  physical time derivative in euler, pseudo step
                                  capped by the physical one (jobsheet runtype)     */
@@ -235,7 +235,7 @@ int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out);
 int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *in);
 int mge_get_cuts(mge_case *h, int lvl, int blk, mge_cutrec *out);
 double mge_get_dtim(mge_case *h, int lvl, int blk);
-/* upload_solut (srcf90/upload_solut.f90:97-241): the output fields of every
+/* upload_solut (srcf90/upload_solut.f90:89-241): the output fields of every
  * interior cell in FP64, field-major with i fastest over ni*nj*nk cells --
  * rho, u, v, w, cp = (p/p0 - 1)*2/(gamma*M^2), Mach, total-pressure loss
  * p*(1 + 0.2*Mach^2)^3.5/p_tot - 1, and for laminar flow rho*E/rho and three
@@ -245,7 +245,7 @@ double mge_get_dtim(mge_case *h, int lvl, int blk);
 int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval);
 
 /* FP64 wall force (Fx,Fy,Fz) on the body at level lvl: the momentum flux
- * the scheme applies at each wall face (efluxb.f90:251-253), in record order */
+ * the scheme applies at each wall face (efluxb.f90:66-68), in record order */
 int mge_wall_force(mge_case *h, int lvl, double out[3]);
 /* restart files of every level: format 0 the reference's text file
  * (write_solut, srcf90/wsolut.f90:47-93: per block a "(5I5)" header code,

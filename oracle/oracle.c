@@ -18,7 +18,7 @@ typedef struct {
   int il, jl, kl, ie, je, ke, ib, jb, kb;
   long E;
   double *x;              /* vertices (0:ib,0:jb,0:kb,3) */
-  double *si, *sj, *sk;   /* face vectors (…,3) (init_metric.f90:208-274) */
+  double *si, *sj, *sk;   /* face vectors (…,3) (init_metric.f90:109-177) */
   double cellvolume;
   int bctype[6];
   double facenvec[6][3];
@@ -97,7 +97,7 @@ void orc_case_free(orc_case *c) {
   free(c);
 }
 
-/* make_flow.f90:36-88 allocations + init_metric.f90:137-315 */
+/* make_flow.f90:36-88 allocations + init_metric.f90:37-218 */
 int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
                   const double *xv, const double *yv, const double *zv,
                   double cellvolume, const int bctype[6],
@@ -145,7 +145,7 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->z = (orc_cut *)calloc((size_t)(nmir > 0 ? nmir : 1), sizeof(orc_cut));
   if (nmir > 0) memcpy(b->z, cuts, sizeof(orc_cut) * (size_t)nmir);
 
-  /* grid points (init_metric.f90:156-169): x(i,j,k) for i,j,k >= 1; index 0 stays 0 */
+  /* grid points (init_metric.f90:47-76): x(i,j,k) for i,j,k >= 1; index 0 stays 0 */
   for (int k = 1; k <= b->kb; k++)
     for (int j = 1; j <= b->jb; j++)
       for (int i = 1; i <= b->ib; i++) {
@@ -154,7 +154,7 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
         b->x[E + q] = yv[j];
         b->x[2 * E + q] = zv[k];
       }
-  /* face vectors (init_metric.f90:216-274) */
+  /* face vectors (init_metric.f90:109-177) */
 #define XV(i, j, k, d) (b->x[(long)(d) * E + IX(b, i, j, k)])
   for (int k = 0; k <= b->ke; k++) {
     int kp = k + 1;
@@ -189,7 +189,7 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
     }
   }
 #undef XV
-  /* blank (init_metric.f90:278-315) */
+  /* blank (init_metric.f90:178-218) */
   for (long q = 0; q < E; q++) b->blank[q] = 1;
   for (int k = 2; k <= b->kl; k++)
     for (int j = 2; j <= b->jl; j++)
@@ -690,7 +690,7 @@ static void dflux(blk_t *b, const orc_params *P, int istage) {
   for (int d = 0; d < 3; d++) free(dfs[d]);
 }
 
-/* efluxb (efluxb.f90:186-302) */
+/* efluxb (efluxb.f90:30-114) */
 static int efluxb(blk_t *b, const double *efs[3]) {
   const long E = b->E;
   for (int m = 0; m < b->nmir; m++) {
@@ -1302,7 +1302,7 @@ static int psmoo_old(blk_t *b, const orc_params *P) {
 
 /* ------------------------------------------------------------ boundaries */
 
-/* bc_far (bc_far.f90:46-331) for one face; dir = face axis */
+/* bc_far (bc_far.f90:26-284) for one face; dir = face axis */
 static void bc_far(blk_t *b, const orc_params *P, int face) {
   const double gamma = P->gamma;
   const double gm = 1.0 / (gamma - 1.0);
@@ -1492,7 +1492,7 @@ static int bcond_blk(blk_t *b, const orc_params *P) {
 
 int orc_bcond(orc_case *c, int lvl) { return bcond_blk(&c->lv[lvl], &c->prm); }
 
-/* bc_cutcell (bc_cutcell.f90:161-432), steady bodies (approxbc off) */
+/* bc_cutcell (bc_cutcell.f90:40-268), steady bodies (approxbc off) */
 static int bc_cutcell_blk(blk_t *b, const orc_params *P) {
   const double gamma = P->gamma;
   for (int m = 0; m < b->nmir; m++) {
@@ -1699,7 +1699,7 @@ int orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl, int 
 
 /* 1-D stencil of the cell-centre trilinear interpolation between aligned
  * levels (the role of eval_x2cell/shft_cell/trilinear_factor in
- * collc.f90:141-166 and addw.f90:386-411), written from integer indices. */
+ * collc.f90:88-114 and addw.f90:122-148), written from integer indices. */
 static void restrict_stencil(int I, int ratio, int *s, double *f0, double *f1) {
   if (ratio == 2) { *s = 2 * I - 2; *f0 = 0.5; *f1 = 0.5; }
   else { *s = I; *f0 = 1.0; *f1 = 0.0; }
@@ -1716,7 +1716,7 @@ static void tri_fac(const double fa[2], const double fb[2], const double fc[2], 
   for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
 }
 
-/* collc (collc.f90:53-220): restrict w and the residual from level mgl+1 */
+/* collc (collc.f90:49-146): restrict w and the residual from level mgl+1 */
 static void collc(orc_case *c, int mgl) {
   blk_t *b = &c->lv[mgl], *f = &c->lv[mgl + 1];
   const long Eb = b->E, Ef = f->E;
@@ -1767,7 +1767,7 @@ int orc_movco(orc_case *c, int mgl) {
   return bcond_blk(b, &c->prm);
 }
 
-/* addw (addw.f90:264-464): prolong to level mgl from mgl-1 */
+/* addw (addw.f90:47-199): prolong to level mgl from mgl-1 */
 static void addw(orc_case *c, int mgl, int mlvl) {
   blk_t *b = &c->lv[mgl], *pc = &c->lv[mgl - 1];
   const long Eb = b->E, Ec = pc->E;
@@ -1816,7 +1816,7 @@ int orc_movfin(orc_case *c, int mgl, int mlvl) {
 
 /* ------------------------------------------------------------ init, solver */
 
-/* init_flow.f90:324-424 (uniform freestream) */
+/* init_flow.f90:35-99 (uniform freestream) */
 int orc_init_flow(orc_case *c) {
   const orc_params *P = &c->prm;
   int warn = ORC_OK;
@@ -1934,7 +1934,7 @@ int orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
   return solve_ramp(c, mcyc, 0, reslog, cap, nlog, cell_stages);
 }
 
-/* upload_solut.f90:97-241: per interior cell rho, u, v, w, cp, Mach,
+/* upload_solut.f90:89-241: per interior cell rho, u, v, w, cp, Mach,
  * total-pressure loss (and for laminar flow rho E / rho and three zero
  * slots), field-major with i fastest; M from the nondimensional freestream
  * (|U| / c0) */
@@ -2053,7 +2053,7 @@ double orc_get_rtrms(orc_case *c, int lvl, int ncyc) {
 double orc_get_dtim(orc_case *c, int lvl) { return c->lv[lvl].dtim; }
 
 /* wall force: momentum flux through each wall face as the scheme applies it
- * (efluxb.f90:251-253), outward from the fluid cell */
+ * (efluxb.f90:66-68), outward from the fluid cell */
 /* viscous part of a record's wall force: minus the viscous momentum flux
  * leaving the fluid through the wall face (zero for Euler) */
 static void visc_wall_force(const blk_t *b, const orc_params *P, const visc_ws *V, const orc_cut *t, double *fv) {

@@ -27,21 +27,21 @@ extern "C" {
 /* error / warning codes shared with the product's C ABI (include/mge.h) */
 #define ORC_OK               0
 #define ORC_ERR_NEG_C2     -10  /* timestep: c^2 < 0 or NaN   (srcf90/timestep.f90:45-49)  */
-#define ORC_ERR_WALL_RHO   -11  /* efluxb: zero wall density   (srcf90/efluxb.f90:246-249)  */
-#define ORC_ERR_WSOL_NAN   -12  /* bc_cutcell: NaN mirror state (srcf90/bc_cutcell.f90:307-317) */
+#define ORC_ERR_WALL_RHO   -11  /* efluxb: zero wall density   (srcf90/efluxb.f90:61-64)  */
+#define ORC_ERR_WSOL_NAN   -12  /* bc_cutcell: NaN mirror state (srcf90/bc_cutcell.f90:147-157) */
 #define ORC_ERR_RES_NAN    -13  /* solver: NaN residual        (srcf90/solver.f90:180)      */
 #define ORC_ERR_PERIODIC   -14  /* bcond: unpaired periodic    (srcf90/bcond.f90:26-31)     */
 #define ORC_ERR_SMOO_DIM   -16  /* psmoo_old with < 2 cells    (srcf90/psmoo.f90:132,166)   */
 #define ORC_ERR_ARG        -20
-#define ORC_WARN_ZERO_CLOUD  1  /* bc_cutcell: empty cloud, early return (bc_cutcell.f90:225-230) */
+#define ORC_WARN_ZERO_CLOUD  1  /* bc_cutcell: empty cloud, early return (bc_cutcell.f90:65-70) */
 #define ORC_WARN_BCTYPE      2  /* bcond: unknown bctype (bcond.f90:35) */
 
 typedef struct {
   int    mstage;
   double cstp[8], cdis[8];
-  double cfl;          /* |cfl| (input_param.f90:461-463) */
+  double cfl;          /* |cfl| (input_param.f90:37-39) */
   int    vt;           /* 0: local time step, 1: global */
-  double vis2, vis4;   /* vis4 already divided by 32 (input_param.f90:464) */
+  double vis2, vis4;   /* vis4 already divided by 32 (input_param.f90:40) */
   double smoo;
   int    mgtype;       /* 1 V-cycle, 2 W-cycle */
   double gamma;
@@ -72,7 +72,7 @@ orc_case *orc_case_new(const orc_params *prm, int nlevel);
 void      orc_case_free(orc_case *c);
 
 /* Level lvl (1..nlevel, 1 = coarsest).  xv/yv/zv: vertex coordinates
- * x(1..ib) (index 0 unused, kept 0 like init_metric.f90:156).
+ * x(1..ib) (index 0 unused, kept 0 like init_metric.f90:59).
  * blank_int: ni*nj*nk bytes, i fastest, 0 = solid.  ratio: coarsening
  * ratio (1 or 2 per direction) from this level to lvl-1. */
 int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
@@ -99,7 +99,7 @@ int  orc_solve(orc_case *c, const int *mcyc, double *reslog, int cap, int *nlog,
 int  orc_solve_step(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
                     long long *cell_stages);
 int  orc_shift_time_levels(orc_case *c);                       /* solver.f90:263-276 */
-/* upload_solut.f90:97-241: 7 (Euler) or 11 (NS) fields per interior cell,
+/* upload_solut.f90:89-241: 7 (Euler) or 11 (NS) fields per interior cell,
  * field-major, i fastest; p_tot <= 0: freestream total pressure */
 int  orc_upload_solut(orc_case *c, int lvl, double p_tot, double *out);
 
@@ -127,7 +127,7 @@ int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);
 double orc_get_rtrms(orc_case *c, int lvl, int ncyc);
 double orc_get_dtim(orc_case *c, int lvl);
 /* build-defined FP64 wall force (Fx,Fy,Fz) from the wall-face flux that the
- * scheme applies (efluxb.f90:251-253), summed over records in order. */
+ * scheme applies (efluxb.f90:66-68), summed over records in order. */
 int    orc_wall_force(orc_case *c, int lvl, double out[3]);
 
 /* viscous-flux test hooks (own laminar flux; see oracle.c) */

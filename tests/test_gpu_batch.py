@@ -117,3 +117,19 @@ def test_batch_equals_alone_ns_fp32(require_gpu):
         assert st["ncyc"] == len(res)
         assert np.array_equal(log, res)
         assert np.array_equal(wb, w) and np.array_equal(pb, p)
+
+
+def test_batch_equals_alone_four_stages(require_gpu):
+    """an even stage count: the stage rotation must return each level's
+    state to the buffer the recorded program starts from after two cycles,
+    or the replays from the second period on would read a stage
+    intermediate"""
+    kw = dict(cstp=(0.25, 0.3333, 0.5, 1.0), cdis=(1.0, 0.0, 0.56, 0.44))
+    specs = _cases(**kw)[:3]
+    ref = [_alone(*s) for s in specs]
+    got = _batch(specs, True)
+    for i, ((res, w, p), (st, log, wb, pb)) in enumerate(zip(ref, got)):
+        print(f"case {i}: alone {len(res)} cycles, batch {st}")
+        assert st["ncyc"] == len(res) and st["ncyc"] > 4
+        assert np.array_equal(log, res)
+        assert np.array_equal(wb, w) and np.array_equal(pb, p)

@@ -1,4 +1,4 @@
-"""Parity of the HIP upload_solut kernel (upload_solut.f90:97-241: rho, u,
+"""Parity of the HIP upload_solut kernel (upload_solut.f90:89-241: rho, u,
 v, w, cp, Mach, total-pressure loss, and for laminar flow the energy and
 gradient slots) against the oracle on a flowed state: FP64 to rounding, the
 FP32 bulk path within 1e-5 in cp and Mach."""

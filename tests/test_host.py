@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_follow_input_param():
-    """input_param.f90:454-493 derivations"""
+    """input_param.f90:37-115 derivations"""
     p = Params(mach=0.5, alpha_deg=2.0, beta_deg=1.0, cfl=-2.5, vis4=1.0)
     d = p.derived()
     assert d["vt"] == 0 and d["cfl"] == 2.5 and d["vis4"] == 1.0 / 32

@@ -25,7 +25,7 @@ def _interior(a, n):
 
 
 def test_face_vectors_closed_form():
-    """init_metric's cross products (init_metric.f90:216-274) on an
+    """init_metric's cross products (init_metric.f90:109-177) on an
     axis-aligned grid are si=(-dy dz,0,0), sj=(0,-dx dz,0), sk=(0,0,-dx dy)
     with the rounded vertex differences; the HIP path uses this form."""
     spec = LevelSpec(n=(7, 5, 6), lo=(-1.3, 0.2, -2.0), hi=(2.1, 1.1, 0.7), ratio=(2, 2, 2),

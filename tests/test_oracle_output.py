@@ -1,4 +1,4 @@
-"""Pins of the oracle's upload_solut fields (upload_solut.f90:97-241; no
+"""Pins of the oracle's upload_solut fields (upload_solut.f90:89-241; no
 GPU) by gas-dynamics closed forms rather than by its own formulas: every
 state on the freestream's isentrope with the freestream total pressure has
 zero total-pressure loss, the Mach number it was built with, and the
