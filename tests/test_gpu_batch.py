@@ -123,8 +123,9 @@ def test_batch_equals_alone_four_stages(require_gpu):
     """an even stage count: the stage rotation must return each level's
     state to the buffer the recorded program starts from after two cycles,
     or the replays from the second period on would read a stage
-    intermediate"""
-    kw = dict(cstp=(0.25, 0.3333, 0.5, 1.0), cdis=(1.0, 0.0, 0.56, 0.44))
+    intermediate (CFL 1.5: this four-stage set diverges at 2.5 on these
+    bodies, in the oracle too)"""
+    kw = dict(cstp=(0.25, 0.3333, 0.5, 1.0), cdis=(1.0, 0.0, 0.56, 0.44), cfl=-1.5)
     specs = _cases(**kw)[:3]
     ref = [_alone(*s) for s in specs]
     got = _batch(specs, True)
