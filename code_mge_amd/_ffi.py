@@ -140,6 +140,7 @@ def lib():
     L.mge_solve_step.argtypes = [P, C.POINTER(I), I, dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
     L.mge_shift_time_levels.argtypes = [P]
     L.mge_upload_solut.argtypes = [P, I, I, C.c_double, dp, C.POINTER(I)]
+    L.mge_upload_solut_noalloc.argtypes = [P, I, I, C.c_double, dp, C.POINTER(I)]
     L.mge_case_bytes.argtypes = [P]
     L.mge_case_bytes.restype = C.c_longlong
     L.mge_autotune.argtypes = [P, I, C.POINTER(I), dp]
@@ -181,5 +182,5 @@ EXPORTED = [
     "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc", "mge_geom_level_tri",
     "mge_read_job", "mge_job_params", "mge_job_level", "mge_read_pin", "mge_read_p3d", "mge_input_error",
     "mge_write_solut", "mge_read_solut", "mge_solve_step", "mge_shift_time_levels",
-    "mge_upload_solut",
+    "mge_upload_solut", "mge_upload_solut_noalloc",
 ]

@@ -137,6 +137,8 @@ struct TStat {
 
 struct mge_case {
   mge_params prm{};
+  int dev = 0;            // HIP device of the case's stream
+  double *scratch = nullptr;  // that device's address of the fixed staging array
   bool fp32 = false;
   size_t es = 8;  // bytes per bulk value
   int nlevel = 0;
@@ -1419,6 +1421,37 @@ int batch_push(mge_batch *b) {
 
 }  // namespace
 
+// ---- fixed device scratch -----------------------------------------------
+// Staging array of mge_upload_solut_noalloc: every device the library runs
+// on holds its own copy for the life of the process.  In host code the name
+// `scratch` is only a stand-in for the symbol; its device address comes from
+// hipGetSymbolAddress, once per device (at the first mge_case_create there),
+// and only that address is ever passed to kernels or copies.
+static __device__ __align__(256) double scratch[MGE_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+std::mutex g_scr_mu;
+std::vector<double *> g_scr_ptr;  // per device, nullptr until resolved
+
+int scratch_on_device(int dev, double **out) {
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (dev < 0) return set_err(MGE_ERR_HIP, "bad device");
+  if ((int)g_scr_ptr.size() <= dev) g_scr_ptr.resize(dev + 1, nullptr);
+  if (!g_scr_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device is dev
+    if (e != hipSuccess || !p) {
+      char b[160];
+      snprintf(b, sizeof b, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return set_err(MGE_ERR_HIP, b);
+    }
+    g_scr_ptr[dev] = (double *)p;
+  }
+  *out = g_scr_ptr[dev];
+  return MGE_OK;
+}
+}  // namespace
+
 // ================================================================== C ABI
 
 extern "C" {
@@ -1436,8 +1469,14 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
   if (prm->execmode == 1 && !(prm->dtstep > 0.0)) return set_err(MGE_ERR_ARG, "unsteady flow needs dtstep > 0");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(MGE_ERR_HIP, "no HIP device");
+  int cur_dev = 0;
+  if (hipGetDevice(&cur_dev) != hipSuccess) return set_err(MGE_ERR_HIP, "hipGetDevice failed");
+  double *scr = nullptr;
+  CHKRC(scratch_on_device(cur_dev, &scr));
   mge_case *h = new mge_case();
   h->prm = *prm;
+  h->dev = cur_dev;
+  h->scratch = scr;
   h->fp32 = prm->precision == 32;
   h->es = h->fp32 ? sizeof(float) : sizeof(double);
   h->nlevel = nlevel;
@@ -1989,8 +2028,56 @@ int do_upload_solut(mge_case *h, Level &L, double ptot, double *host, int nval) 
   if (e != hipSuccess) return set_err(MGE_ERR_HIP, hipGetErrorString(e));
   return check_err(h);
 }
+
+// the same fields through the fixed scratch array: chunks of whole k-planes,
+// each written by one launch and copied out field by field; no allocation
+template <class T>
+int do_upload_solut_noalloc(mge_case *h, Level &L, double ptot, double *host, int nval) {
+  const long plane = (long)L.d.ni * L.d.nj;
+  const long nc = plane * L.d.nk;
+  const long cap = (long)(MGE_SCRATCH_BYTES / sizeof(double));
+  const long pc = std::min<long>(cap / (plane * nval), L.d.nk);  // k-planes per chunk (>= 1, checked)
+  for (long k0 = 0; k0 < L.d.nk; k0 += pc) {
+    const long c0 = k0 * plane;
+    const long ncc = std::min<long>(pc, L.d.nk - k0) * plane;
+    SolutArgs<T> A{L.d, stc<T>(L, L.cur), make_inf<T>(h->prm), h->prm.gamma, h->prm.p0,
+                   freestream_mach(h->prm), ptot, nval == 11, ncc, h->scratch};
+    A.c0 = c0;
+    A.ostride = ncc;
+    hipLaunchKernelGGL(k_upload_solut<T>, dim3((unsigned)((ncc + 255) / 256)), dim3(256), 0, h->stream, A);
+    hipError_t e = hipGetLastError();
+    for (int f = 0; f < nval && e == hipSuccess; f++)
+      e = hipMemcpyAsync(host + (long)f * nc + c0, h->scratch + (long)f * ncc, sizeof(double) * ncc,
+                         hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return set_err(MGE_ERR_HIP, hipGetErrorString(e));
+  }
+  return check_err(h);
+}
 }  // namespace
 extern "C" {
+
+int mge_upload_solut_noalloc(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval) {
+  Level *L = get_level(h, lvl);
+  if (!L || blk != 1) return set_err(MGE_ERR_ARG, "bad level/block");
+  const int nv = h->prm.eqntype == 1 ? 11 : 7;
+  if (nval) *nval = nv;
+  if (!solut) return MGE_OK;
+  const double plane_bytes = (double)L->d.ni * L->d.nj * nv * sizeof(double);
+  if (plane_bytes > (double)MGE_SCRATCH_BYTES) {
+    char b[200];
+    snprintf(b, sizeof b,
+             "mge_upload_solut_noalloc: one k-plane of output fields (%.0f bytes) exceeds the %llu-byte scratch",
+             plane_bytes, (unsigned long long)MGE_SCRATCH_BYTES);
+    return set_err(MGE_ERR_ARG, b);
+  }
+  if (!h->scratch) return set_err(MGE_ERR_STATE, "device scratch not resolved");
+  const mge_params &P = h->prm;
+  if (!(p_tot > 0.0)) {
+    const double m = freestream_mach(P);
+    p_tot = P.p0 * std::pow(1.0 + 0.2 * m * m, 3.5);
+  }
+  return by_type(h, [&](auto z) { return do_upload_solut_noalloc<decltype(z)>(h, *L, p_tot, solut, nv); });
+}
 
 int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval) {
   Level *L = get_level(h, lvl);

@@ -21,15 +21,18 @@ struct SolutArgs {
   Inf<T> inf;
   double gamma, p0, rm, ptot;
   int ns;       // laminar Navier-Stokes: 11 fields, else 7
-  long ncell;   // ni*nj*nk
-  double *out;  // nval*ncell
+  long ncell;   // cells written by this launch: c0 .. c0+ncell-1
+  double *out;  // nval*ostride: field f of cell c0+c at out[f*ostride + c]
+  long c0 = 0;        // first cell (i fastest over ni*nj*nk)
+  long ostride = 0;   // field stride of out (ncell when 0)
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_upload_solut(SolutArgs<T> A) {
   const Dims &d = A.d;
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (c >= A.ncell) return;
+  const long cl = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (cl >= A.ncell) return;
+  const long c = A.c0 + cl;
   const long nij = (long)d.ni * d.nj;
   const int k = (int)(c / nij);
   const int j = (int)((c - (long)k * nij) / d.ni);
@@ -48,8 +51,8 @@ __global__ void __launch_bounds__(256) k_upload_solut(SolutArgs<T> A) {
   const double mm = sqrt((uu * uu + vv * vv + ww * ww) / A.gamma / pp * rr);
   const double ptloss = pp * pow(1.0 + 0.2 * mm * mm, 3.5) / A.ptot - 1.0;
   pp = (pp / A.p0 - 1.0) * 2.0 / (A.gamma * A.rm * A.rm);
-  double *o = A.out + c;
-  const long nc = A.ncell;
+  double *o = A.out + cl;
+  const long nc = A.ostride ? A.ostride : A.ncell;
   o[0] = rr;
   o[nc] = uu;
   o[2 * nc] = vv;

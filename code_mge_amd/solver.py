@@ -148,16 +148,18 @@ class Case:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         _ffi.check(self.lib.mge_upload(self.h, lvl, 1, field.encode(), a.ctypes.data_as(_dp)), "mge_upload")
 
-    def upload_solut(self, lvl, p_tot=0.0):
+    def upload_solut(self, lvl, p_tot=0.0, noalloc=False):
         """upload_solut's fields of level lvl (upload_solut.f90:89-241): a dict
         name -> (nk, nj, ni) array over the interior cells; p_tot <= 0 takes
-        the freestream total pressure"""
+        the freestream total pressure.  noalloc stages them through the
+        library's fixed device scratch (mge_upload_solut_noalloc)."""
+        fn = self.lib.mge_upload_solut_noalloc if noalloc else self.lib.mge_upload_solut
+        what = "mge_upload_solut_noalloc" if noalloc else "mge_upload_solut"
         nv = C.c_int(0)
-        _ffi.check(self.lib.mge_upload_solut(self.h, lvl, 1, p_tot, None, C.byref(nv)), "mge_upload_solut")
+        _ffi.check(fn(self.h, lvl, 1, p_tot, None, C.byref(nv)), what)
         ni, nj, nk = self.levels[lvl - 1].n
         out = np.empty((nv.value, nk, nj, ni), dtype=np.float64)
-        _ffi.check(self.lib.mge_upload_solut(self.h, lvl, 1, p_tot, out.ctypes.data_as(_dp), C.byref(nv)),
-                   "mge_upload_solut")
+        _ffi.check(fn(self.h, lvl, 1, p_tot, out.ctypes.data_as(_dp), C.byref(nv)), what)
         return dict(zip(SOLUT_NAMES[:nv.value], out))
 
     def cuts(self, lvl):

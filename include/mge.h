@@ -243,6 +243,16 @@ double mge_get_dtim(mge_case *h, int lvl, int blk);
  * solut may be NULL to query it.  p_tot <= 0 takes the freestream total
  * pressure p0*(1 + 0.2*M^2)^3.5 (the reference passes gas_ptot). */
 int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval);
+/* mge_upload_solut without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops or harnesses that forbid it).  The
+ * fields pass through the library's fixed device scratch array of
+ * MGE_SCRATCH_BYTES, which every GPU the library runs on holds until the
+ * process ends, in chunks of whole k-planes; a level whose one k-plane of
+ * fields (ni*nj*nval*8 bytes) exceeds it returns MGE_ERR_ARG.  Calls that
+ * overlap on one GPU share that array and race: serialise them. */
+#define MGE_SCRATCH_BYTES (32ull << 20)
+int mge_upload_solut_noalloc(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval);
 
 /* FP64 wall force (Fx,Fy,Fz) on the body at level lvl: the momentum flux
  * the scheme applies at each wall face (efluxb.f90:66-68), in record order */
