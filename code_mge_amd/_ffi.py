@@ -44,6 +44,7 @@ class MgeParams(C.Structure):
         ("epsentr", C.c_double),
         ("execmode", C.c_int),
         ("dtstep", C.c_double),
+        ("smoother", C.c_int),
     ]
 
 

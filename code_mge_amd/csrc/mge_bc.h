@@ -25,7 +25,19 @@ struct TsArgs {
   Inf<T> inf;
   int execmode;  // 1: the pseudo step is capped by dtime
   T dtime;       // 2*dtstep/(3*cfl)
+  T *eps[3];     // psmoo's per-cell coefficients (nullptr: not formed)
+  T smoo, cflrat;  // smo and sqrt(1 + 4 smo)
 };
+
+// one coefficient of timestep.f90:286-309: the ratio r of the other two
+// radii to this one, e = max(0, min(smo, ((sqrt(1+4 smo)/(1+r/16))^2 - 1)/4))
+template <typename T>
+__device__ __attribute__((always_inline)) inline T psmoo_coef(T r, T smoo, T cflrat) {
+  T t = cflrat / (T(1) + T(0.0625) * r);
+  T ex = T(0.25) * (t * t - T(1));
+  ex = smoo < ex ? smoo : ex;
+  return T(0) > ex ? T(0) : ex;
+}
 
 // timestep.f90:36-115 (Euler) and :117-209 (NS): spectral radii, local
 // time step with the adaptive factor, global minimum; blank cells zeroed
@@ -107,6 +119,14 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
     } else {
       A.radsum[q] = T(0) + T(0) + T(0);
       A.vdtim[q] = T(0);
+    }
+    if (A.eps[0]) {
+      // a blank cell has zero radii and takes coefficient 0: its row is the
+      // identity and its zero residual stays out of its lines
+      const bool fl = A.flags[q] & F_FLUID;
+      A.eps[0][q] = fl ? psmoo_coef(rj / ri + rk / ri, A.smoo, A.cflrat) : T(0);
+      A.eps[1][q] = fl ? psmoo_coef(rk / rj + ri / rj, A.smoo, A.cflrat) : T(0);
+      A.eps[2][q] = fl ? psmoo_coef(ri / rk + rj / rk, A.smoo, A.cflrat) : T(0);
     }
   }
   // block minimum of the time step (wavefront shuffles, then the block's

@@ -81,6 +81,7 @@ struct Level {
   void *radsum = nullptr, *vdtim = nullptr;
   void *w1 = nullptr, *w2 = nullptr;  // unsteady: physical time levels n, n-1 (5E each)
   void *corr = nullptr;               // levels below the finest: correction w - ws for movfin (5E)
+  void *eps = nullptr;                // psmoo (smoother 1): coefficients x, y, z, then the sweep scratch (4E)
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
   void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z,1/e1x,1/e1y,1/e1z each npiv
@@ -424,6 +425,9 @@ int do_timestep(mge_case *h, Level &L) {
   A.err = h->err;
   A.execmode = P.execmode;
   A.dtime = (T)(2.0 * P.dtstep / (3.0 * P.cfl));
+  for (int a = 0; a < 3; a++) A.eps[a] = L.eps ? tp<T>(L.eps) + a * L.d.E : nullptr;
+  A.smoo = (T)P.smoo;
+  A.cflrat = (T)std::sqrt(1.0 + 4.0 * P.smoo);
   LAUNCHT(h, "k_timestep", k_timestep, T, cell_grid(L.d), cell_block(), A);
   LAUNCHN(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(64), D);
   return MGE_OK;
@@ -600,7 +604,25 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       LAUNCHT(h, "k_rec_apply", k_rec_apply, T, dim3((L.ntouch + 63) / 64), dim3(64), R);
     }
     if (lres) return MGE_OK;
-    if (A.smoothed) {
+    if (A.smoothed && P.smoother == 1) {
+      // psmoo with progon3's sweep: x, y, then z lines (psmoo.f90:40-87)
+      PsmooArgs<T> S{d, dwp, tp<T>(L.eps), tp<T>(L.eps) + 3 * E};
+      auto lines = [](long nl) { return dim3((unsigned)((nl + 63) / 64)); };
+      LAUNCHT(h, "k_psmoo_x", k_psmoo_x, T, lines((long)d.nj * d.nk), dim3(64), S);
+      S.eps = tp<T>(L.eps) + E;
+      LAUNCHT(h, "k_psmoo_y", k_psmoo_y, T, lines((long)d.ni * d.nk), dim3(64), S);
+      S.eps = tp<T>(L.eps) + 2 * E;
+      LAUNCHT(h, "k_psmoo_z", k_psmoo_z, T, lines((long)d.ni * d.nj), dim3(64), S);
+      UpdArgs<T> U{d, (const T *)tp<T>(L.S[w0]), (const T *)dwp, stc<T>(L, src), stm<T>(L, dst),
+                   (T)P.gamma, make_inf<T>(P)};
+      U.ulam = A.ulam == 2;
+      U.vdtim = A.vdtim;
+      U.dtim = A.dtim;
+      U.fn = A.fn;
+      U.dtstep = A.dtstep;
+      U.vt = A.vt;
+      LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256), U);
+    } else if (A.smoothed) {
       SmoothArgs<T> S;
       S.d = d;
       S.dw = dwp;
@@ -1467,6 +1489,7 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
     return set_err(MGE_ERR_ARG, "precision must be 64 or 32");
   if (prm->execmode != 0 && prm->execmode != 1) return set_err(MGE_ERR_ARG, "execmode must be 0 or 1");
   if (prm->execmode == 1 && !(prm->dtstep > 0.0)) return set_err(MGE_ERR_ARG, "unsteady flow needs dtstep > 0");
+  if (prm->smoother != 0 && prm->smoother != 1) return set_err(MGE_ERR_ARG, "smoother must be 0 (psmoo_old) or 1 (psmoo)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(MGE_ERR_HIP, "no HIP device");
   int cur_dev = 0;
@@ -1544,6 +1567,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc_bytes(L, &L.fw, es * 5 * E));
   CHKRC(dalloc_bytes(L, &L.radsum, es * E));
   CHKRC(dalloc_bytes(L, &L.vdtim, es * E));
+  if (h->prm.smoother == 1 && h->prm.smoo > 0.0) CHKRC(dalloc_bytes(L, &L.eps, es * 4 * E));
   CHKRC(dalloc(L, &L.flags, E));
   CHKRC(dalloc(L, &L.marker, E));
   int nmax = std::max(d.ib, std::max(d.jb, d.kb)) + 2;
@@ -2129,7 +2153,7 @@ int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc) {
     if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
   if (h->lv[h->nlevel].mxcyc < 2) return set_err(MGE_ERR_ARG, "a batched case needs mxcyc >= 2");
   if (h->prm.execmode != 0) return set_err(MGE_ERR_ARG, "unsteady cases are solved one at a time, not batched");
-  const int sm = h->prm.smoo > 0.0;
+  const int sm = h->prm.smoo > 0.0 ? 1 + h->prm.smoother : 0;  // smoothing off, psmoo_old, psmoo
   if (b->fp32 < 0) {
     b->nlevel = h->nlevel;
     b->fp32 = h->fp32;

@@ -476,6 +476,7 @@ int mge_job_params(const mge_job *job, int precision, mge_params *P) {
   const double vel_fs = rm * std::sqrt(g * job->pmean / rhomean);
   P->dtstep = job->dt * vel_fs / 1.0;
   if (job->eqntype == 1 && job->execmode == 1) P->dtstep = P->u0 / 1.0 / 5.0;
+  P->smoother = 0;  // euler.f90:376 calls psmoo_old
   return MGE_OK;
 }
 

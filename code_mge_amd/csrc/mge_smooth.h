@@ -540,4 +540,80 @@ __device__ __attribute__((always_inline)) inline void k_update_ghost_body(const 
 }
 MGE_ENTRY_T(k_update_ghost, UpdArgs, 256, 1)
 
+
+// ---- psmoo (psmoo.f90:1-95) with progon3's sweep (progon3.f90:1-62) ------
+// Rows (1+2e_r) x_r - e_r x_{r-1} - e_r x_{r+1} = d_r along each line of
+// direction DIR with the per-cell coefficients e of timestep (the end
+// couplings removed, as psmoo passes them, so progon3's sweep is the Thomas
+// recurrence).  One thread per line carries all five components: the
+// forward sweep leaves u_{r+1} in dw and v_{r+1} in vb, the backward sweep
+// forms x_r = v_{r+1} x_{r+1} + u_{r+1} in place.  Lines are numbered with
+// the fastest index of the other two first, so neighbouring threads touch
+// neighbouring cells on y and z lines.
+template <typename T>
+struct PsmooArgs {
+  Dims d;
+  T *dw;
+  const T *eps;  // this direction's coefficients
+  T *vb;         // per-cell scratch of the forward sweep (E)
+};
+
+template <int DIR, typename T>
+__device__ __attribute__((always_inline)) inline void k_psmoo_line_body(const PsmooArgs<T> &A, int bx, int, int) {
+  const Dims &d = A.d;
+  const int t = bx * blockDim.x + threadIdx.x;
+  int n, na, nb;
+  long st;
+  if (DIR == 0) { n = d.ni; na = d.nj; nb = d.nk; st = 1; }
+  else if (DIR == 1) { n = d.nj; na = d.ni; nb = d.nk; st = d.sy; }
+  else { n = d.nk; na = d.ni; nb = d.nj; st = d.sz; }
+  if (t >= na * nb) return;
+  const int a = 2 + t % na, b = 2 + t / na;
+  long q;
+  if (DIR == 0) q = ix(d, 2, a, b);
+  else if (DIR == 1) q = ix(d, a, 2, b);
+  else q = ix(d, a, b, 2);
+  const long E = d.E;
+  T v = T(0), u[5] = {T(0), T(0), T(0), T(0), T(0)};
+  for (int r = 1; r <= n; r++, q += st) {
+    const T e = A.eps[q];
+    const T aa = T(1) + T(2) * e;
+    const T bb = r == n ? T(0) : -e;
+    const T cc = r == 1 ? T(0) : -e;
+    const T z = T(1) / (aa + cc * v);
+    v = -bb * z;
+    A.vb[q] = v;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      u[m] = (-cc * u[m] + A.dw[m * E + q]) * z;
+      A.dw[m * E + q] = u[m];
+    }
+  }
+  q -= st;  // last cell: x_n = u_{n+1}, already in place
+  for (int r = n - 1; r >= 1; r--) {
+    const T vn = A.vb[q - st];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      u[m] = vn * u[m] + A.dw[m * E + q - st];
+      A.dw[m * E + q - st] = u[m];
+    }
+    q -= st;
+  }
+}
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_psmoo_x_body(const PsmooArgs<T> &A, int bx, int by, int bz) {
+  k_psmoo_line_body<0, T>(A, bx, by, bz);
+}
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_psmoo_y_body(const PsmooArgs<T> &A, int bx, int by, int bz) {
+  k_psmoo_line_body<1, T>(A, bx, by, bz);
+}
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_psmoo_z_body(const PsmooArgs<T> &A, int bx, int by, int bz) {
+  k_psmoo_line_body<2, T>(A, bx, by, bz);
+}
+MGE_ENTRY_T(k_psmoo_x, PsmooArgs, 64, 1)
+MGE_ENTRY_T(k_psmoo_y, PsmooArgs, 64, 1)
+MGE_ENTRY_T(k_psmoo_z, PsmooArgs, 64, 1)
+
 }  // namespace mge

@@ -21,6 +21,7 @@ module mgeapi
     real(c_double)  :: epsentr
     integer(c_int)  :: execmode
     real(c_double)  :: dtstep
+    integer(c_int)  :: smoother
   end type
 
   ! what a jobsheet defines (mge_job)

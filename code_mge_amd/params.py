@@ -37,6 +37,7 @@ class Params:
     epsentr: float = 0.05             # Roe entropy fix (jobsheet r2_epsentr)
     execmode: int = 0                 # 0 steady, 1 unsteady dual time stepping
     dtstep: float = 0.0               # physical time step in units of ref_leng/|U_inf|
+    smoother: int = 0                 # 0 psmoo_old (constant coefficient), 1 psmoo + progon3
     extra: dict = field(default_factory=dict)
 
     def derived(self):
@@ -95,6 +96,7 @@ class Params:
         P.epsentr = self.epsentr
         P.execmode = d["execmode"]
         P.dtstep = d["dtstep"]
+        P.smoother = self.smoother
         return P
 
     def dynamic_pressure(self):

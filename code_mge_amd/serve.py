@@ -58,10 +58,11 @@ class Result:
         return self.completed - self.arrival
 
 
-def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False):
+def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False):
     """device bytes of one level as the library allocates them (every array
     padded by 64 bytes); wall-record arrays at their upper bound; a level
-    below the finest also holds the prolongation's correction"""
+    below the finest also holds the prolongation's correction; the
+    variable-coefficient smoother holds its coefficients and sweep scratch"""
     ni, nj, nk = n
     E = (ni + 4) * (nj + 4) * (nk + 4)
     nmax = max(ni, nj, nk) + 5
@@ -72,6 +73,8 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False):
         sizes += [es * 4 * E]           # velocity and temperature
     if coarse:
         sizes += [es * 5 * E]           # correction w - ws
+    if psmoo:
+        sizes += [es * 4 * E]           # psmoo coefficients (3) and sweep scratch
     if nmir:
         ntouch, nops = 2 * nmir, 6 * nmir
         sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
@@ -96,7 +99,7 @@ class Server:
 
     def _class(self, prm: Params, levels) -> tuple:
         return (prm.eqntype.lower(), prm.distype.lower(), prm.dissip.lower(), len(levels), len(prm.cstp),
-                prm.mgtype, prm.smoo > 0.0, self.precision)
+                prm.mgtype, prm.smoo > 0.0, prm.smoother, self.precision)
 
     def estimate(self, req: Request, geoms=None) -> int:
         es = 4 if self.precision == 32 else 8
@@ -104,7 +107,8 @@ class Server:
         for li, spec in enumerate(req.levels):
             nmir = geoms[li].nmir if geoms is not None else 0
             total += level_bytes(spec.n, es, nmir, viscous=req.params.eqntype.lower() == "n",
-                                 coarse=li < len(req.levels) - 1)
+                                 coarse=li < len(req.levels) - 1,
+                                 psmoo=req.params.smoother == 1 and req.params.smoo > 0.0)
         return total
 
     def run(self, requests: List[Request], clock=time.monotonic) -> List[Result]:

@@ -91,6 +91,11 @@ typedef struct {
                                  capped by the physical one (jobsheet runtype)     */
   double dtstep;              /* nondimensional physical time step (> 0 when
                                  execmode = 1; input_param.f90:81-96)              */
+  int    smoother;            /* implicit residual smoothing: 0 psmoo_old, the
+                                 constant coefficient smo that euler.f90:376 calls;
+                                 1 psmoo, per-cell coefficients from timestep
+                                 (timestep.f90:286-309) solved line by line as
+                                 progon3 does (psmoo.f90:1-95, progon3.f90:1-62) */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record

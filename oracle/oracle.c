@@ -25,6 +25,7 @@ typedef struct {
   double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
   double *w1, *w2;        /* physical time levels n and n-1 (unsteady, make_flow.f90:38-39) */
   double *radi, *radj, *radk, *vdtim;
+  double *epsx, *epsy, *epsz;   /* smoothing coefficients of psmoo (timestep.f90:286-309) */
   double dtim;
   int *blank, *marker;
   int nmir;
@@ -87,6 +88,7 @@ static void blk_free(blk_t *b) {
   free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
   free(b->w1); free(b->w2);
   free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
+  free(b->epsx); free(b->epsy); free(b->epsz);
   free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
 }
 
@@ -136,6 +138,9 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->radi = (double *)calloc((size_t)E, sizeof(double));
   b->radj = (double *)calloc((size_t)E, sizeof(double));
   b->radk = (double *)calloc((size_t)E, sizeof(double));
+  b->epsx = (double *)calloc((size_t)E, sizeof(double));
+  b->epsy = (double *)calloc((size_t)E, sizeof(double));
+  b->epsz = (double *)calloc((size_t)E, sizeof(double));
   b->vdtim = (double *)calloc((size_t)E, sizeof(double));
   b->blank = (int *)calloc((size_t)E, sizeof(int));
   b->marker = (int *)calloc((size_t)E, sizeof(int));
@@ -223,8 +228,8 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
 
 /* ------------------------------------------------------------ timestep */
 
-/* timestep.f90:1-282 (the smoothing coefficients of :286-309 feed only the
- * unused psmoo and are not formed). */
+/* timestep.f90:1-309; the smoothing coefficients of :286-309 feed psmoo
+ * (the variable-coefficient smoother, selected by smoother = 1). */
 int orc_timestep(orc_case *c, int lvl) {
   blk_t *b = &c->lv[lvl];
   const orc_params *P = &c->prm;
@@ -333,6 +338,39 @@ int orc_timestep(orc_case *c, int lvl) {
         R[r][IX(b, i, je, k)] = R[r][IX(b, i, jl, k)];
         R[r][IX(b, i, jb, k)] = R[r][IX(b, i, jl, k)];
       }
+  /* coefficients of the implicit residual smoothing (timestep.f90:286-309).
+   * A blank cell has zero radii, so the reference's ratios are 0/0 there
+   * and its min/max of a NaN is processor-dependent; here its coefficient
+   * is 0, which keeps the blank cell's zero residual out of its lines. */
+  if (P->smoo > 0.0) {
+    const double cflrat = sqrt(1.0 + 4.0 * P->smoo);
+    for (int k = 2; k <= kl; k++)
+      for (int j = 2; j <= jl; j++)
+        for (int i = 2; i <= il; i++) {
+          long q = IX(b, i, j, k);
+          if (b->blank[q] == 0) {
+            b->epsx[q] = 0.0; b->epsy[q] = 0.0; b->epsz[q] = 0.0;
+            continue;
+          }
+          double ri = b->radi[q], rj = b->radj[q], rk = b->radk[q];
+          double reval, ex, t;
+          reval = rj / ri + rk / ri;
+          t = cflrat / (1.0 + 0.0625 * reval);
+          ex = 0.25 * (t * t - 1.0);
+          ex = min2(P->smoo, ex);
+          b->epsx[q] = max2(0.0, ex);
+          reval = rk / rj + ri / rj;
+          t = cflrat / (1.0 + 0.0625 * reval);
+          ex = 0.25 * (t * t - 1.0);
+          ex = min2(P->smoo, ex);
+          b->epsy[q] = max2(0.0, ex);
+          reval = ri / rk + rj / rk;
+          t = cflrat / (1.0 + 0.0625 * reval);
+          ex = 0.25 * (t * t - 1.0);
+          ex = min2(P->smoo, ex);
+          b->epsz[q] = max2(0.0, ex);
+        }
+  }
   return ORC_OK;
 }
 
@@ -1235,6 +1273,81 @@ static int roe_fluxes(blk_t *b, const orc_params *P, int istage) {
 /* ------------------------------------------------------------ smoothing */
 
 /* psmoo_old (psmoo.f90:97-242) */
+/* progon3 (progon3.f90:1-62): the sweep method for
+ *   a(1)x(1) + b(1)x(2) + c(1)x(n) = d(1),
+ *   c(i)x(i-1) + a(i)x(i) + b(i)x(i+1) = d(i),
+ *   b(n)x(1) + c(n)x(n-1) + a(n)x(n) = d(n);
+ * arrays 1-based as in the reference (index 0 unused), work arrays of n+1 */
+static void progon3(const double *a, const double *bb, const double *c, const double *d, double *x, int n,
+                    double *w, double *s, double *t, double *u, double *v) {
+  u[1] = 0.0;
+  v[1] = 0.0;
+  w[1] = 1.0;
+  for (int i = 1; i <= n; i++) {
+    int i1 = i + 1;
+    double z = 1.0 / (a[i] + c[i] * v[i]);
+    v[i1] = -bb[i] * z;
+    u[i1] = (-c[i] * u[i] + d[i]) * z;
+    w[i1] = -c[i] * w[i] * z;
+  }
+  s[n] = 1.0;
+  t[n] = 0.0;
+  for (int i = n - 1; i >= 1; i--) {
+    s[i] = v[i + 1] * s[i + 1] + w[i + 1];
+    t[i] = v[i + 1] * t[i + 1] + u[i + 1];
+  }
+  x[n] = (d[n] - bb[n] * t[1] - c[n] * t[n - 1]) / (a[n] + bb[n] * s[1] + c[n] * s[n - 1]);
+  for (int i = 1; i <= n - 1; i++) x[i] = s[i] * x[n] + t[i];
+}
+
+/* psmoo (psmoo.f90:1-95): the smoother with the per-cell coefficients of
+ * timestep; rows (1+2e_i, -e_i, -e_i) along each line with the end
+ * couplings removed (bb(il) = cc(2) = 0), solved by progon3 for every
+ * component, i lines, then j lines, then k lines */
+static int psmoo_var(blk_t *b, const orc_params *P) {
+  if (P->smoo <= 0.0) return ORC_OK;
+  if (b->ni < 2 || b->nj < 2 || b->nk < 2) return ORC_ERR_SMOO_DIM;
+  const int il = b->il, jl = b->jl, kl = b->kl;
+  const long E = b->E;
+  int mx = b->ib > b->jb ? b->ib : b->jb;
+  if (b->kb > mx) mx = b->kb;
+  double *buf = (double *)calloc((size_t)(mx + 2) * 10, sizeof(double));
+  double *aa = buf, *bb = aa + mx + 2, *cc = bb + mx + 2, *dd = cc + mx + 2, *xx = dd + mx + 2;
+  double *ww = xx + mx + 2, *ss = ww + mx + 2, *tt = ss + mx + 2, *uu = tt + mx + 2, *vv = uu + mx + 2;
+  for (int dir = 0; dir < 3; dir++) {
+    const double *eps = dir == 0 ? b->epsx : (dir == 1 ? b->epsy : b->epsz);
+    int nl = dir == 0 ? il : (dir == 1 ? jl : kl);
+    int n = nl - 1;
+    int o1max = dir == 0 ? kl : (dir == 1 ? kl : il);   /* outer loops as psmoo.f90:42-87 */
+    int o2max = dir == 0 ? jl : (dir == 1 ? il : jl);
+    for (int o1 = 2; o1 <= o1max; o1++)
+      for (int o2 = 2; o2 <= o2max; o2++) {
+        long q0;
+        long st;
+        if (dir == 0) { q0 = IX(b, 0, o2, o1); st = 1; }
+        else if (dir == 1) { q0 = IX(b, o2, 0, o1); st = b->ib + 1; }
+        else { q0 = IX(b, o1, o2, 0); st = (long)(b->ib + 1) * (b->jb + 1); }
+        /* reference arrays aa(2:nl) map to 1..n here */
+        for (int s_ = 2; s_ <= nl; s_++) {
+          double e = eps[q0 + st * s_];
+          aa[s_ - 1] = 1.0 + 2.0 * e;
+          bb[s_ - 1] = -e;
+          cc[s_ - 1] = -e;
+        }
+        bb[n] = 0.0;
+        cc[1] = 0.0;
+        for (int m = 0; m < 5; m++) {
+          double *dwn = b->dw + (long)m * E + q0;
+          for (int s_ = 2; s_ <= nl; s_++) dd[s_ - 1] = dwn[st * s_];
+          progon3(aa, bb, cc, dd, xx, n, ww, ss, tt, uu, vv);
+          for (int s_ = 2; s_ <= nl; s_++) dwn[st * s_] = xx[s_ - 1];
+        }
+      }
+  }
+  free(buf);
+  return ORC_OK;
+}
+
 static int psmoo_old(blk_t *b, const orc_params *P) {
   if (P->smoo <= 0.0) return ORC_OK;
   const int ilm = b->ni, jlm = b->nj, klm = b->nk;
@@ -1652,8 +1765,8 @@ int orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl, int 
             b->dw[q] = de * b->dw[q];
           }
         }
-    if (m % 2 != 0) {
-      rc = psmoo_old(b, P);
+    if (m % 2 != 0) { /* euler.f90:374-377 calls psmoo_old; smoother = 1 selects psmoo */
+      rc = P->smoother == 1 ? psmoo_var(b, P) : psmoo_old(b, P);
       if (rc) return rc;
     }
     for (int k = 2; k <= kl; k++)
@@ -2019,6 +2132,9 @@ static double *field_ptr(blk_t *b, const char *name, long *n) {
   if (!strcmp(name, "radj")) return b->radj;
   if (!strcmp(name, "radk")) return b->radk;
   if (!strcmp(name, "vdtim")) return b->vdtim;
+  if (!strcmp(name, "epsx")) return b->epsx;
+  if (!strcmp(name, "epsy")) return b->epsy;
+  if (!strcmp(name, "epsz")) return b->epsz;
   return NULL;
 }
 
@@ -2108,6 +2224,17 @@ int orc_wall_force(orc_case *c, int lvl, double out[3]) {
 }
 
 int orc_psmoo(orc_case *c, int lvl) { return psmoo_old(&c->lv[lvl], &c->prm); }
+int orc_psmoo_var(orc_case *c, int lvl) { return psmoo_var(&c->lv[lvl], &c->prm); }
+
+void orc_progon3(const double *a, const double *bb, const double *c, const double *d, double *x, int n) {
+  double *wk = (double *)calloc((size_t)(n + 2) * 10, sizeof(double));
+  double *A = wk, *B = A + n + 2, *Cc = B + n + 2, *D = Cc + n + 2, *X = D + n + 2;
+  double *w = X + n + 2, *s = w + n + 2, *t = s + n + 2, *u = t + n + 2, *v = u + n + 2;
+  for (int i = 0; i < n; i++) { A[i + 1] = a[i]; B[i + 1] = bb[i]; Cc[i + 1] = c[i]; D[i + 1] = d[i]; }
+  progon3(A, B, Cc, D, X, n, w, s, t, u, v);
+  for (int i = 0; i < n; i++) x[i] = X[i + 1];
+  free(wk);
+}
 
 void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const double *dw,
                       const double *ss, double *ff) {

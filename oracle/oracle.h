@@ -54,6 +54,8 @@ typedef struct {
   double epsentr;      /* Roe entropy-fix coefficient */
   int    execmode;     /* 0 steady, 1 unsteady dual time stepping (BDF2 physical time) */
   double dtstep;       /* nondimensional physical time step (input_param.f90:81-96) */
+  int    smoother;     /* 0 psmoo_old (constant coefficient, what euler.f90:376 calls);
+                          1 psmoo + progon3 (per-cell coefficients, psmoo.f90:1-95) */
 } orc_params;
 
 typedef struct {
@@ -106,6 +108,11 @@ int  orc_upload_solut(orc_case *c, int lvl, double p_tot, double *out);
 /* unit-level entry points for the tests: psmoo_old on the level's dw
  * (psmoo.f90:97-242) and dflux_matrix (dflux.f90:283-339) */
 int  orc_psmoo(orc_case *c, int lvl);
+/* psmoo (psmoo.f90:1-95, progon3.f90:1-62) on the level's dw with the
+ * coefficients epsx/epsy/epsz of the last orc_timestep */
+int  orc_psmoo_var(orc_case *c, int lvl);
+/* progon3 itself on 0-based arrays of n (n >= 2) */
+void orc_progon3(const double *a, const double *b, const double *c, const double *d, double *x, int n);
 /* one Roe face (own design, no reference source): primitive states qL, qR
  * (rho,u,v,w,p), face vector S; e = central part 0.5*(F(UL)+F(UR)),
  * d = cdis*0.5*|A_roe|(UR-UL) with the Harten entropy fix eps.  With
@@ -120,7 +127,7 @@ void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const do
                       const double *ss, double *ff);
 
 /* field access.  name: "w","p","dw","fw","ws","wn","wr","qf","radi","radj",
- * "radk","vdtim","si","sj","sk","w1","w2".  Copies the full (0:ib,...) array. */
+ * "radk","vdtim","si","sj","sk","w1","w2","epsx","epsy","epsz".  Copies the full (0:ib,...) array. */
 int    orc_get(orc_case *c, int lvl, const char *name, double *out);
 int    orc_set(orc_case *c, int lvl, const char *name, const double *in);
 int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);

@@ -26,6 +26,7 @@ class OrcParams(C.Structure):
         ("drop", C.c_double),
         ("distype", C.c_int), ("epsentr", C.c_double),
         ("execmode", C.c_int), ("dtstep", C.c_double),
+        ("smoother", C.c_int),
     ]
 
 
@@ -78,6 +79,8 @@ def lib():
     L.orc_get_dtim.restype = C.c_double
     L.orc_wall_force.argtypes = [P, I, _dp]
     L.orc_psmoo.argtypes = [P, I]
+    L.orc_psmoo_var.argtypes = [P, I]
+    L.orc_progon3.argtypes = [_dp, _dp, _dp, _dp, _dp, I]
     L.orc_dflux_matrix.argtypes = [C.c_double, _dp, _dp, _dp, _dp, _dp]
     L.orc_visc_face.argtypes = [I, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                 C.c_double, _dp]
@@ -114,6 +117,7 @@ def params_from(prm):
     P.epsentr = prm.epsentr
     P.execmode = d["execmode"]
     P.dtstep = d["dtstep"]
+    P.smoother = prm.smoother
     return P
 
 
@@ -262,9 +266,21 @@ class OracleCase:
     def psmoo(self, lvl):
         return self._chk(self.L.orc_psmoo(self.h, lvl), "psmoo")
 
+    def psmoo_var(self, lvl):
+        return self._chk(self.L.orc_psmoo_var(self.h, lvl), "psmoo_var")
+
     def visc(self, lvl, cdis=1.0):
         """one viscous pass into fw (zeroed first)"""
         return self._chk(self.L.orc_visc(self.h, lvl, cdis), "visc")
+
+
+def progon3(a, b, c, d):
+    """the oracle's progon3 (0-based arrays of length n)"""
+    L = lib()
+    arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d)]
+    x = np.zeros(len(arr[0]))
+    L.orc_progon3(*[v.ctypes.data_as(_dp) for v in arr], x.ctypes.data_as(_dp), len(x))
+    return x
 
 
 def dflux_matrix(gamma, u1, u2, dw, ss):
