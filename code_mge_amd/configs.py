@@ -12,7 +12,7 @@ import math
 
 import numpy as np
 
-from .grid import body, hierarchy
+from .grid import body, geometric_vertices, hierarchy, stretched_hierarchy
 from .params import Params
 
 FAR, SYM, PER, WALL = -20, -50, -60, -80
@@ -26,6 +26,23 @@ def cylinder2d(n=128, nz=4, nlevel=3, mach=0.2, alpha=0.0, half=10.0, **kw):
     prm = Params(mach=mach, alpha_deg=alpha, **kw)
     bdy = body("cylinder", center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2)
     ref_area = 1.0 * (nz * h)  # diameter x span
+    return prm, levels, bdy, ref_area
+
+
+def cylinder2d_stretched(n=512, nz=4, nlevel=4, mach=0.1, alpha=0.0, half=20.0, ratio=1.02, **kw):
+    """The laminar-cylinder grid: n x n x nz cells on [-half, half]^2 whose
+    widths grow geometrically (`ratio`) away from the body in x and y, z
+    uniform with the mean x spacing and periodic; far field on x/y faces.
+    Laminar Navier-Stokes at Re = 40, Pr = 0.72 unless overridden."""
+    xv = geometric_vertices(n, half, ratio)
+    hz = 2.0 * half / n
+    zv = hz * np.arange(nz + 1, dtype=np.float64)
+    levels = stretched_hierarchy((xv, xv.copy(), zv), nlevel, bctype=(FAR, FAR, PER, FAR, FAR, PER))
+    kw.setdefault("eqntype", "n")
+    kw.setdefault("reynolds", 40.0)
+    prm = Params(mach=mach, alpha_deg=alpha, **kw)
+    bdy = body("cylinder", center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2)
+    ref_area = 1.0 * (nz * hz)  # diameter x span
     return prm, levels, bdy, ref_area
 
 

@@ -18,6 +18,7 @@ struct TsArgs {
   T *radsum, *vdtim;
   unsigned long long *dtim_bits;
   T gamma, cfl, cellvolume;
+  const T *vol;  // stretched level: per-cell volumes (else nullptr)
   int eqntype;
   T viscl, prlmean;
   T cfac;  // viscous factor of the scheme (timestep.f90:119-124)
@@ -69,12 +70,13 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
     T uv = m2 / rho;
     T uw = m3 / rho;
     T dtv = T(0);
+    const T vol = A.vol ? A.vol[q] : A.cellvolume;
     if (A.eqntype == 1) {
       T fmue = A.viscl / A.prlmean;
       T f1 = T(4) / rho / T(3);
       T f2 = A.gamma / rho;
       T fac = maxd(f1, f2);
-      dtv = fac * fmue / A.cellvolume;
+      dtv = fac * fmue / vol;
     }
     T sx = face_s<0>(A.g, i, j, k);
     T sy = face_s<1>(A.g, i, j, k);
@@ -93,8 +95,8 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
     cs = cc * sqrt(ds);
     T rk = qs + cs, rvk = dtv * ds;
     T fict;
-    if (A.eqntype == 1) fict = A.cellvolume / (ri + rj + rk + A.cfac * (rvi + rvj + rvk));
-    else fict = A.cellvolume / (ri + rj + rk);
+    if (A.eqntype == 1) fict = vol / (ri + rj + rk + A.cfac * (rvi + rvj + rvk));
+    else fict = vol / (ri + rj + rk);
     // unsteady: capped by the physical step (timestep.f90:86-99, 180-192)
     if (A.execmode == 1) fict = fict < A.dtime ? fict : A.dtime;
     const T plim = T(0.001);
@@ -646,6 +648,10 @@ struct XferArgs {
   const uint8_t *flagsf;
   T *wfo;                // fine state (prolongation, in place)
   T *corr;               // coarse correction w - ws (5 components, coarse cells)
+  // stretched tensor levels: per-cell volumes and the 1-D weight pairs of
+  // the restriction (per coarse index) and prolongation (per fine index)
+  const T *vc, *vf;
+  const T *rw[3], *pw[3];
 };
 
 // the coarse-level correction w - ws of every coarse cell, formed once so
@@ -687,6 +693,11 @@ __device__ __attribute__((always_inline)) inline void k_restrict_body(const Xfer
   rstencil(i, X.rx, &s[0], fa);
   rstencil(j, X.ry, &s[1], fb);
   rstencil(k, X.rz, &s[2], fc);
+  if (X.rw[0]) {
+    fa[0] = X.rw[0][2 * i]; fa[1] = X.rw[0][2 * i + 1];
+    fb[0] = X.rw[1][2 * j]; fb[1] = X.rw[1][2 * j + 1];
+    fc[0] = X.rw[2][2 * k]; fc[1] = X.rw[2][2 * k + 1];
+  }
   for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
   long qc = ix(c, i, j, k);
   long qf[8];
@@ -694,10 +705,10 @@ __device__ __attribute__((always_inline)) inline void k_restrict_body(const Xfer
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
     for (int c8 = 0; c8 < 8; c8++) {
-      T val = X.dwf[n * f.E + qf[c8]] / X.volf;
+      T val = X.dwf[n * f.E + qf[c8]] / (X.vf ? X.vf[qf[c8]] : X.volf);
       acc = acc + val * fac[c8];
     }
-    X.wr[n * c.E + qc] = acc * X.volc;
+    X.wr[n * c.E + qc] = acc * (X.vc ? X.vc[qc] : X.volc);
   }
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
@@ -722,6 +733,11 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   pstencil(i, X.rx, &s[0], fa);
   pstencil(j, X.ry, &s[1], fb);
   pstencil(k, X.rz, &s[2], fc);
+  if (X.pw[0]) {
+    fa[0] = X.pw[0][2 * i]; fa[1] = X.pw[0][2 * i + 1];
+    fb[0] = X.pw[1][2 * j]; fb[1] = X.pw[1][2 * j + 1];
+    fc[0] = X.pw[2][2 * k]; fc[1] = X.pw[2][2 * k + 1];
+  }
   for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
   long qf = ix(f, i, j, k);
   if (!(X.flagsf[qf] & F_FLUID)) return;

@@ -82,6 +82,12 @@ struct Level {
   void *w1 = nullptr, *w2 = nullptr;  // unsteady: physical time levels n, n-1 (5E each)
   void *corr = nullptr;               // levels below the finest: correction w - ws for movfin (5E)
   void *eps = nullptr;                // psmoo (smoother 1): coefficients x, y, z, then the sweep scratch (4E)
+  // stretched tensor level (cellvolume 0): per-cell volumes (E), the vertex
+  // coordinates, and the 1-D weight pairs of the transfers from and to the
+  // next coarser level (restriction per coarse index, prolongation per index)
+  void *vol = nullptr;
+  std::vector<double> hv[3];
+  void *rw[3] = {nullptr, nullptr, nullptr}, *pw[3] = {nullptr, nullptr, nullptr};
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
   void *piv = nullptr;  // e1x,e2x,e1y,e2y,e1z,e2z,1/e1x,1/e1y,1/e1z each npiv
@@ -406,7 +412,7 @@ int do_timestep(mge_case *h, Level &L) {
   const mge_params &P = h->prm;
   const DtimArgs D{L.dtim_bits, L.dtim};
   LAUNCHN(h, "k_dtim_init", k_dtim_init, dim3(1), dim3(64), D);
-  TsArgs<T> A;
+  TsArgs<T> A{};
   A.d = L.d;
   A.s = stc<T>(L, L.cur);
   A.g = geo<T>(L);
@@ -417,6 +423,7 @@ int do_timestep(mge_case *h, Level &L) {
   A.gamma = (T)P.gamma;
   A.cfl = (T)P.cfl;
   A.cellvolume = (T)L.cellvolume;
+  A.vol = L.vol ? tp<T>(L.vol) : nullptr;
   A.eqntype = P.eqntype;
   A.viscl = (T)P.viscl;
   A.prlmean = (T)P.prlmean;
@@ -492,6 +499,7 @@ StageArgs<T> stage_args(mge_case *h, Level &L, int m, int src, int dst, int w0, 
   A.vdtim = tp<T>(L.vdtim);
   A.dtim = L.dtim;
   A.cellvolume = (T)L.cellvolume;
+  A.vol = L.vol ? tp<T>(L.vol) : nullptr;
   A.blend = (T)(1.0 - P.cdis[m - 1]);
   A.fn = (T)(P.cstp[m - 1] * P.cfl);
   A.m = m;
@@ -719,6 +727,11 @@ int do_movco(mge_case *h, int mgl) {
   X.marker = c.marker;
   X.volc = (T)c.cellvolume;
   X.volf = (T)f.cellvolume;
+  X.vc = c.vol ? tp<T>(c.vol) : nullptr;
+  X.vf = f.vol ? tp<T>(f.vol) : nullptr;
+  for (int a = 0; a < 3; a++) X.rw[a] = f.rw[a] ? tp<T>(f.rw[a]) : nullptr;
+  if ((X.vc != nullptr) != (X.vf != nullptr) || (X.vf && !X.rw[0]))
+    return set_err(MGE_ERR_ARG, "a stretched level and a uniform level do not nest");
   X.fmg = 0;
   X.flagsf = f.flags;
   X.wfo = nullptr;
@@ -743,6 +756,9 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.marker = nullptr;
   X.volc = (T)c.cellvolume;
   X.volf = (T)f.cellvolume;
+  for (int a = 0; a < 3; a++) X.pw[a] = f.pw[a] ? tp<T>(f.pw[a]) : nullptr;
+  if ((c.vol != nullptr) != (f.vol != nullptr) || (f.vol && !X.pw[0]))
+    return set_err(MGE_ERR_ARG, "a stretched level and a uniform level do not nest");
   X.fmg = (mgl - 1 == mlvl);
   X.flagsf = f.flags;
   X.wfo = tp<T>(f.S[f.cur]);
@@ -1152,6 +1168,51 @@ int put_bulk(mge_case *h, void *dev, const double *src, size_t n) {
   std::vector<float> t(n);
   for (size_t q = 0; q < n; q++) t[q] = (float)src[q];
   HIPCHK(hipMemcpy(dev, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  return MGE_OK;
+}
+
+// Transfer weights between a stretched level lvl and the coarser level
+// lvl-1, from the cells' real centres (the mean of their two vertices):
+// restriction -- the coarse centre between its children a, a+1 takes
+// h_a/(h_a+h_{a+1}) and h_{a+1}/(h_a+h_{a+1}); prolongation -- the fine
+// centre between coarse centres x0 < x1 takes (x1-x)/(x1-x0) and
+// (x-x0)/(x1-x0); a ratio-1 direction takes (1, 0).  On a uniform level these
+// are the exact 1/2 and 1/4, 3/4 of the integer stencils.
+int stretched_weights(mge_case *h, int lvl) {
+  if (lvl < 2 || lvl > h->nlevel) return MGE_OK;
+  Level &f = h->lv[lvl], &c = h->lv[lvl - 1];
+  if (!f.defined || !c.defined || (!f.vol && !c.vol)) return MGE_OK;
+  if (!f.vol || !c.vol) return set_err(MGE_ERR_ARG, "a stretched level and a uniform level do not nest");
+  const int nf[3] = {f.d.ni, f.d.nj, f.d.nk}, nc[3] = {c.d.ni, c.d.nj, c.d.nk};
+  for (int a = 0; a < 3; a++) {
+    const std::vector<double> &vf = f.hv[a], &vc = c.hv[a];
+    std::vector<double> r((size_t)2 * (nc[a] + 4), 0.0), p((size_t)2 * (nf[a] + 4), 0.0);
+    for (int I = 2; I <= nc[a] + 1; I++) {
+      if (f.ratio[a] == 2) {
+        const int s = 2 * I - 2;
+        const double ha = vf[s + 1] - vf[s], hb = vf[s + 2] - vf[s + 1];
+        r[2 * I] = ha / (ha + hb);
+        r[2 * I + 1] = hb / (ha + hb);
+      } else {
+        r[2 * I] = 1.0;
+      }
+    }
+    for (int i = 2; i <= nf[a] + 1; i++) {
+      if (f.ratio[a] == 2) {
+        const int m = (i % 2 == 0) ? i / 2 : (i - 1) / 2 + 1;
+        const double x = 0.5 * (vf[i] + vf[i + 1]);
+        const double x0 = 0.5 * (vc[m] + vc[m + 1]), x1 = 0.5 * (vc[m + 1] + vc[m + 2]);
+        p[2 * i] = (x1 - x) / (x1 - x0);
+        p[2 * i + 1] = (x - x0) / (x1 - x0);
+      } else {
+        p[2 * i] = 1.0;
+      }
+    }
+    if (!f.rw[a]) CHKRC(dalloc_bytes(f, &f.rw[a], h->es * r.size()));
+    if (!f.pw[a]) CHKRC(dalloc_bytes(f, &f.pw[a], h->es * p.size()));
+    CHKRC(put_bulk(h, f.rw[a], r.data(), r.size()));
+    CHKRC(put_bulk(h, f.pw[a], p.data(), p.size()));
+  }
   return MGE_OK;
 }
 
@@ -1598,6 +1659,22 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(put_bulk(h, L.dx, hx.data(), nmax));
   CHKRC(put_bulk(h, L.dy, hy.data(), nmax));
   CHKRC(put_bulk(h, L.dz, hz.data(), nmax));
+  // a stretched tensor level (cellvolume <= 0; the reference block has one
+  // cellvolume): each interior cell's volume is the product of its vertex
+  // differences, ghost cells 1
+  if (!(cellvolume > 0.0)) {
+    L.cellvolume = 0.0;
+    std::vector<double> v((size_t)E, 1.0);
+    for (int k = 2; k <= d.kl; k++)
+      for (int j = 2; j <= d.jl; j++)
+        for (int i = 2; i <= d.il; i++)
+          v[(size_t)ix(d, i, j, k)] = ((xv[i + 1] - xv[i]) * (yv[j + 1] - yv[j])) * (zv[k + 1] - zv[k]);
+    CHKRC(dalloc_bytes(L, &L.vol, es * E));
+    CHKRC(put_bulk(h, L.vol, v.data(), (size_t)E));
+    L.hv[0].assign(xv, xv + d.ib + 1);
+    L.hv[1].assign(yv, yv + d.jb + 1);
+    L.hv[2].assign(zv, zv + d.kb + 1);
+  }
   // smoothing pivots (psmoo.f90:134-166), computed once on the host in FP64
   std::vector<double> piv((size_t)9 * nmax, 0.0);
   if (h->prm.smoo > 0.0) {
@@ -1655,6 +1732,8 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   L.cur = 0;
   L.defined = true;
   int rc = build_records(h, L, nmir, recs);
+  if (rc >= 0) rc = stretched_weights(h, lvl);
+  if (rc >= 0) rc = stretched_weights(h, lvl + 1);
   if (rc < 0) {
     level_free(L);
     return rc;

@@ -56,6 +56,7 @@ struct StageArgs {
   const T *vdtim;
   const double *dtim;       // device scalar (global time step)
   T cellvolume;
+  const T *vol;             // stretched level: per-cell volumes (else nullptr)
   T blend;                  // 1 - cdis(m)
   T fn;                     // cstp(m)*cfl
   int m;                    // stage number (1-based)
@@ -116,13 +117,14 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
   if (A.tsrc) {
     // physical time derivative of the stage state (euler.f90:249-275)
     const T *w = A.fx.s.w;
+    const T tc = A.vol ? T(0.5) * A.vol[q] / A.dtstep : A.tcoef;
     if (A.tsrc == 2) {
 #pragma unroll
       for (int n = 0; n < 5; n++)
-        r[n] = r[n] + A.tcoef * (T(3) * w[n * E + q] - T(4) * A.w1[n * E + q] + A.w2[n * E + q]);
+        r[n] = r[n] + tc * (T(3) * w[n * E + q] - T(4) * A.w1[n * E + q] + A.w2[n * E + q]);
     } else {
 #pragma unroll
-      for (int n = 0; n < 5; n++) r[n] = r[n] + A.tcoef * (w[n * E + q] - A.w1[n * E + q]);
+      for (int n = 0; n < 5; n++) r[n] = r[n] + tc * (w[n * E + q] - A.w1[n * E + q]);
     }
   }
   if (A.coarse) {
@@ -158,7 +160,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
   }
   // de = fn*vdtim/V, or fn*dtim/V with the global step: one division
   const T dt = A.vt == 1 ? T(A.dtim[0]) : (PRE ? vdt : A.vdtim[q]);
-  const T de = A.fn * dt / A.cellvolume;
+  const T de = A.fn * dt / (A.vol ? A.vol[q] : A.cellvolume);
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = de * r[n];
 #pragma unroll

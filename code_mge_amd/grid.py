@@ -6,6 +6,12 @@ init_metric.f90:59-72: x(i) = h*(i-2) + origin for i = 1..ib, with
 x(0) left at 0.  Coarsening halves a direction while it keeps >= 2 cells
 (psmoo_old needs two cells per line, psmoo.f90:132,166), otherwise the
 direction is kept (semi-coarsening).
+
+A stretched tensor level (the reference block is uniform, with one
+cellvolume) gives its interior vertex coordinates per direction instead;
+its ghost vertices repeat the nearest interior spacing, each cell's volume
+is the product of its vertex differences (cellvolume() returns 0 to say
+so), and a coarser level takes every other vertex.
 """
 import ctypes as C
 from dataclasses import dataclass
@@ -23,15 +29,27 @@ class LevelSpec:
     hi: tuple              # domain upper corner
     ratio: tuple           # coarsening ratio to the next coarser level
     bctype: tuple          # faces 1..6: i-lo, j-lo, k-lo, i-hi, j-hi, k-hi
+    xs: Optional[tuple] = None   # stretched level: interior vertices x(2..n+2) per direction
+
+    @property
+    def stretched(self):
+        return self.xs is not None
 
     def vertices(self):
         out = []
         for a in range(3):
             n = self.n[a]
-            h = (self.hi[a] - self.lo[a]) / n
             v = np.zeros(n + 4, dtype=np.float64)
-            idx = np.arange(1, n + 4, dtype=np.float64)
-            v[1:] = h * (idx - 2.0) + self.lo[a]
+            if self.xs is not None:
+                xa = np.asarray(self.xs[a], dtype=np.float64)
+                assert xa.shape == (n + 1,)
+                v[2:n + 3] = xa
+                v[1] = xa[0] - (xa[1] - xa[0])
+                v[n + 3] = xa[n] + (xa[n] - xa[n - 1])
+            else:
+                h = (self.hi[a] - self.lo[a]) / n
+                idx = np.arange(1, n + 4, dtype=np.float64)
+                v[1:] = h * (idx - 2.0) + self.lo[a]
             out.append(v)
         return out
 
@@ -39,6 +57,10 @@ class LevelSpec:
         return tuple((self.hi[a] - self.lo[a]) / self.n[a] for a in range(3))
 
     def cellvolume(self):
+        """the block's one cell volume; 0 for a stretched level, whose cells
+        each take the product of their vertex differences"""
+        if self.xs is not None:
+            return 0.0
         hx, hy, hz = self.spacing()
         return hx * hy * hz
 
@@ -68,6 +90,44 @@ def hierarchy(n_fine, lo, hi, nlevel, bctype, min_cells=2) -> List[LevelSpec]:
         levels.append(LevelSpec(n=nn, lo=tuple(lo), hi=tuple(hi), ratio=ratio, bctype=tuple(bctype)))
     levels.reverse()
     return levels
+
+
+def stretched_hierarchy(xs_fine, nlevel, bctype, min_cells=2) -> List[LevelSpec]:
+    """Levels coarsest first for a stretched tensor grid given by its finest
+    interior vertices per direction; a direction is halved (every other
+    vertex) while it keeps >= min_cells cells, like hierarchy()."""
+    xs = [np.asarray(x, dtype=np.float64) for x in xs_fine]
+    specs = []
+    for _ in range(nlevel):
+        specs.append(tuple(xs))
+        nxt = []
+        for a in range(3):
+            n = len(xs[a]) - 1
+            nxt.append(xs[a][::2] if (n % 2 == 0 and n // 2 >= min_cells) else xs[a])
+        xs = nxt
+    levels = []
+    for li in range(nlevel):
+        cur = specs[li]
+        n = tuple(len(x) - 1 for x in cur)
+        if li + 1 < nlevel:
+            ratio = tuple(n[a] // (len(specs[li + 1][a]) - 1) for a in range(3))
+        else:
+            ratio = (2, 2, 2)
+        levels.append(LevelSpec(n=n, lo=tuple(float(x[0]) for x in cur), hi=tuple(float(x[-1]) for x in cur),
+                                ratio=ratio, bctype=tuple(bctype), xs=cur))
+    levels.reverse()
+    return levels
+
+
+def geometric_vertices(n, half, ratio):
+    """2*(n//2) ... n cells on [-half, half] whose widths grow by `ratio`
+    from the centre outward (symmetric about 0); n even"""
+    m = n // 2
+    w = ratio ** np.arange(m, dtype=np.float64)
+    w *= half / w.sum()
+    right = np.concatenate([[0.0], np.cumsum(w)])
+    right[-1] = half
+    return np.concatenate([-right[:0:-1], right])
 
 
 def body(kind, center=(0.0, 0.0, 0.0), semi=(0.5, 0.5, 0.5), axis=2, rot=None):

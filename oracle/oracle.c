@@ -19,7 +19,9 @@ typedef struct {
   long E;
   double *x;              /* vertices (0:ib,0:jb,0:kb,3) */
   double *si, *sj, *sk;   /* face vectors (…,3) (init_metric.f90:109-177) */
-  double cellvolume;
+  double cellvolume;      /* the block's cell volume (> 0), or 0: per-cell volumes in vol */
+  double *vol;            /* stretched tensor levels: (dx_i*dy_j)*dz_k per cell, else NULL */
+  double *v1d[3];         /* vertex coordinates x(0..ib), y(0..jb), z(0..kb) */
   int bctype[6];
   double facenvec[6][3];
   double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
@@ -44,6 +46,9 @@ struct orc_case {
 
 #define IX(b, i, j, k) ((long)(i) + (long)((b)->ib + 1) * ((long)(j) + (long)((b)->jb + 1) * (long)(k)))
 #define W(b, n, c) ((b)->w[(long)(n) * (b)->E + (c)])
+/* the volume of cell q: the block's cellvolume, or the cell's own on a
+ * stretched level */
+#define VOLQ(b, q) ((b)->vol ? (b)->vol[q] : (b)->cellvolume)
 
 static double dimf(double a, double b) { return a > b ? a - b : 0.0; }
 static double max2(double a, double b) { return a > b ? a : b; }
@@ -88,6 +93,7 @@ static void blk_free(blk_t *b) {
   free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
   free(b->w1); free(b->w2);
   free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
+  free(b->vol); free(b->v1d[0]); free(b->v1d[1]); free(b->v1d[2]);
   free(b->epsx); free(b->epsy); free(b->epsz);
   free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
 }
@@ -149,6 +155,26 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->nmir = nmir;
   b->z = (orc_cut *)calloc((size_t)(nmir > 0 ? nmir : 1), sizeof(orc_cut));
   if (nmir > 0) memcpy(b->z, cuts, sizeof(orc_cut) * (size_t)nmir);
+  {
+    const double *vv[3] = {xv, yv, zv};
+    const int nv[3] = {b->ib + 1, b->jb + 1, b->kb + 1};
+    for (int d = 0; d < 3; d++) {
+      b->v1d[d] = (double *)calloc((size_t)nv[d], sizeof(double));
+      memcpy(b->v1d[d], vv[d], sizeof(double) * (size_t)nv[d]);
+    }
+  }
+  /* a stretched tensor level (cellvolume <= 0; the reference block has one
+   * cellvolume): each interior cell's volume is the product of its vertex
+   * differences, ghost cells keep 1 */
+  if (!(cellvolume > 0.0)) {
+    b->cellvolume = 0.0;
+    b->vol = (double *)malloc(sizeof(double) * (size_t)E);
+    for (long q = 0; q < E; q++) b->vol[q] = 1.0;
+    for (int k = 2; k <= b->kl; k++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int i = 2; i <= b->il; i++)
+          b->vol[IX(b, i, j, k)] = ((xv[i + 1] - xv[i]) * (yv[j + 1] - yv[j])) * (zv[k + 1] - zv[k]);
+  }
 
   /* grid points (init_metric.f90:47-76): x(i,j,k) for i,j,k >= 1; index 0 stays 0 */
   for (int k = 1; k <= b->kb; k++)
@@ -257,7 +283,7 @@ int orc_timestep(orc_case *c, int lvl) {
           double f1 = 4.0 / W(b, 0, q) / 3.0;
           double f2 = gamma / W(b, 0, q);
           double fac = max2(f1, f2);
-          dtv = fac * fmue / b->cellvolume;
+          dtv = fac * fmue / VOLQ(b, q);
         }
         long qi = IX(b, i + 1, j, k), qj = IX(b, i, j + 1, k), qk = IX(b, i, j, k + 1);
         double sx, sy, sz, qs, ds, cs;
@@ -287,10 +313,10 @@ int orc_timestep(orc_case *c, int lvl) {
         radvk = dtv * ds;
         double fict;
         if (P->eqntype == 1)
-          fict = b->cellvolume / (b->radi[q] + b->radj[q] + b->radk[q] +
+          fict = VOLQ(b, q) / (b->radi[q] + b->radj[q] + b->radk[q] +
                                   cfac * (radvi + radvj + radvk));
         else
-          fict = b->cellvolume / (b->radi[q] + b->radj[q] + b->radk[q]);
+          fict = VOLQ(b, q) / (b->radi[q] + b->radj[q] + b->radk[q]);
         /* unsteady: the pseudo step is capped by the physical one
          * (timestep.f90:86-99, :180-192) */
         double dtime = 2.0 * P->dtstep / (3.0 * cfl);
@@ -1709,10 +1735,10 @@ int orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl, int 
             for (int n = 0; n < 5; n++) {
               long q = (long)n * E + IX(b, i, j, k);
               if (itime > 1)
-                b->dw[q] = b->dw[q] + 0.5 * b->cellvolume / P->dtstep *
+                b->dw[q] = b->dw[q] + 0.5 * VOLQ(b, IX(b, i, j, k)) / P->dtstep *
                                           (3.0 * b->w[q] - 4.0 * b->w1[q] + b->w2[q]);
               else if (itime == 1)
-                b->dw[q] = b->dw[q] + 0.5 * b->cellvolume / P->dtstep * (b->w[q] - b->w1[q]);
+                b->dw[q] = b->dw[q] + 0.5 * VOLQ(b, IX(b, i, j, k)) / P->dtstep * (b->w[q] - b->w1[q]);
             }
     }
     if (mgl < mlvl) { /* euler.f90:294-318 */
@@ -1758,8 +1784,8 @@ int orc_euler(orc_case *c, int lvl, int itime, int ncyc, int mgl, int mlvl, int 
       for (int j = 2; j <= jl; j++)
         for (int i = 2; i <= il; i++) {
           long q0 = IX(b, i, j, k);
-          double de = fn * b->vdtim[q0] / b->cellvolume;
-          if (P->vt == 1) de = fn * b->dtim / b->cellvolume;
+          double de = fn * b->vdtim[q0] / VOLQ(b, q0);
+          if (P->vt == 1) de = fn * b->dtim / VOLQ(b, q0);
           for (int n = 0; n < 5; n++) {
             long q = (long)n * E + q0;
             b->dw[q] = de * b->dw[q];
@@ -1824,6 +1850,32 @@ static void prolong_stencil(int i, int ratio, int *s, double *f0, double *f1) {
   } else { *s = i; *f0 = 1.0; *f1 = 0.0; }
 }
 
+/* the same stencils between stretched tensor levels, with the cell centres'
+ * real positions (x = the mean of a cell's two vertices).  Restriction: the
+ * coarse centre between its two children a, a+1 takes h_a/(h_a+h_{a+1}) and
+ * h_{a+1}/(h_a+h_{a+1}); prolongation: the fine centre between coarse
+ * centres x0 < x1 takes (x1-x)/(x1-x0) and (x-x0)/(x1-x0).  On a uniform
+ * level they are the 1/2 and 1/4, 3/4 above. */
+static void restrict_stencil_s(const double *vf, int I, int ratio, int *s, double *f0, double *f1) {
+  if (ratio == 2) {
+    *s = 2 * I - 2;
+    double ha = vf[*s + 1] - vf[*s], hb = vf[*s + 2] - vf[*s + 1];
+    *f0 = ha / (ha + hb);
+    *f1 = hb / (ha + hb);
+  } else { *s = I; *f0 = 1.0; *f1 = 0.0; }
+}
+static void prolong_stencil_s(const double *vf, const double *vc, int i, int ratio, int *s, double *f0,
+                              double *f1) {
+  if (ratio == 2) {
+    int m = (i % 2 == 0) ? i / 2 : (i - 1) / 2 + 1;
+    *s = m;
+    double x = 0.5 * (vf[i] + vf[i + 1]);
+    double x0 = 0.5 * (vc[m] + vc[m + 1]), x1 = 0.5 * (vc[m + 1] + vc[m + 2]);
+    *f0 = (x1 - x) / (x1 - x0);
+    *f1 = (x - x0) / (x1 - x0);
+  } else { *s = i; *f0 = 1.0; *f1 = 0.0; }
+}
+
 /* trilinear_factor corner order: column-major over (s:s+1)^3 */
 static void tri_fac(const double fa[2], const double fb[2], const double fc[2], double fac[8]) {
   for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
@@ -1839,19 +1891,25 @@ static void collc(orc_case *c, int mgl) {
       for (int k = 2; k <= b->kl; k++) {
         int s[3];
         double fa[2], fb[2], fc[2], fac[8];
-        restrict_stencil(i, f->ratio[0], &s[0], &fa[0], &fa[1]);
-        restrict_stencil(j, f->ratio[1], &s[1], &fb[0], &fb[1]);
-        restrict_stencil(k, f->ratio[2], &s[2], &fc[0], &fc[1]);
+        if (f->vol) {
+          restrict_stencil_s(f->v1d[0], i, f->ratio[0], &s[0], &fa[0], &fa[1]);
+          restrict_stencil_s(f->v1d[1], j, f->ratio[1], &s[1], &fb[0], &fb[1]);
+          restrict_stencil_s(f->v1d[2], k, f->ratio[2], &s[2], &fc[0], &fc[1]);
+        } else {
+          restrict_stencil(i, f->ratio[0], &s[0], &fa[0], &fa[1]);
+          restrict_stencil(j, f->ratio[1], &s[1], &fb[0], &fb[1]);
+          restrict_stencil(k, f->ratio[2], &s[2], &fc[0], &fc[1]);
+        }
         tri_fac(fa, fb, fc, fac);
         long qb = IX(b, i, j, k);
         for (int n = 0; n < 5; n++) {
           double acc = 0.0;
           for (int c8 = 0; c8 < 8; c8++) {
             long qf = IX(f, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
-            double val = f->dw[(long)n * Ef + qf] / f->cellvolume;
+            double val = f->dw[(long)n * Ef + qf] / VOLQ(f, qf);
             acc = acc + val * fac[c8];
           }
-          b->wr[(long)n * Eb + qb] = acc * b->cellvolume;
+          b->wr[(long)n * Eb + qb] = acc * VOLQ(b, qb);
         }
         for (int n = 0; n < 5; n++) {
           double acc = 0.0;
@@ -1891,9 +1949,15 @@ static void addw(orc_case *c, int mgl, int mlvl) {
       for (int k = 2; k <= b->kl; k++) {
         int s[3];
         double fa[2], fb[2], fc[2], fac[8];
-        prolong_stencil(i, b->ratio[0], &s[0], &fa[0], &fa[1]);
-        prolong_stencil(j, b->ratio[1], &s[1], &fb[0], &fb[1]);
-        prolong_stencil(k, b->ratio[2], &s[2], &fc[0], &fc[1]);
+        if (b->vol) {
+          prolong_stencil_s(b->v1d[0], pc->v1d[0], i, b->ratio[0], &s[0], &fa[0], &fa[1]);
+          prolong_stencil_s(b->v1d[1], pc->v1d[1], j, b->ratio[1], &s[1], &fb[0], &fb[1]);
+          prolong_stencil_s(b->v1d[2], pc->v1d[2], k, b->ratio[2], &s[2], &fc[0], &fc[1]);
+        } else {
+          prolong_stencil(i, b->ratio[0], &s[0], &fa[0], &fa[1]);
+          prolong_stencil(j, b->ratio[1], &s[1], &fb[0], &fb[1]);
+          prolong_stencil(k, b->ratio[2], &s[2], &fc[0], &fc[1]);
+        }
         tri_fac(fa, fb, fc, fac);
         long qb = IX(b, i, j, k);
         double dw[5];
@@ -2135,6 +2199,7 @@ static double *field_ptr(blk_t *b, const char *name, long *n) {
   if (!strcmp(name, "epsx")) return b->epsx;
   if (!strcmp(name, "epsy")) return b->epsy;
   if (!strcmp(name, "epsz")) return b->epsz;
+  if (!strcmp(name, "vol")) return b->vol;
   return NULL;
 }
 

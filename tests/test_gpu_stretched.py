@@ -1,0 +1,97 @@
+"""Stretched tensor levels on the HIP path (cellvolume 0: per-cell volumes
+from the vertex differences, transfers with the real centre positions)
+against the oracle, at the laminar-cylinder configuration's full size:
+512x512x4 cells on [-20,20]^2 clustered toward the body at ratio 1.02, four
+levels, M = 0.1, Re = 40, Pr = 0.72, JST + laminar fluxes."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from code_mge_amd import configs
+from code_mge_amd.grid import level_geometry, stretched_hierarchy
+from code_mge_amd.solver import Case, force_coefficients
+from oracle.oracle import OracleCase
+from tests.gpu_common import cell_rel_err, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CYCLES = 1e-10
+
+
+def test_laminar_cylinder_512_stretched_parity(require_gpu):
+    """three W-cycles of the full-size case: per-cell state on every level
+    within 1e-10, residual history within 1e-9, wall force (pressure and
+    viscous) within 1e-9 of q*A"""
+    prm, levels, bdy, aref = configs.cylinder2d_stretched()
+    assert levels[-1].n == (512, 512, 4) and len(levels) == 4
+    geoms = [level_geometry(s, bdy) for s in levels]
+    print(f"cfg3: records per level {[g.nmir for g in geoms]}", flush=True)
+    gpu = Case(prm, levels, geoms=geoms, mxcyc=10)
+    orc = OracleCase(prm, geoms, mxcyc=10)
+    for c in (gpu, orc):
+        c.init_flow()
+    lg, csg = gpu.solve([0, 0, 0, 3])
+    print("gpu done; oracle (about 30 s)", flush=True)
+    lo, cso = orc.solve([0, 0, 0, 3])
+    assert csg == cso and lg.shape == lo.shape
+    assert rel_err(lg[:, 2], lo[:, 2]) < 1e-9
+    for lvl in range(1, 5):
+        e = cell_rel_err(gpu.download(lvl, "w"), orc.download(lvl, "w"), prm)
+        print(f"level {lvl}: per-cell relative state error {e:.3e}", flush=True)
+        assert e < CYCLES
+    q = prm.dynamic_pressure()
+    fg, fo = gpu.wall_force(4), orc.wall_force(4)
+    print("wall force gpu", fg, "oracle", fo, flush=True)
+    assert np.max(np.abs(fg - fo)) / (q * aref) < 1e-9
+    gpu.close()
+    orc.close()
+
+
+def test_stretched_fp32_tracks_fp64(require_gpu):
+    """the FP32 bulk path on a stretched laminar case: Cd/Cl within 1e-5 of
+    FP64 after 300 cycles (64x64x4, ratio 1.05, three levels)"""
+    prm, levels, bdy, aref = configs.cylinder2d_stretched(n=64, nz=4, nlevel=3, ratio=1.05, half=10.0, mach=0.2)
+    geoms = [level_geometry(s, bdy) for s in levels]
+    out = {}
+    for prec in (64, 32):
+        with Case(dataclasses.replace(prm, precision=prec), levels, geoms=geoms, mxcyc=8) as c:
+            c.init_flow()
+            r0, _ = c.cycles(3, 1)
+            r1, _ = c.cycles(3, 299, ncyc0=2)
+            out[prec] = (force_coefficients(c, aref), r0, r1)
+    (cd64, cl64), a64, b64 = out[64]
+    (cd32, cl32), a32, b32 = out[32]
+    print(f"stretched laminar cylinder: Cd {cd64:.8f}/{cd32:.8f} Cl {cl64:.2e}/{cl32:.2e}; "
+          f"residual fp64 {a64:.2e}->{b64:.2e} fp32 {a32:.2e}->{b32:.2e}", flush=True)
+    assert b64 < 1e-3 * a64
+    assert abs(cd32 - cd64) < 1e-5 and abs(cl32 - cl64) < 1e-5
+
+
+def test_stretched_3d_freestream_and_roe_parity(require_gpu):
+    """a 3-D stretched box (ratio 1.1 in x, 1.05 in y, power law in z) with
+    a sphere: Roe/MUSCL W-cycles match the oracle; without the body the
+    freestream stays to rounding"""
+    from code_mge_amd.grid import body, geometric_vertices
+    xs = (geometric_vertices(48, 4.0, 1.05), geometric_vertices(40, 4.0, 1.08),
+          -4.0 + 8.0 * (np.arange(33, dtype=np.float64) / 32.0) ** 1.1)
+    levels = stretched_hierarchy(xs, 3, (-20,) * 6)
+    prm = configs.Params(mach=0.5, alpha_deg=4.0, beta_deg=2.0, distype="r2")
+    sph = body("sphere", semi=(0.7, 0.7, 0.7), center=(0.1, 0.0, -0.2))
+    geoms = [level_geometry(s, sph) for s in levels]
+    gpu = Case(prm, levels, geoms=geoms, mxcyc=12)
+    orc = OracleCase(prm, geoms, mxcyc=12)
+    for c in (gpu, orc):
+        c.init_flow()
+    lg, _ = gpu.solve([0, 2, 6])
+    lo, _ = orc.solve([0, 2, 6])
+    assert rel_err(lg[:, 2], lo[:, 2]) < 1e-9
+    assert cell_rel_err(gpu.download(3, "w"), orc.download(3, "w"), prm) < CYCLES
+    gpu.close()
+    orc.close()
+    free = [level_geometry(s, None) for s in levels]
+    with Case(prm, levels, geoms=free, mxcyc=8) as c:
+        c.init_flow()
+        w0 = c.download(3, "w")
+        log, _ = c.solve([0, 0, 3])
+        assert np.max(np.abs(c.download(3, "w") - w0)) < 1e-12 * np.abs(w0).max()
