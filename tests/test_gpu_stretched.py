@@ -49,22 +49,27 @@ def test_laminar_cylinder_512_stretched_parity(require_gpu):
 
 
 def test_stretched_fp32_tracks_fp64(require_gpu):
-    """the FP32 bulk path on a stretched laminar case: Cd/Cl within 1e-5 of
-    FP64 after 300 cycles (64x64x4, ratio 1.05, three levels)"""
-    prm, levels, bdy, aref = configs.cylinder2d_stretched(n=64, nz=4, nlevel=3, ratio=1.05, half=10.0, mach=0.2)
+    """the FP32 bulk path on a stretched level hierarchy, both precisions run
+    to convergence: the inviscid cylinder on the 64x64x4 grid clustered at
+    ratio 1.05 (three levels), 1500 cycles; Cd/Cl of the FP32 floor state
+    within 1e-5 of the converged FP64 values.  (The laminar case at this
+    resolution does not settle: its wake's asymmetric mode grows slowly in
+    both precisions alike, so it pins no precision claim.)"""
+    prm, levels, bdy, aref = configs.cylinder2d_stretched(n=64, nz=4, nlevel=3, ratio=1.05, half=10.0, mach=0.2,
+                                                          eqntype="e")
     geoms = [level_geometry(s, bdy) for s in levels]
     out = {}
     for prec in (64, 32):
         with Case(dataclasses.replace(prm, precision=prec), levels, geoms=geoms, mxcyc=8) as c:
             c.init_flow()
             r0, _ = c.cycles(3, 1)
-            r1, _ = c.cycles(3, 299, ncyc0=2)
+            r1, _ = c.cycles(3, 1499, ncyc0=2)
             out[prec] = (force_coefficients(c, aref), r0, r1)
+            print(f"fp{prec}: residual {r0:.2e} -> {r1:.2e}, Cd {out[prec][0][0]:.9f} Cl {out[prec][0][1]:.2e}",
+                  flush=True)
     (cd64, cl64), a64, b64 = out[64]
     (cd32, cl32), a32, b32 = out[32]
-    print(f"stretched laminar cylinder: Cd {cd64:.8f}/{cd32:.8f} Cl {cl64:.2e}/{cl32:.2e}; "
-          f"residual fp64 {a64:.2e}->{b64:.2e} fp32 {a32:.2e}->{b32:.2e}", flush=True)
-    assert b64 < 1e-3 * a64
+    assert b64 < 1e-9 * a64
     assert abs(cd32 - cd64) < 1e-5 and abs(cl32 - cl64) < 1e-5
 
 
