@@ -80,6 +80,31 @@ def load_traffic(tag):
         return None
 
 
+# MI355X: 256 CUs x 4 SIMDs; a 64-lane wave instruction issues in 4 clocks
+# on a 16-lane SIMD (FP64 vector at full rate); peak engine clock 2.4 GHz
+SIMDS = 1024
+CLOCK_GHZ = 2.4
+
+
+def limiter(kname, precision, avg_launch_ms, hbm_frac):
+    """what bounds the kernel, from the committed counter summary
+    (rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES on the same launch): the time
+    its VALU instructions alone need at full issue rate, as a fraction of
+    the measured launch time, beside the HBM fraction"""
+    p = os.path.join(ROOT, "profiles", "k_stage_counters.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)[f"{kname}_fp{precision}"]
+    except (OSError, ValueError, KeyError):
+        return None
+    valu_ms = d["valu_instr_per_wave"] * d["waves_per_launch"] * 4 / (SIMDS * CLOCK_GHZ * 1e9) * 1e3
+    valu_frac = valu_ms / avg_launch_ms
+    kind = "valu" if valu_frac > max(hbm_frac, 0.5) else ("hbm" if hbm_frac > 0.5 else "latency")
+    return {"measured": kind, "valu_floor_ms": round(valu_ms, 4), "valu_frac": round(valu_frac, 4),
+            "hbm_frac": round(hbm_frac, 4), "waves_per_simd": d.get("waves_per_simd"),
+            "source": "profiles/k_stage_counters.json"}
+
+
 def run_leg(prm, levels, geoms, aref, args, precision):
     """warmup, the timed cycles (no instrumentation), then a separate pass of
     a few cycles with HIP events around every launch for the roofline of
@@ -112,9 +137,12 @@ def run_leg(prm, levels, geoms, aref, args, precision):
         if cnt > 0 and ms > 0:
             achieved = by / (ms * 1e-3) / 1e9
             kname = {"r1": "k_stage_roe", "r2": "k_stage_roe"}.get(prm.distype.lower(), "k_stage2")
+            # the roofline of this path is HBM bandwidth (no contraction: no MFMA);
+            # `limiter` names what the counters say holds the kernel below it
             roofline = {"kernel": f"{kname}<{'double' if precision == 64 else 'float'}> on the finest level "
                                   f"({levels[-1].n[0]}x{levels[-1].n[1]}x{levels[-1].n[2]})",
                         "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "limiter": limiter(kname, precision, ms, achieved / PEAK_HBM_GBS),
                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(f"fp{precision}"),
                         "avg_launch_ms": round(ms, 5), "launches": cnt, "alg_bytes_per_launch": round(by),
                         "timed_cycles": nroof,
@@ -281,20 +309,21 @@ def converge_leg(prm, levels, geoms, aref, maxcyc, drop64=1e8, drop32=1e5):
 
 def ns_leg(cycles):
     """Laminar Navier-Stokes cylinder (Re = 40 on the diameter, M = 0.1,
-    Pr = 0.72, constant viscosity) on a uniform 512 x 512 x 4 grid over
-    [-20, 20]^2, 4-level W-cycle, JST + viscous fluxes, no smoothing:
-    `cycles` W-cycles of the plain FP64 path and of the FP32 bulk path from
-    freestream; throughput, residual drop and Cd/Cl of each, and the FP32
-    error against FP64."""
+    Pr = 0.72, constant viscosity) on 512 x 512 x 4 cells over [-20, 20]^2
+    clustered geometrically toward the body (ratio 1.02; z uniform,
+    periodic), 4-level W-cycle, JST + viscous fluxes: `cycles` W-cycles of
+    the plain FP64 path and of the FP32 bulk path from freestream;
+    throughput, residual drop and Cd/Cl of each, and the FP32 error against
+    FP64 at the same iterate."""
     import dataclasses
     from code_mge_amd import configs
     from code_mge_amd.grid import level_geometry
     from code_mge_amd.solver import Case, force_coefficients
-    prm, levels, bdy, aref = configs.cylinder2d(n=512, nz=4, nlevel=4, mach=0.1, half=20.0, eqntype="n",
-                                                reynolds=40.0, prlmean=0.72, smoo=0.0, cfl=-2.0)
+    prm, levels, bdy, aref = configs.cylinder2d_stretched(n=512, nz=4, nlevel=4, mach=0.1, half=20.0, ratio=1.02,
+                                                          eqntype="n", reynolds=40.0, prlmean=0.72)
     geoms = [level_geometry(s, bdy) for s in levels]
-    out = {"workload": "NS cylinder Re=40 M=0.1 Pr=0.72, 512x512x4 on [-20,20]^2 (uniform), 4-level W-cycle, "
-                       "JST + laminar viscous fluxes", "cycles": cycles}
+    out = {"workload": "NS cylinder Re=40 M=0.1 Pr=0.72, 512x512x4 on [-20,20]^2 clustered toward the body at "
+                       "ratio 1.02, 4-level W-cycle, JST + laminar viscous fluxes, smoothing 0.6", "cycles": cycles}
     for prec in (64, 32):
         with Case(dataclasses.replace(prm, precision=prec), levels, geoms=geoms, mxcyc=cycles + 2) as c:
             c.init_flow()
@@ -351,10 +380,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--sweep", type=int, default=16, help="cases of the solves/h leg (0: skip)")
     ap.add_argument("--no-jst", action="store_true", help="skip the JST sphere leg")
-    ap.add_argument("--ns-leg", type=int, default=0, help="W-cycles of the NS cylinder leg (0: skip)")
+    ap.add_argument("--ns-leg", type=int, default=300, help="W-cycles of the NS cylinder leg (0: skip)")
     ap.add_argument("--converge", type=int, default=0,
                     help="run the workload to its residual drop, at most this many cycles (0: skip)")
-    ap.add_argument("--serve-leg", type=int, default=0, help="requests of the Poisson serving leg (0: skip)")
+    ap.add_argument("--serve-leg", type=int, default=32, help="requests of the Poisson serving leg (0: skip)")
     ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
                     help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
     ap.add_argument("--probe-scheme", default="r2", choices=("r2", "c"),

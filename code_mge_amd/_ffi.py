@@ -45,6 +45,7 @@ class MgeParams(C.Structure):
         ("execmode", C.c_int),
         ("dtstep", C.c_double),
         ("smoother", C.c_int),
+        ("ffreeze", C.c_double),
     ]
 
 

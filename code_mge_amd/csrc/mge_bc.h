@@ -316,6 +316,8 @@ struct BcArgs {
   int kind;      // -20 far, -50 symmetry, -60 periodic, -80 wall
   double gamma, rho0, p0, c0, u0, v0, w0;
   Inf<T> inf;
+  uint8_t *ffdec;     // far field: this face's branch per ghost point (1 outflow), or nullptr
+  const int *frozen;  // the case's freeze flag: the branch is read from ffdec, not chosen
 };
 
 // one face of bcond.f90:16-39 -> bc_far.f90 / bc_symm.f90 / bc_periodic.f90 /
@@ -354,7 +356,13 @@ __device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArg
     s = s + Wf(3, q1) * fn[2];
     double vel_norm = s / Wf(0, q1);
     double r, vel[3], pr;
-    if (vel_norm <= 0.0) {
+    bool inflow = vel_norm <= 0.0;
+    if (B.ffdec) {
+      uint8_t &dec = B.ffdec[a + (long)(amax + 1) * c2];
+      if (*B.frozen) inflow = dec == 0;
+      else dec = inflow ? 0 : 1;
+    }
+    if (inflow) {
       double rhoe = Wf(0, q1);
       double ue = Wf(1 + dir, q1) / rhoe;
       double pe = full(f, 5, P[q1]);

@@ -87,6 +87,8 @@ struct Level {
   // next coarser level (restriction per coarse index, prolongation per index)
   void *vol = nullptr;
   std::vector<double> hv[3];
+  uint8_t *ffdec = nullptr;  // far-field branch per ghost point of the six faces (ffreeze > 0)
+  long ffoff[7] = {0, 0, 0, 0, 0, 0, 0};
   void *rw[3] = {nullptr, nullptr, nullptr}, *pw[3] = {nullptr, nullptr, nullptr};
   uint8_t *flags = nullptr, *marker = nullptr;
   void *dx = nullptr, *dy = nullptr, *dz = nullptr;
@@ -145,6 +147,8 @@ struct TStat {
 struct mge_case {
   mge_params prm{};
   int dev = 0;            // HIP device of the case's stream
+  int *ffrozen = nullptr;     // far-field branches frozen (device flag, ffreeze > 0)
+  double *ffini = nullptr;    // first finest-cycle residual for that test (device)
   double *scratch = nullptr;  // that device's address of the fixed staging array
   bool fp32 = false;
   size_t es = 8;  // bytes per bulk value
@@ -367,6 +371,8 @@ int do_bcond(mge_case *h, Level &L, int buf) {
     B.gamma = P.gamma; B.rho0 = P.rho0; B.p0 = P.p0; B.c0 = P.c0;
     B.u0 = P.u0; B.v0 = P.v0; B.w0 = P.w0;
     B.inf = make_inf<T>(P);
+    B.ffdec = (t == -20 && L.ffdec) ? L.ffdec + L.ffoff[f - 1] : nullptr;
+    B.frozen = h->ffrozen;
     int dir = (f - 1) % 3;
     int amax = dir == 0 ? L.d.jb : L.d.ib;
     int cmax = dir == 2 ? L.d.jb : L.d.kb;
@@ -857,6 +863,27 @@ int do_cycle(mge_case *h, int mlvl, int ncyc, long long *cs) {
   }
 }
 
+// the far-field freeze test of one case after a finest-level cycle, on the
+// device so that it needs no host round trip (k_converge makes the same
+// test for batched cases)
+__global__ void k_ffreeze(const double *rtrms_slot, double *ini, int ncyc, double ffreeze, int *frozen) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double r = *rtrms_slot;
+  if (ncyc == 1) *ini = r;
+  if (*ini / r > ffreeze) *frozen = 1;
+}
+
+// after finest-level cycle ncyc of a single case: the freeze test
+int ffreeze_test(mge_case *h, int mlvl, int ncyc) {
+  if (!(h->prm.ffreeze > 0.0) || mlvl != h->nlevel || h->rec) return MGE_OK;
+  Level &B = h->lv[mlvl];
+  const int c = ncyc >= 1 && ncyc <= B.mxcyc ? ncyc : 0;
+  hipLaunchKernelGGL(k_ffreeze, dim3(1), dim3(64), 0, h->stream, (const double *)(B.rtrms + c), h->ffini, ncyc,
+                     h->prm.ffreeze, h->ffrozen);
+  HIPCHK(hipGetLastError());
+  return MGE_OK;
+}
+
 int read_rtrms(mge_case *h, int mlvl, int ncyc, double *r) {
   Level &B = h->lv[mlvl];
   int c = ncyc >= 1 && ncyc <= B.mxcyc ? ncyc : 0;
@@ -882,6 +909,7 @@ int do_solve(mge_case *h, const int *mcyc, int itime, double *reslog, int cap, i
   for (int mlvl = 1; mlvl <= nlevel; mlvl++) {
     for (int ncyc = 1; ncyc <= mcyc[mlvl - 1]; ncyc++) {
       CHKRC(do_cycle<T>(h, mlvl, ncyc, &cs));
+      CHKRC(ffreeze_test(h, mlvl, ncyc));
       double r = 0.0;
       CHKRC(read_rtrms(h, mlvl, ncyc, &r));
       max_res = 1000.0;
@@ -1239,6 +1267,8 @@ struct BCtl {
   double *log;          // residual history (cap entries)
   double ini, last, drop;
   int ncyc, maxcyc, status, cap;
+  int *frozen;          // the case's far-field freeze flag
+  double ffreeze;       // its freeze drop (0: never)
 };
 
 __global__ void k_converge(BCtl *ctl, int *act, int n, int slot) {
@@ -1249,6 +1279,7 @@ __global__ void k_converge(BCtl *ctl, int *act, int n, int slot) {
     C.ncyc += 1;
     if (C.ncyc == 1) C.ini = r;
     C.last = r;
+    if (C.ffreeze > 0.0 && C.ini / r > C.ffreeze) *C.frozen = 1;
     if (C.ncyc - 1 < C.cap) C.log[C.ncyc - 1] = r;
     int e = *C.err;
     if (r != r || e < 0) {
@@ -1570,12 +1601,17 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
     delete h;
     return set_err(MGE_ERR_HIP, "hipStreamCreate failed");
   }
-  if (hipMalloc(&h->err, sizeof(int)) != hipSuccess) {
+  if (hipMalloc(&h->err, sizeof(int)) != hipSuccess || hipMalloc(&h->ffrozen, sizeof(int)) != hipSuccess ||
+      hipMalloc(&h->ffini, sizeof(double)) != hipSuccess) {
+    if (h->err) hipFree(h->err);
+    if (h->ffrozen) hipFree(h->ffrozen);
     hipStreamDestroy(h->stream);
     delete h;
     return set_err(MGE_ERR_NOMEM, "hipMalloc failed");
   }
   hipMemset(h->err, 0, sizeof(int));
+  hipMemset(h->ffrozen, 0, sizeof(int));
+  hipMemset(h->ffini, 0, sizeof(double));
   *out = h;
   return MGE_OK;
 }
@@ -1590,6 +1626,8 @@ int mge_case_destroy(mge_case *h) {
   }
   for (auto e : h->pool) hipEventDestroy(e);
   hipFree(h->err);
+  hipFree(h->ffrozen);
+  hipFree(h->ffini);
   hipStreamDestroy(h->stream);
   delete h;
   return MGE_OK;
@@ -1645,6 +1683,14 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc(L, &L.dtim, 1));
   CHKRC(dalloc(L, &L.rtrms, L.mxcyc + 1));
   CHKRC(dalloc(L, &L.force, 3));
+  if (h->prm.ffreeze > 0.0) {
+    for (int f = 1; f <= 6; f++) {
+      const int dir = (f - 1) % 3;
+      const long amax = dir == 0 ? d.jb : d.ib, cmax = dir == 2 ? d.jb : d.kb;
+      L.ffoff[f] = L.ffoff[f - 1] + (amax + 1) * (cmax + 1);
+    }
+    CHKRC(dalloc(L, &L.ffdec, L.ffoff[6]));
+  }
   if (h->prm.eqntype == 1) CHKRC(dalloc_bytes(L, &L.vprim, es * 4 * E));
   if (h->prm.execmode == 1) {
     CHKRC(dalloc_bytes(L, &L.w1, es * 5 * E));
@@ -1743,6 +1789,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
 
 int mge_init_flow(mge_case *h) {
   if (!h) return set_err(MGE_ERR_ARG, "null handle");
+  HIPCHK(hipMemsetAsync(h->ffrozen, 0, sizeof(int), h->stream));  // a new solve chooses afresh
   int rc = by_type(h, [&](auto z) { return do_init_flow<decltype(z)>(h); });
   if (rc < 0) return rc;
   int e = check_err(h);
@@ -1832,8 +1879,10 @@ int mge_cycles(mge_case *h, int mlvl, int ncyc0, int count, double *res, long lo
   for (int l = 1; l <= mlvl; l++)
     if (!h->lv[l].defined) return set_err(MGE_ERR_STATE, "level not defined");
   long long cs = 0;
-  for (int c = 0; c < count; c++)
+  for (int c = 0; c < count; c++) {
     CHKRC(by_type(h, [&](auto z) { return do_cycle<decltype(z)>(h, mlvl, ncyc0 + c, &cs); }));
+    CHKRC(ffreeze_test(h, mlvl, ncyc0 + c));
+  }
   if (res && count > 0) CHKRC(read_rtrms(h, mlvl, ncyc0 + count - 1, res));
   else CHKRC(check_err(h));
   if (cell_stages) *cell_stages = cs;
@@ -2259,6 +2308,8 @@ int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc) {
   C.drop = drop;
   C.maxcyc = maxcyc;
   C.cap = b->cap;
+  C.frozen = h->ffrozen;
+  C.ffreeze = h->prm.ffreeze;
   b->hact[slot] = 1;
   CHKRC(batch_push(b));
   b->dirty = true;

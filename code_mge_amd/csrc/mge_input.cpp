@@ -477,6 +477,7 @@ int mge_job_params(const mge_job *job, int precision, mge_params *P) {
   P->dtstep = job->dt * vel_fs / 1.0;
   if (job->eqntype == 1 && job->execmode == 1) P->dtstep = P->u0 / 1.0 / 5.0;
   P->smoother = 0;  // euler.f90:376 calls psmoo_old
+  P->ffreeze = 0.0;  // bc_far.f90's rule at every call
   return MGE_OK;
 }
 

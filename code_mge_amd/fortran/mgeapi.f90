@@ -22,6 +22,7 @@ module mgeapi
     integer(c_int)  :: execmode
     real(c_double)  :: dtstep
     integer(c_int)  :: smoother
+    real(c_double)  :: ffreeze
   end type
 
   ! what a jobsheet defines (mge_job)

@@ -38,6 +38,7 @@ class Params:
     execmode: int = 0                 # 0 steady, 1 unsteady dual time stepping
     dtstep: float = 0.0               # physical time step in units of ref_leng/|U_inf|
     smoother: int = 0                 # 0 psmoo_old (constant coefficient), 1 psmoo + progon3
+    ffreeze: float = 0.0              # > 0: freeze the far-field inflow/outflow choice at this drop
     extra: dict = field(default_factory=dict)
 
     def derived(self):
@@ -97,6 +98,7 @@ class Params:
         P.execmode = d["execmode"]
         P.dtstep = d["dtstep"]
         P.smoother = self.smoother
+        P.ffreeze = self.ffreeze
         return P
 
     def dynamic_pressure(self):

@@ -8,9 +8,11 @@ scheme, stages, cycle type, smoothing and precision).  Every tick advances
 each batch by a few periods (two multigrid cycles each, all cases in one
 launch per operation); a case leaves the batch at the cycle its residual
 has dropped by `drop` (or at maxcyc), its Cd/Cl are read, its memory is
-freed and waiting requests are admitted in its place.  Requests that arrive
-while others run are admitted between ticks and overlap with them on the
-device.  A case computes exactly what it computes alone.
+freed and waiting requests are admitted in its place.  Admission (geometry,
+device state, init_flow, autotune) runs on its own thread while the GPU
+works, and the batches of different classes (Euler and Navier-Stokes, say)
+advance concurrently on their own streams.  A case computes exactly what
+it computes alone.
 
 No counterpart in the reference (it runs one case per process).
 """
@@ -96,6 +98,7 @@ class Server:
         self.autotune = autotune
         self.batches: Dict[tuple, Batch] = {}
         self.used = 0
+        self.peak_used = 0     # most device bytes reserved at once
 
     def _class(self, prm: Params, levels) -> tuple:
         return (prm.eqntype.lower(), prm.distype.lower(), prm.dissip.lower(), len(levels), len(prm.cstp),
@@ -111,71 +114,127 @@ class Server:
                                  psmoo=req.params.smoother == 1 and req.params.smoo > 0.0)
         return total
 
+    def _prepare(self, req: Request):
+        """host geometry, device state, init_flow and autotune of one
+        request: the admission work, done off the tick thread"""
+        geoms = [level_geometry(s, req.body) for s in req.levels]
+        need = self.estimate(req, geoms)
+        with self._cv:
+            # wait for room in the HBM budget (an empty server always admits)
+            while self.used + need > self.budget and self._resident > 0 and not self._stop:
+                self._cv.wait(0.05)
+            if self._stop:
+                return None
+            self.used += need
+            self._resident += 1
+            self.peak_used = max(self.peak_used, self.used)
+        prm = dataclasses.replace(req.params, precision=self.precision)
+        case = Case(prm, req.levels, geoms=geoms, mxcyc=4)
+        case.init_flow()
+        if self.autotune:
+            case.autotune()   # cached per level shape after the first request
+        got = case.device_bytes()
+        with self._cv:
+            self.used += got - need
+        return req, case, got
+
+    def _preparer(self, pending, t0, clock):
+        """admits requests in arrival order: sleeps until each arrives, then
+        prepares it and hands it to the tick thread"""
+        for req in pending:
+            while not self._stop:
+                wait = req.arrival - (clock() - t0)
+                if wait <= 0:
+                    break
+                time.sleep(min(wait, 0.05))
+            if self._stop:
+                return
+            try:
+                item = self._prepare(req)
+            except Exception as e:   # reported by the tick thread
+                item = (req, e, 0)
+            if item is None:
+                return
+            with self._cv:
+                self._ready.append(item)
+                self._cv.notify_all()
+
     def run(self, requests: List[Request], clock=time.monotonic) -> List[Result]:
+        """Serve `requests` (arrival times relative to the start).  A preparer
+        thread admits them in arrival order by the HBM budget and builds
+        their device state while the GPU runs; the tick thread adds ready
+        cases to the batch of their class, advances every class batch by
+        `periods` periods -- the batches of different classes on their own
+        streams from their own threads, so they overlap on the device --
+        and retires finished cases, which frees their budget."""
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
         pending = sorted(requests, key=lambda r: r.arrival)
-        queue: List[Request] = []
+        self._cv = threading.Condition()
+        self._ready = []
+        self._stop = False
+        self._resident = 0
         active = {}            # (class, slot) -> (req, case, admitted, bytes)
         results: List[Result] = []
-        geom_cache = {}
         t0 = clock()
-        while pending or queue or active:
-            now = clock() - t0
-            while pending and pending[0].arrival <= now:
-                queue.append(pending.pop(0))
-            # admission in arrival order while the budget allows
-            while queue:
-                req = queue[0]
-                if req.rid not in geom_cache:
-                    geom_cache[req.rid] = [level_geometry(s, req.body) for s in req.levels]
-                geoms = geom_cache[req.rid]
-                need = self.estimate(req, geoms)
-                if self.used + need > self.budget and active:
-                    break
-                queue.pop(0)
-                geom_cache.pop(req.rid, None)
-                prm = dataclasses.replace(req.params, precision=self.precision)
-                case = Case(prm, req.levels, geoms=geoms, mxcyc=4)
-                case.init_flow()
-                if self.autotune:
-                    case.autotune()   # cached per level shape after the first request
-                got = case.device_bytes()
-                key = self._class(prm, req.levels)
-                b = self.batches.get(key)
-                if b is None:
-                    b = Batch(use_graph=self.use_graph, log_cap=8)
-                    self.batches[key] = b
-                slot = b.add(case, self.drop, self.maxcyc)
-                active[(key, slot)] = (req, case, clock() - t0, got)
-                self.used += got
-                if self.log:
-                    self.log(f"[serve] t={clock() - t0:7.2f}s admit {req.rid} ({got / 2**20:.0f} MiB, "
-                             f"{len(active)} active, {len(queue)} waiting)")
-            if not active:
-                if pending:
-                    time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
-                continue
-            for key, b in list(self.batches.items()):
-                if any(k == key for (k, _) in active):
-                    b.run(self.periods)
-            for (key, slot) in list(active):
-                b = self.batches[key]
-                st = b.status(slot)
-                if st["status"] == 0:
+        prep = threading.Thread(target=self._preparer, args=(pending, t0, clock), daemon=True)
+        prep.start()
+        pool = ThreadPoolExecutor(max_workers=4)
+        try:
+            while len(results) < len(pending):
+                with self._cv:
+                    ready, self._ready = self._ready, []
+                    if not ready and not active:
+                        self._cv.wait(0.05)
+                        ready, self._ready = self._ready, []
+                for req, case, got in ready:
+                    if isinstance(case, Exception):
+                        raise case
+                    key = self._class(case.params, req.levels)
+                    b = self.batches.get(key)
+                    if b is None:
+                        b = Batch(use_graph=self.use_graph, log_cap=8)
+                        self.batches[key] = b
+                    slot = b.add(case, self.drop, self.maxcyc)
+                    active[(key, slot)] = (req, case, clock() - t0, got)
+                    if self.log:
+                        self.log(f"[serve] t={clock() - t0:7.2f}s admit {req.rid} ({got / 2**20:.0f} MiB, "
+                                 f"{len(active)} active)")
+                if not active:
                     continue
-                req, case, adm, got = active.pop((key, slot))
-                cd, cl = force_coefficients(case, req.ref_area) if st["status"] > 0 else (math.nan, math.nan)
-                b.remove(slot)
-                case.close()
-                self.used -= got
-                done = clock() - t0
-                ratio = st["last"] / st["ini"] if st["ini"] else math.nan
-                results.append(Result(req.rid, req.arrival, adm, done, st["status"], st["ncyc"], ratio, cd, cl, got))
-                if self.log:
-                    self.log(f"[serve] t={done:7.2f}s done {req.rid} after {st['ncyc']} cycles "
-                             f"(status {st['status']}, {done - req.arrival:.2f}s)")
-        for b in self.batches.values():
-            b.close()
-        self.batches.clear()
+                keys = {k for (k, _) in active}
+                runs = [pool.submit(self.batches[k].run, self.periods) for k in keys]
+                for f in runs:
+                    f.result()
+                for (key, slot) in list(active):
+                    b = self.batches[key]
+                    st = b.status(slot)
+                    if st["status"] == 0:
+                        continue
+                    req, case, adm, got = active.pop((key, slot))
+                    cd, cl = force_coefficients(case, req.ref_area) if st["status"] > 0 else (math.nan, math.nan)
+                    b.remove(slot)
+                    case.close()
+                    with self._cv:
+                        self.used -= got
+                        self._resident -= 1
+                        self._cv.notify_all()
+                    done = clock() - t0
+                    ratio = st["last"] / st["ini"] if st["ini"] else math.nan
+                    results.append(Result(req.rid, req.arrival, adm, done, st["status"], st["ncyc"], ratio, cd, cl,
+                                          got))
+                    if self.log:
+                        self.log(f"[serve] t={done:7.2f}s done {req.rid} after {st['ncyc']} cycles "
+                                 f"(status {st['status']}, {done - req.arrival:.2f}s)")
+        finally:
+            with self._cv:
+                self._stop = True
+                self._cv.notify_all()
+            prep.join()
+            pool.shutdown()
+            for b in self.batches.values():
+                b.close()
+            self.batches.clear()
         return results
 
 

@@ -96,6 +96,15 @@ typedef struct {
                                  1 psmoo, per-cell coefficients from timestep
                                  (timestep.f90:286-309) solved line by line as
                                  progon3 does (psmoo.f90:1-95, progon3.f90:1-62) */
+  double ffreeze;             /* 0: the far-field rule of bc_far.f90:58-112 as is
+                                 (inflow or outflow by the sign of the normal
+                                 velocity at every call).  > 0: once the finest
+                                 level's residual has dropped by this factor from
+                                 its first cycle, each far-field ghost point keeps
+                                 the branch it last took, so points where the
+                                 normal velocity is near zero (a wake crossing a
+                                 face parallel to the stream) stop switching
+                                 between the two branches cycle after cycle       */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record

@@ -22,6 +22,9 @@ typedef struct {
   double cellvolume;      /* the block's cell volume (> 0), or 0: per-cell volumes in vol */
   double *vol;            /* stretched tensor levels: (dx_i*dy_j)*dz_k per cell, else NULL */
   double *v1d[3];         /* vertex coordinates x(0..ib), y(0..jb), z(0..kb) */
+  unsigned char *ffdec;   /* ffreeze > 0: far-field choice per ghost point of faces 1..6 (1 outflow) */
+  long ffoff[7];
+  const int *frozen;      /* the case's freeze flag */
   int bctype[6];
   double facenvec[6][3];
   double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
@@ -40,6 +43,8 @@ typedef struct {
 
 struct orc_case {
   orc_params prm;
+  int frozen;        /* far-field choices frozen (ffreeze > 0) */
+  double ffini;      /* first finest-cycle residual of that test */
   int nlevel;
   blk_t *lv; /* lv[1..nlevel] */
 };
@@ -93,7 +98,7 @@ static void blk_free(blk_t *b) {
   free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
   free(b->w1); free(b->w2);
   free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
-  free(b->vol); free(b->v1d[0]); free(b->v1d[1]); free(b->v1d[2]);
+  free(b->vol); free(b->v1d[0]); free(b->v1d[1]); free(b->v1d[2]); free(b->ffdec);
   free(b->epsx); free(b->epsy); free(b->epsz);
   free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
 }
@@ -155,6 +160,15 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->nmir = nmir;
   b->z = (orc_cut *)calloc((size_t)(nmir > 0 ? nmir : 1), sizeof(orc_cut));
   if (nmir > 0) memcpy(b->z, cuts, sizeof(orc_cut) * (size_t)nmir);
+  b->frozen = &c->frozen;
+  if (c->prm.ffreeze > 0.0) {
+    for (int f = 1; f <= 6; f++) {
+      int dir = (f - 1) % 3;
+      long amax = dir == 0 ? b->jb : b->ib, cmax = dir == 2 ? b->jb : b->kb;
+      b->ffoff[f] = b->ffoff[f - 1] + (amax + 1) * (cmax + 1);
+    }
+    b->ffdec = (unsigned char *)calloc((size_t)b->ffoff[6], 1);
+  }
   {
     const double *vv[3] = {xv, yv, zv};
     const int nv[3] = {b->ib + 1, b->jb + 1, b->kb + 1};
@@ -1467,7 +1481,13 @@ static void bc_far(blk_t *b, const orc_params *P, int face) {
       s = s + W(b, 3, q1) * fn[2];
       double vel_norm = s / W(b, 0, q1);
       double r, vel[3], pr;
-      if (vel_norm <= 0.0) { /* inlet */
+      int inflow = vel_norm <= 0.0;
+      if (b->ffdec) { /* the choice frozen after the ffreeze drop */
+        unsigned char *dec = &b->ffdec[b->ffoff[face - 1] + a + (long)(a_max + 1) * cc2];
+        if (*b->frozen) inflow = *dec == 0;
+        else *dec = inflow ? 0 : 1;
+      }
+      if (inflow) { /* inlet */
         double rhoe = W(b, 0, q1);
         double ue = W(b, 1 + dir, q1) / rhoe;
         double pe = b->p[q1];
@@ -1995,6 +2015,7 @@ int orc_movfin(orc_case *c, int mgl, int mlvl) {
 
 /* init_flow.f90:35-99 (uniform freestream) */
 int orc_init_flow(orc_case *c) {
+  c->frozen = 0;
   const orc_params *P = &c->prm;
   int warn = ORC_OK;
   for (int l = 1; l <= c->nlevel; l++) {
@@ -2089,6 +2110,11 @@ static int solve_ramp(orc_case *c, const int *mcyc, int itime, double *reslog, i
           icyc[mgl] = 0;
         }
         if (exit_mg) break;
+      }
+      if (P->ffreeze > 0.0 && mlvl == nlevel) { /* the far-field freeze test of the cycle */
+        double r = c->lv[mlvl].rtrms[ncyc];
+        if (ncyc == 1) c->ffini = r;
+        if (c->ffini / r > P->ffreeze) c->frozen = 1;
       }
       if (next_cycle) continue;
       if (mgl == mlvl) {

@@ -56,6 +56,9 @@ typedef struct {
   double dtstep;       /* nondimensional physical time step (input_param.f90:81-96) */
   int    smoother;     /* 0 psmoo_old (constant coefficient, what euler.f90:376 calls);
                           1 psmoo + progon3 (per-cell coefficients, psmoo.f90:1-95) */
+  double ffreeze;      /* 0: bc_far's inflow/outflow choice at every call (bc_far.f90:58-60);
+                          > 0: after the finest-level cycle whose residual has dropped by this
+                          factor from the first, each far-field ghost point keeps its last choice */
 } orc_params;
 
 typedef struct {

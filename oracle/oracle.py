@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
         ("distype", C.c_int), ("epsentr", C.c_double),
         ("execmode", C.c_int), ("dtstep", C.c_double),
         ("smoother", C.c_int),
+        ("ffreeze", C.c_double),
     ]
 
 
@@ -118,6 +119,7 @@ def params_from(prm):
     P.execmode = d["execmode"]
     P.dtstep = d["dtstep"]
     P.smoother = prm.smoother
+    P.ffreeze = prm.ffreeze
     return P
 
 

@@ -68,7 +68,9 @@ def test_serve_results_equal_alone(require_gpu, budget_cases):
         est = srv.estimate(reqs[rid], geoms)
         assert nbytes <= est <= 1.05 * nbytes
     p50, p99 = percentiles(res)
-    print(f"p50 {p50:.2f}s p99 {p99:.2f}s")
+    print(f"p50 {p50:.2f}s p99 {p99:.2f}s, peak reserved {srv.peak_used / 2**20:.0f} MiB of "
+          f"{budget / 2**20:.0f} MiB")
+    assert srv.peak_used <= budget
     if budget_cases == 2:
         # with room for two cases at a time, requests wait for admission
         assert max(r.admitted - r.arrival for r in res) > 0.0
