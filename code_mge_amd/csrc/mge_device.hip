@@ -170,10 +170,30 @@ struct mge_case {
 
 namespace {
 
+// ---- host-synchronous copies and clears ---------------------------------
+// On the calling thread's own default stream, after the stream whose work
+// they must follow (if any): the legacy null stream would implicitly
+// synchronise with every blocking stream and so join -- and break -- a
+// batch's graph capture running on another thread.
+hipError_t h2d_sync(void *dst, const void *src, size_t n, hipStream_t after = nullptr) {
+  hipError_t e = after ? hipStreamSynchronize(after) : hipSuccess;
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, hipStreamPerThread);
+  return e == hipSuccess ? hipStreamSynchronize(hipStreamPerThread) : e;
+}
+hipError_t d2h_sync(void *dst, const void *src, size_t n, hipStream_t after = nullptr) {
+  hipError_t e = after ? hipStreamSynchronize(after) : hipSuccess;
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, hipStreamPerThread);
+  return e == hipSuccess ? hipStreamSynchronize(hipStreamPerThread) : e;
+}
+hipError_t clear_sync(void *dst, size_t n) {
+  hipError_t e = hipMemsetAsync(dst, 0, n, hipStreamPerThread);
+  return e == hipSuccess ? hipStreamSynchronize(hipStreamPerThread) : e;
+}
+
 int dalloc_bytes(Level &L, void **p, size_t bytes) {
   void *q = nullptr;
   if (hipMalloc(&q, bytes + 64) != hipSuccess) return set_err(MGE_ERR_NOMEM, "hipMalloc failed");
-  if (hipMemset(q, 0, bytes + 64) != hipSuccess) {
+  if (clear_sync(q, bytes + 64) != hipSuccess) {
     hipFree(q);
     return set_err(MGE_ERR_HIP, "hipMemset failed");
   }
@@ -342,7 +362,7 @@ int check_err(mge_case *h) {
     else if (e == MGE_ERR_WSOL_NAN) m = "bc_cutcell: mirror state is NaN";
     else if (e == MGE_ERR_RES_NAN) m = "solver: residual is NaN";
     int z = 0;
-    hipMemcpy(h->err, &z, sizeof(int), hipMemcpyHostToDevice);
+    h2d_sync(h->err, &z, sizeof(int), h->stream);
     return set_err(e, m);
   }
   return MGE_OK;
@@ -1136,7 +1156,7 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
   CHKRC(dalloc(L, &L.beg, begv.size()));
   CHKRC(dalloc(L, &L.ops, opv.size() + 1));
   CHKRC(dalloc(L, &L.opval, (opv.size() + 1) * 10));
-  HIPCHK(hipMemcpy(L.recs, dr.data(), sizeof(DevRec) * nmir, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(L.recs, dr.data(), sizeof(DevRec) * nmir, h->stream));
   {
     // cut cells with wall records, in order of their first record, and
     // their records in record order (viscous wall terms)
@@ -1166,36 +1186,36 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
     CHKRC(dalloc(L, &L.vcell, vc.size()));
     CHKRC(dalloc(L, &L.vbeg, vb.size()));
     CHKRC(dalloc(L, &L.vrec, vr.size()));
-    HIPCHK(hipMemcpy(L.vcell, vc.data(), sizeof(long) * vc.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L.vbeg, vb.data(), sizeof(int) * vb.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L.vrec, vr.data(), sizeof(int) * vr.size(), hipMemcpyHostToDevice));
+    HIPCHK(h2d_sync(L.vcell, vc.data(), sizeof(long) * vc.size(), h->stream));
+    HIPCHK(h2d_sync(L.vbeg, vb.data(), sizeof(int) * vb.size()));
+    HIPCHK(h2d_sync(L.vrec, vr.data(), sizeof(int) * vr.size()));
   }
-  HIPCHK(hipMemcpy(L.tcell, tcell.data(), sizeof(long) * tcell.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(L.beg, begv.data(), sizeof(int) * begv.size(), hipMemcpyHostToDevice));
-  if (!opv.empty()) HIPCHK(hipMemcpy(L.ops, opv.data(), sizeof(int2) * opv.size(), hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(L.tcell, tcell.data(), sizeof(long) * tcell.size(), h->stream));
+  HIPCHK(h2d_sync(L.beg, begv.data(), sizeof(int) * begv.size()));
+  if (!opv.empty()) HIPCHK(h2d_sync(L.ops, opv.data(), sizeof(int2) * opv.size()));
   std::vector<uint8_t> fl(d.E);
-  HIPCHK(hipMemcpy(fl.data(), L.flags, d.E, hipMemcpyDeviceToHost));
+  HIPCHK(d2h_sync(fl.data(), L.flags, d.E, h->stream));
   for (long q : tcell) fl[q] |= F_TOUCHED;
-  HIPCHK(hipMemcpy(L.flags, fl.data(), d.E, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(L.flags, fl.data(), d.E));
   std::vector<double> w5(5 * (size_t)nmir), p1(nmir);
   for (int m = 0; m < nmir; m++) {
     for (int n = 0; n < 5; n++) w5[(size_t)n * nmir + m] = recs[m].wsol[n];
     p1[m] = recs[m].psol;
   }
-  HIPCHK(hipMemcpy(L.wsol, w5.data(), sizeof(double) * w5.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(L.psol, p1.data(), sizeof(double) * p1.size(), hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(L.wsol, w5.data(), sizeof(double) * w5.size(), h->stream));
+  HIPCHK(h2d_sync(L.psol, p1.data(), sizeof(double) * p1.size()));
   return MGE_OK;
 }
 
 // host FP64 array -> device bulk array of the case's type
 int put_bulk(mge_case *h, void *dev, const double *src, size_t n) {
   if (!h->fp32) {
-    HIPCHK(hipMemcpy(dev, src, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(h2d_sync(dev, src, sizeof(double) * n, h->stream));
     return MGE_OK;
   }
   std::vector<float> t(n);
   for (size_t q = 0; q < n; q++) t[q] = (float)src[q];
-  HIPCHK(hipMemcpy(dev, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(dev, t.data(), sizeof(float) * n, h->stream));
   return MGE_OK;
 }
 
@@ -1342,9 +1362,8 @@ int batch_grow(mge_batch *b, int need) {
   b->hact.resize(cap, 0);
   // move the residual logs of existing slots to the new log array
   std::vector<double> lg((size_t)cap * b->cap, 0.0);
-  if (b->nslot_cap) HIPCHK(hipMemcpy(lg.data(), b->d_log, sizeof(double) * (size_t)b->nslot_cap * b->cap,
-                                     hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(nl, lg.data(), sizeof(double) * lg.size(), hipMemcpyHostToDevice));
+  if (b->nslot_cap) HIPCHK(d2h_sync(lg.data(), b->d_log, sizeof(double) * (size_t)b->nslot_cap * b->cap));
+  HIPCHK(h2d_sync(nl, lg.data(), sizeof(double) * lg.size()));
   for (int c = 0; c < cap; c++) b->hctl[c].log = nl + (size_t)c * b->cap;
   if (b->d_ctl) hipFree(b->d_ctl);
   if (b->d_act) hipFree(b->d_act);
@@ -1353,8 +1372,8 @@ int batch_grow(mge_batch *b, int need) {
   b->d_act = na;
   b->d_log = nl;
   b->nslot_cap = cap;
-  HIPCHK(hipMemcpy(b->d_ctl, b->hctl.data(), sizeof(BCtl) * cap, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->d_act, b->hact.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(b->d_ctl, b->hctl.data(), sizeof(BCtl) * cap));
+  HIPCHK(h2d_sync(b->d_act, b->hact.data(), sizeof(int) * cap));
   return MGE_OK;
 }
 
@@ -1503,7 +1522,7 @@ int batch_build(mge_batch *b) {
     st.nc = ng;
     b->prog.push_back(st);
   }
-  if (total) HIPCHK(hipMemcpy(b->d_arena, host.data(), total, hipMemcpyHostToDevice));
+  if (total) HIPCHK(h2d_sync(b->d_arena, host.data(), total, b->stream));
   if (b->use_graph) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = batch_enqueue(b);
@@ -1597,7 +1616,7 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
   h->nlevel = nlevel;
   h->lv.resize(nlevel + 1);
   h->smoo_bb = -prm->smoo;
-  if (hipStreamCreate(&h->stream) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return set_err(MGE_ERR_HIP, "hipStreamCreate failed");
   }
@@ -1609,9 +1628,9 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
     delete h;
     return set_err(MGE_ERR_NOMEM, "hipMalloc failed");
   }
-  hipMemset(h->err, 0, sizeof(int));
-  hipMemset(h->ffrozen, 0, sizeof(int));
-  hipMemset(h->ffini, 0, sizeof(double));
+  clear_sync(h->err, sizeof(int));
+  clear_sync(h->ffrozen, sizeof(int));
+  clear_sync(h->ffini, sizeof(double));
   *out = h;
   return MGE_OK;
 }
@@ -1774,7 +1793,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
       bl[X(i, j, d.ke)] = bl[X(i, j, d.kl)];
       bl[X(i, j, d.kb)] = bl[X(i, j, d.kl)];
     }
-  HIPCHK(hipMemcpy(L.flags, bl.data(), E, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(L.flags, bl.data(), E, h->stream));
   L.cur = 0;
   L.defined = true;
   int rc = build_records(h, L, nmir, recs);
@@ -1954,7 +1973,7 @@ int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *i
     double off = nc == 5 ? (double)f.w[q / E] : (double)f.p;
     t[q] = (float)(in[q] - off);
   }
-  HIPCHK(hipMemcpy(p, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  HIPCHK(h2d_sync(p, t.data(), sizeof(float) * n, h->stream));
   return MGE_OK;
 }
 
@@ -2104,8 +2123,8 @@ int mge_read_solut(mge_case *h, const char *path, int format) {
       rc = mge_upload(h, lvl, 1, "p", p.data());
       if (rc < 0) break;
       if (nmir > 0) {
-        HIPCHK(hipMemcpy(L.wsol, ws.data(), sizeof(double) * ws.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(L.psol, ps.data(), sizeof(double) * ps.size(), hipMemcpyHostToDevice));
+        HIPCHK(h2d_sync(L.wsol, ws.data(), sizeof(double) * ws.size(), h->stream));
+        HIPCHK(h2d_sync(L.psol, ps.data(), sizeof(double) * ps.size()));
       }
     } else {
       rc = by_type(h, [&](auto z) { return restore_level<decltype(z)>(h, L, true); });
@@ -2254,7 +2273,7 @@ int mge_batch_create(int use_graph, int log_cap, mge_batch **out) {
   mge_batch *b = new mge_batch();
   b->use_graph = use_graph != 0;
   b->cap = log_cap > 0 ? log_cap : 1;
-  if (hipStreamCreate(&b->stream) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return set_err(MGE_ERR_HIP, "hipStreamCreate failed");
   }

@@ -30,7 +30,7 @@ def test_ffreeze_parity_sphere(require_gpu):
 def test_ffreeze_batch_equals_own_runs(require_gpu):
     specs = []
     for n, a in ((32, 8.0), (40, 12.0)):
-        prm, levels, bdy, _ = configs.sphere3d(n=n, nlevel=3, box=3.0, alpha=a, ffreeze=30.0)
+        prm, levels, bdy, _ = configs.sphere3d(n=n, nlevel=3, box=3.0, alpha=a, ffreeze=10.0)
         specs.append((prm, levels, [level_geometry(s, bdy) for s in levels]))
     ncyc = 24
     ref = []
@@ -38,7 +38,7 @@ def test_ffreeze_batch_equals_own_runs(require_gpu):
         with Case(prm, levels, geoms=geoms, mxcyc=4) as c:
             c.init_flow()
             res = [c.cycles(3, 1, ncyc0=k)[0] for k in range(1, ncyc + 1)]
-            assert res[0] / min(res) > 30.0
+            assert res[0] / min(res) > 10.0       # the freeze triggers within the run
             ref.append((np.array(res), c.download(3, "w")))
     cases, slots = [], []
     with Batch(use_graph=True, log_cap=ncyc + 2) as b:
