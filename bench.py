@@ -115,8 +115,8 @@ def run_leg(prm, levels, geoms, aref, args, precision):
     nroof = 0 if args.no_timing else args.roofline_steps
     case = Case(pp, levels, geoms=geoms, mxcyc=args.warmup + args.steps + nroof + 2)
     case.init_flow()
-    tuned = case.autotune()   # k-chunk length of the stage kernel per level
-    print(f"[bench] FP{precision}: autotuned k-chunks {tuned}", file=sys.stderr, flush=True)
+    tuned = case.autotune()   # tile rows and k-chunk length of the stage kernel per level
+    print(f"[bench] FP{precision}: autotuned (tile rows, k-chunk, us) {tuned}", file=sys.stderr, flush=True)
     nl = args.levels
     print(f"[bench] FP{precision}: warmup {args.warmup} cycles", file=sys.stderr, flush=True)
     r0, _ = case.cycles(nl, args.warmup, ncyc0=1)
@@ -152,7 +152,7 @@ def run_leg(prm, levels, geoms, aref, args, precision):
     kernels = finest_kernels(case, levels[-1].n, nl, precision) if nroof > 0 else None
     case.close()
     return dict(value=cs / dt / 1e6, dt=dt, cs=cs, cd=cd, cl=cl, roofline=roofline, r0=r0, r1=r1,
-                kc={lvl: kc for lvl, (kc, us) in tuned.items()}, kernels=kernels)
+                tile={lvl: [tj, kc] for lvl, (tj, kc, us) in tuned.items()}, kernels=kernels)
 
 
 def finest_kernels(case, n, lvl, precision):
@@ -200,7 +200,7 @@ def jst_leg(args):
     return out
 
 
-def sweep_leg(k, drop=1e5, maxcyc=3000):
+def sweep_leg(k, drop=1e5, maxcyc=3000, ffreeze=1e3):
     """Converged solves per hour on the first k cases of the seeded ellipsoid
     sweep: FP32 served (continuous batching, all requests at t = 0) against
     the plain FP64 path one case at a time (solver.f90's loop with the drop
@@ -214,7 +214,7 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
     sweep = configs.ellipsoid_sweep()[:k]
     reqs = [Request(i, 0.0, prm, levels, bdy, aref, info) for i, (prm, levels, bdy, aref, info) in enumerate(sweep)]
     print(f"[bench] sweep: {k} cases, FP32 served", file=sys.stderr, flush=True)
-    srv = Server(hbm_budget=200e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4,
+    srv = Server(hbm_budget=200e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4, ffreeze=ffreeze,
                  log=lambda m: print(m, file=sys.stderr, flush=True) if " done " in m else None)
     t0 = time.perf_counter()
     served = srv.run(reqs)
@@ -224,7 +224,7 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
     t0 = time.perf_counter()
     alone = {}
     for i, (prm, levels, bdy, aref, info) in enumerate(sweep):
-        p64 = dataclasses.replace(prm, precision=64, drop=drop)
+        p64 = dataclasses.replace(prm, precision=64, drop=drop, ffreeze=ffreeze)
         geoms = [level_geometry(s, bdy) for s in levels]
         with Case(p64, levels, geoms=geoms, mxcyc=maxcyc + 2) as c:
             c.init_flow()
@@ -242,7 +242,7 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
     s32 = conv32 / t32 * 3600.0
     s64 = conv64 / t64 * 3600.0
     return {"sample": f"first {k} cases of the seeded ellipsoid sweep (PCG64(20261202), N 48..160, 3 levels), "
-                      f"JST, residual drop {drop:g}",
+                      f"JST, residual drop {drop:g}, far-field choice frozen at a drop of {ffreeze:g} (both sides)",
             "fp32_served_solves_per_h": round(s32, 1), "fp64_one_at_a_time_solves_per_h": round(s64, 1),
             "ratio": round(s32 / s64, 2) if s64 > 0 else None,
             "fp32_converged": conv32, "fp64_converged": conv64, "cases": k,
@@ -256,8 +256,9 @@ def sweep_leg(k, drop=1e5, maxcyc=3000):
 def traffic_probe(args, precision):
     """A short program for rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE):
     a 1 GiB device-to-device copy (the calibration dispatch: 1 GiB read,
-    1 GiB written) and then one cycle of the bench workload, whose
-    finest-level stage-kernel dispatches give the HBM bytes per launch."""
+    1 GiB written) and then, after the autotuner's launches, one cycle of
+    the bench workload, whose finest-level stage-kernel dispatches give the
+    HBM bytes per launch."""
     import dataclasses
     import torch
     from code_mge_amd import configs
@@ -273,6 +274,7 @@ def traffic_probe(args, precision):
     geoms = [level_geometry(s, bdy) for s in levels]
     with Case(dataclasses.replace(prm, precision=precision), levels, geoms=geoms, mxcyc=4) as c:
         c.init_flow()
+        c.autotune()       # the launch shapes of the timed bench
         c.cycles(args.levels, 1, ncyc0=1)
         c.sync()
     print(f"[bench] traffic probe FP{precision} done", file=sys.stderr, flush=True)
@@ -343,7 +345,7 @@ def ns_leg(cycles):
     return out
 
 
-def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000):
+def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000, ffreeze=1e3):
     """Poisson serving stream (PCG64(20261203), `rate` requests/s, half of
     them laminar Navier-Stokes): ellipsoids, grids 32^3..192^3, 3 levels,
     FP32 served under an HBM budget to a residual drop of `drop`;
@@ -351,7 +353,7 @@ def serve_leg(count, rate=4.0, budget_gb=200.0, drop=1e5, maxcyc=3000):
     from code_mge_amd.serve import Server, outcome_counts, percentiles, poisson_requests
     reqs = poisson_requests(count=count, rate=rate)
     srv = Server(hbm_budget=budget_gb * 1e9, drop=drop, maxcyc=maxcyc, precision=32, periods=4,
-                 log=lambda m: print(m, file=sys.stderr, flush=True))
+                 ffreeze=ffreeze, log=lambda m: print(m, file=sys.stderr, flush=True))
     t0 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -446,7 +448,7 @@ def main():
         "config": {"workload": f"sphere {args.n}^3 in [-8,8]^3, M=0.5, alpha=2, {args.levels}-level W-cycle, "
                                "5-stage RK with residual smoothing, Roe flux + MUSCL-minmod, FP64",
                    "n": args.n, "levels": args.levels, "cell_stages_per_step": cs // args.steps,
-                   "cd": round(cd, 6), "cl": round(cl, 6), "stage_kchunk_per_level": main_leg["kc"]},
+                   "cd": round(cd, 6), "cl": round(cl, 6), "stage_tile_per_level": main_leg["tile"]},
         "roofline": roofline,
         "kernels": main_leg["kernels"],
         "cpu_baseline": cpu,

@@ -88,6 +88,7 @@ struct Level {
   void *vol = nullptr;
   std::vector<double> hv[3];
   uint8_t *ffdec = nullptr;  // far-field branch per ghost point of the six faces (ffreeze > 0)
+  unsigned long long *lim = nullptr;  // limiter words per direction and cell (Roe r2, lfreeze > 0)
   long ffoff[7] = {0, 0, 0, 0, 0, 0, 0};
   void *rw[3] = {nullptr, nullptr, nullptr}, *pw[3] = {nullptr, nullptr, nullptr};
   uint8_t *flags = nullptr, *marker = nullptr;
@@ -111,6 +112,7 @@ struct Level {
   double *rtrms = nullptr;
   double *force = nullptr;
   int kc = 0;                   // autotuned k planes per stage workgroup (0: fill rule)
+  int tj = 0;                   // autotuned tile rows of the stage kernels (0: the precision's default)
   void *vprim = nullptr;        // Navier-Stokes: velocity and temperature per cell (4E)
   int nvcell = 0;               // cut cells with wall records (viscous wall terms)
   long *vcell = nullptr;
@@ -147,7 +149,8 @@ struct TStat {
 struct mge_case {
   mge_params prm{};
   int dev = 0;            // HIP device of the case's stream
-  int *ffrozen = nullptr;     // far-field branches frozen (device flag, ffreeze > 0)
+  int *ffrozen = nullptr;     // device flags: [0] far-field branches frozen (ffreeze > 0),
+                              // [1] limiter choices frozen (lfreeze > 0)
   double *ffini = nullptr;    // first finest-cycle residual for that test (device)
   double *scratch = nullptr;  // that device's address of the fixed staging array
   bool fp32 = false;
@@ -185,10 +188,11 @@ hipError_t d2h_sync(void *dst, const void *src, size_t n, hipStream_t after = nu
   if (e == hipSuccess) e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, hipStreamPerThread);
   return e == hipSuccess ? hipStreamSynchronize(hipStreamPerThread) : e;
 }
-hipError_t clear_sync(void *dst, size_t n) {
-  hipError_t e = hipMemsetAsync(dst, 0, n, hipStreamPerThread);
+hipError_t fill_sync(void *dst, int byte, size_t n) {
+  hipError_t e = hipMemsetAsync(dst, byte, n, hipStreamPerThread);
   return e == hipSuccess ? hipStreamSynchronize(hipStreamPerThread) : e;
 }
+hipError_t clear_sync(void *dst, size_t n) { return fill_sync(dst, 0, n); }
 
 int dalloc_bytes(Level &L, void **p, size_t bytes) {
   void *q = nullptr;
@@ -538,6 +542,8 @@ StageArgs<T> stage_args(mge_case *h, Level &L, int m, int src, int dst, int w0, 
   A.vt = P.vt;
   A.order = P.distype == 2 ? 2 : 1;
   A.eps = (T)P.epsentr;
+  A.lim = L.lim;
+  A.lfrozen = h->ffrozen + 1;
   A.g1 = A.fx.gamma / (A.fx.gamma - T(1));
   if (P.execmode == 1) {
     A.tsrc = h->itime > 1 ? 2 : (h->itime == 1 ? 1 : 0);
@@ -550,6 +556,14 @@ StageArgs<T> stage_args(mge_case *h, Level &L, int m, int src, int dst, int w0, 
   return A;
 }
 
+// tile rows of the LDS-tiled stage kernels: the level's autotuned value
+// when set, else the precision's default
+template <class T>
+int stage_tj(const Level &L, int distype) {
+  if (L.tj > 0) return L.tj;
+  return distype != 0 ? RoeTile<T>::TJ : Stage2Tile<T>::TJ;
+}
+
 // k planes per workgroup of the LDS-tiled stage kernels: the level's
 // autotuned value when set, else halve from nk while the launch has fewer
 // than 2048 workgroups (enough to fill 256 CUs)
@@ -560,6 +574,23 @@ int stage_kc(const Level &L) {
   long tiles = (long)((d.ni + TI - 1) / TI) * ((d.nj + TJ - 1) / TJ);
   while (kc > 8 && tiles * ((d.nk + kc - 1) / kc) < 2048) kc = (kc + 1) / 2;
   return kc;
+}
+
+// single-case and batch entry points of the LDS-tiled stage kernel of a
+// scheme (distype 0: JST, else Roe, with the limiter-freezing paths when
+// lim) with tj tile rows (4 or 8)
+template <class T>
+void stage_fns(int distype, int tj, bool lim, const void **fs, const void **fb) {
+  if (distype != 0 && lim) {
+    *fs = tj == 4 ? (const void *)&k_stage_roe<T, 4, true> : (const void *)&k_stage_roe<T, 8, true>;
+    *fb = tj == 4 ? (const void *)&k_stage_roe_b<T, 4, true> : (const void *)&k_stage_roe_b<T, 8, true>;
+  } else if (distype != 0) {
+    *fs = tj == 4 ? (const void *)&k_stage_roe<T, 4, false> : (const void *)&k_stage_roe<T, 8, false>;
+    *fb = tj == 4 ? (const void *)&k_stage_roe_b<T, 4, false> : (const void *)&k_stage_roe_b<T, 8, false>;
+  } else {
+    *fs = tj == 4 ? (const void *)&k_stage2<T, 4> : (const void *)&k_stage2<T, 8>;
+    *fb = tj == 4 ? (const void *)&k_stage2_b<T, 4> : (const void *)&k_stage2_b<T, 8>;
+  }
 }
 
 template <class T>
@@ -612,12 +643,16 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
     }
     {
       const int kc = stage_kc(L);
-      const int tj = P.distype != 0 ? RoeTile<T>::TJ : Stage2Tile<T>::TJ;  // tile rows of the LDS-tiled kernel
+      const int tj = stage_tj<T>(L, P.distype);  // tile rows of the LDS-tiled kernel
       dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
       A.kc = kc;
-      if (P.distype != 0) LAUNCHT(h, "k_stage", k_stage_roe, T, g, dim3(TI, tj), A);
-      else if (P.dissip == 0) LAUNCHT(h, "k_stage", k_stage2, T, g, dim3(TI, tj), A);
-      else LAUNCHT(h, "k_stage", k_stage, T, cell_grid(d), cell_block(), A);  // matrix dissipation
+      if (P.distype != 0 || P.dissip == 0) {
+        const void *fs, *fb;
+        stage_fns<T>(P.distype, tj, A.lim != nullptr, &fs, &fb);
+        launch(h, "k_stage", fs, fb, g, dim3(TI, tj), A);
+      } else {
+        LAUNCHT(h, "k_stage", k_stage, T, cell_grid(d), cell_block(), A);  // matrix dissipation
+      }
     }
     {
       RecArgs<T> R;
@@ -886,20 +921,23 @@ int do_cycle(mge_case *h, int mlvl, int ncyc, long long *cs) {
 // the far-field freeze test of one case after a finest-level cycle, on the
 // device so that it needs no host round trip (k_converge makes the same
 // test for batched cases)
-__global__ void k_ffreeze(const double *rtrms_slot, double *ini, int ncyc, double ffreeze, int *frozen) {
+__global__ void k_ffreeze(const double *rtrms_slot, double *ini, int ncyc, double ffreeze, double lfreeze,
+                          int *frozen) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double r = *rtrms_slot;
   if (ncyc == 1) *ini = r;
-  if (*ini / r > ffreeze) *frozen = 1;
+  if (ffreeze > 0.0 && *ini / r > ffreeze) frozen[0] = 1;
+  if (lfreeze > 0.0 && *ini / r > lfreeze) frozen[1] = 1;
 }
 
-// after finest-level cycle ncyc of a single case: the freeze test
+// after finest-level cycle ncyc of a single case: the freeze tests (far
+// field and limiter)
 int ffreeze_test(mge_case *h, int mlvl, int ncyc) {
-  if (!(h->prm.ffreeze > 0.0) || mlvl != h->nlevel || h->rec) return MGE_OK;
+  if (!(h->prm.ffreeze > 0.0 || h->prm.lfreeze > 0.0) || mlvl != h->nlevel || h->rec) return MGE_OK;
   Level &B = h->lv[mlvl];
   const int c = ncyc >= 1 && ncyc <= B.mxcyc ? ncyc : 0;
   hipLaunchKernelGGL(k_ffreeze, dim3(1), dim3(64), 0, h->stream, (const double *)(B.rtrms + c), h->ffini, ncyc,
-                     h->prm.ffreeze, h->ffrozen);
+                     h->prm.ffreeze, h->prm.lfreeze, h->ffrozen);
   HIPCHK(hipGetLastError());
   return MGE_OK;
 }
@@ -954,23 +992,29 @@ int do_solve(mge_case *h, const int *mcyc, int itime, double *reslog, int cap, i
 
 // ---- autotuner for the LDS-tiled stage kernels ---------------------------
 //
-// The one launch parameter of k_stage2/k_stage_roe that changes neither the
-// arithmetic nor the results is kc, the number of k planes a workgroup
-// marches through (the column carry restarts at every chunk): few planes
-// give many workgroups, many planes amortise the chunk start.  The tuner
-// times stage 1 of the finest-level configuration for kc = 1, 2, 4, ... nk
-// (one warm-up launch, then the best of three) on the level's own arrays,
-// writing only a state buffer that is not current, dw, fw and the time-step
-// arrays (all rewritten by the next timestep/euler call).  Results are
-// cached per (shape, precision, scheme) for the process.
+// The launch shape of k_stage2/k_stage_roe changes neither the arithmetic
+// nor the results: every face value has the same expression whatever tile
+// evaluates it.  Two parameters are tuned together: the tile rows tj (4 or
+// 8: 256 or 512 threads, the LDS tile and its halo share) and kc, the number
+// of k planes a workgroup marches through (the column carry restarts at
+// every chunk): few planes give many workgroups, many planes amortise the
+// chunk start.  The tuner times stage 1 of the level for tj = 4, 8 and
+// kc = 1, 2, 4, ... nk (one warm-up launch, then the best of three) on the
+// level's own arrays, writing only a state buffer that is not current, dw,
+// fw and the time-step arrays (all rewritten by the next timestep/euler
+// call).  Results are cached per (shape, precision, scheme) for the process.
 struct TuneKey {
   int ni, nj, nk, fp32, distype, smoothed;
   bool operator<(const TuneKey &o) const {
     return std::tie(ni, nj, nk, fp32, distype, smoothed) < std::tie(o.ni, o.nj, o.nk, o.fp32, o.distype, o.smoothed);
   }
 };
+struct TuneVal {
+  int tj, kc;
+  double us;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, double>> g_tune;
+std::map<TuneKey, TuneVal> g_tune;
 
 template <class T>
 int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
@@ -979,6 +1023,7 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   if (h->rec) return set_err(MGE_ERR_STATE, "autotune while a batch is being recorded");
   if (P.distype == 0 && P.dissip != 0) {  // matrix dissipation: one thread per cell, nothing to tune
     L.kc = 0;
+    L.tj = 0;
     if (kc_out) *kc_out = 0;
     if (us_out) *us_out = 0.0;
     return MGE_OK;
@@ -988,9 +1033,10 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      L.kc = it->second.first;
+      L.kc = it->second.kc;
+      L.tj = it->second.tj;
       if (kc_out) *kc_out = L.kc;
-      if (us_out) *us_out = it->second.second;
+      if (us_out) *us_out = it->second.us;
       return MGE_OK;
     }
   }
@@ -1000,43 +1046,47 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   while (dst == w0) dst++;
   StageArgs<T> A = stage_args<T>(h, L, 1, w0, dst, w0, 1, 1, 0, 0);
   A.fw_in = 0;
-  const void *fn = P.distype != 0 ? (const void *)&k_stage_roe<T> : (const void *)&k_stage2<T>;
-  const int tj = P.distype != 0 ? RoeTile<T>::TJ : Stage2Tile<T>::TJ;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  int best = 0;
+  int best = 0, best_tj = 0;
   double best_us = 1e300;
   hipError_t err = hipSuccess;
-  for (int kc = 1;; kc = std::min(2 * kc, d.nk)) {
-    A.kc = kc;
-    dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
-    void *args[1] = {(void *)&A};
-    double t = 1e300;
-    for (int rep = 0; rep < 4 && err == hipSuccess; rep++) {
-      err = hipEventRecord(e0, h->stream);
-      if (err == hipSuccess) err = hipLaunchKernel(fn, g, dim3(TI, tj), args, 0, h->stream);
-      if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
-      if (err == hipSuccess) err = hipEventSynchronize(e1);
-      float ms = 0.f;
-      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-      if (rep > 0 && err == hipSuccess) t = std::min(t, 1e3 * (double)ms);
+  for (int tj : {4, 8}) {
+    const void *fn, *fb;
+    stage_fns<T>(P.distype, tj, A.lim != nullptr, &fn, &fb);
+    for (int kc = 1; err == hipSuccess; kc = std::min(2 * kc, d.nk)) {
+      A.kc = kc;
+      dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);
+      void *args[1] = {(void *)&A};
+      double t = 1e300;
+      for (int rep = 0; rep < 4 && err == hipSuccess; rep++) {
+        err = hipEventRecord(e0, h->stream);
+        if (err == hipSuccess) err = hipLaunchKernel(fn, g, dim3(TI, tj), args, 0, h->stream);
+        if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && err == hipSuccess) t = std::min(t, 1e3 * (double)ms);
+      }
+      if (err != hipSuccess) break;
+      if (t < best_us) {
+        best_us = t;
+        best = kc;
+        best_tj = tj;
+      }
+      if (kc == d.nk) break;
     }
-    if (err != hipSuccess) break;
-    if (t < best_us) {
-      best_us = t;
-      best = kc;
-    }
-    if (kc == d.nk) break;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (err != hipSuccess) return set_err(MGE_ERR_HIP, hipGetErrorString(err));
   CHKRC(check_err(h));
   L.kc = best;
+  L.tj = best_tj;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {best, best_us};
+    g_tune[key] = {best_tj, best, best_us};
   }
   if (kc_out) *kc_out = best;
   if (us_out) *us_out = best_us;
@@ -1287,8 +1337,9 @@ struct BCtl {
   double *log;          // residual history (cap entries)
   double ini, last, drop;
   int ncyc, maxcyc, status, cap;
-  int *frozen;          // the case's far-field freeze flag
-  double ffreeze;       // its freeze drop (0: never)
+  int *frozen;          // the case's freeze flags (far field, limiter)
+  double ffreeze;       // its far-field freeze drop (0: never)
+  double lfreeze;       // its limiter freeze drop (0: never)
 };
 
 __global__ void k_converge(BCtl *ctl, int *act, int n, int slot) {
@@ -1299,7 +1350,8 @@ __global__ void k_converge(BCtl *ctl, int *act, int n, int slot) {
     C.ncyc += 1;
     if (C.ncyc == 1) C.ini = r;
     C.last = r;
-    if (C.ffreeze > 0.0 && C.ini / r > C.ffreeze) *C.frozen = 1;
+    if (C.ffreeze > 0.0 && C.ini / r > C.ffreeze) C.frozen[0] = 1;
+    if (C.lfreeze > 0.0 && C.ini / r > C.lfreeze) C.frozen[1] = 1;
     if (C.ncyc - 1 < C.cap) C.log[C.ncyc - 1] = r;
     int e = *C.err;
     if (r != r || e < 0) {
@@ -1620,7 +1672,7 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
     delete h;
     return set_err(MGE_ERR_HIP, "hipStreamCreate failed");
   }
-  if (hipMalloc(&h->err, sizeof(int)) != hipSuccess || hipMalloc(&h->ffrozen, sizeof(int)) != hipSuccess ||
+  if (hipMalloc(&h->err, sizeof(int)) != hipSuccess || hipMalloc(&h->ffrozen, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&h->ffini, sizeof(double)) != hipSuccess) {
     if (h->err) hipFree(h->err);
     if (h->ffrozen) hipFree(h->ffrozen);
@@ -1629,7 +1681,7 @@ int mge_case_create(const mge_params *prm, int nlevel, mge_case **out) {
     return set_err(MGE_ERR_NOMEM, "hipMalloc failed");
   }
   clear_sync(h->err, sizeof(int));
-  clear_sync(h->ffrozen, sizeof(int));
+  clear_sync(h->ffrozen, 2 * sizeof(int));
   clear_sync(h->ffini, sizeof(double));
   *out = h;
   return MGE_OK;
@@ -1709,6 +1761,12 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
       L.ffoff[f] = L.ffoff[f - 1] + (amax + 1) * (cmax + 1);
     }
     CHKRC(dalloc(L, &L.ffdec, L.ffoff[6]));
+  }
+  if (h->prm.lfreeze > 0.0 && h->prm.distype == 2) {
+    // every byte starts "not recorded" (255): the ghost cells, whose values
+    // no stage records, keep the limiter itself
+    CHKRC(dalloc(L, &L.lim, 3 * E));
+    HIPCHK(fill_sync(L.lim, 0xff, (size_t)E * 3 * sizeof(unsigned long long)));
   }
   if (h->prm.eqntype == 1) CHKRC(dalloc_bytes(L, &L.vprim, es * 4 * E));
   if (h->prm.execmode == 1) {
@@ -1808,7 +1866,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
 
 int mge_init_flow(mge_case *h) {
   if (!h) return set_err(MGE_ERR_ARG, "null handle");
-  HIPCHK(hipMemsetAsync(h->ffrozen, 0, sizeof(int), h->stream));  // a new solve chooses afresh
+  HIPCHK(hipMemsetAsync(h->ffrozen, 0, 2 * sizeof(int), h->stream));  // a new solve chooses afresh
   int rc = by_type(h, [&](auto z) { return do_init_flow<decltype(z)>(h); });
   if (rc < 0) return rc;
   int e = check_err(h);
@@ -1937,6 +1995,18 @@ static void *field_ptr(Level &L, size_t es, const char *name, long *n) {
 int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1 || !field || !out) return set_err(MGE_ERR_ARG, "bad download arguments");
+  if (!strcmp(field, "lim")) {  // limiter bytes (limiter freezing): out[(5 dir + n) E + q]
+    if (!L->lim) return set_err(MGE_ERR_ARG, "the level has no limiter words (lfreeze = 0 or not Roe r2)");
+    const long E = L->d.E;
+    std::vector<unsigned long long> t(3 * E);
+    HIPCHK(hipMemcpyAsync(t.data(), L->lim, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost,
+                          h->stream));
+    int e = check_err(h);
+    for (int d = 0; d < 3; d++)
+      for (int n = 0; n < 5; n++)
+        for (long q = 0; q < E; q++) out[(long)(5 * d + n) * E + q] = (double)((t[d * E + q] >> (8 * n)) & 255u);
+    return e;
+  }
   long n;
   void *p = field_ptr(*L, h->es, field, &n);
   if (!p) return set_err(MGE_ERR_ARG, "unknown field");
@@ -2329,6 +2399,7 @@ int mge_batch_add(mge_batch *b, mge_case *h, double drop, int maxcyc) {
   C.cap = b->cap;
   C.frozen = h->ffrozen;
   C.ffreeze = h->prm.ffreeze;
+  C.lfreeze = h->prm.lfreeze;
   b->hact[slot] = 1;
   CHKRC(batch_push(b));
   b->dirty = true;
@@ -2429,6 +2500,30 @@ int mge_autotune(mge_case *h, int lvl, int *kc, double *us) {
 int mge_set_kc(mge_case *h, int lvl, int kc) {
   Level *L = get_level(h, lvl);
   if (!L || kc < 0) return set_err(MGE_ERR_ARG, "bad level or kc");
+  L->kc = kc;
+  return MGE_OK;
+}
+
+int mge_set_freeze(mge_case *h, int far, int lim) {
+  if (!h) return set_err(MGE_ERR_ARG, "null handle");
+  const int v[2] = {far ? 1 : 0, lim ? 1 : 0};
+  HIPCHK(h2d_sync(h->ffrozen, v, sizeof(v), h->stream));
+  return MGE_OK;
+}
+
+int mge_stage_tile(mge_case *h, int lvl, int *tj, int *kc) {
+  Level *L = get_level(h, lvl);
+  if (!L || !tj || !kc) return set_err(MGE_ERR_ARG, "bad level or output pointer");
+  const int distype = h->prm.distype;
+  *tj = h->fp32 ? stage_tj<float>(*L, distype) : stage_tj<double>(*L, distype);
+  *kc = stage_kc(*L);
+  return MGE_OK;
+}
+
+int mge_set_tile(mge_case *h, int lvl, int tj, int kc) {
+  Level *L = get_level(h, lvl);
+  if (!L || !(tj == 0 || tj == 4 || tj == 8) || kc < 0) return set_err(MGE_ERR_ARG, "bad level, tile rows or kc");
+  L->tj = tj;
   L->kc = kc;
   return MGE_OK;
 }

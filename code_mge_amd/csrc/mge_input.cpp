@@ -478,6 +478,7 @@ int mge_job_params(const mge_job *job, int precision, mge_params *P) {
   if (job->eqntype == 1 && job->execmode == 1) P->dtstep = P->u0 / 1.0 / 5.0;
   P->smoother = 0;  // euler.f90:376 calls psmoo_old
   P->ffreeze = 0.0;  // bc_far.f90's rule at every call
+  P->lfreeze = 0.0;  // the limiter at every stage
   return MGE_OK;
 }
 

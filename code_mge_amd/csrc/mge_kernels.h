@@ -263,8 +263,9 @@ __device__ inline T sqrt_pos(T x) {
 // ceflux with ss having only component DIR non-zero (value S)
 // (the quotients by rho as products with the refined reciprocal, which
 // agree with the divisions to rounding)
+// (h = v[4] + p, the total enthalpy per volume, formed by the caller)
 template <int DIR, typename T>
-__device__ inline void cflux(const T *v, T p, T S, T *fs) {
+__device__ inline void cflux_h(const T *v, T h, T p, T S, T *fs) {
   const T rr = rcp_pos(v[0]);
   T f1 = v[1 + DIR] * S;
   fs[0] = f1;
@@ -272,7 +273,11 @@ __device__ inline void cflux(const T *v, T p, T S, T *fs) {
   fs[2] = f1 * v[2] * rr;
   fs[3] = f1 * v[3] * rr;
   fs[1 + DIR] = fs[1 + DIR] + p * S;
-  fs[4] = f1 * (v[4] + p) * rr;
+  fs[4] = f1 * h * rr;
+}
+template <int DIR, typename T>
+__device__ inline void cflux(const T *v, T p, T S, T *fs) {
+  cflux_h<DIR>(v, v[4] + p, p, S, fs);
 }
 
 // ceflux of a stored state: the reference form for FP64; for FP32 the flux
@@ -281,8 +286,10 @@ __device__ inline void cflux(const T *v, T p, T S, T *fs) {
 // rho' m_inf,c f1_inf) / (rho rho_inf) + p' S delta_c,DIR.  F_inf is the
 // same for every cell of a grid line (S does not vary along it), so face
 // differences of F' equal those of F.
+// The _h form takes h = v[4] + p (stored form) from the caller; v[4] itself
+// is not read.
 template <int DIR, typename T>
-__device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
+__device__ inline void cflux_st_h(const Inf<T> &f, const T *v, T h, T p, T S, T *fs) {
   if constexpr (kDelta<T>) {
     const T r0 = f.w[0];
     T f1i = f.w[1 + DIR] * S, f1d = v[1 + DIR] * S;
@@ -294,11 +301,15 @@ __device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
       fs[1 + c] = (r0 * (mi * f1d + md * f1i + md * f1d) - v[0] * mi * f1i) / den;
     }
     fs[1 + DIR] = fs[1 + DIR] + p * S;
-    T hi = f.w[4] + f.p, hd = v[4] + p;
+    T hi = f.w[4] + f.p, hd = h;
     fs[4] = (r0 * (hi * f1d + hd * f1i + hd * f1d) - v[0] * hi * f1i) / den;
   } else {
-    cflux<DIR>(v, p, S, fs);
+    cflux_h<DIR>(v, h, p, S, fs);
   }
+}
+template <int DIR, typename T>
+__device__ inline void cflux_st(const Inf<T> &f, const T *v, T p, T S, T *fs) {
+  cflux_st_h<DIR>(f, v, v[4] + p, p, S, fs);
 }
 
 // general ceflux (full face vector), used by the wall records
@@ -388,12 +399,10 @@ __device__ inline int dnl(const Dims &d) {
 // Branch-free: the loads use an index clamped into [1, ie] (always inside
 // the array) and the boundary convention is applied by a select, so the
 // compiler can batch the loads of neighbouring calls.
+// the switch value from the pressures at s-1, s, s+1 (the reference sums
+// the j line in the opposite order)
 template <int DIR, typename T>
-__device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
-  int nb = dne<DIR>(c.d) + 1;
-  int sc = s < 1 ? 1 : (s > nb - 1 ? nb - 1 : s);
-  long st = dstep<DIR>(c.d);
-  T pm = c.s.p[base + (sc - 1) * st], pc = c.s.p[base + sc * st], pp = c.s.p[base + (sc + 1) * st];
+__device__ inline T fp_val(const Inf<T> &inf, T pm, T pc, T pp) {
   T pb, num;
   if (DIR == 1) {
     pb = pp + T(2) * pc + pm;
@@ -402,14 +411,22 @@ __device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
     pb = pm + T(2) * pc + pp;
     num = fabs(pm - T(2) * pc + pp);
   }
-  if constexpr (kDelta<T>) pb = pb + T(4) * c.inf.p;
+  if constexpr (kDelta<T>) pb = pb + T(4) * inf.p;
   pb = fabs(pb);
   // FP64: times the refined reciprocal (pb = 0, never for physical
   // pressures, is selected away); FP32 divides
   T v;
   if constexpr (kDelta<T>) v = num / pb;
   else v = num * rcp_pos(pb);
-  v = fabs(pb) > T(0) ? v : T(0);
+  return fabs(pb) > T(0) ? v : T(0);
+}
+template <int DIR, typename T>
+__device__ inline T fp_at(const FluxCtx<T> &c, long base, int s) {
+  int nb = dne<DIR>(c.d) + 1;
+  int sc = s < 1 ? 1 : (s > nb - 1 ? nb - 1 : s);
+  long st = dstep<DIR>(c.d);
+  T pm = c.s.p[base + (sc - 1) * st], pc = c.s.p[base + sc * st], pp = c.s.p[base + (sc + 1) * st];
+  T v = fp_val<DIR>(c.inf, pm, pc, pp);
   return (s <= 0 || s >= nb) ? T(0) : v;
 }
 

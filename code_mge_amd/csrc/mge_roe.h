@@ -131,6 +131,59 @@ __device__ inline void muscl(int order, const T *qm2, const T *qm1, const T *qp0
   }
 }
 
+// Limiter freezing (lfreeze > 0, second order).  For cell c, direction dir
+// and component n the limiter value phi = minmod(a,b) / (0.5 (a+b)) of its
+// backward and forward differences a, b (0 when they differ in sign or one
+// is zero, else in (0,1]) is kept as the byte round(254 phi) at byte n of
+// the cell's word of that direction (lim[dir*E + q]); 255 means not
+// recorded (ghost cells, which keep the minmod itself).  Until the freeze
+// each stage records the values of its cells; after it a cell's slope is
+// (byte/254) * 0.5 (a+b): a fixed blend of first order and Fromm's central
+// slope, linear in the state and always dissipative, so the iteration
+// converges instead of cycling through limiter switches.  The wall rule (no
+// slope reaching a solid cell) applies in both modes.
+template <typename T>
+__device__ inline T lim_frozen(T back, T fwd, unsigned byte) {
+  const T k = T(0.5 / 254.0);
+  return byte == 255u ? minmod(back, fwd) : (T(byte) * k) * (back + fwd);
+}
+template <typename T>
+__device__ inline unsigned lim_byte(T m, T back, T fwd) {
+  const T phi = back * fwd > T(0) ? m / (T(0.5) * (back + fwd)) : T(0);
+  return (unsigned)(T(254) * phi + T(0.5));
+}
+
+// muscl with the limiter mode lm (1 record: *rec gets the word of cell a;
+// 2 frozen: the slopes of cells a-1 and a from their words wl, wr)
+template <typename T>
+__device__ inline void muscl_lim(int lm, const T *qm2, const T *qm1, const T *qp0, const T *qp1, bool fm2, bool fp1,
+                                 unsigned long long wl, unsigned long long wr, T *qL, T *qR,
+                                 unsigned long long *rec) {
+  unsigned long long word = ~0ull;
+#pragma unroll
+  for (int n = 0; n < 5; n++) {
+    const T a1 = qm1[n] - qm2[n], b1 = qp0[n] - qm1[n], b2 = qp1[n] - qp0[n];
+    T sl, sr;
+    if (lm == 2) {
+      sl = lim_frozen(a1, b1, (unsigned)(wl >> (8 * n)) & 255u);
+      sr = lim_frozen(b1, b2, (unsigned)(wr >> (8 * n)) & 255u);
+    } else {
+      sl = minmod(a1, b1);
+      sr = minmod(b1, b2);
+      word &= ~(0xffull << (8 * n));
+      word |= (unsigned long long)lim_byte(sr, b1, b2) << (8 * n);
+    }
+    sl = fm2 ? sl : T(0);
+    sr = fp1 ? sr : T(0);
+    {
+#pragma clang fp contract(fast)
+      qL[n] = qm1[n] + T(0.5) * sl;
+      qR[n] = qp0[n] - T(0.5) * sr;
+    }
+  }
+  *rec = word;
+}
+
 // one Roe face whose face vector has the single non-zero component S in
 // direction DIR (axis-aligned blocks); e central part, d upwind part (only
 // when diss).  |S| and S/|S| are formed as fabs(S) and sign(S), which equal
@@ -285,21 +338,24 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 // converted to primitive variables in LDS; every i and j face of the tile
 // is evaluated once (each cell its low face, plus the faces at the tile's
 // high edges) into LDS; k faces are carried from plane to plane in
-// registers with the primitives of the cell column k-1..k+2.  FP64 uses an
-// 8-row tile (512 threads, 121 KB LDS, one workgroup per CU: the two waves
-// per SIMD its registers allow anyway, with half the halo per cell); FP32
-// keeps 4 rows (8 rows would need more than the 128 registers of its four
-// waves per SIMD).
+// registers with the primitives of the cell column k-1..k+2.  By default
+// FP64 uses an 8-row tile (512 threads, 121 KB LDS, one workgroup per CU:
+// the two waves per SIMD its registers allow anyway, with half the halo per
+// cell) and FP32 4 rows (four waves per SIMD fit its registers only at 256
+// threads per workgroup); both row counts are built for both precisions and
+// the autotuner may pick the other one per level shape.
 template <typename T>
 struct RoeTile {
   static constexpr int TJ = kDelta<T> ? 4 : 8;
   static constexpr int MINW = 2;   // waves per SIMD the launch bounds ask for
 };
 
-template <typename T, int TJ_>
+template <typename T, int TJ_, bool LIM>
 struct RoeSmem {
   T q[5][TJ_ + 4][TI + 4];
   uint8_t fl[TJ_ + 4][TI + 4];
+  // limiter words of i and j (frozen limiter; one placeholder word without)
+  unsigned long long lw[LIM ? 2 : 1][LIM ? TJ_ + 4 : 1][LIM ? TI + 4 : 1];
   T xe[5][TJ_][TI + 1], xd[5][TJ_][TI + 1];
   T ye[5][TJ_ + 1][TI], yd[5][TJ_ + 1][TI];
 };
@@ -312,9 +368,12 @@ __device__ inline void roe_load(const FluxCtx<T> &c, const uint8_t *flags, long 
 
 // the face between two cells of the LDS tile at (row, col-1) | (row, col)
 // for DIR 0, or (row-1, col) | (row, col) for DIR 1
-template <int DIR, typename T, int TJ_>
-__device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem<T, TJ_> &sm, const StageArgs<T> &A, int row, int col, T S, T *e,
-                                    T *dd) {
+// lm: limiter mode (0 none, 1 record into *rec, 2 frozen); a face that is
+// not evaluated records "not recorded" (its slopes are never used: a cell
+// next to a solid one takes no slope toward it)
+template <int DIR, typename T, int TJ_, bool LIM>
+__device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem<T, TJ_, LIM> &sm, const StageArgs<T> &A, int row, int col, T S, T *e,
+                                    T *dd, int lm = 0, unsigned long long *rec = nullptr) {
   const bool diss = A.has_diss;
   bool f[4];
   T v[4][5];
@@ -325,6 +384,7 @@ __device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem
 #pragma unroll
     for (int n = 0; n < 5; n++) v[t][n] = sm.q[n][r][cc];
   }
+  if (rec) *rec = ~0ull;
   if (!(f[1] && f[2])) {
 #pragma unroll
     for (int n = 0; n < 5; n++) {
@@ -334,7 +394,15 @@ __device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem
     return;
   }
   T qL[5], qR[5];
-  muscl(A.order, v[0], v[1], v[2], v[3], f[0], f[3], qL, qR);
+  if (!LIM || lm == 0) {
+    muscl(A.order, v[0], v[1], v[2], v[3], f[0], f[3], qL, qR);
+  } else {
+    const unsigned long long wl = lm == 2 ? sm.lw[DIR][DIR == 1 ? row - 1 : row][DIR == 0 ? col - 1 : col] : 0ull;
+    const unsigned long long wr = lm == 2 ? sm.lw[DIR][row][col] : 0ull;
+    unsigned long long word;
+    muscl_lim(lm, v[0], v[1], v[2], v[3], f[0], f[3], wl, wr, qL, qR, &word);
+    if (rec) *rec = word;
+  }
   roe_face<DIR>(A.fx.gamma, A.g1, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
   if (!diss)
 #pragma unroll
@@ -342,8 +410,11 @@ __device__ __attribute__((always_inline)) inline void roe_lds_face(const RoeSmem
 }
 
 template <typename T>
-__device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs<T> &A, T (*zq)[5], const bool *zf, T S, T *e, T *dd) {
+__device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs<T> &A, T (*zq)[5], const bool *zf, T S, T *e, T *dd,
+                                                               int lm = 0, const unsigned long long *zw = nullptr,
+                                                               unsigned long long *rec = nullptr) {
   const bool diss = A.has_diss;
+  if (rec) *rec = ~0ull;
   if (!(zf[1] && zf[2])) {
 #pragma unroll
     for (int n = 0; n < 5; n++) {
@@ -353,17 +424,24 @@ __device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs
     return;
   }
   T qL[5], qR[5];
-  muscl(A.order, zq[0], zq[1], zq[2], zq[3], zf[0], zf[3], qL, qR);
+  if (lm == 0) {
+    muscl(A.order, zq[0], zq[1], zq[2], zq[3], zf[0], zf[3], qL, qR);
+  } else {
+    unsigned long long word;
+    muscl_lim(lm, zq[0], zq[1], zq[2], zq[3], zf[0], zf[3], lm == 2 ? zw[1] : 0ull, lm == 2 ? zw[2] : 0ull, qL, qR,
+              &word);
+    if (rec) *rec = word;
+  }
   roe_face<2>(A.fx.gamma, A.g1, A.eps, A.fx.cdis, diss, qL, qR, S, e, dd);
   if (!diss)
 #pragma unroll
     for (int n = 0; n < 5; n++) dd[n] = T(0);
 }
 
-template <typename T>
+// LIM: built with the limiter-freezing paths (A.lim set), else without
+template <typename T, int TJr, bool LIM>
 __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const StageArgs<T> &A, int bx, int by, int bz) {
-  constexpr int TJr = RoeTile<T>::TJ;
-  __shared__ RoeSmem<T, TJr> sm;
+  __shared__ RoeSmem<T, TJr, LIM> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
   const long E = d.E;
@@ -377,6 +455,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
   const bool inarr = i <= d.ib && j <= d.jb;
   static_assert(TJr >= 3, "the high-edge faces are spread over waves 1 and 2");
   const bool diss = A.has_diss;
+  const int lm = LIM ? (*A.lfrozen ? 2 : 1) : 0;   // limiter: none, record, frozen
   const T Sz = face_s<2>(c.g, min(i, d.ib), min(j, d.jb), 0);  // clamped: read only for valid cells
   // halo ring items of this thread (t = tid, tid + TI*TJr, ...): their LDS
   // slot and cell do not change with k.  FP64 loads the next plane's items
@@ -390,6 +469,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
   bool hok[NS] = {};
   T hraw[NS][5] = {};
   uint8_t hfl[NS] = {};
+  unsigned long long hlw[NS][2] = {};
   if constexpr (PRE) {
 #pragma unroll
     for (int h = 0; h < NS; h++) {
@@ -419,25 +499,36 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
         for (int n = 0; n < 4; n++) hraw[h][n] = c.s.w[n * E + q];
         hraw[h][4] = c.s.p[q];
         hfl[h] = A.flags[q];
+        if (lm == 2) {
+          hlw[h][0] = A.lim[q];
+          hlw[h][1] = A.lim[E + q];
+        }
       }
   }
-  // column carry: primitives of k-1..k+2 and the bottom face of plane k
+  // column carry: primitives of k-1..k+2 and the bottom face of plane k;
+  // with the limiter frozen also the cells' limiter words, and while
+  // recording the k codes of cell k (from its bottom face)
   T zq[4][5] = {}, Ez[5], Dz[5];
   bool zf[4] = {false, false, false, false};
+  unsigned long long zw[4] = {~0ull, ~0ull, ~0ull, ~0ull}, zbits = ~0ull;
   if (valid) {
     const long bz = ix(d, i, j, 0);
-    roe_load(c, A.flags, bz + (long)(kb0 - 2) * d.sz, zq[0], &zf[0]);
-    roe_load(c, A.flags, bz + (long)(kb0 - 1) * d.sz, zq[1], &zf[1]);
-    roe_load(c, A.flags, bz + (long)kb0 * d.sz, zq[2], &zf[2]);
-    roe_load(c, A.flags, bz + (long)(kb0 + 1) * d.sz, zq[3], &zf[3]);
-    roe_z_face(A, zq, zf, Sz, Ez, Dz);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const long q = bz + (long)(kb0 - 2 + t) * d.sz;
+      roe_load(c, A.flags, q, zq[t], &zf[t]);
+      if (lm == 2) zw[t] = A.lim[2 * E + q];
+    }
+    roe_z_face(A, zq, zf, Sz, Ez, Dz, lm, zw, &zbits);
 #pragma unroll
     for (int t = 0; t < 3; t++) {
 #pragma unroll
       for (int n = 0; n < 5; n++) zq[t][n] = zq[t + 1][n];
       zf[t] = zf[t + 1];
+      zw[t] = zw[t + 1];
     }
     roe_load(c, A.flags, bz + (long)(kb0 + 2) * d.sz, zq[3], &zf[3]);
+    if (lm == 2) zw[3] = A.lim[2 * E + bz + (long)(kb0 + 2) * d.sz];
   }
   for (int k = kb0; k <= kb1; k++) {
     // ---- FP64: the finish inputs of this plane's cell, loaded first so
@@ -479,6 +570,11 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 #pragma unroll
       for (int n = 0; n < 5; n++) sm.q[n][ty + 2][tx + 2] = o[n];
       sm.fl[ty + 2][tx + 2] = f;
+      if constexpr (LIM) if (lm == 2) {
+        const long q = ix(d, i, j, k);
+        sm.lw[0][ty + 2][tx + 2] = A.lim[q];
+        sm.lw[1][ty + 2][tx + 2] = A.lim[E + q];
+      }
     }
     if constexpr (PRE) {
 #pragma unroll
@@ -489,6 +585,10 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 #pragma unroll
           for (int n = 0; n < 5; n++) sm.q[n][hr[h]][hc[h]] = o[n];
           sm.fl[hr[h]][hc[h]] = (hfl[h] & F_FLUID) != 0;
+          if constexpr (LIM) if (lm == 2) {
+            sm.lw[0][hr[h]][hc[h]] = hlw[h][0];
+            sm.lw[1][hr[h]][hc[h]] = hlw[h][1];
+          }
         }
     } else {
       for (int t = tid; t < NH; t += TI * TJr) {
@@ -511,6 +611,11 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 #pragma unroll
           for (int n = 0; n < 5; n++) sm.q[n][r][col] = o[n];
           sm.fl[r][col] = f;
+          if constexpr (LIM) if (lm == 2) {
+            const long q = ix(d, ii, jj, k);
+            sm.lw[0][r][col] = A.lim[q];
+            sm.lw[1][r][col] = A.lim[E + q];
+          }
         }
       }
     }
@@ -525,13 +630,18 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           for (int n = 0; n < 4; n++) hraw[h][n] = c.s.w[n * E + q];
           hraw[h][4] = c.s.p[q];
           hfl[h] = A.flags[q];
+          if (lm == 2) {
+            hlw[h][0] = A.lim[q];
+            hlw[h][1] = A.lim[E + q];
+          }
         }
     }
     // ---- faces: own i face and j face, then the tile's high-edge faces
+    unsigned long long rx = ~0ull, ry = ~0ull;   // the i and j words of the own cell (recording)
     {
       T e[5], dd[5];
       if (j <= d.jl && i <= d.ie) {
-        roe_lds_face<0>(sm, A, ty + 2, tx + 2, face_s<0>(c.g, i, j, k), e, dd);
+        roe_lds_face<0>(sm, A, ty + 2, tx + 2, face_s<0>(c.g, i, j, k), e, dd, lm, &rx);
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           sm.xe[n][ty][tx] = e[n];
@@ -539,7 +649,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
         }
       }
       if (i <= d.il && j <= d.je) {
-        roe_lds_face<1>(sm, A, ty + 2, tx + 2, face_s<1>(c.g, i, j, k), e, dd);
+        roe_lds_face<1>(sm, A, ty + 2, tx + 2, face_s<1>(c.g, i, j, k), e, dd, lm, &ry);
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           sm.ye[n][ty][tx] = e[n];
@@ -551,7 +661,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       if (tid >= 2 * TI && tid < 2 * TI + TJr) {
         int jj = j0 + tid - 2 * TI, a = i0 + TI;
         if (jj <= d.jl && a <= d.ie) {
-          roe_lds_face<0>(sm, A, tid - 2 * TI + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd);
+          roe_lds_face<0>(sm, A, tid - 2 * TI + 2, TI + 2, face_s<0>(c.g, a, jj, k), e, dd, lm);
 #pragma unroll
           for (int n = 0; n < 5; n++) {
             sm.xe[n][tid - 2 * TI][TI] = e[n];
@@ -561,7 +671,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       } else if (tid >= TI && tid < 2 * TI) {
         int cidx = tid - TI, ii = i0 + cidx, a = j0 + TJr;
         if (ii <= d.il && a <= d.je) {
-          roe_lds_face<1>(sm, A, TJr + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd);
+          roe_lds_face<1>(sm, A, TJr + 2, cidx + 2, face_s<1>(c.g, ii, a, k), e, dd, lm);
 #pragma unroll
           for (int n = 0; n < 5; n++) {
             sm.ye[n][TJr][cidx] = e[n];
@@ -571,7 +681,14 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
       }
     }
     T Ez1[5], Dz1[5];
-    roe_z_face(A, zq, zf, Sz, Ez1, Dz1);  // a lane outside the block computes from zeros and stores nothing
+    unsigned long long zn = ~0ull;   // the k word of cell k+1 (recording)
+    roe_z_face(A, zq, zf, Sz, Ez1, Dz1, lm, zw, &zn);  // a lane outside the block computes from zeros and stores nothing
+    if (lm == 1 && valid) {
+      A.lim[qk] = rx;
+      A.lim[E + qk] = ry;
+      A.lim[2 * E + qk] = zbits;
+    }
+    zbits = zn;
     __syncthreads();
     // ---- accumulate and finish
     if (valid) {
@@ -622,29 +739,31 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 #pragma unroll
           for (int n = 0; n < 5; n++) zq[t][n] = zq[t + 1][n];
           zf[t] = zf[t + 1];
+          zw[t] = zw[t + 1];
         }
         roe_load(c, A.flags, ix(d, i, j, k + 3), zq[3], &zf[3]);
+        if (lm == 2) zw[3] = A.lim[2 * E + ix(d, i, j, k + 3)];
       }
     }
   }
 }
 // entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
-// the single-case entry), with launch bounds that follow the precision's tile
-template <typename T>
-__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW) k_stage_roe(StageArgs<T> A) {
+// the single-case entry), with launch bounds that follow the tile's rows
+template <typename T, int TJ_, bool LIM>
+__global__ void __launch_bounds__(TI * TJ_, RoeTile<T>::MINW) k_stage_roe(StageArgs<T> A) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
-  k_stage_roe_body(A, bx, by, bz);
+  k_stage_roe_body<T, TJ_, LIM>(A, bx, by, bz);
 }
-template <typename T>
-__global__ void __launch_bounds__(TI * RoeTile<T>::TJ, RoeTile<T>::MINW)
+template <typename T, int TJ_, bool LIM>
+__global__ void __launch_bounds__(TI * TJ_, RoeTile<T>::MINW)
     k_stage_roe_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
   const int b = (int)blockIdx.x;
   const int c = batch_case(cg, nc, b);
   const int4 g = cg[c];
   if (!act[g.w]) return;
   const int l = b - g.x;
-  k_stage_roe_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+  k_stage_roe_body<T, TJ_, LIM>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
 }
 
 }  // namespace mge

@@ -69,6 +69,9 @@ struct StageArgs {
   int vt;
   int order;                // Roe schemes: 1 first order, 2 MUSCL-minmod
   T eps;                    // Roe entropy-fix coefficient
+  unsigned long long *lim;  // Roe second order with limiter freezing: limiter words per direction and cell
+                            // (lim[dir*E + q], byte n = component n; else nullptr)
+  const int *lfrozen;       // the case's limiter-freeze flag
   T g1;                     // gamma/(gamma-1)
   int kc;                   // k planes per workgroup (LDS-tiled kernels)
   int fw_in;                // fw already holds this stage's blend and viscous part
@@ -254,20 +257,28 @@ MGE_ENTRY_T(k_stage, StageArgs, 256, 1)
 // ------------------------------------------------------------------ k_stage2
 //
 // LDS-tiled form of k_stage.  A workgroup owns a TI x TJ column tile of
-// (i,j) and marches through a chunk of k planes.  Per plane the i- and
-// j-direction quantities every face needs -- the convective cell fluxes
-// F (ceflux.f90), the pressure switches fp and the enthalpy differences diw
-// (dflux.f90:70-86) -- are computed once per cell into LDS (tile + halo);
-// each thread then forms its faces from LDS.  The k-direction face values
-// are carried from plane to plane in registers, so every k face is
-// evaluated once.  Each face value is computed with the same expression
-// tree as the reference, so results equal k_stage's bit for bit.
+// (i,j) and marches through a chunk of k planes.  Per plane the raw state of
+// the tile and its halo -- w with slot 4 replaced by w4 + p (the enthalpy
+// term of the flux and of the dissipation differences), and p -- is staged
+// in LDS once: a cross of two-cell arms for w and three-cell arms for p (the
+// pressure switch at the tile's second halo cell reads one cell further).
+// From it the i- and j-direction quantities every face needs -- the
+// convective cell fluxes F (ceflux.f90) and the pressure switches fp
+// (dflux.f90:70-81) -- are computed once per cell into LDS, and each thread
+// forms the enthalpy differences diw (dflux.f90:82-86) of its faces from the
+// staged state.  In k the raw state of the cell's own column k, k+1 (and p
+// at k+2, k+3) is carried in registers, so each plane loads one new column
+// value per cell;
+// the k face values are carried as well, so every k face is evaluated once.
+// Each face value is computed with the same expression tree as the
+// reference, so results equal k_stage's bit for bit.
 constexpr int TI = 64, TJ = 4;
 
-// rows of the JST tile: FP64 uses 8 (512 threads, 107 KB LDS, one workgroup
-// per CU -- the same two waves per SIMD its registers allow anyway, with half
-// the y-halo work per cell); FP32 keeps 4 rows so that three waves per SIMD
-// stay resident
+// rows of the JST tile by default: FP64 uses 8 (512 threads, about 100 KB
+// LDS, one workgroup per CU -- the two waves per SIMD its registers allow
+// anyway, with half the y-halo work per cell); FP32 keeps 4 rows.  Both row
+// counts (4 and 8) are built for both precisions; the autotuner may pick the
+// other one per level shape.
 template <typename T>
 struct Stage2Tile {
   static constexpr int TJ = kDelta<T> ? 4 : 8;
@@ -276,12 +287,15 @@ struct Stage2Tile {
 
 template <typename T, int TJ_>
 struct StageSmem {
-  T fx[5][TJ_][TI + 2];   // F_x at i = i0-1 .. i0+TI
-  T fy[5][TJ_ + 2][TI];   // F_y at j = j0-1 .. j0+TJ
-  T px[TJ_][TI + 4];      // fp_x at i = i0-2 .. i0+TI+1
-  T py[TJ_ + 4][TI];      // fp_y at j = j0-2 .. j0+TJ+1
-  T dx[5][TJ_][TI + 3];   // diw_x at i = i0-1 .. i0+TI+1
-  T dy[5][TJ_ + 3][TI];   // diw_y at j = j0-1 .. j0+TJ+1
+  T rw[5][TJ_ + 4][TI + 4];   // w, slot 4 = w4 + p, at i0-2+col, j0-2+row
+  T rp[TJ_ + 6][TI + 6];      // p at i0-3+col, j0-3+row
+  T fx[5][TJ_][TI + 2];       // F_x at i = i0-1 .. i0+TI
+  T fy[5][TJ_ + 2][TI];       // F_y at j = j0-1 .. j0+TJ
+  T px[TJ_][TI + 4];          // fp_x at i = i0-2 .. i0+TI+1
+  T py[TJ_ + 4][TI];          // fp_y at j = j0-2 .. j0+TJ+1
+  T rd[TJ_ + 2][TI + 2];      // radius sums at i0-1+col, j0-1+row (line-clamped)
+  T gx[TI + 2];               // dx at i0-1 .. i0+TI
+  T gy[TJ_ + 2];              // dy at j0-1 .. j0+TJ
 };
 
 // JST face coefficients from the switches fp(a-2..a+1) and the radii sums
@@ -310,9 +324,109 @@ __device__ inline void cell_flux(const FluxCtx<T> &c, int i, int j, int k, T *o)
   cflux_st<DIR>(c.inf, v, c.s.p[q], face_s<DIR>(c.g, i, j, k), o);
 }
 
+// raw state of one cell: w0..w3, w4 + p, p
 template <typename T>
+__device__ inline void raw_load(const FluxCtx<T> &c, long q, T *o) {
+  const long E = c.d.E;
+#pragma unroll
+  for (int n = 0; n < 5; n++) o[n] = c.s.w[n * E + q];
+  o[5] = c.s.p[q];
+  o[4] = o[4] + o[5];
+}
+
+// convective flux of a staged cell (LDS row r, column col of rw / rp)
+template <int DIR, typename T, int TJ_>
+__device__ inline void lds_flux(const FluxCtx<T> &c, const StageSmem<T, TJ_> &sm, int r, int col, T S, T *o) {
+  T v[5];
+#pragma unroll
+  for (int n = 0; n < 5; n++) v[n] = sm.rw[n][r][col];
+  cflux_st_h<DIR>(c.inf, v, v[4], sm.rp[r + 1][col + 1], S, o);
+}
+
+// pressure switch fp(s) of the tile's i line (DIR 0, rp row r) or j line
+// (DIR 1, rp column r) from the staged pressures, with fp_at's end
+// convention; s0 is the line index of rp's first entry
+template <int DIR, typename T, int TJ_>
+__device__ inline T lds_fp(const FluxCtx<T> &c, const StageSmem<T, TJ_> &sm, int r, int s, int s0) {
+  const int nb = dne<DIR>(c.d) + 1;
+  const int sc = (s < 1 ? 1 : (s > nb - 1 ? nb - 1 : s)) - s0;
+  T pm, pc, pp;
+  if (DIR == 0) {
+    pm = sm.rp[r][sc - 1]; pc = sm.rp[r][sc]; pp = sm.rp[r][sc + 1];
+  } else {
+    pm = sm.rp[sc - 1][r]; pc = sm.rp[sc][r]; pp = sm.rp[sc + 1][r];
+  }
+  T v = fp_val<DIR>(c.inf, pm, pc, pp);
+  return (s <= 0 || s >= nb) ? T(0) : v;
+}
+
+// one halo item of the raw staging (t < 6 TJ + 6 TI): x arms (6 columns of
+// the tile's rows), then y arms (6 rows of the tile's columns), at rp
+// positions (row r, column col); the outermost column / row of each arm is
+// needed for p only (w = false) and, like the rest, only where it lies
+// inside the arrays
+struct HaloItem {
+  int r, col;
+  long q;   // cell index in plane 0
+  bool ok, w;
+};
+template <int TJ_>
+__device__ inline HaloItem halo_item(const Dims &d, int t, int i0, int j0, bool diss) {
+  HaloItem h;
+  if (t < 6 * TJ_) {
+    const int a = t / TJ_;
+    h.r = 3 + t % TJ_;
+    h.col = a < 3 ? a : TI + a;
+    h.w = h.col != 0 && h.col != TI + 5;
+  } else {
+    const int u = t - 6 * TJ_, a = u / TI;
+    h.col = 3 + u % TI;
+    h.r = a < 3 ? a : TJ_ + a;
+    h.w = h.r != 0 && h.r != TJ_ + 5;
+  }
+  const int ii = i0 - 3 + h.col, jj = j0 - 3 + h.r;
+  h.ok = t < 6 * TJ_ + 6 * TI && ii >= 0 && jj >= 0 && ii <= d.ib && jj <= d.jb && (h.w || diss);
+  h.q = h.ok ? ix(d, ii, jj, 0) : 0;
+  return h;
+}
+// one arm item of the radius-sum staging (t < 2 TJ + 2 TI): x arms at
+// i0-1 and i0+TI (clamped into the interior along i), y arms at j0-1 and
+// j0+TJ (clamped along j), at rd position (r, col)
+struct RadItem {
+  int r, col;
+  long q;
+  bool ok;
+};
+template <int TJ_>
+__device__ inline RadItem rad_item(const Dims &d, int t, int i0, int j0, bool diss) {
+  RadItem h;
+  int ii, jj;
+  if (t < 2 * TJ_) {
+    h.r = 1 + t % TJ_;
+    h.col = t / TJ_ ? TI + 1 : 0;
+    ii = max(2, min(i0 - 1 + h.col, d.il));
+    jj = j0 - 1 + h.r;
+    h.ok = jj <= d.jl;
+  } else {
+    const int u = t - 2 * TJ_;
+    h.col = 1 + u % TI;
+    h.r = u / TI ? TJ_ + 1 : 0;
+    ii = i0 - 1 + h.col;
+    jj = max(2, min(j0 - 1 + h.r, d.jl));
+    h.ok = ii <= d.il;
+  }
+  h.ok = h.ok && diss && t < 2 * TJ_ + 2 * TI;
+  h.q = h.ok ? ix(d, ii, jj, 0) : 0;
+  return h;
+}
+
+template <typename T, int TJs>
 __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageArgs<T> &A, int gbx, int gby, int gbz) {
-  constexpr int TJs = Stage2Tile<T>::TJ;
+  constexpr int NT = TI * TJs;
+  // FP64 loads the next plane's halo items during the current plane's
+  // faces (two waves per SIMD hide little latency); FP32 loads them in place
+  constexpr bool PRE = !kDelta<T>;
+  constexpr bool HPRE = PRE;
   __shared__ StageSmem<T, TJs> sm;
   const FluxCtx<T> &c = A.fx;
   const Dims &d = c.d;
@@ -325,180 +439,247 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
   const int kb1 = min(kb0 + kc - 1, d.kl);
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = i <= d.il && j <= d.jl;
+  const bool inarr = i <= d.ib && j <= d.jb;
   const bool diss = A.has_diss;
-  const long bz = ix(d, i, j, 0);
-  // k-direction carry (face k) and rings for face k+1
-  T Dk[5], Ek[5], fz[4], dzr[3][5], rz[2], Fz[2][5];
+  const long bz = inarr ? ix(d, i, j, 0) : 0;
+  const long sz = d.sz;
+
+  // ---- geometry of the tile's lines
+  for (int t = tid; t < TI + 2 + TJs + 2; t += NT) {
+    if (t < TI + 2) {
+      const int ii = i0 - 1 + t;
+      if (ii <= d.ib) sm.gx[t] = c.g.dx[ii];
+    } else {
+      const int jj = j0 - 1 + (t - TI - 2);
+      if (jj <= d.jb) sm.gy[t - TI - 2] = c.g.dy[jj];
+    }
+  }
+  const T Sz = valid ? face_s<2>(c.g, i, j, 0) : T(0);
+
+  // ---- staging items of this thread.  Their indices are rederived from
+  // an opaque copy of the thread index in every plane: held across the
+  // plane loop, the item addresses alone would take dozens of registers
+  constexpr int NH = 6 * TJs + 6 * TI;
+  constexpr int NS = (NH + NT - 1) / NT;
+  const bool rown = diss && i <= d.ie && j <= d.je && !(i > d.il && j > d.jl);
+  const long rq_own = rown ? ix(d, min(i, d.il), min(j, d.jl), 0) : 0;
+  T hraw[NS][6], rdo = T(0), rdh = T(0), dzk = T(0);
+  auto load_items = [&](int k) {
+    int t0 = tid;
+    asm volatile("" : "+v"(t0));
+#pragma unroll
+    for (int h = 0; h < NS; h++) {
+      const HaloItem it = halo_item<TJs>(d, t0 + h * NT, i0, j0, diss);
+      if (it.ok) {
+        const long q = it.q + (long)k * sz;
+        if (it.w) {
+          raw_load(c, q, hraw[h]);
+        } else {
+          hraw[h][5] = c.s.p[q];
+        }
+      }
+    }
+    const RadItem ri = rad_item<TJs>(d, t0, i0, j0, diss);
+    if (rown) rdo = c.radsum[rq_own + (long)k * sz];
+    if (ri.ok) rdh = c.radsum[ri.q + (long)k * sz];
+    dzk = c.g.dz[k];
+  };
+  if constexpr (HPRE) load_items(kb0);
+
+  // ---- k-direction: raw state of the own column at k and k+1, p at k+2
+  // and k+3 (cells inside the arrays); face k values, q(k-1) and the rings
+  // for face k+1 (valid cells)
+  T R0[6] = {}, R1[6] = {}, P2 = T(0), P3 = T(0);
+  T Dk[5], Ek[5], fz[4], rz[2], Fz0[5], Qm[5];
+  if (inarr) {
+    raw_load(c, bz + (long)kb0 * sz, R0);
+    raw_load(c, bz + (long)(kb0 + 1) * sz, R1);
+    P2 = c.s.p[bz + (long)(kb0 + 2) * sz];
+    P3 = c.s.p[bz + (long)min(kb0 + 3, d.kb) * sz];
+  }
   if (valid) {
-    int k = kb0;
+    const int k = kb0;
     eface<2>(c, bz, k, i, j, k, Ek);
-    cell_flux<2>(c, i, j, k, Fz[0]);
-    cell_flux<2>(c, i, j, k + 1, Fz[1]);
+    cflux_st_h<2>(c.inf, R0, R0[4], R0[5], Sz, Fz0);
     if (diss) {
+      T t6[6];
+      raw_load(c, bz + (long)(k - 1) * sz, t6);
+#pragma unroll
+      for (int n = 0; n < 5; n++) Qm[n] = t6[n];
       dface<2>(c, bz, k, i, j, k, Dk);
       for (int t = 0; t < 4; t++) fz[t] = fp_at<2>(c, bz, k - 1 + t);
-      for (int t = 0; t < 3; t++) diw_at<2>(c, bz, k + t, dzr[t]);
       rz[0] = rad_at<2>(c, bz, k);
       rz[1] = rad_at<2>(c, bz, k + 1);
     }
   }
   for (int k = kb0; k <= kb1; k++) {
-    // ---- FP64: the finish inputs of this plane's cell, loaded first so
-    // that their latency overlaps the plane's fill and faces (two waves per
-    // SIMD hide little); FP32 keeps its registers for occupancy
-    constexpr bool PRE = !kDelta<T>;
-    const long qk = ix(d, i, j, k);
-    T pfw[5] = {}, pwn[5] = {}, pvd = T(0);
+    const long qk = bz + (long)k * sz;
+    const bool more = k < kb1;
+    // ---- the finish inputs of this plane's cell and the next column
+    // values, loaded first so that their latency overlaps the plane's
+    // staging and faces
+    T pfw[5] = {}, pwn[5] = {}, pvd = T(0), W2[5] = {}, P4 = T(0), rz2 = T(0);
     uint8_t pfl = 0;
-    if (PRE && valid) {
+    if (valid) {
       if (A.fw_in || A.m > 1) {
 #pragma unroll
         for (int n = 0; n < 5; n++) pfw[n] = A.fw[n * E + qk];
       }
-      if (!A.lres && !A.smoothed) {
+      if (PRE && !A.lres && !A.smoothed) {
 #pragma unroll
         for (int n = 0; n < 5; n++) pwn[n] = A.wn[n * E + qk];
       }
-      pvd = A.vdtim[qk];
+      if (PRE) pvd = A.vdtim[qk];
       pfl = A.flags[qk];
     }
-    // the radii sums of the dissipation faces: i-1, i, i+1 and j-1, j+1
-    T prad[5] = {};
-    if (PRE && valid && diss) {
-      const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
-      prad[0] = rad_at<0>(c, brow, i - 1);
-      prad[1] = rad_at<0>(c, brow, i);
-      prad[2] = rad_at<0>(c, brow, i + 1);
-      prad[3] = rad_at<1>(c, bcol, j - 1);
-      prad[4] = rad_at<1>(c, bcol, j + 1);
+    if (inarr) {
+      const long q2 = bz + (long)(k + 2) * sz;   // k + 2 <= kl + 2 = kb
+#pragma unroll
+      for (int n = 0; n < 5; n++) W2[n] = c.s.w[n * E + q2];
+      if (more) P4 = c.s.p[bz + (long)min(k + 4, d.kb) * sz];
+      if (valid && diss && more) rz2 = rad_at<2>(c, bz, k + 2);
     }
-    // ---- fill the plane's i/j quantities (cells within the arrays)
-    {
-      // own cell
-      if (i <= d.ib && j <= d.jb) {
-        long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
-        T o[5];
-        cell_flux<0>(c, i, j, k, o);
-        for (int n = 0; n < 5; n++) sm.fx[n][ty][tx + 1] = o[n];
-        cell_flux<1>(c, i, j, k, o);
-        for (int n = 0; n < 5; n++) sm.fy[n][ty + 1][tx] = o[n];
-        if (diss) {
-          sm.px[ty][tx + 2] = fp_at<0>(c, brow, i);
-          sm.py[ty + 2][tx] = fp_at<1>(c, bcol, j);
-          diw_at<0>(c, brow, i, o);
-          for (int n = 0; n < 5; n++) sm.dx[n][ty][tx + 1] = o[n];
-          diw_at<1>(c, bcol, j, o);
-          for (int n = 0; n < 5; n++) sm.dy[n][ty + 1][tx] = o[n];
+    const T w24 = W2[4] + P2;   // slot 4 of q(k+2)
+    // ---- stage the plane: own cell from the column carry, halo items
+    if (!HPRE) load_items(k);
+    if (inarr) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) sm.rw[n][ty + 2][tx + 2] = R0[n];
+      sm.rp[ty + 3][tx + 3] = R0[5];
+    }
+    int tk = tid;
+    asm volatile("" : "+v"(tk));
+#pragma unroll
+    for (int h = 0; h < NS; h++) {
+      const HaloItem it = halo_item<TJs>(d, tk + h * NT, i0, j0, diss);
+      if (it.ok) {
+        if (it.w) {
+#pragma unroll
+          for (int n = 0; n < 5; n++) sm.rw[n][it.r - 1][it.col - 1] = hraw[h][n];
         }
+        sm.rp[it.r][it.col] = hraw[h][5];
       }
-      // halo items
+    }
+    if (rown) sm.rd[ty + 1][tx + 1] = rdo;
+    {
+      const RadItem ri = rad_item<TJs>(d, tk, i0, j0, diss);
+      if (ri.ok) sm.rd[ri.r][ri.col] = rdh;
+    }
+    const T dz = dzk;
+    if constexpr (HPRE) if (more) load_items(k + 1);
+    __syncthreads();
+    // ---- fill the plane's F and fp (cells within the arrays)
+    if (inarr) {
+      T o[5];
+      lds_flux<0>(c, sm, ty + 2, tx + 2, -(sm.gy[ty + 1] * dz), o);
+#pragma unroll
+      for (int n = 0; n < 5; n++) sm.fx[n][ty][tx + 1] = o[n];
+      lds_flux<1>(c, sm, ty + 2, tx + 2, -(sm.gx[tx + 1] * dz), o);
+#pragma unroll
+      for (int n = 0; n < 5; n++) sm.fy[n][ty + 1][tx] = o[n];
+      if (diss) {
+        sm.px[ty][tx + 2] = lds_fp<0>(c, sm, ty + 3, i, i0 - 3);
+        sm.py[ty + 2][tx] = lds_fp<1>(c, sm, tx + 3, j, j0 - 3);
+      }
+    }
+    {
       const int nF = 2 * TJs + 2 * TI;
       const int nP = diss ? 4 * TJs + 4 * TI : 0;
-      const int nD = diss ? 3 * TJs + 3 * TI : 0;
-      for (int t = tid; t < nF + nP + nD; t += TI * TJs) {
+      for (int t = tk; t < nF + nP; t += NT) {
         T o[5];
         if (t < 2 * TJs) {  // F_x at i0-1, i0+TI
-          int r = t % TJs, side = t / TJs;
-          int ii = side ? i0 + TI : i0 - 1, jj = j0 + r, col = side ? TI + 1 : 0;
+          const int r = t % TJs, side = t / TJs;
+          const int ii = side ? i0 + TI : i0 - 1, jj = j0 + r, col = side ? TI + 1 : 0;
           if (ii <= d.ib && jj <= d.jb) {
-            cell_flux<0>(c, ii, jj, k, o);
+            lds_flux<0>(c, sm, r + 2, ii - i0 + 2, -(sm.gy[r + 1] * dz), o);
+#pragma unroll
             for (int n = 0; n < 5; n++) sm.fx[n][r][col] = o[n];
           }
         } else if (t < nF) {  // F_y at j0-1, j0+TJs
-          int u = t - 2 * TJs;
-          int col = u % TI, side = u / TI;
-          int ii = i0 + col, jj = side ? j0 + TJs : j0 - 1, row = side ? TJs + 1 : 0;
+          const int u = t - 2 * TJs;
+          const int cl = u % TI, side = u / TI;
+          const int ii = i0 + cl, jj = side ? j0 + TJs : j0 - 1, row = side ? TJs + 1 : 0;
           if (ii <= d.ib && jj <= d.jb) {
-            cell_flux<1>(c, ii, jj, k, o);
-            for (int n = 0; n < 5; n++) sm.fy[n][row][col] = o[n];
+            lds_flux<1>(c, sm, jj - j0 + 2, cl + 2, -(sm.gx[cl + 1] * dz), o);
+#pragma unroll
+            for (int n = 0; n < 5; n++) sm.fy[n][row][cl] = o[n];
           }
         } else if (t < nF + 4 * TJs) {  // fp_x at i0-2, i0-1, i0+TI, i0+TI+1
-          int u = t - nF;
-          int r = u % TJs, w4 = u / TJs;
-          int off = w4 < 2 ? w4 - 2 : TI + (w4 - 2);
-          int ii = i0 + off, jj = j0 + r, col = off + 2;
-          if (jj <= d.jb) sm.px[r][col] = (ii >= 0 && ii <= d.ib) ? fp_at<0>(c, ix(d, 0, jj, k), ii) : T(0);
-        } else if (t < nF + nP) {  // fp_y at j0-2, j0-1, j0+TJs, j0+TJs+1
-          int u = t - nF - 4 * TJs;
-          int col = u % TI, w4 = u / TI;
-          int off = w4 < 2 ? w4 - 2 : TJs + (w4 - 2);
-          int ii = i0 + col, jj = j0 + off, row = off + 2;
-          if (ii <= d.ib) sm.py[row][col] = (jj >= 0 && jj <= d.jb) ? fp_at<1>(c, ix(d, ii, 0, k), jj) : T(0);
-        } else if (t < nF + nP + 3 * TJs) {  // diw_x at i0-1, i0+TI, i0+TI+1
-          int u = t - nF - nP;
-          int r = u % TJs, w3 = u / TJs;
-          int off = w3 == 0 ? -1 : TI + (w3 - 1);
-          int ii = i0 + off, jj = j0 + r, col = off + 1;
-          if (jj <= d.jb && ii <= d.ib) {
-            diw_at<0>(c, ix(d, 0, jj, k), ii, o);
-            for (int n = 0; n < 5; n++) sm.dx[n][r][col] = o[n];
-          }
-        } else {  // diw_y at j0-1, j0+TJs, j0+TJs+1
-          int u = t - nF - nP - 3 * TJs;
-          int col = u % TI, w3 = u / TI;
-          int off = w3 == 0 ? -1 : TJs + (w3 - 1);
-          int ii = i0 + col, jj = j0 + off, row = off + 1;
-          if (ii <= d.ib && jj <= d.jb) {
-            diw_at<1>(c, ix(d, ii, 0, k), jj, o);
-            for (int n = 0; n < 5; n++) sm.dy[n][row][col] = o[n];
-          }
+          const int u = t - nF;
+          const int r = u % TJs, w4 = u / TJs;
+          const int off = w4 < 2 ? w4 - 2 : TI + (w4 - 2);
+          if (j0 + r <= d.jb) sm.px[r][off + 2] = lds_fp<0>(c, sm, r + 3, i0 + off, i0 - 3);
+        } else {  // fp_y at j0-2, j0-1, j0+TJs, j0+TJs+1
+          const int u = t - nF - 4 * TJs;
+          const int cl = u % TI, w4 = u / TI;
+          const int off = w4 < 2 ? w4 - 2 : TJs + (w4 - 2);
+          if (i0 + cl <= d.ib) sm.py[off + 2][cl] = lds_fp<1>(c, sm, cl + 3, j0 + off, j0 - 3);
         }
       }
     }
     __syncthreads();
     if (valid) {
-      const long q = qk;
       T f[5];
       if (A.fw_in) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = PRE ? pfw[n] : A.fw[n * E + q];
+        for (int n = 0; n < 5; n++) f[n] = pfw[n];
       } else if (A.m > 1) {
 #pragma unroll
-        for (int n = 0; n < 5; n++) f[n] = (PRE ? pfw[n] : A.fw[n * E + q]) * A.blend;
+        for (int n = 0; n < 5; n++) f[n] = pfw[n] * A.blend;
       } else {
 #pragma unroll
         for (int n = 0; n < 5; n++) f[n] = T(0);
       }
-      T Dk1[5], Ek1[5];
+      T Dk1[5], Ek1[5], Fz1[5];
       if (diss) {
         // face coefficients (dflux.f90:88-96), then per component the two
-        // faces of each direction straight from LDS
+        // faces of each direction; diw(s) = q(s) - q(s-1) from the staged
+        // state at s = a-1, a, a+1, a+2 with the line-end copies diw(1) =
+        // diw(2), diw(ie+1) = diw(ie)
         T da2, da4, dal, db2, db4, dbl;
-        const long brow = ix(d, 0, j, k), bcol = ix(d, i, 0, k);
         {
-          T r_m = PRE ? prad[0] : rad_at<0>(c, brow, i - 1), r_0 = PRE ? prad[1] : rad_at<0>(c, brow, i),
-            r_p = PRE ? prad[2] : rad_at<0>(c, brow, i + 1);
-          face_coef(c, sm.px[ty][tx], sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], r_m, r_0, &da2, &da4,
-                    &dal);
-          face_coef(c, sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.px[ty][tx + 4], r_0, r_p, &db2,
-                    &db4, &dbl);
+          face_coef(c, sm.px[ty][tx], sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.rd[ty + 1][tx],
+                    sm.rd[ty + 1][tx + 1], &da2, &da4, &dal);
+          face_coef(c, sm.px[ty][tx + 1], sm.px[ty][tx + 2], sm.px[ty][tx + 3], sm.px[ty][tx + 4],
+                    sm.rd[ty + 1][tx + 1], sm.rd[ty + 1][tx + 2], &db2, &db4, &dbl);
+          const bool lo = i == 2, hi = i == d.il;
 #pragma unroll
           for (int n = 0; n < 5; n++) {
-            T d0 = sm.dx[n][ty][tx], d1 = sm.dx[n][ty][tx + 1], d2 = sm.dx[n][ty][tx + 2], d3 = sm.dx[n][ty][tx + 3];
+            T q0 = sm.rw[n][ty + 2][tx], q1 = sm.rw[n][ty + 2][tx + 1], q2 = sm.rw[n][ty + 2][tx + 2],
+              q3 = sm.rw[n][ty + 2][tx + 3], q4 = sm.rw[n][ty + 2][tx + 4];
+            T d1 = q2 - q1, d2 = q3 - q2;
+            T d0 = lo ? d1 : q1 - q0, d3 = hi ? d2 : q4 - q3;
             T fa = (da2 * d1 - da4 * (d2 - two * d1 + d0)) * dal;
             T fb = (db2 * d2 - db4 * (d3 - two * d2 + d1)) * dbl;
             f[n] = f[n] + (fa - fb);
           }
         }
         {
-          T r_m = PRE ? prad[3] : rad_at<1>(c, bcol, j - 1), r_0 = PRE ? prad[1] : rad_at<1>(c, bcol, j),
-            r_p = PRE ? prad[4] : rad_at<1>(c, bcol, j + 1);
-          face_coef(c, sm.py[ty][tx], sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], r_m, r_0, &da2, &da4,
-                    &dal);
-          face_coef(c, sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.py[ty + 4][tx], r_0, r_p, &db2,
-                    &db4, &dbl);
+          face_coef(c, sm.py[ty][tx], sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.rd[ty][tx + 1],
+                    sm.rd[ty + 1][tx + 1], &da2, &da4, &dal);
+          face_coef(c, sm.py[ty + 1][tx], sm.py[ty + 2][tx], sm.py[ty + 3][tx], sm.py[ty + 4][tx],
+                    sm.rd[ty + 1][tx + 1], sm.rd[ty + 2][tx + 1], &db2, &db4, &dbl);
+          const bool lo = j == 2, hi = j == d.jl;
 #pragma unroll
           for (int n = 0; n < 5; n++) {
-            T d0 = sm.dy[n][ty][tx], d1 = sm.dy[n][ty + 1][tx], d2 = sm.dy[n][ty + 2][tx], d3 = sm.dy[n][ty + 3][tx];
+            T q0 = sm.rw[n][ty][tx + 2], q1 = sm.rw[n][ty + 1][tx + 2], q2 = sm.rw[n][ty + 2][tx + 2],
+              q3 = sm.rw[n][ty + 3][tx + 2], q4 = sm.rw[n][ty + 4][tx + 2];
+            T d1 = q2 - q1, d2 = q3 - q2;
+            T d0 = lo ? d1 : q1 - q0, d3 = hi ? d2 : q4 - q3;
             T fa = (da2 * d1 - da4 * (d2 - two * d1 + d0)) * dal;
             T fb = (db2 * d2 - db4 * (d3 - two * d2 + d1)) * dbl;
             f[n] = f[n] + (fa - fb);
           }
         }
+        // k face k+1 from diw(k) .. diw(k+2) of the column carry; diw(k+2)
+        // is diw(ke) where k+2 = ke+1
         face_coef(c, fz[0], fz[1], fz[2], fz[3], rz[0], rz[1], &da2, &da4, &dal);
+        const bool top = k + 2 > d.ke;
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-          Dk1[n] = (da2 * dzr[1][n] - da4 * (dzr[2][n] - two * dzr[1][n] + dzr[0][n])) * dal;
+          const T q2 = n < 4 ? W2[n] : w24;
+          const T z0 = R0[n] - Qm[n], z1 = R1[n] - R0[n], z2 = top ? z1 : q2 - R1[n];
+          Dk1[n] = (da2 * z1 - da4 * (z2 - two * z1 + z0)) * dal;
           f[n] = f[n] + (Dk[n] - Dk1[n]);
         }
       }
@@ -515,65 +696,77 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
         T eb = half * (sm.fy[n][ty + 2][tx] + sm.fy[n][ty + 1][tx]);
         e[n] = e[n] + ea - eb;
       }
+      cflux_st_h<2>(c.inf, R1, R1[4], R1[5], Sz, Fz1);
 #pragma unroll
       for (int n = 0; n < 5; n++) {
-        Ek1[n] = half * (Fz[1][n] + Fz[0][n]);
+        Ek1[n] = half * (Fz1[n] + Fz0[n]);
         e[n] = e[n] + Ek[n] - Ek1[n];
       }
-      if ((PRE ? pfl : A.flags[q]) & F_TOUCHED) {
+      if (pfl & F_TOUCHED) {
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-          A.fw[n * E + q] = f[n];
-          A.dw[n * E + q] = e[n];
+          A.fw[n * E + qk] = f[n];
+          A.dw[n * E + qk] = e[n];
         }
       } else {
-        finish_cell<T, PRE>(A, q, f, e, pwn, pvd, pfl);
+        finish_cell<T, PRE>(A, qk, f, e, pwn, pvd, pfl);
       }
       // advance the k-direction carry to face k+2
-      if (k < kb1) {
+      if (more) {
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           Ek[n] = Ek1[n];
-          Fz[0][n] = Fz[1][n];
+          Fz0[n] = Fz1[n];
         }
-        cell_flux<2>(c, i, j, k + 2, Fz[1]);
         if (diss) {
+          // fp(k+3) from p(k+2..k+4) (0 at and beyond the line end)
+          const int s = k + 3, nb = d.ke + 1;
+          const T fpn = fp_val<2>(c.inf, P2, P3, P4);
 #pragma unroll
           for (int n = 0; n < 5; n++) {
             Dk[n] = Dk1[n];
-            dzr[0][n] = dzr[1][n];
-            dzr[1][n] = dzr[2][n];
+            Qm[n] = R0[n];
           }
           fz[0] = fz[1];
           fz[1] = fz[2];
           fz[2] = fz[3];
-          fz[3] = fp_at<2>(c, bz, k + 3);
-          diw_at<2>(c, bz, k + 3, dzr[2]);
+          fz[3] = s >= nb ? T(0) : fpn;
           rz[0] = rz[1];
-          rz[1] = rad_at<2>(c, bz, k + 2);
+          rz[1] = rz2;
         }
       }
+    }
+    // ---- shift the column carry: k+1, k+2 become k, k+1
+    if (inarr && more) {
+#pragma unroll
+      for (int n = 0; n < 6; n++) R0[n] = R1[n];
+#pragma unroll
+      for (int n = 0; n < 4; n++) R1[n] = W2[n];
+      R1[4] = w24;
+      R1[5] = P2;
+      P2 = P3;
+      P3 = P4;
     }
     __syncthreads();
   }
 }
 // entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
-// the single-case entry), with launch bounds that follow the precision's tile
-template <typename T>
-__global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW) k_stage2(StageArgs<T> A) {
+// the single-case entry), with launch bounds that follow the tile's rows
+template <typename T, int TJ_>
+__global__ void __launch_bounds__(TI * TJ_, Stage2Tile<T>::MINW) k_stage2(StageArgs<T> A) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
-  k_stage2_body(A, bx, by, bz);
+  k_stage2_body<T, TJ_>(A, bx, by, bz);
 }
-template <typename T>
-__global__ void __launch_bounds__(TI * Stage2Tile<T>::TJ, Stage2Tile<T>::MINW)
+template <typename T, int TJ_>
+__global__ void __launch_bounds__(TI * TJ_, Stage2Tile<T>::MINW)
     k_stage2_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
   const int b = (int)blockIdx.x;
   const int c = batch_case(cg, nc, b);
   const int4 g = cg[c];
   if (!act[g.w]) return;
   const int l = b - g.x;
-  k_stage2_body(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+  k_stage2_body<T, TJ_>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
 }
 
 // ------------------------------------------------------------------ wall records

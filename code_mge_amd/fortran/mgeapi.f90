@@ -23,6 +23,7 @@ module mgeapi
     real(c_double)  :: dtstep
     integer(c_int)  :: smoother
     real(c_double)  :: ffreeze
+    real(c_double)  :: lfreeze
   end type
 
   ! what a jobsheet defines (mge_job)

@@ -39,6 +39,7 @@ class Params:
     dtstep: float = 0.0               # physical time step in units of ref_leng/|U_inf|
     smoother: int = 0                 # 0 psmoo_old (constant coefficient), 1 psmoo + progon3
     ffreeze: float = 0.0              # > 0: freeze the far-field inflow/outflow choice at this drop
+    lfreeze: float = 0.0              # > 0: freeze the MUSCL-minmod slope choices at this drop (Roe r2)
     extra: dict = field(default_factory=dict)
 
     def derived(self):
@@ -99,6 +100,7 @@ class Params:
         P.dtstep = d["dtstep"]
         P.smoother = self.smoother
         P.ffreeze = self.ffreeze
+        P.lfreeze = self.lfreeze
         return P
 
     def dynamic_pressure(self):

@@ -60,11 +60,12 @@ class Result:
         return self.completed - self.arrival
 
 
-def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False):
+def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False, lfreeze=False):
     """device bytes of one level as the library allocates them (every array
     padded by 64 bytes); wall-record arrays at their upper bound; a level
     below the finest also holds the prolongation's correction; the
-    variable-coefficient smoother holds its coefficients and sweep scratch"""
+    variable-coefficient smoother holds its coefficients and sweep scratch;
+    limiter freezing three 8-byte words per cell"""
     ni, nj, nk = n
     E = (ni + 4) * (nj + 4) * (nk + 4)
     nmax = max(ni, nj, nk) + 5
@@ -77,6 +78,8 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False
         sizes += [es * 5 * E]           # correction w - ws
     if psmoo:
         sizes += [es * 4 * E]           # psmoo coefficients (3) and sweep scratch
+    if lfreeze:
+        sizes += [24 * E]               # limiter words (three per cell)
     if nmir:
         ntouch, nops = 2 * nmir, 6 * nmir
         sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
@@ -87,7 +90,8 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False
 
 class Server:
     def __init__(self, hbm_budget: float, drop: float = 1e5, maxcyc: int = 3000, precision: int = 32,
-                 use_graph: bool = True, periods: int = 4, log=None, autotune: bool = True):
+                 use_graph: bool = True, periods: int = 4, log=None, autotune: bool = True,
+                 ffreeze: Optional[float] = None):
         self.budget = float(hbm_budget)
         self.drop = drop
         self.maxcyc = maxcyc
@@ -96,13 +100,14 @@ class Server:
         self.periods = periods
         self.log = log
         self.autotune = autotune
+        self.ffreeze = ffreeze   # None: each request's own far-field freeze setting
         self.batches: Dict[tuple, Batch] = {}
         self.used = 0
         self.peak_used = 0     # most device bytes reserved at once
 
     def _class(self, prm: Params, levels) -> tuple:
         return (prm.eqntype.lower(), prm.distype.lower(), prm.dissip.lower(), len(levels), len(prm.cstp),
-                prm.mgtype, prm.smoo > 0.0, prm.smoother, self.precision)
+                prm.mgtype, prm.smoo > 0.0, prm.smoother, prm.lfreeze > 0.0, self.precision)
 
     def estimate(self, req: Request, geoms=None) -> int:
         es = 4 if self.precision == 32 else 8
@@ -111,7 +116,8 @@ class Server:
             nmir = geoms[li].nmir if geoms is not None else 0
             total += level_bytes(spec.n, es, nmir, viscous=req.params.eqntype.lower() == "n",
                                  coarse=li < len(req.levels) - 1,
-                                 psmoo=req.params.smoother == 1 and req.params.smoo > 0.0)
+                                 psmoo=req.params.smoother == 1 and req.params.smoo > 0.0,
+                                 lfreeze=req.params.lfreeze > 0.0 and req.params.distype.lower() == "r2")
         return total
 
     def _prepare(self, req: Request):
@@ -129,6 +135,8 @@ class Server:
             self._resident += 1
             self.peak_used = max(self.peak_used, self.used)
         prm = dataclasses.replace(req.params, precision=self.precision)
+        if self.ffreeze is not None:
+            prm = dataclasses.replace(prm, ffreeze=self.ffreeze)
         case = Case(prm, req.levels, geoms=geoms, mxcyc=4)
         case.init_flow()
         if self.autotune:

@@ -139,10 +139,10 @@ class Case:
 
     def download(self, lvl, field):
         ib1, jb1, kb1 = self.dims(lvl)
-        nc = 5 if field in ("w", "dw", "fw", "ws", "wr", "qf", "w1", "w2") else 1
+        nc = 5 if field in ("w", "dw", "fw", "ws", "wr", "qf", "w1", "w2") else (15 if field == "lim" else 1)
         out = np.empty((nc, kb1, jb1, ib1), dtype=np.float64)
         _ffi.check(self.lib.mge_download(self.h, lvl, 1, field.encode(), out.ctypes.data_as(_dp)), "mge_download")
-        return out if nc == 5 else out[0]
+        return out if nc > 1 else out[0]
 
     def upload(self, lvl, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
@@ -192,18 +192,33 @@ class Case:
         _ffi.check(self.lib.mge_read_solut(self.h, str(path).encode(), 1 if binary else 0), "mge_read_solut")
 
     def autotune(self, levels=None):
-        """pick the stage kernels' k-chunk length per level by timing
-        (mge_autotune); returns {level: (kc, best launch us)}"""
+        """pick the stage kernels' tile rows and k-chunk length per level by
+        timing (mge_autotune); returns {level: (tile rows, kc, best launch us)}"""
         out = {}
         for lvl in (levels or range(1, len(self.levels) + 1)):
             kc = C.c_int(0)
             us = C.c_double(0.0)
             _ffi.check(self.lib.mge_autotune(self.h, lvl, C.byref(kc), C.byref(us)), "mge_autotune")
-            out[lvl] = (kc.value, us.value)
+            out[lvl] = (self.stage_tile(lvl)[0], kc.value, us.value)
         return out
 
     def set_kc(self, lvl, kc):
         _ffi.check(self.lib.mge_set_kc(self.h, lvl, kc), "mge_set_kc")
+
+    def set_freeze(self, far=0, lim=0):
+        """set the far-field and limiter freeze flags directly (mge_set_freeze)"""
+        _ffi.check(self.lib.mge_set_freeze(self.h, int(far), int(lim)), "mge_set_freeze")
+
+    def stage_tile(self, lvl):
+        """(tile rows, k-chunk length) the stage kernel of level lvl launches with"""
+        tj, kc = C.c_int(0), C.c_int(0)
+        _ffi.check(self.lib.mge_stage_tile(self.h, lvl, C.byref(tj), C.byref(kc)), "mge_stage_tile")
+        return tj.value, kc.value
+
+    def set_tile(self, lvl, tj, kc=0):
+        """force the stage kernel's tile rows (4 or 8; 0: the precision's
+        default) and k-chunk length (0: the fill rule) of level lvl"""
+        _ffi.check(self.lib.mge_set_tile(self.h, lvl, tj, kc), "mge_set_tile")
 
     def sync(self):
         return _ffi.check(self.lib.mge_synchronize(self.h), "mge_synchronize")

@@ -105,6 +105,17 @@ typedef struct {
                                  normal velocity is near zero (a wake crossing a
                                  face parallel to the stream) stop switching
                                  between the two branches cycle after cycle       */
+  double lfreeze;             /* Roe + MUSCL-minmod only (distype 2).  0: the
+                                 minmod limiter at every stage.  > 0: once the
+                                 finest level's residual has dropped by this
+                                 factor from its first cycle, every cell keeps
+                                 the slope choice (zero, backward or forward
+                                 difference, per direction and component) its
+                                 limiter made last, so the face states stop
+                                 switching between limiter branches and the
+                                 iteration converges to the solution of the
+                                 frozen scheme (no counterpart in the reference,
+                                 whose roepak source is absent)                    */
 } mge_params;
 
 /* one wall (cut-cell, solid-cell) face pair: the fields of a txcut record
@@ -291,16 +302,29 @@ int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, lon
                       double *bytes);
 
 /* Autotuner of the LDS-tiled stage kernels (no counterpart in the
- * reference): times stage 1 of level lvl for every k-chunk length kc = 1,
- * 2, 4, ... nk and keeps the fastest for the level; results are cached per
- * (shape, precision, scheme) for the process.  The chunk length changes no
- * arithmetic.  Overwrites only scratch state (a non-current state buffer,
- * dw, fw, the time-step arrays); call it between solves, not while a batch
- * is recording.  *kc (0 for the matrix-dissipation kernel, which has no
- * chunks) and *us (best launch time, microseconds) may be NULL. */
+ * reference): times stage 1 of level lvl for the tile rows tj = 4 and 8
+ * (256 or 512 threads per workgroup) and every k-chunk length kc = 1, 2,
+ * 4, ... nk, and keeps the fastest pair for the level; results are cached
+ * per (shape, precision, scheme) for the process.  The launch shape changes
+ * no arithmetic.  Overwrites only scratch state (a non-current state
+ * buffer, dw, fw, the time-step arrays); call it between solves, not while
+ * a batch is recording.  *kc (0 for the matrix-dissipation kernel, which has
+ * no chunks) and *us (best launch time, microseconds) may be NULL; the
+ * chosen tile rows are read with mge_stage_tile. */
 int mge_autotune(mge_case *h, int lvl, int *kc, double *us);
 /* force the k-chunk length of level lvl (0: the built-in fill rule) */
 int mge_set_kc(mge_case *h, int lvl, int kc);
+/* set the case's freeze flags directly: far-field branches (ffreeze) and
+ * limiter choices (lfreeze); each flag is otherwise set by the case's own
+ * freeze test and cleared by mge_init_flow (no counterpart in the
+ * reference; for tests and callers that decide the freeze themselves) */
+int mge_set_freeze(mge_case *h, int far, int lim);
+/* the tile rows and k-chunk length the stage kernel of level lvl launches
+ * with (no counterpart in the reference) */
+int mge_stage_tile(mge_case *h, int lvl, int *tj, int *kc);
+/* force the tile rows (4 or 8; 0: the precision's default) and k-chunk
+ * length (0: the fill rule) of level lvl (no counterpart in the reference) */
+int mge_set_tile(mge_case *h, int lvl, int tj, int kc);
 const char *mge_last_error(void);
 
 /* ---- batches of cases --------------------------------------------------

@@ -25,6 +25,9 @@ typedef struct {
   unsigned char *ffdec;   /* ffreeze > 0: far-field choice per ghost point of faces 1..6 (1 outflow) */
   long ffoff[7];
   const int *frozen;      /* the case's freeze flag */
+  unsigned long long *lim; /* lfreeze > 0, Roe r2: limiter values, one word per direction and cell
+                              (lim[dir*E + q], byte n = component n), else NULL */
+  const int *lfrozen;     /* the case's limiter-freeze flag */
   int bctype[6];
   double facenvec[6][3];
   double *w, *p, *ws, *dw, *fw, *wn, *wr, *qf;
@@ -44,6 +47,7 @@ typedef struct {
 struct orc_case {
   orc_params prm;
   int frozen;        /* far-field choices frozen (ffreeze > 0) */
+  int lfrozen;       /* limiter choices frozen (lfreeze > 0) */
   double ffini;      /* first finest-cycle residual of that test */
   int nlevel;
   blk_t *lv; /* lv[1..nlevel] */
@@ -98,7 +102,7 @@ static void blk_free(blk_t *b) {
   free(b->w); free(b->p); free(b->ws); free(b->dw); free(b->fw); free(b->wn);
   free(b->w1); free(b->w2);
   free(b->wr); free(b->qf); free(b->radi); free(b->radj); free(b->radk);
-  free(b->vol); free(b->v1d[0]); free(b->v1d[1]); free(b->v1d[2]); free(b->ffdec);
+  free(b->vol); free(b->v1d[0]); free(b->v1d[1]); free(b->v1d[2]); free(b->ffdec); free(b->lim);
   free(b->epsx); free(b->epsy); free(b->epsz);
   free(b->vdtim); free(b->blank); free(b->marker); free(b->z); free(b->rtrms);
 }
@@ -161,6 +165,11 @@ int orc_set_level(orc_case *c, int lvl, int ni, int nj, int nk,
   b->z = (orc_cut *)calloc((size_t)(nmir > 0 ? nmir : 1), sizeof(orc_cut));
   if (nmir > 0) memcpy(b->z, cuts, sizeof(orc_cut) * (size_t)nmir);
   b->frozen = &c->frozen;
+  b->lfrozen = &c->lfrozen;
+  if (c->prm.lfreeze > 0.0 && c->prm.distype == 2) { /* every code 3: not recorded */
+    b->lim = (unsigned long long *)malloc((size_t)E * 3 * sizeof(unsigned long long));
+    memset(b->lim, 0xff, (size_t)E * 3 * sizeof(unsigned long long));
+  }
   if (c->prm.ffreeze > 0.0) {
     for (int f = 1; f <= 6; f++) {
       int dir = (f - 1) % 3;
@@ -1004,6 +1013,47 @@ void orc_muscl(int order, const double *qm2, const double *qm1, const double *qp
   }
 }
 
+/* Limiter freezing (lfreeze > 0, second order; this build's own option,
+ * no reference counterpart).  For cell c, direction dir and component n the
+ * limiter value phi = minmod(a,b) / (0.5 (a+b)) of its backward and forward
+ * differences a, b (0 when they differ in sign or one is zero, else in
+ * (0,1]) is kept as a byte round(254 phi) at byte n of the cell's word of
+ * that direction; 255 means not recorded (ghost cells: the minmod itself).
+ * Until the freeze every evaluation of face a records the values of its
+ * interior cell a; after it a cell's slope is (byte/254) * 0.5 (a+b): a fixed
+ * blend of first order and Fromm's central slope, linear in the state and
+ * always dissipative.  The wall rule applies in both. */
+static const double LIM_K = 0.5 / 254.0;
+static double lim_frozen(double back, double fwd, unsigned byte) {
+  return byte == 255u ? minmod(back, fwd) : ((double)byte * LIM_K) * (back + fwd);
+}
+static unsigned lim_byte(double m, double back, double fwd) {
+  double phi = back * fwd > 0.0 ? m / (0.5 * (back + fwd)) : 0.0;
+  return (unsigned)(254.0 * phi + 0.5);
+}
+static void muscl_lim(int frozen, const double *qm2, const double *qm1, const double *qp0, const double *qp1,
+                      const int *fl, unsigned long long wl, unsigned long long wr, double *qL, double *qR,
+                      unsigned long long *rec) {
+  unsigned long long word = ~0ull;
+  for (int n = 0; n < 5; n++) {
+    double a1 = qm1[n] - qm2[n], b1 = qp0[n] - qm1[n], b2 = qp1[n] - qp0[n], sl, sr;
+    if (frozen) {
+      sl = lim_frozen(a1, b1, (unsigned)(wl >> (8 * n)) & 255u);
+      sr = lim_frozen(b1, b2, (unsigned)(wr >> (8 * n)) & 255u);
+    } else {
+      sl = minmod(a1, b1);
+      sr = minmod(b1, b2);
+      word &= ~(0xffull << (8 * n));
+      word |= (unsigned long long)lim_byte(sr, b1, b2) << (8 * n);
+    }
+    if (!fl[0]) sl = 0.0;
+    if (!fl[3]) sr = 0.0;
+    qL[n] = qm1[n] + 0.5 * sl;
+    qR[n] = qp0[n] - 0.5 * sr;
+  }
+  *rec = word;
+}
+
 /* the Roe face of direction dir at face a of the line through (i,j,k);
  * zero unless both neighbours are fluid */
 static void roe_bulk_face(const blk_t *b, const orc_params *P, double cdis, int dir, int i, int j, int k,
@@ -1017,13 +1067,24 @@ static void roe_bulk_face(const blk_t *b, const orc_params *P, double cdis, int 
     q[t] = IX(b, i + s * o[0], j + s * o[1], k + s * o[2]);
     fl[t] = b->blank[q[t]] != 0;
   }
+  /* the limiter word of cell a records while not frozen (interior cells);
+   * a face that is not evaluated records "not recorded" */
+  const int lrec = b->lim && !*b->lfrozen && i <= b->il && j <= b->jl && k <= b->kl;
+  unsigned long long *lw = b->lim ? b->lim + (long)dir * b->E : NULL;
+  if (lrec) lw[q[2]] = ~0ull;
   if (!(fl[1] && fl[2])) {
     for (int n = 0; n < 5; n++) e[n] = d[n] = 0.0;
     return;
   }
   double v[4][5], qL[5], qR[5], S[3];
   for (int t = 0; t < 4; t++) prim_cell(b, q[t], v[t]);
-  orc_muscl(P->distype, v[0], v[1], v[2], v[3], fl, qL, qR);
+  if (lw) {
+    unsigned long long word;
+    muscl_lim(*b->lfrozen, v[0], v[1], v[2], v[3], fl, lw[q[1]], lw[q[2]], qL, qR, &word);
+    if (lrec) lw[q[2]] = word;
+  } else {
+    orc_muscl(P->distype, v[0], v[1], v[2], v[3], fl, qL, qR);
+  }
   const double *SV = dir == 0 ? b->si : (dir == 1 ? b->sj : b->sk);
   svec(SV, b->E, q[2], S);
   orc_roe_face(P->gamma, P->epsentr, cdis, qL, qR, S, 0, e, d);
@@ -2016,6 +2077,7 @@ int orc_movfin(orc_case *c, int mgl, int mlvl) {
 /* init_flow.f90:35-99 (uniform freestream) */
 int orc_init_flow(orc_case *c) {
   c->frozen = 0;
+  c->lfrozen = 0;
   const orc_params *P = &c->prm;
   int warn = ORC_OK;
   for (int l = 1; l <= c->nlevel; l++) {
@@ -2051,9 +2113,17 @@ int orc_init_flow(orc_case *c) {
     if (_r < 0) return _r;    \
   } while (0)
 
-/* solver.f90:127-254: the FMG ramp of one physical time step at level itime */
+/* solver.f90:127-254: the FMG ramp of one physical time step at level itime.
+ * With mlvl_only > 0 (orc_cycles) only that level's cycles run, numbered
+ * from ncyc0, and no FMG transfer follows them. */
+static int solve_ramp_from(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                           long long *cell_stages, int mlvl_only, int ncyc0);
 static int solve_ramp(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
                       long long *cell_stages) {
+  return solve_ramp_from(c, mcyc, itime, reslog, cap, nlog, cell_stages, 0, 1);
+}
+static int solve_ramp_from(orc_case *c, const int *mcyc, int itime, double *reslog, int cap, int *nlog,
+                           long long *cell_stages, int mlvl_only, int ncyc0) {
   const int nlevel = c->nlevel;
   const orc_params *P = &c->prm;
   int icyc[64];
@@ -2061,9 +2131,9 @@ static int solve_ramp(orc_case *c, const int *mcyc, int itime, double *reslog, i
   double ini_res = 0.0, max_res = 0.0;
   int nl = 0;
   long long cs = 0;
-  for (int mlvl = 1; mlvl <= nlevel; mlvl++) {
+  for (int mlvl = mlvl_only > 0 ? mlvl_only : 1; mlvl <= (mlvl_only > 0 ? mlvl_only : nlevel); mlvl++) {
     mgl = mlvl;
-    for (int ncyc = 1; ncyc <= mcyc[mlvl - 1]; ncyc++) {
+    for (int ncyc = ncyc0; ncyc < ncyc0 + mcyc[mlvl - 1]; ncyc++) {
       for (int l = 0; l <= nlevel; l++) icyc[l] = 0;
       int next_cycle = 0;
       for (;;) { /* lopmg */
@@ -2111,23 +2181,37 @@ static int solve_ramp(orc_case *c, const int *mcyc, int itime, double *reslog, i
         }
         if (exit_mg) break;
       }
-      if (P->ffreeze > 0.0 && mlvl == nlevel) { /* the far-field freeze test of the cycle */
+      if ((P->ffreeze > 0.0 || P->lfreeze > 0.0) && mlvl == nlevel) { /* the freeze tests of the cycle */
         double r = c->lv[mlvl].rtrms[ncyc];
         if (ncyc == 1) c->ffini = r;
-        if (c->ffini / r > P->ffreeze) c->frozen = 1;
+        if (P->ffreeze > 0.0 && c->ffini / r > P->ffreeze) c->frozen = 1;
+        if (P->lfreeze > 0.0 && c->ffini / r > P->lfreeze) c->lfrozen = 1;
       }
       if (next_cycle) continue;
       if (mgl == mlvl) {
         if (ini_res / max_res > P->drop) break;
       }
     }
-    if (mlvl < nlevel) {
+    if (mlvl < nlevel && mlvl_only == 0) {
       mgl = mlvl + 1;
       CHK(orc_movfin(c, mgl, mlvl));
     }
   }
   if (nlog) *nlog = nl;
   if (cell_stages) *cell_stages = cs;
+  return ORC_OK;
+}
+
+/* count W-cycles at finest active level mlvl numbered ncyc0.. (the cycle
+ * body of solver.f90:131-240 without the FMG transfer; the freeze tests run,
+ * the drop test only when ncyc0 = 1); *res the last finest residual */
+int orc_cycles(orc_case *c, int mlvl, int ncyc0, int count, double *res, long long *cell_stages) {
+  if (mlvl < 1 || mlvl > c->nlevel || count < 0 || ncyc0 < 1 || ncyc0 + count - 1 > c->lv[mlvl].mxcyc)
+    return ORC_ERR_ARG;  /* every cycle's residual has its slot */
+  int mcyc[64] = {0};
+  mcyc[mlvl - 1] = count;
+  CHK(solve_ramp_from(c, mcyc, 0, NULL, 0, NULL, cell_stages, mlvl, ncyc0));
+  if (res) *res = count > 0 ? c->lv[mlvl].rtrms[ncyc0 + count - 1] : 0.0;
   return ORC_OK;
 }
 
@@ -2229,7 +2313,22 @@ static double *field_ptr(blk_t *b, const char *name, long *n) {
   return NULL;
 }
 
+/* the freeze flags, set directly (tests) */
+void orc_set_frozen(orc_case *c, int far, int lim) {
+  c->frozen = far;
+  c->lfrozen = lim;
+}
+
 int orc_get(orc_case *c, int lvl, const char *name, double *out) {
+  if (!strcmp(name, "lim")) { /* limiter bytes: out[(5 dir + n) E + q] */
+    blk_t *b = &c->lv[lvl];
+    if (!b->lim) return ORC_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+      for (int n = 0; n < 5; n++)
+        for (long q = 0; q < b->E; q++)
+          out[(long)(5 * d + n) * b->E + q] = (double)((b->lim[(long)d * b->E + q] >> (8 * n)) & 255u);
+    return ORC_OK;
+  }
   long n;
   double *p = field_ptr(&c->lv[lvl], name, &n);
   if (!p) return ORC_ERR_ARG;
@@ -2238,6 +2337,20 @@ int orc_get(orc_case *c, int lvl, const char *name, double *out) {
 }
 
 int orc_set(orc_case *c, int lvl, const char *name, const double *in) {
+  if (!strcmp(name, "lim")) { /* limiter bytes in[(5 dir + n) E + q] */
+    blk_t *b = &c->lv[lvl];
+    if (!b->lim) return ORC_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+      for (long q = 0; q < b->E; q++) {
+        unsigned long long wd = ~0ull;
+        for (int n = 0; n < 5; n++) {
+          wd &= ~(0xffull << (8 * n));
+          wd |= (unsigned long long)in[(long)(5 * d + n) * b->E + q] << (8 * n);
+        }
+        b->lim[(long)d * b->E + q] = wd;
+      }
+    return ORC_OK;
+  }
   long n;
   double *p = field_ptr(&c->lv[lvl], name, &n);
   if (!p) return ORC_ERR_ARG;

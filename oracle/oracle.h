@@ -59,6 +59,9 @@ typedef struct {
   double ffreeze;      /* 0: bc_far's inflow/outflow choice at every call (bc_far.f90:58-60);
                           > 0: after the finest-level cycle whose residual has dropped by this
                           factor from the first, each far-field ghost point keeps its last choice */
+  double lfreeze;      /* Roe r2 only.  0: minmod at every call; > 0: after the finest-level cycle
+                          whose residual has dropped by this factor, every cell keeps the slope
+                          choices (2-bit codes per direction and component) its limiter made last */
 } orc_params;
 
 typedef struct {
@@ -132,6 +135,8 @@ void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const do
 /* field access.  name: "w","p","dw","fw","ws","wn","wr","qf","radi","radj",
  * "radk","vdtim","si","sj","sk","w1","w2","epsx","epsy","epsz".  Copies the full (0:ib,...) array. */
 int    orc_get(orc_case *c, int lvl, const char *name, double *out);
+void   orc_set_frozen(orc_case *c, int far, int lim);   /* freeze flags, set directly (tests) */
+int    orc_cycles(orc_case *c, int mlvl, int ncyc0, int count, double *res, long long *cell_stages);
 int    orc_set(orc_case *c, int lvl, const char *name, const double *in);
 int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);
 double orc_get_rtrms(orc_case *c, int lvl, int ncyc);

@@ -28,6 +28,7 @@ class OrcParams(C.Structure):
         ("execmode", C.c_int), ("dtstep", C.c_double),
         ("smoother", C.c_int),
         ("ffreeze", C.c_double),
+        ("lfreeze", C.c_double),
     ]
 
 
@@ -72,6 +73,9 @@ def lib():
     L.orc_shift_time_levels.argtypes = [P]
     L.orc_upload_solut.argtypes = [P, I, C.c_double, _dp]
     L.orc_get.argtypes = [P, I, C.c_char_p, _dp]
+    L.orc_set_frozen.argtypes = [P, I, I]
+    L.orc_cycles.argtypes = [P, I, I, I, _dp, C.POINTER(C.c_longlong)]
+    L.orc_set_frozen.restype = None
     L.orc_set.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_get_cuts.argtypes = [P, I, C.c_void_p]
     L.orc_get_rtrms.argtypes = [P, I, I]
@@ -120,6 +124,7 @@ def params_from(prm):
     P.dtstep = d["dtstep"]
     P.smoother = prm.smoother
     P.ffreeze = prm.ffreeze
+    P.lfreeze = prm.lfreeze
     return P
 
 
@@ -241,7 +246,7 @@ class OracleCase:
     def download(self, lvl, field):
         ib1, jb1, kb1 = self.dims(lvl)
         nc = {"w": 5, "dw": 5, "fw": 5, "ws": 5, "wn": 5, "wr": 5, "qf": 5, "w1": 5, "w2": 5,
-              "si": 3, "sj": 3, "sk": 3}.get(field, 1)
+              "si": 3, "sj": 3, "sk": 3, "lim": 15}.get(field, 1)
         out = np.empty((nc, kb1, jb1, ib1))
         self._chk(self.L.orc_get(self.h, lvl, field.encode(), out.ctypes.data_as(_dp)), "get")
         return out if nc > 1 else out[0]
@@ -249,6 +254,18 @@ class OracleCase:
     def upload(self, lvl, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self._chk(self.L.orc_set(self.h, lvl, field.encode(), a.ctypes.data_as(_dp)), "set")
+
+    def cycles(self, mlvl, count, ncyc0=1):
+        """count W-cycles at finest active level mlvl without the FMG
+        transfer (numbered from ncyc0, as mge_cycles)"""
+        r = C.c_double(0.0)
+        cs = C.c_longlong(0)
+        self._chk(self.L.orc_cycles(self.h, mlvl, ncyc0, count, C.byref(r), C.byref(cs)), "cycles")
+        return r.value, cs.value
+
+    def set_frozen(self, far=0, lim=0):
+        """set the far-field and limiter freeze flags directly"""
+        self.L.orc_set_frozen(self.h, int(far), int(lim))
 
     def cuts(self, lvl):
         g = self.geoms[lvl - 1]

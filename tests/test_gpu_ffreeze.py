@@ -2,12 +2,14 @@
 oracle through the freeze cycle and beyond, and batched cases (whose freeze
 test runs in the device-side convergence kernel) bit for bit equal to
 their own runs (whose test runs after each cycle on the case stream)."""
+import dataclasses
+
 import numpy as np
 import pytest
 
 from code_mge_amd import configs
 from code_mge_amd.grid import level_geometry
-from code_mge_amd.solver import Batch, Case
+from code_mge_amd.solver import Batch, Case, force_coefficients
 from tests.gpu_common import cell_rel_err, make_pair, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -56,3 +58,26 @@ def test_ffreeze_batch_equals_own_runs(require_gpu):
             b.remove(slots[i])
     for c in cases:
         c.close()
+
+
+@pytest.mark.parametrize("ci", [11, 12])
+def test_ffreeze_sweep_stalls_converge(require_gpu, ci):
+    """the two cases of the seeded ellipsoid sweep (PCG64(20261202)) whose
+    residual stalls near a drop of 1e4 without the freeze (a far-field face
+    point flipping between inflow and outflow where the wake leaves the box):
+    with the choice frozen at a drop of 1e3 both precisions reach the 1e5 drop
+    within 600 W-cycles, with Cd/Cl of the two within 1e-5"""
+    prm, levels, bdy, aref, info = configs.ellipsoid_sweep(ci + 1)[ci]
+    geoms = [level_geometry(s, bdy) for s in levels]
+    out = {}
+    for prec in (64, 32):
+        p = dataclasses.replace(prm, precision=prec, drop=1e5, ffreeze=1e3)
+        with Case(p, levels, geoms=geoms, mxcyc=602) as c:
+            c.init_flow()
+            log, _ = c.solve([0, 0, 600])
+            r = log[log[:, 0] == 3][:, 2]
+            out[prec] = force_coefficients(c, aref)
+            print(f"case {ci} ({info['n']}^3) fp{prec}: {len(r)} cycles, drop {r[0] / r[-1]:.3e}, "
+                  f"Cd {out[prec][0]:.8f} Cl {out[prec][1]:.8f}", flush=True)
+            assert r[0] / r[-1] >= 1e5 and len(r) < 600
+    assert abs(out[32][0] - out[64][0]) < 1e-5 and abs(out[32][1] - out[64][1]) < 1e-5
