@@ -64,6 +64,88 @@ def cpu_baseline(n, prm_kw, nz_slab=48):
                              f"through the sphere of the {n}^3 grid ({cs/1e6:.1f} M cell-stages, {dt:.1f} s)"
 
 
+def cfg1_leg(cycles=500):
+    """The reference's CPU-runnable case in full (128x128x4 cylinder, M=0.2,
+    alpha=0, JST, CFL 2.5, smoothing 0.6, 3-level W-cycle, `cycles` cycles
+    after a 2-cycle FMG ramp per coarse level): the CPU restatement on one
+    core and the HIP path (FP64) on the same case; wall time, throughput,
+    Cd and the residual drop of each."""
+    from code_mge_amd import configs
+    from code_mge_amd.grid import level_geometry
+    from code_mge_amd.solver import Case
+    from oracle.oracle import OracleCase
+    prm, levels, bdy, aref = configs.cylinder2d(n=128, nz=4, nlevel=3)
+    assert prm.alpha_deg == 0.0 and prm.beta_deg == 0.0
+    geoms = [level_geometry(s, bdy) for s in levels]
+    mcyc = [2, 2, cycles]
+    out = {"workload": f"cylinder 128x128x4, M=0.2, JST, 3-level W-cycle, FMG 2:2:{cycles}", "cycles": cycles}
+    for kind in ("gpu", "cpu"):
+        if kind == "gpu":
+            c = Case(prm, levels, geoms=geoms, mxcyc=cycles + 4)
+        else:
+            c = OracleCase(prm, geoms, mxcyc=cycles + 4)
+        c.init_flow()
+        if kind == "gpu":
+            c.sync()
+        t0 = time.perf_counter()
+        log, cs = c.solve(mcyc)
+        dt = time.perf_counter() - t0
+        F = c.wall_force(3)          # FP64 wall force; drag along the stream (alpha = beta = 0 here)
+        cd = float(F[0] / (prm.dynamic_pressure() * aref))
+        c.close()
+        fine = log[log[:, 0] == 3]
+        out[kind] = {"s": round(dt, 3), "mcell_stage_per_s": round(cs / dt / 1e6, 2), "cell_stages": int(cs),
+                     "cd": cd, "residual_drop": float(fine[0, 2] / fine[-1, 2])}
+        print(f"[bench] cfg1 {kind}: {dt:.2f} s, {cs / dt / 1e6:.1f} Mcs/s, Cd {cd:.8f}", file=sys.stderr,
+              flush=True)
+    out["cpu"]["cores"] = 1
+    out["speedup_gpu_vs_cpu"] = round(out["cpu"]["s"] / out["gpu"]["s"], 1)
+    return out
+
+
+def _cpu_sweep_case(item):
+    """one sweep case on the CPU restatement: geometry and setup, then two
+    W-cycles on the finest level (timed)"""
+    import dataclasses
+    from code_mge_amd.grid import level_geometry
+    from oracle.oracle import OracleCase
+    prm, levels, bdy = item
+    t0 = time.perf_counter()
+    geoms = [level_geometry(s, bdy) for s in levels]
+    oc = OracleCase(dataclasses.replace(prm, precision=64, drop=1e30), geoms, mxcyc=4)
+    oc.init_flow()
+    t1 = time.perf_counter()
+    oc.solve([0] * (len(levels) - 1) + [2])
+    t2 = time.perf_counter()
+    oc.close()
+    return t1 - t0, (t2 - t1) / 2.0
+
+
+def cpu_sweep(sweep, cycles, threads):
+    """all-core CPU estimate of the sweep's solves per hour: every case's
+    setup and two W-cycles measured on the CPU restatement (FP64, one case
+    per thread, `threads` at once; the restatement keeps no global state and
+    its ctypes calls release the interpreter lock), each case's time to its
+    drop extrapolated as setup + cycles_needed x (cycle time), with
+    cycles_needed the FP64 count of the same case on the GPU, and the cases
+    packed longest first onto the threads"""
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        meas = list(ex.map(_cpu_sweep_case, [(prm, levels, bdy) for prm, levels, bdy, aref, info in sweep]))
+    wall = time.perf_counter() - t0
+    est = sorted((s + c * n for (s, c), n in zip(meas, cycles)), reverse=True)
+    load = [0.0] * threads
+    for e in est:
+        load[load.index(min(load))] += e
+    makespan = max(load)
+    return {"threads": threads, "kind": "port", "cpu": _cpu_model(),
+            "sample": f"setup + 2 W-cycles of each of the {len(sweep)} cases measured ({wall:.1f} s wall), "
+                      "time to the drop extrapolated with each case's FP64 cycle count",
+            "cycle_s": [round(c, 3) for s, c in meas], "setup_s": [round(s, 3) for s, c in meas],
+            "est_makespan_s": round(makespan, 1), "est_solves_per_h": round(len(sweep) / makespan * 3600.0, 1)}
+
+
 def prm_stages(prm):
     return len(prm.cstp)
 
@@ -200,7 +282,7 @@ def jst_leg(args):
     return out
 
 
-def sweep_leg(k, drop=1e5, maxcyc=3000, ffreeze=1e3):
+def sweep_leg(k, drop=1e5, maxcyc=3000, ffreeze=1e3, cpu_threads=0):
     """Converged solves per hour on the first k cases of the seeded ellipsoid
     sweep: FP32 served (continuous batching, all requests at t = 0) against
     the plain FP64 path one case at a time (solver.f90's loop with the drop
@@ -234,6 +316,10 @@ def sweep_leg(k, drop=1e5, maxcyc=3000, ffreeze=1e3):
               f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t64 = time.perf_counter() - t0
     print(f"[bench] sweep: FP64 one at a time in {t64:.1f} s", file=sys.stderr, flush=True)
+    cpu = None
+    if cpu_threads > 0:
+        print(f"[bench] sweep: CPU restatement, {cpu_threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_sweep(sweep, [alone[i][0] for i in range(len(sweep))], cpu_threads)
     conv32 = sum(1 for r in served.values() if r.status == 1)
     conv64 = sum(1 for v in alone.values() if v[2] < 1.0 / drop)
     dcd = max(abs(served[i].cd - alone[i][1][0]) for i in alone)
@@ -250,15 +336,17 @@ def sweep_leg(k, drop=1e5, maxcyc=3000, ffreeze=1e3):
             "max_cd_err_fp32_vs_fp64": dcd, "max_cl_err_fp32_vs_fp64": dcl,
             "ttc_p50_s": round(p50, 3), "ttc_p99_s": round(p99, 3),
             "ttc_over": "converged requests only", "fp32_outcomes": outcome_counts(list(served.values())),
-            "cycles_fp32": [served[i].ncyc for i in sorted(served)], "cycles_fp64": [alone[i][0] for i in sorted(alone)]}
+            "cycles_fp32": [served[i].ncyc for i in sorted(served)], "cycles_fp64": [alone[i][0] for i in sorted(alone)],
+            "cpu_all_cores": cpu}
 
 
 def traffic_probe(args, precision):
     """A short program for rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE):
     a 1 GiB device-to-device copy (the calibration dispatch: 1 GiB read,
-    1 GiB written) and then, after the autotuner's launches, one cycle of
-    the bench workload, whose finest-level stage-kernel dispatches give the
-    HBM bytes per launch."""
+    1 GiB written) and then, after the autotuner's launches and a small fill
+    dispatch that marks the start, one cycle of the bench workload: its
+    first six stage-kernel dispatches (the finest level's five stages and
+    its residual evaluation) give the HBM bytes and counters per launch."""
     import dataclasses
     import torch
     from code_mge_amd import configs
@@ -275,22 +363,29 @@ def traffic_probe(args, precision):
     with Case(dataclasses.replace(prm, precision=precision), levels, geoms=geoms, mxcyc=4) as c:
         c.init_flow()
         c.autotune()       # the launch shapes of the timed bench
+        c.sync()
+        mark = torch.zeros(64, device="cuda")
+        mark.fill_(1.0)    # a fill dispatch marks where the cycle's dispatches begin
+        torch.cuda.synchronize()
         c.cycles(args.levels, 1, ncyc0=1)
         c.sync()
     print(f"[bench] traffic probe FP{precision} done", file=sys.stderr, flush=True)
 
 
-def converge_leg(prm, levels, geoms, aref, maxcyc, drop64=1e8, drop32=1e5):
+def converge_leg(prm, levels, geoms, aref, maxcyc, lfreeze, ffreeze=1e3, drop64=1e8, drop32=1e5):
     """The bench workload run from freestream to its stopping residual drop
     (1e-8 FP64, 1e-5 FP32; the W-cycle on the finest level after the FMG
-    ramp, 2 cycles per coarser level) or maxcyc cycles; time, cycles and
+    ramp, 2 cycles per coarser level) or maxcyc cycles, with the MUSCL-minmod
+    limiter values frozen once the residual has dropped by lfreeze (0: never;
+    without the freeze the limiter's switching stalls the residual near a
+    drop of 3e2) and the far-field branches at ffreeze; time, cycles and
     Cd/Cl of each, and the FP32 error against FP64."""
     import dataclasses
     from code_mge_amd.solver import Case, force_coefficients
-    out = {"workload": "the bench workload to convergence", "maxcyc": maxcyc}
+    out = {"workload": "the bench workload to convergence", "maxcyc": maxcyc, "lfreeze": lfreeze, "ffreeze": ffreeze}
     nl = len(levels)
     for prec, drop in ((64, drop64), (32, drop32)):
-        pp = dataclasses.replace(prm, precision=prec, drop=drop)
+        pp = dataclasses.replace(prm, precision=prec, drop=drop, lfreeze=lfreeze, ffreeze=ffreeze)
         with Case(pp, levels, geoms=geoms, mxcyc=maxcyc + 8) as c:
             c.init_flow()
             c.autotune()
@@ -386,6 +481,12 @@ def main():
     ap.add_argument("--converge", type=int, default=0,
                     help="run the workload to its residual drop, at most this many cycles (0: skip)")
     ap.add_argument("--serve-leg", type=int, default=32, help="requests of the Poisson serving leg (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the all-core CPU estimate of the sweep (0: skip)")
+    ap.add_argument("--cfg1-leg", type=int, default=0,
+                    help="cycles of the full cfg1 cylinder on the CPU restatement and the HIP path (0: skip)")
+    ap.add_argument("--lfreeze", type=float, default=100.0,
+                    help="residual drop at which --converge freezes the limiter values (0: never)")
     ap.add_argument("--traffic-probe", type=int, default=0, choices=(0, 32, 64),
                     help="run only the short program profiled with rocprofv3 --pmc (precision 32 or 64)")
     ap.add_argument("--probe-scheme", default="r2", choices=("r2", "c"),
@@ -420,10 +521,12 @@ def main():
     cd, cl, roofline = main_leg["cd"], main_leg["cl"], main_leg["roofline"]
 
     jst = jst_leg(args) if not args.no_jst else None
-    solves = sweep_leg(args.sweep) if args.sweep > 0 else None
+    solves = sweep_leg(args.sweep, cpu_threads=args.cpu_threads if not args.no_cpu_baseline else 0) \
+        if args.sweep > 0 else None
     ns = ns_leg(args.ns_leg) if args.ns_leg > 0 else None
-    conv = converge_leg(prm, levels, geoms, aref, args.converge) if args.converge > 0 else None
+    conv = converge_leg(prm, levels, geoms, aref, args.converge, args.lfreeze) if args.converge > 0 else None
     served = serve_leg(args.serve_leg) if args.serve_leg > 0 else None
+    cfg1 = cfg1_leg(args.cfg1_leg) if args.cfg1_leg > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -458,6 +561,8 @@ def main():
     }
     if ns is not None:
         out["ns_cylinder"] = ns
+    if cfg1 is not None:
+        out["cfg1"] = cfg1
     if conv is not None:
         out["converged"] = conv
     if served is not None:

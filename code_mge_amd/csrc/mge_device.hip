@@ -578,19 +578,35 @@ int stage_kc(const Level &L) {
 
 // single-case and batch entry points of the LDS-tiled stage kernel of a
 // scheme (distype 0: JST, else Roe, with the limiter-freezing paths when
-// lim) with tj tile rows (4 or 8)
-template <class T>
-void stage_fns(int distype, int tj, bool lim, const void **fs, const void **fb) {
+// lim) with tj tile rows (4 or 8), in finish_cell's general form when gen
+template <class T, int TJ_, bool GEN>
+void stage_fns_t(int distype, bool lim, const void **fs, const void **fb) {
   if (distype != 0 && lim) {
-    *fs = tj == 4 ? (const void *)&k_stage_roe<T, 4, true> : (const void *)&k_stage_roe<T, 8, true>;
-    *fb = tj == 4 ? (const void *)&k_stage_roe_b<T, 4, true> : (const void *)&k_stage_roe_b<T, 8, true>;
+    *fs = (const void *)&k_stage_roe<T, TJ_, true, GEN>;
+    *fb = (const void *)&k_stage_roe_b<T, TJ_, true, GEN>;
   } else if (distype != 0) {
-    *fs = tj == 4 ? (const void *)&k_stage_roe<T, 4, false> : (const void *)&k_stage_roe<T, 8, false>;
-    *fb = tj == 4 ? (const void *)&k_stage_roe_b<T, 4, false> : (const void *)&k_stage_roe_b<T, 8, false>;
+    *fs = (const void *)&k_stage_roe<T, TJ_, false, GEN>;
+    *fb = (const void *)&k_stage_roe_b<T, TJ_, false, GEN>;
   } else {
-    *fs = tj == 4 ? (const void *)&k_stage2<T, 4> : (const void *)&k_stage2<T, 8>;
-    *fb = tj == 4 ? (const void *)&k_stage2_b<T, 4> : (const void *)&k_stage2_b<T, 8>;
+    *fs = (const void *)&k_stage2<T, TJ_, GEN>;
+    *fb = (const void *)&k_stage2_b<T, TJ_, GEN>;
   }
+}
+template <class T>
+void stage_fns(int distype, int tj, bool lim, bool gen, const void **fs, const void **fb) {
+  if (tj == 4) {
+    if (gen) stage_fns_t<T, 4, true>(distype, lim, fs, fb);
+    else stage_fns_t<T, 4, false>(distype, lim, fs, fb);
+  } else {
+    if (gen) stage_fns_t<T, 8, true>(distype, lim, fs, fb);
+    else stage_fns_t<T, 8, false>(distype, lim, fs, fb);
+  }
+}
+// whether a stage launch needs finish_cell's general form: a coarse level,
+// unsteady terms, per-cell volumes or the global time step
+template <class T>
+bool stage_gen(const StageArgs<T> &A) {
+  return A.coarse || A.tsrc || A.ulam != 0 || A.vol != nullptr || A.vt == 1;
 }
 
 template <class T>
@@ -648,7 +664,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       A.kc = kc;
       if (P.distype != 0 || P.dissip == 0) {
         const void *fs, *fb;
-        stage_fns<T>(P.distype, tj, A.lim != nullptr, &fs, &fb);
+        stage_fns<T>(P.distype, tj, A.lim != nullptr, stage_gen(A), &fs, &fb);
         launch(h, "k_stage", fs, fb, g, dim3(TI, tj), A);
       } else {
         LAUNCHT(h, "k_stage", k_stage, T, cell_grid(d), cell_block(), A);  // matrix dissipation
@@ -1054,7 +1070,7 @@ int do_autotune(mge_case *h, Level &L, int *kc_out, double *us_out) {
   hipError_t err = hipSuccess;
   for (int tj : {4, 8}) {
     const void *fn, *fb;
-    stage_fns<T>(P.distype, tj, A.lim != nullptr, &fn, &fb);
+    stage_fns<T>(P.distype, tj, A.lim != nullptr, stage_gen(A), &fn, &fb);
     for (int kc = 1; err == hipSuccess; kc = std::min(2 * kc, d.nk)) {
       A.kc = kc;
       dim3 g((d.ni + TI - 1) / TI, (d.nj + tj - 1) / tj, (d.nk + kc - 1) / kc);

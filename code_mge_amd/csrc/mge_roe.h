@@ -438,8 +438,9 @@ __device__ __attribute__((always_inline)) inline void roe_z_face(const StageArgs
     for (int n = 0; n < 5; n++) dd[n] = T(0);
 }
 
-// LIM: built with the limiter-freezing paths (A.lim set), else without
-template <typename T, int TJr, bool LIM>
+// LIM: built with the limiter-freezing paths (A.lim set), else without;
+// GEN: finish_cell's general form (see there)
+template <typename T, int TJr, bool LIM, bool GEN>
 __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const StageArgs<T> &A, int bx, int by, int bz) {
   __shared__ RoeSmem<T, TJr, LIM> sm;
   const FluxCtx<T> &c = A.fx;
@@ -725,7 +726,7 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
           A.dw[n * E + q] = e[n];
         }
       } else {
-        finish_cell<T, PRE>(A, q, f, e, pwn, pvd, pfl);
+        finish_cell<T, PRE, GEN>(A, q, f, e, pwn, pvd, pfl);
       }
       // advance the column carry
 #pragma unroll
@@ -749,13 +750,13 @@ __device__ __attribute__((always_inline)) inline void k_stage_roe_body(const Sta
 }
 // entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
 // the single-case entry), with launch bounds that follow the tile's rows
-template <typename T, int TJ_, bool LIM>
+template <typename T, int TJ_, bool LIM, bool GEN>
 __global__ void __launch_bounds__(TI * TJ_, RoeTile<T>::MINW) k_stage_roe(StageArgs<T> A) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
-  k_stage_roe_body<T, TJ_, LIM>(A, bx, by, bz);
+  k_stage_roe_body<T, TJ_, LIM, GEN>(A, bx, by, bz);
 }
-template <typename T, int TJ_, bool LIM>
+template <typename T, int TJ_, bool LIM, bool GEN>
 __global__ void __launch_bounds__(TI * TJ_, RoeTile<T>::MINW)
     k_stage_roe_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
   const int b = (int)blockIdx.x;
@@ -763,7 +764,7 @@ __global__ void __launch_bounds__(TI * TJ_, RoeTile<T>::MINW)
   const int4 g = cg[c];
   if (!act[g.w]) return;
   const int l = b - g.x;
-  k_stage_roe_body<T, TJ_, LIM>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+  k_stage_roe_body<T, TJ_, LIM, GEN>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
 }
 
 }  // namespace mge

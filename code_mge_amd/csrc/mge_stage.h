@@ -109,15 +109,19 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
 // dw = dw + fw, forcing, blanking, time-step scaling and (unsmoothed) the
 // update with the EOS (euler.f90:217-223, 294-342, 363-399).  With PRE the
 // caller has already loaded the cell's W(0) (wnp), local time step (vdt) and
-// flags (fl), so that their latency overlapped its face evaluation.
-template <typename T, bool PRE = false>
+// flags (fl), so that their latency overlapped its face evaluation.  GEN
+// false is the form for a launch that is not on a coarse level, not
+// unsteady, not on a stretched level and uses the local time step (the
+// host picks it only then): those branches and their pointers drop out, which
+// frees the scalar registers they held in the stage kernels' plane loops.
+template <typename T, bool PRE = false, bool GEN = true>
 __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, const T *e, const T *wnp = nullptr,
                                    T vdt = T(0), uint8_t fl = 0) {
   const long E = A.fx.d.E;
   T r[5];
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = e[n] + f[n];
-  if (A.tsrc) {
+  if (GEN && A.tsrc) {
     // physical time derivative of the stage state (euler.f90:249-275)
     const T *w = A.fx.s.w;
     const T tc = A.vol ? T(0.5) * A.vol[q] / A.dtstep : A.tcoef;
@@ -130,7 +134,7 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
       for (int n = 0; n < 5; n++) r[n] = r[n] + tc * (w[n * E + q] - A.w1[n * E + q]);
     }
   }
-  if (A.coarse) {
+  if (GEN && A.coarse) {
 #pragma unroll
     for (int n = 0; n < 5; n++) {
       T qv;
@@ -162,15 +166,15 @@ __device__ inline void finish_cell(const StageArgs<T> &A, long q, const T *f, co
     return;
   }
   // de = fn*vdtim/V, or fn*dtim/V with the global step: one division
-  const T dt = A.vt == 1 ? T(A.dtim[0]) : (PRE ? vdt : A.vdtim[q]);
-  const T de = A.fn * dt / (A.vol ? A.vol[q] : A.cellvolume);
+  const T dt = GEN && A.vt == 1 ? T(A.dtim[0]) : (PRE ? vdt : A.vdtim[q]);
+  const T de = A.fn * dt / (GEN && A.vol ? A.vol[q] : A.cellvolume);
 #pragma unroll
   for (int n = 0; n < 5; n++) r[n] = de * r[n];
 #pragma unroll
   for (int n = 0; n < 5; n++) A.dw[n * E + q] = r[n];
   if (!A.smoothed) {
     T v[5];
-    if (A.ulam == 2) {
+    if (GEN && A.ulam == 2) {
       const T lam = unsteady_lam(A.fn, A.vdtim, A.dtim, A.vt, A.dtstep, q);
 #pragma unroll
       for (int n = 0; n < 5; n++) r[n] = r[n] / lam;
@@ -420,7 +424,7 @@ __device__ inline RadItem rad_item(const Dims &d, int t, int i0, int j0, bool di
   return h;
 }
 
-template <typename T, int TJs>
+template <typename T, int TJs, bool GEN>
 __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageArgs<T> &A, int gbx, int gby, int gbz) {
   constexpr int NT = TI * TJs;
   // FP64 loads the next plane's halo items during the current plane's
@@ -709,7 +713,7 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
           A.dw[n * E + qk] = e[n];
         }
       } else {
-        finish_cell<T, PRE>(A, qk, f, e, pwn, pvd, pfl);
+        finish_cell<T, PRE, GEN>(A, qk, f, e, pwn, pvd, pfl);
       }
       // advance the k-direction carry to face k+2
       if (more) {
@@ -752,13 +756,13 @@ __device__ __attribute__((always_inline)) inline void k_stage2_body(const StageA
 }
 // entry points as MGE_ENTRY_T_XCD makes them (XCD-contiguous block order in
 // the single-case entry), with launch bounds that follow the tile's rows
-template <typename T, int TJ_>
+template <typename T, int TJ_, bool GEN>
 __global__ void __launch_bounds__(TI * TJ_, Stage2Tile<T>::MINW) k_stage2(StageArgs<T> A) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
-  k_stage2_body<T, TJ_>(A, bx, by, bz);
+  k_stage2_body<T, TJ_, GEN>(A, bx, by, bz);
 }
-template <typename T, int TJ_>
+template <typename T, int TJ_, bool GEN>
 __global__ void __launch_bounds__(TI * TJ_, Stage2Tile<T>::MINW)
     k_stage2_b(const StageArgs<T> *As, const int4 *cg, int nc, const int *act) {
   const int b = (int)blockIdx.x;
@@ -766,7 +770,7 @@ __global__ void __launch_bounds__(TI * TJ_, Stage2Tile<T>::MINW)
   const int4 g = cg[c];
   if (!act[g.w]) return;
   const int l = b - g.x;
-  k_stage2_body<T, TJ_>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
+  k_stage2_body<T, TJ_, GEN>(As[c], l % g.y, (l / g.y) % g.z, l / (g.y * g.z));
 }
 
 // ------------------------------------------------------------------ wall records
