@@ -159,6 +159,9 @@ def lib():
     L.mge_stage_tile.argtypes = [P, I, C.POINTER(I), C.POINTER(I)]
     L.mge_set_tile.argtypes = [P, I, I, I]
     L.mge_set_freeze.argtypes = [P, I, I]
+    L.mge_comp_create.argtypes = [I, C.POINTER(P), C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(P)]
+    L.mge_comp_solve.argtypes = [P, C.POINTER(I), dp, I, C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.mge_comp_destroy.argtypes = [P]
     L.mge_batch_create.argtypes = [I, I, C.POINTER(P)]
     L.mge_batch_destroy.argtypes = [P]
     L.mge_batch_add.argtypes = [P, P, D, I]
@@ -186,7 +189,7 @@ EXPORTED = [
     "mge_kernel_timing", "mge_last_error", "mge_cycles",
     "mge_batch_create", "mge_batch_destroy", "mge_batch_add", "mge_batch_remove", "mge_batch_run",
     "mge_batch_status", "mge_batch_log", "mge_case_bytes", "mge_autotune", "mge_set_kc", "mge_geom_level_tri",
-    "mge_stage_tile", "mge_set_tile", "mge_set_freeze",
+    "mge_stage_tile", "mge_set_tile", "mge_set_freeze", "mge_comp_create", "mge_comp_solve", "mge_comp_destroy",
     "mge_read_job", "mge_job_params", "mge_job_level", "mge_read_pin", "mge_read_p3d", "mge_input_error",
     "mge_write_solut", "mge_read_solut", "mge_solve_step", "mge_shift_time_levels",
     "mge_upload_solut", "mge_upload_solut_noalloc",

@@ -660,6 +660,11 @@ struct XferArgs {
   // the restriction (per coarse index) and prolongation (per fine index)
   const T *vc, *vf;
   const T *rw[3], *pw[3];
+  // embedded blocks: the fine block's cell 2 lies in coarse cell 2 + off[a]
+  // of the coarse block, whose cells 2 .. 1 + nrc[a] it covers (restriction
+  // range); a full-coverage level pair has off = 0 and nrc = the coarse
+  // interior extent
+  int off[3], nrc[3];
 };
 
 // the coarse-level correction w - ws of every coarse cell, formed once so
@@ -692,15 +697,17 @@ __device__ inline void pstencil(int i, int r, int *s, T *f) {
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_restrict_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
-  int i = 2 + bx * blockDim.x + threadIdx.x;
-  int j = 2 + by * blockDim.y + threadIdx.y;
-  int k = 2 + bz;
-  if (i > c.il || j > c.jl || k > c.kl) return;
+  // (ii, jj, kk): the coarse cell in the fine block's own coarse numbering
+  const int ii = 2 + bx * blockDim.x + threadIdx.x;
+  const int jj = 2 + by * blockDim.y + threadIdx.y;
+  const int kk = 2 + bz;
+  if (ii > 1 + X.nrc[0] || jj > 1 + X.nrc[1] || kk > 1 + X.nrc[2]) return;
+  const int i = ii + X.off[0], j = jj + X.off[1], k = kk + X.off[2];
   int s[3];
   T fa[2], fb[2], fc[2], fac[8];
-  rstencil(i, X.rx, &s[0], fa);
-  rstencil(j, X.ry, &s[1], fb);
-  rstencil(k, X.rz, &s[2], fc);
+  rstencil(ii, X.rx, &s[0], fa);
+  rstencil(jj, X.ry, &s[1], fb);
+  rstencil(kk, X.rz, &s[2], fc);
   if (X.rw[0]) {
     fa[0] = X.rw[0][2 * i]; fa[1] = X.rw[0][2 * i + 1];
     fb[0] = X.rw[1][2 * j]; fb[1] = X.rw[1][2 * j + 1];
@@ -750,7 +757,8 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   long qf = ix(f, i, j, k);
   if (!(X.flagsf[qf] & F_FLUID)) return;
   long qc[8];
-  for (int c8 = 0; c8 < 8; c8++) qc[c8] = ix(c, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+  for (int c8 = 0; c8 < 8; c8++)
+    qc[c8] = ix(c, X.off[0] + s[0] + (c8 & 1), X.off[1] + s[1] + ((c8 >> 1) & 1), X.off[2] + s[2] + ((c8 >> 2) & 1));
   const T *src = X.fmg ? X.wc : X.corr;
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
@@ -760,6 +768,87 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   }
 }
 MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
+
+// bc_embed.f90:50-143 for an aligned ratio-2 embedding: every ghost cell of
+// the fine block (outside its cells 2..il, 2..jl, 2..kl) whose centre lies
+// inside the coarse block's cells takes the cell-centre trilinear
+// interpolation of the coarse state (the prolongation stencil, through
+// off[]) where the fine cell is fluid, and p from the EOS of its state
+// (depvars); ghost cells whose centre falls outside the coarse block's cells
+// are left as they are (eval_x2cell's error return)
+template <typename T>
+struct EmbedArgs {
+  Dims df, dc;
+  State<T> sf;       // fine block state (ghosts written)
+  StateC<T> sc;      // coarse block state
+  const uint8_t *flagsf;
+  int off[3];        // as XferArgs::off
+  int nb[3];         // the other block's extent for k_overlap (cells 2 .. 1 + nb)
+  T gamma;
+  Inf<T> inf;
+};
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_embed_body(const EmbedArgs<T> &X, int bx, int, int) {
+  const Dims &f = X.df, &c = X.dc;
+  const long q = bx * (long)blockDim.x + threadIdx.x;
+  if (q >= f.E) return;
+  const int i = (int)(q % f.sy), j = (int)((q / f.sy) % (f.jb + 1)), k = (int)(q / f.sz);
+  if (i >= 2 && i <= f.il && j >= 2 && j <= f.jl && k >= 2 && k <= f.kl) return;   // a cell of the block
+  const int id[3] = {i, j, k};
+  const int cl[3] = {c.il, c.jl, c.kl};
+  int s[3];
+  T fw[3][2];
+  for (int a = 0; a < 3; a++) {
+    // the coarse cell holding the fine centre: fine cells 2m-2, 2m-1 of the
+    // block's own coarse numbering lie in its cell m (fine 0, 1 in 1)
+    const int mc = X.off[a] + 2 + (id[a] >= 2 ? (id[a] - 2) / 2 : -1);
+    if (mc < 2 || mc > cl[a]) return;
+    pstencil(id[a], 2, &s[a], fw[a]);
+  }
+  T fac[8];
+  for (int c8 = 0; c8 < 8; c8++) fac[c8] = fw[0][c8 & 1] * fw[1][(c8 >> 1) & 1] * fw[2][(c8 >> 2) & 1];
+  T v[5];
+  if (X.flagsf[q] & F_FLUID) {
+    for (int n = 0; n < 5; n++) {
+      T acc = T(0);
+      for (int c8 = 0; c8 < 8; c8++) {
+        const long qc = ix(c, X.off[0] + s[0] + (c8 & 1), X.off[1] + s[1] + ((c8 >> 1) & 1),
+                           X.off[2] + s[2] + ((c8 >> 2) & 1));
+        acc = acc + X.sc.w[n * c.E + qc] * fac[c8];
+      }
+      v[n] = acc;
+      X.sf.w[n * f.E + q] = acc;
+    }
+  } else {
+    for (int n = 0; n < 5; n++) v[n] = X.sf.w[n * f.E + q];
+  }
+  X.sf.p[q] = eos_st(X.gamma, X.inf, v);
+}
+MGE_ENTRY_T(k_embed, EmbedArgs, 256, 1)
+
+// bc_overlap.f90:32-90 for blocks of one level on one lattice: every ghost
+// cell of the block (df, sf) whose cell lies among the cells 2 .. 1 + nb of
+// the neighbour (dc, sc) -- its index there is its own plus off[] -- takes
+// the neighbour's state (the trilinear weights of coinciding centres are 1
+// and 0) and p from the EOS of it (depvars)
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_overlap_body(const EmbedArgs<T> &X, int bx, int, int) {
+  const Dims &f = X.df, &c = X.dc;
+  const long q = bx * (long)blockDim.x + threadIdx.x;
+  if (q >= f.E) return;
+  const int i = (int)(q % f.sy), j = (int)((q / f.sy) % (f.jb + 1)), k = (int)(q / f.sz);
+  if (i >= 2 && i <= f.il && j >= 2 && j <= f.jl && k >= 2 && k <= f.kl) return;
+  const int in = i + X.off[0], jn = j + X.off[1], kn = k + X.off[2];
+  if (in < 2 || in > 1 + X.nb[0] || jn < 2 || jn > 1 + X.nb[1] || kn < 2 || kn > 1 + X.nb[2]) return;
+  const long qn = ix(c, in, jn, kn);
+  T v[5];
+  for (int n = 0; n < 5; n++) {
+    v[n] = X.sc.w[n * c.E + qn];
+    X.sf.w[n * f.E + q] = v[n];
+  }
+  X.sf.p[q] = eos_st(X.gamma, X.inf, v);
+}
+MGE_ENTRY_T(k_overlap, EmbedArgs, 256, 1)
 
 // wall force: momentum flux of the scheme's wall faces (efluxb.f90:66-68
 // for JST; the full Roe wall flux, central + upwind, for the Roe schemes)

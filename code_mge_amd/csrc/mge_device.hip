@@ -812,6 +812,7 @@ int do_movco(mge_case *h, int mgl) {
   X.fmg = 0;
   X.flagsf = f.flags;
   X.wfo = nullptr;
+  X.nrc[0] = c.d.ni; X.nrc[1] = c.d.nj; X.nrc[2] = c.d.nk;
   LAUNCHT(h, "k_restrict", k_restrict, T, cell_grid(c.d), cell_block(), X);
   LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(c.d.E), dim3(256), (EosArgs<T>{c.d, stm<T>(c, c.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
   return do_bcond<T>(h, c, c.cur);
@@ -2571,3 +2572,6 @@ int mge_kernel_timing(mge_case *h, int enable, const char *name, double *ms, lon
 }
 
 }  // extern "C"
+
+// embedded multi-block hierarchies (composite grids)
+#include "mge_comp.h"

@@ -325,6 +325,32 @@ int mge_stage_tile(mge_case *h, int lvl, int *tj, int *kc);
 /* force the tile rows (4 or 8; 0: the precision's default) and k-chunk
  * length (0: the fill rule) of level lvl (no counterpart in the reference) */
 int mge_set_tile(mge_case *h, int lvl, int tj, int kc);
+/* ---- embedded multi-block hierarchies (composite grids) ----------------
+ * Several blocks per multigrid level, each finer level covering part of the
+ * coarser one (the reference's embedded grids: bc_embed.f90:50-143,
+ * bc_overlap.f90:32-90, partial-coverage collc.f90:58-146 and
+ * addw.f90:84-188 inside movco.f90:21-49 / movfin.f90:20-40, with the block
+ * loops of solver.f90:119-261).  Each block is a one-level case (its own
+ * mge_case_create + mge_level_define, one level, the same parameters);
+ * block i has composite level level[i] (1 = coarsest), parent[i] (the
+ * block of level - 1 holding it, -1 on level 1) and g0[3i..3i+2], the index
+ * of its cell 2 on its level's lattice (the level-1 lattice origin at 0;
+ * each level refines the one below by 2 in every direction, so a child's
+ * g0 and extents are even and lie within twice its parent's).  A face of a
+ * child strictly inside its parent has bctype 0 (filled by bc_embed and
+ * bc_overlap); a face on its parent's boundary keeps the domain condition.
+ * While in a composite the blocks run on its stream and must not be used
+ * on their own. */
+typedef struct mge_comp mge_comp;
+int mge_comp_create(int nblock, mge_case *const *blocks, const int *level, const int *parent, const int *g0,
+                    mge_comp **out);
+/* solver.f90:119-261 over the composite (steady): mcyc[l] cycles with the
+ * finest active level l + 1, the FMG transfer after each; the residual of
+ * a cycle is the smallest over the active level's blocks (solver.f90:164-179);
+ * reslog triples (level, cycle, residual) as mge_solve */
+int mge_comp_solve(mge_comp *c, const int *mcyc, double *reslog, int cap, int *nlog, long long *cell_stages);
+/* release the composite; its blocks return to their own streams */
+int mge_comp_destroy(mge_comp *c);
 const char *mge_last_error(void);
 
 /* ---- batches of cases --------------------------------------------------

@@ -2458,3 +2458,300 @@ int orc_visc(orc_case *c, int lvl, double cdis) {
   visc_fluxes(b, &c->prm, cdis);
   return ORC_OK;
 }
+
+/* ------------------------------------------------------------ embedded multi-block hierarchies
+ *
+ * Several blocks per multigrid level (each a one-level orc_case), every
+ * block of level L > 1 inside one block of level L-1 on the lattice refined
+ * by 2; restated from bc_embed.f90:50-143, bc_overlap.f90:32-90,
+ * collc.f90:58-146, addw.f90:84-188, movco.f90:21-49, movfin.f90:20-40 and
+ * the block loops of solver.f90:119-261.  The cell location helpers of the
+ * reference (cell_center, eval_x2cell, shft_cell, trilinear_factor) are
+ * absent from its tree; on aligned lattices their weights are the integer
+ * stencils above (1/2; 1/4 and 3/4; 1 for coinciding centres), a stencil
+ * next to the parent's boundary reading the parent's ghost cell. */
+
+typedef struct {
+  orc_case **c;
+  int nb, nlevel;
+  const int *level, *parent, *g0;
+} comp_t;
+
+static blk_t *cb(const comp_t *C, int i) { return &C->c[i]->lv[1]; }
+static int c_off(const comp_t *C, int f, int a) { return C->g0[3 * f + a] / 2 - C->g0[3 * C->parent[f] + a]; }
+
+/* bc_overlap.f90:32-90 on level L: ghost cells of each block lying among a
+ * same-level neighbour's cells take its state; p from the EOS (depvars) */
+static void comp_overlap(const comp_t *C, int L) {
+  for (int a = 0; a < C->nb; a++) {
+    if (C->level[a] != L) continue;
+    blk_t *A = cb(C, a);
+    for (int n = 0; n < C->nb; n++) {
+      if (n == a || C->level[n] != L) continue;
+      blk_t *N = cb(C, n);
+      int off[3];
+      for (int d = 0; d < 3; d++) off[d] = C->g0[3 * a + d] - C->g0[3 * n + d];
+      for (int k = 0; k <= A->kb; k++)
+        for (int j = 0; j <= A->jb; j++)
+          for (int i = 0; i <= A->ib; i++) {
+            if (i >= 2 && i <= A->il && j >= 2 && j <= A->jl && k >= 2 && k <= A->kl) continue;
+            int in = i + off[0], jn = j + off[1], kn = k + off[2];
+            if (in < 2 || in > N->il || jn < 2 || jn > N->jl || kn < 2 || kn > N->kl) continue;
+            long q = IX(A, i, j, k), qn = IX(N, in, jn, kn);
+            for (int m = 0; m < 5; m++) A->w[(long)m * A->E + q] = N->w[(long)m * N->E + qn];
+            eos_cell(A, C->c[a]->prm.gamma, q);
+          }
+    }
+  }
+}
+
+/* bc_embed.f90:50-143 on level L: ghost cells of each block whose centre
+ * lies in one of its parent's cells take the trilinear interpolation of the
+ * parent's state where fluid; p from the EOS of every such cell */
+static void comp_embed(const comp_t *C, int L) {
+  for (int f = 0; f < C->nb; f++) {
+    if (C->level[f] != L || C->parent[f] < 0) continue;
+    blk_t *F = cb(C, f), *P = cb(C, C->parent[f]);
+    int off[3];
+    for (int d = 0; d < 3; d++) off[d] = c_off(C, f, d);
+    const int pl[3] = {P->il, P->jl, P->kl};
+    for (int k = 0; k <= F->kb; k++)
+      for (int j = 0; j <= F->jb; j++)
+        for (int i = 0; i <= F->ib; i++) {
+          if (i >= 2 && i <= F->il && j >= 2 && j <= F->jl && k >= 2 && k <= F->kl) continue;
+          const int id[3] = {i, j, k};
+          int s[3], inside = 1;
+          double w2[3][2];
+          for (int d = 0; d < 3; d++) {
+            int mc = off[d] + 2 + (id[d] >= 2 ? (id[d] - 2) / 2 : -1);
+            if (mc < 2 || mc > pl[d]) inside = 0;
+            prolong_stencil(id[d], 2, &s[d], &w2[d][0], &w2[d][1]);
+          }
+          if (!inside) continue;
+          double fac[8];
+          tri_fac(w2[0], w2[1], w2[2], fac);
+          long q = IX(F, i, j, k);
+          if (F->blank[q] != 0) {
+            for (int m = 0; m < 5; m++) {
+              double acc = 0.0;
+              for (int c8 = 0; c8 < 8; c8++) {
+                long qc = IX(P, off[0] + s[0] + (c8 & 1), off[1] + s[1] + ((c8 >> 1) & 1),
+                             off[2] + s[2] + ((c8 >> 2) & 1));
+                acc = acc + P->w[(long)m * P->E + qc] * fac[c8];
+              }
+              F->w[(long)m * F->E + q] = acc;
+            }
+          }
+          eos_cell(F, C->c[f]->prm.gamma, q);
+        }
+  }
+}
+
+static int comp_bcond(const comp_t *C, int L) {
+  for (int i = 0; i < C->nb; i++)
+    if (C->level[i] == L) CHK(bcond_blk(cb(C, i), &C->c[i]->prm));
+  return ORC_OK;
+}
+
+/* movco.f90:21-49 at coarse level mgl; collc over the cells each child covers */
+static int comp_movco(const comp_t *C, int mgl) {
+  for (int c = 0; c < C->nb; c++) {
+    if (C->level[c] != mgl) continue;
+    blk_t *b = cb(C, c);
+    const long Eb = b->E;
+    for (long q = 0; q < Eb; q++) b->marker[q] = 0;
+    for (int f = 0; f < C->nb; f++) {
+      if (C->parent[f] != c) continue;
+      blk_t *F = cb(C, f);
+      const long Ef = F->E;
+      int off[3];
+      for (int d = 0; d < 3; d++) off[d] = c_off(C, f, d);
+      for (int ii = 2; ii <= 1 + F->ni / 2; ii++)
+        for (int jj = 2; jj <= 1 + F->nj / 2; jj++)
+          for (int kk = 2; kk <= 1 + F->nk / 2; kk++) {
+            int s[3];
+            double fa[2], fb[2], fc[2], fac[8];
+            restrict_stencil(ii, 2, &s[0], &fa[0], &fa[1]);
+            restrict_stencil(jj, 2, &s[1], &fb[0], &fb[1]);
+            restrict_stencil(kk, 2, &s[2], &fc[0], &fc[1]);
+            tri_fac(fa, fb, fc, fac);
+            long qb = IX(b, ii + off[0], jj + off[1], kk + off[2]);
+            for (int n = 0; n < 5; n++) {
+              double acc = 0.0;
+              for (int c8 = 0; c8 < 8; c8++) {
+                long qf = IX(F, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+                double val = F->dw[(long)n * Ef + qf] / VOLQ(F, qf);
+                acc = acc + val * fac[c8];
+              }
+              b->wr[(long)n * Eb + qb] = acc * VOLQ(b, qb);
+            }
+            for (int n = 0; n < 5; n++) {
+              double acc = 0.0;
+              for (int c8 = 0; c8 < 8; c8++) {
+                long qf = IX(F, s[0] + (c8 & 1), s[1] + ((c8 >> 1) & 1), s[2] + ((c8 >> 2) & 1));
+                acc = acc + F->w[(long)n * Ef + qf] * fac[c8];
+              }
+              b->w[(long)n * Eb + qb] = acc;
+            }
+            b->marker[qb] = 1;
+          }
+    }
+    for (long q = 0; q < Eb; q++) eos_cell(b, C->c[c]->prm.gamma, q);
+    for (int i = 2; i <= b->il; i++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int k = 2; k <= b->kl; k++)
+          for (int n = 0; n < 5; n++) {
+            long q = (long)n * Eb + IX(b, i, j, k);
+            b->wn[q] = b->w[q];
+          }
+  }
+  comp_overlap(C, mgl);
+  return comp_bcond(C, mgl);
+}
+
+/* movfin.f90:20-40 at fine level mgl; addw from each block's parent */
+static int comp_movfin(const comp_t *C, int mgl, int mlvl) {
+  for (int f = 0; f < C->nb; f++) {
+    if (C->level[f] != mgl) continue;
+    blk_t *b = cb(C, f), *pc = cb(C, C->parent[f]);
+    const long Eb = b->E, Ec = pc->E;
+    int off[3];
+    for (int d = 0; d < 3; d++) off[d] = c_off(C, f, d);
+    double *store = (double *)malloc(sizeof(double) * (size_t)Eb * 5);
+    memcpy(store, b->w, sizeof(double) * (size_t)Eb * 5);
+    for (int i = 2; i <= b->il; i++)
+      for (int j = 2; j <= b->jl; j++)
+        for (int k = 2; k <= b->kl; k++) {
+          int s[3];
+          double fa[2], fb[2], fc[2], fac[8];
+          prolong_stencil(i, 2, &s[0], &fa[0], &fa[1]);
+          prolong_stencil(j, 2, &s[1], &fb[0], &fb[1]);
+          prolong_stencil(k, 2, &s[2], &fc[0], &fc[1]);
+          tri_fac(fa, fb, fc, fac);
+          long qb = IX(b, i, j, k);
+          double dw[5];
+          for (int n = 0; n < 5; n++) {
+            double acc = 0.0;
+            for (int c8 = 0; c8 < 8; c8++) {
+              long qc = IX(pc, off[0] + s[0] + (c8 & 1), off[1] + s[1] + ((c8 >> 1) & 1),
+                           off[2] + s[2] + ((c8 >> 2) & 1));
+              double val = pc->w[(long)n * Ec + qc];
+              if (mgl - 1 != mlvl) val = val - pc->ws[(long)n * Ec + qc];
+              acc = acc + val * fac[c8];
+            }
+            dw[n] = acc;
+          }
+          if (b->blank[qb] != 0)
+            for (int n = 0; n < 5; n++) {
+              long q = (long)n * Eb + qb;
+              if (mgl - 1 == mlvl) b->w[q] = dw[n];
+              else b->w[q] = store[q] + dw[n];
+            }
+        }
+    free(store);
+    for (long q = 0; q < Eb; q++) eos_cell(b, C->c[f]->prm.gamma, q);
+  }
+  comp_embed(C, mgl);
+  comp_overlap(C, mgl);
+  return comp_bcond(C, mgl);
+}
+
+/* timestep + euler of every block of level mgl, each followed by the
+ * level's bc_overlap (solver.f90:151-157, 196-201, 214-221) */
+static int comp_relax(const comp_t *C, int mgl, int ncyc, int mlvl, int lres, int lfir, long long *cs) {
+  for (int i = 0; i < C->nb; i++) {
+    if (C->level[i] != mgl) continue;
+    blk_t *b = cb(C, i);
+    if (!lres) CHK(orc_timestep(C->c[i], 1));
+    CHK(orc_euler(C->c[i], 1, 0, ncyc, mgl, mlvl, lres, lfir));
+    *cs += (long long)(lres ? 1 : C->c[i]->prm.mstage) * b->ni * b->nj * b->nk;
+    comp_overlap(C, mgl);
+  }
+  return ORC_OK;
+}
+
+/* one transfer of the composite on its own (tests): op 0 bc_overlap of
+ * level L, 1 bc_embed of level L, 2 movco to coarse level L, 3 movfin to
+ * fine level L with finest active level mlvl */
+int orc_comp_op(orc_case **cases, int nb, const int *level, const int *parent, const int *g0, int op, int L,
+                int mlvl) {
+  comp_t C = {cases, nb, 0, level, parent, g0};
+  for (int i = 0; i < nb; i++) {
+    if (cases[i]->nlevel != 1) return ORC_ERR_ARG;
+    if (level[i] > C.nlevel) C.nlevel = level[i];
+  }
+  if (L < 1 || L > C.nlevel) return ORC_ERR_ARG;
+  switch (op) {
+    case 0: comp_overlap(&C, L); return ORC_OK;
+    case 1: comp_embed(&C, L); return ORC_OK;
+    case 2: return L < C.nlevel ? comp_movco(&C, L) : ORC_ERR_ARG;
+    case 3: return L > 1 ? comp_movfin(&C, L, mlvl) : ORC_ERR_ARG;
+  }
+  return ORC_ERR_ARG;
+}
+
+int orc_comp_solve(orc_case **cases, int nb, const int *level, const int *parent, const int *g0, const int *mcyc,
+                   double *reslog, int cap, int *nlog, long long *cell_stages) {
+  comp_t C = {cases, nb, 0, level, parent, g0};
+  for (int i = 0; i < nb; i++) {
+    if (cases[i]->nlevel != 1) return ORC_ERR_ARG;
+    if (level[i] > C.nlevel) C.nlevel = level[i];
+  }
+  const orc_params *P = &cases[0]->prm;
+  int nl = 0, icyc[64];
+  long long cs = 0;
+  double ini_res = 0.0, max_res = 0.0;
+  for (int mlvl = 1; mlvl <= C.nlevel; mlvl++) {
+    int mgl = mlvl;
+    for (int ncyc = 1; ncyc <= mcyc[mlvl - 1]; ncyc++) {
+      for (int l = 0; l <= C.nlevel; l++) icyc[l] = 0;
+      int next_cycle = 0;
+      for (;;) { /* lopmg */
+        for (;;) { /* lopr */
+          icyc[mgl] = icyc[mgl] + 1;
+          int lfir = (mgl == mlvl) ? 0 : 1;
+          CHK(comp_relax(&C, mgl, ncyc, mlvl, 0, lfir, &cs));
+          if (mgl == mlvl) {
+            max_res = 1000.0;
+            for (int i = 0; i < nb; i++) {
+              if (level[i] != mgl) continue;
+              double r = cb(&C, i)->rtrms[ncyc];
+              if (ncyc == 1) ini_res = r;
+              if (r < max_res) max_res = r;
+              if (r != r) return ORC_ERR_RES_NAN;
+            }
+            if (reslog && nl < cap) {
+              reslog[3 * nl] = mgl;
+              reslog[3 * nl + 1] = ncyc;
+              reslog[3 * nl + 2] = max_res;
+            }
+            nl++;
+          }
+          if (mlvl == 1) { next_cycle = 1; break; }
+          CHK(comp_relax(&C, mgl, ncyc, mlvl, 1, 0, &cs));
+          mgl = mgl - 1;
+          CHK(comp_movco(&C, mgl));
+          if (mgl == 1) break;
+        }
+        if (next_cycle) break;
+        int lfir = 1, exit_mg = 0;
+        for (;;) { /* lopp */
+          CHK(comp_relax(&C, mgl, ncyc, mlvl, 0, lfir, &cs));
+          lfir = 0;
+          mgl = mgl + 1;
+          CHK(comp_movfin(&C, mgl, mlvl));
+          if (mgl == mlvl) { exit_mg = 1; break; }
+          if (icyc[mgl] != P->mgtype) break;
+          icyc[mgl] = 0;
+        }
+        if (exit_mg) break;
+      }
+      if (next_cycle) continue;
+      if (mgl == mlvl && ini_res / max_res > P->drop) break;
+    }
+    if (mlvl < C.nlevel) CHK(comp_movfin(&C, mlvl + 1, mlvl));
+  }
+  if (nlog) *nlog = nl;
+  if (cell_stages) *cell_stages = cs;
+  return ORC_OK;
+}

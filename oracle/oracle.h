@@ -137,6 +137,12 @@ void orc_dflux_matrix(double gamma, const double *u1, const double *u2, const do
 int    orc_get(orc_case *c, int lvl, const char *name, double *out);
 void   orc_set_frozen(orc_case *c, int far, int lim);   /* freeze flags, set directly (tests) */
 int    orc_cycles(orc_case *c, int mlvl, int ncyc0, int count, double *res, long long *cell_stages);
+/* embedded multi-block hierarchy over one-level cases (solver.f90:119-261
+ * with bc_embed, bc_overlap and partial-coverage collc/addw) */
+int    orc_comp_solve(orc_case **cases, int nb, const int *level, const int *parent, const int *g0,
+                      const int *mcyc, double *reslog, int cap, int *nlog, long long *cell_stages);
+int    orc_comp_op(orc_case **cases, int nb, const int *level, const int *parent, const int *g0, int op,
+                   int L, int mlvl);
 int    orc_set(orc_case *c, int lvl, const char *name, const double *in);
 int    orc_get_cuts(orc_case *c, int lvl, orc_cut *out);
 double orc_get_rtrms(orc_case *c, int lvl, int ncyc);

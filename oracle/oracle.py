@@ -75,6 +75,9 @@ def lib():
     L.orc_get.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_set_frozen.argtypes = [P, I, I]
     L.orc_cycles.argtypes = [P, I, I, I, _dp, C.POINTER(C.c_longlong)]
+    L.orc_comp_solve.argtypes = [C.POINTER(P), I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), _dp, I,
+                                 C.POINTER(I), C.POINTER(C.c_longlong)]
+    L.orc_comp_op.argtypes = [C.POINTER(P), I, C.POINTER(I), C.POINTER(I), C.POINTER(I), I, I, I]
     L.orc_set_frozen.restype = None
     L.orc_set.argtypes = [P, I, C.c_char_p, _dp]
     L.orc_get_cuts.argtypes = [P, I, C.c_void_p]
@@ -317,3 +320,40 @@ def visc_face(direction, pL, pR, t1L, t1R, t2L, t2R, hn, S, mu, kap):
     g = np.zeros(5)
     L.orc_visc_face(direction, *[x.ctypes.data_as(_dp) for x in a], hn, S, mu, kap, g.ctypes.data_as(_dp))
     return g
+
+
+def comp_solve(cases, blocks, mcyc, cap=100000):
+    """the embedded multi-block solve (orc_comp_solve) over one-level
+    OracleCases; blocks carry level, parent and g0 as
+    code_mge_amd.composite.BlockSpec"""
+    L = lib()
+    nb = len(cases)
+    hs = (C.c_void_p * nb)(*[c.h for c in cases])
+    lev = (C.c_int * nb)(*[b.level for b in blocks])
+    par = (C.c_int * nb)(*[b.parent for b in blocks])
+    g0 = (C.c_int * (3 * nb))(*[v for b in blocks for v in b.g0])
+    nlevel = max(b.level for b in blocks)
+    mc = (C.c_int * nlevel)(*mcyc)
+    log = np.zeros(3 * cap)
+    nlog = C.c_int(0)
+    cs = C.c_longlong(0)
+    rc = L.orc_comp_solve(hs, nb, lev, par, g0, mc, log.ctypes.data_as(_dp), cap, C.byref(nlog), C.byref(cs))
+    if rc < 0:
+        raise RuntimeError(f"oracle comp_solve failed: {rc}")
+    n = min(nlog.value, cap)
+    return log[: 3 * n].reshape(n, 3), cs.value
+
+
+def comp_op(cases, blocks, op, L, mlvl=0):
+    """one composite transfer alone (orc_comp_op): "overlap" or "embed" on
+    level L, "movco" to coarse level L, "movfin" to fine level L"""
+    L_ = lib()
+    nb = len(cases)
+    hs = (C.c_void_p * nb)(*[c.h for c in cases])
+    lev = (C.c_int * nb)(*[b.level for b in blocks])
+    par = (C.c_int * nb)(*[b.parent for b in blocks])
+    g0 = (C.c_int * (3 * nb))(*[v for b in blocks for v in b.g0])
+    code = {"overlap": 0, "embed": 1, "movco": 2, "movfin": 3}[op]
+    rc = L_.orc_comp_op(hs, nb, lev, par, g0, code, L, mlvl)
+    if rc < 0:
+        raise RuntimeError(f"oracle comp_op {op} failed: {rc}")
