@@ -308,142 +308,274 @@ MGE_ENTRY_T(k_bc_cut, CutArgs, 64, 1)
 
 // ------------------------------------------------------------------ bcond
 
+// ---- the six faces of bcond in one launch
+//
+// bcond.f90:16-39 runs the faces one after another in place, each over its
+// whole (n+4)^2 face, so a ghost cell beyond two or three faces (an edge or
+// corner of the ghost shell) is written by each of them and keeps the value
+// of the last, the highest face number f.  Face f read that cell's sources
+// (phy1/phy2 along its own direction) when every other face writing them had
+// run, since their writers are the cell's other faces, all numbered below f.
+// So the value bcond leaves in a ghost cell is a chain: start from cells no
+// face writes, apply the cell's faces in increasing order, each to the
+// source cells of the previous.  k_bc_faces gives every ghost cell of the
+// shell a thread that evaluates its chain (at most three faces deep),
+// reading only cells that no face writes, so the threads are independent.
+// The chain argument needs every source to be a cell with fewer writers,
+// which fails where a direction has one cell (phy2 = nl - 1 is then a ghost
+// cell); such a level runs the six faces one after another inside a single
+// workgroup instead.  Far-field branch bytes are recorded where the face is
+// the cell's last writer: those are the only positions a frozen solve reads.
 template <typename T>
-struct BcArgs {
+struct FacesArgs {
   Dims d;
   State<T> s;
-  int face;      // 1..6
-  int kind;      // -20 far, -50 symmetry, -60 periodic, -80 wall
+  int kind[6];        // faces 1..6: -20 far, -50 symmetry, -60 periodic, -80 wall, 0 not written
+  int seq;            // 1: faces in order inside one workgroup (a direction with one cell)
   double gamma, rho0, p0, c0, u0, v0, w0;
   Inf<T> inf;
-  uint8_t *ffdec;     // far field: this face's branch per ghost point (1 outflow), or nullptr
-  const int *frozen;  // the case's freeze flag: the branch is read from ffdec, not chosen
+  uint8_t *ffdec[6];  // far field: each face's branch per ghost point, or nullptr
+  const int *frozen;
 };
 
-// one face of bcond.f90:16-39 -> bc_far.f90 / bc_symm.f90 / bc_periodic.f90 /
-// bc_wall.f90, over the whole (n+4)^2 face like the reference loops.
-// Evaluated in FP64 on full state values (FP32 stores deviations).
 template <typename T>
-__device__ __attribute__((always_inline)) inline void k_bc_face_body(const BcArgs<T> &B, int bx, int by, int) {
-  const Dims &d = B.d;
-  const Inf<T> &f = B.inf;
-  int dir = (B.face - 1) % 3;
-  int amax = dir == 0 ? d.jb : d.ib;
-  int cmax = dir == 2 ? d.jb : d.kb;
-  int a = bx * blockDim.x + threadIdx.x;
-  int c2 = by;
-  if (a > amax || c2 > cmax) return;
-  int nl = dir == 0 ? d.il : (dir == 1 ? d.jl : d.kl);
-  int ne = nl + 1, nb = nl + 2;
-  auto L = [&](int s) -> long {
-    return dir == 0 ? ix(d, s, a, c2) : (dir == 1 ? ix(d, a, s, c2) : ix(d, a, c2, s));
-  };
-  const long E = d.E;
-  T *W = B.s.w, *P = B.s.p;
-  auto Wf = [&](int n, long q) { return full(f, n, W[n * E + q]); };
-  if (B.kind == -20) {
-    int phy1, dum1, dum2;
-    if (B.face <= 3) { phy1 = 2; dum1 = 1; dum2 = 0; }
-    else { phy1 = nl; dum1 = ne; dum2 = nb; }
-    double fn[3] = {0.0, 0.0, 0.0};
-    fn[dir] = B.face <= 3 ? -1.0 : 1.0;
-    double vinf[3] = {B.u0, B.v0, B.w0};
-    const double gamma = B.gamma, gm = 1.0 / (gamma - 1.0);
-    long q1 = L(phy1), qd1 = L(dum1), qd2 = L(dum2);
-    double s = 0.0;
-    s = s + Wf(1, q1) * fn[0];
-    s = s + Wf(2, q1) * fn[1];
-    s = s + Wf(3, q1) * fn[2];
-    double vel_norm = s / Wf(0, q1);
-    double r, vel[3], pr;
-    bool inflow = vel_norm <= 0.0;
-    if (B.ffdec) {
-      uint8_t &dec = B.ffdec[a + (long)(amax + 1) * c2];
-      if (*B.frozen) inflow = dec == 0;
-      else dec = inflow ? 0 : 1;
+struct CellV {
+  T w[5];
+  T p;
+};
+
+// the highest-numbered face that writes cell (x, y, z) (0: none)
+template <typename T>
+__device__ inline int last_face(const FacesArgs<T> &A, int x, int y, int z) {
+  const Dims &d = A.d;
+  int f = 0;
+  if (x <= 1 && A.kind[0]) f = 1;
+  if (y <= 1 && A.kind[1]) f = 2;
+  if (z <= 1 && A.kind[2]) f = 3;
+  if (x >= d.ie && A.kind[3]) f = 4;
+  if (y >= d.je && A.kind[4]) f = 5;
+  if (z >= d.ke && A.kind[5]) f = 6;
+  return f;
+}
+
+__device__ inline double pick3(int dir, double a, double b, double c) { return dir == 0 ? a : (dir == 1 ? b : c); }
+
+// bc_far.f90:26-284: the ghost state of a far-field face from the state at
+// phy1, in FP64 on full state values (FP32 stores deviations); components
+// are picked by selects rather than indexed by the face direction
+template <typename T>
+__device__ inline CellV<T> far_state(const FacesArgs<T> &A, int f, CellV<T> v, uint8_t *dec,
+                                                        bool rec) {
+  CellV<T> o;
+  const Inf<T> &fi = A.inf;
+  const int dir = (f - 1) % 3;
+  const double sg = f <= 3 ? -1.0 : 1.0;
+  const double fn0 = dir == 0 ? sg : 0.0, fn1 = dir == 1 ? sg : 0.0, fn2 = dir == 2 ? sg : 0.0;
+  const double gamma = A.gamma, gm = 1.0 / (gamma - 1.0);
+  const double w0 = full(fi, 0, v.w[0]), w1 = full(fi, 1, v.w[1]), w2 = full(fi, 2, v.w[2]),
+               w3 = full(fi, 3, v.w[3]);
+  double s = 0.0;
+  s = s + w1 * fn0;
+  s = s + w2 * fn1;
+  s = s + w3 * fn2;
+  const double vel_norm = s / w0;
+  bool inflow = vel_norm <= 0.0;
+  if (dec) {
+    if (*A.frozen) inflow = *dec == 0;
+    else if (rec) *dec = inflow ? 0 : 1;
+  }
+  const double fnd = sg, vinfd = pick3(dir, A.u0, A.v0, A.w0);
+  double r, v0, v1, v2, pr;
+  if (inflow) {
+    double rhoe = w0;
+    double ue = pick3(dir, w1, w2, w3) / rhoe;
+    double pe = full(fi, 5, v.p);
+    double ce = sqrt(gamma * pe / rhoe);
+    double rpo = ue * fnd + 2.0 * ce / (gamma - 1.0);
+    double rmi = vinfd * fnd - 2.0 * A.c0 / (gamma - 1.0);
+    double un = 0.5 * (rpo + rmi) * fnd;
+    double ci = 0.25 * (gamma - 1.0) * (rpo - rmi);
+    v0 = dir == 0 ? un : A.u0;
+    v1 = dir == 1 ? un : A.v0;
+    v2 = dir == 2 ? un : A.w0;
+    double si = pow(A.rho0, gamma) / A.p0;
+    double cc = ci * ci / gamma;
+    r = pow(si * cc, gm);
+    pr = r * cc;
+  } else {
+    double rhoi = w0;
+    double vv0 = w1 / rhoi, vv1 = w2 / rhoi, vv2 = w3 / rhoi;
+    double prei = full(fi, 5, v.p);
+    double si = pow(rhoi, gamma) / prei;
+    double ci = sqrt(gamma * prei / rhoi);
+    double rpo = pick3(dir, vv0, vv1, vv2) * fnd + 2.0 * ci / (gamma - 1.0);
+    double rmi = vinfd * fnd - 2.0 * A.c0 / (gamma - 1.0);
+    double un = 0.5 * (rpo + rmi) * fnd;
+    double ce = 0.25 * (gamma - 1.0) * (rpo - rmi);
+    v0 = dir == 0 ? un : vv0;
+    v1 = dir == 1 ? un : vv1;
+    v2 = dir == 2 ? un : vv2;
+    double cc = ce * ce / gamma;
+    r = pow(si * cc, gm);
+    pr = r * cc;
+  }
+  o.w[0] = stored(fi, 0, r);
+  o.w[1] = stored(fi, 1, r * v0);
+  o.w[2] = stored(fi, 2, r * v1);
+  o.w[3] = stored(fi, 3, r * v2);
+  o.w[4] = stored(fi, 4, pr * gm + 0.5 * r * (v0 * v0 + v1 * v1 + v2 * v2));
+  o.p = stored(fi, 5, pr);
+  return o;
+}
+
+template <typename T, int D>
+__device__ __attribute__((always_inline)) inline void face_chain(const FacesArgs<T> &A, int x, int y, int z,
+                                                                 bool top, CellV<T> &o);
+
+// face f's value in its ghost cell (x, y, z) (bc_far.f90, bc_symm.f90,
+// bc_periodic.f90, bc_wall.f90), its sources taken as face_chain<D> of
+// them; `top`: the thread's own cell (it records the far-field branch)
+template <typename T, int D>
+__device__ __attribute__((always_inline)) inline void face_step(const FacesArgs<T> &A, int f, int x, int y, int z,
+                                                                bool top, CellV<T> &o) {
+  const Dims &d = A.d;
+  const int dir = (f - 1) % 3;
+  const bool lo = f <= 3;
+  const int nl = dir == 0 ? d.il : (dir == 1 ? d.jl : d.kl);
+  const int cd = dir == 0 ? x : (dir == 1 ? y : z);
+  const bool l1 = lo ? cd == 1 : cd == nl + 1;  // first ghost layer
+  const int kind = A.kind[f - 1];
+  // source index along dir: phy1 (far field, the first layer, the wall's
+  // pressure) or phy2 (the second layer)
+  const int p1 = kind == -60 ? (lo ? nl : 2) : (lo ? 2 : nl);
+  const int p2 = kind == -60 ? (lo ? nl - 1 : 3) : (lo ? 3 : nl - 1);
+  const int sd = (kind == -20 || l1) ? p1 : p2;
+  const int sx = dir == 0 ? sd : x, sy = dir == 1 ? sd : y, sz = dir == 2 ? sd : z;
+  face_chain<T, D>(A, sx, sy, sz, false, o);
+  if (kind == -20) {
+    uint8_t *dec = nullptr;
+    if (A.ffdec[f - 1]) {
+      const int a = dir == 0 ? y : x, c2 = dir == 2 ? y : z;
+      const int amax = dir == 0 ? d.jb : d.ib;
+      dec = A.ffdec[f - 1] + a + (long)(amax + 1) * c2;
     }
-    if (inflow) {
-      double rhoe = Wf(0, q1);
-      double ue = Wf(1 + dir, q1) / rhoe;
-      double pe = full(f, 5, P[q1]);
-      double ce = sqrt(gamma * pe / rhoe);
-      double rpo = ue * fn[dir] + 2.0 * ce / (gamma - 1.0);
-      double rmi = vinf[dir] * fn[dir] - 2.0 * B.c0 / (gamma - 1.0);
-      double un = 0.5 * (rpo + rmi) * fn[dir];
-      double ci = 0.25 * (gamma - 1.0) * (rpo - rmi);
-      vel[0] = B.u0; vel[1] = B.v0; vel[2] = B.w0;
-      vel[dir] = un;
-      double si = pow(B.rho0, gamma) / B.p0;
-      double cc = ci * ci / gamma;
-      r = pow(si * cc, gm);
-      pr = r * cc;
-    } else {
-      double rhoi = Wf(0, q1);
-      double vv[3] = {Wf(1, q1) / rhoi, Wf(2, q1) / rhoi, Wf(3, q1) / rhoi};
-      double prei = full(f, 5, P[q1]);
-      double si = pow(rhoi, gamma) / prei;
-      double ci = sqrt(gamma * prei / rhoi);
-      double rpo = vv[dir] * fn[dir] + 2.0 * ci / (gamma - 1.0);
-      double rmi = vinf[dir] * fn[dir] - 2.0 * B.c0 / (gamma - 1.0);
-      double un = 0.5 * (rpo + rmi) * fn[dir];
-      double ce = 0.25 * (gamma - 1.0) * (rpo - rmi);
-      vel[0] = vv[0]; vel[1] = vv[1]; vel[2] = vv[2];
-      vel[dir] = un;
-      double cc = ce * ce / gamma;
-      r = pow(si * cc, gm);
-      pr = r * cc;
-    }
-    double o[5];
-    o[0] = r;
-    o[1] = r * vel[0];
-    o[2] = r * vel[1];
-    o[3] = r * vel[2];
-    o[4] = pr * gm + 0.5 * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
-    for (int n = 0; n < 5; n++) {
-      T v = stored(f, n, o[n]);
-      W[n * E + qd1] = v;
-      W[n * E + qd2] = v;
-    }
-    T pv = stored(f, 5, pr);
-    P[qd1] = pv;
-    P[qd2] = pv;
+    o = far_state(A, f, o, dec, top && l1);
     return;
   }
-  int phy1, phy2, dum1, dum2;
-  if (B.kind == -60) {
-    if (B.face <= 3) { dum1 = 1; dum2 = 0; phy1 = nl; phy2 = nl - 1; }
-    else { phy1 = 2; phy2 = 3; dum1 = ne; dum2 = nb; }
-  } else {
-    if (B.face <= 3) { phy1 = 2; phy2 = 3; dum1 = 1; dum2 = 0; }
-    else { phy1 = nl; phy2 = nl - 1; dum1 = ne; dum2 = nb; }
+  const Inf<T> &fi = A.inf;
+  if (kind == -50 || kind == -80) {
+#pragma unroll
+    for (int n = 1; n < 4; n++)
+      if (kind == -80 || n == 1 + dir) o.w[n] = stored(fi, n, -full(fi, n, o.w[n]));
   }
-  long q1 = L(phy1), q2 = L(phy2), qd1 = L(dum1), qd2 = L(dum2);
-  // a mirrored (negated) component of the full state
-  auto neg = [&](int n, long q) { return stored(f, n, -Wf(n, q)); };
-  if (B.kind == -60) {
-    for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
-    for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
-    P[qd2] = P[q2];
-    P[qd1] = P[q1];
-  } else if (B.kind == -50) {
-    for (int n = 0; n < 5; n++) W[n * E + qd1] = W[n * E + q1];
-    W[(1 + dir) * E + qd1] = neg(1 + dir, q1);
-    P[qd1] = P[q1];
-    for (int n = 0; n < 5; n++) W[n * E + qd2] = W[n * E + q2];
-    W[(1 + dir) * E + qd2] = neg(1 + dir, q2);
-    P[qd2] = P[q2];
-  } else if (B.kind == -80) {
-    W[qd1] = W[q1];
-    W[4 * E + qd1] = W[4 * E + q1];
-    for (int n = 1; n < 4; n++) W[n * E + qd1] = neg(n, q1);
-    W[qd2] = W[q2];
-    W[4 * E + qd2] = W[4 * E + q2];
-    for (int n = 1; n < 4; n++) W[n * E + qd2] = neg(n, q2);
-    P[qd1] = P[q1];
-    P[qd2] = P[q1];
+  if (kind == -80 && !l1) {  // bc_wall: the second ghost layer takes the pressure of phy1
+    CellV<T> v;
+    face_chain<T, D>(A, dir == 0 ? p1 : x, dir == 1 ? p1 : y, dir == 2 ? p1 : z, false, v);
+    o.p = v.p;
   }
 }
-MGE_ENTRY_T(k_bc_face, BcArgs, 256, 1)
+
+// the value bcond leaves in cell (x, y, z), with at most D faces writing it
+template <typename T, int D>
+__device__ __attribute__((always_inline)) inline void face_chain(const FacesArgs<T> &A, int x, int y, int z,
+                                                                 bool top, CellV<T> &o) {
+  if constexpr (D > 0) {
+    const int f = last_face(A, x, y, z);
+    if (f != 0) {
+      face_step<T, D - 1>(A, f, x, y, z, top, o);
+      return;
+    }
+  }
+  const Dims &d = A.d;
+  const long q = ix(d, x, y, z), E = d.E;
+#pragma unroll
+  for (int n = 0; n < 5; n++) o.w[n] = A.s.w[n * E + q];
+  o.p = A.s.p[q];
+}
+
+// ghost cell t of the shell: the k planes {0,1,ke,kb} over all (i,j), then
+// the j rows {0,1,je,jb} over all i and k = 2..kl, then the i columns
+// {0,1,ie,ib} over the interior (j,k)
+__device__ inline bool shell_cell(const Dims &d, long t, int &x, int &y, int &z) {
+  const long ni4 = d.ib + 1, nj4 = d.jb + 1;
+  const long nK = 4 * ni4 * nj4, nJ = 4 * ni4 * d.nk, nI = 4L * d.nj * d.nk;
+  auto g = [](long m, int e) { return m < 2 ? (int)m : e + (int)m - 2; };
+  if (t < nK) {
+    x = (int)(t % ni4);
+    const long r = t / ni4;
+    y = (int)(r % nj4);
+    z = g(r / nj4, d.ke);
+    return true;
+  }
+  t -= nK;
+  if (t < nJ) {
+    x = (int)(t % ni4);
+    const long r = t / ni4;
+    z = 2 + (int)(r % d.nk);
+    y = g(r / d.nk, d.je);
+    return true;
+  }
+  t -= nJ;
+  if (t < nI) {
+    x = g(t % 4, d.ie);
+    const long r = t / 4;
+    y = 2 + (int)(r % d.nj);
+    z = 2 + (int)(r / d.nj);
+    return true;
+  }
+  return false;
+}
+
+__host__ __device__ inline long shell_cells(const Dims &d) {
+  return 4L * (d.ib + 1) * (d.jb + 1) + 4L * (d.ib + 1) * d.nk + 4L * d.nj * d.nk;
+}
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_bc_faces_body(const FacesArgs<T> &A, int bx, int, int) {
+  const Dims &d = A.d;
+  if (A.seq) {  // one workgroup: the faces in order, as bcond.f90
+    for (int f = 1; f <= 6; f++) {
+      if (A.kind[f - 1] == 0) continue;
+      const int dir = (f - 1) % 3;
+      const int nl = dir == 0 ? d.il : (dir == 1 ? d.jl : d.kl);
+      const int g1 = f <= 3 ? 1 : nl + 1, g2 = f <= 3 ? 0 : nl + 2;  // the two ghost layers
+      const int amax = dir == 0 ? d.jb : d.ib, cmax = dir == 2 ? d.jb : d.kb;
+      const long npos = (long)(amax + 1) * (cmax + 1);
+      for (long m = threadIdx.x; m < npos; m += blockDim.x) {
+        const int a = (int)(m % (amax + 1)), c2 = (int)(m / (amax + 1));
+        // both layers from the cells as they stand, then both written (a
+        // source may be the other layer when the direction has one cell)
+        CellV<T> v1, v2;
+        int x1, y1, z1, x2, y2, z2;
+        if (dir == 0) { x1 = g1; x2 = g2; y1 = y2 = a; z1 = z2 = c2; }
+        else if (dir == 1) { y1 = g1; y2 = g2; x1 = x2 = a; z1 = z2 = c2; }
+        else { z1 = g1; z2 = g2; x1 = x2 = a; y1 = y2 = c2; }
+        face_step<T, 0>(A, f, x1, y1, z1, true, v1);
+        face_step<T, 0>(A, f, x2, y2, z2, false, v2);
+        const long E = d.E, q1 = ix(d, x1, y1, z1), q2 = ix(d, x2, y2, z2);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          A.s.w[n * E + q2] = v2.w[n];
+          A.s.w[n * E + q1] = v1.w[n];
+        }
+        A.s.p[q2] = v2.p;
+        A.s.p[q1] = v1.p;
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  const long t = (long)bx * blockDim.x + threadIdx.x;
+  int x, y, z;
+  if (!shell_cell(d, t, x, y, z)) return;
+  if (last_face(A, x, y, z) == 0) return;
+  CellV<T> o;
+  face_chain<T, 3>(A, x, y, z, true, o);
+  const long q = ix(d, x, y, z), E = d.E;
+  for (int n = 0; n < 5; n++) A.s.w[n * E + q] = o.w[n];
+  A.s.p[q] = o.p;
+}
+MGE_ENTRY_T(k_bc_faces, FacesArgs, 256, 1)
 
 // the ordered edge extrapolation of bc_corner.f90 (one target cell)
 template <typename T>

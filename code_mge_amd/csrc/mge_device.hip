@@ -374,34 +374,35 @@ int check_err(mge_case *h) {
 
 // ---- boundary conditions ------------------------------------------------
 
+// bcond.f90:16-39: the six faces in one launch (k_bc_faces), then bc_corner
 template <class T>
 int do_bcond(mge_case *h, Level &L, int buf) {
   const mge_params &P = h->prm;
   int warn = MGE_OK;
+  FacesArgs<T> F{};
+  F.d = L.d;
+  F.s = stm<T>(L, buf);
+  int active = 0;
   for (int f = 1; f <= 6; f++) {
     int t = L.bctype[f - 1];
     if (t == 0 || (t != -20 && t != -50 && t != -60 && t != -80)) {
       if (t != 0) warn = MGE_WARN_BCTYPE;
-      LAUNCHT(h, "k_bc_face", k_bc_face, T, dim3(0), dim3(256), BcArgs<T>{});
       continue;
     }
     if (t == -60 && f <= 3 && L.bctype[f + 2] != -60)
       return set_err(MGE_ERR_PERIODIC, "Periodic boundary must appear in pairs");
-    BcArgs<T> B;
-    B.d = L.d;
-    B.s = stm<T>(L, buf);
-    B.face = f;
-    B.kind = t;
-    B.gamma = P.gamma; B.rho0 = P.rho0; B.p0 = P.p0; B.c0 = P.c0;
-    B.u0 = P.u0; B.v0 = P.v0; B.w0 = P.w0;
-    B.inf = make_inf<T>(P);
-    B.ffdec = (t == -20 && L.ffdec) ? L.ffdec + L.ffoff[f - 1] : nullptr;
-    B.frozen = h->ffrozen;
-    int dir = (f - 1) % 3;
-    int amax = dir == 0 ? L.d.jb : L.d.ib;
-    int cmax = dir == 2 ? L.d.jb : L.d.kb;
-    dim3 g((amax + 1 + 255) / 256, cmax + 1);
-    LAUNCHT(h, "k_bc_face", k_bc_face, T, g, dim3(256), B);
+    F.kind[f - 1] = t;
+    F.ffdec[f - 1] = (t == -20 && L.ffdec) ? L.ffdec + L.ffoff[f - 1] : nullptr;
+    active++;
+  }
+  F.seq = (L.d.ni < 2 || L.d.nj < 2 || L.d.nk < 2) ? 1 : 0;
+  F.gamma = P.gamma; F.rho0 = P.rho0; F.p0 = P.p0; F.c0 = P.c0;
+  F.u0 = P.u0; F.v0 = P.v0; F.w0 = P.w0;
+  F.inf = make_inf<T>(P);
+  F.frozen = h->ffrozen;
+  {
+    const dim3 g = active == 0 ? dim3(0) : (F.seq ? dim3(1) : dim3((unsigned)((shell_cells(L.d) + 255) / 256)));
+    LAUNCHT(h, "k_bc_faces", k_bc_faces, T, g, dim3(256), F);
   }
   {
     const Dims &d = L.d;
