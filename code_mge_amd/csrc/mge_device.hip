@@ -104,6 +104,8 @@ struct Level {
   long *tcell = nullptr;
   int *beg = nullptr;
   int2 *ops = nullptr;
+  int2 *chunks = nullptr;  // k_rec's workgroups (touched-cell ranges)
+  int nchunk = 0;
   double *opval = nullptr;
   double *partial = nullptr;
   long npartial = 0;
@@ -682,12 +684,11 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       R.tcell = L.tcell;
       R.beg = L.beg;
       R.ops = L.ops;
-      R.opval = L.opval;
+      R.chunks = L.chunks;
       R.ntouch = L.ntouch;
       R.nops = L.nops;
       R.err = h->err;
-      LAUNCHT(h, "k_rec_ops", k_rec_ops, T, dim3((L.nops + 63) / 64), dim3(64), R);
-      LAUNCHT(h, "k_rec_apply", k_rec_apply, T, dim3((L.ntouch + 63) / 64), dim3(64), R);
+      LAUNCHT(h, "k_rec", k_rec, T, dim3((unsigned)L.nchunk), dim3(kRecOps), R);
     }
     if (lres) return MGE_OK;
     if (A.smoothed && P.smoother == 1) {
@@ -1217,13 +1218,25 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
   }
   L.ntouch = (int)tcell.size();
   L.nops = (int)opv.size();
+  // workgroups of k_rec: consecutive touched cells with at most kRecOps ops
+  std::vector<int2> chunks;
+  for (int t = 0, t0 = 0; t < L.ntouch; t++) {
+    if (begv[t + 1] - begv[t] > kRecOps) return set_err(MGE_ERR_ARG, "too many wall ops on one cell");
+    if (begv[t + 1] - begv[t0] > kRecOps) {
+      chunks.push_back(make_int2(t0, t));
+      t0 = t;
+    }
+    if (t == L.ntouch - 1) chunks.push_back(make_int2(t0, L.ntouch));
+  }
+  L.nchunk = (int)chunks.size();
   CHKRC(dalloc(L, &L.recs, nmir));
   CHKRC(dalloc(L, &L.wsol, (size_t)5 * nmir));
   CHKRC(dalloc(L, &L.psol, nmir));
   CHKRC(dalloc(L, &L.tcell, tcell.size()));
   CHKRC(dalloc(L, &L.beg, begv.size()));
   CHKRC(dalloc(L, &L.ops, opv.size() + 1));
-  CHKRC(dalloc(L, &L.opval, (opv.size() + 1) * 10));
+  CHKRC(dalloc(L, &L.chunks, chunks.size() + 1));
+  CHKRC(dalloc(L, &L.opval, (opv.size() + 1) * 10));  // the wall force's per-record terms
   HIPCHK(h2d_sync(L.recs, dr.data(), sizeof(DevRec) * nmir, h->stream));
   {
     // cut cells with wall records, in order of their first record, and
@@ -1261,6 +1274,7 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
   HIPCHK(h2d_sync(L.tcell, tcell.data(), sizeof(long) * tcell.size(), h->stream));
   HIPCHK(h2d_sync(L.beg, begv.data(), sizeof(int) * begv.size()));
   if (!opv.empty()) HIPCHK(h2d_sync(L.ops, opv.data(), sizeof(int2) * opv.size()));
+  if (!chunks.empty()) HIPCHK(h2d_sync(L.chunks, chunks.data(), sizeof(int2) * chunks.size()));
   std::vector<uint8_t> fl(d.E);
   HIPCHK(d2h_sync(fl.data(), L.flags, d.E, h->stream));
   for (long q : tcell) fl[q] |= F_TOUCHED;

@@ -5,8 +5,8 @@
 //   k_stage2 / k_stage  fluxes + dissipation + blending + forcing + dt
 //               scaling (+ the update when the stage is not smoothed) for
 //               every cell not touched by a wall record
-//   k_rec_ops / k_rec_apply  the wall corrections of dfluxb.f90/efluxb.f90
-//               in record order for the touched cells, then the same finish
+//   k_rec       the wall corrections of dfluxb.f90/efluxb.f90 in record
+//               order for the touched cells, then the same finish
 // The wall terms that involve the mirror state (wsol/psol, always FP64) are
 // evaluated in FP64 whatever T is.
 #pragma once
@@ -94,10 +94,14 @@ struct RecArgs {
   const long *tcell;        // touched cells
   const int *beg;           // op range of each touched cell (ntouch+1)
   const int2 *ops;          // (record, kind), fw ops then dw ops per cell
-  double *opval;            // 10 doubles per op
+  const int2 *chunks;       // touched cells [x, y) of each workgroup, at most kRecOps ops
   int ntouch, nops;
   int *err;
 };
+
+// ops of one workgroup of k_rec (its threads); no touched cell has more
+// than 36 (6 records as cut cell x 4, 6 as neighbour x 2)
+constexpr int kRecOps = 64;
 
 // the Roe wall-face term of a record (defined with the Roe scheme)
 template <typename T>
@@ -978,16 +982,12 @@ __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
 }
 
-// the value each op contributes (dfluxb.f90 / efluxb.f90 terms), one
-// thread per op; k_rec_apply adds them in the reference order.  The undo
-// terms repeat the bulk kernel's face values in T.
+// the value op o contributes (dfluxb.f90 / efluxb.f90 terms); the undo
+// terms repeat the bulk kernel's face values in T
 template <typename T>
-__device__ __attribute__((always_inline)) inline void k_rec_ops_body(const RecArgs<T> &R, int bx, int, int) {
-  int o = bx * blockDim.x + threadIdx.x;
-  if (o >= R.nops) return;
+__device__ __attribute__((always_inline)) inline void rec_op_value(const RecArgs<T> &R, int o, double *out) {
   const StageArgs<T> &A = R.st;
   int2 op = R.ops[o];
-  double *out = R.opval + (long)o * 10;
   const DevRec &r = R.recs[op.x];
   if (op.y < OP_E_UNDO && !A.has_diss) return;
   int zz = r.zz;
@@ -1058,14 +1058,21 @@ __device__ __attribute__((always_inline)) inline void k_rec_ops_body(const RecAr
       break;
   }
 }
-MGE_ENTRY_T(k_rec_ops, RecArgs, 64, 1)
 
-// per touched cell: apply the op values in record order (in FP64), then
-// finish in T
+// The wall records of a stage: each workgroup takes whole touched cells
+// with their ops (at most kRecOps), one thread per op computes its value
+// into LDS, then one thread per cell adds its ops' values in the reference
+// order (in FP64) and finishes the cell in T.
 template <typename T>
-__device__ __attribute__((always_inline)) inline void k_rec_apply_body(const RecArgs<T> &R, int bx, int, int) {
-  int t = bx * blockDim.x + threadIdx.x;
-  if (t >= R.ntouch) return;
+__device__ __attribute__((always_inline)) inline void k_rec_body(const RecArgs<T> &R, int bx, int, int) {
+  __shared__ double val[kRecOps * 10];
+  const int2 ch = R.chunks[bx];
+  const int o0 = R.beg[ch.x], o1 = R.beg[ch.y];
+  const int tid = threadIdx.x;
+  if (o0 + tid < o1) rec_op_value(R, o0 + tid, val + tid * 10);
+  __syncthreads();
+  const int t = ch.x + tid;
+  if (t >= ch.y) return;
   const StageArgs<T> &A = R.st;
   const Dims &d = A.fx.d;
   const long E = d.E;
@@ -1077,7 +1084,7 @@ __device__ __attribute__((always_inline)) inline void k_rec_apply_body(const Rec
   }
   for (int o = R.beg[t]; o < R.beg[t + 1]; o++) {
     int kind = R.ops[o].y;
-    const double *v = R.opval + (long)o * 10;
+    const double *v = val + (o - o0) * 10;
     switch (kind) {
       case OP_HI_C_UNDO:
       case OP_HI_N_UNDO:
@@ -1116,6 +1123,6 @@ __device__ __attribute__((always_inline)) inline void k_rec_apply_body(const Rec
   }
   finish_cell(A, q, ft, et);
 }
-MGE_ENTRY_T(k_rec_apply, RecArgs, 64, 1)
+MGE_ENTRY_T(k_rec, RecArgs, 64, 1)
 
 }  // namespace mge

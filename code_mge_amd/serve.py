@@ -65,7 +65,8 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False
     padded by 64 bytes); wall-record arrays at their upper bound; a level
     below the finest also holds the prolongation's correction; the
     variable-coefficient smoother holds its coefficients and sweep scratch;
-    limiter freezing three 8-byte words per cell"""
+    limiter freezing three 8-byte words per cell; the wall-record workgroup
+    ranges at one per touched cell at most"""
     ni, nj, nk = n
     E = (ni + 4) * (nj + 4) * (nk + 4)
     nmax = max(ni, nj, nk) + 5
@@ -83,7 +84,7 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False
     if nmir:
         ntouch, nops = 2 * nmir, 6 * nmir
         sizes += [DEVREC_BYTES * nmir, 40 * nmir, 8 * nmir, 8 * ntouch, 4 * (ntouch + 1), 8 * (nops + 1),
-                  80 * (nops + 1)]
+                  8 * (ntouch + 1), 80 * (nops + 1)]
         sizes += [8 * nmir, 4 * (nmir + 1), 4 * nmir]   # cut cells with their record lists
     return sum(s + 64 for s in sizes)
 
