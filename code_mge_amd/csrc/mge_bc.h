@@ -336,6 +336,11 @@ struct FacesArgs {
   Inf<T> inf;
   uint8_t *ffdec[6];  // far field: each face's branch per ghost point, or nullptr
   const int *frozen;
+  // bc_cutcell's mirror states in the same launch (blocks nfb..): it runs
+  // before bcond, reads only interior cells (the level's clouds are checked
+  // on the host) and writes only wsol/psol, so the two are independent
+  long nfb;           // blocks of the faces
+  CutArgs<T> cut;     // cut.nmir == 0: no records in this launch
 };
 
 template <typename T>
@@ -563,6 +568,10 @@ __device__ __attribute__((always_inline)) inline void k_bc_faces_body(const Face
       }
       __syncthreads();
     }
+    return;
+  }
+  if (bx >= A.nfb) {
+    k_bc_cut_body(A.cut, (int)(bx - A.nfb), 0, 0);
     return;
   }
   const long t = (long)bx * blockDim.x + threadIdx.x;

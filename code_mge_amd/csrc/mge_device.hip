@@ -97,6 +97,7 @@ struct Level {
   void *fax = nullptr;  // -bb/e1x
   int npiv = 0;
   int nmir = 0, nupd = 0, zero_at = -1;
+  bool cut_interior = false;  // every cloud cell inside the ghost shell (bcond writes none of them)
   DevRec *recs = nullptr;
   double *wsol = nullptr, *psol = nullptr;
   std::vector<mge_cutrec> hrecs;
@@ -378,7 +379,26 @@ int check_err(mge_case *h) {
 
 // bcond.f90:16-39: the six faces in one launch (k_bc_faces), then bc_corner
 template <class T>
-int do_bcond(mge_case *h, Level &L, int buf) {
+CutArgs<T> cut_args(mge_case *h, Level &L, int buf) {
+  CutArgs<T> C{};
+  C.d = L.d;
+  C.s = stc<T>(L, buf);
+  C.inf = make_inf<T>(h->prm);
+  C.recs = L.recs;
+  C.wsol = L.wsol;
+  C.psol = L.psol;
+  C.nmir = L.nmir;
+  C.nupd = L.nupd;
+  C.zero_at = L.zero_at;
+  C.gamma = h->prm.gamma;
+  C.eqntype = h->prm.eqntype;
+  C.err = h->err;
+  return C;
+}
+
+// bcond; with `cut`, bc_cutcell's records go into the faces' launch
+template <class T>
+int do_bcond(mge_case *h, Level &L, int buf, const CutArgs<T> *cut = nullptr) {
   const mge_params &P = h->prm;
   int warn = MGE_OK;
   FacesArgs<T> F{};
@@ -403,8 +423,13 @@ int do_bcond(mge_case *h, Level &L, int buf) {
   F.inf = make_inf<T>(P);
   F.frozen = h->ffrozen;
   {
-    const dim3 g = active == 0 ? dim3(0) : (F.seq ? dim3(1) : dim3((unsigned)((shell_cells(L.d) + 255) / 256)));
-    LAUNCHT(h, "k_bc_faces", k_bc_faces, T, g, dim3(256), F);
+    F.nfb = active == 0 ? 0 : (F.seq ? 1 : (shell_cells(L.d) + 255) / 256);
+    long ncb = 0;
+    if (cut) {
+      F.cut = *cut;
+      ncb = (cut->nmir + 255) / 256;
+    }
+    LAUNCHT(h, "k_bc_faces", k_bc_faces, T, dim3((unsigned)(F.nfb + ncb)), dim3(256), F);
   }
   {
     const Dims &d = L.d;
@@ -421,21 +446,27 @@ int do_bc_cutcell(mge_case *h, Level &L, int buf) {
     LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3(0), dim3(64), CutArgs<T>{});
     return MGE_OK;
   }
-  CutArgs<T> C;
-  C.d = L.d;
-  C.s = stc<T>(L, buf);
-  C.inf = make_inf<T>(h->prm);
-  C.recs = L.recs;
-  C.wsol = L.wsol;
-  C.psol = L.psol;
-  C.nmir = L.nmir;
-  C.nupd = L.nupd;
-  C.zero_at = L.zero_at;
-  C.gamma = h->prm.gamma;
-  C.eqntype = h->prm.eqntype;
-  C.err = h->err;
-  LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3((L.nmir + 63) / 64), dim3(64), C);
+  LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3((L.nmir + 63) / 64), dim3(64), cut_args<T>(h, L, buf));
   return L.zero_at >= 0 ? MGE_WARN_ZERO_CLOUD : MGE_OK;
+}
+
+// bc_cutcell then bcond, as each RK stage ends (euler.f90): one launch for
+// both when the level's clouds lie in the interior and its faces run in
+// parallel (k_bc_cut keeps its place in the sequence with no blocks, so
+// every case of a batch records the same launches)
+template <class T>
+int do_cut_bcond(mge_case *h, Level &L, int buf, int *warn) {
+  const Dims &d = L.d;
+  const bool seq = d.ni < 2 || d.nj < 2 || d.nk < 2;
+  if (L.nmir == 0 || !L.cut_interior || seq) {
+    int rc = do_bc_cutcell<T>(h, L, buf);
+    if (rc > 0) *warn = rc;
+    return do_bcond<T>(h, L, buf);
+  }
+  LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3(0), dim3(64), CutArgs<T>{});
+  if (L.zero_at >= 0) *warn = MGE_WARN_ZERO_CLOUD;
+  const CutArgs<T> C = cut_args<T>(h, L, buf);
+  return do_bcond<T>(h, L, buf, &C);
 }
 
 // ---- timestep --------------------------------------------------------------
@@ -752,9 +783,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, flat_grid(E), dim3(256),
               (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
     }
-    int rc = do_bc_cutcell<T>(h, L, dst);
-    if (rc > 0) warn = rc;
-    rc = do_bcond<T>(h, L, dst);
+    int rc = do_cut_bcond<T>(h, L, dst, &warn);
     if (rc < 0) return rc;
     if (rc > 0) warn = rc;
     src = dst;
@@ -1148,6 +1177,7 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
       break;
     }
   if (nmir == 0) return MGE_OK;
+  bool interior = true;
   std::vector<DevRec> dr(nmir);
   for (int m = 0; m < nmir; m++) {
     const mge_cutrec &r = recs[m];
@@ -1173,12 +1203,15 @@ int build_records(mge_case *h, Level &L, int nmir, const mge_cutrec *recs) {
     if (s0 < 3 || s0 > nl - 1) return set_err(MGE_ERR_ARG, "wall record too close to the block edge");
     o.npoint = r.npoint;
     for (int c = 0; c < r.npoint; c++) {
-      for (int a = 0; a < 3; a++)
+      for (int a = 0; a < 3; a++) {
         if (r.ijk[c][a] < 0 || r.ijk[c][a] > cmax[a] + 2) return set_err(MGE_ERR_ARG, "cloud point outside");
+        if (r.ijk[c][a] < 2 || r.ijk[c][a] > cmax[a]) interior = false;
+      }
       o.cloud[c] = ix(d, r.ijk[c][0], r.ijk[c][1], r.ijk[c][2]);
       o.coef[c] = r.coef[c];
     }
   }
+  L.cut_interior = interior;
   // ordered op streams per touched cell
   std::map<long, std::pair<std::vector<int2>, std::vector<int2>>> per;
   for (int m = 0; m < nmir; m++) {
