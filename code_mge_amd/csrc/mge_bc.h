@@ -719,13 +719,9 @@ __device__ __attribute__((always_inline)) inline void k_ghost_copy_body(const Gh
   const Dims &d = G.d;
   const StateC<T> &src = G.src;
   const State<T> &dst = G.dst;
-  long q = bx * (long)blockDim.x + threadIdx.x;
-  if (q >= d.E) return;
-  int k = (int)(q / d.sz);
-  int j = (int)((q - (long)k * d.sz) / d.sy);
-  int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
-  bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
-  if (inner) return;
+  int i, j, k;
+  if (!shell_cell(d, bx * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const long q = ix(d, i, j, k);
   for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
   dst.p[q] = src.p[q];
 }

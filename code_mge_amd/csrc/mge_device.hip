@@ -780,7 +780,16 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       U.vt = A.vt;
       LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256), U);
     } else {
-      LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, flat_grid(E), dim3(256),
+      // the ghost shell of the new state is the old one until bcond; when
+      // bcond writes every ghost cell from interior cells (all six faces
+      // written, no one-cell direction) and bc_cutcell's clouds are interior,
+      // nothing reads the copy and it is skipped
+      bool all = !(d.ni < 2 || d.nj < 2 || d.nk < 2) && (L.nmir == 0 || L.cut_interior);
+      for (int f = 0; f < 6; f++) {
+        const int t = L.bctype[f];
+        all = all && (t == -20 || t == -50 || t == -60 || t == -80);
+      }
+      LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, all ? dim3(0) : flat_grid(shell_cells(d)), dim3(256),
               (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
     }
     int rc = do_cut_bcond<T>(h, L, dst, &warn);
