@@ -225,7 +225,8 @@ def run_leg(prm, levels, geoms, aref, args, precision):
                                   f"({levels[-1].n[0]}x{levels[-1].n[1]}x{levels[-1].n[2]})",
                         "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "limiter": limiter(kname, precision, ms, achieved / PEAK_HBM_GBS),
-                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(f"fp{precision}"),
+                        "frac": round(achieved / PEAK_HBM_GBS, 4),
+                        "traffic": load_traffic(("jst_" if kname == "k_stage2" else "") + f"fp{precision}"),
                         "avg_launch_ms": round(ms, 5), "launches": cnt, "alg_bytes_per_launch": round(by),
                         "timed_cycles": nroof,
                         "all_levels": {"avg_launch_ms": round(ms_all, 5), "launches": cnt_all,
