@@ -802,7 +802,39 @@ struct XferArgs {
   // range); a full-coverage level pair has off = 0 and nrc = the coarse
   // interior extent
   int off[3], nrc[3];
+  // eos: the launch also sets p from the EOS on every cell of the target
+  // level (the coarse level of a restriction, the fine level of a
+  // prolongation), as a k_eos_all pass after it would: its own cells from
+  // the values it writes (or, untouched, reads), the ghost shell from blocks
+  // past the cells' grid (gx x gy x gz0 blocks)
+  int eos;
+  T *peos;
+  T gamma;
+  Inf<T> inf;
+  int gx, gy, gz0;
 };
+
+// the EOS of ghost-shell cell t of a level
+template <typename T>
+__device__ inline void eos_shell(const Dims &d, const T *w, T *p, T gamma, const Inf<T> &inf, long t) {
+  int i, j, k;
+  if (!shell_cell(d, t, i, j, k)) return;
+  const long q = ix(d, i, j, k);
+  T v[5];
+#pragma unroll
+  for (int n = 0; n < 5; n++) v[n] = w[n * d.E + q];
+  p[q] = eos_st(gamma, inf, v);
+}
+
+// blocks past the cells' grid: the shell's EOS (true when this block was one)
+template <typename T>
+__device__ inline bool xfer_shell_block(const XferArgs<T> &X, const Dims &d, const T *w, int bx, int by, int bz) {
+  if (!X.eos || bz < X.gz0) return false;
+  const long t = (((long)(bz - X.gz0) * X.gy + by) * X.gx + bx) * (blockDim.x * blockDim.y) + threadIdx.x +
+                 blockDim.x * threadIdx.y;
+  eos_shell(d, w, X.peos, X.gamma, X.inf, t);
+  return true;
+}
 
 // the coarse-level correction w - ws of every coarse cell, formed once so
 // that the prolongation reads one value per coarse neighbour and component
@@ -834,6 +866,7 @@ __device__ inline void pstencil(int i, int r, int *s, T *f) {
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_restrict_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
+  if (xfer_shell_block(X, c, X.wc, bx, by, bz)) return;
   // (ii, jj, kk): the coarse cell in the fine block's own coarse numbering
   const int ii = 2 + bx * blockDim.x + threadIdx.x;
   const int jj = 2 + by * blockDim.y + threadIdx.y;
@@ -862,12 +895,15 @@ __device__ __attribute__((always_inline)) inline void k_restrict_body(const Xfer
     }
     X.wr[n * c.E + qc] = acc * (X.vc ? X.vc[qc] : X.volc);
   }
+  T v[5];
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
     for (int c8 = 0; c8 < 8; c8++) acc = acc + X.wf[n * f.E + qf[c8]] * fac[c8];
     X.wc[n * c.E + qc] = acc;
+    v[n] = acc;
   }
   X.marker[qc] = 1;
+  if (X.eos) X.peos[qc] = eos_st(X.gamma, X.inf, v);
 }
 MGE_ENTRY_T(k_restrict, XferArgs, 256, 1)
 
@@ -876,6 +912,7 @@ MGE_ENTRY_T(k_restrict, XferArgs, 256, 1)
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
+  if (xfer_shell_block(X, f, (const T *)X.wfo, bx, by, bz)) return;
   int i = 2 + bx * blockDim.x + threadIdx.x;
   int j = 2 + by * blockDim.y + threadIdx.y;
   int k = 2 + bz;
@@ -892,7 +929,15 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   }
   for (int c8 = 0; c8 < 8; c8++) fac[c8] = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
   long qf = ix(f, i, j, k);
-  if (!(X.flagsf[qf] & F_FLUID)) return;
+  T v[5];
+  if (!(X.flagsf[qf] & F_FLUID)) {
+    if (X.eos) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) v[n] = X.wfo[n * f.E + qf];
+      X.peos[qf] = eos_st(X.gamma, X.inf, v);
+    }
+    return;
+  }
   long qc[8];
   for (int c8 = 0; c8 < 8; c8++)
     qc[c8] = ix(c, X.off[0] + s[0] + (c8 & 1), X.off[1] + s[1] + ((c8 >> 1) & 1), X.off[2] + s[2] + ((c8 >> 2) & 1));
@@ -900,9 +945,10 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   for (int n = 0; n < 5; n++) {
     T acc = T(0);
     for (int c8 = 0; c8 < 8; c8++) acc = acc + src[n * c.E + qc[c8]] * fac[c8];
-    if (X.fmg) X.wfo[n * f.E + qf] = acc;
-    else X.wfo[n * f.E + qf] = X.wfo[n * f.E + qf] + acc;
+    v[n] = X.fmg ? acc : X.wfo[n * f.E + qf] + acc;
+    X.wfo[n * f.E + qf] = v[n];
   }
+  if (X.eos) X.peos[qf] = eos_st(X.gamma, X.inf, v);
 }
 MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
 

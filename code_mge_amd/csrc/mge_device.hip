@@ -828,6 +828,23 @@ bool levels_nest(const Level &c, const Level &f) {
   return true;
 }
 
+// the grid of a transfer onto level L that also forms p on every cell of L
+// (XferArgs::eos): the cells' grid and then enough z-slices of blocks for
+// the ghost shell
+template <class T>
+dim3 xfer_eos_grid(XferArgs<T> &X, Level &L, const mge_params &P) {
+  const dim3 g = cell_grid(L.d);
+  const long per = (long)g.x * g.y * 256;
+  X.eos = 1;
+  X.peos = tp<T>(L.S[L.cur]) + 5 * L.d.E;
+  X.gamma = (T)P.gamma;
+  X.inf = make_inf<T>(P);
+  X.gx = (int)g.x;
+  X.gy = (int)g.y;
+  X.gz0 = (int)g.z;
+  return dim3(g.x, g.y, g.z + (unsigned)((shell_cells(L.d) + per - 1) / per));
+}
+
 template <class T>
 int do_movco(mge_case *h, int mgl) {
   Level &c = h->lv[mgl], &f = h->lv[mgl + 1];
@@ -853,8 +870,9 @@ int do_movco(mge_case *h, int mgl) {
   X.flagsf = f.flags;
   X.wfo = nullptr;
   X.nrc[0] = c.d.ni; X.nrc[1] = c.d.nj; X.nrc[2] = c.d.nk;
-  LAUNCHT(h, "k_restrict", k_restrict, T, cell_grid(c.d), cell_block(), X);
-  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(c.d.E), dim3(256), (EosArgs<T>{c.d, stm<T>(c, c.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
+  // the restriction covers every coarse cell: it sets their p, and blocks
+  // past its grid the ghost shell's (the EOS pass over the level)
+  LAUNCHT(h, "k_restrict", k_restrict, T, xfer_eos_grid(X, c, h->prm), cell_block(), X);
   return do_bcond<T>(h, c, c.cur);
 }
 
@@ -882,8 +900,7 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.wfo = tp<T>(f.S[f.cur]);
   X.corr = tp<T>(c.corr);
   if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, flat_grid(5 * c.d.E), dim3(256), X);
-  LAUNCHT(h, "k_prolong", k_prolong, T, cell_grid(f.d), cell_block(), X);
-  LAUNCHT(h, "k_eos_all", k_eos_all, T, flat_grid(f.d.E), dim3(256), (EosArgs<T>{f.d, stm<T>(f, f.cur), (T)h->prm.gamma, make_inf<T>(h->prm)}));
+  LAUNCHT(h, "k_prolong", k_prolong, T, xfer_eos_grid(X, f, h->prm), cell_block(), X);
   return do_bcond<T>(h, f, f.cur);
 }
 
