@@ -498,43 +498,6 @@ __device__ __attribute__((always_inline)) inline void face_chain(const FacesArgs
   o.p = A.s.p[q];
 }
 
-// ghost cell t of the shell: the k planes {0,1,ke,kb} over all (i,j), then
-// the j rows {0,1,je,jb} over all i and k = 2..kl, then the i columns
-// {0,1,ie,ib} over the interior (j,k)
-__device__ inline bool shell_cell(const Dims &d, long t, int &x, int &y, int &z) {
-  const long ni4 = d.ib + 1, nj4 = d.jb + 1;
-  const long nK = 4 * ni4 * nj4, nJ = 4 * ni4 * d.nk, nI = 4L * d.nj * d.nk;
-  auto g = [](long m, int e) { return m < 2 ? (int)m : e + (int)m - 2; };
-  if (t < nK) {
-    x = (int)(t % ni4);
-    const long r = t / ni4;
-    y = (int)(r % nj4);
-    z = g(r / nj4, d.ke);
-    return true;
-  }
-  t -= nK;
-  if (t < nJ) {
-    x = (int)(t % ni4);
-    const long r = t / ni4;
-    z = 2 + (int)(r % d.nk);
-    y = g(r / d.nk, d.je);
-    return true;
-  }
-  t -= nJ;
-  if (t < nI) {
-    x = g(t % 4, d.ie);
-    const long r = t / 4;
-    y = 2 + (int)(r % d.nj);
-    z = 2 + (int)(r / d.nj);
-    return true;
-  }
-  return false;
-}
-
-__host__ __device__ inline long shell_cells(const Dims &d) {
-  return 4L * (d.ib + 1) * (d.jb + 1) + 4L * (d.ib + 1) * d.nk + 4L * d.nj * d.nk;
-}
-
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_bc_faces_body(const FacesArgs<T> &A, int bx, int, int) {
   const Dims &d = A.d;

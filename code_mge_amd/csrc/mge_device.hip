@@ -643,6 +643,21 @@ bool stage_gen(const StageArgs<T> &A) {
   return A.coarse || A.tsrc || A.ulam != 0 || A.vol != nullptr || A.vt == 1;
 }
 
+// whether bcond rewrites every ghost cell of the level from interior cells
+// and nothing between a stage's update and bcond reads the ghost shell (all
+// six faces written, no one-cell direction, the cut-cell clouds interior):
+// then a stage need not carry the old ghost shell into its new state
+bool ghosts_rewritten(const Level &L) {
+  const Dims &d = L.d;
+  bool all = !(d.ni < 2 || d.nj < 2 || d.nk < 2) && (L.nmir == 0 || L.cut_interior);
+  for (int f = 0; f < 6; f++) {
+    const int t = L.bctype[f];
+    all = all && (t == -20 || t == -50 || t == -60 || t == -80);
+  }
+  return all;
+}
+
+
 template <class T>
 int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir) {
   const mge_params &P = h->prm;
@@ -739,6 +754,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       U.fn = A.fn;
       U.dtstep = A.dtstep;
       U.vt = A.vt;
+      U.carry = ghosts_rewritten(L) ? 0 : 1;
       LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256), U);
     } else if (A.smoothed) {
       SmoothArgs<T> S;
@@ -778,18 +794,13 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
       U.fn = A.fn;
       U.dtstep = A.dtstep;
       U.vt = A.vt;
+      U.carry = ghosts_rewritten(L) ? 0 : 1;
       LAUNCHT(h, "k_update_ghost", k_update_ghost, T, flat_grid(E), dim3(256), U);
     } else {
       // the ghost shell of the new state is the old one until bcond; when
-      // bcond writes every ghost cell from interior cells (all six faces
-      // written, no one-cell direction) and bc_cutcell's clouds are interior,
-      // nothing reads the copy and it is skipped
-      bool all = !(d.ni < 2 || d.nj < 2 || d.nk < 2) && (L.nmir == 0 || L.cut_interior);
-      for (int f = 0; f < 6; f++) {
-        const int t = L.bctype[f];
-        all = all && (t == -20 || t == -50 || t == -60 || t == -80);
-      }
-      LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, all ? dim3(0) : flat_grid(shell_cells(d)), dim3(256),
+      // bcond rewrites every ghost cell nothing reads the copy and it is
+      // skipped
+      LAUNCHT(h, "k_ghost_copy", k_ghost_copy, T, ghosts_rewritten(L) ? dim3(0) : flat_grid(shell_cells(d)), dim3(256),
               (GhostArgs<T>{d, stc<T>(L, src), stm<T>(L, dst)}));
     }
     int rc = do_cut_bcond<T>(h, L, dst, &warn);

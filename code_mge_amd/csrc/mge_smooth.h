@@ -491,8 +491,10 @@ __device__ __attribute__((always_inline)) inline void k_smooth_z_body(const Smoo
 MGE_ENTRY_T_XCD(k_smooth_z, SmoothArgs, 64, 1)
 
 // after smoothing: w = wn - dw and the EOS on interior cells
-// (euler.f90:381-399); the ghost shell of the previous state is carried
-// over, as k_ghost_copy does on unsmoothed stages
+// (euler.f90:381-399), one thread per cell of the whole array so that the
+// accesses stream through aligned rows; the ghost shell of the previous
+// state is carried over where anything reads it before bcond rewrites it,
+// as k_ghost_copy does on unsmoothed stages
 template <typename T>
 struct UpdArgs {
   Dims d;
@@ -508,6 +510,7 @@ struct UpdArgs {
   const double *dtim;
   T fn, dtstep;
   int vt;
+  int carry;   // carry the ghost shell over
 };
 
 template <typename T>
@@ -517,26 +520,36 @@ __device__ __attribute__((always_inline)) inline void k_update_ghost_body(const 
   const StateC<T> &src = U.src;
   const State<T> &dst = U.dst;
   const T gamma = U.gamma;
-  long q = bx * (long)blockDim.x + threadIdx.x;
+  const long q = bx * (long)blockDim.x + threadIdx.x;
   if (q >= d.E) return;
-  int k = (int)(q / d.sz);
-  int j = (int)((q - (long)k * d.sz) / d.sy);
-  int i = (int)(q - (long)k * d.sz - (long)j * d.sy);
-  bool inner = i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl;
-  if (inner) {
-    T v[5];
-    if (U.ulam) {
-      const T lam = unsteady_lam(U.fn, U.vdtim, U.dtim, U.vt, U.dtstep, q);
-      for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q] / lam;
-    } else {
-      for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q];
-    }
-    for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = v[n];
-    dst.p[q] = eos_st(gamma, U.inf, v);
+  int i, j, k;
+  if (d.E <= 0xffffffffL) {  // 32-bit index arithmetic
+    const unsigned uq = (unsigned)q, sz = (unsigned)d.sz, sy = (unsigned)d.sy;
+    const unsigned kk = uq / sz, r = uq - kk * sz, jj = r / sy;
+    i = (int)(r - jj * sy);
+    j = (int)jj;
+    k = (int)kk;
   } else {
-    for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
-    dst.p[q] = src.p[q];
+    k = (int)(q / d.sz);
+    j = (int)((q - (long)k * d.sz) / d.sy);
+    i = (int)(q - (long)k * d.sz - (long)j * d.sy);
   }
+  if (!(i >= 2 && i <= d.il && j >= 2 && j <= d.jl && k >= 2 && k <= d.kl)) {
+    if (U.carry) {
+      for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = src.w[n * d.E + q];
+      dst.p[q] = src.p[q];
+    }
+    return;
+  }
+  T v[5];
+  if (U.ulam) {
+    const T lam = unsteady_lam(U.fn, U.vdtim, U.dtim, U.vt, U.dtstep, q);
+    for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q] / lam;
+  } else {
+    for (int n = 0; n < 5; n++) v[n] = wn[n * d.E + q] - dw[n * d.E + q];
+  }
+  for (int n = 0; n < 5; n++) dst.w[n * d.E + q] = v[n];
+  dst.p[q] = eos_st(gamma, U.inf, v);
 }
 MGE_ENTRY_T(k_update_ghost, UpdArgs, 256, 1)
 
