@@ -54,11 +54,14 @@ def csg_equal(a, b):
     return a.shape == b.shape and np.array_equal(a, b)
 
 
-def test_cylinder_11122_parity(require_gpu):
-    """the reference's seven-block hierarchy of the cylinder job (JST, the
-    jobsheet's parameters), FMG ramp 1:1:1:1:2: the residual history and the
-    state of every block against the oracle"""
+@pytest.mark.parametrize("distype", ["c", "r2"])
+def test_cylinder_11122_parity(require_gpu, distype):
+    """the reference's seven-block hierarchy of the cylinder job (the
+    jobsheet's parameters: JST; and the same with Roe/MUSCL), FMG ramp
+    1:1:1:1:2: the residual history and the state of every block against the
+    oracle"""
     prm, blocks, body, mcyc, aref = _cylinder_11122()
+    prm = dataclasses.replace(prm, distype=distype)
     geoms = [level_geometry(b.spec, body) for b in blocks]
     print("blocks", [(b.name, b.level, b.parent, b.g0, b.spec.n) for b in blocks], flush=True)
     gcases = [Case(prm, [b.spec], geoms=[g], mxcyc=8) for b, g in zip(blocks, geoms)]
