@@ -357,6 +357,10 @@ int mge_comp_create(int nblock, mge_case *const *blocks, const int *level, const
     return set_err(MGE_ERR_HIP, "composite stream creation failed");
   }
   for (auto &B : C->b) {
+    // the composite's transfers read the blocks' ghost cells at any time:
+    // every bcond forms the corner cells at once
+    sync_corners(B.h, B.h->lv[1]);
+    B.h->lazy_corners = false;
     hipStreamSynchronize(B.own);
     B.h->stream = C->stream;
   }
@@ -380,6 +384,7 @@ int mge_comp_destroy(mge_comp *C) {
   hipStreamSynchronize(C->stream);
   for (auto &B : C->b) {
     B.h->stream = B.own;
+    B.h->lazy_corners = true;
     if (B.corr) hipFree(B.corr);
   }
   hipStreamDestroy(C->stream);

@@ -98,6 +98,7 @@ struct Level {
   int npiv = 0;
   int nmir = 0, nupd = 0, zero_at = -1;
   bool cut_interior = false;  // every cloud cell inside the ghost shell (bcond writes none of them)
+  bool corner_stale = false;  // the current state's edge and corner ghost cells await bc_corner
   DevRec *recs = nullptr;
   double *wsol = nullptr, *psol = nullptr;
   std::vector<mge_cutrec> hrecs;
@@ -172,6 +173,7 @@ struct mge_case {
   std::vector<Rec> *rec = nullptr;  // when set, launches are recorded, not run
   hipError_t launch_err = hipSuccess;  // first failed launch, reported by check_err
   int itime = 0;            // physical time level of the current euler call (unsteady)
+  bool lazy_corners = true; // stages may leave bc_corner until it is read (off inside a composite)
 };
 
 namespace {
@@ -378,6 +380,44 @@ int check_err(mge_case *h) {
 // ---- boundary conditions ------------------------------------------------
 
 // bcond.f90:16-39: the six faces in one launch (k_bc_faces), then bc_corner
+// whether bcond rewrites every ghost cell of the level from interior cells
+// and nothing between a stage's update and bcond reads the ghost shell (all
+// six faces written, no one-cell direction, the cut-cell clouds interior):
+// then a stage need not carry the old ghost shell into its new state
+bool ghosts_rewritten(const Level &L) {
+  const Dims &d = L.d;
+  bool all = !(d.ni < 2 || d.nj < 2 || d.nk < 2) && (L.nmir == 0 || L.cut_interior);
+  for (int f = 0; f < 6; f++) {
+    const int t = L.bctype[f];
+    all = all && (t == -20 || t == -50 || t == -60 || t == -80);
+  }
+  return all;
+}
+
+// Of the kernels a steady Euler solve runs between two bcond calls on a
+// level, none reads a ghost cell with two or three ghost indices (the edge
+// and corner cells bc_corner writes): the stage kernels, the time step and
+// the smoothing read along one axis at a time, the faces' chains start from
+// interior cells when all six faces are written, and the cut-cell clouds are
+// interior.  There a stage's bcond leaves bc_corner for later and marks the
+// level; the corners are formed (from the faces as they stand, the values
+// bcond.f90:42 would have formed after the last stage) before anything that
+// does read them: the prolongation from the level, the copy ws of the state,
+// the end of a cycle, and the host reading the state.
+bool corners_lazy(const mge_case *h, const Level &L) {
+  return h->lazy_corners && h->prm.eqntype == 0 && ghosts_rewritten(L);
+}
+
+// bc_corner.f90 on buffer buf of the level (no blocks unless run)
+template <class T>
+void launch_corners(mge_case *h, Level &L, int buf, bool run) {
+  const Dims &d = L.d;
+  const mge_params &P = h->prm;
+  const int items = 4 * (d.kl - 1) + 4 * (d.jl - 1) + 4 * (d.ib + 1) + 8;
+  LAUNCHT(h, "k_bc_corner", k_bc_corner, T, run ? dim3((items + 255) / 256) : dim3(0), dim3(256),
+          (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma, make_inf<T>(P)}));
+}
+
 template <class T>
 CutArgs<T> cut_args(mge_case *h, Level &L, int buf) {
   CutArgs<T> C{};
@@ -396,9 +436,10 @@ CutArgs<T> cut_args(mge_case *h, Level &L, int buf) {
   return C;
 }
 
-// bcond; with `cut`, bc_cutcell's records go into the faces' launch
+// bcond; with `cut`, bc_cutcell's records go into the faces' launch; with
+// `lazy`, bc_corner is left for ensure_corners
 template <class T>
-int do_bcond(mge_case *h, Level &L, int buf, const CutArgs<T> *cut = nullptr) {
+int do_bcond(mge_case *h, Level &L, int buf, const CutArgs<T> *cut = nullptr, bool lazy = false) {
   const mge_params &P = h->prm;
   int warn = MGE_OK;
   FacesArgs<T> F{};
@@ -431,13 +472,18 @@ int do_bcond(mge_case *h, Level &L, int buf, const CutArgs<T> *cut = nullptr) {
     }
     LAUNCHT(h, "k_bc_faces", k_bc_faces, T, dim3((unsigned)(F.nfb + ncb)), dim3(256), F);
   }
-  {
-    const Dims &d = L.d;
-    const int items = 4 * (d.kl - 1) + 4 * (d.jl - 1) + 4 * (d.ib + 1) + 8;
-    LAUNCHT(h, "k_bc_corner", k_bc_corner, T, dim3((items + 255) / 256), dim3(256),
-            (CornerArgs<T>{L.d, stm<T>(L, buf), (T)P.gamma, make_inf<T>(P)}));
-  }
+  launch_corners<T>(h, L, buf, !lazy);
+  L.corner_stale = lazy;
   return warn;
+}
+
+// bc_corner on the level's current state if a lazy bcond left it (a launch
+// with no blocks otherwise, so that every case of a batch records the same
+// launches)
+template <class T>
+void ensure_corners(mge_case *h, Level &L) {
+  launch_corners<T>(h, L, L.cur, L.corner_stale);
+  L.corner_stale = false;
 }
 
 template <class T>
@@ -458,15 +504,16 @@ template <class T>
 int do_cut_bcond(mge_case *h, Level &L, int buf, int *warn) {
   const Dims &d = L.d;
   const bool seq = d.ni < 2 || d.nj < 2 || d.nk < 2;
+  const bool lazy = corners_lazy(h, L);
   if (L.nmir == 0 || !L.cut_interior || seq) {
     int rc = do_bc_cutcell<T>(h, L, buf);
     if (rc > 0) *warn = rc;
-    return do_bcond<T>(h, L, buf);
+    return do_bcond<T>(h, L, buf, nullptr, lazy);
   }
   LAUNCHT(h, "k_bc_cut", k_bc_cut, T, dim3(0), dim3(64), CutArgs<T>{});
   if (L.zero_at >= 0) *warn = MGE_WARN_ZERO_CLOUD;
   const CutArgs<T> C = cut_args<T>(h, L, buf);
-  return do_bcond<T>(h, L, buf, &C);
+  return do_bcond<T>(h, L, buf, &C, lazy);
 }
 
 // ---- timestep --------------------------------------------------------------
@@ -643,19 +690,6 @@ bool stage_gen(const StageArgs<T> &A) {
   return A.coarse || A.tsrc || A.ulam != 0 || A.vol != nullptr || A.vt == 1;
 }
 
-// whether bcond rewrites every ghost cell of the level from interior cells
-// and nothing between a stage's update and bcond reads the ghost shell (all
-// six faces written, no one-cell direction, the cut-cell clouds interior):
-// then a stage need not carry the old ghost shell into its new state
-bool ghosts_rewritten(const Level &L) {
-  const Dims &d = L.d;
-  bool all = !(d.ni < 2 || d.nj < 2 || d.nk < 2) && (L.nmir == 0 || L.cut_interior);
-  for (int f = 0; f < 6; f++) {
-    const int t = L.bctype[f];
-    all = all && (t == -20 || t == -50 || t == -60 || t == -80);
-  }
-  return all;
-}
 
 
 template <class T>
@@ -670,9 +704,11 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   const long E = d.E;
   if (P.smoo > 0.0 && (d.ni < 2 || d.nj < 2 || d.nk < 2))
     return set_err(MGE_ERR_SMOO_DIM, "residual smoothing needs at least 2 cells per direction");
-  if (lfir)
+  if (lfir) {
+    ensure_corners<T>(h, L);
     LAUNCHT(h, "k_copy", k_copy, T, flat_grid(5 * E), dim3(256),
             (CopyArgs<T>{(const T *)tp<T>(L.S[L.cur]), tp<T>(L.ws), 5 * E}));
+  }
   T *dwp = tp<T>(L.dw);
   const int w0 = L.cur;
   // the last stage always lands in `target` (0 <-> 1 from a start in 0 or
@@ -910,6 +946,7 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.flagsf = f.flags;
   X.wfo = tp<T>(f.S[f.cur]);
   X.corr = tp<T>(c.corr);
+  ensure_corners<T>(h, c);  // the prolongation stencils reach the coarse level's edge cells
   if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, flat_grid(5 * c.d.E), dim3(256), X);
   LAUNCHT(h, "k_prolong", k_prolong, T, xfer_eos_grid(X, f, h->prm), cell_block(), X);
   return do_bcond<T>(h, f, f.cur);
@@ -967,7 +1004,20 @@ int do_init_flow(mge_case *h) {
 // L.rtrms[ncyc] (read by the caller after the cycle: the finest level is
 // relaxed once per cycle, so the value is final when the cycle ends).
 template <class T>
+int do_cycle_levels(mge_case *h, int mlvl, int ncyc, long long *cs);
+
+// one W/V-cycle, the levels' corner cells formed at its end
+template <class T>
 int do_cycle(mge_case *h, int mlvl, int ncyc, long long *cs) {
+  const int rc = do_cycle_levels<T>(h, mlvl, ncyc, cs);
+  if (rc < 0) return rc;
+  for (int l = 1; l <= h->nlevel; l++)
+    if (h->lv[l].defined) ensure_corners<T>(h, h->lv[l]);
+  return rc;
+}
+
+template <class T>
+int do_cycle_levels(mge_case *h, int mlvl, int ncyc, long long *cs) {
   const int nlevel = h->nlevel;
   const mge_params &P = h->prm;
   std::vector<int> icyc(nlevel + 1, 0);
@@ -2094,9 +2144,20 @@ static void *field_ptr(Level &L, size_t es, const char *name, long *n) {
   return nullptr;
 }
 
+// the level's corner cells formed before the host reads or writes its state
+int sync_corners(mge_case *h, Level &L) {
+  if (!L.corner_stale) return MGE_OK;
+  by_type(h, [&](auto z) {
+    ensure_corners<decltype(z)>(h, L);
+    return MGE_OK;
+  });
+  return MGE_OK;
+}
+
 int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1 || !field || !out) return set_err(MGE_ERR_ARG, "bad download arguments");
+  sync_corners(h, *L);
   if (!strcmp(field, "lim")) {  // limiter bytes (limiter freezing): out[(5 dir + n) E + q]
     if (!L->lim) return set_err(MGE_ERR_ARG, "the level has no limiter words (lfreeze = 0 or not Roe r2)");
     const long E = L->d.E;
@@ -2132,6 +2193,7 @@ int mge_download(mge_case *h, int lvl, int blk, const char *field, double *out) 
 int mge_upload(mge_case *h, int lvl, int blk, const char *field, const double *in) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1 || !field || !in) return set_err(MGE_ERR_ARG, "bad upload arguments");
+  sync_corners(h, *L);
   long n;
   void *p = field_ptr(*L, h->es, field, &n);
   if (!p) return set_err(MGE_ERR_ARG, "unknown field");
@@ -2180,6 +2242,8 @@ extern "C" {
 
 int mge_write_solut(mge_case *h, const char *path, int format) {
   if (!h || !path || (format != 0 && format != 1)) return set_err(MGE_ERR_ARG, "bad write_solut arguments");
+  for (int l = 1; l <= h->nlevel; l++)
+    if (h->lv[l].defined) sync_corners(h, h->lv[l]);
   FILE *f = std::fopen(path, format ? "wb" : "w");
   if (!f) return set_err(MGE_ERR_ARG, "cannot open the restart file for writing");
   int rc = MGE_OK;
@@ -2402,6 +2466,7 @@ extern "C" {
 int mge_upload_solut_noalloc(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1) return set_err(MGE_ERR_ARG, "bad level/block");
+  sync_corners(h, *L);
   const int nv = h->prm.eqntype == 1 ? 11 : 7;
   if (nval) *nval = nv;
   if (!solut) return MGE_OK;
@@ -2425,6 +2490,7 @@ int mge_upload_solut_noalloc(mge_case *h, int lvl, int blk, double p_tot, double
 int mge_upload_solut(mge_case *h, int lvl, int blk, double p_tot, double *solut, int *nval) {
   Level *L = get_level(h, lvl);
   if (!L || blk != 1) return set_err(MGE_ERR_ARG, "bad level/block");
+  sync_corners(h, *L);
   const int nv = h->prm.eqntype == 1 ? 11 : 7;
   if (nval) *nval = nv;
   if (!solut) return MGE_OK;
