@@ -28,6 +28,7 @@ struct TsArgs {
   T dtime;       // 2*dtstep/(3*cfl)
   T *eps[3];     // psmoo's per-cell coefficients (nullptr: not formed)
   T smoo, cflrat;  // smo and sqrt(1 + 4 smo)
+  int want_dt;     // form the global minimum (dtim_bits)
 };
 
 // one coefficient of timestep.f90:286-309: the ratio r of the other two
@@ -135,6 +136,7 @@ __device__ __attribute__((always_inline)) inline void k_timestep_body(const TsAr
   // waves) -> one atomic per block on the FP64 bit pattern (positive
   // doubles order like their bits), skipped when the block cannot lower the
   // value already there, so the single address sees few atomics
+  if (!A.want_dt) return;  // uniform over the launch
   T m = mine;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {

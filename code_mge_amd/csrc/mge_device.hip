@@ -518,11 +518,14 @@ int do_cut_bcond(mge_case *h, Level &L, int buf, int *warn) {
 
 // ---- timestep --------------------------------------------------------------
 
+// timestep.f90; the global minimum of the time step only when need_dt (the
+// global time step, vt = 1, and the host's reads): with the local step no
+// kernel reads it, and its launches then have no blocks
 template <class T>
-int do_timestep(mge_case *h, Level &L) {
+int do_timestep(mge_case *h, Level &L, bool need_dt = true) {
   const mge_params &P = h->prm;
   const DtimArgs D{L.dtim_bits, L.dtim};
-  LAUNCHN(h, "k_dtim_init", k_dtim_init, dim3(1), dim3(64), D);
+  LAUNCHN(h, "k_dtim_init", k_dtim_init, need_dt ? dim3(1) : dim3(0), dim3(64), D);
   TsArgs<T> A{};
   A.d = L.d;
   A.s = stc<T>(L, L.cur);
@@ -546,8 +549,9 @@ int do_timestep(mge_case *h, Level &L) {
   for (int a = 0; a < 3; a++) A.eps[a] = L.eps ? tp<T>(L.eps) + a * L.d.E : nullptr;
   A.smoo = (T)P.smoo;
   A.cflrat = (T)std::sqrt(1.0 + 4.0 * P.smoo);
+  A.want_dt = need_dt ? 1 : 0;
   LAUNCHT(h, "k_timestep", k_timestep, T, cell_grid(L.d), cell_block(), A);
-  LAUNCHN(h, "k_dtim_decode", k_dtim_decode, dim3(1), dim3(64), D);
+  LAUNCHN(h, "k_dtim_decode", k_dtim_decode, need_dt ? dim3(1) : dim3(0), dim3(64), D);
   return MGE_OK;
 }
 
@@ -692,8 +696,10 @@ bool stage_gen(const StageArgs<T> &A) {
 
 
 
+// euler.f90; the residual norm only when want_res (a cycle reads it on the
+// finest active level only; its launches then have no blocks elsewhere)
 template <class T>
-int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir) {
+int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int lfir, bool want_res = true) {
   const mge_params &P = h->prm;
   const Dims &d = L.d;
   struct LevelTag {  // timer statistics of this call carry the level number
@@ -857,10 +863,10 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   dim3 g = cell_grid(d);
   R.gx = (int)g.x;
   R.gy = (int)g.y;
-  LAUNCHT(h, "k_rtrms", k_rtrms, T, g, cell_block(), R);
+  LAUNCHT(h, "k_rtrms", k_rtrms, T, want_res ? g : dim3(0), cell_block(), R);
   long nb = (long)g.x * g.y * g.z;
   int c = ncyc >= 1 && ncyc <= L.mxcyc ? ncyc : 0;
-  LAUNCHN(h, "k_rtrms_final", k_rtrms_final, dim3(1), dim3(256),
+  LAUNCHN(h, "k_rtrms_final", k_rtrms_final, want_res ? dim3(1) : dim3(0), dim3(256),
           (RfArgs{(const double *)L.partial, nb, (double)((long)d.ni * d.nj * d.nk), L.rtrms + c, h->err}));
   return warn;
 }
@@ -1027,8 +1033,8 @@ int do_cycle_levels(mge_case *h, int mlvl, int ncyc, long long *cs) {
       icyc[mgl] += 1;
       int lfir = (mgl == mlvl) ? 0 : 1;
       Level &B = h->lv[mgl];
-      CHKRC(do_timestep<T>(h, B));
-      CHKRC(do_euler<T>(h, B, ncyc, mgl, mlvl, 0, lfir));
+      CHKRC(do_timestep<T>(h, B, P.vt == 1));
+      CHKRC(do_euler<T>(h, B, ncyc, mgl, mlvl, 0, lfir, mgl == mlvl));
       *cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
       if (mlvl == 1) return MGE_OK;
       CHKRC(do_euler<T>(h, B, ncyc, mgl, mlvl, 1, 0));
@@ -1040,8 +1046,8 @@ int do_cycle_levels(mge_case *h, int mlvl, int ncyc, long long *cs) {
     int lfir = 1;
     for (;;) {
       Level &B = h->lv[mgl];
-      CHKRC(do_timestep<T>(h, B));
-      CHKRC(do_euler<T>(h, B, ncyc, mgl, mlvl, 0, lfir));
+      CHKRC(do_timestep<T>(h, B, P.vt == 1));
+      CHKRC(do_euler<T>(h, B, ncyc, mgl, mlvl, 0, lfir, mgl == mlvl));
       *cs += (long long)P.mstage * B.d.ni * B.d.nj * B.d.nk;
       lfir = 0;
       mgl += 1;
