@@ -173,6 +173,8 @@ MGE_ENTRY(k_dtim_decode, DtimArgs, 64, 1)
 
 // ------------------------------------------------------------------ residual norm
 
+constexpr int kRtrmsPlanes = 8;  // k planes per thread of k_rtrms
+
 template <typename T>
 struct RtArgs {
   Dims d;
@@ -194,15 +196,18 @@ __device__ __attribute__((always_inline)) inline void k_rtrms_body(const RtArgs<
   const Dims &d = R.d;
   int i = 2 + bx * blockDim.x + threadIdx.x;
   int j = 2 + by * blockDim.y + threadIdx.y;
-  int k = 2 + bz;
   double v = 0.0;
-  if (i <= d.il && j <= d.jl && k <= d.kl) {
-    long q = ix(d, i, j, k);
-    double de = R.cfl * (double)R.vdtim[q];
-    if (R.vt == 1) de = R.cfl * R.dtim[0];
-    double rt = 0.0;
-    if (R.flags[q] & F_FLUID) rt = (double)R.dw[q] / de;
-    v = rt * rt;
+  // kRtrmsPlanes planes per thread, summed in k order, so that the final
+  // pass reads few block partials
+  for (int k = 2 + bz * kRtrmsPlanes; k < 2 + (bz + 1) * kRtrmsPlanes; k++) {
+    if (i <= d.il && j <= d.jl && k <= d.kl) {
+      long q = ix(d, i, j, k);
+      double de = R.cfl * (double)R.vdtim[q];
+      if (R.vt == 1) de = R.cfl * R.dtim[0];
+      double rt = 0.0;
+      if (R.flags[q] & F_FLUID) rt = (double)R.dw[q] / de;
+      v = v + rt * rt;
+    }
   }
   __shared__ double sm[256];
   int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -916,6 +921,64 @@ __device__ __attribute__((always_inline)) inline void k_prolong_body(const XferA
   if (X.eos) X.peos[qf] = eos_st(X.gamma, X.inf, v);
 }
 MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
+
+// k_prolong on a ratio-2 uniform level pair without block offsets (the
+// single-case path): one thread per coarse cell c forms its eight fine cells
+// (2c-2 and 2c-1 per direction) from the 3 x 3 x 3 coarse cells around it,
+// loaded once per component, each fine value with k_prolong's stencil,
+// weights and summation order (so the same bits), and their p when eos
+template <typename T>
+__device__ __attribute__((always_inline)) inline void k_prolong2_body(const XferArgs<T> &X, int bx, int by, int bz) {
+  const Dims &c = X.dc, &f = X.df;
+  if (xfer_shell_block(X, f, (const T *)X.wfo, bx, by, bz)) return;
+  const int ci = 2 + bx * blockDim.x + threadIdx.x;
+  const int cj = 2 + by * blockDim.y + threadIdx.y;
+  const int ck = 2 + bz;
+  if (ci > c.il || cj > c.jl || ck > c.kl) return;
+  const T *src = X.fmg ? X.wc : X.corr;
+  // the fine cell 2c-2 takes 1/4 of c-1 and 3/4 of c, the cell 2c-1 3/4 of c
+  // and 1/4 of c+1 (pstencil)
+  const T wlo[2] = {T(0.25), T(0.75)}, whi[2] = {T(0.75), T(0.25)};
+  long qf[8];
+  bool fl[8];
+#pragma unroll
+  for (int h = 0; h < 8; h++) {
+    qf[h] = ix(f, 2 * ci - 2 + (h & 1), 2 * cj - 2 + ((h >> 1) & 1), 2 * ck - 2 + ((h >> 2) & 1));
+    fl[h] = (X.flagsf[qf[h]] & F_FLUID) != 0;
+  }
+  // one component at a time (its 27 coarse values in registers); the EOS
+  // then reads the eight cells' states back
+#pragma unroll 1
+  for (int n = 0; n < 5; n++) {
+    T cv[27];
+#pragma unroll
+    for (int g = 0; g < 27; g++)
+      cv[g] = src[n * c.E + ix(c, ci - 1 + g % 3, cj - 1 + (g / 3) % 3, ck - 1 + g / 9)];
+#pragma unroll
+    for (int h = 0; h < 8; h++) {
+      if (!fl[h]) continue;
+      const int hi = h & 1, hj = (h >> 1) & 1, hk = (h >> 2) & 1;
+      const T *fa = hi ? whi : wlo, *fb = hj ? whi : wlo, *fc = hk ? whi : wlo;
+      T acc = T(0);
+#pragma unroll
+      for (int c8 = 0; c8 < 8; c8++) {
+        const T fac = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
+        acc = acc + cv[(hi + (c8 & 1)) + 3 * (hj + ((c8 >> 1) & 1)) + 9 * (hk + ((c8 >> 2) & 1))] * fac;
+      }
+      T *dst = X.wfo + n * f.E + qf[h];
+      *dst = X.fmg ? acc : *dst + acc;
+    }
+  }
+  if (!X.eos) return;
+#pragma unroll
+  for (int h = 0; h < 8; h++) {
+    T v[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) v[n] = X.wfo[n * f.E + qf[h]];
+    X.peos[qf[h]] = eos_st(X.gamma, X.inf, v);
+  }
+}
+MGE_ENTRY_T(k_prolong2, XferArgs, 256, 1)
 
 // bc_embed.f90:50-143 for an aligned ratio-2 embedding: every ghost cell of
 // the fine block (outside its cells 2..il, 2..jl, 2..kl) whose centre lies

@@ -861,6 +861,7 @@ int do_euler(mge_case *h, Level &L, int ncyc, int mgl, int mlvl, int lres, int l
   R.vt = P.vt;
   R.partial = L.partial;
   dim3 g = cell_grid(d);
+  g.z = (g.z + kRtrmsPlanes - 1) / kRtrmsPlanes;
   R.gx = (int)g.x;
   R.gy = (int)g.y;
   LAUNCHT(h, "k_rtrms", k_rtrms, T, want_res ? g : dim3(0), cell_block(), R);
@@ -885,8 +886,7 @@ bool levels_nest(const Level &c, const Level &f) {
 // (XferArgs::eos): the cells' grid and then enough z-slices of blocks for
 // the ghost shell
 template <class T>
-dim3 xfer_eos_grid(XferArgs<T> &X, Level &L, const mge_params &P) {
-  const dim3 g = cell_grid(L.d);
+dim3 xfer_eos_grid(XferArgs<T> &X, Level &L, const mge_params &P, dim3 g) {
   const long per = (long)g.x * g.y * 256;
   X.eos = 1;
   X.peos = tp<T>(L.S[L.cur]) + 5 * L.d.E;
@@ -925,7 +925,8 @@ int do_movco(mge_case *h, int mgl) {
   X.nrc[0] = c.d.ni; X.nrc[1] = c.d.nj; X.nrc[2] = c.d.nk;
   // the restriction covers every coarse cell: it sets their p, and blocks
   // past its grid the ghost shell's (the EOS pass over the level)
-  LAUNCHT(h, "k_restrict", k_restrict, T, xfer_eos_grid(X, c, h->prm), cell_block(), X);
+  const dim3 gr = xfer_eos_grid(X, c, h->prm, cell_grid(c.d));
+  LAUNCHT(h, "k_restrict", k_restrict, T, gr, cell_block(), X);
   return do_bcond<T>(h, c, c.cur);
 }
 
@@ -954,7 +955,14 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.corr = tp<T>(c.corr);
   ensure_corners<T>(h, c);  // the prolongation stencils reach the coarse level's edge cells
   if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, flat_grid(5 * c.d.E), dim3(256), X);
-  LAUNCHT(h, "k_prolong", k_prolong, T, xfer_eos_grid(X, f, h->prm), cell_block(), X);
+  if (f.ratio[0] == 2 && f.ratio[1] == 2 && f.ratio[2] == 2 && !X.pw[0]) {
+    // a thread per coarse cell and its eight fine cells
+    const dim3 gp = xfer_eos_grid(X, f, h->prm, cell_grid(c.d));
+    LAUNCHT(h, "k_prolong", k_prolong2, T, gp, cell_block(), X);
+  } else {
+    const dim3 gp = xfer_eos_grid(X, f, h->prm, cell_grid(f.d));
+    LAUNCHT(h, "k_prolong", k_prolong, T, gp, cell_block(), X);
+  }
   return do_bcond<T>(h, f, f.cur);
 }
 
@@ -1906,7 +1914,7 @@ int mge_level_define(mge_case *h, int lvl, int ni, int nj, int nk, const double 
   CHKRC(dalloc_bytes(L, &L.piv, es * 9 * nmax));
   CHKRC(dalloc_bytes(L, &L.fax, es * nmax));
   dim3 g = cell_grid(d);
-  L.npartial = (long)g.x * g.y * g.z;
+  L.npartial = (long)g.x * g.y * ((g.z + kRtrmsPlanes - 1) / kRtrmsPlanes);
   CHKRC(dalloc(L, &L.partial, L.npartial));
   CHKRC(dalloc(L, &L.dtim_bits, 1));
   CHKRC(dalloc(L, &L.dtim, 1));

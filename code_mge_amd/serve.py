@@ -70,7 +70,7 @@ def level_bytes(n, es, nmir=0, mxcyc=4, viscous=False, coarse=False, psmoo=False
     ni, nj, nk = n
     E = (ni + 4) * (nj + 4) * (nk + 4)
     nmax = max(ni, nj, nk) + 5
-    ngrid = -(-ni // 64) * -(-nj // 4) * nk
+    ngrid = -(-ni // 64) * -(-nj // 4) * -(-nk // 8)   # residual-norm partials (8 planes per block)
     sizes = [es * 6 * E] * 3 + [es * 5 * E] * 5 + [es * E] * 2 + [E] * 2
     sizes += [es * nmax] * 3 + [es * 9 * nmax, es * nmax, 8 * ngrid, 8, 8, 8 * (mxcyc + 1), 24]
     if viscous:
