@@ -287,10 +287,29 @@ __device__ __attribute__((always_inline)) inline void k_bc_cut_body(const CutArg
   s = s + nv[2] * nv[2];
   double nn = sqrt(s);
   for (int dd = 0; dd < 3; dd++) nv[dd] = nv[dd] / nn;
+  // every cloud value is loaded before the sums use any (the loads are in
+  // flight together rather than one round trip each); the sums keep the
+  // reference order
+  const int np = r.npoint;
+  T cw[8][6];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    if (c < np) {
+      const long qc = r.cloud[c];
+#pragma unroll
+      for (int n = 0; n < 5; n++) cw[c][n] = C.s.w[n * E + qc];
+      cw[c][5] = C.s.p[qc];
+    }
+  }
   double vr[5] = {0, 0, 0, 0, 0}, pr = 0.0;
+#pragma unroll
   for (int n = 0; n < 5; n++)
-    for (int c = 0; c < r.npoint; c++) vr[n] = vr[n] + r.coef[c] * full(C.inf, n, C.s.w[n * E + r.cloud[c]]);
-  for (int c = 0; c < r.npoint; c++) pr = pr + r.coef[c] * full(C.inf, 5, C.s.p[r.cloud[c]]);
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (c < np) vr[n] = vr[n] + r.coef[c] * full(C.inf, n, cw[c][n]);
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    if (c < np) pr = pr + r.coef[c] * full(C.inf, 5, cw[c][5]);
   double psol = pr, ws[5];
   for (int n = 0; n < 5; n++) ws[n] = vr[n];
   if (C.eqntype == 0) {
@@ -578,22 +597,32 @@ __device__ inline void edge_fill(const Dims &d, State<T> s, T gamma, const Inf<T
 // eight corners, which :141-157 overwrite without reading them; each corner
 // reads three edge cells, which its thread evaluates itself with the same
 // expression.  So every item is independent: one launch over all of them.
-__device__ inline void edge_cells(const Dims &d, int ph, int e, int x, long *q) {
+__device__ __attribute__((always_inline)) inline void edge_cells(const Dims &d, int ph, int e, int x, long *q) {
   const int il = d.il, jl = d.jl, kl = d.kl, ie = d.ie, je = d.je, ke = d.ke;
+  // the row e of each table as a sum over the unrolled entries (a runtime
+  // row index, or a select chain the compiler turns into one, would put the
+  // tables in scratch memory)
+  auto row = [e](const int (&T)[4][12], int m) {
+    return (e == 0) * T[0][m] + (e == 1) * T[1][m] + (e == 2) * T[2][m] + (e == 3) * T[3][m];
+  };
   int c[12];
   if (ph == 1) {  // edges along k, x = k
     const int T[4][12] = {{1, 1, 0, 2, 1, 0, 1, 2, 0, 2, 2, 0}, {ie, 1, 0, il, 1, 0, ie, 2, 0, il, 2, 0},
                           {1, je, 0, 2, je, 0, 1, jl, 0, 2, jl, 0}, {ie, je, 0, il, je, 0, ie, jl, 0, il, jl, 0}};
-    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 2) ? x : T[e][m];
+#pragma unroll
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 2) ? x : row(T, m);
   } else if (ph == 2) {  // edges along j, x = j
     const int T[4][12] = {{1, 0, 1, 2, 0, 1, 1, 0, 2, 2, 0, 2}, {ie, 0, 1, il, 0, 1, ie, 0, 2, il, 0, 2},
                           {1, 0, ke, 2, 0, ke, 1, 0, kl, 2, 0, kl}, {ie, 0, ke, il, 0, ke, ie, 0, kl, il, 0, kl}};
-    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 1) ? x : T[e][m];
+#pragma unroll
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 1) ? x : row(T, m);
   } else {  // edges along i, x = i
     const int T[4][12] = {{0, 1, 1, 0, 1, 2, 0, 2, 1, 0, 2, 2}, {0, 1, ke, 0, 2, ke, 0, 1, kl, 0, 2, kl},
                           {0, je, 1, 0, je, 1, 0, jl, 2, 0, jl, 2}, {0, je, ke, 0, jl, ke, 0, je, kl, 0, jl, kl}};
-    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 0) ? x : T[e][m];
+#pragma unroll
+    for (int m = 0; m < 12; m++) c[m] = (m % 3 == 0) ? x : row(T, m);
   }
+#pragma unroll
   for (int m = 0; m < 4; m++) q[m] = ix(d, c[3 * m], c[3 * m + 1], c[3 * m + 2]);
 }
 
@@ -1082,9 +1111,9 @@ __global__ void k_wall_terms(FluxCtx<T> fc, const DevRec *recs, const double *ws
     return;
   }
   int zz = r.zz;
-  double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
-  sc[zz - 1] = face_s_any(g, zz, r.c[0], r.c[1], r.c[2]);
-  ss[zz - 1] = face_s_any(g, zz, r.s[0], r.s[1], r.s[2]);
+  double sc[3], ss[3];
+  axis_vec(sc, zz, (double)face_s_any(g, zz, r.c[0], r.c[1], r.c[2]));
+  axis_vec(ss, zz, (double)face_s_any(g, zz, r.s[0], r.s[1], r.s[2]));
   long qc = ix(d, r.c[0], r.c[1], r.c[2]);
   double wc[5], ws[5], f1[5], f2[5];
   for (int n = 0; n < 5; n++) wc[n] = full(fc.inf, n, s.w[n * d.E + qc]);

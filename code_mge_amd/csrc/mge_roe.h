@@ -295,7 +295,7 @@ __device__ inline void roe_face_vec(double gamma, double eps, double cdis, const
 // wall face of record r: the signed contributions (ve, vd) to the cut cell,
 // in FP64 from the bulk state and the mirror state
 template <typename T>
-__device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
+__device__ __attribute__((always_inline)) inline void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
                              double eps, double cdis, double *ve, double *vd, int *err) {
   const Dims &d = c.d;
   int zz = r.zz;
@@ -308,7 +308,7 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
     else if (zz == 2) fj++;
     else fk++;
   }
-  double qcv[5], qm[5], S[3] = {0.0, 0.0, 0.0}, e[5], dv[5];
+  double qcv[5], qm[5], S[3], e[5], dv[5];
   double rc = full(c.inf, 0, c.s.w[qc]);
   double rr = 1.0 / rc;
   qcv[0] = rc;
@@ -322,7 +322,7 @@ __device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double 
   qm[2] = wsol[2] * rm;
   qm[3] = wsol[3] * rm;
   qm[4] = psol;
-  S[zz - 1] = face_s_any(c.g, zz, fi, fj, fk);
+  axis_vec(S, zz, (double)face_s_any(c.g, zz, fi, fj, fk));
   if (hi) roe_face_vec(gamma, eps, cdis, qcv, qm, S, e, dv);
   else roe_face_vec(gamma, eps, cdis, qm, qcv, S, e, dv);
   double sgn = hi ? -1.0 : 1.0;

@@ -105,7 +105,7 @@ constexpr int kRecOps = 64;
 
 // the Roe wall-face term of a record (defined with the Roe scheme)
 template <typename T>
-__device__ void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
+__device__ __attribute__((always_inline)) inline void roe_wall_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double gamma,
                              double eps, double cdis, double *ve, double *vd, int *err);
 
 // ------------------------------------------------------------------ finish of a cell
@@ -802,18 +802,31 @@ __device__ inline void rec_efs_any(const FluxCtx<T> &c, int zz, int i, int j, in
   else rec_efs<2>(c, i, j, k, a, o);
 }
 
+// face_s<zz-1> for a runtime zz: the three vertex differences are loaded
+// and the product selected (branches on zz that each load through a
+// different pointer of g are merged into one load through a selected field
+// address, which keeps the kernel's argument struct in scratch memory)
 template <typename T>
 __device__ inline T face_s_any(const Geo<T> &g, int zz, int i, int j, int k) {
-  if (zz == 1) return face_s<0>(g, i, j, k);
-  if (zz == 2) return face_s<1>(g, i, j, k);
-  return face_s<2>(g, i, j, k);
+  const T ax = g.dx[i], ay = g.dy[j], az = g.dz[k];
+  const T s0 = -(ay * az), s1 = -(ax * az), s2 = -(ax * ay);
+  return zz == 1 ? s0 : (zz == 2 ? s1 : s2);
+}
+
+// the face vector of a zz-face (zz = 1, 2, 3): component zz-1 is s, the
+// others 0, set by selects (a runtime index into a local array would put
+// the array in scratch memory)
+__device__ inline void axis_vec(double *v, int zz, double s) {
+  v[0] = zz == 1 ? s : 0.0;
+  v[1] = zz == 2 ? s : 0.0;
+  v[2] = zz == 3 ? s : 0.0;
 }
 
 // the two recomputed faces of dfluxb.f90 for record r: fs1 (inner face of
 // the pair) and fs2 (the wall face or the face behind it).  They involve the
 // FP64 mirror state, so they are evaluated in FP64 from the bulk values.
 template <typename T>
-__device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsolv, double psol, double *fs1,
+__device__ __attribute__((always_inline)) inline void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsolv, double psol, double *fs1,
                            double *fs2) {
   const Dims &d = c.d;
   const long E = d.E;
@@ -822,7 +835,9 @@ __device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   int s0 = zz == 1 ? i : (zz == 2 ? j : k);
   long st = zz == 1 ? 1L : (zz == 2 ? d.sy : d.sz);
   long base = ix(d, i, j, k) - (long)s0 * st;
-  int nbmax = zz == 1 ? d.ib : (zz == 2 ? d.jb : d.kb);
+  // a sum of the three values rather than a select between the fields (see
+  // face_s_any)
+  const int nbmax = (zz == 1) * d.ib + (zz == 2) * d.jb + (zz == 3) * d.kb;
   const double vis2 = c.vis2, vis4 = c.vis4, cdis = c.cdis;
   double wsol[5];
   for (int n = 0; n < 5; n++) wsol[n] = wsolv[n];
@@ -834,9 +849,8 @@ __device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   };
   auto RS = [&](int s) { return (double)c.radsum[base + (long)s * st]; };
   auto SS = [&](int s, double *ss) {
-    ss[0] = ss[1] = ss[2] = 0.0;
     int fi = zz == 1 ? s : i, fj = zz == 2 ? s : j, fk = zz == 3 ? s : k;
-    ss[zz - 1] = face_s_any(c.g, zz, fi, fj, fk);
+    axis_vec(ss, zz, (double)face_s_any(c.g, zz, fi, fj, fk));
   };
   double fp[3], ep, lm, lp, lam, dis2, dis4, diw[3][5];
   int hi = (id >= 4);
@@ -952,15 +966,15 @@ __device__ void dfluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
 // the wall flux fs and the opposite-face flux fa of efluxb.f90:50-80,
 // in FP64 like dfluxb_rec
 template <typename T>
-__device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double *fs,
+__device__ __attribute__((always_inline)) inline void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *wsol, double psol, double *fs,
                            double *fa, int *err) {
   const Dims &d = c.d;
   int zz = r.zz, id = r.faceid;
   int i = r.c[0], j = r.c[1], k = r.c[2];
   long qc = ix(d, i, j, k);
-  double sc[3] = {0, 0, 0}, ss[3] = {0, 0, 0};
-  sc[zz - 1] = face_s_any(c.g, zz, i, j, k);
-  ss[zz - 1] = face_s_any(c.g, zz, r.s[0], r.s[1], r.s[2]);
+  double sc[3], ss[3];
+  axis_vec(sc, zz, (double)face_s_any(c.g, zz, i, j, k));
+  axis_vec(ss, zz, (double)face_s_any(c.g, zz, r.s[0], r.s[1], r.s[2]));
   double wc[5];
   for (int n = 0; n < 5; n++) wc[n] = full(c.inf, n, c.s.w[n * d.E + qc]);
   if (fabs(wc[0]) < (double)1.0e-8f) raise_err(err, MGE_ERR_WALL_RHO);
@@ -975,9 +989,9 @@ __device__ void efluxb_rec(const FluxCtx<T> &c, const DevRec &r, const double *w
   else if (zz == 2) oj += dlt;
   else ok += dlt;
   long qo = ix(d, oi, oj, ok);
-  double wo[5], so[3] = {0, 0, 0};
+  double wo[5], so[3];
   for (int n = 0; n < 5; n++) wo[n] = full(c.inf, n, c.s.w[n * d.E + qo]);
-  so[zz - 1] = face_s_any(c.g, zz, oi, oj, ok);
+  axis_vec(so, zz, (double)face_s_any(c.g, zz, oi, oj, ok));
   cflux3(wo, full(c.inf, 5, c.s.p[qo]), so, fs3);
   for (int n = 0; n < 5; n++) fa[n] = 0.5 * (fs1[n] + fs3[n]);
 }
@@ -1069,19 +1083,23 @@ __device__ __attribute__((always_inline)) inline void k_rec_body(const RecArgs<T
   const int2 ch = R.chunks[bx];
   const int o0 = R.beg[ch.x], o1 = R.beg[ch.y];
   const int tid = threadIdx.x;
-  if (o0 + tid < o1) rec_op_value(R, o0 + tid, val + tid * 10);
-  __syncthreads();
   const int t = ch.x + tid;
-  if (t >= ch.y) return;
   const StageArgs<T> &A = R.st;
   const Dims &d = A.fx.d;
   const long E = d.E;
-  long q = R.tcell[t];
+  // the cell's fw and dw are loaded before the op values are formed, so
+  // their round trips overlap the ops' own
+  const bool cell = t < ch.y;
+  const long q = cell ? R.tcell[t] : 0;
   double f[5], e[5];
-  for (int n = 0; n < 5; n++) {
-    f[n] = A.fw[n * E + q];
-    e[n] = A.dw[n * E + q];
-  }
+  if (cell)
+    for (int n = 0; n < 5; n++) {
+      f[n] = A.fw[n * E + q];
+      e[n] = A.dw[n * E + q];
+    }
+  if (o0 + tid < o1) rec_op_value(R, o0 + tid, val + tid * 10);
+  __syncthreads();
+  if (!cell) return;
   for (int o = R.beg[t]; o < R.beg[t + 1]; o++) {
     int kind = R.ops[o].y;
     const double *v = val + (o - o0) * 10;
