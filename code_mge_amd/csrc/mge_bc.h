@@ -359,6 +359,7 @@ struct FacesArgs {
   int kind[6];        // faces 1..6: -20 far, -50 symmetry, -60 periodic, -80 wall, 0 not written
   int seq;            // 1: faces in order inside one workgroup (a direction with one cell)
   double gamma, rho0, p0, c0, u0, v0, w0;
+  double s0;          // freestream entropy rho0^gamma / p0
   Inf<T> inf;
   uint8_t *ffdec[6];  // far field: each face's branch per ghost point, or nullptr
   const int *frozen;
@@ -429,7 +430,7 @@ __device__ inline CellV<T> far_state(const FacesArgs<T> &A, int f, CellV<T> v, u
     v0 = dir == 0 ? un : A.u0;
     v1 = dir == 1 ? un : A.v0;
     v2 = dir == 2 ? un : A.w0;
-    double si = pow(A.rho0, gamma) / A.p0;
+    double si = A.s0;
     double cc = ci * ci / gamma;
     r = pow(si * cc, gm);
     pr = r * cc;
@@ -565,7 +566,7 @@ __device__ __attribute__((always_inline)) inline void k_bc_faces_body(const Face
   }
   const long t = (long)bx * blockDim.x + threadIdx.x;
   int x, y, z;
-  if (!shell_cell(d, t, x, y, z)) return;
+  if (!shell_cell_by_depth(d, t, x, y, z)) return;
   if (last_face(A, x, y, z) == 0) return;
   CellV<T> o;
   face_chain<T, 3>(A, x, y, z, true, o);

@@ -460,6 +460,7 @@ int do_bcond(mge_case *h, Level &L, int buf, const CutArgs<T> *cut = nullptr, bo
   }
   F.seq = (L.d.ni < 2 || L.d.nj < 2 || L.d.nk < 2) ? 1 : 0;
   F.gamma = P.gamma; F.rho0 = P.rho0; F.p0 = P.p0; F.c0 = P.c0;
+  F.s0 = std::pow(P.rho0, P.gamma) / P.p0;
   F.u0 = P.u0; F.v0 = P.v0; F.w0 = P.w0;
   F.inf = make_inf<T>(P);
   F.frozen = h->ffrozen;

@@ -67,6 +67,77 @@ __device__ inline bool shell_cell(const Dims &d, long t, int &x, int &y, int &z)
   return false;
 }
 
+// ghost cell t of the same shell, numbered by how many faces meet there:
+// the faces' four layers over the interior of the face (i faces with the
+// layer fastest, j and k faces with it slowest), then the twelve edges' 4 x 4 ghost pairs along their
+// interior, then the 4 x 4 x 4 corner cells.  Consecutive threads then take
+// the same faces, so a wave of k_bc_faces runs one chain shape.
+__host__ __device__ inline bool shell_cell_by_depth(const Dims &d, long t, int &x, int &y, int &z) {
+  const long ni = d.ni, nj = d.nj, nk = d.nk;
+  auto g = [](long m, int e) { return m < 2 ? (int)m : e + (int)m - 2; };
+  long n = 4 * nj * nk;
+  if (t < n) {  // the i faces' four layers fastest: a row's ghost cells side by side
+    const long r = t / 4;
+    x = g(t % 4, d.ie);
+    y = 2 + (int)(r % nj);
+    z = 2 + (int)(r / nj);
+    return true;
+  }
+  t -= n;
+  n = 4 * ni * nk;
+  if (t < n) {
+    const long r = t % (ni * nk);
+    y = g(t / (ni * nk), d.je);
+    x = 2 + (int)(r % ni);
+    z = 2 + (int)(r / ni);
+    return true;
+  }
+  t -= n;
+  n = 4 * ni * nj;
+  if (t < n) {
+    const long r = t % (ni * nj);
+    z = g(t / (ni * nj), d.ke);
+    x = 2 + (int)(r % ni);
+    y = 2 + (int)(r / ni);
+    return true;
+  }
+  t -= n;
+  n = 16 * ni;
+  if (t < n) {
+    const long m = t / ni;
+    x = 2 + (int)(t % ni);
+    y = g(m & 3, d.je);
+    z = g(m >> 2, d.ke);
+    return true;
+  }
+  t -= n;
+  n = 16 * nj;
+  if (t < n) {
+    const long m = t / nj;
+    y = 2 + (int)(t % nj);
+    x = g(m & 3, d.ie);
+    z = g(m >> 2, d.ke);
+    return true;
+  }
+  t -= n;
+  n = 16 * nk;
+  if (t < n) {
+    const long m = t / nk;
+    z = 2 + (int)(t % nk);
+    x = g(m & 3, d.ie);
+    y = g(m >> 2, d.je);
+    return true;
+  }
+  t -= n;
+  if (t < 64) {
+    x = g(t & 3, d.ie);
+    y = g((t >> 2) & 3, d.je);
+    z = g(t >> 4, d.ke);
+    return true;
+  }
+  return false;
+}
+
 __host__ __device__ inline long shell_cells(const Dims &d) {
   return 4L * (d.ib + 1) * (d.jb + 1) + 4L * (d.ib + 1) * d.nk + 4L * d.nj * d.nk;
 }
