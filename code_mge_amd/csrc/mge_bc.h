@@ -955,57 +955,83 @@ MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
 // k_prolong on a ratio-2 uniform level pair without block offsets (the
 // single-case path): one thread per coarse cell c forms its eight fine cells
 // (2c-2 and 2c-1 per direction) from the 3 x 3 x 3 coarse cells around it,
-// loaded once per component, each fine value with k_prolong's stencil,
-// weights and summation order (so the same bits), and their p when eos
+// each fine value with k_prolong's stencil, weights and summation order (so
+// the same bits), and their p when eos.  A component's coarse values over
+// the block's 64 x 4 cells, a one-cell halo and the planes ck-1..ck+1 are
+// staged in LDS by coalesced row loads; the fine cells 2c-2, 2c-1 of a row
+// move as one pair (rows, planes and components of a ratio-2 fine level have
+// even lengths; the host checks the alignment).  blockDim = (64, 4).
+constexpr int kPro2X = 66, kPro2Y = 6, kPro2N = 3 * kPro2Y * kPro2X;
+
 template <typename T>
 __device__ __attribute__((always_inline)) inline void k_prolong2_body(const XferArgs<T> &X, int bx, int by, int bz) {
   const Dims &c = X.dc, &f = X.df;
-  if (xfer_shell_block(X, f, (const T *)X.wfo, bx, by, bz)) return;
-  const int ci = 2 + bx * blockDim.x + threadIdx.x;
-  const int cj = 2 + by * blockDim.y + threadIdx.y;
-  const int ck = 2 + bz;
-  if (ci > c.il || cj > c.jl || ck > c.kl) return;
+  if (xfer_shell_block(X, f, (const T *)X.wfo, bx, by, bz)) return;  // uniform per block
+  __shared__ T cs[3][kPro2Y][kPro2X];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
+  const int ci0 = 2 + bx * 64, cj0 = 2 + by * 4, ck = 2 + bz;
+  const int ci = ci0 + tx, cj = cj0 + ty;
+  const bool act = ci <= c.il && cj <= c.jl;
   const T *src = X.fmg ? X.wc : X.corr;
   // the fine cell 2c-2 takes 1/4 of c-1 and 3/4 of c, the cell 2c-1 3/4 of c
   // and 1/4 of c+1 (pstencil)
   const T wlo[2] = {T(0.25), T(0.75)}, whi[2] = {T(0.75), T(0.25)};
-  long qf[8];
-  bool fl[8];
+  // fine rows r = (hj, hk) = (r & 1, r >> 1): the pair starts and flags
+  long qr[4];
+  unsigned fl[4];
 #pragma unroll
-  for (int h = 0; h < 8; h++) {
-    qf[h] = ix(f, 2 * ci - 2 + (h & 1), 2 * cj - 2 + ((h >> 1) & 1), 2 * ck - 2 + ((h >> 2) & 1));
-    fl[h] = (X.flagsf[qf[h]] & F_FLUID) != 0;
+  for (int r = 0; r < 4; r++) {
+    qr[r] = ix(f, 2 * ci - 2, 2 * cj - 2 + (r & 1), 2 * ck - 2 + (r >> 1));
+    fl[r] = act ? *reinterpret_cast<const uint16_t *>(X.flagsf + qr[r]) : 0u;
   }
-  // one component at a time (its 27 coarse values in registers); the EOS
-  // then reads the eight cells' states back
 #pragma unroll 1
   for (int n = 0; n < 5; n++) {
-    T cv[27];
-#pragma unroll
-    for (int g = 0; g < 27; g++)
-      cv[g] = src[n * c.E + ix(c, ci - 1 + g % 3, cj - 1 + (g / 3) % 3, ck - 1 + g / 9)];
-#pragma unroll
-    for (int h = 0; h < 8; h++) {
-      if (!fl[h]) continue;
-      const int hi = h & 1, hj = (h >> 1) & 1, hk = (h >> 2) & 1;
-      const T *fa = hi ? whi : wlo, *fb = hj ? whi : wlo, *fc = hk ? whi : wlo;
-      T acc = T(0);
-#pragma unroll
-      for (int c8 = 0; c8 < 8; c8++) {
-        const T fac = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
-        acc = acc + cv[(hi + (c8 & 1)) + 3 * (hj + ((c8 >> 1) & 1)) + 9 * (hk + ((c8 >> 2) & 1))] * fac;
-      }
-      T *dst = X.wfo + n * f.E + qf[h];
-      *dst = X.fmg ? acc : *dst + acc;
+    const T *sn = src + n * c.E;
+    for (int e = tid; e < kPro2N; e += 256) {
+      const int xx = e % kPro2X, rr = e / kPro2X, yy = rr % kPro2Y, zz = rr / kPro2Y;
+      const int gi = ci0 - 1 + xx < c.ib ? ci0 - 1 + xx : c.ib;  // past the block: never read
+      const int gj = cj0 - 1 + yy < c.jb ? cj0 - 1 + yy : c.jb;
+      (&cs[0][0][0])[e] = sn[ix(c, gi, gj, ck - 1 + zz)];
     }
+    __syncthreads();
+    if (act) {
+      T cv[27];
+#pragma unroll
+      for (int g = 0; g < 27; g++) cv[g] = cs[g / 9][ty + (g / 3) % 3][tx + g % 3];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int hj = r & 1, hk = r >> 1;
+        Pair<T> *dp = reinterpret_cast<Pair<T> *>(X.wfo + n * f.E + qr[r]);
+        const Pair<T> old = *dp;
+        T v[2] = {old.x, old.y};
+#pragma unroll
+        for (int hi = 0; hi < 2; hi++) {
+          if (!((fl[r] >> (8 * hi)) & F_FLUID)) continue;  // blank: keeps its value
+          const T *fa = hi ? whi : wlo, *fb = hj ? whi : wlo, *fc = hk ? whi : wlo;
+          T acc = T(0);
+#pragma unroll
+          for (int c8 = 0; c8 < 8; c8++) {
+            const T fac = fa[c8 & 1] * fb[(c8 >> 1) & 1] * fc[(c8 >> 2) & 1];
+            acc = acc + cv[(hi + (c8 & 1)) + 3 * (hj + ((c8 >> 1) & 1)) + 9 * (hk + ((c8 >> 2) & 1))] * fac;
+          }
+          v[hi] = X.fmg ? acc : v[hi] + acc;
+        }
+        *dp = Pair<T>{v[0], v[1]};
+      }
+    }
+    __syncthreads();  // the next component reuses the LDS
   }
-  if (!X.eos) return;
+  if (!X.eos || !act) return;
 #pragma unroll
-  for (int h = 0; h < 8; h++) {
-    T v[5];
+  for (int r = 0; r < 4; r++) {
+    T a[5], b[5];
 #pragma unroll
-    for (int n = 0; n < 5; n++) v[n] = X.wfo[n * f.E + qf[h]];
-    X.peos[qf[h]] = eos_st(X.gamma, X.inf, v);
+    for (int n = 0; n < 5; n++) {
+      const Pair<T> v = *reinterpret_cast<const Pair<T> *>(X.wfo + n * f.E + qr[r]);
+      a[n] = v.x;
+      b[n] = v.y;
+    }
+    *reinterpret_cast<Pair<T> *>(X.peos + qr[r]) = Pair<T>{eos_st(X.gamma, X.inf, a), eos_st(X.gamma, X.inf, b)};
   }
 }
 MGE_ENTRY_T(k_prolong2, XferArgs, 256, 1)

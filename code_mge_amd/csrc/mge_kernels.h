@@ -145,6 +145,12 @@ __host__ __device__ inline long shell_cells(const Dims &d) {
 // flags byte per cell
 enum : uint8_t { F_FLUID = 1, F_TOUCHED = 2 };
 
+// two-element vectors for paired loads and stores of neighbouring cells
+template <typename T> struct PairOf;
+template <> struct PairOf<double> { using type = double2; };
+template <> struct PairOf<float> { using type = float2; };
+template <typename T> using Pair = typename PairOf<T>::type;
+
 // read-only view of one state buffer: w (5 components) followed by p
 template <typename T>
 struct StateC {

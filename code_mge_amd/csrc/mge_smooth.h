@@ -33,12 +33,6 @@ __device__ __attribute__((always_inline)) inline T piv_div(T x, const T *__restr
   else return x * r1[s];
 }
 
-// two-element vectors for the x scan's loads and stores
-template <typename T> struct PairOf;
-template <> struct PairOf<double> { using type = double2; };
-template <> struct PairOf<float> { using type = float2; };
-template <typename T> using Pair = typename PairOf<T>::type;
-
 // a lane moves its CPL cells as pairs when CPL is even, the line's first
 // cell (i = 2) sits on a pair boundary (so does every lane's first cell,
 // i0 - 2 being a multiple of CPL) and all its cells are inside the line
