@@ -958,7 +958,8 @@ MGE_ENTRY_T(k_prolong, XferArgs, 256, 1)
 // each fine value with k_prolong's stencil, weights and summation order (so
 // the same bits), and their p when eos.  A component's coarse values over
 // the block's 64 x 4 cells, a one-cell halo and the planes ck-1..ck+1 are
-// staged in LDS by coalesced row loads; the fine cells 2c-2, 2c-1 of a row
+// staged in LDS by coalesced row loads (the correction w - ws is formed
+// there, so no k_corr pass runs first); the fine cells 2c-2, 2c-1 of a row
 // move as one pair (rows, planes and components of a ratio-2 fine level have
 // even lengths; the host checks the alignment).  blockDim = (64, 4).
 constexpr int kPro2X = 66, kPro2Y = 6, kPro2N = 3 * kPro2Y * kPro2X;
@@ -972,7 +973,6 @@ __device__ __attribute__((always_inline)) inline void k_prolong2_body(const Xfer
   const int ci0 = 2 + bx * 64, cj0 = 2 + by * 4, ck = 2 + bz;
   const int ci = ci0 + tx, cj = cj0 + ty;
   const bool act = ci <= c.il && cj <= c.jl;
-  const T *src = X.fmg ? X.wc : X.corr;
   // the fine cell 2c-2 takes 1/4 of c-1 and 3/4 of c, the cell 2c-1 3/4 of c
   // and 1/4 of c+1 (pstencil)
   const T wlo[2] = {T(0.25), T(0.75)}, whi[2] = {T(0.75), T(0.25)};
@@ -986,12 +986,15 @@ __device__ __attribute__((always_inline)) inline void k_prolong2_body(const Xfer
   }
 #pragma unroll 1
   for (int n = 0; n < 5; n++) {
-    const T *sn = src + n * c.E;
+    // the solution (full multigrid) or the correction w - ws, formed here as
+    // k_corr forms it
+    const T *wn = X.wc + n * c.E, *sn = X.wsc + n * c.E;
     for (int e = tid; e < kPro2N; e += 256) {
       const int xx = e % kPro2X, rr = e / kPro2X, yy = rr % kPro2Y, zz = rr / kPro2Y;
       const int gi = ci0 - 1 + xx < c.ib ? ci0 - 1 + xx : c.ib;  // past the block: never read
       const int gj = cj0 - 1 + yy < c.jb ? cj0 - 1 + yy : c.jb;
-      (&cs[0][0][0])[e] = sn[ix(c, gi, gj, ck - 1 + zz)];
+      const long qc = ix(c, gi, gj, ck - 1 + zz);
+      (&cs[0][0][0])[e] = X.fmg ? wn[qc] : wn[qc] - sn[qc];
     }
     __syncthreads();
     if (act) {

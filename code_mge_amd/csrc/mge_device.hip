@@ -955,10 +955,14 @@ int do_movfin(mge_case *h, int mgl, int mlvl) {
   X.wfo = tp<T>(f.S[f.cur]);
   X.corr = tp<T>(c.corr);
   ensure_corners<T>(h, c);  // the prolongation stencils reach the coarse level's edge cells
-  if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, flat_grid(5 * c.d.E), dim3(256), X);
   auto pair_aligned = [&](const void *p) { return (reinterpret_cast<uintptr_t>(p) & (2 * sizeof(T) - 1)) == 0; };
-  if (f.ratio[0] == 2 && f.ratio[1] == 2 && f.ratio[2] == 2 && !X.pw[0] && f.d.sy % 2 == 0 && f.d.sz % 2 == 0 &&
-      f.d.E % 2 == 0 && pair_aligned(X.wfo) && (reinterpret_cast<uintptr_t>(f.flags) & 1) == 0) {
+  const bool two = f.ratio[0] == 2 && f.ratio[1] == 2 && f.ratio[2] == 2 && !X.pw[0] && f.d.sy % 2 == 0 &&
+                   f.d.sz % 2 == 0 && f.d.E % 2 == 0 && pair_aligned(X.wfo) &&
+                   (reinterpret_cast<uintptr_t>(f.flags) & 1) == 0;
+  // k_prolong2 forms the correction itself: its k_corr launch has no blocks
+  // (kept, so that the cases of a batch record the same launches)
+  if (!X.fmg) LAUNCHT(h, "k_corr", k_corr, T, two ? dim3(0) : flat_grid(5 * c.d.E), dim3(256), X);
+  if (two) {
     // a thread per coarse cell and its eight fine cells (pairs of fine cells
     // in a row move together: even strides and aligned arrays)
     const dim3 gp = xfer_eos_grid(X, f, h->prm, cell_grid(c.d));
