@@ -1087,16 +1087,25 @@ __device__ __attribute__((always_inline)) inline void k_rec_body(const RecArgs<T
   const StageArgs<T> &A = R.st;
   const Dims &d = A.fx.d;
   const long E = d.E;
-  // the cell's fw and dw are loaded before the op values are formed, so
-  // their round trips overlap the ops' own
+  // the cell's fw and dw, and the finish's inputs (W(0) on an unsmoothed
+  // stage, the local time step, the flags), are loaded before the op values
+  // are formed, so their round trips overlap the ops' own
   const bool cell = t < ch.y;
   const long q = cell ? R.tcell[t] : 0;
   double f[5], e[5];
-  if (cell)
+  T wnp[5] = {T(0), T(0), T(0), T(0), T(0)};
+  T vdt = T(0);
+  uint8_t fl = 0;
+  if (cell) {
     for (int n = 0; n < 5; n++) {
       f[n] = A.fw[n * E + q];
       e[n] = A.dw[n * E + q];
     }
+    fl = A.flags[q];
+    vdt = A.vdtim[q];
+    if (!A.smoothed && !A.lres)
+      for (int n = 0; n < 5; n++) wnp[n] = A.wn[n * E + q];
+  }
   if (o0 + tid < o1) rec_op_value(R, o0 + tid, val + tid * 10);
   __syncthreads();
   if (!cell) return;
@@ -1139,7 +1148,7 @@ __device__ __attribute__((always_inline)) inline void k_rec_body(const RecArgs<T
     ft[n] = (T)f[n];
     et[n] = (T)e[n];
   }
-  finish_cell(A, q, ft, et);
+  finish_cell<T, true>(A, q, ft, et, wnp, vdt, fl);
 }
 MGE_ENTRY_T(k_rec, RecArgs, 64, 1)
 
